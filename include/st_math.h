@@ -1,0 +1,77 @@
+/*
+ * st_math.h - f32 helpers shared by host and device code of the FLIP substep.
+ *
+ * WGSL leaves the precision of exp() and the evaluation of mix()/smoothstep()
+ * to the implementation.  This project fixes one definition of each so the HIP
+ * kernels give the same bits on every run: the definitions below, compiled with
+ * floating-point contraction off, are the ones the kernels use.
+ */
+#ifndef ST_MATH_H
+#define ST_MATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define ST_HD __host__ __device__ inline
+#else
+#define ST_HD static inline
+#endif
+
+/* exp(x) in f32.  Cody-Waite range reduction x = k*ln2 + r with ln2 split into
+ * 355/512 (exact for k*hi, |k| < 2^15) and the remainder, then the degree-7
+ * Taylor polynomial of e^r on |r| <= ln2/2 in Horner form (truncation below
+ * 6e-9 relative), then an exact scaling by 2^k.  Overflow gives +inf,
+ * underflow 0. */
+ST_HD float st_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.8f) return INFINITY;
+  if (x < -104.0f) return 0.0f;
+  const float inv_ln2 = 1.44269504088896341f;
+  const float ln2_hi = 0.693359375f;             /* 355/512 */
+  const float ln2_lo = -2.12194440054690583e-4f; /* ln2 - 355/512 */
+  float kf = rintf(x * inv_ln2);
+  float r = x - kf * ln2_hi;
+  r = r - kf * ln2_lo;
+  float p = 1.0f / 5040.0f;
+  p = p * r + 1.0f / 720.0f;
+  p = p * r + 1.0f / 120.0f;
+  p = p * r + 1.0f / 24.0f;
+  p = p * r + 1.0f / 6.0f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int k = (int)kf;
+  /* two exact steps so 2^k never overflows on its own for results near the
+   * smallest normals */
+  int k1 = k / 2;
+  int k2 = k - k1;
+  return ldexpf(ldexpf(p, k1), k2);
+}
+
+ST_HD float st_mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+ST_HD float st_clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+/* WGSL smoothstep: t = clamp((x - lo) / (hi - lo), 0, 1); t * t * (3 - 2 t) */
+ST_HD float st_smoothstepf(float lo, float hi, float x) {
+  float t = st_clampf((x - lo) / (hi - lo), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+ST_HD float st_fractf(float x) { return x - floorf(x); }
+ST_HD uint32_t st_f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+ST_HD float st_u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+/* WGSL i32(f32): truncation toward zero, saturating, NaN -> 0 */
+ST_HD int32_t st_f2i_sat(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483520.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (-2147483647 - 1);
+  return (int32_t)x;
+}
+/* WGSL u32(f32): truncation toward zero, saturating to [0, 2^32-1], NaN -> 0 */
+ST_HD uint32_t st_f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967040.0f) return 4294967295u;
+  return (uint32_t)x;
+}
+
+#endif /* ST_MATH_H */

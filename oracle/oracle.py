@@ -1,0 +1,98 @@
+"""TEST INFRASTRUCTURE: ctypes driver of the CPU oracle (oracle/liboracle.so).
+
+The oracle restates the reference's FLIP substep on the CPU (see
+flip_oracle.cpp).  Only tests/, the smoke check and bench.py's CPU-baseline leg
+use it, as the checker; it is never on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from splashtank_amd.capi import StDims, StError, StOpts, StParams, _Base
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_PROTOS = {
+    "orc_last_error": (C.c_char_p, []),
+    "orc_create": (C.c_int, [C.POINTER(StOpts), C.c_int, C.POINTER(C.c_void_p)]),
+    "orc_destroy": (C.c_int, [C.c_void_p]),
+    "orc_get_dims": (C.c_int, [C.c_void_p, C.POINTER(StDims)]),
+    "orc_upload_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "orc_set_count": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "orc_set_colliders": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.c_uint64]),
+    "orc_begin_step": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
+    "orc_stage": (C.c_int, [C.c_void_p, C.c_int32]),
+    "orc_step": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
+    "orc_read_ctl": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "orc_dump_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
+    "orc_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
+    "orc_threads": (C.c_int, []),
+    "orc_expf": (C.c_float, [C.c_float]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(path):
+        raise RuntimeError(f"oracle not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Oracle(_Base):
+    """CPU restatement of one FlipSim; fp64=True runs the same algorithm in f64."""
+
+    prefix = "orc"
+
+    def __init__(self, opts: StOpts, fp64: bool = False):
+        self.lib = load()
+        self.fdtype = np.float64 if fp64 else np.float32
+        h = C.c_void_p()
+        rc = self.lib.orc_create(C.byref(opts), 1 if fp64 else 0, C.byref(h))
+        if rc != 0:
+            raise StError(f"orc_create -> {rc}: {self._err()}")
+        self.h = h
+        self.dims = StDims()
+        self._call("get_dims", self.h, C.byref(self.dims))
+
+    def _err(self):
+        return self.lib.orc_last_error().decode()
+
+    def step_n(self, prm, n):
+        for _ in range(n):
+            self.step(prm)
+
+    def read_particles(self, n=None):
+        return self.dump("pos")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def threads() -> int:
+    return int(load().orc_threads())
+
+
+def expf(x: float) -> float:
+    return float(load().orc_expf(float(x)))

@@ -1,0 +1,264 @@
+"""ctypes binding of the C ABI in include/splashtank.h (libsplashtank_hip.so).
+
+`FlipSimC` mirrors the members of the reference's JS `FlipSim`
+(src/flip/sim.js:46-547) that the C ABI backs: construction options, particle
+upload, colliders, step / stage, readback, control words and canonical dumps.
+Loading fails loudly when the HIP library is not built: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsplashtank_hip.so")
+
+ST_OPEN_X0, ST_OPEN_X1, ST_OPEN_Y0, ST_OPEN_Y1, ST_OPEN_Z0, ST_OPEN_Z1 = 1, 2, 4, 8, 16, 32
+
+STAGES = ["sort", "blocks", "clear", "p2g", "narrow band", "classify", "normFaces",
+          "divergence", "pressure", "project", "g2p", "spray", "reseed"]
+(ST_STAGE_SORT, ST_STAGE_BLOCKS, ST_STAGE_CLEAR, ST_STAGE_P2G, ST_STAGE_NARROWBAND,
+ ST_STAGE_CLASSIFY, ST_STAGE_NORMFACES, ST_STAGE_DIVERGENCE, ST_STAGE_PRESSURE,
+ ST_STAGE_PROJECT, ST_STAGE_G2P, ST_STAGE_SPRAY, ST_STAGE_RESEED) = range(13)
+
+DUMP = {"tiles": 0, "cnt": 1, "ctype": 2, "pres": 3, "rhs": 4, "accv": 5, "accw": 6,
+        "facev": 7, "faceo": 8, "velg": 9, "pos": 10, "vel": 11, "tmap": 12}
+# element type and values per entry (per cell, per face triple or per particle)
+DUMP_KIND = {"tiles": ("u4", 1), "cnt": ("i4", 1), "ctype": ("u4", 1), "pres": ("f", 1),
+             "rhs": ("f", 1), "accv": ("i4", 3), "accw": ("i4", 3), "facev": ("f", 3),
+             "faceo": ("f", 3), "velg": ("f", 3), "pos": ("f", 4), "vel": ("f", 4),
+             "tmap": ("u4", 1)}
+
+
+class StOpts(C.Structure):
+    _fields_ = [("d", C.c_double), ("ratio", C.c_double), ("size", C.c_double * 3),
+                ("open", C.c_uint32), ("max_n", C.c_uint32), ("slots", C.c_uint32),
+                ("device", C.c_int32)]
+
+
+class StDims(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("d", "h", "ratio", "rest", "x0", "y0", "z0", "x1", "y1", "z1")] + \
+               [(n, C.c_int32) for n in ("nx", "ny", "nz", "tx", "ty", "tz")] + \
+               [(n, C.c_uint32) for n in ("ntiles", "scap", "scap_max", "max_n", "ncell")]
+
+
+class StParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("g", C.c_float), ("flip", C.c_float), ("comp", C.c_float),
+                ("omega", C.c_float), ("vmax", C.c_float), ("tear", C.c_float), ("iters", C.c_int32),
+                ("wind", C.c_float * 3), ("drag", C.c_float), ("foam_tau", C.c_float),
+                ("foam_sens", C.c_float), ("gust", C.c_float), ("no_foam", C.c_uint32),
+                ("no_cull", C.c_uint32), ("fast_air", C.c_uint32), ("no_solid_cache", C.c_uint32),
+                ("nb_on", C.c_uint32), ("nb_band", C.c_float), ("only_tiles", C.c_uint32),
+                ("nb_init", C.c_uint32), ("nfill", C.c_uint32), ("nfv", C.c_uint32),
+                ("fills", C.c_float * 64), ("reserved", C.c_uint32 * 4)]
+
+
+def make_opts(d, size=(3.0, 2.6, 3.0), ratio=2.0, open_=0, max_n=1, slots=0, device=0) -> StOpts:
+    o = StOpts()
+    o.d = float(d)
+    o.ratio = float(ratio)
+    for k in range(3):
+        o.size[k] = float(size[k])
+    o.open = int(open_)
+    o.max_n = int(max_n)
+    o.slots = int(slots)
+    o.device = int(device)
+    return o
+
+
+def make_params(**kw) -> StParams:
+    """The reference's step() defaults (sim.js:197-199) overridden by keywords."""
+    p = StParams()
+    p.dt = 1.0 / 60.0
+    p.g = 9.81
+    p.flip = 0.95
+    p.comp = 6.0
+    p.omega = 0.0
+    p.vmax = 14.0
+    p.tear = 0.0
+    p.iters = 30
+    p.foam_tau = 1.2
+    p.foam_sens = 1.0
+    p.nb_band = 3.0
+    for k, v in kw.items():
+        if k == "wind":
+            for q in range(3):
+                p.wind[q] = float(v[q])
+        elif k == "fills":
+            for q, x in enumerate(v):
+                p.fills[q] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+PROTOS = {
+    "st_last_error": (C.c_char_p, []),
+    "st_version": (C.c_char_p, []),
+    "st_params_default": (None, [C.POINTER(StParams)]),
+    "st_create": (C.c_int, [C.POINTER(StOpts), C.POINTER(C.c_void_p)]),
+    "st_destroy": (C.c_int, [C.c_void_p]),
+    "st_get_dims": (C.c_int, [C.c_void_p, C.POINTER(StDims)]),
+    "st_upload_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "st_set_count": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_clear": (C.c_int, [C.c_void_p]),
+    "st_get_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "st_set_colliders": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.c_uint64]),
+    "st_step": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
+    "st_step_n": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.c_int32]),
+    "st_begin_step": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
+    "st_stage": (C.c_int, [C.c_void_p, C.c_int32]),
+    "st_read_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "st_read_ctl": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "st_grow": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_dump_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
+    "st_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
+    "st_sync": (C.c_int, [C.c_void_p]),
+    "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library; raises when it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class StError(RuntimeError):
+    pass
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class _Base:
+    """Shared driver of the C ABI and of the oracle's mirror of it."""
+
+    prefix = "st"
+    fdtype = np.float32
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, f"{self.prefix}_{name}")(*args)
+        if rc != 0:
+            raise StError(f"{self.prefix}_{name} -> {rc}: {self._err()}")
+        return rc
+
+    def _err(self):
+        return self.lib.st_last_error().decode()
+
+    # --- particles and colliders ---
+    def upload(self, pos4, vel4, first=0):
+        pos4 = _f32c(pos4).reshape(-1, 4)
+        vel4 = _f32c(vel4).reshape(-1, 4)
+        assert pos4.shape == vel4.shape
+        self._call("upload_particles", self.h, first, pos4.shape[0], pos4.ctypes.data, vel4.ctypes.data)
+
+    def set_colliders(self, cf32, sdf, sig):
+        cf32 = _f32c(cf32).reshape(-1)
+        sdf = _f32c(sdf).reshape(-1)
+        nc = cf32.size // 32
+        self._call("set_colliders", self.h, cf32.ctypes.data if nc else None, nc,
+                   sdf.ctypes.data if sdf.size else None, sdf.size, int(sig))
+        self._keep = (cf32, sdf)
+
+    # --- stepping ---
+    def step(self, prm: StParams):
+        self._call("step", self.h, C.byref(prm))
+
+    def begin_step(self, prm: StParams):
+        self._call("begin_step", self.h, C.byref(prm))
+
+    def stage(self, k: int):
+        self._call("stage", self.h, int(k))
+
+    def stages(self, upto: int):
+        """Run stages 0..upto inclusive of a begun step."""
+        for k in range(upto + 1):
+            self.stage(k)
+
+    # --- readback ---
+    def ctl(self):
+        a = np.zeros(16, np.uint32)
+        self._call("read_ctl", self.h, a.ctypes.data)
+        return a
+
+    def dump(self, what: str):
+        sz = C.c_size_t(0)
+        self._call("dump_size", self.h, DUMP[what], C.byref(sz))
+        kind, per = DUMP_KIND[what]
+        dt = self.fdtype if kind == "f" else np.dtype(kind)
+        n = sz.value // np.dtype(dt).itemsize
+        a = np.zeros(max(n, 1), dtype=dt)
+        self._call("dump", self.h, DUMP[what], a.ctypes.data, sz.value)
+        a = a[:n]
+        if per > 1:
+            a = a.reshape(-1, per)
+        return a
+
+
+class FlipSimC(_Base):
+    """One tank on one GPU through libsplashtank_hip.so."""
+
+    prefix = "st"
+
+    def __init__(self, opts: StOpts):
+        self.lib = load()
+        h = C.c_void_p()
+        self._call_create(opts, h)
+        self.h = h
+        self.dims = StDims()
+        self._call("get_dims", self.h, C.byref(self.dims))
+
+    def _call_create(self, opts, h):
+        rc = self.lib.st_create(C.byref(opts), C.byref(h))
+        if rc != 0:
+            raise StError(f"st_create -> {rc}: {self.lib.st_last_error().decode()}")
+
+    def step_n(self, prm: StParams, n: int, graph: bool = False):
+        self._call("step_n", self.h, C.byref(prm), int(n), 1 if graph else 0)
+
+    def read_particles(self, n=None):
+        cnt = C.c_uint32(0)
+        self._call("get_count", self.h, C.byref(cnt))
+        n = cnt.value if n is None else n
+        a = np.zeros((max(n, 1), 4), np.float32)
+        self._call("read_particles", self.h, a.ctypes.data, n)
+        return a[:n]
+
+    def sync(self):
+        self._call("sync", self.h)
+
+    def grow(self, slots):
+        self._call("grow", self.h, int(slots))
+        self._call("get_dims", self.h, C.byref(self.dims))
+
+    def mem_bytes(self):
+        b = C.c_uint64(0)
+        self._call("mem_bytes", self.h, C.byref(b))
+        return b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.st_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

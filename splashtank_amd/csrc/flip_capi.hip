@@ -1,0 +1,638 @@
+// flip_capi.hip - the C ABI of include/splashtank.h over HIP (host code).
+//
+// One Sim per tank: device buffers, one HIP stream, and a ring of step slots
+// (parameter block + collider records) so consecutive steps can be enqueued
+// without waiting for each other.  Mirrors the WebGPU side of the reference's
+// FlipSim (src/flip/sim.js:46-547).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flip_device.h"
+#include "flip_kernels.h"
+#include "splashtank.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail(ST_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct StepSlot {
+  DParams P;
+  DCols C;
+};
+const int RING = 64;
+
+template <class T> int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(ST_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return ST_OK;
+}
+template <class T> void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+struct Sim {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  // sizes (sim.js:47-65)
+  double hd = 0, dd = 0;
+  float d = 0, h = 0, ratio = 0, rest = 0;
+  int32_t nx = 0, ny = 0, nz = 0;
+  uint32_t tx = 0, ty = 0, tz = 0, ntiles = 0, scap = 0, scapMax = 0, maxn = 0, ncell = 0, open = 0;
+  float x0 = 0, y0 = 0, z0 = 0, x1 = 0, y1 = 0, z1 = 0;
+  uint32_t N = 0;
+  DGrid G{};
+  // step slots
+  StepSlot* dSlots = nullptr;
+  StepSlot* hSlots = nullptr;
+  hipEvent_t ev[RING];
+  bool evUsed[RING];
+  uint32_t ringPos = 0;
+  StepSlot* cur = nullptr;
+  DParams hp{};
+  // colliders
+  float* dSdf = nullptr;
+  size_t sdfCap = 0, sdfN = 0;
+  uint64_t sdfSig = ~0ull, sdfSigPrev = ~0ull;
+  std::vector<float> cf, cfPrev;
+  uint32_t ncTotal = 0;
+  bool haveCols = false, solidDirty = true;
+  // host state of FlipSim
+  double tm = 0;
+  uint32_t seed = 0;
+  bool velGOn = false, nbOn = false, onlyTiles = false, begun = false;
+  uint32_t initStep = 0;
+  int iters = 1;
+  uint32_t lcap = 0;
+
+  LaunchCtx ctx() const {
+    LaunchCtx L;
+    L.P = &cur->P;
+    L.C = &cur->C;
+    L.G = G;
+    L.n = N;
+    L.ntiles = ntiles;
+    L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
+    return L;
+  }
+};
+
+int allocSlots(Sim* s, uint32_t n) {
+  size_t C = size_t(n) * 512;
+  int rc;
+  if ((rc = dalloc(&s->G.cnt, C))) return rc;
+  if ((rc = dalloc(&s->G.ctype, C))) return rc;
+  if ((rc = dalloc(&s->G.pres, C))) return rc;
+  if ((rc = dalloc(&s->G.rhs, C))) return rc;
+  if ((rc = dalloc(&s->G.sten, C))) return rc;
+  if ((rc = dalloc(&s->G.accV, C * 3))) return rc;
+  if ((rc = dalloc(&s->G.accW, C * 3))) return rc;
+  if (s->velGOn && (rc = dalloc(&s->G.velG, C * 3))) return rc;
+  if ((rc = dalloc(&s->G.slotTile, n))) return rc;
+  if ((rc = dalloc(&s->G.fstack, n))) return rc;
+  if ((rc = dalloc(&s->G.slotList, n))) return rc;
+  if ((rc = dalloc(&s->G.nbr, size_t(n) * 6))) return rc;
+  return ST_OK;
+}
+
+void freeSlotArrays(DGrid& g) {
+  dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.rhs); dfree(g.sten);
+  dfree(g.accV); dfree(g.accW); dfree(g.velG);
+  dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr);
+}
+void freeSlots(Sim* s) { freeSlotArrays(s->G); }
+
+int zeroSlots(Sim* s, uint32_t from, uint32_t to) {
+  if (to <= from) return ST_OK;
+  size_t a = size_t(from) * 512, n = size_t(to - from) * 512;
+  HIPCHK(hipMemsetAsync(s->G.cnt + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.ctype + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.pres + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.rhs + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.sten + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.accV + a * 3, 0, n * 12, s->st));
+  HIPCHK(hipMemsetAsync(s->G.accW + a * 3, 0, n * 12, s->st));
+  if (s->G.velG) HIPCHK(hipMemsetAsync(s->G.velG + a * 3, 0, n * 12, s->st));
+  return ST_OK;
+}
+
+void destroySim(Sim* s) {
+  if (!s) return;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
+  dfree(s->G.binKey); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.binCursor);
+  dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl);
+  dfree(s->G.scratch4);
+  freeSlots(s);
+  dfree(s->dSdf);
+  dfree(s->dSlots);
+  if (s->hSlots) (void)hipHostFree(s->hSlots);
+  for (int k = 0; k < RING; k++)
+    if (s->evUsed[k]) (void)hipEventDestroy(s->ev[k]);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+}
+
+Sim* getSim(void* p) { return (Sim*)p; }
+
+// sets and uploads the step's parameter block and collider records (sim.js:179-231)
+int beginStep(Sim* s, const st_params* o) {
+  if (s->N == 0) return fail(ST_ERR_STATE, "st_begin_step: no particles (step() does nothing when N = 0)");
+  s->nbOn = o->nb_on != 0;
+  if (s->nbOn && !s->velGOn) {  // enableNB (sim.js:120-126)
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->velGOn = true;
+    int rc = dalloc(&s->G.velG, size_t(s->scap) * 512 * 3);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(s->G.velG, 0, size_t(s->scap) * 512 * 12, s->st));
+    s->lcap = s->scap * 512u;
+  }
+  s->initStep = s->nbOn ? o->nb_init : 0u;
+  uint32_t nc = s->ncTotal >= o->nfv ? s->ncTotal - o->nfv : 0u;
+  {  // the per-cell collider lookup only changes when a collider moves (sim.js:191-195)
+    bool same = s->haveCols && s->sdfSig == s->sdfSigPrev && !o->no_solid_cache && s->cfPrev.size() == size_t(nc) * 32;
+    if (same)
+      for (size_t q = 0; q < size_t(nc) * 32; q++)
+        if (s->cfPrev[q] != s->cf[q]) { same = false; break; }
+    if (!same) {
+      s->solidDirty = true;
+      s->cfPrev.assign(s->cf.begin(), s->cf.begin() + size_t(nc) * 32);
+      s->sdfSigPrev = s->sdfSig;
+      s->haveCols = true;
+    }
+  }
+  s->onlyTiles = o->only_tiles != 0;
+  DParams& P = s->hp;
+  std::memset(&P, 0, sizeof P);
+  P.n = s->N; P.nx = uint32_t(s->nx); P.ny = uint32_t(s->ny); P.nz = uint32_t(s->nz); P.ncols = nc; P.open = s->open; P.ncell = s->ncell; P.scap = s->scap;
+  double omegaDef = std::min(1.9, 2.0 / (1.0 + std::sin(M_PI / std::max(s->nx, std::max(s->ny, s->nz)))));
+  P.dt = o->dt; P.h = float(s->hd); P.ih = float(1.0 / s->hd); P.gravity = o->g; P.flip = o->flip; P.rest = s->rest; P.comp = o->comp;
+  P.omega = o->omega != 0.0f ? o->omega : float(omegaDef);
+  P.x0 = s->x0; P.y0 = s->y0; P.z0 = s->z0; P.vmax = o->vmax != 0.0f ? o->vmax : 14.0f;
+  P.margin = float(0.3 * s->dd); P.scale = 65536.0f; P.x1 = s->x1; P.y1 = s->y1; P.z1 = s->z1; P.q0 = float(0.3 * s->hd);
+  P.q1 = o->wind[0]; P.q2 = o->wind[1]; P.q3 = o->wind[2]; P.q4 = o->drag; P.q5 = o->foam_tau; P.q6 = o->foam_sens; P.q7 = o->gust;
+  P.q8 = float(s->tm); P.q9 = o->no_foam ? 1.0f : 0.0f; P.q10 = o->no_cull ? 1.0f : 0.0f; P.q11 = 1.0f; P.q12 = o->fast_air ? 1.0f : 0.0f;
+  if (!s->onlyTiles) s->tm = s->tm + double(o->dt);
+  P.sheetMin = o->tear;
+  P.band = std::max(2.0f, o->nb_band > 0 ? o->nb_band : 3.0f);
+  P.tx = s->tx; P.ty = s->ty; P.tz = s->tz; P.lcap = s->lcap;
+  if (!s->onlyTiles) s->seed = s->seed + 1u;
+  P.seed = s->seed;
+  P.nb = s->nbOn ? 1u : 0u;
+  P.nbInit = s->initStep;
+  P.nfill = s->initStep ? std::min<uint32_t>(o->nfill, 8u) : 0u;
+  P.nfv = s->initStep ? o->nfv : 0u;
+  P.ntiles = s->ntiles;
+  for (int k = 0; k < 16; k++)
+    for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
+  s->iters = std::max(1, int(o->iters));
+  // step slot: wait until the ring entry's last copy finished, fill, copy
+  uint32_t k = s->ringPos++ % RING;
+  if (s->evUsed[k]) HIPCHK(hipEventSynchronize(s->ev[k]));
+  StepSlot& hs = s->hSlots[k];
+  hs.P = P;
+  size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
+  std::memcpy(hs.C.c, s->cf.data(), ncAll * 32 * sizeof(float));
+  size_t bytes = offsetof(StepSlot, C) + ncAll * 32 * sizeof(float);
+  HIPCHK(hipMemcpyAsync(&s->dSlots[k], &hs, bytes, hipMemcpyHostToDevice, s->st));
+  if (!s->evUsed[k]) {
+    HIPCHK(hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming));
+    s->evUsed[k] = true;
+  }
+  HIPCHK(hipEventRecord(s->ev[k], s->st));
+  s->cur = &s->dSlots[k];
+  s->begun = true;
+  return ST_OK;
+}
+
+int runStage(Sim* s, int stage) {
+  if (!s->begun) return fail(ST_ERR_STATE, "st_stage: no step begun");
+  LaunchCtx L = s->ctx();
+  hipStream_t st = s->st;
+  bool full = !s->onlyTiles;
+  switch (stage) {
+    case ST_STAGE_SORT: break;
+    case ST_STAGE_BLOCKS: launch_blocks(st, L); break;
+    case ST_STAGE_CLEAR: if (full) launch_clear(st, L); break;
+    case ST_STAGE_P2G: if (full) launch_p2g(st, L); break;
+    case ST_STAGE_NARROWBAND: break;
+    case ST_STAGE_CLASSIFY:
+      if (full) {
+        launch_classify(st, L, s->solidDirty);
+        s->solidDirty = false;
+      }
+      break;
+    case ST_STAGE_NORMFACES: if (full) launch_norm_faces(st, L); break;
+    case ST_STAGE_DIVERGENCE: if (full) launch_divergence(st, L); break;
+    case ST_STAGE_PRESSURE: if (full) launch_pressure(st, L, s->iters); break;
+    case ST_STAGE_PROJECT: if (full) launch_project(st, L); break;
+    case ST_STAGE_G2P: if (full) launch_g2p(st, L); break;
+    case ST_STAGE_SPRAY: break;
+    case ST_STAGE_RESEED: break;
+    default: return fail(ST_ERR_ARG, "st_stage: unknown stage");
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* st_last_error(void) { return g_err.c_str(); }
+const char* st_version(void) { return "splashtank-mi355x 0.1 (gfx950)"; }
+
+void st_params_default(st_params* p) {
+  std::memset(p, 0, sizeof *p);
+  p->dt = 1.0f / 60.0f;
+  p->g = 9.81f;
+  p->flip = 0.95f;
+  p->comp = 6.0f;
+  p->omega = 0.0f;
+  p->vmax = 14.0f;
+  p->tear = 0.0f;
+  p->iters = 30;
+  p->foam_tau = 1.2f;
+  p->foam_sens = 1.0f;
+  p->nb_band = 3.0f;
+}
+
+int st_create(const st_opts* o, void** out) {
+  if (!o || !out) return fail(ST_ERR_ARG, "st_create: null argument");
+  *out = nullptr;
+  if (!(o->d > 0) || o->max_n == 0) return fail(ST_ERR_ARG, "st_create: d must be > 0 and max_n > 0");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (o->device < 0 || o->device >= ndev) return fail(ST_ERR_ARG, "st_create: no such HIP device");
+  HIPCHK(hipSetDevice(o->device));
+  Sim* s = new Sim();
+  for (int k = 0; k < RING; k++) s->evUsed[k] = false;
+  s->dev = o->device;
+  double rt = o->ratio > 0 ? o->ratio : 2.0;
+  rt = std::max(1.4, std::min(4.0, rt));
+  s->ratio = float(rt);
+  s->dd = o->d;
+  s->hd = rt * o->d;
+  s->d = float(o->d);
+  s->h = float(s->hd);
+  s->rest = float(rt * rt * rt);
+  double sz[3] = {o->size[0] > 0 ? o->size[0] : 3.0, o->size[1] > 0 ? o->size[1] : 2.6, o->size[2] > 0 ? o->size[2] : 3.0};
+  s->nx = std::max(4, (int)std::floor(sz[0] / s->hd + 0.5));
+  s->ny = std::max(4, (int)std::floor(sz[1] / s->hd + 0.5));
+  s->nz = std::max(4, (int)std::floor(sz[2] / s->hd + 0.5));
+  double W = s->nx * s->hd, H = s->ny * s->hd, D = s->nz * s->hd;
+  s->x0 = float(-W / 2); s->x1 = float(W / 2); s->y0 = 0.0f; s->y1 = float(H); s->z0 = float(-D / 2); s->z1 = float(D / 2);
+  s->open = o->open;
+  s->maxn = o->max_n;
+  s->ncell = uint32_t(s->nx) * s->ny * s->nz;
+  s->tx = (s->nx + 1 + 7) / 8; s->ty = (s->ny + 1 + 7) / 8; s->tz = (s->nz + 1 + 7) / 8;
+  s->ntiles = s->tx * s->ty * s->tz;
+  s->scapMax = s->ntiles + 1;
+  s->scap = o->slots ? std::max<uint32_t>(2, std::min(s->scapMax, o->slots)) : s->scapMax;
+  s->lcap = std::min<uint32_t>(s->scap * 512u, s->maxn);
+  int rc = ST_OK;
+  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { destroySim(s); return fail(ST_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  size_t M = s->maxn;
+  if ((rc = dalloc(&s->G.pos, M)) || (rc = dalloc(&s->G.vel, M)) || (rc = dalloc(&s->G.pid, M)) || (rc = dalloc(&s->G.scratch4, M)) ||
+      (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
+      (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
+      (rc = dalloc(&s->dSlots, RING))) {
+    destroySim(s);
+    return rc;
+  }
+  e = hipHostMalloc((void**)&s->hSlots, sizeof(StepSlot) * RING, hipHostMallocDefault);
+  if (e != hipSuccess) { destroySim(s); return fail(ST_ERR_NOMEM, "hipHostMalloc failed"); }
+  std::memset(s->hSlots, 0, sizeof(StepSlot) * RING);
+  // empty tile map, every slot but the spare slot 0 on the free stack (sim.js:94-98)
+  std::vector<uint32_t> host(std::max<size_t>(M, std::max<size_t>(s->ntiles, s->scap)));
+  for (size_t k = 0; k < M; k++) host[k] = uint32_t(k);
+  if ((e = hipMemcpy(s->G.pid, host.data(), M * 4, hipMemcpyHostToDevice)) != hipSuccess) { destroySim(s); return fail(ST_ERR_HIP, "pid upload failed"); }
+  std::vector<uint32_t> st(s->scap, 0u);
+  for (uint32_t k = 0; k + 1 < s->scap; k++) st[k] = s->scap - 1 - k;
+  std::vector<uint32_t> tm(s->ntiles, ST_NONE);
+  uint32_t ctl[16] = {0, 0, s->scap - 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (hipMemcpy(s->G.fstack, st.data(), size_t(s->scap) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->G.tmap, tm.data(), size_t(s->ntiles) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->G.ctl, ctl, 64, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(s->G.pos, 0, M * 16) != hipSuccess || hipMemset(s->G.vel, 0, M * 16) != hipSuccess ||
+      hipMemset(s->G.need, 0, size_t(s->ntiles) * 4) != hipSuccess) {
+    destroySim(s);
+    return fail(ST_ERR_HIP, "st_create: initial upload failed");
+  }
+  if ((rc = zeroSlots(s, 0, s->scap))) { destroySim(s); return rc; }
+  e = hipStreamSynchronize(s->st);
+  if (e != hipSuccess) { destroySim(s); return fail(ST_ERR_HIP, "st_create: sync failed"); }
+  *out = s;
+  return ST_OK;
+}
+
+int st_destroy(void* p) {
+  if (!p) return fail(ST_ERR_ARG, "st_destroy: null sim");
+  destroySim(getSim(p));
+  return ST_OK;
+}
+
+int st_get_dims(void* p, st_dims* o) {
+  Sim* s = getSim(p);
+  if (!s || !o) return fail(ST_ERR_ARG, "st_get_dims: null argument");
+  o->d = s->d; o->h = s->h; o->ratio = s->ratio; o->rest = s->rest;
+  o->x0 = s->x0; o->y0 = s->y0; o->z0 = s->z0; o->x1 = s->x1; o->y1 = s->y1; o->z1 = s->z1;
+  o->nx = s->nx; o->ny = s->ny; o->nz = s->nz; o->tx = int32_t(s->tx); o->ty = int32_t(s->ty); o->tz = int32_t(s->tz);
+  o->ntiles = s->ntiles; o->scap = s->scap; o->scap_max = s->scapMax; o->max_n = s->maxn; o->ncell = s->ncell;
+  return ST_OK;
+}
+
+int st_upload_particles(void* p, uint32_t first, uint32_t count, const float* pos4, const float* vel4) {
+  Sim* s = getSim(p);
+  if (!s || (count && (!pos4 || !vel4))) return fail(ST_ERR_ARG, "st_upload_particles: null argument");
+  if (size_t(first) + count > s->maxn) return fail(ST_ERR_ARG, "st_upload_particles: beyond max_n");
+  if (!count) return ST_OK;
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipMemcpyAsync(s->G.pos + first, pos4, size_t(count) * 16, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(s->G.vel + first, vel4, size_t(count) * 16, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->N = std::max(s->N, first + count);
+  return ST_OK;
+}
+
+int st_set_count(void* p, uint32_t n) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_set_count: null sim");
+  if (n > s->maxn) return fail(ST_ERR_ARG, "st_set_count: beyond max_n");
+  s->N = n;
+  return ST_OK;
+}
+
+int st_get_count(void* p, uint32_t* n) {
+  Sim* s = getSim(p);
+  if (!s || !n) return fail(ST_ERR_ARG, "st_get_count: null argument");
+  *n = s->N;
+  return ST_OK;
+}
+
+int st_clear(void* p) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_clear: null sim");
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->N = 0;
+  s->tm = 0;
+  std::vector<uint32_t> ids(s->maxn);
+  for (size_t k = 0; k < ids.size(); k++) ids[k] = uint32_t(k);
+  HIPCHK(hipMemcpy(s->G.pid, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  return ST_OK;
+}
+
+int st_set_colliders(void* p, const float* cf32, uint32_t nc, const float* sdf, size_t nsdf, uint64_t sig) {
+  Sim* s = getSim(p);
+  if (!s || (nc && !cf32) || (nsdf && !sdf)) return fail(ST_ERR_ARG, "st_set_colliders: null argument");
+  if (nc > ST_MAX_COLS) return fail(ST_ERR_ARG, "st_set_colliders: at most 32 colliders");
+  HIPCHK(hipSetDevice(s->dev));
+  // every grid index a collider record can reach must lie inside the uploaded grids
+  for (uint32_t k = 0; k < nc; k++) {
+    const float* c = cf32 + size_t(k) * 32;
+    double off = c[0], gx = c[1], gy = c[2], gz = c[3];
+    if (!(off >= 0) || !(gx >= 2) || !(gy >= 2) || !(gz >= 2) || !(c[7] > 0) || off + gx * gy * gz > double(nsdf))
+      return fail(ST_ERR_ARG, "st_set_colliders: collider " + std::to_string(k) + " grid outside the sdf data");
+  }
+  s->cf.assign(cf32, cf32 + size_t(nc) * 32);
+  s->ncTotal = nc;
+  if (sig != s->sdfSig || nsdf != s->sdfN) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (nsdf > s->sdfCap) {
+      dfree(s->dSdf);
+      size_t cap = std::max<size_t>(nsdf + nsdf / 3, 16);
+      int rc = dalloc(&s->dSdf, cap);
+      if (rc) return rc;
+      s->sdfCap = cap;
+    }
+    if (nsdf) HIPCHK(hipMemcpy(s->dSdf, sdf, nsdf * 4, hipMemcpyHostToDevice));
+    s->sdfN = nsdf;
+    s->sdfSig = sig;
+  }
+  if (!s->dSdf) {
+    int rc = dalloc(&s->dSdf, 16);
+    if (rc) return rc;
+    s->sdfCap = 16;
+  }
+  s->G.sdf = s->dSdf;
+  return ST_OK;
+}
+
+int st_begin_step(void* p, const st_params* prm) {
+  Sim* s = getSim(p);
+  if (!s || !prm) return fail(ST_ERR_ARG, "st_begin_step: null argument");
+  HIPCHK(hipSetDevice(s->dev));
+  if (!s->dSdf) {
+    int rc = st_set_colliders(p, nullptr, 0, nullptr, 0, 0);
+    if (rc) return rc;
+  }
+  return beginStep(s, prm);
+}
+
+int st_stage(void* p, int32_t stage) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_stage: null sim");
+  return runStage(s, stage);
+}
+
+int st_step(void* p, const st_params* prm) {
+  int rc = st_begin_step(p, prm);
+  if (rc) return rc;
+  Sim* s = getSim(p);
+  for (int k = 0; k < ST_STAGE_COUNT; k++)
+    if ((rc = runStage(s, k))) return rc;
+  return ST_OK;
+}
+
+int st_step_n(void* p, const st_params* prm, int32_t nsteps, int32_t use_graph) {
+  (void)use_graph;
+  for (int32_t k = 0; k < nsteps; k++) {
+    int rc = st_step(p, prm);
+    if (rc) return rc;
+  }
+  return ST_OK;
+}
+
+int st_read_particles(void* p, float* dst4, uint32_t n) {
+  Sim* s = getSim(p);
+  if (!s || (n && !dst4)) return fail(ST_ERR_ARG, "st_read_particles: null argument");
+  n = std::min(n, s->N);
+  if (!n) return ST_OK;
+  HIPCHK(hipSetDevice(s->dev));
+  LaunchCtx L = s->ctx();
+  launch_unsort(s->st, L, s->G.scratch4, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst4, s->G.scratch4, size_t(n) * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ST_OK;
+}
+
+int st_read_ctl(void* p, uint32_t ctl[16]) {
+  Sim* s = getSim(p);
+  if (!s || !ctl) return fail(ST_ERR_ARG, "st_read_ctl: null argument");
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipMemcpyAsync(ctl, s->G.ctl, 64, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ST_OK;
+}
+
+int st_grow(void* p, uint32_t n) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_grow: null sim");
+  n = std::min(s->scapMax, std::max(n, s->scap + 1));
+  if (n <= s->scap) return ST_OK;
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  uint32_t old = s->scap;
+  DGrid keep = s->G;
+  s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
+  s->G.accV = nullptr; s->G.accW = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
+  s->G.slotList = nullptr; s->G.nbr = nullptr;
+  int rc = allocSlots(s, n);
+  if (rc) { freeSlots(s); s->G = keep; return rc; }
+  // the wg scratch and grid sizes follow the pool
+  size_t C = size_t(old) * 512;
+  HIPCHK(hipMemcpy(s->G.cnt, keep.cnt, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.ctype, keep.ctype, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.pres, keep.pres, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.rhs, keep.rhs, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.sten, keep.sten, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.accV, keep.accV, C * 12, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.accW, keep.accW, C * 12, hipMemcpyDeviceToDevice));
+  if (keep.velG) HIPCHK(hipMemcpy(s->G.velG, keep.velG, C * 12, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.slotTile, keep.slotTile, size_t(old) * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.fstack, keep.fstack, size_t(old) * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.slotList, keep.slotList, size_t(old) * 4, hipMemcpyDeviceToDevice));
+  rc = zeroSlots(s, old, n);
+  if (rc) return rc;
+  // growFree (wgsl-blocks.js:57-60): the new slots go on the free stack
+  uint32_t ctl[16];
+  HIPCHK(hipMemcpy(ctl, s->G.ctl, 64, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> add(n - old);
+  for (uint32_t t = 0; t < n - old; t++) add[t] = old + t;
+  HIPCHK(hipMemcpy(s->G.fstack + ctl[2], add.data(), add.size() * 4, hipMemcpyHostToDevice));
+  ctl[2] += n - old;
+  HIPCHK(hipMemcpy(s->G.ctl, ctl, 64, hipMemcpyHostToDevice));
+  freeSlotArrays(keep);
+  s->scap = n;
+  if (!s->velGOn) s->lcap = std::min<uint32_t>(n * 512u, s->maxn);
+  else s->lcap = n * 512u;
+  return ST_OK;
+}
+
+int st_dump_size(void* p, int32_t what, size_t* bytes) {
+  Sim* s = getSim(p);
+  if (!s || !bytes) return fail(ST_ERR_ARG, "st_dump_size: null argument");
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  std::vector<uint32_t> tm(s->ntiles);
+  HIPCHK(hipMemcpy(tm.data(), s->G.tmap, tm.size() * 4, hipMemcpyDeviceToHost));
+  size_t nt = 0;
+  for (uint32_t v : tm) nt += v != ST_NONE;
+  size_t C = 512;
+  switch (what) {
+    case ST_DUMP_TILES: *bytes = nt * 4; break;
+    case ST_DUMP_CNT: case ST_DUMP_CTYPE: case ST_DUMP_PRES: case ST_DUMP_RHS: *bytes = nt * C * 4; break;
+    case ST_DUMP_ACCV: case ST_DUMP_ACCW: case ST_DUMP_FACEV: case ST_DUMP_FACEO: case ST_DUMP_VELG: *bytes = nt * C * 12; break;
+    case ST_DUMP_POS: case ST_DUMP_VEL: *bytes = size_t(s->N) * 16; break;
+    case ST_DUMP_TMAP: *bytes = size_t(s->ntiles) * 4; break;
+    default: return fail(ST_ERR_ARG, "st_dump_size: unknown field");
+  }
+  return ST_OK;
+}
+
+int st_dump(void* p, int32_t what, void* dst, size_t bytes) {
+  Sim* s = getSim(p);
+  if (!s || !dst) return fail(ST_ERR_ARG, "st_dump: null argument");
+  size_t need = 0;
+  int rc = st_dump_size(p, what, &need);
+  if (rc) return rc;
+  if (bytes < need) return fail(ST_ERR_ARG, "st_dump: buffer too small");
+  std::vector<uint32_t> tm(s->ntiles);
+  HIPCHK(hipMemcpy(tm.data(), s->G.tmap, tm.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> ts;
+  for (uint32_t t = 0; t < s->ntiles; t++)
+    if (tm[t] != ST_NONE) ts.push_back(t);
+  char* o = (char*)dst;
+  auto perCell = [&](const void* src, size_t per) -> int {
+    if (!src) {
+      std::memset(o, 0, ts.size() * 512 * per * 4);
+      return ST_OK;
+    }
+    for (size_t q = 0; q < ts.size(); q++) {
+      size_t sl = tm[ts[q]];
+      hipError_t e = hipMemcpy(o + q * 512 * per * 4, (const char*)src + sl * 512 * per * 4, 512 * per * 4, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return fail(ST_ERR_HIP, "st_dump copy failed");
+    }
+    return ST_OK;
+  };
+  switch (what) {
+    case ST_DUMP_TILES: std::memcpy(o, ts.data(), ts.size() * 4); return ST_OK;
+    case ST_DUMP_CNT: return perCell(s->G.cnt, 1);
+    case ST_DUMP_CTYPE: return perCell(s->G.ctype, 1);
+    case ST_DUMP_PRES: return perCell(s->G.pres, 1);
+    case ST_DUMP_RHS: return perCell(s->G.rhs, 1);
+    case ST_DUMP_ACCV: case ST_DUMP_FACEV: return perCell(s->G.accV, 3);
+    case ST_DUMP_ACCW: case ST_DUMP_FACEO: return perCell(s->G.accW, 3);
+    case ST_DUMP_VELG: return perCell(s->G.velG, 3);
+    case ST_DUMP_TMAP: std::memcpy(o, tm.data(), tm.size() * 4); return ST_OK;
+    case ST_DUMP_POS:
+    case ST_DUMP_VEL: {
+      if (!s->N) return ST_OK;
+      LaunchCtx L = s->ctx();
+      (void)L;
+      // id order: out[pid[i]] = field[i]
+      std::vector<float> f(size_t(s->N) * 4);
+      std::vector<uint32_t> id(s->N);
+      HIPCHK(hipMemcpy(f.data(), what == ST_DUMP_POS ? (const void*)s->G.pos : (const void*)s->G.vel, f.size() * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(id.data(), s->G.pid, id.size() * 4, hipMemcpyDeviceToHost));
+      float* out = (float*)o;
+      for (size_t i = 0; i < s->N; i++) {
+        if (id[i] >= s->N) return fail(ST_ERR_STATE, "st_dump: particle id out of range");
+        std::memcpy(out + size_t(id[i]) * 4, f.data() + i * 4, 16);
+      }
+      return ST_OK;
+    }
+  }
+  return fail(ST_ERR_ARG, "st_dump: unknown field");
+}
+
+int st_sync(void* p) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_sync: null sim");
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ST_OK;
+}
+
+int st_mem_bytes(void* p, uint64_t* bytes) {
+  Sim* s = getSim(p);
+  if (!s || !bytes) return fail(ST_ERR_ARG, "st_mem_bytes: null argument");
+  uint64_t M = s->maxn, C = uint64_t(s->scap) * 512;
+  *bytes = M * (16 + 16 + 4 + 16) + uint64_t(s->ntiles) * 16 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
+  return ST_OK;
+}
+
+}  // extern "C"
