@@ -132,6 +132,17 @@ template <class R> struct Sim {
   uint32_t initStep = 0;
   bool onlyTiles = false;
 
+  // Face values: from normFaces on, the reference keeps the new and old face
+  // velocity as f32 bits in the same words that held the fixed-point sums
+  // (wgsl-core.js:24-31).  The f32 instance does the same, so a face that no
+  // normFaces pass converted (the spare slot) reads its integer bits as a float
+  // exactly as the reference does; the f64 instance keeps separate arrays.
+  static constexpr bool kF32 = std::is_same<R, float>::value;
+  R fvR(uint32_t f) const { if constexpr (kF32) return st_u2f(uint32_t(accVi[f])); else return faceV[f]; }
+  R foR(uint32_t f) const { if constexpr (kF32) return st_u2f(uint32_t(accWi[f])); else return faceO[f]; }
+  void fvW(uint32_t f, R v) { if constexpr (kF32) accVi[f] = int32_t(st_f2u(float(v))); else faceV[f] = v; }
+  void foW(uint32_t f, R v) { if constexpr (kF32) accWi[f] = int32_t(st_f2u(float(v))); else faceO[f] = v; }
+
   void makeSlots(uint32_t n) {
     const size_t C = 512;
     cnt.assign(n * C, 0);
@@ -140,8 +151,10 @@ template <class R> struct Sim {
     ctype.assign(n * C, 0u);
     pres.assign(n * C, R(0));
     rhs.assign(n * C, R(0));
-    faceV.assign(n * C * 3, R(0));
-    faceO.assign(n * C * 3, R(0));
+    if (!kF32) {
+      faceV.assign(n * C * 3, R(0));
+      faceO.assign(n * C * 3, R(0));
+    }
     velG.assign(n * C * 3, R(0));
     slotList.assign(n, 0u);
     slotTile.assign(n, 0u);
@@ -303,8 +316,8 @@ template <class R> struct Sim {
         int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
         R w = (dx == 1 ? fr.x : R(1) - fr.x) * (dy == 1 ? fr.y : R(1) - fr.y) * (dz == 1 ? fr.z : R(1) - fr.z);
         uint32_t f = faceIndex(axis, i0.x + dx, i0.y + dy, i0.z + dz);
-        sn = sn + w * faceV[f];
-        so = so + w * faceO[f];
+        sn = sn + w * fvR(f);
+        so = so + w * foR(f);
       }
       vn[int(axis)] = sn;
       vo[int(axis)] = so;
@@ -620,8 +633,8 @@ template <class R> struct Sim {
           if (ci >= 0) sv = colSurfVel(uint32_t(ci), fp)[int(axis)];
           v = sv; old = sv;
         }
-        faceO[f] = old;
-        faceV[f] = v;
+        foW(f, old);
+        fvW(f, v);
       }
     });
   }
@@ -629,8 +642,8 @@ template <class R> struct Sim {
     forActive([&](uint32_t c, I3 a) {
       if ((ctype[c] & 0xffu) != 1u) { rhs[c] = R(0); return; }
       int32_t i = a.x, j = a.y, k = a.z;
-      R div = (faceV[faceIndex(0, i + 1, j, k)] - faceV[faceIndex(0, i, j, k)] + faceV[faceIndex(1, i, j + 1, k)] -
-               faceV[faceIndex(1, i, j, k)] + faceV[faceIndex(2, i, j, k + 1)] - faceV[faceIndex(2, i, j, k)]) * P.ih;
+      R div = (fvR(faceIndex(0, i + 1, j, k)) - fvR(faceIndex(0, i, j, k)) + fvR(faceIndex(1, i, j + 1, k)) -
+               fvR(faceIndex(1, i, j, k)) + fvR(faceIndex(2, i, j, k + 1)) - fvR(faceIndex(2, i, j, k))) * P.ih;
       bool interior = true;
       if (ctypeOf(i - 1, j, k) == 0u || ctypeOf(i + 1, j, k) == 0u || ctypeOf(i, j - 1, k) == 0u || ctypeOf(i, j + 1, k) == 0u ||
           ctypeOf(i, j, k - 1) == 0u || ctypeOf(i, j, k + 1) == 0u)
@@ -681,12 +694,12 @@ template <class R> struct Sim {
         R pl = R(0), ph = R(0);
         if (tl == 1u) pl = pres[cellIdx(lo.x, lo.y, lo.z)];
         if (th == 1u) ph = pres[cellIdx(hi.x, hi.y, hi.z)];
-        faceV[f] = faceV[f] - P.dt * P.ih * (ph - pl);
+        fvW(f, fvR(f) - P.dt * P.ih * (ph - pl));
       }
     });
   }
   void saveVel() {  // wgsl-narrowband.js:88-91
-    forActive([&](uint32_t c, I3) { for (uint32_t ax = 0; ax < 3; ax++) velG[c * 3 + ax] = faceV[c * 3 + ax]; });
+    forActive([&](uint32_t c, I3) { for (uint32_t ax = 0; ax < 3; ax++) velG[c * 3 + ax] = fvR(c * 3 + ax); });
   }
   V3<R> collideP(V3<R> pin) const {  // wgsl-solver.js:185-197
     V3<R> p = pin;
@@ -995,8 +1008,8 @@ template <class R> struct Sim {
       case ST_DUMP_RHS: perCell(rhs.data(), sizeof(R), 1); break;
       case ST_DUMP_ACCV: perCell(accVi.data(), 4, 3); break;
       case ST_DUMP_ACCW: perCell(accWi.data(), 4, 3); break;
-      case ST_DUMP_FACEV: perCell(faceV.data(), sizeof(R), 3); break;
-      case ST_DUMP_FACEO: perCell(faceO.data(), sizeof(R), 3); break;
+      case ST_DUMP_FACEV: if constexpr (kF32) perCell(accVi.data(), 4, 3); else perCell(faceV.data(), sizeof(R), 3); break;
+      case ST_DUMP_FACEO: if constexpr (kF32) perCell(accWi.data(), 4, 3); else perCell(faceO.data(), sizeof(R), 3); break;
       case ST_DUMP_VELG: perCell(velG.data(), sizeof(R), 3); break;
       case ST_DUMP_POS: std::memcpy(o, pos.data(), size_t(N) * 4 * sizeof(R)); break;
       case ST_DUMP_VEL: std::memcpy(o, vel.data(), size_t(N) * 4 * sizeof(R)); break;

@@ -22,6 +22,7 @@ STAGES = ["sort", "blocks", "clear", "p2g", "narrow band", "classify", "normFace
 (ST_STAGE_SORT, ST_STAGE_BLOCKS, ST_STAGE_CLEAR, ST_STAGE_P2G, ST_STAGE_NARROWBAND,
  ST_STAGE_CLASSIFY, ST_STAGE_NORMFACES, ST_STAGE_DIVERGENCE, ST_STAGE_PRESSURE,
  ST_STAGE_PROJECT, ST_STAGE_G2P, ST_STAGE_SPRAY, ST_STAGE_RESEED) = range(13)
+ST_STAGE_COUNT = 13
 
 DUMP = {"tiles": 0, "cnt": 1, "ctype": 2, "pres": 3, "rhs": 4, "accv": 5, "accw": 6,
         "facev": 7, "faceo": 8, "velg": 9, "pos": 10, "vel": 11, "tmap": 12}

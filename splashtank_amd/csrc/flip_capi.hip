@@ -91,7 +91,9 @@ struct Sim {
     L.G = G;
     L.n = N;
     L.ntiles = ntiles;
+    L.scap = scap;
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
+    L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     return L;
   }
 };
@@ -139,7 +141,7 @@ void destroySim(Sim* s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
-  dfree(s->G.binKey); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.binCursor);
+  dfree(s->G.keyRank); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl);
   dfree(s->G.scratch4);
   freeSlots(s);
@@ -314,7 +316,11 @@ int st_create(const st_opts* o, void** out) {
   hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   if (e != hipSuccess) { destroySim(s); return fail(ST_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
   size_t M = s->maxn;
+  uint32_t nbin = s->ntiles + 1;
   if ((rc = dalloc(&s->G.pos, M)) || (rc = dalloc(&s->G.vel, M)) || (rc = dalloc(&s->G.pid, M)) || (rc = dalloc(&s->G.scratch4, M)) ||
+      (rc = dalloc(&s->G.posB, M)) || (rc = dalloc(&s->G.velB, M)) || (rc = dalloc(&s->G.pidB, M)) || (rc = dalloc(&s->G.keyRank, M)) ||
+      (rc = dalloc(&s->G.binCount, nbin)) || (rc = dalloc(&s->G.binStart, size_t(nbin) + 1)) ||
+      (rc = dalloc(&s->G.scanTmp, size_t(nbin + 1023) / 1024 + 1)) ||
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
       (rc = dalloc(&s->dSlots, RING))) {
@@ -481,8 +487,7 @@ int st_read_particles(void* p, float* dst4, uint32_t n) {
   n = std::min(n, s->N);
   if (!n) return ST_OK;
   HIPCHK(hipSetDevice(s->dev));
-  LaunchCtx L = s->ctx();
-  launch_unsort(s->st, L, s->G.scratch4, n);
+  launch_unsort(s->st, s->G.pos, s->G.pid, s->G.scratch4, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dst4, s->G.scratch4, size_t(n) * 16, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -601,8 +606,6 @@ int st_dump(void* p, int32_t what, void* dst, size_t bytes) {
     case ST_DUMP_POS:
     case ST_DUMP_VEL: {
       if (!s->N) return ST_OK;
-      LaunchCtx L = s->ctx();
-      (void)L;
       // id order: out[pid[i]] = field[i]
       std::vector<float> f(size_t(s->N) * 4);
       std::vector<uint32_t> id(s->N);
@@ -631,7 +634,7 @@ int st_mem_bytes(void* p, uint64_t* bytes) {
   Sim* s = getSim(p);
   if (!s || !bytes) return fail(ST_ERR_ARG, "st_mem_bytes: null argument");
   uint64_t M = s->maxn, C = uint64_t(s->scap) * 512;
-  *bytes = M * (16 + 16 + 4 + 16) + uint64_t(s->ntiles) * 16 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
+  *bytes = M * (2 * (16 + 16 + 4) + 8 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
   return ST_OK;
 }
 

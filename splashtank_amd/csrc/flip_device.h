@@ -48,10 +48,10 @@ struct DGrid {
   float4* posB;  // bin-sorted copy read by p2g and g2p
   float4* velB;
   uint32_t* pidB;
-  uint32_t* binKey;    // per particle (sorted order): tile of its cell, or ntiles if dead
+  uint2* keyRank;      // per particle (A order): bin (tile of its cell, ntiles if dead), rank in bin
   uint32_t* binCount;  // per tile (+1 dead bin): particles whose cell is in the tile
-  uint32_t* binStart;  // exclusive prefix of binCount (ntiles + 2 entries)
-  uint32_t* binCursor;
+  uint32_t* binStart;  // exclusive prefix of binCount (ntiles + 2 entries, last = N)
+  uint32_t* scanTmp;   // per-1024-block sums of the bin scan
   uint32_t* tmap;
   uint32_t* need;
   uint32_t* newList;   // pairs: tile, slot (or NONE)

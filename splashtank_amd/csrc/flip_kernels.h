@@ -12,7 +12,9 @@ struct LaunchCtx {
   DGrid G;
   uint32_t n;         // particles (host copy of P->n)
   uint32_t ntiles;
+  uint32_t scap;
   uint32_t cellGrid;  // workgroups of grid-stride loops over active cells
+  uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
@@ -25,4 +27,4 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L);
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
-void launch_unsort(hipStream_t st, const LaunchCtx& L, float4* out, uint32_t n);
+void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);

@@ -1,16 +1,37 @@
 // flip_kernels.hip - HIP kernels of the FLIP substep for gfx950 (wave64).
 //
-// One kernel per step of the reference's stage list (src/flip/sim.js:245-310),
-// except where noted.  Work over active cells is a grid-stride loop over the
-// device-side count of tiles in use (ctl[4]), so no launch needs a host readback
-// and every launch can be captured in a graph.
+// Layout of the work:
+//  * particles are counting-sorted by the tile of their cell every substep (the
+//    tile marking pass also counts bins), so p2g and g2p run one workgroup per
+//    tile over a contiguous run of particles;
+//  * p2g accumulates the tile's 10^3-cell window (the tile plus one cell all
+//    round) with LDS integer atomics and adds the window to HBM once; the sums
+//    are fixed point, so the order the atomics land in does not matter;
+//  * g2p stages the same window of face velocities and counts in LDS;
+//  * cell kernels are grid-stride loops over the device-side count of tiles in
+//    use (ctl[4]), so no launch needs a host readback and all can live in a graph;
+//  * the red-black SOR runs one workgroup per tile and colour, 256 threads for
+//    the 256 cells of the colour, with cross-tile neighbours from a per-slot
+//    table of the six face-neighbour slots.
+// The reference's kernels each of these replaces are cited at the kernel.
 #include "flip_device.h"
 #include "flip_kernels.h"
 
 #define WG 256
+#define WIN 10      // window edge: tile of 8 plus one cell each side
+#define WIN3 1000
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
 
 // ---------------------------------------------------------------------------
 // generic fills (no hipMemsetAsync, so the launches can live in a graph)
@@ -20,25 +41,122 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// blocks (wgsl-blocks.js:4-60)
+// blocks (wgsl-blocks.js:4-60) and the particle bins
 
 // tileMark + markAround (wgsl-blocks.js:10-14, wgsl-core.js:239-246): every tile
-// within two cells of a live particle is needed.  All writers store 1.
-__global__ void k_tile_mark(const DParams* __restrict__ PP, DGrid G, const float4* __restrict__ pos) {
+// within two cells of a live particle is needed.  The same pass counts each
+// particle into the bin of its cell's tile (dead particles into bin ntiles) and
+// keeps its rank in the bin.  Consecutive particles mostly share a bin (the
+// array is in last step's bin order), so the counting is one atomic per run of
+// equal bins in a wave.
+__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
   const DParams& P = *PP;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  float4 q = pos[i];
-  if (q.y < ST_DEADY) return;
-  int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
-  int32_t lx = max(cc.x - 2, 0) >> 3, ly = max(cc.y - 2, 0) >> 3, lz = max(cc.z - 2, 0) >> 3;
-  int32_t hx = min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3, hy = min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3, hz = min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3;
-  for (int32_t tk = lz; tk <= hz; tk++)
-    for (int32_t tj = ly; tj <= hy; tj++)
-      for (int32_t ti = lx; ti <= hx; ti++) {
-        uint32_t t = uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk));
-        if (G.need[t] == 0u) G.need[t] = 1u;
-      }
+  bool valid = i < P.n;
+  float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
+  bool live = valid && !(q.y < ST_DEADY);
+  uint32_t key = P.ntiles;
+  if (live) {
+    int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
+    key = d_tileOf(P, cc.x, cc.y, cc.z);
+    int32_t lx = max(cc.x - 2, 0) >> 3, ly = max(cc.y - 2, 0) >> 3, lz = max(cc.z - 2, 0) >> 3;
+    int32_t hx = min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3, hy = min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3,
+            hz = min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3;
+    for (int32_t tk = lz; tk <= hz; tk++)
+      for (int32_t tj = ly; tj <= hy; tj++)
+        for (int32_t ti = lx; ti <= hx; ti++) {
+          uint32_t t = uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk));
+          if (G.need[t] == 0u) G.need[t] = 1u;
+        }
+  }
+  uint64_t rem = __ballot(valid);
+  uint32_t rank = 0;
+  while (rem) {
+    int leader = __ffsll((unsigned long long)rem) - 1;
+    uint32_t k = (uint32_t)__shfl((int)key, leader);
+    uint64_t m = __ballot(valid && key == k) & rem;
+    uint32_t base = 0;
+    if (int(lane_id()) == leader) base = atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if ((m >> lane_id()) & 1ull) rank = base + (uint32_t)__popcll(m & lanemask_lt());
+    rem &= ~m;
+  }
+  if (valid) G.keyRank[i] = make_uint2(key, rank);
+}
+
+// exclusive scan of n counts into out[0..n], out[n] = total; three launches
+__global__ void __launch_bounds__(WG) k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+                                                    uint32_t* __restrict__ sums) {
+  __shared__ uint32_t wsum[WG / 64];
+  uint32_t base = blockIdx.x * (WG * 4) + threadIdx.x * 4;
+  uint32_t v[4], s = 0;
+  for (int q = 0; q < 4; q++) {
+    v[q] = base + q < n ? in[base + q] : 0u;
+    s += v[q];
+  }
+  uint32_t inc = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t o = (uint32_t)__shfl_up((int)inc, off);
+    if (int(lane_id()) >= off) inc += o;
+  }
+  uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (uint32_t u = 0; u < WG / 64; u++) {
+    if (u < w) before += wsum[u];
+    tot += wsum[u];
+  }
+  uint32_t run = before + inc - s;
+  for (int q = 0; q < 4; q++) {
+    if (base + q < n) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n) {
+  __shared__ uint32_t part[1024 / 64];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+    uint32_t i = b0 + threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0u, inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t o = (uint32_t)__shfl_up((int)inc, off);
+      if (int(lane_id()) >= off) inc += o;
+    }
+    uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 63) part[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t u = 0; u < 1024 / 64; u++) {
+      if (u < w) before += part[u];
+      tot += part[u];
+    }
+    if (i < nb) sums[i] = carry + before + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums) {
+  uint32_t base = blockIdx.x * (WG * 4) + threadIdx.x * 4;
+  uint32_t add = sums[blockIdx.x];
+  for (int q = 0; q < 4; q++)
+    if (base + q < n) out[base + q] += add;
+}
+
+// scatter into bin order: A (pos, vel, pid) -> B
+__global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= PP->n) return;
+  uint2 kr = G.keyRank[i];
+  uint32_t dst = G.binStart[kr.x] + kr.y;
+  G.posB[dst] = G.pos[i];
+  G.velB[dst] = G.vel[i];
+  G.pidB[dst] = G.pid[i];
 }
 
 // Flags of one tile: freed (no longer needed, had a slot), new (needed, had
@@ -84,9 +202,8 @@ __device__ __forceinline__ Ranks3 wg_ranks3(bool a, bool b, bool c) {
 }
 
 __global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ PP, DGrid G) {
-  const DParams& P = *PP;
   uint32_t t = blockIdx.x * WG + threadIdx.x;
-  TileFlags f = tile_flags(G, t, P.ntiles);
+  TileFlags f = tile_flags(G, t, PP->ntiles);
   Ranks3 R = wg_ranks3(f.fr, f.nw, f.kp);
   if (threadIdx.x == 0) {
     G.wg[blockIdx.x * 3 + 0] = R.tot[0];
@@ -99,8 +216,7 @@ __global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ P
 // arithmetic of tileFree/tileAlloc/tileFix (wgsl-blocks.js:16-35) in closed form:
 // the k-th new tile (tile order) pops stack entry top1-1-k, so the first
 // min(new, top1) new tiles get slots and the rest count as no room.
-__global__ void __launch_bounds__(1024) k_tile_scan(const DParams* __restrict__ PP, DGrid G, uint32_t nwg) {
-  const DParams& P = *PP;
+__global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) {
   __shared__ uint32_t part[3][1024 / 64];
   __shared__ uint32_t carry[3];
   if (threadIdx.x < 3) carry[threadIdx.x] = 0;
@@ -109,11 +225,10 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const DParams* __restrict__ 
     uint32_t i = base + threadIdx.x;
     uint32_t v[3];
     for (int q = 0; q < 3; q++) v[q] = i < nwg ? G.wg[i * 3 + q] : 0u;
-    // inclusive wave scan
     uint32_t s[3] = {v[0], v[1], v[2]};
     for (int off = 1; off < 64; off <<= 1)
       for (int q = 0; q < 3; q++) {
-        uint32_t o = __shfl_up(s[q], off);
+        uint32_t o = (uint32_t)__shfl_up((int)s[q], off);
         if (int(lane_id()) >= off) s[q] += o;
       }
     uint32_t w = threadIdx.x >> 6;
@@ -146,15 +261,13 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const DParams* __restrict__ 
     G.ctl[13] = top1;
     G.ctl[14] = nalloc;
   }
-  (void)P;
 }
 
 // Freed tiles push their slot (tile order), new tiles are listed (tile order),
 // kept tiles take their place in slotList (tile order).
 __global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__ PP, DGrid G) {
-  const DParams& P = *PP;
   uint32_t t = blockIdx.x * WG + threadIdx.x;
-  TileFlags f = tile_flags(G, t, P.ntiles);
+  TileFlags f = tile_flags(G, t, PP->ntiles);
   Ranks3 R = wg_ranks3(f.fr, f.nw, f.kp);
   uint32_t baseF = G.wg[blockIdx.x * 3 + 0], baseN = G.wg[blockIdx.x * 3 + 1], baseK = G.wg[blockIdx.x * 3 + 2];
   uint32_t top0 = G.ctl[12], nalloc = G.ctl[14];
@@ -172,7 +285,7 @@ __global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__
 }
 
 // tileAlloc (wgsl-blocks.js:26-33), deterministic pop order
-__global__ void k_tile_alloc(const DParams* __restrict__ PP, DGrid G) {
+__global__ void k_tile_alloc(DGrid G) {
   uint32_t nn = G.ctl[3], top1 = G.ctl[13], nalloc = G.ctl[14];
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += gridDim.x * blockDim.x) {
     if (k < nalloc) {
@@ -185,7 +298,28 @@ __global__ void k_tile_alloc(const DParams* __restrict__ PP, DGrid G) {
       G.newList[k * 2 + 1] = ST_NONE;
     }
   }
-  (void)PP;
+}
+
+// slots of the six face-neighbour tiles of every tile in use (0: none)
+__global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  uint32_t sc = G.ctl[4];
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < sc; b += gridDim.x * blockDim.x) {
+    uint32_t s = G.slotList[b], t = G.slotTile[s];
+    int32_t ti = int32_t(t % P.tx), tj = int32_t((t / P.tx) % P.ty), tk = int32_t(t / (P.tx * P.ty));
+    int32_t X = int32_t(P.tx), Y = int32_t(P.ty), Z = int32_t(P.tz);
+    uint32_t nb[6];
+    for (int a = 0; a < 6; a++) {
+      int32_t x = ti + (a == 0 ? -1 : (a == 1 ? 1 : 0)), y = tj + (a == 2 ? -1 : (a == 3 ? 1 : 0)), z = tk + (a == 4 ? -1 : (a == 5 ? 1 : 0));
+      uint32_t v = 0u;
+      if (x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z) {
+        uint32_t q = G.tmap[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))];
+        v = q == ST_NONE ? 0u : q;
+      }
+      nb[a] = v;
+    }
+    for (int a = 0; a < 6; a++) G.nbr[s * 6 + a] = nb[a];
+  }
 }
 
 // initNew (wgsl-blocks.js:38-47): no pressure, collider lookup of each cell
@@ -210,7 +344,7 @@ __global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restri
 
 // ---------------------------------------------------------------------------
 // clearSlots (wgsl-solver.js:5-15): every slot in use plus the spare slot 0
-__global__ void k_clear(const DParams* __restrict__ PP, DGrid G) {
+__global__ void k_clear(DGrid G) {
   uint32_t sc = G.ctl[4];
   uint32_t tot = (sc + 1u) * 512u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -226,56 +360,127 @@ __global__ void k_clear(const DParams* __restrict__ PP, DGrid G) {
     G.accW[c * 3] = 0; G.accW[c * 3 + 1] = 0; G.accW[c * 3 + 2] = 0;
     if (s == 0u) G.rhs[c] = 8.0f;
   }
-  (void)PP;
 }
 
 // ---------------------------------------------------------------------------
-// p2g (wgsl-solver.js:16-40): fixed-point sums, so the result does not depend on
-// the order the atomics land in.
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-  return v;
-}
-__global__ void __launch_bounds__(WG) k_p2g_global(const DParams* __restrict__ PP, DGrid G, const float4* __restrict__ pos, const float4* __restrict__ vel) {
-  const DParams& P = *PP;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool ok = i < P.n;
-  float4 p4 = ok ? pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
-  ok = ok && !(p4.y < ST_DEADY);
-  float3 p = make_float3(p4.x, p4.y, p4.z);
-  int3 cc = d_particleCell(P, p);
-  ok = ok && G.tmap[d_tileOf(P, cc.x, cc.y, cc.z)] != ST_NONE;
-  float4 v4 = ok ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  float3 v = make_float3(v4.x, v4.y, v4.z);
-  uint32_t bits = ok ? st_f2u(f3len(v)) : 0u;
-  bits = wave_max_u32(bits);
-  if (lane_id() == 0 && bits) atomicMax(&G.ctl[6], bits);
-  if (!ok) return;
-  for (uint32_t axis = 0; axis < 3; axis++) {
-    float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
-    if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
-    else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
-    else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
-    float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
-    float3 fr = f3sub(q, f0);
-    int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
-    float vc = f3get(v, axis);
-    for (uint32_t c = 0; c < 8; c++) {
-      int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
-      float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-      if (w <= 0.0f) continue;
-      uint32_t idx = d_faceIndex(P, G.tmap, axis, i0.x + dx, i0.y + dy, i0.z + dz);
-      atomicAdd(&G.accV[idx], st_f2i_sat(rintf(w * vc * P.scale)));
-      atomicAdd(&G.accW[idx], st_f2i_sat(rintf(w * P.scale)));
+// Tile windows.  The window of tile (ti, tj, tk) is cells [8ti-1, 8ti+8] per axis.
+// A window cell's tile is one of the 27 around the tile: offset -1 at window
+// index 0, +1 at 9, else 0.  wslots[27] holds their slots (0 when the tile has
+// no storage, as cellIdx maps it; WSLOT_OUT when outside the tile grid).
+#define WSLOT_OUT 0xfffffffeu
+
+__device__ __forceinline__ void load_wslots(const DParams& P, const DGrid& G, uint32_t t, uint32_t* wslots, int3& org) {
+  int32_t ti = int32_t(t % P.tx), tj = int32_t((t / P.tx) % P.ty), tk = int32_t(t / (P.tx * P.ty));
+  org = make_int3(ti * 8 - 1, tj * 8 - 1, tk * 8 - 1);
+  if (threadIdx.x < 27) {
+    int32_t dx = int32_t(threadIdx.x % 3) - 1, dy = int32_t((threadIdx.x / 3) % 3) - 1, dz = int32_t(threadIdx.x / 9) - 1;
+    int32_t x = ti + dx, y = tj + dy, z = tk + dz;
+    uint32_t v = WSLOT_OUT;
+    if (x >= 0 && y >= 0 && z >= 0 && x < int32_t(P.tx) && y < int32_t(P.ty) && z < int32_t(P.tz)) {
+      uint32_t q = G.tmap[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))];
+      v = q == ST_NONE ? 0u : q;
     }
+    wslots[threadIdx.x] = v;
   }
-  atomicAdd(&G.cnt[d_cellIdx(P, G.tmap, cc.x, cc.y, cc.z)], 1);
+}
+__device__ __forceinline__ int32_t woff(int32_t w) { return w == 0 ? 0 : (w == WIN - 1 ? 2 : 1); }
+// storage index of window cell (wx, wy, wz), or ST_NONE outside the tile grid
+__device__ __forceinline__ uint32_t wcell(const uint32_t* wslots, int32_t wx, int32_t wy, int32_t wz, int3 org) {
+  uint32_t s = wslots[woff(wx) + 3 * (woff(wy) + 3 * woff(wz))];
+  if (s == WSLOT_OUT) return ST_NONE;
+  int32_t gx = org.x + wx, gy = org.y + wy, gz = org.z + wz;
+  return s * 512u + uint32_t(gx & 7) + 8u * (uint32_t(gy & 7) + 8u * uint32_t(gz & 7));
+}
+// face (axis, i, j, k) clamped as faceIndex does (wgsl-core.js:84-89), as a window index or -1
+__device__ __forceinline__ int32_t wface(const DParams& P, uint32_t axis, int32_t i, int32_t j, int32_t k, int3 org) {
+  int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
+  int32_t ci = d_iclamp(i, 0, axis == 0 ? X : X - 1), cj = d_iclamp(j, 0, axis == 1 ? Y : Y - 1), ck = d_iclamp(k, 0, axis == 2 ? Z : Z - 1);
+  int32_t wx = ci - org.x, wy = cj - org.y, wz = ck - org.z;
+  if (uint32_t(wx) >= uint32_t(WIN) || uint32_t(wy) >= uint32_t(WIN) || uint32_t(wz) >= uint32_t(WIN)) return -1;
+  return wx + WIN * (wy + WIN * wz);
+}
+
+// ---------------------------------------------------------------------------
+// p2g (wgsl-solver.js:16-40), one workgroup per tile in use
+__global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  __shared__ int32_t sV[3][WIN3];
+  __shared__ int32_t sW[3][WIN3];
+  __shared__ int32_t sC[512];
+  __shared__ uint32_t wslots[27];
+  uint32_t vmaxBits = 0;
+  uint32_t sc = G.ctl[4];
+  for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
+    uint32_t s = G.slotList[b], t = G.slotTile[s];
+    int3 org;
+    load_wslots(P, G, t, wslots, org);
+    for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
+      sV[0][q] = 0; sV[1][q] = 0; sV[2][q] = 0;
+      sW[0][q] = 0; sW[1][q] = 0; sW[2][q] = 0;
+    }
+    for (uint32_t q = threadIdx.x; q < 512u; q += WG) sC[q] = 0;
+    __syncthreads();
+    uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
+    for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
+      float4 p4 = G.posB[k];
+      float4 v4 = G.velB[k];
+      float3 p = make_float3(p4.x, p4.y, p4.z);
+      float3 v = make_float3(v4.x, v4.y, v4.z);
+      int3 cc = d_particleCell(P, p);
+      vmaxBits = max(vmaxBits, st_f2u(f3len(v)));
+      for (uint32_t axis = 0; axis < 3; axis++) {
+        float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
+        if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
+        else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
+        else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
+        float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
+        float3 fr = f3sub(q, f0);
+        int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
+        float vc = f3get(v, axis);
+        for (uint32_t c = 0; c < 8; c++) {
+          int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+          float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+          if (w <= 0.0f) continue;
+          int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
+          int32_t aw = st_f2i_sat(rintf(w * P.scale));
+          int32_t wi = wface(P, axis, i0.x + dx, i0.y + dy, i0.z + dz, org);
+          if (wi >= 0) {
+            atomicAdd(&sV[axis][wi], av);
+            atomicAdd(&sW[axis][wi], aw);
+          } else {  // outside the window: only for particles far outside the box
+            uint32_t idx = d_faceIndex(P, G.tmap, axis, i0.x + dx, i0.y + dy, i0.z + dz);
+            atomicAdd(&G.accV[idx], av);
+            atomicAdd(&G.accW[idx], aw);
+          }
+        }
+      }
+      atomicAdd(&sC[uint32_t(cc.x & 7) + 8u * (uint32_t(cc.y & 7) + 8u * uint32_t(cc.z & 7))], 1);
+    }
+    __syncthreads();
+    // the window into HBM: counts are the tile's own (plain stores), faces add
+    for (uint32_t q = threadIdx.x; q < 512u; q += WG) G.cnt[s * 512u + q] = sC[q];
+    for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
+      int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
+      int32_t a0 = sW[0][q], a1 = sW[1][q], a2 = sW[2][q];
+      if ((a0 | a1 | a2 | sV[0][q] | sV[1][q] | sV[2][q]) == 0) continue;
+      uint32_t c = wcell(wslots, wx, wy, wz, org);
+      if (c == ST_NONE) continue;
+      for (int a = 0; a < 3; a++) {
+        int32_t vv = sV[a][q], ww = sW[a][q];
+        if (vv) atomicAdd(&G.accV[c * 3 + a], vv);
+        if (ww) atomicAdd(&G.accW[c * 3 + a], ww);
+      }
+    }
+    __syncthreads();
+  }
+  vmaxBits = wave_max_u32(vmaxBits);
+  if (lane_id() == 0 && vmaxBits) atomicMax(&G.ctl[6], vmaxBits);
 }
 
 // ---------------------------------------------------------------------------
 // per active cell kernels: t -> slot slotList[t >> 9], local t & 511 (acell,
 // wgsl-core.js:96-103)
-#define FOR_ACTIVE(G)                                                                   \
+#define FOR_ACTIVE(G)                                                                          \
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, tot_ = (G).ctl[4] * 512u; t < tot_; \
        t += gridDim.x * blockDim.x)
 
@@ -298,9 +503,7 @@ __global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP,
   uint32_t tot = G.ctl[4] * 512u;
   uint32_t stride = gridDim.x * blockDim.x;
   uint32_t nred = 0, nblack = 0;
-  for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < tot; t0 += stride) {
-    uint32_t t = t0 + threadIdx.x;
-    if (t >= tot) break;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += stride) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
     int3 a = d_slotCoord(P, G.slotTile, s, local);
     uint32_t ty = 0;
@@ -330,11 +533,8 @@ __global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP,
       if (((a.x + a.y + a.z) & 1) == 0) nred++; else nblack++;
     }
   }
-  // wave sums, then one atomic per wave and colour
-  for (int off = 32; off > 0; off >>= 1) {
-    nred += (uint32_t)__shfl_xor((int)nred, off);
-    nblack += (uint32_t)__shfl_xor((int)nblack, off);
-  }
+  nred = wave_sum_u32(nred);
+  nblack = wave_sum_u32(nblack);
   if (lane_id() == 0) {
     if (nred) atomicAdd(&G.ctl[0], nred);
     if (nblack) atomicAdd(&G.ctl[1], nblack);
@@ -418,32 +618,39 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
   }
 }
 
-// relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164): one colour over the
-// cells of every tile in use.  Cells of one colour read only the other colour,
-// so this is the reference's iterate exactly.
-__global__ void k_sor(const DParams* __restrict__ PP, DGrid G, uint32_t color) {
+// relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164): one workgroup per
+// tile in use and one thread per cell of the colour (a tile holds 256 of each,
+// tile origins being even).  Cells of one colour read only the other colour, so
+// this is the reference's iterate exactly; neighbours are summed in its order.
+__global__ void __launch_bounds__(WG) k_sor_tile(const DParams* __restrict__ PP, DGrid G, uint32_t color) {
   const DParams& P = *PP;
-  FOR_ACTIVE(G) {
-    uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+  uint32_t sc = G.ctl[4];
+  const float omega = P.omega;
+  uint32_t tid = threadIdx.x;
+  uint32_t y = (tid >> 2) & 7u, z = tid >> 5;
+  uint32_t x = 2u * (tid & 3u) + ((y + z + color) & 1u);
+  uint32_t local = x + 8u * (y + 8u * z);
+  for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
+    uint32_t s = G.slotList[b];
+    uint32_t c = s * 512u + local;
     uint32_t st = G.sten[c];
     if (!(st & STEN_FLUID)) continue;
-    int3 p = d_slotCoord(P, G.slotTile, s, local);
-    if (uint32_t((p.x + p.y + p.z) & 1) != color) continue;
     uint32_t nns = st & 7u, mask = (st >> 4) & 63u;
     if (nns == 0u) {
       G.pres[c] = 0.0f;
       continue;
     }
+    const uint32_t* nb = G.nbr + s * 6u;
     float sum = 0.0f;
-    if (mask & 1u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x - 1, p.y, p.z)];
-    if (mask & 2u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x + 1, p.y, p.z)];
-    if (mask & 4u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x, p.y - 1, p.z)];
-    if (mask & 8u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x, p.y + 1, p.z)];
-    if (mask & 16u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x, p.y, p.z - 1)];
-    if (mask & 32u) sum = sum + G.pres[d_cellIdx(P, G.tmap, p.x, p.y, p.z + 1)];
+    if (mask & 1u) sum = sum + G.pres[x > 0u ? c - 1u : nb[0] * 512u + local + 7u];
+    if (mask & 2u) sum = sum + G.pres[x < 7u ? c + 1u : nb[1] * 512u + local - 7u];
+    if (mask & 4u) sum = sum + G.pres[y > 0u ? c - 8u : nb[2] * 512u + local + 56u];
+    if (mask & 8u) sum = sum + G.pres[y < 7u ? c + 8u : nb[3] * 512u + local - 56u];
+    if (mask & 16u) sum = sum + G.pres[z > 0u ? c - 64u : nb[4] * 512u + local + 448u];
+    if (mask & 32u) sum = sum + G.pres[z < 7u ? c + 64u : nb[5] * 512u + local - 448u];
     float pg = (G.rhs[c] + sum) / float(nns);
     float pc = G.pres[c];
-    G.pres[c] = pc + P.omega * (pg - pc);
+    G.pres[c] = pc + omega * (pg - pc);
   }
 }
 
@@ -500,23 +707,55 @@ __device__ __forceinline__ float d_vnz(float3 x) {
   return st_mixf(a, b, u.z);
 }
 
-__device__ __forceinline__ void d_sampleBoth(const DParams& P, const DGrid& G, float3 p, float3& vn, float3& vo) {
+// Grid access for g2p: from the global arrays (tmap lookups), or from a tile
+// window staged in LDS.  Both return the same words.
+struct GlobalGrid {
+  const DParams& P;
+  const DGrid& G;
+  __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
+  __device__ float fold(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gv0R(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
+  __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const { return G.cnt[d_cellIdx(P, G.tmap, i, j, k)]; }
+};
+struct WindowGrid {
+  const DParams& P;
+  const DGrid& G;
+  const float (*sN)[WIN3];
+  const float (*sO)[WIN3];
+  const int32_t* sC;
+  int3 org;
+  __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
+    int32_t w = wface(P, axis, i, j, k, org);
+    return w >= 0 ? sN[axis][w] : gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k));
+  }
+  __device__ float fold(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
+    int32_t w = wface(P, axis, i, j, k, org);
+    return w >= 0 ? sO[axis][w] : gv0R(G, d_faceIndex(P, G.tmap, axis, i, j, k));
+  }
+  __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const {
+    int32_t wx = i - org.x, wy = j - org.y, wz = k - org.z;
+    if (uint32_t(wx) < uint32_t(WIN) && uint32_t(wy) < uint32_t(WIN) && uint32_t(wz) < uint32_t(WIN)) return sC[wx + WIN * (wy + WIN * wz)];
+    return G.cnt[d_cellIdx(P, G.tmap, i, j, k)];
+  }
+};
+
+// sampleBoth (wgsl-core.js:138-154)
+template <class GridT>
+__device__ __forceinline__ void d_sampleBoth(const DParams& P, const GridT& g, float3 p, float3& vn, float3& vo) {
   float rn[3], ro[3];
   for (uint32_t axis = 0; axis < 3; axis++) {
-    float3 g = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
-    if (axis == 0) { g.y = g.y - 0.5f; g.z = g.z - 0.5f; }
-    else if (axis == 1) { g.x = g.x - 0.5f; g.z = g.z - 0.5f; }
-    else { g.x = g.x - 0.5f; g.y = g.y - 0.5f; }
-    float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
-    float3 fr = f3sub(g, f0);
+    float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
+    if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
+    else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
+    else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
+    float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
+    float3 fr = f3sub(q, f0);
     int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
     float sn = 0.0f, so = 0.0f;
     for (uint32_t c = 0; c < 8; c++) {
       int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
       float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-      uint32_t f = d_faceIndex(P, G.tmap, axis, i0.x + dx, i0.y + dy, i0.z + dz);
-      sn = sn + w * gvR(G, f);
-      so = so + w * gv0R(G, f);
+      sn = sn + w * g.fnew(axis, i0.x + dx, i0.y + dy, i0.z + dz);
+      so = so + w * g.fold(axis, i0.x + dx, i0.y + dy, i0.z + dz);
     }
     rn[axis] = sn;
     ro[axis] = so;
@@ -525,39 +764,31 @@ __device__ __forceinline__ void d_sampleBoth(const DParams& P, const DGrid& G, f
   vo = make_float3(ro[0], ro[1], ro[2]);
 }
 
-__global__ void __launch_bounds__(WG) k_g2p(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G,
-                                            const float4* __restrict__ posIn, const float4* __restrict__ velIn,
-                                            float4* __restrict__ posOut, float4* __restrict__ velOut) {
-  const DParams& P = *PP;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  float4 p4 = posIn[i];
-  float4 v04 = velIn[i];
-  if (p4.y < ST_DEADY) {
-    posOut[i] = p4;
-    velOut[i] = v04;
-    return;
-  }
+// one particle of g2p: FLIP/PIC blend, foam, air, speed cap, Euler advection,
+// collisions, open faces (wgsl-solver.js:208-275)
+template <class GridT>
+__device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, const GridT& g,
+                                             bool storage, float4 p4, float4 v04, float4& po, float4& vo4) {
   float3 p = make_float3(p4.x, p4.y, p4.z);
   float3 v0 = make_float3(v04.x, v04.y, v04.z);
   int3 fcell = d_particleCell(P, p);
   float3 vn = make_float3(v0.x - 0.0f, v0.y - P.gravity * P.dt, v0.z - 0.0f);
   float3 vo = v0;
-  if (G.tmap[d_tileOf(P, fcell.x, fcell.y, fcell.z)] != ST_NONE) d_sampleBoth(P, G, p, vn, vo);
+  if (storage) d_sampleBoth(P, g, p, vn, vo);
   float3 fl = f3add(v0, f3sub(vn, vo));
   float3 v = make_float3(st_mixf(vn.x, fl.x, P.flip), st_mixf(vn.y, fl.y, P.flip), st_mixf(vn.z, fl.z, P.flip));
   int3 bc = make_int3(st_f2i_sat(floorf((p.x - P.x0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.y - P.y0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.z - P.z0) * P.ih - 0.5f)));
   float csum = 8.0f * P.rest;
   bool fast = P.q12 > 0.5f;
   if (P.q9 < 0.5f && fast) {
-    csum = 8.0f * float(G.cnt[d_cellIdx(P, G.tmap, fcell.x, fcell.y, fcell.z)]);
+    csum = 8.0f * float(g.cnt(fcell.x, fcell.y, fcell.z));
   } else if (P.q9 < 0.5f) {
     csum = 0.0f;
     for (int32_t dz = 0; dz <= 1; dz++)
       for (int32_t dy = 0; dy <= 1; dy++)
         for (int32_t dx = 0; dx <= 1; dx++) {
           int32_t cx = d_iclamp(bc.x + dx, 0, int32_t(P.nx) - 1), cy = d_iclamp(bc.y + dy, 0, int32_t(P.ny) - 1), cz = d_iclamp(bc.z + dz, 0, int32_t(P.nz) - 1);
-          csum = csum + float(G.cnt[d_cellIdx(P, G.tmap, cx, cy, cz)]);
+          csum = csum + float(g.cnt(cx, cy, cz));
         }
   }
   float rho = csum / (8.0f * P.rest);
@@ -588,7 +819,7 @@ __global__ void __launch_bounds__(WG) k_g2p(const DParams* __restrict__ PP, cons
   float sp = f3len(v);
   if (sp > P.vmax) v = f3mul(v, P.vmax / sp);
   float3 pt = f3add(p, f3mul(v, P.dt));
-  float3 pc = d_collideP(P, C, G.sdf, pt);
+  float3 pc = d_collideP(P, C, sdf, pt);
   float3 dsp = f3sub(pc, pt);
   if (f3dot(dsp, dsp) > 1.0e-12f) {
     float3 nrm = f3div(dsp, f3len(dsp));
@@ -598,12 +829,72 @@ __global__ void __launch_bounds__(WG) k_g2p(const DParams* __restrict__ PP, cons
   uint32_t o = P.open;
   if (o != 0u && (((o & 1u) && pc.x < P.x0) || ((o & 2u) && pc.x > P.x1) || ((o & 4u) && pc.y < P.y0) || ((o & 8u) && pc.y > P.y1) ||
                   ((o & 16u) && pc.z < P.z0) || ((o & 32u) && pc.z > P.z1))) {
-    posOut[i] = make_float4(0.0f, -1.0e6f, 0.0f, 0.0f);
-    velOut[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    po = make_float4(0.0f, -1.0e6f, 0.0f, 0.0f);
+    vo4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
-  posOut[i] = make_float4(pc.x, pc.y, pc.z, fminf(f3len(v), 63.0f) + 64.0f * floorf(foam * 15.0f + 0.5f));
-  velOut[i] = make_float4(v.x, v.y, v.z, foam);
+  po = make_float4(pc.x, pc.y, pc.z, fminf(f3len(v), 63.0f) + 64.0f * floorf(foam * 15.0f + 0.5f));
+  vo4 = make_float4(v.x, v.y, v.z, foam);
+}
+
+// g2p over the bins: one workgroup per tile (bin ntiles holds the dead
+// particles, which are copied).  Reads B, writes A in the same (bin) order.
+__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  __shared__ float sN[3][WIN3];
+  __shared__ float sO[3][WIN3];
+  __shared__ int32_t sC[WIN3];
+  __shared__ uint32_t wslots[27];
+  for (uint32_t t = blockIdx.x; t <= P.ntiles; t += gridDim.x) {
+    uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
+    if (beg == end) continue;
+    if (t == P.ntiles) {
+      for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
+        G.pos[k] = G.posB[k];
+        G.vel[k] = G.velB[k];
+        G.pid[k] = G.pidB[k];
+      }
+      continue;
+    }
+    bool storage = G.tmap[t] != ST_NONE;
+    if (!storage) {  // its block had no room this step: falls freely (wgsl-solver.js:215)
+      GlobalGrid gg{P, G};
+      for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
+        float4 po, vo;
+        g2p_particle(P, C, G.sdf, gg, false, G.posB[k], G.velB[k], po, vo);
+        G.pos[k] = po;
+        G.vel[k] = vo;
+        G.pid[k] = G.pidB[k];
+      }
+      continue;
+    }
+    int3 org;
+    load_wslots(P, G, t, wslots, org);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
+      int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
+      uint32_t c = wcell(wslots, wx, wy, wz, org);
+      if (c == ST_NONE) {
+        sN[0][q] = 0.0f; sN[1][q] = 0.0f; sN[2][q] = 0.0f;
+        sO[0][q] = 0.0f; sO[1][q] = 0.0f; sO[2][q] = 0.0f;
+        sC[q] = 0;
+        continue;
+      }
+      sN[0][q] = gvR(G, c * 3); sN[1][q] = gvR(G, c * 3 + 1); sN[2][q] = gvR(G, c * 3 + 2);
+      sO[0][q] = gv0R(G, c * 3); sO[1][q] = gv0R(G, c * 3 + 1); sO[2][q] = gv0R(G, c * 3 + 2);
+      sC[q] = G.cnt[c];
+    }
+    __syncthreads();
+    WindowGrid wg{P, G, sN, sO, sC, org};
+    for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
+      float4 po, vo;
+      g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
+      G.pos[k] = po;
+      G.vel[k] = vo;
+      G.pid[k] = G.pidB[k];
+    }
+    __syncthreads();
+  }
 }
 
 // kept grid velocity at a point, all three components (wgsl-core.js:185-197)
@@ -629,11 +920,10 @@ __device__ __forceinline__ float3 d_velGVec(const DParams& P, const DGrid& G, fl
 
 // ---------------------------------------------------------------------------
 // readback in id order (unsort, wgsl-blocks.js:98-99)
-__global__ void k_unsort(const DParams* __restrict__ PP, const float4* __restrict__ pos, const uint32_t* __restrict__ pid, float4* __restrict__ out, uint32_t n) {
+__global__ void k_unsort(const float4* __restrict__ src, const uint32_t* __restrict__ pid, float4* __restrict__ out, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[pid[i]] = pos[i];
-  (void)PP;
+  out[pid[i]] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -647,25 +937,37 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(WG), 0, st, p, n, v);
 }
 
+static void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums) {
+  uint32_t nb = gridFor(n, WG * 4);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(WG), 0, st, in, out, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nb, out, n);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(WG), 0, st, out, n, (const uint32_t*)sums);
+}
+
 void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   uint32_t nt = L.ntiles;
   launch_fill_u32(st, L.G.need, nt, 0u);
-  if (L.n) hipLaunchKernelGGL(k_tile_mark, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G, (const float4*)L.G.pos);
+  launch_fill_u32(st, L.G.binCount, size_t(nt) + 1, 0u);
+  if (L.n) hipLaunchKernelGGL(k_mark_bin, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
   uint32_t nwg = gridFor(nt, WG);
   hipLaunchKernelGGL(k_tile_count, dim3(nwg), dim3(WG), 0, st, L.P, L.G);
-  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, L.P, L.G, nwg);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, L.G, nwg);
   hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(WG), 0, st, L.P, L.G);
-  hipLaunchKernelGGL(k_tile_alloc, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(nt, WG), 256u))), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_tile_alloc, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(nt, WG), 256u))), dim3(WG), 0, st, L.G);
   hipLaunchKernelGGL(k_init_new, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
+  hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.scap, WG), 512u))), dim3(WG), 0, st, L.P, L.G);
+  // particle bins: exclusive scan of the counts, then A -> B in bin order
+  launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp);
+  if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
 }
 
 void launch_clear(hipStream_t st, const LaunchCtx& L) {
-  hipLaunchKernelGGL(k_clear, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_clear, dim3(L.cellGrid), dim3(WG), 0, st, L.G);
 }
 
 void launch_p2g(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
-  hipLaunchKernelGGL(k_p2g_global, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G, (const float4*)L.G.pos, (const float4*)L.G.vel);
+  hipLaunchKernelGGL(k_p2g_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G);
 }
 
 void launch_classify(hipStream_t st, const LaunchCtx& L, bool solid) {
@@ -683,8 +985,8 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
 
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
   for (int it = 0; it < iters; it++) {
-    hipLaunchKernelGGL(k_sor, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, 0u);
-    hipLaunchKernelGGL(k_sor, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, 1u);
+    hipLaunchKernelGGL(k_sor_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, 0u);
+    hipLaunchKernelGGL(k_sor_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, 1u);
   }
 }
 
@@ -694,11 +996,11 @@ void launch_project(hipStream_t st, const LaunchCtx& L) {
 
 void launch_g2p(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
-  hipLaunchKernelGGL(k_g2p, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.C, L.G, (const float4*)L.G.pos, (const float4*)L.G.vel,
-                     L.G.pos, L.G.vel);
+  uint32_t g = std::min<uint32_t>(L.ntiles + 1, 8192u);
+  hipLaunchKernelGGL(k_g2p_tile, dim3(g), dim3(WG), 0, st, L.P, L.C, L.G);
 }
 
-void launch_unsort(hipStream_t st, const LaunchCtx& L, float4* out, uint32_t n) {
+void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n) {
   if (!n) return;
-  hipLaunchKernelGGL(k_unsort, dim3(gridFor(n, WG)), dim3(WG), 0, st, L.P, (const float4*)L.G.pos, (const uint32_t*)L.G.pid, out, n);
+  hipLaunchKernelGGL(k_unsort, dim3(gridFor(n, WG)), dim3(WG), 0, st, src, pid, out, n);
 }
