@@ -104,6 +104,7 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.cnt, C))) return rc;
   if ((rc = dalloc(&s->G.ctype, C))) return rc;
   if ((rc = dalloc(&s->G.pres, C))) return rc;
+  if ((rc = dalloc(&s->G.pres2, C))) return rc;
   if ((rc = dalloc(&s->G.rhs, C))) return rc;
   if ((rc = dalloc(&s->G.sten, C))) return rc;
   if ((rc = dalloc(&s->G.accV, C * 3))) return rc;
@@ -117,7 +118,7 @@ int allocSlots(Sim* s, uint32_t n) {
 }
 
 void freeSlotArrays(DGrid& g) {
-  dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.rhs); dfree(g.sten);
+  dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.pres2); dfree(g.rhs); dfree(g.sten);
   dfree(g.accV); dfree(g.accW); dfree(g.velG);
   dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr);
 }
@@ -129,6 +130,7 @@ int zeroSlots(Sim* s, uint32_t from, uint32_t to) {
   HIPCHK(hipMemsetAsync(s->G.cnt + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.ctype + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.pres + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.pres2 + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.rhs + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.sten + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.accV + a * 3, 0, n * 12, s->st));
@@ -142,7 +144,7 @@ void destroySim(Sim* s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
   dfree(s->G.keyRank); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
-  dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl);
+  dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part);
   dfree(s->G.scratch4);
   freeSlots(s);
   dfree(s->dSdf);
@@ -323,6 +325,7 @@ int st_create(const st_opts* o, void** out) {
       (rc = dalloc(&s->G.scanTmp, size_t(nbin + 1023) / 1024 + 1)) ||
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
+      (rc = dalloc(&s->G.part, 2 * 8192 + 64)) ||
       (rc = dalloc(&s->dSlots, RING))) {
     destroySim(s);
     return rc;
@@ -512,7 +515,7 @@ int st_grow(void* p, uint32_t n) {
   HIPCHK(hipStreamSynchronize(s->st));
   uint32_t old = s->scap;
   DGrid keep = s->G;
-  s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
+  s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.pres2 = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
   s->G.accV = nullptr; s->G.accW = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
   s->G.slotList = nullptr; s->G.nbr = nullptr;
   int rc = allocSlots(s, n);
@@ -522,6 +525,7 @@ int st_grow(void* p, uint32_t n) {
   HIPCHK(hipMemcpy(s->G.cnt, keep.cnt, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.ctype, keep.ctype, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.pres, keep.pres, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.pres2, keep.pres2, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.rhs, keep.rhs, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.sten, keep.sten, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.accV, keep.accV, C * 12, hipMemcpyDeviceToDevice));

@@ -60,10 +60,12 @@ struct DGrid {
   uint32_t* slotList;
   uint32_t* nbr;       // per slot: slots of the 6 face-neighbour tiles (0 if none)
   uint32_t* wg;        // per-workgroup scan scratch
+  uint32_t* part;      // per-workgroup statistics partials (vmax; fluid cells per colour)
   uint32_t* ctl;       // control words, as in the reference's ctl
   int32_t* cnt;
   uint32_t* ctype;
   float* pres;
+  float* pres2;        // SOR ping-pong partner of pres
   float* rhs;
   int32_t* accV;       // 3 per cell: fixed-point sums, then new face velocity bits
   int32_t* accW;       // 3 per cell: fixed-point weights, then old face velocity bits

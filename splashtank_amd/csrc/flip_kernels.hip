@@ -56,29 +56,47 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
   float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
   bool live = valid && !(q.y < ST_DEADY);
   uint32_t key = P.ntiles;
+  // tiles [cc-2, cc+2] >> 3 relative to the particle's own tile, as bits
+  // (dx+1) + 3 (dy+1) + 9 (dz+1); the span is at most two tiles per axis
+  uint32_t nmask = 0;
   if (live) {
     int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
     key = d_tileOf(P, cc.x, cc.y, cc.z);
-    int32_t lx = max(cc.x - 2, 0) >> 3, ly = max(cc.y - 2, 0) >> 3, lz = max(cc.z - 2, 0) >> 3;
-    int32_t hx = min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3, hy = min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3,
-            hz = min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3;
-    for (int32_t tk = lz; tk <= hz; tk++)
-      for (int32_t tj = ly; tj <= hy; tj++)
-        for (int32_t ti = lx; ti <= hx; ti++) {
-          uint32_t t = uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk));
-          if (G.need[t] == 0u) G.need[t] = 1u;
-        }
+    int32_t ox = cc.x >> 3, oy = cc.y >> 3, oz = cc.z >> 3;
+    int32_t lx = (max(cc.x - 2, 0) >> 3) - ox, ly = (max(cc.y - 2, 0) >> 3) - oy, lz = (max(cc.z - 2, 0) >> 3) - oz;
+    int32_t hx = (min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3) - ox, hy = (min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3) - oy,
+            hz = (min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3) - oz;
+    for (int32_t dz = lz; dz <= hz; dz++)
+      for (int32_t dy = ly; dy <= hy; dy++)
+        for (int32_t dx = lx; dx <= hx; dx++) nmask |= 1u << uint32_t((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
   }
+  // runs of equal bins in the wave: one bin count atomic per run, and the run's
+  // leader marks the union of its particles' tiles
   uint64_t rem = __ballot(valid);
   uint32_t rank = 0;
   while (rem) {
     int leader = __ffsll((unsigned long long)rem) - 1;
     uint32_t k = (uint32_t)__shfl((int)key, leader);
     uint64_t m = __ballot(valid && key == k) & rem;
+    bool inrun = (m >> lane_id()) & 1ull;
+    uint32_t um = inrun ? nmask : 0u;
+    for (int off = 32; off > 0; off >>= 1) um |= (uint32_t)__shfl_xor((int)um, off);
     uint32_t base = 0;
-    if (int(lane_id()) == leader) base = atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+    if (int(lane_id()) == leader) {
+      base = atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+      if (k < P.ntiles) {
+        int32_t ti = int32_t(k % P.tx), tj = int32_t((k / P.tx) % P.ty), tk = int32_t(k / (P.tx * P.ty));
+        while (um) {
+          uint32_t bit = uint32_t(__ffs(um) - 1);
+          um &= um - 1u;
+          int32_t x = ti + int32_t(bit % 3u) - 1, y = tj + int32_t((bit / 3u) % 3u) - 1, z = tk + int32_t(bit / 9u) - 1;
+          uint32_t t = uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z));
+          if (G.need[t] == 0u) G.need[t] = 1u;
+        }
+      }
+    }
     base = (uint32_t)__shfl((int)base, leader);
-    if ((m >> lane_id()) & 1ull) rank = base + (uint32_t)__popcll(m & lanemask_lt());
+    if (inrun) rank = base + (uint32_t)__popcll(m & lanemask_lt());
     rem &= ~m;
   }
   if (valid) G.keyRank[i] = make_uint2(key, rank);
@@ -421,6 +439,9 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) sC[q] = 0;
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
+    // a tile whose window lies inside [0, n-1] on every axis never clamps a face
+    const bool inner = org.x >= 0 && org.y >= 0 && org.z >= 0 && org.x + WIN - 1 <= int32_t(P.nx) - 1 &&
+                       org.y + WIN - 1 <= int32_t(P.ny) - 1 && org.z + WIN - 1 <= int32_t(P.nz) - 1;
     for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
       float4 p4 = G.posB[k];
       float4 v4 = G.velB[k];
@@ -437,13 +458,16 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
         float3 fr = f3sub(q, f0);
         int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
         float vc = f3get(v, axis);
+        int32_t bx = i0.x - org.x, by = i0.y - org.y, bz = i0.z - org.z;
+        bool fast = inner && uint32_t(bx) <= uint32_t(WIN - 2) && uint32_t(by) <= uint32_t(WIN - 2) && uint32_t(bz) <= uint32_t(WIN - 2);
+        int32_t base = bx + WIN * (by + WIN * bz);
         for (uint32_t c = 0; c < 8; c++) {
           int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
           float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
           if (w <= 0.0f) continue;
           int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
           int32_t aw = st_f2i_sat(rintf(w * P.scale));
-          int32_t wi = wface(P, axis, i0.x + dx, i0.y + dy, i0.z + dz, org);
+          int32_t wi = fast ? base + dx + WIN * dy + WIN * WIN * dz : wface(P, axis, i0.x + dx, i0.y + dy, i0.z + dz, org);
           if (wi >= 0) {
             atomicAdd(&sV[axis][wi], av);
             atomicAdd(&sW[axis][wi], aw);
@@ -473,8 +497,41 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
     }
     __syncthreads();
   }
+  // fastest particle: one partial per workgroup (k_reduce_stats folds them into
+  // ctl[6]); same-address atomics from every wave would serialise chip-wide
+  __shared__ uint32_t wmax[WG / 64];
   vmaxBits = wave_max_u32(vmaxBits);
-  if (lane_id() == 0 && vmaxBits) atomicMax(&G.ctl[6], vmaxBits);
+  if (lane_id() == 0) wmax[threadIdx.x >> 6] = vmaxBits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (int q = 0; q < WG / 64; q++) m = max(m, wmax[q]);
+    G.part[blockIdx.x] = m;
+  }
+}
+
+// statistics partials -> control words: which 0: ctl[6] = max(ctl[6], vmax
+// partials); which 1: ctl[0], ctl[1] = sums of the per-colour fluid partials
+__global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint32_t which) {
+  __shared__ uint32_t ws[2][1024 / 64];
+  uint32_t a = 0, b = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+    if (which == 0) a = max(a, G.part[i]);
+    else { a += G.part[2 * i]; b += G.part[2 * i + 1]; }
+  }
+  if (which == 0) a = wave_max_u32(a);
+  else { a = wave_sum_u32(a); b = wave_sum_u32(b); }
+  if (lane_id() == 0) { ws[0][threadIdx.x >> 6] = a; ws[1][threadIdx.x >> 6] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t x = 0, y = 0;
+    for (int q = 0; q < 1024 / 64; q++) {
+      if (which == 0) x = max(x, ws[0][q]);
+      else { x += ws[0][q]; y += ws[1][q]; }
+    }
+    if (which == 0) G.ctl[6] = max(G.ctl[6], x);
+    else { G.ctl[0] = x; G.ctl[1] = y; }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -533,11 +590,16 @@ __global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP,
       if (((a.x + a.y + a.z) & 1) == 0) nred++; else nblack++;
     }
   }
+  __shared__ uint32_t ws[2][WG / 64];
   nred = wave_sum_u32(nred);
   nblack = wave_sum_u32(nblack);
-  if (lane_id() == 0) {
-    if (nred) atomicAdd(&G.ctl[0], nred);
-    if (nblack) atomicAdd(&G.ctl[1], nblack);
+  if (lane_id() == 0) { ws[0][threadIdx.x >> 6] = nred; ws[1][threadIdx.x >> 6] = nblack; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = 0, b = 0;
+    for (int q = 0; q < WG / 64; q++) { r += ws[0][q]; b += ws[1][q]; }
+    G.part[2 * blockIdx.x] = r;
+    G.part[2 * blockIdx.x + 1] = b;
   }
 }
 
@@ -618,39 +680,102 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
   }
 }
 
-// relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164): one workgroup per
-// tile in use and one thread per cell of the colour (a tile holds 256 of each,
-// tile origins being even).  Cells of one colour read only the other colour, so
-// this is the reference's iterate exactly; neighbours are summed in its order.
-__global__ void __launch_bounds__(WG) k_sor_tile(const DParams* __restrict__ PP, DGrid G, uint32_t color) {
+// relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
+// sim.js:291-293), one full iteration (red sweep, then black sweep) per launch.
+// Each workgroup stages a 12^3 pressure window around its tile (two cells of
+// halo) and the 10^3 rhs/stencil window in LDS, relaxes the red cells of the
+// 10^3 region, then the black cells of the tile, and writes the tile to the
+// other pressure buffer.  Red cells read only black ones and black only red, so
+// the halo's red values equal those the neighbouring tile computes, and the
+// result is the reference's red-black iterate exactly.  Neighbours are summed in
+// the reference's order (x-, x+, y-, y+, z-, z+).
+#define SW 12
+#define SW3 1728
+
+__device__ __forceinline__ void relax_lds(float* sp, int32_t p12, uint32_t st, float rhs, float omega) {
+  uint32_t nns = st & 7u, mask = (st >> 4) & 63u;
+  if (nns == 0u) {
+    sp[p12] = 0.0f;
+    return;
+  }
+  float sum = 0.0f;
+  if (mask & 1u) sum = sum + sp[p12 - 1];
+  if (mask & 2u) sum = sum + sp[p12 + 1];
+  if (mask & 4u) sum = sum + sp[p12 - SW];
+  if (mask & 8u) sum = sum + sp[p12 + SW];
+  if (mask & 16u) sum = sum + sp[p12 - SW * SW];
+  if (mask & 32u) sum = sum + sp[p12 + SW * SW];
+  float pg = (rhs + sum) / float(nns);
+  float pc = sp[p12];
+  sp[p12] = pc + omega * (pg - pc);
+}
+
+__global__ void __launch_bounds__(WG) k_sor_iter(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
+                                                 float* __restrict__ pout) {
   const DParams& P = *PP;
-  uint32_t sc = G.ctl[4];
+  __shared__ float sp[SW3];
+  __shared__ float sr[WIN3];
+  __shared__ uint32_t ss[WIN3];
+  __shared__ uint32_t wslots[27];
   const float omega = P.omega;
-  uint32_t tid = threadIdx.x;
-  uint32_t y = (tid >> 2) & 7u, z = tid >> 5;
-  uint32_t x = 2u * (tid & 3u) + ((y + z + color) & 1u);
-  uint32_t local = x + 8u * (y + 8u * z);
+  uint32_t sc = G.ctl[4];
   for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
-    uint32_t s = G.slotList[b];
-    uint32_t c = s * 512u + local;
-    uint32_t st = G.sten[c];
-    if (!(st & STEN_FLUID)) continue;
-    uint32_t nns = st & 7u, mask = (st >> 4) & 63u;
-    if (nns == 0u) {
-      G.pres[c] = 0.0f;
-      continue;
+    uint32_t s = G.slotList[b], t = G.slotTile[s];
+    int3 org;
+    load_wslots(P, G, t, wslots, org);
+    __syncthreads();
+    // pressure window: cells org - 1 .. org + 10 (window index 0..11)
+    for (uint32_t q = threadIdx.x; q < SW3; q += WG) {
+      int32_t wx = int32_t(q % SW), wy = int32_t((q / SW) % SW), wz = int32_t(q / (SW * SW));
+      int32_t ox = wx < 2 ? 0 : (wx >= 10 ? 2 : 1), oy = wy < 2 ? 0 : (wy >= 10 ? 2 : 1), oz = wz < 2 ? 0 : (wz >= 10 ? 2 : 1);
+      uint32_t sl = wslots[ox + 3 * (oy + 3 * oz)];
+      float v = 0.0f;
+      if (sl != WSLOT_OUT) {
+        int32_t gx = org.x - 1 + wx, gy = org.y - 1 + wy, gz = org.z - 1 + wz;
+        v = pin[sl * 512u + uint32_t(gx & 7) + 8u * (uint32_t(gy & 7) + 8u * uint32_t(gz & 7))];
+      }
+      sp[q] = v;
     }
-    const uint32_t* nb = G.nbr + s * 6u;
-    float sum = 0.0f;
-    if (mask & 1u) sum = sum + G.pres[x > 0u ? c - 1u : nb[0] * 512u + local + 7u];
-    if (mask & 2u) sum = sum + G.pres[x < 7u ? c + 1u : nb[1] * 512u + local - 7u];
-    if (mask & 4u) sum = sum + G.pres[y > 0u ? c - 8u : nb[2] * 512u + local + 56u];
-    if (mask & 8u) sum = sum + G.pres[y < 7u ? c + 8u : nb[3] * 512u + local - 56u];
-    if (mask & 16u) sum = sum + G.pres[z > 0u ? c - 64u : nb[4] * 512u + local + 448u];
-    if (mask & 32u) sum = sum + G.pres[z < 7u ? c + 64u : nb[5] * 512u + local - 448u];
-    float pg = (G.rhs[c] + sum) / float(nns);
-    float pc = G.pres[c];
-    G.pres[c] = pc + omega * (pg - pc);
+    for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
+      int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
+      uint32_t c = wcell(wslots, wx, wy, wz, org);
+      sr[q] = c == ST_NONE ? 0.0f : G.rhs[c];
+      ss[q] = c == ST_NONE ? 0u : G.sten[c];
+    }
+    __syncthreads();
+    // red cells of the 10^3 region: global parity (8t - 1 + w) even <=> wx+wy+wz odd;
+    // 5 per row of 10, 100 rows
+    for (uint32_t m = threadIdx.x; m < 500u; m += WG) {
+      int32_t row = int32_t(m / 5u), wy = row % WIN, wz = row / WIN;
+      int32_t wx = 2 * int32_t(m % 5u) + ((wy + wz + 1) & 1);
+      int32_t q = wx + WIN * (wy + WIN * wz);
+      uint32_t st = ss[q];
+      if (st & STEN_FLUID) relax_lds(sp, (wx + 1) + SW * ((wy + 1) + SW * (wz + 1)), st, sr[q], omega);
+    }
+    __syncthreads();
+    // black cells of the tile: local x+y+z odd, one per thread
+    {
+      int32_t y = int32_t((threadIdx.x >> 2) & 7u), z = int32_t(threadIdx.x >> 5);
+      int32_t x = 2 * int32_t(threadIdx.x & 3u) + ((y + z + 1) & 1);
+      int32_t q = (x + 1) + WIN * ((y + 1) + WIN * (z + 1));
+      uint32_t st = ss[q];
+      if (st & STEN_FLUID) relax_lds(sp, (x + 2) + SW * ((y + 2) + SW * (z + 2)), st, sr[q], omega);
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < 512u; q += WG) {
+      uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
+      pout[s * 512u + q] = sp[(x + 2) + SW * ((y + 2) + SW * (z + 2))];
+    }
+    __syncthreads();
+  }
+}
+
+// after an odd number of iterations the result sits in the partner buffer
+__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst) {
+  uint32_t tot = G.ctl[4] * 512u;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+    uint32_t c = G.slotList[t >> 9] * 512u + (t & 511u);
+    dst[c] = src[c];
   }
 }
 
@@ -709,9 +834,15 @@ __device__ __forceinline__ float d_vnz(float3 x) {
 
 // Grid access for g2p: from the global arrays (tmap lookups), or from a tile
 // window staged in LDS.  Both return the same words.
+struct GlobalGrid;
+struct WindowGrid;
+__device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGrid& g, float3 p, float3& vn, float3& vo);
+template <class GridT>
+__device__ __forceinline__ void d_sampleBoth(const DParams& P, const GridT& g, float3 p, float3& vn, float3& vo);
 struct GlobalGrid {
   const DParams& P;
   const DGrid& G;
+  __device__ void sample(float3 p, float3& vn, float3& vo) const { d_sampleBoth(P, *this, p, vn, vo); }
   __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
   __device__ float fold(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gv0R(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
   __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const { return G.cnt[d_cellIdx(P, G.tmap, i, j, k)]; }
@@ -723,6 +854,8 @@ struct WindowGrid {
   const float (*sO)[WIN3];
   const int32_t* sC;
   int3 org;
+  bool inner;  // window inside [0, n-1] on every axis: no face index is ever clamped
+  __device__ void sample(float3 p, float3& vn, float3& vo) const { d_sampleBothWin(P, *this, p, vn, vo); }
   __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
     int32_t w = wface(P, axis, i, j, k, org);
     return w >= 0 ? sN[axis][w] : gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k));
@@ -737,6 +870,44 @@ struct WindowGrid {
     return G.cnt[d_cellIdx(P, G.tmap, i, j, k)];
   }
 };
+
+// sampleBoth (wgsl-core.js:138-154) from a staged window: one base index per
+// axis when no clamp can apply, the clamped general path otherwise
+__device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGrid& g, float3 p, float3& vn, float3& vo) {
+  float rn[3], ro[3];
+  for (uint32_t axis = 0; axis < 3; axis++) {
+    float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
+    if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
+    else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
+    else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
+    float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
+    float3 fr = f3sub(q, f0);
+    int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
+    int32_t bx = i0.x - g.org.x, by = i0.y - g.org.y, bz = i0.z - g.org.z;
+    bool fast = g.inner && uint32_t(bx) <= uint32_t(WIN - 2) && uint32_t(by) <= uint32_t(WIN - 2) && uint32_t(bz) <= uint32_t(WIN - 2);
+    int32_t base = bx + WIN * (by + WIN * bz);
+    float sn = 0.0f, so = 0.0f;
+    for (uint32_t c = 0; c < 8; c++) {
+      int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+      float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+      float fn, fo;
+      if (fast) {
+        int32_t wi = base + dx + WIN * dy + WIN * WIN * dz;
+        fn = g.sN[axis][wi];
+        fo = g.sO[axis][wi];
+      } else {
+        fn = g.fnew(axis, i0.x + dx, i0.y + dy, i0.z + dz);
+        fo = g.fold(axis, i0.x + dx, i0.y + dy, i0.z + dz);
+      }
+      sn = sn + w * fn;
+      so = so + w * fo;
+    }
+    rn[axis] = sn;
+    ro[axis] = so;
+  }
+  vn = make_float3(rn[0], rn[1], rn[2]);
+  vo = make_float3(ro[0], ro[1], ro[2]);
+}
 
 // sampleBoth (wgsl-core.js:138-154)
 template <class GridT>
@@ -774,7 +945,7 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   int3 fcell = d_particleCell(P, p);
   float3 vn = make_float3(v0.x - 0.0f, v0.y - P.gravity * P.dt, v0.z - 0.0f);
   float3 vo = v0;
-  if (storage) d_sampleBoth(P, g, p, vn, vo);
+  if (storage) g.sample(p, vn, vo);
   float3 fl = f3add(v0, f3sub(vn, vo));
   float3 v = make_float3(st_mixf(vn.x, fl.x, P.flip), st_mixf(vn.y, fl.y, P.flip), st_mixf(vn.z, fl.z, P.flip));
   int3 bc = make_int3(st_f2i_sat(floorf((p.x - P.x0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.y - P.y0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.z - P.z0) * P.ih - 0.5f)));
@@ -885,7 +1056,9 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       sC[q] = G.cnt[c];
     }
     __syncthreads();
-    WindowGrid wg{P, G, sN, sO, sC, org};
+    const bool inner = org.x >= 0 && org.y >= 0 && org.z >= 0 && org.x + WIN - 1 <= int32_t(P.nx) - 1 &&
+                       org.y + WIN - 1 <= int32_t(P.ny) - 1 && org.z + WIN - 1 <= int32_t(P.nz) - 1;
+    WindowGrid wg{P, G, sN, sO, sC, org, inner};
     for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
       float4 po, vo;
       g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
@@ -968,11 +1141,13 @@ void launch_clear(hipStream_t st, const LaunchCtx& L) {
 void launch_p2g(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
   hipLaunchKernelGGL(k_p2g_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, st, L.G, L.tileGrid, 0u);
 }
 
 void launch_classify(hipStream_t st, const LaunchCtx& L, bool solid) {
   if (solid) hipLaunchKernelGGL(k_solid_build, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   hipLaunchKernelGGL(k_classify, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, st, L.G, L.cellGrid, 1u);
 }
 
 void launch_norm_faces(hipStream_t st, const LaunchCtx& L) {
@@ -985,9 +1160,11 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
 
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
   for (int it = 0; it < iters; it++) {
-    hipLaunchKernelGGL(k_sor_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, 0u);
-    hipLaunchKernelGGL(k_sor_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, 1u);
+    const float* pin = (it & 1) ? L.G.pres2 : L.G.pres;
+    float* pout = (it & 1) ? L.G.pres : L.G.pres2;
+    hipLaunchKernelGGL(k_sor_iter, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
   }
+  if (iters & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
 }
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {

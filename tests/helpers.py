@@ -22,6 +22,9 @@ def f32_report(a, b):
     d[both_nan] = 0.0
     scale = np.maximum(np.abs(b.astype(np.float64)), F32_ATOL / F32_RTOL)
     rel = d / scale
+    rel[both_nan] = 0.0
+    one_nan = np.isnan(a) ^ np.isnan(b)
+    rel[one_nan] = np.inf
     return {"n": int(a.size), "bit_identical": int(same), "max_abs": float(d.max(initial=0.0)),
             "max_rel": float(rel.max(initial=0.0))}
 
