@@ -66,7 +66,8 @@ def build_napi() -> str | None:
     if not os.path.isfile(src):
         return None
     tmp = ADDON + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", node_inc, "-I", INCLUDE, src,
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DNODE_GYP_MODULE_NAME=splashtank_hip",
+          "-I", node_inc, "-I", INCLUDE, src,
           "-L", HERE, "-lsplashtank_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
     os.replace(tmp, ADDON)
     return ADDON

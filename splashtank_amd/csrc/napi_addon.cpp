@@ -1,0 +1,346 @@
+// napi_addon.cpp - Node N-API binding of the C ABI in include/splashtank.h.
+//
+// Thin by design: each export converts JS numbers, objects and typed arrays to
+// the plain pointers and sizes of one st_* call and back.  The JS FlipSim facade
+// (splashtank_amd/flip/sim.js) builds the reference's API (src/flip/sim.js) on top.
+// Errors from the library are thrown as JS Errors carrying st_last_error().
+#include <node_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "splashtank.h"
+
+namespace {
+
+napi_value throwErr(napi_env env, const std::string& msg) {
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+#define NAPI_OK(call)                                                   \
+  do {                                                                  \
+    if ((call) != napi_ok) return throwErr(env, "napi call failed: " #call); \
+  } while (0)
+
+#define ST_OK_OR_THROW(call)                                                              \
+  do {                                                                                    \
+    int rc_ = (call);                                                                     \
+    if (rc_ != ST_OK) return throwErr(env, std::string(#call " failed: ") + st_last_error()); \
+  } while (0)
+
+bool getArgs(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok) return false;
+  for (size_t k = argc; k < want; k++) napi_get_undefined(env, &argv[k]);
+  return true;
+}
+
+double numProp(napi_env env, napi_value obj, const char* name, double dflt) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return dflt;
+  napi_value v;
+  napi_get_named_property(env, obj, name, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b = false;
+    napi_get_value_bool(env, v, &b);
+    return b ? 1.0 : 0.0;
+  }
+  if (t != napi_number) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+void* handleOf(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
+  return p;
+}
+
+// typed array data pointer and element count
+bool typed(napi_env env, napi_value v, void** data, size_t* len, napi_typedarray_type want) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &t, len, data, &ab, &off) != napi_ok) return false;
+  return t == want;
+}
+
+napi_value mkNum(napi_env env, double d) {
+  napi_value v;
+  napi_create_double(env, d, &v);
+  return v;
+}
+
+// create({d, ratio, size:[W,H,D], open, maxN, slots, device}) -> handle
+napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!getArgs(env, info, 1, argv)) return throwErr(env, "create: bad arguments");
+  st_opts o;
+  std::memset(&o, 0, sizeof o);
+  o.d = numProp(env, argv[0], "d", 0.0);
+  o.ratio = numProp(env, argv[0], "ratio", 2.0);
+  o.open = uint32_t(numProp(env, argv[0], "open", 0));
+  o.max_n = uint32_t(numProp(env, argv[0], "maxN", 0));
+  o.slots = uint32_t(numProp(env, argv[0], "slots", 0));
+  o.device = int32_t(numProp(env, argv[0], "device", 0));
+  bool hasSize = false;
+  napi_has_named_property(env, argv[0], "size", &hasSize);
+  if (hasSize) {
+    napi_value sz;
+    napi_get_named_property(env, argv[0], "size", &sz);
+    for (uint32_t k = 0; k < 3; k++) {
+      napi_value e;
+      if (napi_get_element(env, sz, k, &e) == napi_ok) napi_get_value_double(env, e, &o.size[k]);
+    }
+  }
+  void* sim = nullptr;
+  ST_OK_OR_THROW(st_create(&o, &sim));
+  napi_value h;
+  NAPI_OK(napi_create_external(env, sim, nullptr, nullptr, &h));
+  return h;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  void* sim = handleOf(env, argv[0]);
+  if (sim) ST_OK_OR_THROW(st_destroy(sim));
+  return nullptr;
+}
+
+napi_value GetDims(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  st_dims d;
+  ST_OK_OR_THROW(st_get_dims(handleOf(env, argv[0]), &d));
+  napi_value o;
+  napi_create_object(env, &o);
+  const struct { const char* n; double v; } f[] = {
+      {"d", d.d}, {"h", d.h}, {"ratio", d.ratio}, {"rest", d.rest}, {"x0", d.x0}, {"y0", d.y0}, {"z0", d.z0},
+      {"x1", d.x1}, {"y1", d.y1}, {"z1", d.z1}, {"nx", double(d.nx)}, {"ny", double(d.ny)}, {"nz", double(d.nz)},
+      {"tx", double(d.tx)}, {"ty", double(d.ty)}, {"tz", double(d.tz)}, {"ntiles", double(d.ntiles)},
+      {"scap", double(d.scap)}, {"scapMax", double(d.scap_max)}, {"maxN", double(d.max_n)}, {"ncell", double(d.ncell)}};
+  for (const auto& e : f) napi_set_named_property(env, o, e.n, mkNum(env, e.v));
+  return o;
+}
+
+// upload(h, first, Float32Array pos4, Float32Array vel4, count)
+napi_value Upload(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  getArgs(env, info, 5, argv);
+  uint32_t first = 0, count = 0;
+  napi_get_value_uint32(env, argv[1], &first);
+  void *p = nullptr, *v = nullptr;
+  size_t np = 0, nv = 0;
+  if (!typed(env, argv[2], &p, &np, napi_float32_array) || !typed(env, argv[3], &v, &nv, napi_float32_array))
+    return throwErr(env, "upload: pos and vel must be Float32Arrays");
+  napi_get_value_uint32(env, argv[4], &count);
+  if (size_t(count) * 4 > np || size_t(count) * 4 > nv) return throwErr(env, "upload: arrays shorter than count*4");
+  ST_OK_OR_THROW(st_upload_particles(handleOf(env, argv[0]), first, count, (const float*)p, (const float*)v));
+  return nullptr;
+}
+
+napi_value SetCount(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t n = 0;
+  napi_get_value_uint32(env, argv[1], &n);
+  ST_OK_OR_THROW(st_set_count(handleOf(env, argv[0]), n));
+  return nullptr;
+}
+
+napi_value Clear(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  ST_OK_OR_THROW(st_clear(handleOf(env, argv[0])));
+  return nullptr;
+}
+
+// setColliders(h, Float32Array cf, nc, Float32Array sdf, sig)
+napi_value SetColliders(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  getArgs(env, info, 5, argv);
+  void *cf = nullptr, *sdf = nullptr;
+  size_t ncf = 0, nsdf = 0;
+  if (!typed(env, argv[1], &cf, &ncf, napi_float32_array) || !typed(env, argv[3], &sdf, &nsdf, napi_float32_array))
+    return throwErr(env, "setColliders: cf and sdf must be Float32Arrays");
+  uint32_t nc = 0;
+  napi_get_value_uint32(env, argv[2], &nc);
+  if (size_t(nc) * 32 > ncf) return throwErr(env, "setColliders: cf shorter than nc*32");
+  double sig = 0;
+  napi_get_value_double(env, argv[4], &sig);
+  ST_OK_OR_THROW(st_set_colliders(handleOf(env, argv[0]), nc ? (const float*)cf : nullptr, nc, nsdf ? (const float*)sdf : nullptr, nsdf,
+                                  uint64_t(sig)));
+  return nullptr;
+}
+
+// params object -> st_params (field names as in the reference's step(dt, o))
+void readParams(napi_env env, napi_value o, st_params* p) {
+  st_params_default(p);
+  p->dt = float(numProp(env, o, "dt", p->dt));
+  p->g = float(numProp(env, o, "g", p->g));
+  p->flip = float(numProp(env, o, "flip", p->flip));
+  p->comp = float(numProp(env, o, "comp", p->comp));
+  p->omega = float(numProp(env, o, "omega", 0));
+  p->vmax = float(numProp(env, o, "vmax", 0));
+  p->tear = float(numProp(env, o, "tear", 0));
+  p->iters = int32_t(numProp(env, o, "iters", p->iters));
+  p->drag = float(numProp(env, o, "drag", 0));
+  p->foam_tau = float(numProp(env, o, "foamTau", p->foam_tau));
+  p->foam_sens = float(numProp(env, o, "foamSens", p->foam_sens));
+  p->gust = float(numProp(env, o, "gust", 0));
+  p->no_foam = uint32_t(numProp(env, o, "noFoam", 0));
+  p->no_cull = uint32_t(numProp(env, o, "noCull", 0));
+  p->fast_air = uint32_t(numProp(env, o, "fastAir", 0));
+  p->no_solid_cache = uint32_t(numProp(env, o, "noSolidCache", 0));
+  p->nb_on = uint32_t(numProp(env, o, "nbOn", 0));
+  p->nb_band = float(numProp(env, o, "nbBand", 3));
+  p->only_tiles = uint32_t(numProp(env, o, "onlyTiles", 0));
+  p->nb_init = uint32_t(numProp(env, o, "nbInit", 0));
+  p->nfill = uint32_t(numProp(env, o, "nfill", 0));
+  p->nfv = uint32_t(numProp(env, o, "nfv", 0));
+  bool has = false;
+  napi_has_named_property(env, o, "wind", &has);
+  if (has) {
+    napi_value w;
+    napi_get_named_property(env, o, "wind", &w);
+    for (uint32_t k = 0; k < 3; k++) {
+      napi_value e;
+      double d = 0;
+      if (napi_get_element(env, w, k, &e) == napi_ok && napi_get_value_double(env, e, &d) == napi_ok) p->wind[k] = float(d);
+    }
+  }
+  has = false;
+  napi_has_named_property(env, o, "fills", &has);
+  if (has) {
+    napi_value fv;
+    napi_get_named_property(env, o, "fills", &fv);
+    void* data = nullptr;
+    size_t n = 0;
+    if (typed(env, fv, &data, &n, napi_float32_array)) std::memcpy(p->fills, data, std::min<size_t>(n, 64) * 4);
+  }
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  st_params p;
+  readParams(env, argv[1], &p);
+  ST_OK_OR_THROW(st_step(handleOf(env, argv[0]), &p));
+  return nullptr;
+}
+
+napi_value BeginStep(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  st_params p;
+  readParams(env, argv[1], &p);
+  ST_OK_OR_THROW(st_begin_step(handleOf(env, argv[0]), &p));
+  return nullptr;
+}
+
+napi_value Stage(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  int32_t k = 0;
+  napi_get_value_int32(env, argv[1], &k);
+  ST_OK_OR_THROW(st_stage(handleOf(env, argv[0]), k));
+  return nullptr;
+}
+
+// readParticles(h, Float32Array dst, n) -> n
+napi_value ReadParticles(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  void* dst = nullptr;
+  size_t len = 0;
+  if (!typed(env, argv[1], &dst, &len, napi_float32_array)) return throwErr(env, "readParticles: dst must be a Float32Array");
+  uint32_t n = 0;
+  napi_get_value_uint32(env, argv[2], &n);
+  if (size_t(n) * 4 > len) return throwErr(env, "readParticles: dst shorter than n*4");
+  ST_OK_OR_THROW(st_read_particles(handleOf(env, argv[0]), (float*)dst, n));
+  return mkNum(env, n);
+}
+
+napi_value ReadCtl(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  napi_value ab, arr;
+  void* data = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, 64, &data, &ab));
+  ST_OK_OR_THROW(st_read_ctl(handleOf(env, argv[0]), (uint32_t*)data));
+  NAPI_OK(napi_create_typedarray(env, napi_uint32_array, 16, ab, 0, &arr));
+  return arr;
+}
+
+napi_value Grow(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t n = 0;
+  napi_get_value_uint32(env, argv[1], &n);
+  ST_OK_OR_THROW(st_grow(handleOf(env, argv[0]), n));
+  return nullptr;
+}
+
+// dump(h, what) -> ArrayBuffer (canonical layout, see splashtank.h)
+napi_value Dump(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  int32_t what = 0;
+  napi_get_value_int32(env, argv[1], &what);
+  void* sim = handleOf(env, argv[0]);
+  size_t bytes = 0;
+  ST_OK_OR_THROW(st_dump_size(sim, what, &bytes));
+  napi_value ab;
+  void* data = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, bytes ? bytes : 4, &data, &ab));
+  if (bytes) ST_OK_OR_THROW(st_dump(sim, what, data, bytes));
+  return ab;
+}
+
+napi_value Sync(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  ST_OK_OR_THROW(st_sync(handleOf(env, argv[0])));
+  return nullptr;
+}
+
+napi_value MemBytes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  uint64_t b = 0;
+  ST_OK_OR_THROW(st_mem_bytes(handleOf(env, argv[0]), &b));
+  return mkNum(env, double(b));
+}
+
+napi_value Version(napi_env env, napi_callback_info) {
+  napi_value v;
+  napi_create_string_utf8(env, st_version(), NAPI_AUTO_LENGTH, &v);
+  return v;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  const struct { const char* n; napi_callback f; } fns[] = {
+      {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
+      {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
+      {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
+      {"memBytes", MemBytes}, {"version", Version}};
+  for (const auto& e : fns) {
+    napi_value f;
+    napi_create_function(env, e.n, NAPI_AUTO_LENGTH, e.f, nullptr, &f);
+    napi_set_named_property(env, exports, e.n, f);
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,272 @@
+'use strict';
+// FlipSim on one MI355X: the JS API of the reference's WebGPU FlipSim
+// (src/flip/sim.js:46-547) over the HIP library, through the N-API addon
+// (splashtank_hip.node -> libsplashtank_hip.so, the C ABI of include/splashtank.h).
+//
+// Same constructor options, fields, methods and stage names as the reference, so
+// its callers (SceneRunner.step, the bake loop, the benchmark) drive this class
+// unchanged.  What differs, and why:
+//  - `device` is a HIP device handle ({ ordinal }) from initHip(), not a GPUDevice;
+//  - there is no pipeline compile step: `ready` is true and `whenReady` resolves at once;
+//  - step() enqueues on the sim's HIP stream; readbacks wait for the stream, then
+//    resolve their Promise (the reference maps buffers asynchronously);
+//  - `stats` is read when asked, so it is the last step's (the reference's lags one);
+//  - the block pool defaults to every tile of the box (no growth needed on a
+//    288 GB GPU); opt.slots still sets a smaller pool, with the reference's
+//    overflow counting and growth policy.
+// Whitewater, the live-view packing and deep-water display points are outside
+// the substep this library provides; their members are stubs.
+
+const path = require('path');
+
+let addon = null;
+function loadAddon() {
+  if (!addon) addon = require(path.join(__dirname, 'splashtank_hip.node'));
+  return addon;
+}
+
+const STAGE_IDS = { sort: 0, blocks: 1, clear: 2, p2g: 3, 'narrow band': 4, classify: 5, normFaces: 6, divergence: 7,
+  pressure: 8, project: 9, g2p: 10, spray: 11, reseed: 12 };
+const DUMP = { tiles: 0, cnt: 1, ctype: 2, pres: 3, rhs: 4, accv: 5, accw: 6, facev: 7, faceo: 8, velg: 9, pos: 10, vel: 11, tmap: 12 };
+
+// initGpu's counterpart (gpu.js:492-509): the HIP device the sims run on
+function initHip(ordinal) {
+  const a = loadAddon();
+  return { ordinal: ordinal | 0, version: a.version(), limits: { maxStorageBufferBindingSize: Infinity } };
+}
+
+// packColliders (gpu.js:42-70): 32 floats per collider into sim.cf, distance
+// grids deduplicated by sdf.id and uploaded when the set of ids changes
+function packColliders(sim, cols) {
+  const nc = Math.min(cols.length, 32), cf = sim.cf;
+  let sig = '', off = 0; const offs = new Map(), grids = [];
+  for (let k = 0; k < nc; k++) {
+    const sd = cols[k].sdf;
+    if (!offs.has(sd.id)) { offs.set(sd.id, off); off += sd.data.length; sig += sd.id + ','; grids.push(sd.data); }
+  }
+  if (sig !== sim.sdfSig) {
+    const all = new Float32Array(Math.max(off, 1)); let o = 0;
+    for (const g of grids) { all.set(g, o); o += g.length; }
+    sim.sdfAll = all.subarray(0, off); sim.sdfSig = sig; sim.sdfSigNum = (sim.sdfSigNum || 0) + 1;
+  }
+  for (let k = 0; k < nc; k++) {
+    const c = cols[k], sd = c.sdf, R = c.R, j = k * 32;
+    cf[j] = offs.get(sd.id); cf[j + 1] = sd.dims[0]; cf[j + 2] = sd.dims[1]; cf[j + 3] = sd.dims[2];
+    cf[j + 4] = sd.origin[0]; cf[j + 5] = sd.origin[1]; cf[j + 6] = sd.origin[2]; cf[j + 7] = sd.cell;
+    cf[j + 8] = R[0]; cf[j + 9] = R[3]; cf[j + 10] = R[6]; cf[j + 11] = 0;
+    cf[j + 12] = R[1]; cf[j + 13] = R[4]; cf[j + 14] = R[7]; cf[j + 15] = 0;
+    cf[j + 16] = R[2]; cf[j + 17] = R[5]; cf[j + 18] = R[8]; cf[j + 19] = 0;
+    cf[j + 20] = c.x; cf[j + 21] = c.y; cf[j + 22] = c.z; cf[j + 23] = 0;
+    cf[j + 24] = c.vx; cf[j + 25] = c.vy; cf[j + 26] = c.vz; cf[j + 27] = 0;
+    cf[j + 28] = c.wx; cf[j + 29] = c.wy; cf[j + 30] = c.wz; cf[j + 31] = 0;
+  }
+  loadAddon().setColliders(sim.handle, cf, nc, sim.sdfAll || new Float32Array(0), sim.sdfSigNum || 0);
+  return nc;
+}
+
+class FlipSim {
+  constructor(device, opt) {
+    const a = loadAddon();
+    const ratio = Math.max(1.4, Math.min(4, opt.ratio || 2));
+    const sz = opt.size || [3, 2.6, 3];
+    this.dev = device || { ordinal: 0 };
+    this.handle = a.create({ d: opt.d, ratio, size: sz, open: opt.open | 0, maxN: opt.maxN, slots: opt.slots | 0,
+      device: (device && device.ordinal) | 0 });
+    const D = a.getDims(this.handle);
+    this.d = opt.d; this.ratio = ratio; this.h = ratio * opt.d; this.rest = ratio * ratio * ratio;
+    this.opt = { noFoam: false, noCull: false, noSolidCache: false, fastAir: false, sort: false, sortEvery: 16, iters: 0 };
+    this.nx = D.nx; this.ny = D.ny; this.nz = D.nz;
+    const W = this.nx * this.h, H = this.ny * this.h, Dd = this.nz * this.h;
+    this.b = { x0: -W / 2, x1: W / 2, y0: 0, y1: H, z0: -Dd / 2, z1: Dd / 2 };
+    this.size = [W, H, Dd];
+    this.open = opt.open | 0;
+    this.MAXN = opt.maxN; this.cap = opt.maxN; this.N = 0; this.ring = 0; this.recycle = true;
+    this.lastMs = 0; this.slowMs = 0; this.inflight = 0; this.out = opt.out || null; this.outN = 0;
+    this.ncell = D.ncell; this.tx = D.tx; this.ty = D.ty; this.tz = D.tz; this.ntiles = D.ntiles;
+    this.scap = D.scap; this.scapMax = D.scapMax;
+    this.pS = new Float32Array(this.MAXN * 4); this.vS = new Float32Array(this.MAXN * 4);
+    this.runStart = 0; this.runLen = 0; this.devN = 0;
+    this.cf = new Float32Array(32 * 32); this.sdfSig = null;
+    this.tm = 0; this.seed = 0; this.nbOn = false; this.sortedEver = true;
+    this.nbInit = 0; this.nbFills = []; this.nbFillVols = []; this.nbCfg = null; this.oceanCfg = null;
+    this.ready = true; this.whenReady = Promise.resolve(this);
+    this.statsCache = null; this.peak = 0; this.grew = 0; this.ovfTotal = 0;
+    this.rand = Math.random;   // jitter source of fillBox / fillSphere
+  }
+  // ---- particles (GpuSim.setCap/clear/add/flushRun, gpu.js:388-408) ----
+  setCap(c) { this.cap = Math.min(c, this.MAXN); if (this.N > this.cap) this.N = this.cap; if (this.ring >= this.cap) this.ring = 0; }
+  add(x, y, z, vx, vy, vz) {
+    let i;
+    if (this.N < this.cap) i = this.N++;
+    else if (this.recycle) { i = this.ring++; if (this.ring >= this.cap) this.ring = 0; }
+    else return false;
+    const o = i * 4;
+    this.pS[o] = x; this.pS[o + 1] = y; this.pS[o + 2] = z; this.pS[o + 3] = Math.sqrt(vx * vx + vy * vy + vz * vz);
+    this.vS[o] = vx; this.vS[o + 1] = vy; this.vS[o + 2] = vz; this.vS[o + 3] = 0;
+    if (this.runLen > 0 && i === this.runStart + this.runLen) this.runLen++;
+    else { this.flushRun(); this.runStart = i; this.runLen = 1; }
+    return true;
+  }
+  flushRun() {
+    if (!this.runLen) return;
+    const a = loadAddon(), s = this.runStart * 4, e = (this.runStart + this.runLen) * 4;
+    a.upload(this.handle, this.runStart, this.pS.subarray(s, e), this.vS.subarray(s, e), this.runLen);
+    this.runLen = 0;
+    if (this.devN !== this.N) { a.setCount(this.handle, this.N); this.devN = this.N; }
+  }
+  // fillBox / fillSphere (shared.js:285-297)
+  fillBox(x0, x1, y0, y1, z0, z1) {
+    const d = this.d, j = 0.01 * d / 0.12, r = this.rand;
+    for (let y = y0 + d / 2; y < y1; y += d)
+      for (let x = x0 + d / 2; x < x1; x += d)
+        for (let z = z0 + d / 2; z < z1; z += d)
+          this.add(x + (r() - 0.5) * j, y + (r() - 0.5) * j, z + (r() - 0.5) * j, 0, 0, 0);
+  }
+  fillSphere(cx, cy, cz, R, vx, vy, vz) {
+    const d = this.d, j = 0.01 * d / 0.12, r = this.rand;
+    for (let y = -R; y <= R; y += d) for (let x = -R; x <= R; x += d) for (let z = -R; z <= R; z += d)
+      if (x * x + y * y + z * z <= R * R)
+        this.add(cx + x + (r() - 0.5) * j, cy + y + (r() - 0.5) * j, cz + z + (r() - 0.5) * j, vx, vy, vz);
+  }
+  clear() {
+    this.flushRun(); this.N = 0; this.ring = 0; this.devN = 0;
+    loadAddon().clear(this.handle);
+    this.tm = 0; this.nbInit = 1; this.nbFills = []; this.nbFillVols = []; this.outFrame = null;
+  }
+  // ---- stepping (sim.js:179-243) ----
+  omega() { return Math.min(1.9, 2 / (1 + Math.sin(Math.PI / Math.max(this.nx, this.ny, this.nz)))); }
+  params(dt, o) {
+    const nbc = o.nb !== undefined ? o.nb : this.nbCfg;
+    const nbOn = !!(nbc && nbc.on);
+    const init = nbOn && this.nbInit ? this.nbInit : 0;
+    const fl = init ? this.nbFills.slice(0, 8) : [];
+    const fills = new Float32Array(64);
+    fl.forEach((f, k) => { const q = k * 8; if (f.t === 'box') fills.set([1, f.x0, f.y0, f.z0, f.x1, f.y1, f.z1, 0], q); else fills.set([2, f.x, f.y, f.z, f.r, 0, 0, 0], q); });
+    return { dt, g: o.g, flip: o.flip === undefined ? 0.95 : o.flip, comp: o.comp === undefined ? 6 : o.comp,
+      omega: o.omega || this.omega(), vmax: o.vmax || 14, tear: o.tear || 0,
+      iters: Math.max(1, (this.opt.iters || o.iters) | 0), wind: o.wind || [0, 0, 0], drag: o.drag || 0,
+      foamTau: o.foamTau || 1.2, foamSens: o.foamSens || 1, gust: o.gust || 0,
+      noFoam: this.opt.noFoam, noCull: this.opt.noCull, fastAir: this.opt.fastAir, noSolidCache: this.opt.noSolidCache,
+      nbOn, nbBand: Math.max(2, (nbc && nbc.band) || 3), onlyTiles: this.only === 'tiles', nbInit: init, nfill: fl.length,
+      nfv: init ? this.nbFillVols.length : 0, fills };
+  }
+  prepare(dt, o) {
+    this.flushRun();
+    if (this.N === 0) return null;
+    if (!this.only) this.maybeGrow();
+    const prm = this.params(dt, o);
+    const init = prm.nbInit;
+    const cols = init && this.nbFillVols.length ? (o.cols || []).concat(this.nbFillVols) : (o.cols || []);
+    const nc = Math.min((o.cols || []).length, packColliders(this, cols));
+    this.nbOn = prm.nbOn;
+    if (init && !this.only) { this.nbInit = 0; this.nbFills = []; this.nbFillVols = []; }
+    if (!this.only) { this.tm += dt; this.seed = (this.seed + 1) >>> 0; }
+    return { prm, nc };
+  }
+  step(dt, o) {
+    const t = this.prepare(dt, o);
+    if (!t) return undefined;
+    const t0 = Date.now();
+    loadAddon().step(this.handle, t.prm);
+    this.statsCache = null; this.stepped = true;
+    if (!this.only) this.track(t0);
+    return t.nc;
+  }
+  track(t0) { const ms = Date.now() - t0; this.lastMs = ms; this.slowMs = Math.max(this.slowMs, ms); }
+  // the named stages of one step (sim.js:245-310): fn() enqueues that stage
+  stages() {
+    const a = loadAddon(), h = this.handle;
+    const names = ['blocks', 'clear', 'p2g'].concat(this.nbOn ? ['narrow band'] : [],
+      ['classify', 'normFaces', 'divergence', 'pressure', 'project', 'g2p'], this.nbOn ? ['reseed'] : []);
+    if (this.only === 'tiles') return [{ name: 'blocks', fn: () => a.stage(h, STAGE_IDS.blocks) }];
+    return names.map((name) => ({ name, fn: () => a.stage(h, STAGE_IDS[name]) }));
+  }
+  // one real step with every stage timed on its own (sim.js:312-331)
+  async stepTimed(dt, o) {
+    const t = this.prepare(dt, o);
+    if (!t) return null;
+    const a = loadAddon();
+    a.beginStep(this.handle, t.prm);
+    const out = {};
+    for (const st of this.stages()) {
+      const t0 = process.hrtime.bigint();
+      st.fn(); a.sync(this.handle);
+      out[st.name] = Number(process.hrtime.bigint() - t0) / 1e6;
+    }
+    this.statsCache = null;
+    return out;
+  }
+  // ---- pool (sim.js:137-178) ----
+  get stats() {
+    if (this.statsCache) return this.statsCache;
+    const a = loadAddon().readCtl(this.handle);
+    const vmax = new Float32Array(a.buffer)[6];
+    this.statsCache = { sc: a[4], ovf: a[5], fluid: a[0] + a[1], vmax, seen: false };
+    this.peak = Math.max(this.peak, a[4]);
+    return this.statsCache;
+  }
+  grow(n) {
+    n = Math.min(this.scapMax, Math.max(n, this.scap + 1)); if (n <= this.scap) return false;
+    loadAddon().grow(this.handle, n);
+    this.scap = loadAddon().getDims(this.handle).scap;
+    this.grew = (this.grew || 0) + 1;
+    return true;
+  }
+  maybeGrow() {
+    // a pool that holds every tile of the box never overflows: no stats read, no sync
+    if (this.scap >= this.scapMax || !this.stepped) return;
+    const s = this.stats; if (s.seen) return;
+    s.seen = true; this.ovfTotal += s.ovf;
+    const want = s.sc + s.ovf;
+    if (s.ovf > 0 || want > 0.8 * this.scap) this.grow(Math.max(Math.ceil(this.scap * 1.5), Math.ceil(want * 1.4) + 8));
+  }
+  async prime(o) {
+    for (let k = 0; k < 8; k++) {
+      this.only = 'tiles'; this.step(1 / 60, o); this.only = null;
+      const s = this.stats; s.seen = true;
+      if (s.ovf === 0 && s.sc <= 0.8 * this.scap) return;
+      this.grow(Math.max(Math.ceil(this.scap * 1.5), Math.ceil((s.sc + s.ovf) * 1.4) + 8));
+    }
+  }
+  // ---- readback (sim.js:439-495) ----
+  readbackInto(dst) {
+    this.flushRun();
+    const n = this.N;
+    if (n > 0) loadAddon().readParticles(this.handle, dst, n);
+    return Promise.resolve(n);
+  }
+  async activeCells() { const s = this.stats; this.blocksUsed = s.sc; return s.fluid; }
+  memBytes() { return loadAddon().memBytes(this.handle); }
+  dump(what) { return loadAddon().dump(this.handle, DUMP[what]); }
+  // one column / row of cells for debugging (sim.js:457-482)
+  cellsAt(list) {
+    const tmap = new Uint32Array(this.dump('tmap')), tiles = new Uint32Array(this.dump('tiles'));
+    const rank = new Map(); tiles.forEach((t, q) => rank.set(t, q));
+    const ty = new Uint32Array(this.dump('ctype')), cn = new Int32Array(this.dump('cnt')), pr = new Float32Array(this.dump('pres'));
+    const vg = this.nbOn ? new Float32Array(this.dump('velg')) : null;
+    return list.map(([i, j, k]) => {
+      const t = (i >> 3) + this.tx * ((j >> 3) + this.ty * (k >> 3));
+      if (tmap[t] === 0xffffffff) return null;
+      const c = rank.get(t) * 512 + (i & 7) + 8 * ((j & 7) + 8 * (k & 7));
+      return { ty: ty[c] & 255, col: (ty[c] >> 8) & 255, phi: +(8 - (ty[c] >>> 16) / 4096).toFixed(2), n: cn[c],
+        vy: vg ? +vg[c * 3 + 1].toFixed(3) : 0, vx: vg ? vg[c * 3] : 0, p: +pr[c].toFixed(4) };
+    });
+  }
+  async dbgColumn(i, k) {
+    const js = []; for (let j = 0; j < this.ny; j++) js.push([i, j, k]);
+    return this.cellsAt(js).map((c, j) => (c ? Object.assign({ j }, c) : { j, none: true }));
+  }
+  async dbgRow(j, k) {
+    const is = []; for (let i = 0; i < this.nx; i++) is.push([i, j, k]);
+    return this.cellsAt(is).map((c) => (c ? c.ty + '/' + c.phi.toFixed(1) + '/' + c.n + '/' + c.vx.toFixed(3) + '/' + c.vy.toFixed(3) : '-'));
+  }
+  destroy() { if (this.handle) { loadAddon().destroy(this.handle); this.handle = null; } this.dead = true; }
+  // ---- outside the substep: whitewater and the live view ----
+  setSprayCap() {}
+  async readSpray() { return null; }
+  setDeepLook() {}
+  async readDeep() { return null; }
+  pump() {}
+}
+
+module.exports = { FlipSim, initHip, packColliders, STAGE_IDS };

@@ -1,0 +1,62 @@
+"""The JS FlipSim facade (Node, N-API addon) over the same C ABI.
+
+CPU: the addon loads in Node and fails loudly when no GPU is present.
+GPU: Node drives FlipSim.add / step / stepTimed / readbackInto on a scene and
+the Python C-ABI driver runs the same particles and settings; the two results
+must be bit-identical (same library, same kernels).
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from splashtank_amd import capi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "splashtank_amd", "flip", "splashtank_hip.node")
+NODE = shutil.which("node")
+
+needs_node = pytest.mark.skipif(NODE is None or not os.path.isfile(ADDON), reason="node or the N-API addon is missing")
+
+
+@needs_node
+def test_addon_loads_and_reports_missing_gpu():
+    code = ("const m = require(process.argv[1]); const d = m.initHip(0); console.log(d.version);"
+            "try { new m.FlipSim(d, {d: 1/32, maxN: 8, size: [1,1,1]}); console.log('GPU'); }"
+            "catch (e) { console.log('ERR ' + e.message); }")
+    r = subprocess.run([NODE, "-e", code, os.path.join(ROOT, "splashtank_amd", "flip")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0].startswith("splashtank-mi355x")
+    if lines[1] != "GPU":
+        assert lines[1].startswith("ERR ") and "device" in lines[1]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_facade_matches_c_abi(tmp_path):
+    sc = scenes.dam_break(d=1.0 / 48, frac=(0.4, 0.6, 1.0))
+    steps = 5
+    parts = np.concatenate([sc.pos, sc.vel], axis=1).astype(np.float32)
+    (tmp_path / "p.bin").write_bytes(parts.tobytes())
+    cfg = dict(particles=str(tmp_path / "p.bin"), out=str(tmp_path / "out.bin"), d=sc.d, size=list(sc.size), open=sc.open_,
+               ratio=sc.ratio, steps=steps, params=dict(sc.params))
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "facade_run.js"), str(tmp_path / "cfg.json")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["n"] == sc.n and info["stages"][0] == "blocks" and "pressure" in info["stages"]
+    js = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float32).reshape(-1, 4)
+    # the same run through the C ABI: the facade's stepTimed is one more full step
+    g = capi.FlipSimC(capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n))
+    g.upload(sc.pos, sc.vel)
+    prm = capi.make_params(**sc.params)
+    for _ in range(steps + 1):
+        g.step(prm)
+    ref = g.read_particles()
+    np.testing.assert_array_equal(js.view(np.uint32), ref.view(np.uint32))
+    assert info["stats"]["sc"] == int(g.ctl()[4])
