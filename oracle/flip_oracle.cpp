@@ -65,7 +65,11 @@ template <class R> V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z
 template <class R> R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class R> R len(V3<R> a) { return M<R>::sq(dot(a, a)); }
 template <class R> R mixs(R a, R b, R t) { return a * (R(1) - t) + b * t; }
-template <class R> R clampr(R x, R lo, R hi) { return std::min(std::max(x, lo), hi); }
+// WGSL leaves max/min of a NaN to the implementation; this restatement and the
+// kernels both take IEEE maxNum/minNum (the non-NaN operand)
+template <class R> R fmx(R a, R b) { return std::fmax(a, b); }
+template <class R> R fmn(R a, R b) { return std::fmin(a, b); }
+template <class R> R clampr(R x, R lo, R hi) { return fmn(fmx(x, lo), hi); }
 template <class R> R smooth(R lo, R hi, R x) {
   R t = clampr((x - lo) / (hi - lo), R(0), R(1));
   return t * t * (R(3) - R(2) * t);
@@ -258,7 +262,7 @@ template <class R> struct Sim {
       gl = v3(mixs(mixs(v100 - v000, v110 - v010, f.y), mixs(v101 - v001, v111 - v011, f.y), f.z),
               mixs(c10 - c00, c11 - c01, f.z), c1 - c0);
       R l = len(gl);
-      gl = gl / std::max(l, R(1e-9));
+      gl = gl / fmx<R>(l, R(1e-9));
     }
     wnOut = r0 * gl.x + r1 * gl.y + r2 * gl.z;
     phiOut = phi;
@@ -377,17 +381,17 @@ template <class R> struct Sim {
       const R* A = P.fills[k * 2];
       const R* B = P.fills[k * 2 + 1];
       if (A[0] < R(1.5)) {
-        V3<R> q = v3(std::max(A[1] - x.x, x.x - B[0]), std::max(A[2] - x.y, x.y - B[1]), std::max(A[3] - x.z, x.z - B[2]));
-        V3<R> qp = v3(std::max(q.x, R(0)), std::max(q.y, R(0)), std::max(q.z, R(0)));
-        dd = std::min(dd, len(qp) + std::min(std::max(q.x, std::max(q.y, q.z)), R(0)));
+        V3<R> q = v3(fmx<R>(A[1] - x.x, x.x - B[0]), fmx<R>(A[2] - x.y, x.y - B[1]), fmx<R>(A[3] - x.z, x.z - B[2]));
+        V3<R> qp = v3(fmx<R>(q.x, R(0)), fmx<R>(q.y, R(0)), fmx<R>(q.z, R(0)));
+        dd = fmn<R>(dd, len(qp) + fmn<R>(fmx<R>(q.x, fmx<R>(q.y, q.z)), R(0)));
       } else {
-        dd = std::min(dd, len(x - v3(A[1], A[2], A[3])) - B[0]);
+        dd = fmn<R>(dd, len(x - v3(A[1], A[2], A[3])) - B[0]);
       }
     }
     for (uint32_t k = P.ncols; k < P.ncols + P.nfv; k++) {
       R phi; V3<R> wn;
       sdfEval(k, x, phi, wn);
-      dd = std::min(dd, phi);
+      dd = fmn<R>(dd, phi);
     }
     return dd * P.ih;
   }
@@ -478,9 +482,9 @@ template <class R> struct Sim {
   void phiInitAll() {  // wgsl-narrowband.js:21-35
     forActive([&](uint32_t c, I3 p) {
       R ph = clampr(fillPhi(cellCenter(p.x, p.y, p.z)), R(-7.9), R(7.9));
-      if (ph < R(0)) ph = std::min(ph, R(-1.5));
+      if (ph < R(0)) ph = fmn<R>(ph, R(-1.5));
       if (!inDom(p)) ph = R(8.0);
-      if (P.nbInit == 2u) ph = std::min(ph, phiOf(c));
+      if (P.nbInit == 2u) ph = fmn<R>(ph, phiOf(c));
       phiPut(c, ph);
       if (P.nbInit == 1u) for (uint32_t ax = 0; ax < 3; ax++) velG[c * 3 + ax] = R(0);
     });
@@ -649,7 +653,7 @@ template <class R> struct Sim {
           ctypeOf(i, j, k - 1) == 0u || ctypeOf(i, j, k + 1) == 0u)
         interior = false;
       R dens = R(cnt[c]) / P.rest - R(1.0);
-      R tgt = P.comp * (interior ? dens : std::max(dens, R(0)));
+      R tgt = P.comp * (interior ? dens : fmx<R>(dens, R(0)));
       if (P.nb != 0u && phiOf(c) < R(-1.0)) tgt = R(0);
       rhs[c] = (tgt - div) * P.h * P.h / P.dt;
     });
@@ -761,15 +765,15 @@ template <class R> struct Sim {
       R shield = smooth(R(0.1), R(0.65), rho);
       R exposure = R(1.0) - R(0.97) * shield;
       R spd0 = len(v);
-      R sens = std::max(P.q6, R(0.05));
+      R sens = fmx<R>(P.q6, R(0.05));
       R pacc = len(vn - vo) / P.dt;
       R settle = smooth(R(0.1), R(0.5), P.q8);
       R impact = clampr((pacc / sens - R(15.0)) / R(80.0), R(0.0), R(1.0)) * settle;
       R turb = clampr((len(v0 - vo) / sens - R(0.35)) / R(1.5), R(0.0), R(1.0)) * settle;
       R spray = air * (R(0.25) + R(0.75) * clampr(spd0 / R(4.0), R(0.0), R(1.0)));
-      R tau = std::max(P.q5, R(0.05)) * mixs(R(1.0), R(0.35), smooth(R(0.7), R(1.0), rho)) * mixs(R(1.0), R(2.5), air);
+      R tau = fmx<R>(P.q5, R(0.05)) * mixs(R(1.0), R(0.35), smooth(R(0.7), R(1.0), rho)) * mixs(R(1.0), R(2.5), air);
       R foam = vel[i * 4 + 3] * M<R>::exp(-P.dt / tau);
-      foam = std::max(foam, std::max(spray * R(0.9), std::max(impact, turb * R(0.8))));
+      foam = fmx<R>(foam, fmx<R>(spray * R(0.9), fmx<R>(impact, turb * R(0.8))));
       V3<R> gust = v3(R(0), R(0), R(0));
       if (P.q7 > R(0.0) && P.q4 > R(0.0) && (!fast || shield < R(0.9))) {
         V3<R> gp = p * R(0.55);
@@ -797,7 +801,7 @@ template <class R> struct Sim {
         continue;
       }
       pos[i * 4] = pc.x; pos[i * 4 + 1] = pc.y; pos[i * 4 + 2] = pc.z;
-      pos[i * 4 + 3] = std::min(len(v), R(63.0)) + R(64.0) * M<R>::flo(foam * R(15.0) + R(0.5));
+      pos[i * 4 + 3] = fmn<R>(len(v), R(63.0)) + R(64.0) * M<R>::flo(foam * R(15.0) + R(0.5));
       vel[i * 4] = v.x; vel[i * 4 + 1] = v.y; vel[i * 4 + 2] = v.z; vel[i * 4 + 3] = foam;
     }
   }
@@ -812,7 +816,7 @@ template <class R> struct Sim {
       if (!inDom(n)) continue;
       uint32_t cq = cellIdx(n.x, n.y, n.z);
       if (colBits(cq) != 0u) continue;
-      m = std::max(m, fromRhs ? rhs[cq] : phiOf(cq));
+      m = fmx<R>(m, fromRhs ? rhs[cq] : phiOf(cq));
     }
     return clampr(m - R(1.0), R(-7.9), R(-0.5));
   }
@@ -887,7 +891,7 @@ template <class R> struct Sim {
         uint32_t hh = (e * 9781u + m * 6271u + 17u) ^ (P.seed * 2654435761u);
         V3<R> x = v3(P.x0, P.y0, P.z0) + (cp + v3(hash1(hh + 1u), hash1(hh + 2u), hash1(hh + 3u))) * P.h;
         V3<R> v = velGVec(x);
-        pos[i * 4] = x.x; pos[i * 4 + 1] = x.y; pos[i * 4 + 2] = x.z; pos[i * 4 + 3] = std::min(len(v), R(63.0));
+        pos[i * 4] = x.x; pos[i * 4 + 1] = x.y; pos[i * 4 + 2] = x.z; pos[i * 4 + 3] = fmn<R>(len(v), R(63.0));
         vel[i * 4] = v.x; vel[i * 4 + 1] = v.y; vel[i * 4 + 2] = v.z; vel[i * 4 + 3] = R(0);
       }
     }

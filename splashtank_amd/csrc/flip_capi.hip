@@ -94,6 +94,7 @@ struct Sim {
     L.scap = scap;
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
+    L.sorGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 65535u));
     return L;
   }
 };
@@ -113,7 +114,7 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.slotTile, n))) return rc;
   if ((rc = dalloc(&s->G.fstack, n))) return rc;
   if ((rc = dalloc(&s->G.slotList, n))) return rc;
-  if ((rc = dalloc(&s->G.nbr, size_t(n) * 6))) return rc;
+  if ((rc = dalloc(&s->G.nbr, size_t(n) * 27))) return rc;
   return ST_OK;
 }
 

@@ -90,8 +90,7 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
           uint32_t bit = uint32_t(__ffs(um) - 1);
           um &= um - 1u;
           int32_t x = ti + int32_t(bit % 3u) - 1, y = tj + int32_t((bit / 3u) % 3u) - 1, z = tk + int32_t(bit / 9u) - 1;
-          uint32_t t = uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z));
-          if (G.need[t] == 0u) G.need[t] = 1u;
+          G.need[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))] = 1u;
         }
       }
     }
@@ -318,25 +317,25 @@ __global__ void k_tile_alloc(DGrid G) {
   }
 }
 
-// slots of the six face-neighbour tiles of every tile in use (0: none)
+// Slots of the 27 tiles around every tile in use (itself included), index
+// (dx+1) + 3 (dy+1) + 9 (dz+1): 0 for a tile without storage (cellIdx maps it to
+// the spare slot), WSLOT_OUT outside the tile grid.  Built once per substep after
+// allocation so the window kernels need no tile-map lookups.
+#define WSLOT_OUT 0xfffffffeu
 __global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
   const DParams& P = *PP;
-  uint32_t sc = G.ctl[4];
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < sc; b += gridDim.x * blockDim.x) {
+  uint32_t tot = G.ctl[4] * 27u;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+    uint32_t b = e / 27u, q = e - b * 27u;
     uint32_t s = G.slotList[b], t = G.slotTile[s];
-    int32_t ti = int32_t(t % P.tx), tj = int32_t((t / P.tx) % P.ty), tk = int32_t(t / (P.tx * P.ty));
-    int32_t X = int32_t(P.tx), Y = int32_t(P.ty), Z = int32_t(P.tz);
-    uint32_t nb[6];
-    for (int a = 0; a < 6; a++) {
-      int32_t x = ti + (a == 0 ? -1 : (a == 1 ? 1 : 0)), y = tj + (a == 2 ? -1 : (a == 3 ? 1 : 0)), z = tk + (a == 4 ? -1 : (a == 5 ? 1 : 0));
-      uint32_t v = 0u;
-      if (x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z) {
-        uint32_t q = G.tmap[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))];
-        v = q == ST_NONE ? 0u : q;
-      }
-      nb[a] = v;
+    int32_t x = int32_t(t % P.tx) + int32_t(q % 3u) - 1, y = int32_t((t / P.tx) % P.ty) + int32_t((q / 3u) % 3u) - 1,
+            z = int32_t(t / (P.tx * P.ty)) + int32_t(q / 9u) - 1;
+    uint32_t v = WSLOT_OUT;
+    if (x >= 0 && y >= 0 && z >= 0 && x < int32_t(P.tx) && y < int32_t(P.ty) && z < int32_t(P.tz)) {
+      uint32_t m = G.tmap[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))];
+      v = m == ST_NONE ? 0u : m;
     }
-    for (int a = 0; a < 6; a++) G.nbr[s * 6 + a] = nb[a];
+    G.nbr[s * 27u + q] = v;
   }
 }
 
@@ -385,21 +384,10 @@ __global__ void k_clear(DGrid G) {
 // A window cell's tile is one of the 27 around the tile: offset -1 at window
 // index 0, +1 at 9, else 0.  wslots[27] holds their slots (0 when the tile has
 // no storage, as cellIdx maps it; WSLOT_OUT when outside the tile grid).
-#define WSLOT_OUT 0xfffffffeu
-
-__device__ __forceinline__ void load_wslots(const DParams& P, const DGrid& G, uint32_t t, uint32_t* wslots, int3& org) {
+__device__ __forceinline__ void load_wslots(const DParams& P, const DGrid& G, uint32_t s, uint32_t t, uint32_t* wslots, int3& org) {
   int32_t ti = int32_t(t % P.tx), tj = int32_t((t / P.tx) % P.ty), tk = int32_t(t / (P.tx * P.ty));
   org = make_int3(ti * 8 - 1, tj * 8 - 1, tk * 8 - 1);
-  if (threadIdx.x < 27) {
-    int32_t dx = int32_t(threadIdx.x % 3) - 1, dy = int32_t((threadIdx.x / 3) % 3) - 1, dz = int32_t(threadIdx.x / 9) - 1;
-    int32_t x = ti + dx, y = tj + dy, z = tk + dz;
-    uint32_t v = WSLOT_OUT;
-    if (x >= 0 && y >= 0 && z >= 0 && x < int32_t(P.tx) && y < int32_t(P.ty) && z < int32_t(P.tz)) {
-      uint32_t q = G.tmap[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))];
-      v = q == ST_NONE ? 0u : q;
-    }
-    wslots[threadIdx.x] = v;
-  }
+  if (threadIdx.x < 27) wslots[threadIdx.x] = G.nbr[s * 27u + threadIdx.x];
 }
 __device__ __forceinline__ int32_t woff(int32_t w) { return w == 0 ? 0 : (w == WIN - 1 ? 2 : 1); }
 // storage index of window cell (wx, wy, wz), or ST_NONE outside the tile grid
@@ -422,8 +410,9 @@ __device__ __forceinline__ int32_t wface(const DParams& P, uint32_t axis, int32_
 // p2g (wgsl-solver.js:16-40), one workgroup per tile in use
 __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) {
   const DParams& P = *PP;
-  __shared__ int32_t sV[3][WIN3];
-  __shared__ int32_t sW[3][WIN3];
+  // per face: weight in the low 32 bits, velocity in the high 32 bits of one
+  // 64-bit sum; weights are >= 0 and their sum < 2^31, so the halves never mix
+  __shared__ unsigned long long sVW[3][WIN3];
   __shared__ int32_t sC[512];
   __shared__ uint32_t wslots[27];
   uint32_t vmaxBits = 0;
@@ -431,10 +420,9 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
   for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
     uint32_t s = G.slotList[b], t = G.slotTile[s];
     int3 org;
-    load_wslots(P, G, t, wslots, org);
+    load_wslots(P, G, s, t, wslots, org);
     for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
-      sV[0][q] = 0; sV[1][q] = 0; sV[2][q] = 0;
-      sW[0][q] = 0; sW[1][q] = 0; sW[2][q] = 0;
+      sVW[0][q] = 0ull; sVW[1][q] = 0ull; sVW[2][q] = 0ull;
     }
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) sC[q] = 0;
     __syncthreads();
@@ -469,8 +457,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
           int32_t aw = st_f2i_sat(rintf(w * P.scale));
           int32_t wi = fast ? base + dx + WIN * dy + WIN * WIN * dz : wface(P, axis, i0.x + dx, i0.y + dy, i0.z + dz, org);
           if (wi >= 0) {
-            atomicAdd(&sV[axis][wi], av);
-            atomicAdd(&sW[axis][wi], aw);
+            atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
           } else {  // outside the window: only for particles far outside the box
             uint32_t idx = d_faceIndex(P, G.tmap, axis, i0.x + dx, i0.y + dy, i0.z + dz);
             atomicAdd(&G.accV[idx], av);
@@ -485,12 +472,13 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) G.cnt[s * 512u + q] = sC[q];
     for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
       int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
-      int32_t a0 = sW[0][q], a1 = sW[1][q], a2 = sW[2][q];
-      if ((a0 | a1 | a2 | sV[0][q] | sV[1][q] | sV[2][q]) == 0) continue;
+      unsigned long long e0 = sVW[0][q], e1 = sVW[1][q], e2 = sVW[2][q];
+      if ((e0 | e1 | e2) == 0ull) continue;
       uint32_t c = wcell(wslots, wx, wy, wz, org);
       if (c == ST_NONE) continue;
+      unsigned long long e[3] = {e0, e1, e2};
       for (int a = 0; a < 3; a++) {
-        int32_t vv = sV[a][q], ww = sW[a][q];
+        int32_t ww = int32_t(uint32_t(e[a])), vv = int32_t(uint32_t(e[a] >> 32));
         if (vv) atomicAdd(&G.accV[c * 3 + a], vv);
         if (ww) atomicAdd(&G.accW[c * 3 + a], ww);
       }
@@ -681,90 +669,86 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
 }
 
 // relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
-// sim.js:291-293), one full iteration (red sweep, then black sweep) per launch.
-// Each workgroup stages a 12^3 pressure window around its tile (two cells of
-// halo) and the 10^3 rhs/stencil window in LDS, relaxes the red cells of the
-// 10^3 region, then the black cells of the tile, and writes the tile to the
-// other pressure buffer.  Red cells read only black ones and black only red, so
-// the halo's red values equal those the neighbouring tile computes, and the
-// result is the reference's red-black iterate exactly.  Neighbours are summed in
-// the reference's order (x-, x+, y-, y+, z-, z+).
-#define SW 12
-#define SW3 1728
-
-__device__ __forceinline__ void relax_lds(float* sp, int32_t p12, uint32_t st, float rhs, float omega) {
+// sim.js:291-293), K full iterations (red sweep then black sweep, K times) per
+// launch.  Each workgroup owns one tile.  It stages the pressure of a window of
+// W = 8 + 4K cells per axis (2K cells of halo) and the rhs and stencil of the
+// window less one cell each side in LDS, then runs 2K sweeps in place, sweep s
+// relaxing the cells of its colour in the window shrunk by s on every side
+// (odd s red, even s black), and writes the tile to the other pressure buffer.
+// A red cell reads only black neighbours and a black cell only red ones, so each
+// sweep needs exactly the previous sweep's values one cell further out; the
+// halo values equal those the owning tiles compute, and the result is the
+// reference's red-black iterate bit for bit.  Neighbours are summed in the
+// reference's order (x-, x+, y-, y+, z-, z+).
+template <int W>
+__device__ __forceinline__ void relax_lds(float* sp, int32_t p, uint32_t st, float rhs, float omega) {
   uint32_t nns = st & 7u, mask = (st >> 4) & 63u;
   if (nns == 0u) {
-    sp[p12] = 0.0f;
+    sp[p] = 0.0f;
     return;
   }
   float sum = 0.0f;
-  if (mask & 1u) sum = sum + sp[p12 - 1];
-  if (mask & 2u) sum = sum + sp[p12 + 1];
-  if (mask & 4u) sum = sum + sp[p12 - SW];
-  if (mask & 8u) sum = sum + sp[p12 + SW];
-  if (mask & 16u) sum = sum + sp[p12 - SW * SW];
-  if (mask & 32u) sum = sum + sp[p12 + SW * SW];
+  if (mask & 1u) sum = sum + sp[p - 1];
+  if (mask & 2u) sum = sum + sp[p + 1];
+  if (mask & 4u) sum = sum + sp[p - W];
+  if (mask & 8u) sum = sum + sp[p + W];
+  if (mask & 16u) sum = sum + sp[p - W * W];
+  if (mask & 32u) sum = sum + sp[p + W * W];
   float pg = (rhs + sum) / float(nns);
-  float pc = sp[p12];
-  sp[p12] = pc + omega * (pg - pc);
+  float pc = sp[p];
+  sp[p] = pc + omega * (pg - pc);
 }
 
-__global__ void __launch_bounds__(WG) k_sor_iter(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
-                                                 float* __restrict__ pout) {
+template <int K>
+__global__ void __launch_bounds__(WG) k_sor_iters(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
+                                                  float* __restrict__ pout) {
+  constexpr int W = 8 + 4 * K;   // pressure window edge
+  constexpr int R = W - 2;       // rhs / stencil window edge
   const DParams& P = *PP;
-  __shared__ float sp[SW3];
-  __shared__ float sr[WIN3];
-  __shared__ uint32_t ss[WIN3];
+  __shared__ float sp[W * W * W];
+  __shared__ float sr[R * R * R];
+  __shared__ uint32_t ss[R * R * R];
   __shared__ uint32_t wslots[27];
   const float omega = P.omega;
   uint32_t sc = G.ctl[4];
   for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
-    uint32_t s = G.slotList[b], t = G.slotTile[s];
-    int3 org;
-    load_wslots(P, G, t, wslots, org);
+    uint32_t s = G.slotList[b];
+    if (threadIdx.x < 27) wslots[threadIdx.x] = G.nbr[s * 27u + threadIdx.x];
     __syncthreads();
-    // pressure window: cells org - 1 .. org + 10 (window index 0..11)
-    for (uint32_t q = threadIdx.x; q < SW3; q += WG) {
-      int32_t wx = int32_t(q % SW), wy = int32_t((q / SW) % SW), wz = int32_t(q / (SW * SW));
-      int32_t ox = wx < 2 ? 0 : (wx >= 10 ? 2 : 1), oy = wy < 2 ? 0 : (wy >= 10 ? 2 : 1), oz = wz < 2 ? 0 : (wz >= 10 ? 2 : 1);
-      uint32_t sl = wslots[ox + 3 * (oy + 3 * oz)];
-      float v = 0.0f;
-      if (sl != WSLOT_OUT) {
-        int32_t gx = org.x - 1 + wx, gy = org.y - 1 + wy, gz = org.z - 1 + wz;
-        v = pin[sl * 512u + uint32_t(gx & 7) + 8u * (uint32_t(gy & 7) + 8u * uint32_t(gz & 7))];
+    // window cell w <-> tile-local cell w - 2K: tile offset -1 below 2K, +1 from 2K + 8
+    for (uint32_t q = threadIdx.x; q < uint32_t(W * W * W); q += WG) {
+      int32_t wx = int32_t(q % W), wy = int32_t((q / W) % W), wz = int32_t(q / (W * W));
+      int32_t lx = wx - 2 * K, ly = wy - 2 * K, lz = wz - 2 * K;
+      uint32_t sl = wslots[((lx >> 3) + 1) + 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1))];
+      sp[q] = sl == WSLOT_OUT ? 0.0f : pin[sl * 512u + uint32_t(lx & 7) + 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7))];
+    }
+    for (uint32_t q = threadIdx.x; q < uint32_t(R * R * R); q += WG) {
+      int32_t lx = int32_t(q % R) + 1 - 2 * K, ly = int32_t((q / R) % R) + 1 - 2 * K, lz = int32_t(q / (R * R)) + 1 - 2 * K;
+      uint32_t sl = wslots[((lx >> 3) + 1) + 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1))];
+      uint32_t c = sl * 512u + uint32_t(lx & 7) + 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
+      sr[q] = sl == WSLOT_OUT ? 0.0f : G.rhs[c];
+      ss[q] = sl == WSLOT_OUT ? 0u : G.sten[c];
+    }
+    __syncthreads();
+    // sweep s: window cells [s, W-1-s]^3 of colour (s odd: red, global x+y+z even;
+    // the window origin 8t - 2K is even, so window parity is global parity)
+#pragma unroll
+    for (int sw = 1; sw <= 2 * K; sw++) {
+      const int L = W - 2 * sw, H = L / 2;
+      const int col = (sw & 1) ? 0 : 1;
+      for (int m = threadIdx.x; m < L * L * H; m += WG) {
+        int row = m / H;
+        int wy = sw + row % L, wz = sw + row / L;
+        int wx = sw + 2 * (m % H) + ((col + wy + wz + sw) & 1);
+        int qr = (wx - 1) + R * ((wy - 1) + R * (wz - 1));
+        uint32_t st = ss[qr];
+        if (st & STEN_FLUID) relax_lds<W>(sp, wx + W * (wy + W * wz), st, sr[qr], omega);
       }
-      sp[q] = v;
+      __syncthreads();
     }
-    for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
-      int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
-      uint32_t c = wcell(wslots, wx, wy, wz, org);
-      sr[q] = c == ST_NONE ? 0.0f : G.rhs[c];
-      ss[q] = c == ST_NONE ? 0u : G.sten[c];
-    }
-    __syncthreads();
-    // red cells of the 10^3 region: global parity (8t - 1 + w) even <=> wx+wy+wz odd;
-    // 5 per row of 10, 100 rows
-    for (uint32_t m = threadIdx.x; m < 500u; m += WG) {
-      int32_t row = int32_t(m / 5u), wy = row % WIN, wz = row / WIN;
-      int32_t wx = 2 * int32_t(m % 5u) + ((wy + wz + 1) & 1);
-      int32_t q = wx + WIN * (wy + WIN * wz);
-      uint32_t st = ss[q];
-      if (st & STEN_FLUID) relax_lds(sp, (wx + 1) + SW * ((wy + 1) + SW * (wz + 1)), st, sr[q], omega);
-    }
-    __syncthreads();
-    // black cells of the tile: local x+y+z odd, one per thread
-    {
-      int32_t y = int32_t((threadIdx.x >> 2) & 7u), z = int32_t(threadIdx.x >> 5);
-      int32_t x = 2 * int32_t(threadIdx.x & 3u) + ((y + z + 1) & 1);
-      int32_t q = (x + 1) + WIN * ((y + 1) + WIN * (z + 1));
-      uint32_t st = ss[q];
-      if (st & STEN_FLUID) relax_lds(sp, (x + 2) + SW * ((y + 2) + SW * (z + 2)), st, sr[q], omega);
-    }
-    __syncthreads();
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) {
-      uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
-      pout[s * 512u + q] = sp[(x + 2) + SW * ((y + 2) + SW * (z + 2))];
+      int32_t x = int32_t(q & 7u), y = int32_t((q >> 3) & 7u), z = int32_t(q >> 6);
+      pout[s * 512u + q] = sp[(x + 2 * K) + W * ((y + 2 * K) + W * (z + 2 * K))];
     }
     __syncthreads();
   }
@@ -1040,7 +1024,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       continue;
     }
     int3 org;
-    load_wslots(P, G, t, wslots, org);
+    load_wslots(P, G, G.tmap[t], t, wslots, org);
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
       int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
@@ -1128,7 +1112,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_tile_scatter, dim3(nwg), dim3(WG), 0, st, L.P, L.G);
   hipLaunchKernelGGL(k_tile_alloc, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(nt, WG), 256u))), dim3(WG), 0, st, L.G);
   hipLaunchKernelGGL(k_init_new, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
-  hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.scap, WG), 512u))), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(L.scap) * 27, WG), 2048u))), dim3(WG), 0, st, L.P, L.G);
   // particle bins: exclusive scan of the counts, then A -> B in bin order
   launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp);
   if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
@@ -1159,12 +1143,20 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
 }
 
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
-  for (int it = 0; it < iters; it++) {
-    const float* pin = (it & 1) ? L.G.pres2 : L.G.pres;
-    float* pout = (it & 1) ? L.G.pres : L.G.pres2;
-    hipLaunchKernelGGL(k_sor_iter, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
+  // two iterations per launch; an odd count ends with a one-iteration launch
+  int launches = 0;
+  for (int it = 0; it < iters; launches++) {
+    const float* pin = (launches & 1) ? L.G.pres2 : L.G.pres;
+    float* pout = (launches & 1) ? L.G.pres : L.G.pres2;
+    if (iters - it >= 2) {
+      hipLaunchKernelGGL(k_sor_iters<2>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
+      it += 2;
+    } else {
+      hipLaunchKernelGGL(k_sor_iters<1>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
+      it += 1;
+    }
   }
-  if (iters & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
+  if (launches & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
 }
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {
