@@ -715,19 +715,62 @@ __global__ void __launch_bounds__(WG) k_sor_iters(const DParams* __restrict__ PP
     uint32_t s = G.slotList[b];
     if (threadIdx.x < 27) wslots[threadIdx.x] = G.nbr[s * 27u + threadIdx.x];
     __syncthreads();
-    // window cell w <-> tile-local cell w - 2K: tile offset -1 below 2K, +1 from 2K + 8
-    for (uint32_t q = threadIdx.x; q < uint32_t(W * W * W); q += WG) {
-      int32_t wx = int32_t(q % W), wy = int32_t((q / W) % W), wz = int32_t(q / (W * W));
-      int32_t lx = wx - 2 * K, ly = wy - 2 * K, lz = wz - 2 * K;
-      uint32_t sl = wslots[((lx >> 3) + 1) + 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1))];
-      sp[q] = sl == WSLOT_OUT ? 0.0f : pin[sl * 512u + uint32_t(lx & 7) + 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7))];
+    // Window loads by rows: a window row (fixed y, z) is the last 2K cells of
+    // a row of the tile on the left, a whole 8-cell row of the tile itself and
+    // the first 2K cells of a row of the tile on the right; tile storage is
+    // x-fastest, so each part is one or two aligned vector loads.
+    for (uint32_t r = threadIdx.x; r < uint32_t(W * W); r += WG) {
+      int32_t ly = int32_t(r % W) - 2 * K, lz = int32_t(r / W) - 2 * K;
+      uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
+      int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
+      uint32_t sl = wslots[so], sc0 = wslots[so + 1], sr = wslots[so + 2];
+      float* dst = sp + W * int32_t(r);
+      if (K == 1) {
+        float2 l = sl == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sl * 512u + rowoff + 6u);
+        float2 h = sr == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sr * 512u + rowoff);
+        dst[0] = l.x; dst[1] = l.y; dst[10] = h.x; dst[11] = h.y;
+      } else {
+        float4 l = sl == WSLOT_OUT ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4*)(pin + sl * 512u + rowoff + 4u);
+        float4 h = sr == WSLOT_OUT ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4*)(pin + sr * 512u + rowoff);
+        dst[0] = l.x; dst[1] = l.y; dst[2] = l.z; dst[3] = l.w;
+        dst[12] = h.x; dst[13] = h.y; dst[14] = h.z; dst[15] = h.w;
+      }
+      float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+      if (sc0 != WSLOT_OUT) {
+        c0 = *(const float4*)(pin + sc0 * 512u + rowoff);
+        c1 = *(const float4*)(pin + sc0 * 512u + rowoff + 4u);
+      }
+      dst[2 * K + 0] = c0.x; dst[2 * K + 1] = c0.y; dst[2 * K + 2] = c0.z; dst[2 * K + 3] = c0.w;
+      dst[2 * K + 4] = c1.x; dst[2 * K + 5] = c1.y; dst[2 * K + 6] = c1.z; dst[2 * K + 7] = c1.w;
     }
-    for (uint32_t q = threadIdx.x; q < uint32_t(R * R * R); q += WG) {
-      int32_t lx = int32_t(q % R) + 1 - 2 * K, ly = int32_t((q / R) % R) + 1 - 2 * K, lz = int32_t(q / (R * R)) + 1 - 2 * K;
-      uint32_t sl = wslots[((lx >> 3) + 1) + 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1))];
-      uint32_t c = sl * 512u + uint32_t(lx & 7) + 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
-      sr[q] = sl == WSLOT_OUT ? 0.0f : G.rhs[c];
-      ss[q] = sl == WSLOT_OUT ? 0u : G.sten[c];
+    // rhs and stencil rows of the window less one cell: 2K-1 cells left, 8, 2K-1 right
+    for (uint32_t r = threadIdx.x; r < uint32_t(R * R); r += WG) {
+      int32_t ly = int32_t(r % R) + 1 - 2 * K, lz = int32_t(r / R) + 1 - 2 * K;
+      uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
+      int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
+      uint32_t sl = wslots[so], sc0 = wslots[so + 1], sh = wslots[so + 2];
+      float* dr = sr + R * int32_t(r);
+      uint32_t* ds = ss + R * int32_t(r);
+      for (int j = 0; j < 2 * K - 1; j++) {
+        uint32_t cl = sl * 512u + rowoff + uint32_t(9 - 2 * K + j), ch = sh * 512u + rowoff + uint32_t(j);
+        dr[j] = sl == WSLOT_OUT ? 0.0f : G.rhs[cl];
+        ds[j] = sl == WSLOT_OUT ? 0u : G.sten[cl];
+        dr[2 * K + 7 + j] = sh == WSLOT_OUT ? 0.0f : G.rhs[ch];
+        ds[2 * K + 7 + j] = sh == WSLOT_OUT ? 0u : G.sten[ch];
+      }
+      float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
+      uint4 t0 = make_uint4(0u, 0u, 0u, 0u), t1 = t0;
+      if (sc0 != WSLOT_OUT) {
+        r0 = *(const float4*)(G.rhs + sc0 * 512u + rowoff);
+        r1 = *(const float4*)(G.rhs + sc0 * 512u + rowoff + 4u);
+        t0 = *(const uint4*)(G.sten + sc0 * 512u + rowoff);
+        t1 = *(const uint4*)(G.sten + sc0 * 512u + rowoff + 4u);
+      }
+      const int o = 2 * K - 1;
+      dr[o + 0] = r0.x; dr[o + 1] = r0.y; dr[o + 2] = r0.z; dr[o + 3] = r0.w;
+      dr[o + 4] = r1.x; dr[o + 5] = r1.y; dr[o + 6] = r1.z; dr[o + 7] = r1.w;
+      ds[o + 0] = t0.x; ds[o + 1] = t0.y; ds[o + 2] = t0.z; ds[o + 3] = t0.w;
+      ds[o + 4] = t1.x; ds[o + 5] = t1.y; ds[o + 6] = t1.z; ds[o + 7] = t1.w;
     }
     __syncthreads();
     // sweep s: window cells [s, W-1-s]^3 of colour (s odd: red, global x+y+z even;
@@ -1142,13 +1185,15 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_divergence, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
 }
 
+// iterations per SOR launch: 1 (12^3 window) costs the least work per iteration
+#define SOR_ITERS_PER_LAUNCH 1
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
   // two iterations per launch; an odd count ends with a one-iteration launch
   int launches = 0;
   for (int it = 0; it < iters; launches++) {
     const float* pin = (launches & 1) ? L.G.pres2 : L.G.pres;
     float* pout = (launches & 1) ? L.G.pres : L.G.pres2;
-    if (iters - it >= 2) {
+    if (SOR_ITERS_PER_LAUNCH == 2 && iters - it >= 2) {
       hipLaunchKernelGGL(k_sor_iters<2>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
       it += 2;
     } else {
