@@ -95,6 +95,7 @@ struct Sim {
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     L.sorGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 65535u));
+    L.pipeGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 2048u));
     return L;
   }
 };
@@ -115,13 +116,14 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.fstack, n))) return rc;
   if ((rc = dalloc(&s->G.slotList, n))) return rc;
   if ((rc = dalloc(&s->G.nbr, size_t(n) * 27))) return rc;
+  if ((rc = dalloc(&s->G.nbrL, size_t(n) * 27))) return rc;
   return ST_OK;
 }
 
 void freeSlotArrays(DGrid& g) {
   dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.pres2); dfree(g.rhs); dfree(g.sten);
   dfree(g.accV); dfree(g.accW); dfree(g.velG);
-  dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr);
+  dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr); dfree(g.nbrL);
 }
 void freeSlots(Sim* s) { freeSlotArrays(s->G); }
 
@@ -144,7 +146,7 @@ void destroySim(Sim* s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
-  dfree(s->G.keyRank); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
+  dfree(s->G.binKey); dfree(s->G.binCursor); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part);
   dfree(s->G.scratch4);
   freeSlots(s);
@@ -321,8 +323,8 @@ int st_create(const st_opts* o, void** out) {
   size_t M = s->maxn;
   uint32_t nbin = s->ntiles + 1;
   if ((rc = dalloc(&s->G.pos, M)) || (rc = dalloc(&s->G.vel, M)) || (rc = dalloc(&s->G.pid, M)) || (rc = dalloc(&s->G.scratch4, M)) ||
-      (rc = dalloc(&s->G.posB, M)) || (rc = dalloc(&s->G.velB, M)) || (rc = dalloc(&s->G.pidB, M)) || (rc = dalloc(&s->G.keyRank, M)) ||
-      (rc = dalloc(&s->G.binCount, nbin)) || (rc = dalloc(&s->G.binStart, size_t(nbin) + 1)) ||
+      (rc = dalloc(&s->G.posB, M)) || (rc = dalloc(&s->G.velB, M)) || (rc = dalloc(&s->G.pidB, M)) || (rc = dalloc(&s->G.binKey, M)) ||
+      (rc = dalloc(&s->G.binCount, nbin)) || (rc = dalloc(&s->G.binStart, size_t(nbin) + 1)) || (rc = dalloc(&s->G.binCursor, nbin)) ||
       (rc = dalloc(&s->G.scanTmp, size_t(nbin + 1023) / 1024 + 1)) ||
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
@@ -518,7 +520,7 @@ int st_grow(void* p, uint32_t n) {
   DGrid keep = s->G;
   s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.pres2 = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
   s->G.accV = nullptr; s->G.accW = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
-  s->G.slotList = nullptr; s->G.nbr = nullptr;
+  s->G.slotList = nullptr; s->G.nbr = nullptr; s->G.nbrL = nullptr;
   int rc = allocSlots(s, n);
   if (rc) { freeSlots(s); s->G = keep; return rc; }
   // the wg scratch and grid sizes follow the pool
@@ -639,7 +641,7 @@ int st_mem_bytes(void* p, uint64_t* bytes) {
   Sim* s = getSim(p);
   if (!s || !bytes) return fail(ST_ERR_ARG, "st_mem_bytes: null argument");
   uint64_t M = s->maxn, C = uint64_t(s->scap) * 512;
-  *bytes = M * (2 * (16 + 16 + 4) + 8 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
+  *bytes = M * (2 * (16 + 16 + 4) + 4 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
   return ST_OK;
 }
 

@@ -48,7 +48,8 @@ struct DGrid {
   float4* posB;  // bin-sorted copy read by p2g and g2p
   float4* velB;
   uint32_t* pidB;
-  uint2* keyRank;      // per particle (A order): bin (tile of its cell, ntiles if dead), rank in bin
+  uint32_t* binKey;    // per particle (A order): bin = tile of its cell, ntiles if dead
+  uint32_t* binCursor; // per bin: next free place while scattering into bin order
   uint32_t* binCount;  // per tile (+1 dead bin): particles whose cell is in the tile
   uint32_t* binStart;  // exclusive prefix of binCount (ntiles + 2 entries, last = N)
   uint32_t* scanTmp;   // per-1024-block sums of the bin scan
@@ -58,7 +59,8 @@ struct DGrid {
   uint32_t* slotTile;
   uint32_t* fstack;
   uint32_t* slotList;
-  uint32_t* nbr;       // per slot: slots of the 6 face-neighbour tiles (0 if none)
+  uint32_t* nbr;       // per slot: slots of the 27 tiles around it (0 none, WSLOT_OUT outside)
+  uint32_t* nbrL;      // the same rows in slotList order (row b for slotList[b])
   uint32_t* wg;        // per-workgroup scan scratch
   uint32_t* part;      // per-workgroup statistics partials (vmax; fluid cells per colour)
   uint32_t* ctl;       // control words, as in the reference's ctl

@@ -16,6 +16,7 @@ struct LaunchCtx {
   uint32_t cellGrid;  // workgroups of grid-stride loops over active cells
   uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
   uint32_t sorGrid;   // workgroups of the SOR launches (one tile each when it fits)
+  uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);

@@ -20,6 +20,7 @@
 #define WG 256
 #define WIN 10      // window edge: tile of 8 plus one cell each side
 #define WIN3 1000
+#define SW3_1 1728  // 12^3: pressure window of one SOR iteration
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
@@ -70,10 +71,9 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
       for (int32_t dy = ly; dy <= hy; dy++)
         for (int32_t dx = lx; dx <= hx; dx++) nmask |= 1u << uint32_t((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
   }
-  // runs of equal bins in the wave: one bin count atomic per run, and the run's
-  // leader marks the union of its particles' tiles
+  // runs of equal bins in the wave: one bin count atomic per run (its result is
+  // not waited for), and the run's leader marks the union of its particles' tiles
   uint64_t rem = __ballot(valid);
-  uint32_t rank = 0;
   while (rem) {
     int leader = __ffsll((unsigned long long)rem) - 1;
     uint32_t k = (uint32_t)__shfl((int)key, leader);
@@ -81,9 +81,8 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
     bool inrun = (m >> lane_id()) & 1ull;
     uint32_t um = inrun ? nmask : 0u;
     for (int off = 32; off > 0; off >>= 1) um |= (uint32_t)__shfl_xor((int)um, off);
-    uint32_t base = 0;
     if (int(lane_id()) == leader) {
-      base = atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+      atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
       if (k < P.ntiles) {
         int32_t ti = int32_t(k % P.tx), tj = int32_t((k / P.tx) % P.ty), tk = int32_t(k / (P.tx * P.ty));
         while (um) {
@@ -94,11 +93,9 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
         }
       }
     }
-    base = (uint32_t)__shfl((int)base, leader);
-    if (inrun) rank = base + (uint32_t)__popcll(m & lanemask_lt());
     rem &= ~m;
   }
-  if (valid) G.keyRank[i] = make_uint2(key, rank);
+  if (valid) G.binKey[i] = key;
 }
 
 // exclusive scan of n counts into out[0..n], out[n] = total; three launches
@@ -158,22 +155,44 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, 
   }
   if (threadIdx.x == 0) out[n] = carry;
 }
-__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums) {
+__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums,
+                                                 uint32_t* __restrict__ copy) {
   uint32_t base = blockIdx.x * (WG * 4) + threadIdx.x * 4;
   uint32_t add = sums[blockIdx.x];
   for (int q = 0; q < 4; q++)
-    if (base + q < n) out[base + q] += add;
+    if (base + q < n) {
+      uint32_t v = out[base + q] + add;
+      out[base + q] = v;
+      if (copy) copy[base + q] = v;
+    }
 }
 
-// scatter into bin order: A (pos, vel, pid) -> B
+// scatter into bin order: A (pos, vel, pid) -> B.  Each run of equal bins in a
+// wave takes its places with one cursor atomic; the particle loads do not wait
+// for it.
 __global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= PP->n) return;
-  uint2 kr = G.keyRank[i];
-  uint32_t dst = G.binStart[kr.x] + kr.y;
-  G.posB[dst] = G.pos[i];
-  G.velB[dst] = G.vel[i];
-  G.pidB[dst] = G.pid[i];
+  bool valid = i < PP->n;
+  uint32_t key = valid ? G.binKey[i] : 0u;
+  float4 p = valid ? G.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 v = valid ? G.vel[i] : p;
+  uint32_t id = valid ? G.pid[i] : 0u;
+  uint64_t rem = __ballot(valid);
+  uint32_t dst = 0;
+  while (rem) {
+    int leader = __ffsll((unsigned long long)rem) - 1;
+    uint32_t k = (uint32_t)__shfl((int)key, leader);
+    uint64_t m = __ballot(valid && key == k) & rem;
+    uint32_t base = 0;
+    if (int(lane_id()) == leader) base = atomicAdd(&G.binCursor[k], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if ((m >> lane_id()) & 1ull) dst = base + (uint32_t)__popcll(m & lanemask_lt());
+    rem &= ~m;
+  }
+  if (!valid) return;
+  G.posB[dst] = p;
+  G.velB[dst] = v;
+  G.pidB[dst] = id;
 }
 
 // Flags of one tile: freed (no longer needed, had a slot), new (needed, had
@@ -336,6 +355,7 @@ __global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
       v = m == ST_NONE ? 0u : m;
     }
     G.nbr[s * 27u + q] = v;
+    G.nbrL[e] = v;
   }
 }
 
@@ -797,6 +817,132 @@ __global__ void __launch_bounds__(WG) k_sor_iters(const DParams* __restrict__ PP
   }
 }
 
+// One red-black iteration (as k_sor_iters<1>) by persistent workgroups, about
+// eight per CU, each walking tiles b, b + G, b + 2G, ... of slotList.  While a
+// workgroup relaxes tile b in LDS, the window rows of tile b + G are already in
+// flight into registers (thread r < 144 holds pressure row r of the 12^3
+// window; thread 144 + r < 244 holds rhs and stencil row r of the 10^3
+// window), and the 27 slots of tile b + 2G are being fetched.  Tiles are
+// independent within the launch (they read pin, write pout), so the pipeline
+// only reorders loads.
+struct SorRows {
+  float4 a, b, c;   // pressure: l.xy in a.xy, centre in b, c (x..w), h.xy in a.zw
+  uint4 t0, t1;     // rhs/stencil: centre stencil
+  float4 r0, r1;    // centre rhs
+  float rl, rh;     // rhs of the left / right cell
+  uint32_t tl, th;  // stencil of the left / right cell
+};
+
+__device__ __forceinline__ void sor_fetch(const DGrid& G, const float* __restrict__ pin, const uint32_t* ws, uint32_t tid, SorRows& R) {
+  if (tid < 144u) {
+    int32_t ly = int32_t(tid % 12u) - 2, lz = int32_t(tid / 12u) - 2;
+    uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
+    int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
+    uint32_t sl = ws[so], sc = ws[so + 1], sh = ws[so + 2];
+    float2 l = sl == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sl * 512u + rowoff + 6u);
+    float2 h = sh == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sh * 512u + rowoff);
+    R.a = make_float4(l.x, l.y, h.x, h.y);
+    if (sc != WSLOT_OUT) {
+      R.b = *(const float4*)(pin + sc * 512u + rowoff);
+      R.c = *(const float4*)(pin + sc * 512u + rowoff + 4u);
+    } else {
+      R.b = make_float4(0.f, 0.f, 0.f, 0.f);
+      R.c = R.b;
+    }
+  } else if (tid < 244u) {
+    uint32_t r = tid - 144u;
+    int32_t ly = int32_t(r % 10u) - 1, lz = int32_t(r / 10u) - 1;
+    uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
+    int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
+    uint32_t sl = ws[so], sc = ws[so + 1], sh = ws[so + 2];
+    R.rl = sl == WSLOT_OUT ? 0.0f : G.rhs[sl * 512u + rowoff + 7u];
+    R.tl = sl == WSLOT_OUT ? 0u : G.sten[sl * 512u + rowoff + 7u];
+    R.rh = sh == WSLOT_OUT ? 0.0f : G.rhs[sh * 512u + rowoff];
+    R.th = sh == WSLOT_OUT ? 0u : G.sten[sh * 512u + rowoff];
+    if (sc != WSLOT_OUT) {
+      R.r0 = *(const float4*)(G.rhs + sc * 512u + rowoff);
+      R.r1 = *(const float4*)(G.rhs + sc * 512u + rowoff + 4u);
+      R.t0 = *(const uint4*)(G.sten + sc * 512u + rowoff);
+      R.t1 = *(const uint4*)(G.sten + sc * 512u + rowoff + 4u);
+    } else {
+      R.r0 = make_float4(0.f, 0.f, 0.f, 0.f); R.r1 = R.r0;
+      R.t0 = make_uint4(0u, 0u, 0u, 0u); R.t1 = R.t0;
+    }
+  }
+}
+__device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, uint32_t tid, const SorRows& R) {
+  if (tid < 144u) {
+    float* d = sp + 12u * tid;
+    d[0] = R.a.x; d[1] = R.a.y;
+    d[2] = R.b.x; d[3] = R.b.y; d[4] = R.b.z; d[5] = R.b.w;
+    d[6] = R.c.x; d[7] = R.c.y; d[8] = R.c.z; d[9] = R.c.w;
+    d[10] = R.a.z; d[11] = R.a.w;
+  } else if (tid < 244u) {
+    uint32_t r = tid - 144u;
+    float* d = sr + 10u * r;
+    uint32_t* e = ss + 10u * r;
+    d[0] = R.rl; d[9] = R.rh; e[0] = R.tl; e[9] = R.th;
+    d[1] = R.r0.x; d[2] = R.r0.y; d[3] = R.r0.z; d[4] = R.r0.w;
+    d[5] = R.r1.x; d[6] = R.r1.y; d[7] = R.r1.z; d[8] = R.r1.w;
+    e[1] = R.t0.x; e[2] = R.t0.y; e[3] = R.t0.z; e[4] = R.t0.w;
+    e[5] = R.t1.x; e[6] = R.t1.y; e[7] = R.t1.z; e[8] = R.t1.w;
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
+                                                 float* __restrict__ pout) {
+  __shared__ float sp[SW3_1];
+  __shared__ float sr[WIN3];
+  __shared__ uint32_t ss[WIN3];
+  __shared__ uint32_t wsl[2][27];
+  const float omega = PP->omega;
+  const uint32_t sc = G.ctl[4], gs = gridDim.x, tid = threadIdx.x;
+  uint32_t b = blockIdx.x;
+  if (b >= sc) return;
+  // prologue: slots of tiles b and b + G, window rows of tile b
+  if (tid < 27u) {
+    wsl[0][tid] = G.nbrL[b * 27u + tid];
+    wsl[1][tid] = b + gs < sc ? G.nbrL[(b + gs) * 27u + tid] : 0u;
+  }
+  __syncthreads();
+  SorRows R;
+  sor_fetch(G, pin, wsl[0], tid, R);
+  uint32_t cur = 0;
+  for (; b < sc; b += gs, cur ^= 1u) {
+    const uint32_t s = wsl[cur][13];
+    const bool more = b + gs < sc;
+    uint32_t ahead = 0u;
+    if (tid < 27u && b + 2u * gs < sc) ahead = G.nbrL[(b + 2u * gs) * 27u + tid];
+    sor_store(sp, sr, ss, tid, R);
+    __syncthreads();
+    if (more) sor_fetch(G, pin, wsl[cur ^ 1u], tid, R);
+    // red cells of the 10^3 region (window parity = global parity, origin 8t - 2)
+    for (uint32_t m = tid; m < 500u; m += WG) {
+      int32_t row = int32_t(m / 5u), wy = 1 + row % 10, wz = 1 + row / 10;
+      int32_t wx = 1 + 2 * int32_t(m % 5u) + ((wy + wz + 1) & 1);
+      int32_t qr = (wx - 1) + 10 * ((wy - 1) + 10 * (wz - 1));
+      uint32_t st = ss[qr];
+      if (st & STEN_FLUID) relax_lds<12>(sp, wx + 12 * (wy + 12 * wz), st, sr[qr], omega);
+    }
+    __syncthreads();
+    {  // black cells of the tile, one per thread
+      int32_t y = int32_t((tid >> 2) & 7u), z = int32_t(tid >> 5);
+      int32_t x = 2 * int32_t(tid & 3u) + ((y + z + 1) & 1);
+      int32_t qr = (x + 1) + 10 * ((y + 1) + 10 * (z + 1));
+      uint32_t st = ss[qr];
+      if (st & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), st, sr[qr], omega);
+    }
+    __syncthreads();
+    for (uint32_t q = tid; q < 512u; q += WG) {
+      uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
+      pout[s * 512u + q] = sp[(x + 2) + 12 * ((y + 2) + 12 * (z + 2))];
+    }
+    // slots of tile b + 2G go where tile b's were (read above into s)
+    if (tid < 27u) wsl[cur][tid] = ahead;
+    __syncthreads();
+  }
+}
+
 // after an odd number of iterations the result sits in the partner buffer
 __global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst) {
   uint32_t tot = G.ctl[4] * 512u;
@@ -1137,11 +1283,11 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(WG), 0, st, p, n, v);
 }
 
-static void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums) {
+static void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* copy) {
   uint32_t nb = gridFor(n, WG * 4);
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(WG), 0, st, in, out, n, sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nb, out, n);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(WG), 0, st, out, n, (const uint32_t*)sums);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(WG), 0, st, out, n, (const uint32_t*)sums, copy);
 }
 
 void launch_blocks(hipStream_t st, const LaunchCtx& L) {
@@ -1157,7 +1303,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_init_new, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(L.scap) * 27, WG), 2048u))), dim3(WG), 0, st, L.P, L.G);
   // particle bins: exclusive scan of the counts, then A -> B in bin order
-  launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp);
+  launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp, L.G.binCursor);
   if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
 }
 
@@ -1187,6 +1333,7 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
 
 // iterations per SOR launch: 1 (12^3 window) costs the least work per iteration
 #define SOR_ITERS_PER_LAUNCH 1
+#define SOR_PIPELINED 1
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
   // two iterations per launch; an odd count ends with a one-iteration launch
   int launches = 0;
@@ -1196,6 +1343,9 @@ void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
     if (SOR_ITERS_PER_LAUNCH == 2 && iters - it >= 2) {
       hipLaunchKernelGGL(k_sor_iters<2>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
       it += 2;
+    } else if (SOR_PIPELINED) {
+      hipLaunchKernelGGL(k_sor_pipe, dim3(L.pipeGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
+      it += 1;
     } else {
       hipLaunchKernelGGL(k_sor_iters<1>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
       it += 1;
