@@ -71,8 +71,16 @@ def run_gpu(sc, args):
     alive = int(np.sum(pos[:, 1] >= -1e5))
     ctl = sim.ctl()
     stats = dict(blocks=int(ctl[4]), fluid=int(ctl[0] + ctl[1]), vmax=float(ctl[6:7].view(np.float32)[0]))
-    # per-stage times of one more substep (events around each stage on the sim stream)
-    stage_ms = sim.profile_step(prm) if hasattr(sim, "profile_step") else None
+    # per-stage times of a few more substeps: HIP events around each stage on the
+    # sim's stream (the stream every kernel of the stage is launched on)
+    runs = []
+    for _ in range(args.stage_steps):
+        t += dt
+        if sc.colliders:
+            cf, sdf, sig = sc.collider_state(t, dt)
+            sim.set_colliders(cf, sdf, sig)
+        runs.append(sim.time_stages(prm))
+    stage_ms = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]} if runs else None
     sim.close()
     return wall, alive, stats, stage_ms
 
@@ -109,6 +117,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the substep runs on a single MI355X; "
@@ -118,6 +127,22 @@ def main():
     wall, alive, stats, stage_ms = run_gpu(sc, args)
     value = alive * args.steps / wall
     ms_per_step = wall * 1e3 / args.steps
+    iters = int(sc.params["iters"])
+    fc, blocks = stats["fluid"], stats["blocks"]
+    A = blocks * 512
+    T = ((256 + 1 + 7) // 8) ** 3 if args.workload == "splash256" else ((64 + 1 + 7) // 8) ** 3
+    # algorithmic bytes of one substep: 112 N + 224 A + (20 + 16 I) Fc + 16 T
+    sub_bytes = 112 * alive + 224 * A + (20 + 16 * iters) * fc + 16 * T
+    roof = None
+    if stage_ms:
+        # dominant kernel: the SOR iteration (k_sor_pipe), one launch per red-black
+        # iteration; 16 B per fluid cell per iteration (rhs, type, pressure read+write)
+        launch_us = stage_ms["pressure"] * 1e3 / iters
+        sor_bytes = 16 * fc
+        ach = sor_bytes / (launch_us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "kernel": "k_sor_pipe (one red-black SOR iteration per launch)",
+                "algorithmic_bytes_per_launch": sor_bytes, "launch_us": launch_us}
     out = {
         "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -125,6 +150,10 @@ def main():
         "config": {"workload": sc.name + ("_256^3" if args.workload == "splash256" else "_64^3"), "particles": sc.n,
                    "sor_iters": sc.params["iters"], "dt": sc.params["dt"], **stats},
         "stage_ms": stage_ms,
+        "roofline": roof,
+        "substep_roofline": {"algorithmic_bytes": sub_bytes, "achieved": sub_bytes / (ms_per_step * 1e-3) / 1e9,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": sub_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_cpu:
         v, cores, cw = cpu_baseline(sc, args)

@@ -171,6 +171,11 @@ int st_begin_step(void* sim, const st_params* p);
 /* Enqueue one stage of the substep begun by st_begin_step (stepTimed, sim.js:320-328). */
 int st_stage(void* sim, int32_t stage);
 
+/* stepTimed (sim.js:312-331): one full substep as st_step, with a HIP event pair
+ * around every stage on the sim's stream; ms[k] = elapsed milliseconds of stage k
+ * (0 for stages that launch nothing).  Blocks until the substep is done. */
+int st_time_stages(void* sim, const st_params* p, float ms[ST_STAGE_COUNT]);
+
 /* FlipSim.readbackInto: N float4 positions in id order (sim.js:439-455); blocks. */
 int st_read_particles(void* sim, float* dst4, uint32_t n);
 /* readStats: copy of the 16 control words (sim.js:160-168): 0,1 red/black fluid

@@ -111,6 +111,7 @@ PROTOS = {
     "st_step_n": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.c_int32]),
     "st_begin_step": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
     "st_stage": (C.c_int, [C.c_void_p, C.c_int32]),
+    "st_time_stages": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_void_p]),
     "st_read_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "st_read_ctl": (C.c_int, [C.c_void_p, C.c_void_p]),
     "st_grow": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -232,6 +233,12 @@ class FlipSimC(_Base):
 
     def step_n(self, prm: StParams, n: int, graph: bool = False):
         self._call("step_n", self.h, C.byref(prm), int(n), 1 if graph else 0)
+
+    def time_stages(self, prm: StParams):
+        """One substep with HIP events around each stage: {stage name: ms}."""
+        ms = np.zeros(ST_STAGE_COUNT, np.float32)
+        self._call("time_stages", self.h, C.byref(prm), ms.ctypes.data)
+        return {STAGES[k]: float(ms[k]) for k in range(ST_STAGE_COUNT)}
 
     def read_particles(self, n=None):
         cnt = C.c_uint32(0)

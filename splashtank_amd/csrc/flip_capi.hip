@@ -487,6 +487,30 @@ int st_step_n(void* p, const st_params* prm, int32_t nsteps, int32_t use_graph) 
   return ST_OK;
 }
 
+int st_time_stages(void* p, const st_params* prm, float ms[ST_STAGE_COUNT]) {
+  Sim* s = getSim(p);
+  if (!s || !prm || !ms) return fail(ST_ERR_ARG, "st_time_stages: null argument");
+  int rc = st_begin_step(p, prm);
+  if (rc) return rc;
+  hipEvent_t e[ST_STAGE_COUNT + 1];
+  for (int k = 0; k <= ST_STAGE_COUNT; k++) HIPCHK(hipEventCreate(&e[k]));
+  HIPCHK(hipEventRecord(e[0], s->st));
+  for (int k = 0; k < ST_STAGE_COUNT; k++) {
+    if ((rc = runStage(s, k))) break;
+    HIPCHK(hipEventRecord(e[k + 1], s->st));
+  }
+  if (!rc) {
+    HIPCHK(hipEventSynchronize(e[ST_STAGE_COUNT]));
+    for (int k = 0; k < ST_STAGE_COUNT; k++) {
+      float t = 0.0f;
+      HIPCHK(hipEventElapsedTime(&t, e[k], e[k + 1]));
+      ms[k] = t;
+    }
+  }
+  for (int k = 0; k <= ST_STAGE_COUNT; k++) (void)hipEventDestroy(e[k]);
+  return rc;
+}
+
 int st_read_particles(void* p, float* dst4, uint32_t n) {
   Sim* s = getSim(p);
   if (!s || (n && !dst4)) return fail(ST_ERR_ARG, "st_read_particles: null argument");
