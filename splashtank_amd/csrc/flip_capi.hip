@@ -83,6 +83,7 @@ struct Sim {
   uint32_t initStep = 0;
   int iters = 1;
   uint32_t lcap = 0;
+  NbScratch nb{};
 
   LaunchCtx ctx() const {
     LaunchCtx L;
@@ -96,6 +97,8 @@ struct Sim {
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     L.sorGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 65535u));
     L.pipeGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 2048u));
+    L.nbOn = nbOn;
+    L.initStep = initStep;
     return L;
   }
 };
@@ -142,9 +145,33 @@ int zeroSlots(Sim* s, uint32_t from, uint32_t to) {
   return ST_OK;
 }
 
+void freeNb(NbScratch& S) {
+  dfree(S.deadFlag); dfree(S.deadStart); dfree(S.deadList); dfree(S.need); dfree(S.needStart); dfree(S.ent); dfree(S.entIdx);
+  dfree(S.sums);
+  S.ncap = S.nmax = 0;
+}
+// reseed scratch sized for the pool and the particle capacity
+int ensureNb(Sim* s) {
+  uint32_t ncap = s->scap * 512u;
+  if (s->nb.deadFlag && s->nb.ncap == ncap && s->nb.nmax == s->maxn) return ST_OK;
+  freeNb(s->nb);
+  size_t M = s->maxn, big = std::max<size_t>(M, ncap);
+  int rc;
+  if ((rc = dalloc(&s->nb.deadFlag, M)) || (rc = dalloc(&s->nb.deadStart, M + 1)) || (rc = dalloc(&s->nb.deadList, M)) ||
+      (rc = dalloc(&s->nb.need, ncap)) || (rc = dalloc(&s->nb.needStart, size_t(ncap) + 1)) || (rc = dalloc(&s->nb.ent, ncap)) ||
+      (rc = dalloc(&s->nb.entIdx, size_t(ncap) + 1)) || (rc = dalloc(&s->nb.sums, big / 1024 + 2))) {
+    freeNb(s->nb);
+    return rc;
+  }
+  s->nb.ncap = ncap;
+  s->nb.nmax = s->maxn;
+  return ST_OK;
+}
+
 void destroySim(Sim* s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
+  freeNb(s->nb);
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
   dfree(s->G.binKey); dfree(s->G.binCursor); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part);
@@ -174,6 +201,10 @@ int beginStep(Sim* s, const st_params* o) {
     s->lcap = s->scap * 512u;
   }
   s->initStep = s->nbOn ? o->nb_init : 0u;
+  if (s->nbOn) {
+    int rc = ensureNb(s);
+    if (rc) return rc;
+  }
   uint32_t nc = s->ncTotal >= o->nfv ? s->ncTotal - o->nfv : 0u;
   {  // the per-cell collider lookup only changes when a collider moves (sim.js:191-195)
     bool same = s->haveCols && s->sdfSig == s->sdfSigPrev && !o->no_solid_cache && s->cfPrev.size() == size_t(nc) * 32;
@@ -241,7 +272,7 @@ int runStage(Sim* s, int stage) {
     case ST_STAGE_BLOCKS: launch_blocks(st, L); break;
     case ST_STAGE_CLEAR: if (full) launch_clear(st, L); break;
     case ST_STAGE_P2G: if (full) launch_p2g(st, L); break;
-    case ST_STAGE_NARROWBAND: break;
+    case ST_STAGE_NARROWBAND: if (full && s->nbOn) launch_narrowband(st, L); break;
     case ST_STAGE_CLASSIFY:
       if (full) {
         launch_classify(st, L, s->solidDirty);
@@ -254,7 +285,7 @@ int runStage(Sim* s, int stage) {
     case ST_STAGE_PROJECT: if (full) launch_project(st, L); break;
     case ST_STAGE_G2P: if (full) launch_g2p(st, L); break;
     case ST_STAGE_SPRAY: break;
-    case ST_STAGE_RESEED: break;
+    case ST_STAGE_RESEED: if (full && s->nbOn) launch_reseed(st, L, s->nb); break;
     default: return fail(ST_ERR_ARG, "st_stage: unknown stage");
   }
   hipError_t e = hipGetLastError();

@@ -17,6 +17,22 @@ struct LaunchCtx {
   uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
   uint32_t sorGrid;   // workgroups of the SOR launches (one tile each when it fits)
   uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
+  bool nbOn;          // narrow band on this step
+  uint32_t initStep;  // narrow-band start: 1 restart, 2 water added, 0 none
+};
+
+// scratch of the narrow-band reseed: dead-slot compaction over particles and the
+// deficit scan over active cells (ncap = pool cells)
+struct NbScratch {
+  uint32_t* deadFlag;   // N
+  uint32_t* deadStart;  // N + 1
+  uint32_t* deadList;   // N
+  uint32_t* need;       // ncap
+  uint32_t* needStart;  // ncap + 1
+  uint32_t* ent;        // ncap
+  uint32_t* entIdx;     // ncap + 1
+  uint32_t* sums;       // per-1024 block sums of the scans
+  uint32_t ncap, nmax;
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
@@ -24,6 +40,8 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L);
 void launch_clear(hipStream_t st, const LaunchCtx& L);
 void launch_p2g(hipStream_t st, const LaunchCtx& L);
 void launch_classify(hipStream_t st, const LaunchCtx& L, bool solid);
+void launch_narrowband(hipStream_t st, const LaunchCtx& L);
+void launch_reseed(hipStream_t st, const LaunchCtx& L, const NbScratch& S);
 void launch_norm_faces(hipStream_t st, const LaunchCtx& L);
 void launch_divergence(hipStream_t st, const LaunchCtx& L);
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters);

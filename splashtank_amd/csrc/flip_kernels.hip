@@ -1265,6 +1265,267 @@ __device__ __forceinline__ float3 d_velGVec(const DParams& P, const DGrid& G, fl
 }
 
 // ---------------------------------------------------------------------------
+// narrow band (wgsl-narrowband.js, wgsl-core.js:156-238); the ocean is not part
+// of this library, so its branches are the ocean-off ones
+
+// signed distance in cells to the starting water: boxes, spheres, water volumes
+// (wgsl-core.js:228-238)
+__device__ __forceinline__ float d_fillPhi(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, float3 x) {
+  float dd = 1.0e9f;
+  for (uint32_t k = 0; k < P.nfill; k++) {
+    const float* A = P.fills[k * 2];
+    const float* B = P.fills[k * 2 + 1];
+    if (A[0] < 1.5f) {
+      float3 q = make_float3(fmaxf(A[1] - x.x, x.x - B[0]), fmaxf(A[2] - x.y, x.y - B[1]), fmaxf(A[3] - x.z, x.z - B[2]));
+      float3 qp = make_float3(fmaxf(q.x, 0.0f), fmaxf(q.y, 0.0f), fmaxf(q.z, 0.0f));
+      dd = fminf(dd, f3len(qp) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f));
+    } else {
+      dd = fminf(dd, f3len(f3sub(x, make_float3(A[1], A[2], A[3]))) - B[0]);
+    }
+  }
+  for (uint32_t k = P.ncols; k < P.ncols + P.nfv; k++) {
+    float phi;
+    float3 wn;
+    d_sdfEval(C, sdf, k, x, phi, wn);
+    dd = fminf(dd, phi);
+  }
+  return dd * P.ih;
+}
+// depth at a point from the cell centres around it, collider cells left out
+// (wgsl-core.js:169-183)
+__device__ __forceinline__ float d_phiSample(const DParams& P, const DGrid& G, float3 x) {
+  float3 g = make_float3((x.x - P.x0) * P.ih - 0.5f, (x.y - P.y0) * P.ih - 0.5f, (x.z - P.z0) * P.ih - 0.5f);
+  float3 gc = make_float3(st_clampf(g.x, 0.0f, float(P.nx) - 1.001f), st_clampf(g.y, 0.0f, float(P.ny) - 1.001f),
+                          st_clampf(g.z, 0.0f, float(P.nz) - 1.001f));
+  float3 f0 = make_float3(floorf(gc.x), floorf(gc.y), floorf(gc.z));
+  float3 fr = f3sub(gc, f0);
+  int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
+  float s = 0.0f, ws = 0.0f;
+  for (uint32_t c = 0; c < 8; c++) {
+    int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+    float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+    uint32_t ct = G.ctype[d_cellIdx(P, G.tmap, i0.x + dx, i0.y + dy, i0.z + dz)];
+    if (((ct >> 8) & 0xffu) != 0u || w <= 0.0f) continue;
+    s = s + w * d_phiOfBits(ct);
+    ws = ws + w;
+  }
+  if (ws < 1.0e-4f) return 0.5f;
+  return s / ws;
+}
+
+// tileMarkPhi (wgsl-narrowband.js:6-10): last step's tiles holding water by depth
+__global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    uint32_t s = G.slotList[t >> 9], local = t & 511u;
+    int3 cc = d_slotCoord(P, G.slotTile, s, local);
+    if (!d_inDom(P, cc) || d_phiOfBits(G.ctype[s * 512u + local]) >= 0.5f) continue;
+    int32_t lx = max(cc.x - 2, 0) >> 3, ly = max(cc.y - 2, 0) >> 3, lz = max(cc.z - 2, 0) >> 3;
+    int32_t hx = min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3, hy = min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3,
+            hz = min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3;
+    for (int32_t tk = lz; tk <= hz; tk++)
+      for (int32_t tj = ly; tj <= hy; tj++)
+        for (int32_t ti = lx; ti <= hx; ti++) G.need[uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk))] = 1u;
+  }
+}
+// tileMarkFill (wgsl-narrowband.js:13-18): on a start, every tile the starting water reaches
+__global__ void k_nb_mark_fill(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.ntiles) return;
+  uint32_t tt = t / P.tx;
+  float3 tc = make_float3(float(t % P.tx), float(tt % P.ty), float(tt / P.ty));
+  tc = f3add(f3mul(tc, 8.0f), make_float3(4.0f, 4.0f, 4.0f));
+  float3 q = f3add(make_float3(P.x0, P.y0, P.z0), f3mul(tc, P.h));
+  if (d_fillPhi(P, C, G.sdf, q) < 9.0f) G.need[t] = 1u;
+}
+// phiInitAll (wgsl-narrowband.js:21-35)
+__global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    int3 p = d_slotCoord(P, G.slotTile, s, local);
+    float ph = st_clampf(d_fillPhi(P, C, G.sdf, d_cellCenter(P, p.x, p.y, p.z)), -7.9f, 7.9f);
+    if (ph < 0.0f) ph = fminf(ph, -1.5f);
+    if (!d_inDom(P, p)) ph = 8.0f;
+    uint32_t ct = G.ctype[c];
+    if (P.nbInit == 2u) ph = fminf(ph, d_phiOfBits(ct));
+    G.ctype[c] = d_phiPutBits(ct, ph);
+    if (P.nbInit == 1u) {
+      G.velG[c * 3] = 0.0f;
+      G.velG[c * 3 + 1] = 0.0f;
+      G.velG[c * 3 + 2] = 0.0f;
+    }
+  }
+}
+// nbPhi (wgsl-narrowband.js:39-49): depth carried by the kept grid velocity
+__global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    int3 p = d_slotCoord(P, G.slotTile, s, local);
+    if (!d_inDom(P, p)) { G.rhs[c] = 8.0f; continue; }
+    if (((G.ctype[c] >> 8) & 0xffu) != 0u) { G.rhs[c] = 0.5f; continue; }
+    float3 x = d_cellCenter(P, p.x, p.y, p.z);
+    float phA = d_phiSample(P, G, f3sub(x, f3mul(d_velGVec(P, G, x), P.dt)));
+    bool inside = G.cnt[c] > 0 || phA < -1.0f;
+    G.rhs[c] = inside ? st_clampf(phA, -7.9f, -0.5f) : st_clampf(phA, 0.5f, 7.9f);
+  }
+}
+// jacNb, nbJacA (fromRhs: phi <- from rhs) and nbJacB (rhs <- from phi) (wgsl-narrowband.js:51-68)
+__global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromRhs) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    int3 p = d_slotCoord(P, G.slotTile, s, local);
+    float cur = fromRhs ? G.rhs[c] : d_phiOfBits(G.ctype[c]);
+    float out;
+    if (!d_inDom(P, p)) out = 8.0f;
+    else if (cur >= 0.0f) out = cur;
+    else {
+      float m = -8.0f;
+      for (uint32_t q = 0; q < 6; q++) {
+        int3 n = p;
+        if (q == 0) n.x -= 1; else if (q == 1) n.x += 1; else if (q == 2) n.y -= 1; else if (q == 3) n.y += 1; else if (q == 4) n.z -= 1; else n.z += 1;
+        if (!d_inDom(P, n)) continue;
+        uint32_t cq = d_cellIdx(P, G.tmap, n.x, n.y, n.z);
+        uint32_t ct = G.ctype[cq];
+        if (((ct >> 8) & 0xffu) != 0u) continue;
+        m = fmaxf(m, fromRhs ? G.rhs[cq] : d_phiOfBits(ct));
+      }
+      out = st_clampf(m - 1.0f, -7.9f, -0.5f);
+    }
+    if (fromRhs) G.ctype[c] = d_phiPutBits(G.ctype[c], out);
+    else G.rhs[c] = out;
+  }
+}
+// nbVel (wgsl-narrowband.js:71-85): water faces no particle reached take the kept
+// grid velocity, carried along
+__global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    int3 p = d_slotCoord(P, G.slotTile, s, local);
+    if (!d_inDom(P, p) || (G.ctype[c] & 0xffu) != 1u) continue;
+    for (uint32_t ax = 0; ax < 3; ax++) {
+      int3 lo = p;
+      if (ax == 0) lo.x -= 1; else if (ax == 1) lo.y -= 1; else lo.z -= 1;
+      if (!d_inDom(P, lo)) continue;
+      if ((G.ctype[d_cellIdx(P, G.tmap, lo.x, lo.y, lo.z)] & 0xffu) != 1u) continue;
+      uint32_t f = c * 3u + ax;
+      if (G.accW[f] > 0) continue;
+      float3 fp = d_facePos(P, p, ax);
+      float vc = d_velGAxis(P, G, f3sub(fp, f3mul(d_velGVec(P, G, fp), P.dt)), ax);
+      G.accV[f] = st_f2i_sat(rintf(vc * P.scale));
+      G.accW[f] = st_f2i_sat(P.scale);
+    }
+  }
+}
+// saveVel (wgsl-narrowband.js:88-91)
+__global__ void k_save_vel(DGrid G) {
+  FOR_ACTIVE(G) {
+    uint32_t c = G.slotList[t >> 9] * 512u + (t & 511u);
+    G.velG[c * 3] = gvR(G, c * 3);
+    G.velG[c * 3 + 1] = gvR(G, c * 3 + 1);
+    G.velG[c * 3 + 2] = gvR(G, c * 3 + 2);
+  }
+}
+// nbKill (wgsl-narrowband.js:94-102): calm particles below the band are removed
+__global__ void k_nb_kill(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 q = G.pos[i];
+  if (q.y < ST_DEADY) return;
+  bool kill = false;
+  float4 v = G.vel[i];
+  if (P.nb != 0u && d_calmV(P, make_float3(v.x, v.y, v.z))) {
+    int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
+    uint32_t sl = G.tmap[d_tileOf(P, cc.x, cc.y, cc.z)];
+    if (sl != ST_NONE && d_phiOfBits(G.ctype[d_cellIdx(P, G.tmap, cc.x, cc.y, cc.z)]) < -P.band) kill = true;
+  }
+  if (kill) {
+    G.pos[i] = make_float4(0.0f, -1.0e6f, 0.0f, 0.0f);
+    G.vel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+// nbDead (wgsl-narrowband.js:107-110) as a flag per particle, compacted in index order
+__global__ void k_nb_dead_flags(const DParams* __restrict__ PP, DGrid G, uint32_t* __restrict__ flag) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= PP->n) return;
+  flag[i] = G.pos[i].y < ST_DEADY ? 1u : 0u;
+}
+__global__ void k_nb_dead_list(const DParams* __restrict__ PP, DGrid G, const uint32_t* __restrict__ flag,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ list) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= PP->n) return;
+  if (flag[i]) list[start[i]] = i;
+}
+// nbDeficit (wgsl-narrowband.js:113-130): per active cell (slotList order) how
+// many particles it is short; a scan of these gives each cell's entry number and
+// the first dead slot it fills, the order the reference's atomics would give
+// when its threads run in cell order
+__global__ void k_nb_deficit(const DParams* __restrict__ PP, DGrid G, uint32_t* __restrict__ needA, uint32_t* __restrict__ entA,
+                             uint32_t ncap) {
+  const DParams& P = *PP;
+  uint32_t tot = G.ctl[4] * 512u;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ncap; t += gridDim.x * blockDim.x) {
+    uint32_t need = 0u;
+    if (t < tot) {
+      uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+      int3 p = d_slotCoord(P, G.slotTile, s, local);
+      uint32_t ct = G.ctype[c];
+      float ph = d_phiOfBits(ct);
+      bool ok = d_inDom(P, p) && ((ct >> 8) & 0xffu) == 0u && (ct & 0xffu) == 1u && !(ph > -1.0f) && !(ph < -P.band);
+      if (ok && d_calmV(P, d_velGVec(P, G, d_cellCenter(P, p.x, p.y, p.z)))) {
+        uint32_t n = uint32_t(max(G.cnt[c], 0));
+        if (!(float(n) >= 0.55f * P.rest)) need = min(st_f2u_sat(rintf(0.85f * P.rest)) - n, 64u);
+      }
+    }
+    needA[t] = need;
+    entA[t] = need ? 1u : 0u;
+  }
+}
+__device__ __forceinline__ float d_hash1(uint32_t n0) {
+  uint32_t x = n0;
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return float(x) * 2.3283064e-10f;
+}
+// nbSpawn (wgsl-narrowband.js:135-149): fill the short cells from the dead slots
+__global__ void k_nb_spawn(const DParams* __restrict__ PP, DGrid G, const uint32_t* __restrict__ needA,
+                           const uint32_t* __restrict__ needStart, const uint32_t* __restrict__ entIdx,
+                           const uint32_t* __restrict__ deadList, const uint32_t* __restrict__ deadCount, uint32_t ncap) {
+  const DParams& P = *PP;
+  uint32_t tot = G.ctl[4] * 512u, ndead = *deadCount, avail = min(ndead, P.lcap), nent = entIdx[ncap];
+  uint32_t maxe = P.lcap / 2u;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+    uint32_t need = needA[t];
+    if (!need) continue;
+    uint32_t e = entIdx[t];
+    if (e >= maxe) continue;
+    uint32_t s = G.slotList[t >> 9], local = t & 511u;
+    int3 p = d_slotCoord(P, G.slotTile, s, local);
+    float3 cp = make_float3(float(uint32_t(p.x) & 0xffffu), float(uint32_t(p.y) & 0xffffu), float(uint32_t(p.z) & 0xffffu));
+    uint32_t base = needStart[t];
+    for (uint32_t m = 0; m < need; m++) {
+      uint32_t k = base + m;
+      if (k >= avail) break;
+      uint32_t i = deadList[k];
+      uint32_t hh = (e * 9781u + m * 6271u + 17u) ^ (P.seed * 2654435761u);
+      float3 x = f3add(make_float3(P.x0, P.y0, P.z0),
+                       f3mul(f3add(cp, make_float3(d_hash1(hh + 1u), d_hash1(hh + 2u), d_hash1(hh + 3u))), P.h));
+      float3 v = d_velGVec(P, G, x);
+      G.pos[i] = make_float4(x.x, x.y, x.z, fminf(f3len(v), 63.0f));
+      G.vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    G.ctl[8] = ndead;
+    G.ctl[9] = nent;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // readback in id order (unsort, wgsl-blocks.js:98-99)
 __global__ void k_unsort(const float4* __restrict__ src, const uint32_t* __restrict__ pid, float4* __restrict__ out, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1295,6 +1556,10 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   launch_fill_u32(st, L.G.need, nt, 0u);
   launch_fill_u32(st, L.G.binCount, size_t(nt) + 1, 0u);
   if (L.n) hipLaunchKernelGGL(k_mark_bin, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
+  // narrow band: last step's tiles holding water by depth, and on a start the
+  // tiles the starting water reaches (both before the counters reset)
+  if (L.nbOn && L.initStep != 1) hipLaunchKernelGGL(k_nb_mark_phi, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  if (L.initStep) hipLaunchKernelGGL(k_nb_mark_fill, dim3(gridFor(nt, WG)), dim3(WG), 0, st, L.P, L.C, L.G);
   uint32_t nwg = gridFor(nt, WG);
   hipLaunchKernelGGL(k_tile_count, dim3(nwg), dim3(WG), 0, st, L.P, L.G);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, L.G, nwg);
@@ -1302,6 +1567,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_tile_alloc, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(nt, WG), 256u))), dim3(WG), 0, st, L.G);
   hipLaunchKernelGGL(k_init_new, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(L.scap) * 27, WG), 2048u))), dim3(WG), 0, st, L.P, L.G);
+  if (L.initStep) hipLaunchKernelGGL(k_nb_phi_init, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   // particle bins: exclusive scan of the counts, then A -> B in bin order
   launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp, L.G.binCursor);
   if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
@@ -1321,6 +1587,29 @@ void launch_classify(hipStream_t st, const LaunchCtx& L, bool solid) {
   if (solid) hipLaunchKernelGGL(k_solid_build, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   hipLaunchKernelGGL(k_classify, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
   hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, st, L.G, L.cellGrid, 1u);
+  if (L.nbOn) hipLaunchKernelGGL(k_nb_vel, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+}
+
+void launch_narrowband(hipStream_t st, const LaunchCtx& L) {
+  hipLaunchKernelGGL(k_nb_phi, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_nb_jac, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, 1u);
+  hipLaunchKernelGGL(k_nb_jac, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, 0u);
+  hipLaunchKernelGGL(k_nb_jac, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, 1u);
+}
+
+void launch_reseed(hipStream_t st, const LaunchCtx& L, const NbScratch& S) {
+  if (!L.n) return;
+  uint32_t gp = gridFor(L.n, WG);
+  hipLaunchKernelGGL(k_nb_kill, dim3(gp), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_nb_dead_flags, dim3(gp), dim3(WG), 0, st, L.P, L.G, S.deadFlag);
+  launch_scan(st, S.deadFlag, S.deadStart, L.n, S.sums, nullptr);
+  hipLaunchKernelGGL(k_nb_dead_list, dim3(gp), dim3(WG), 0, st, L.P, L.G, (const uint32_t*)S.deadFlag, (const uint32_t*)S.deadStart,
+                     S.deadList);
+  hipLaunchKernelGGL(k_nb_deficit, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, S.need, S.ent, S.ncap);
+  launch_scan(st, S.need, S.needStart, S.ncap, S.sums, nullptr);
+  launch_scan(st, S.ent, S.entIdx, S.ncap, S.sums, nullptr);
+  hipLaunchKernelGGL(k_nb_spawn, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, (const uint32_t*)S.need, (const uint32_t*)S.needStart,
+                     (const uint32_t*)S.entIdx, (const uint32_t*)S.deadList, (const uint32_t*)(S.deadStart + L.n), S.ncap);
 }
 
 void launch_norm_faces(hipStream_t st, const LaunchCtx& L) {
@@ -1356,6 +1645,7 @@ void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_project, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  if (L.nbOn) hipLaunchKernelGGL(k_save_vel, dim3(L.cellGrid), dim3(WG), 0, st, L.G);
 }
 
 void launch_g2p(hipStream_t st, const LaunchCtx& L) {

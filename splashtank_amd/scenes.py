@@ -181,3 +181,67 @@ def sparse_splash(d=1.0 / 512, seed=2, pool=0.25, sphere_r=0.12, sphere_y=0.7, s
                path=lambda t, bs=bs: bs + np.array([box_speed * t, 0.0, 0.0]))
     prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0)
     return Scene("sparse_splash", d, (1.0, 1.0, 1.0), 0, pos, vel, prm, colliders=[col])
+
+
+def nb_seed(d, h, size, fills, band=3, rng=None, cap=None):
+    """Narrow-band start (nbSeed, shared.js:805-853) for box and sphere fills:
+    particles only within `band` cells of each shape's free surface (box faces
+    on the tank walls are not free surfaces), then spare parked particles the
+    solver refills the band from.  Returns pos, vel and the packed fill shapes
+    (8 floats each: box 1,x0,y0,z0,x1,y1,z1,0 / sphere 2,x,y,z,r,0,0,0) with
+    wall-touching box faces pushed out to +-1e4 as nbSeed does."""
+    rng = rng or np.random.default_rng(0)
+    W, H, D = size
+    b = dict(x0=-W / 2, x1=W / 2, y0=0.0, y1=H, z0=-D / 2, z1=D / 2)
+    bandw = max(2, band) * h
+    near = 0.6 * h
+    jit = 0.01 * d / 0.12
+    ps, packed = [], []
+    for f in fills:
+        if f["t"] == "box":
+            e = dict(x0=-1e4 if f["x0"] <= b["x0"] + near else f["x0"], x1=1e4 if f["x1"] >= b["x1"] - near else f["x1"],
+                     y0=-1e4 if f["y0"] <= b["y0"] + near else f["y0"], y1=1e4 if f["y1"] >= b["y1"] - near else f["y1"],
+                     z0=-1e4 if f["z0"] <= b["z0"] + near else f["z0"], z1=1e4 if f["z1"] >= b["z1"] - near else f["z1"])
+            ys, xs, zs = _axis(f["y0"], f["y1"], d), _axis(f["x0"], f["x1"], d), _axis(f["z0"], f["z1"], d)
+            Y, X, Z = np.meshgrid(ys, xs, zs, indexing="ij")
+            X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+            dist = np.minimum.reduce([X - e["x0"], e["x1"] - X, Y - e["y0"], e["y1"] - Y, Z - e["z0"], e["z1"] - Z])
+            m = dist <= bandw
+            ps.append(np.stack([X[m], Y[m], Z[m]], 1))
+            packed.append([1, e["x0"], e["y0"], e["z0"], e["x1"], e["y1"], e["z1"], 0])
+        else:
+            r = f["r"]
+            a = _axis_closed(r, d)
+            Y, X, Z = np.meshgrid(a, a, a, indexing="ij")
+            X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+            dist = r - np.sqrt(X * X + Y * Y + Z * Z)
+            m = (dist >= 0) & (dist <= bandw)
+            ps.append(np.stack([f["x"] + X[m], f["y"] + Y[m], f["z"] + Z[m]], 1))
+            packed.append([2, f["x"], f["y"], f["z"], r, 0, 0, 0])
+    P = np.concatenate(ps) if ps else np.zeros((0, 3))
+    n = P.shape[0]
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = P + (rng.random((n, 3)) - 0.5) * jit
+    spare = int(np.ceil(0.35 * n)) + 4000
+    if cap is not None:
+        spare = max(0, min(spare, cap - n))
+    park = np.zeros((spare, 4), np.float32)
+    park[:, 1] = -1e6
+    pos = np.concatenate([pos, park])
+    vel = np.zeros_like(pos)
+    return pos, vel, np.asarray(packed, np.float32).reshape(-1)
+
+
+def nb_pool(d=1.0 / 64, depth=0.5, band=3, seed=4, iters=30, dt=1.0 / 240, drop=None):
+    """A deep pool in narrow-band mode (only the band under the surface has
+    particles), optionally with a ball of water dropped into it."""
+    h = 2 * d
+    fills = [dict(t="box", x0=-0.5, x1=0.5, y0=0.0, y1=depth, z0=-0.5, z1=0.5)]
+    if drop:
+        fills.append(dict(t="sphere", x=drop[0], y=drop[1], z=drop[2], r=drop[3]))
+    pos, vel, packed = nb_seed(d, h, (1.0, 1.0, 1.0), fills, band=band, rng=np.random.default_rng(seed))
+    prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0,
+               nb_on=1, nb_band=float(band))
+    sc = Scene("nb_pool", d, (1.0, 1.0, 1.0), 0, pos, vel, prm)
+    sc.nb_fills = packed
+    return sc
