@@ -417,6 +417,16 @@ __device__ __forceinline__ uint32_t wcell(const uint32_t* wslots, int32_t wx, in
   int32_t gx = org.x + wx, gy = org.y + wy, gz = org.z + wz;
   return s * 512u + uint32_t(gx & 7) + 8u * (uint32_t(gy & 7) + 8u * uint32_t(gz & 7));
 }
+// window offset, clamped into the window: a guard for LDS only -- every face or
+// cell a particle of the tile touches already lies inside (faceIndex's clamp moves
+// an index toward the particle's own clamped cell, which is in the tile)
+__device__ __forceinline__ int32_t wclamp(int32_t w) { return min(max(w, 0), WIN - 1); }
+// the two clamped window offsets along one axis of a trilinear stencil at i0
+// (faceIndex, wgsl-core.js:84-89): own axis clamps to [0, n], others to [0, n-1]
+__device__ __forceinline__ void wpair(int32_t i0, int32_t hi, int32_t org, int32_t* w) {
+  w[0] = wclamp(d_iclamp(i0, 0, hi) - org);
+  w[1] = wclamp(d_iclamp(i0 + 1, 0, hi) - org);
+}
 // face (axis, i, j, k) clamped as faceIndex does (wgsl-core.js:84-89), as a window index or -1
 __device__ __forceinline__ int32_t wface(const DParams& P, uint32_t axis, int32_t i, int32_t j, int32_t k, int3 org) {
   int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
@@ -447,9 +457,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) sC[q] = 0;
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
-    // a tile whose window lies inside [0, n-1] on every axis never clamps a face
-    const bool inner = org.x >= 0 && org.y >= 0 && org.z >= 0 && org.x + WIN - 1 <= int32_t(P.nx) - 1 &&
-                       org.y + WIN - 1 <= int32_t(P.ny) - 1 && org.z + WIN - 1 <= int32_t(P.nz) - 1;
+    const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
     for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
       float4 p4 = G.posB[k];
       float4 v4 = G.velB[k];
@@ -466,23 +474,19 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
         float3 fr = f3sub(q, f0);
         int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
         float vc = f3get(v, axis);
-        int32_t bx = i0.x - org.x, by = i0.y - org.y, bz = i0.z - org.z;
-        bool fast = inner && uint32_t(bx) <= uint32_t(WIN - 2) && uint32_t(by) <= uint32_t(WIN - 2) && uint32_t(bz) <= uint32_t(WIN - 2);
-        int32_t base = bx + WIN * (by + WIN * bz);
+        int32_t wx[2], wy[2], wz[2];
+        wpair(i0.x, axis == 0 ? X : X - 1, org.x, wx);
+        wpair(i0.y, axis == 1 ? Y : Y - 1, org.y, wy);
+        wpair(i0.z, axis == 2 ? Z : Z - 1, org.z, wz);
+        // every corner is added: a zero weight adds round(0) = 0 to an integer sum,
+        // exactly as skipping it (wgsl-solver.js:33)
         for (uint32_t c = 0; c < 8; c++) {
           int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
           float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-          if (w <= 0.0f) continue;
           int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
           int32_t aw = st_f2i_sat(rintf(w * P.scale));
-          int32_t wi = fast ? base + dx + WIN * dy + WIN * WIN * dz : wface(P, axis, i0.x + dx, i0.y + dy, i0.z + dz, org);
-          if (wi >= 0) {
-            atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
-          } else {  // outside the window: only for particles far outside the box
-            uint32_t idx = d_faceIndex(P, G.tmap, axis, i0.x + dx, i0.y + dy, i0.z + dz);
-            atomicAdd(&G.accV[idx], av);
-            atomicAdd(&G.accW[idx], aw);
-          }
+          int32_t wi = wx[dx] + WIN * (wy[dy] + WIN * wz[dz]);
+          atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
         }
       }
       atomicAdd(&sC[uint32_t(cc.x & 7) + 8u * (uint32_t(cc.y & 7) + 8u * uint32_t(cc.z & 7))], 1);
@@ -707,13 +711,17 @@ __device__ __forceinline__ void relax_lds(float* sp, int32_t p, uint32_t st, flo
     sp[p] = 0.0f;
     return;
   }
+  // all six neighbours are read, a non-fluid one contributes +0.0: the
+  // reference's partial sums start at +0 and are never -0, and x + 0 = x for
+  // every other x, so the sum is the reference's to the bit
+  float n0 = sp[p - 1], n1 = sp[p + 1], n2 = sp[p - W], n3 = sp[p + W], n4 = sp[p - W * W], n5 = sp[p + W * W];
   float sum = 0.0f;
-  if (mask & 1u) sum = sum + sp[p - 1];
-  if (mask & 2u) sum = sum + sp[p + 1];
-  if (mask & 4u) sum = sum + sp[p - W];
-  if (mask & 8u) sum = sum + sp[p + W];
-  if (mask & 16u) sum = sum + sp[p - W * W];
-  if (mask & 32u) sum = sum + sp[p + W * W];
+  sum = sum + ((mask & 1u) ? n0 : 0.0f);
+  sum = sum + ((mask & 2u) ? n1 : 0.0f);
+  sum = sum + ((mask & 4u) ? n2 : 0.0f);
+  sum = sum + ((mask & 8u) ? n3 : 0.0f);
+  sum = sum + ((mask & 16u) ? n4 : 0.0f);
+  sum = sum + ((mask & 32u) ? n5 : 0.0f);
   float pg = (rhs + sum) / float(nns);
   float pc = sp[p];
   sp[p] = pc + omega * (pg - pc);
@@ -1027,26 +1035,17 @@ struct WindowGrid {
   const float (*sO)[WIN3];
   const int32_t* sC;
   int3 org;
-  bool inner;  // window inside [0, n-1] on every axis: no face index is ever clamped
   __device__ void sample(float3 p, float3& vn, float3& vo) const { d_sampleBothWin(P, *this, p, vn, vo); }
-  __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
-    int32_t w = wface(P, axis, i, j, k, org);
-    return w >= 0 ? sN[axis][w] : gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k));
-  }
-  __device__ float fold(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
-    int32_t w = wface(P, axis, i, j, k, org);
-    return w >= 0 ? sO[axis][w] : gv0R(G, d_faceIndex(P, G.tmap, axis, i, j, k));
-  }
+  // cells of the density sum come clamped to the box, so they lie in the window
   __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const {
-    int32_t wx = i - org.x, wy = j - org.y, wz = k - org.z;
-    if (uint32_t(wx) < uint32_t(WIN) && uint32_t(wy) < uint32_t(WIN) && uint32_t(wz) < uint32_t(WIN)) return sC[wx + WIN * (wy + WIN * wz)];
-    return G.cnt[d_cellIdx(P, G.tmap, i, j, k)];
+    return sC[wclamp(i - org.x) + WIN * (wclamp(j - org.y) + WIN * wclamp(k - org.z))];
   }
 };
 
-// sampleBoth (wgsl-core.js:138-154) from a staged window: one base index per
-// axis when no clamp can apply, the clamped general path otherwise
+// sampleBoth (wgsl-core.js:138-154) from a staged window: the two clamped window
+// offsets per axis give the eight face indices of each trilinear stencil
 __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGrid& g, float3 p, float3& vn, float3& vo) {
+  const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
   float rn[3], ro[3];
   for (uint32_t axis = 0; axis < 3; axis++) {
     float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
@@ -1056,24 +1055,17 @@ __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGr
     float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
     float3 fr = f3sub(q, f0);
     int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
-    int32_t bx = i0.x - g.org.x, by = i0.y - g.org.y, bz = i0.z - g.org.z;
-    bool fast = g.inner && uint32_t(bx) <= uint32_t(WIN - 2) && uint32_t(by) <= uint32_t(WIN - 2) && uint32_t(bz) <= uint32_t(WIN - 2);
-    int32_t base = bx + WIN * (by + WIN * bz);
+    int32_t wx[2], wy[2], wz[2];
+    wpair(i0.x, axis == 0 ? X : X - 1, g.org.x, wx);
+    wpair(i0.y, axis == 1 ? Y : Y - 1, g.org.y, wy);
+    wpair(i0.z, axis == 2 ? Z : Z - 1, g.org.z, wz);
     float sn = 0.0f, so = 0.0f;
     for (uint32_t c = 0; c < 8; c++) {
       int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
       float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-      float fn, fo;
-      if (fast) {
-        int32_t wi = base + dx + WIN * dy + WIN * WIN * dz;
-        fn = g.sN[axis][wi];
-        fo = g.sO[axis][wi];
-      } else {
-        fn = g.fnew(axis, i0.x + dx, i0.y + dy, i0.z + dz);
-        fo = g.fold(axis, i0.x + dx, i0.y + dy, i0.z + dz);
-      }
-      sn = sn + w * fn;
-      so = so + w * fo;
+      int32_t wi = wx[dx] + WIN * (wy[dy] + WIN * wz[dz]);
+      sn = sn + w * g.sN[axis][wi];
+      so = so + w * g.sO[axis][wi];
     }
     rn[axis] = sn;
     ro[axis] = so;
@@ -1229,9 +1221,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       sC[q] = G.cnt[c];
     }
     __syncthreads();
-    const bool inner = org.x >= 0 && org.y >= 0 && org.z >= 0 && org.x + WIN - 1 <= int32_t(P.nx) - 1 &&
-                       org.y + WIN - 1 <= int32_t(P.ny) - 1 && org.z + WIN - 1 <= int32_t(P.nz) - 1;
-    WindowGrid wg{P, G, sN, sO, sC, org, inner};
+    WindowGrid wg{P, G, sN, sO, sC, org};
     for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
       float4 po, vo;
       g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
