@@ -85,6 +85,24 @@ def run_gpu(sc, args):
     return wall, alive, stats, stage_ms
 
 
+def calib_copy(mb: int):
+    """A plain device-to-device copy of `mb` MiB (HIP's blit kernel), so a counter
+    pass can scale FETCH_SIZE / WRITE_SIZE by what they read for known traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mb << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    for _ in range(3):
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] calibration copy of {mb} MiB x3 done")
+
+
 def cpu_baseline(sc, args):
     from splashtank_amd import capi
     from oracle.oracle import Oracle, threads
@@ -118,11 +136,14 @@ def main():
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=3)
+    ap.add_argument("--calib-copy-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the substep runs on a single MI355X; "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy_mb:
+        calib_copy(args.calib_copy_mb)
     sc = make_scene(args)
     wall, alive, stats, stage_ms = run_gpu(sc, args)
     value = alive * args.steps / wall
