@@ -96,7 +96,7 @@ struct Sim {
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     L.sorGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 65535u));
-    L.pipeGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 2048u));
+    L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, 2048u);  // whole groups of 8 (one per XCD)
     L.nbOn = nbOn;
     L.initStep = initStep;
     return L;

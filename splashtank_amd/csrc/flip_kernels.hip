@@ -904,10 +904,17 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
   __shared__ uint32_t ss[WIN3];
   __shared__ uint32_t wsl[2][27];
   const float omega = PP->omega;
-  const uint32_t sc = G.ctl[4], gs = gridDim.x, tid = threadIdx.x;
-  uint32_t b = blockIdx.x;
+  const uint32_t tid = threadIdx.x;
+  // Workgroup w runs on XCD w % 8.  Each XCD walks one contiguous eighth of the
+  // tile-ordered active list, so x and y neighbours of a tile are processed by
+  // the same XCD at about the same time and their halo rows hit its L2.
+  // The grid is a multiple of 8 workgroups (host side).
+  const uint32_t nact = G.ctl[4], gs = gridDim.x >> 3, xcd = blockIdx.x & 7u;
+  const uint32_t per = (nact + 7u) >> 3;
+  const uint32_t sc = min(nact, (xcd + 1u) * per);
+  uint32_t b = xcd * per + (blockIdx.x >> 3);
   if (b >= sc) return;
-  // prologue: slots of tiles b and b + G, window rows of tile b
+  // prologue: slots of tiles b and b + gs, window rows of tile b
   if (tid < 27u) {
     wsl[0][tid] = G.nbrL[b * 27u + tid];
     wsl[1][tid] = b + gs < sc ? G.nbrL[(b + gs) * 27u + tid] : 0u;
@@ -945,7 +952,7 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
       uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
       pout[s * 512u + q] = sp[(x + 2) + 12 * ((y + 2) + 12 * (z + 2))];
     }
-    // slots of tile b + 2G go where tile b's were (read above into s)
+    // slots of tile b + 2gs go where tile b's were (read above into s)
     if (tid < 27u) wsl[cur][tid] = ahead;
     __syncthreads();
   }
