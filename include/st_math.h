@@ -74,4 +74,21 @@ ST_HD uint32_t st_f2u_sat(float x) {
   return (uint32_t)x;
 }
 
+/* a / n for n in 1..6, equal to the correctly rounded f32 division, given
+ * y = 1.0f / n (itself correctly rounded).  q = a*y is within an ulp of a/n; the
+ * residual r = a - q*n is exact in one fused multiply-add, and q + r*y rounds to
+ * the quotient (Markstein's correction).  An exhaustive run over all 2^32 values
+ * of a for each n finds this equal to the division for 2^-124 <= |a| < inf; it
+ * differs only below 2^-125 for n = 6 (gradual underflow), at +-0 and at +-inf,
+ * so |a| outside that range (and NaN) takes the division itself. */
+ST_HD float st_div_small(float a, float n, float y) {
+  uint32_t ua = st_f2u(a) & 0x7fffffffu;
+  if (ua - 0x01800000u < 0x7e000000u) {
+    float q = a * y;
+    float r = fmaf(-q, n, a);
+    return fmaf(r, y, q);
+  }
+  return a / n;
+}
+
 #endif /* ST_MATH_H */

@@ -83,6 +83,7 @@ struct Sim {
   uint32_t initStep = 0;
   int iters = 1;
   uint32_t lcap = 0;
+  uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
   NbScratch nb{};
 
   LaunchCtx ctx() const {
@@ -95,8 +96,7 @@ struct Sim {
     L.scap = scap;
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
-    L.sorGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 65535u));
-    L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, 2048u);  // whole groups of 8 (one per XCD)
+    L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, sorResident);  // whole groups of 8 (one per XCD)
     L.nbOn = nbOn;
     L.initStep = initStep;
     return L;
@@ -326,6 +326,11 @@ int st_create(const st_opts* o, void** out) {
   Sim* s = new Sim();
   for (int k = 0; k < RING; k++) s->evUsed[k] = false;
   s->dev = o->device;
+  {
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o->device));
+    s->sorResident = sor_resident_grid(uint32_t(cus));
+  }
   double rt = o->ratio > 0 ? o->ratio : 2.0;
   rt = std::max(1.4, std::min(4.0, rt));
   s->ratio = float(rt);

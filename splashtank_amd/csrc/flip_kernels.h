@@ -15,7 +15,6 @@ struct LaunchCtx {
   uint32_t scap;
   uint32_t cellGrid;  // workgroups of grid-stride loops over active cells
   uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
-  uint32_t sorGrid;   // workgroups of the SOR launches (one tile each when it fits)
   uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
   bool nbOn;          // narrow band on this step
   uint32_t initStep;  // narrow-band start: 1 restart, 2 water added, 0 none
@@ -45,6 +44,9 @@ void launch_reseed(hipStream_t st, const LaunchCtx& L, const NbScratch& S);
 void launch_norm_faces(hipStream_t st, const LaunchCtx& L);
 void launch_divergence(hipStream_t st, const LaunchCtx& L);
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters);
+// workgroups of the persistent SOR launch that fit on `cus` CUs at once, a
+// multiple of 8 (the XCD count)
+uint32_t sor_resident_grid(uint32_t cus);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);

@@ -60,38 +60,39 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
   // tiles [cc-2, cc+2] >> 3 relative to the particle's own tile, as bits
   // (dx+1) + 3 (dy+1) + 9 (dz+1); the span is at most two tiles per axis
   uint32_t nmask = 0;
+  int32_t ox = 0, oy = 0, oz = 0;
   if (live) {
     int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
     key = d_tileOf(P, cc.x, cc.y, cc.z);
-    int32_t ox = cc.x >> 3, oy = cc.y >> 3, oz = cc.z >> 3;
-    int32_t lx = (max(cc.x - 2, 0) >> 3) - ox, ly = (max(cc.y - 2, 0) >> 3) - oy, lz = (max(cc.z - 2, 0) >> 3) - oz;
-    int32_t hx = (min(cc.x + 2, int32_t(P.tx) * 8 - 1) >> 3) - ox, hy = (min(cc.y + 2, int32_t(P.ty) * 8 - 1) >> 3) - oy,
-            hz = (min(cc.z + 2, int32_t(P.tz) * 8 - 1) >> 3) - oz;
-    for (int32_t dz = lz; dz <= hz; dz++)
-      for (int32_t dy = ly; dy <= hy; dy++)
-        for (int32_t dx = lx; dx <= hx; dx++) nmask |= 1u << uint32_t((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+    ox = cc.x >> 3; oy = cc.y >> 3; oz = cc.z >> 3;
+    // per axis, bits of the offsets -1, 0, +1 that the span covers
+    uint32_t bx = 2u | (cc.x - 2 >= 0 && ((cc.x - 2) >> 3) < ox ? 1u : 0u) |
+                  (cc.x + 2 <= int32_t(P.tx) * 8 - 1 && ((cc.x + 2) >> 3) > ox ? 4u : 0u);
+    uint32_t by = 2u | (cc.y - 2 >= 0 && ((cc.y - 2) >> 3) < oy ? 1u : 0u) |
+                  (cc.y + 2 <= int32_t(P.ty) * 8 - 1 && ((cc.y + 2) >> 3) > oy ? 4u : 0u);
+    uint32_t bz = 2u | (cc.z - 2 >= 0 && ((cc.z - 2) >> 3) < oz ? 1u : 0u) |
+                  (cc.z + 2 <= int32_t(P.tz) * 8 - 1 && ((cc.z + 2) >> 3) > oz ? 4u : 0u);
+    uint32_t xy = ((by & 1u) ? bx : 0u) | ((by & 2u) ? bx << 3 : 0u) | ((by & 4u) ? bx << 6 : 0u);
+    nmask = ((bz & 1u) ? xy : 0u) | ((bz & 2u) ? xy << 9 : 0u) | ((bz & 4u) ? xy << 18 : 0u);
   }
   // runs of equal bins in the wave: one bin count atomic per run (its result is
-  // not waited for), and the run's leader marks the union of its particles' tiles
+  // not waited for); lane l < 27 marks neighbour l of the run's tile if any
+  // particle of the run needs it
+  const uint32_t ln = lane_id();
+  const int32_t mdx = int32_t(ln % 3u) - 1, mdy = int32_t((ln / 3u) % 3u) - 1, mdz = int32_t(ln / 9u) - 1;
   uint64_t rem = __ballot(valid);
   while (rem) {
     int leader = __ffsll((unsigned long long)rem) - 1;
     uint32_t k = (uint32_t)__shfl((int)key, leader);
     uint64_t m = __ballot(valid && key == k) & rem;
-    bool inrun = (m >> lane_id()) & 1ull;
+    bool inrun = (m >> ln) & 1ull;
     uint32_t um = inrun ? nmask : 0u;
     for (int off = 32; off > 0; off >>= 1) um |= (uint32_t)__shfl_xor((int)um, off);
-    if (int(lane_id()) == leader) {
-      atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
-      if (k < P.ntiles) {
-        int32_t ti = int32_t(k % P.tx), tj = int32_t((k / P.tx) % P.ty), tk = int32_t(k / (P.tx * P.ty));
-        while (um) {
-          uint32_t bit = uint32_t(__ffs(um) - 1);
-          um &= um - 1u;
-          int32_t x = ti + int32_t(bit % 3u) - 1, y = tj + int32_t((bit / 3u) % 3u) - 1, z = tk + int32_t(bit / 9u) - 1;
-          G.need[uint32_t(x) + P.tx * (uint32_t(y) + P.ty * uint32_t(z))] = 1u;
-        }
-      }
+    if (int(ln) == leader) atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+    if (k < P.ntiles) {
+      int32_t tx0 = __shfl(ox, leader), ty0 = __shfl(oy, leader), tz0 = __shfl(oz, leader);
+      if (ln < 27u && ((um >> ln) & 1u))
+        G.need[uint32_t(tx0 + mdx) + P.tx * (uint32_t(ty0 + mdy) + P.ty * uint32_t(tz0 + mdz))] = 1u;
     }
     rem &= ~m;
   }
@@ -173,10 +174,14 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uin
 __global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = i < PP->n;
-  uint32_t key = valid ? G.binKey[i] : 0u;
-  float4 p = valid ? G.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 v = valid ? G.vel[i] : p;
-  uint32_t id = valid ? G.pid[i] : 0u;
+  uint32_t key = 0u, id = 0u;
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p;
+  if (valid) {
+    key = G.binKey[i];
+    p = G.pos[i];
+    v = G.vel[i];
+    id = G.pid[i];
+  }
   uint64_t rem = __ballot(valid);
   uint32_t dst = 0;
   while (rem) {
@@ -693,207 +698,130 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
 }
 
 // relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
-// sim.js:291-293), K full iterations (red sweep then black sweep, K times) per
-// launch.  Each workgroup owns one tile.  It stages the pressure of a window of
-// W = 8 + 4K cells per axis (2K cells of halo) and the rhs and stencil of the
-// window less one cell each side in LDS, then runs 2K sweeps in place, sweep s
-// relaxing the cells of its colour in the window shrunk by s on every side
-// (odd s red, even s black), and writes the tile to the other pressure buffer.
-// A red cell reads only black neighbours and a black cell only red ones, so each
-// sweep needs exactly the previous sweep's values one cell further out; the
-// halo values equal those the owning tiles compute, and the result is the
-// reference's red-black iterate bit for bit.  Neighbours are summed in the
-// reference's order (x-, x+, y-, y+, z-, z+).
+// sim.js:291-293): one red-black iteration per launch, by tiles.  A workgroup
+// stages the pressure of a 12^3 window (the tile and two cells of halo) and the
+// rhs and stencil of the 10^3 window in LDS, relaxes the red cells of the 10^3
+// window, then the black cells of the tile, and writes the tile to the other
+// pressure buffer.  A red cell reads only black neighbours and a black cell only
+// red ones, so the black sweep sees exactly the red values the neighbouring
+// tiles compute, and the result is the reference's red-black iterate bit for bit.
+//
+// Every non-fluid cell of the window holds +0.0 (classify zeroes the pressure
+// of every non-fluid cell, and missing tiles load as zero), so all six
+// neighbours are summed in the reference's order (x-, x+, y-, y+, z-, z+): a
+// non-fluid neighbour adds +0.0, the reference's partial sums start at +0.0 and
+// are never -0.0, and x + 0 = x for every other x.  The quotient by the count
+// of non-solid neighbours is st_div_small, equal to the division.
 template <int W>
-__device__ __forceinline__ void relax_lds(float* sp, int32_t p, uint32_t st, float rhs, float omega) {
-  uint32_t nns = st & 7u, mask = (st >> 4) & 63u;
+__device__ __forceinline__ void relax_lds(float* sp, int32_t p, uint32_t st, float rhs, float omega, const float* ytab) {
+  uint32_t nns = st & 7u;
   if (nns == 0u) {
     sp[p] = 0.0f;
     return;
   }
-  // all six neighbours are read, a non-fluid one contributes +0.0: the
-  // reference's partial sums start at +0 and are never -0, and x + 0 = x for
-  // every other x, so the sum is the reference's to the bit
   float n0 = sp[p - 1], n1 = sp[p + 1], n2 = sp[p - W], n3 = sp[p + W], n4 = sp[p - W * W], n5 = sp[p + W * W];
+  float y = ytab[nns], pc = sp[p];
   float sum = 0.0f;
-  sum = sum + ((mask & 1u) ? n0 : 0.0f);
-  sum = sum + ((mask & 2u) ? n1 : 0.0f);
-  sum = sum + ((mask & 4u) ? n2 : 0.0f);
-  sum = sum + ((mask & 8u) ? n3 : 0.0f);
-  sum = sum + ((mask & 16u) ? n4 : 0.0f);
-  sum = sum + ((mask & 32u) ? n5 : 0.0f);
-  float pg = (rhs + sum) / float(nns);
-  float pc = sp[p];
+  sum = sum + n0;
+  sum = sum + n1;
+  sum = sum + n2;
+  sum = sum + n3;
+  sum = sum + n4;
+  sum = sum + n5;
+  float pg = st_div_small(rhs + sum, float(nns), y);
   sp[p] = pc + omega * (pg - pc);
 }
 
-template <int K>
-__global__ void __launch_bounds__(WG) k_sor_iters(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
-                                                  float* __restrict__ pout) {
-  constexpr int W = 8 + 4 * K;   // pressure window edge
-  constexpr int R = W - 2;       // rhs / stencil window edge
-  const DParams& P = *PP;
-  __shared__ float sp[W * W * W];
-  __shared__ float sr[R * R * R];
-  __shared__ uint32_t ss[R * R * R];
-  __shared__ uint32_t wslots[27];
-  const float omega = P.omega;
-  uint32_t sc = G.ctl[4];
-  for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
-    uint32_t s = G.slotList[b];
-    if (threadIdx.x < 27) wslots[threadIdx.x] = G.nbr[s * 27u + threadIdx.x];
-    __syncthreads();
-    // Window loads by rows: a window row (fixed y, z) is the last 2K cells of
-    // a row of the tile on the left, a whole 8-cell row of the tile itself and
-    // the first 2K cells of a row of the tile on the right; tile storage is
-    // x-fastest, so each part is one or two aligned vector loads.
-    for (uint32_t r = threadIdx.x; r < uint32_t(W * W); r += WG) {
-      int32_t ly = int32_t(r % W) - 2 * K, lz = int32_t(r / W) - 2 * K;
-      uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
-      int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
-      uint32_t sl = wslots[so], sc0 = wslots[so + 1], sr = wslots[so + 2];
-      float* dst = sp + W * int32_t(r);
-      if (K == 1) {
-        float2 l = sl == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sl * 512u + rowoff + 6u);
-        float2 h = sr == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sr * 512u + rowoff);
-        dst[0] = l.x; dst[1] = l.y; dst[10] = h.x; dst[11] = h.y;
-      } else {
-        float4 l = sl == WSLOT_OUT ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4*)(pin + sl * 512u + rowoff + 4u);
-        float4 h = sr == WSLOT_OUT ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4*)(pin + sr * 512u + rowoff);
-        dst[0] = l.x; dst[1] = l.y; dst[2] = l.z; dst[3] = l.w;
-        dst[12] = h.x; dst[13] = h.y; dst[14] = h.z; dst[15] = h.w;
-      }
-      float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
-      if (sc0 != WSLOT_OUT) {
-        c0 = *(const float4*)(pin + sc0 * 512u + rowoff);
-        c1 = *(const float4*)(pin + sc0 * 512u + rowoff + 4u);
-      }
-      dst[2 * K + 0] = c0.x; dst[2 * K + 1] = c0.y; dst[2 * K + 2] = c0.z; dst[2 * K + 3] = c0.w;
-      dst[2 * K + 4] = c1.x; dst[2 * K + 5] = c1.y; dst[2 * K + 6] = c1.z; dst[2 * K + 7] = c1.w;
-    }
-    // rhs and stencil rows of the window less one cell: 2K-1 cells left, 8, 2K-1 right
-    for (uint32_t r = threadIdx.x; r < uint32_t(R * R); r += WG) {
-      int32_t ly = int32_t(r % R) + 1 - 2 * K, lz = int32_t(r / R) + 1 - 2 * K;
-      uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
-      int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
-      uint32_t sl = wslots[so], sc0 = wslots[so + 1], sh = wslots[so + 2];
-      float* dr = sr + R * int32_t(r);
-      uint32_t* ds = ss + R * int32_t(r);
-      for (int j = 0; j < 2 * K - 1; j++) {
-        uint32_t cl = sl * 512u + rowoff + uint32_t(9 - 2 * K + j), ch = sh * 512u + rowoff + uint32_t(j);
-        dr[j] = sl == WSLOT_OUT ? 0.0f : G.rhs[cl];
-        ds[j] = sl == WSLOT_OUT ? 0u : G.sten[cl];
-        dr[2 * K + 7 + j] = sh == WSLOT_OUT ? 0.0f : G.rhs[ch];
-        ds[2 * K + 7 + j] = sh == WSLOT_OUT ? 0u : G.sten[ch];
-      }
-      float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
-      uint4 t0 = make_uint4(0u, 0u, 0u, 0u), t1 = t0;
-      if (sc0 != WSLOT_OUT) {
-        r0 = *(const float4*)(G.rhs + sc0 * 512u + rowoff);
-        r1 = *(const float4*)(G.rhs + sc0 * 512u + rowoff + 4u);
-        t0 = *(const uint4*)(G.sten + sc0 * 512u + rowoff);
-        t1 = *(const uint4*)(G.sten + sc0 * 512u + rowoff + 4u);
-      }
-      const int o = 2 * K - 1;
-      dr[o + 0] = r0.x; dr[o + 1] = r0.y; dr[o + 2] = r0.z; dr[o + 3] = r0.w;
-      dr[o + 4] = r1.x; dr[o + 5] = r1.y; dr[o + 6] = r1.z; dr[o + 7] = r1.w;
-      ds[o + 0] = t0.x; ds[o + 1] = t0.y; ds[o + 2] = t0.z; ds[o + 3] = t0.w;
-      ds[o + 4] = t1.x; ds[o + 5] = t1.y; ds[o + 6] = t1.z; ds[o + 7] = t1.w;
-    }
-    __syncthreads();
-    // sweep s: window cells [s, W-1-s]^3 of colour (s odd: red, global x+y+z even;
-    // the window origin 8t - 2K is even, so window parity is global parity)
-#pragma unroll
-    for (int sw = 1; sw <= 2 * K; sw++) {
-      const int L = W - 2 * sw, H = L / 2;
-      const int col = (sw & 1) ? 0 : 1;
-      for (int m = threadIdx.x; m < L * L * H; m += WG) {
-        int row = m / H;
-        int wy = sw + row % L, wz = sw + row / L;
-        int wx = sw + 2 * (m % H) + ((col + wy + wz + sw) & 1);
-        int qr = (wx - 1) + R * ((wy - 1) + R * (wz - 1));
-        uint32_t st = ss[qr];
-        if (st & STEN_FLUID) relax_lds<W>(sp, wx + W * (wy + W * wz), st, sr[qr], omega);
-      }
-      __syncthreads();
-    }
-    for (uint32_t q = threadIdx.x; q < 512u; q += WG) {
-      int32_t x = int32_t(q & 7u), y = int32_t((q >> 3) & 7u), z = int32_t(q >> 6);
-      pout[s * 512u + q] = sp[(x + 2 * K) + W * ((y + 2 * K) + W * (z + 2 * K))];
-    }
-    __syncthreads();
-  }
-}
+// a window slot that holds no data: outside the box, or a tile without storage
+// (the spare slot 0 stands in for it)
+__device__ __forceinline__ bool sor_missing(uint32_t slot) { return slot + 2u <= 2u; }
 
-// One red-black iteration (as k_sor_iters<1>) by persistent workgroups, about
-// eight per CU, each walking tiles b, b + G, b + 2G, ... of slotList.  While a
-// workgroup relaxes tile b in LDS, the window rows of tile b + G are already in
+// The launch is persistent: about eight workgroups per CU, each walking tiles
+// b, b + G, b + 2G, ... of its share of slotList.  While a workgroup relaxes
+// tile b in LDS, the window rows of tile b + G are already in
 // flight into registers (thread r < 144 holds pressure row r of the 12^3
 // window; thread 144 + r < 244 holds rhs and stencil row r of the 10^3
 // window), and the 27 slots of tile b + 2G are being fetched.  Tiles are
 // independent within the launch (they read pin, write pout), so the pipeline
 // only reorders loads.
+// A thread holds either a pressure row (threads 0..143) or an rhs and stencil
+// row (threads 144..243) in the same 20 registers:
+//   pressure: q0, q1 the 8 cells of the row's own tile, q2 = (two cells to the
+//             left, two to the right);
+//   rhs/sten: q0, q1 rhs of the own tile, q2, q3 its stencil words,
+//             q4 = (rhs left, rhs right, stencil left, stencil right).
 struct SorRows {
-  float4 a, b, c;   // pressure: l.xy in a.xy, centre in b, c (x..w), h.xy in a.zw
-  uint4 t0, t1;     // rhs/stencil: centre stencil
-  float4 r0, r1;    // centre rhs
-  float rl, rh;     // rhs of the left / right cell
-  uint32_t tl, th;  // stencil of the left / right cell
+  uint4 q0, q1, q2, q3, q4;
 };
 
-__device__ __forceinline__ void sor_fetch(const DGrid& G, const float* __restrict__ pin, const uint32_t* ws, uint32_t tid, SorRows& R) {
+__device__ __forceinline__ SorRows sor_fetch(const DGrid& G, const float* __restrict__ pin, const uint32_t* ws, uint32_t tid) {
+  const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+  SorRows R{z4, z4, z4, z4, z4};
   if (tid < 144u) {
     int32_t ly = int32_t(tid % 12u) - 2, lz = int32_t(tid / 12u) - 2;
     uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
     int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
     uint32_t sl = ws[so], sc = ws[so + 1], sh = ws[so + 2];
-    float2 l = sl == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sl * 512u + rowoff + 6u);
-    float2 h = sh == WSLOT_OUT ? make_float2(0.f, 0.f) : *(const float2*)(pin + sh * 512u + rowoff);
-    R.a = make_float4(l.x, l.y, h.x, h.y);
-    if (sc != WSLOT_OUT) {
-      R.b = *(const float4*)(pin + sc * 512u + rowoff);
-      R.c = *(const float4*)(pin + sc * 512u + rowoff + 4u);
-    } else {
-      R.b = make_float4(0.f, 0.f, 0.f, 0.f);
-      R.c = R.b;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(pin);
+    uint2 l = sor_missing(sl) ? make_uint2(0u, 0u) : *(const uint2*)(src + sl * 512u + rowoff + 6u);
+    uint2 h = sor_missing(sh) ? make_uint2(0u, 0u) : *(const uint2*)(src + sh * 512u + rowoff);
+    uint4 c0 = z4, c1 = z4;
+    if (!sor_missing(sc)) {
+      c0 = *(const uint4*)(src + sc * 512u + rowoff);
+      c1 = *(const uint4*)(src + sc * 512u + rowoff + 4u);
     }
+    R.q0 = c0;
+    R.q1 = c1;
+    R.q2 = make_uint4(l.x, l.y, h.x, h.y);
   } else if (tid < 244u) {
     uint32_t r = tid - 144u;
     int32_t ly = int32_t(r % 10u) - 1, lz = int32_t(r / 10u) - 1;
     uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
     int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
     uint32_t sl = ws[so], sc = ws[so + 1], sh = ws[so + 2];
-    R.rl = sl == WSLOT_OUT ? 0.0f : G.rhs[sl * 512u + rowoff + 7u];
-    R.tl = sl == WSLOT_OUT ? 0u : G.sten[sl * 512u + rowoff + 7u];
-    R.rh = sh == WSLOT_OUT ? 0.0f : G.rhs[sh * 512u + rowoff];
-    R.th = sh == WSLOT_OUT ? 0u : G.sten[sh * 512u + rowoff];
-    if (sc != WSLOT_OUT) {
-      R.r0 = *(const float4*)(G.rhs + sc * 512u + rowoff);
-      R.r1 = *(const float4*)(G.rhs + sc * 512u + rowoff + 4u);
-      R.t0 = *(const uint4*)(G.sten + sc * 512u + rowoff);
-      R.t1 = *(const uint4*)(G.sten + sc * 512u + rowoff + 4u);
-    } else {
-      R.r0 = make_float4(0.f, 0.f, 0.f, 0.f); R.r1 = R.r0;
-      R.t0 = make_uint4(0u, 0u, 0u, 0u); R.t1 = R.t0;
+    const uint32_t* rhs = reinterpret_cast<const uint32_t*>(G.rhs);
+    uint32_t rl = sor_missing(sl) ? 0u : rhs[sl * 512u + rowoff + 7u];
+    uint32_t tl = sor_missing(sl) ? 0u : G.sten[sl * 512u + rowoff + 7u];
+    uint32_t rh = sor_missing(sh) ? 0u : rhs[sh * 512u + rowoff];
+    uint32_t th = sor_missing(sh) ? 0u : G.sten[sh * 512u + rowoff];
+    uint4 r0 = z4, r1 = z4, t0 = z4, t1 = z4;
+    if (!sor_missing(sc)) {
+      r0 = *(const uint4*)(rhs + sc * 512u + rowoff);
+      r1 = *(const uint4*)(rhs + sc * 512u + rowoff + 4u);
+      t0 = *(const uint4*)(G.sten + sc * 512u + rowoff);
+      t1 = *(const uint4*)(G.sten + sc * 512u + rowoff + 4u);
     }
+    R.q0 = r0;
+    R.q1 = r1;
+    R.q2 = t0;
+    R.q3 = t1;
+    R.q4 = make_uint4(rl, rh, tl, th);
   }
+  return R;
 }
-__device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, uint32_t tid, const SorRows& R) {
+__device__ __forceinline__ void sor_row8(float* d, uint4 a, uint4 b) {
+  d[0] = __uint_as_float(a.x); d[1] = __uint_as_float(a.y); d[2] = __uint_as_float(a.z); d[3] = __uint_as_float(a.w);
+  d[4] = __uint_as_float(b.x); d[5] = __uint_as_float(b.y); d[6] = __uint_as_float(b.z); d[7] = __uint_as_float(b.w);
+}
+__device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, uint32_t tid, SorRows R) {
   if (tid < 144u) {
     float* d = sp + 12u * tid;
-    d[0] = R.a.x; d[1] = R.a.y;
-    d[2] = R.b.x; d[3] = R.b.y; d[4] = R.b.z; d[5] = R.b.w;
-    d[6] = R.c.x; d[7] = R.c.y; d[8] = R.c.z; d[9] = R.c.w;
-    d[10] = R.a.z; d[11] = R.a.w;
+    d[0] = __uint_as_float(R.q2.x);
+    d[1] = __uint_as_float(R.q2.y);
+    sor_row8(d + 2, R.q0, R.q1);
+    d[10] = __uint_as_float(R.q2.z);
+    d[11] = __uint_as_float(R.q2.w);
   } else if (tid < 244u) {
     uint32_t r = tid - 144u;
     float* d = sr + 10u * r;
     uint32_t* e = ss + 10u * r;
-    d[0] = R.rl; d[9] = R.rh; e[0] = R.tl; e[9] = R.th;
-    d[1] = R.r0.x; d[2] = R.r0.y; d[3] = R.r0.z; d[4] = R.r0.w;
-    d[5] = R.r1.x; d[6] = R.r1.y; d[7] = R.r1.z; d[8] = R.r1.w;
-    e[1] = R.t0.x; e[2] = R.t0.y; e[3] = R.t0.z; e[4] = R.t0.w;
-    e[5] = R.t1.x; e[6] = R.t1.y; e[7] = R.t1.z; e[8] = R.t1.w;
+    d[0] = __uint_as_float(R.q4.x);
+    sor_row8(d + 1, R.q0, R.q1);
+    d[9] = __uint_as_float(R.q4.y);
+    e[0] = R.q4.z;
+    e[1] = R.q2.x; e[2] = R.q2.y; e[3] = R.q2.z; e[4] = R.q2.w;
+    e[5] = R.q3.x; e[6] = R.q3.y; e[7] = R.q3.z; e[8] = R.q3.w;
+    e[9] = R.q4.w;
   }
 }
 
@@ -903,6 +831,8 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
   __shared__ float sr[WIN3];
   __shared__ uint32_t ss[WIN3];
   __shared__ uint32_t wsl[2][27];
+  __shared__ uint32_t redIdx[500];  // red cells of the 10^3 region: 12^3 index | 10^3 index << 16
+  __shared__ float ytab[8];         // 1 / n, correctly rounded
   const float omega = PP->omega;
   const uint32_t tid = threadIdx.x;
   // Workgroup w runs on XCD w % 8.  Each XCD walks one contiguous eighth of the
@@ -914,14 +844,20 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
   const uint32_t sc = min(nact, (xcd + 1u) * per);
   uint32_t b = xcd * per + (blockIdx.x >> 3);
   if (b >= sc) return;
+  for (uint32_t m = tid; m < 500u; m += WG) {
+    // window parity = global parity (the window origin 8t - 2 is even)
+    uint32_t row = m / 5u, wy = 1u + row % 10u, wz = 1u + row / 10u;
+    uint32_t wx = 1u + 2u * (m % 5u) + ((wy + wz + 1u) & 1u);
+    redIdx[m] = (wx + 12u * (wy + 12u * wz)) | (((wx - 1u) + 10u * ((wy - 1u) + 10u * (wz - 1u))) << 16);
+  }
+  if (tid < 8u) ytab[tid] = (tid >= 1u && tid <= 6u) ? 1.0f / float(tid) : 0.0f;
   // prologue: slots of tiles b and b + gs, window rows of tile b
   if (tid < 27u) {
     wsl[0][tid] = G.nbrL[b * 27u + tid];
     wsl[1][tid] = b + gs < sc ? G.nbrL[(b + gs) * 27u + tid] : 0u;
   }
   __syncthreads();
-  SorRows R;
-  sor_fetch(G, pin, wsl[0], tid, R);
+  SorRows R = sor_fetch(G, pin, wsl[0], tid);
   uint32_t cur = 0;
   for (; b < sc; b += gs, cur ^= 1u) {
     const uint32_t s = wsl[cur][13];
@@ -930,14 +866,12 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
     if (tid < 27u && b + 2u * gs < sc) ahead = G.nbrL[(b + 2u * gs) * 27u + tid];
     sor_store(sp, sr, ss, tid, R);
     __syncthreads();
-    if (more) sor_fetch(G, pin, wsl[cur ^ 1u], tid, R);
-    // red cells of the 10^3 region (window parity = global parity, origin 8t - 2)
+    if (more) R = sor_fetch(G, pin, wsl[cur ^ 1u], tid);
+    // red cells of the 10^3 region
     for (uint32_t m = tid; m < 500u; m += WG) {
-      int32_t row = int32_t(m / 5u), wy = 1 + row % 10, wz = 1 + row / 10;
-      int32_t wx = 1 + 2 * int32_t(m % 5u) + ((wy + wz + 1) & 1);
-      int32_t qr = (wx - 1) + 10 * ((wy - 1) + 10 * (wz - 1));
+      uint32_t e = redIdx[m], qr = e >> 16;
       uint32_t st = ss[qr];
-      if (st & STEN_FLUID) relax_lds<12>(sp, wx + 12 * (wy + 12 * wz), st, sr[qr], omega);
+      if (st & STEN_FLUID) relax_lds<12>(sp, int32_t(e & 0xffffu), st, sr[qr], omega, ytab);
     }
     __syncthreads();
     {  // black cells of the tile, one per thread
@@ -945,7 +879,7 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
       int32_t x = 2 * int32_t(tid & 3u) + ((y + z + 1) & 1);
       int32_t qr = (x + 1) + 10 * ((y + 1) + 10 * (z + 1));
       uint32_t st = ss[qr];
-      if (st & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), st, sr[qr], omega);
+      if (st & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), st, sr[qr], omega, ytab);
     }
     __syncthreads();
     for (uint32_t q = tid; q < 512u; q += WG) {
@@ -1618,26 +1552,22 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
 }
 
 // iterations per SOR launch: 1 (12^3 window) costs the least work per iteration
-#define SOR_ITERS_PER_LAUNCH 1
-#define SOR_PIPELINED 1
+uint32_t sor_resident_grid(uint32_t cus) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sor_pipe, WG, 0) != hipSuccess || nb < 1) nb = 1;
+  uint32_t g = (cus * uint32_t(nb)) & ~7u;
+  return g < 8u ? 8u : g;
+}
+
 void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
-  // two iterations per launch; an odd count ends with a one-iteration launch
-  int launches = 0;
-  for (int it = 0; it < iters; launches++) {
-    const float* pin = (launches & 1) ? L.G.pres2 : L.G.pres;
-    float* pout = (launches & 1) ? L.G.pres : L.G.pres2;
-    if (SOR_ITERS_PER_LAUNCH == 2 && iters - it >= 2) {
-      hipLaunchKernelGGL(k_sor_iters<2>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
-      it += 2;
-    } else if (SOR_PIPELINED) {
-      hipLaunchKernelGGL(k_sor_pipe, dim3(L.pipeGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
-      it += 1;
-    } else {
-      hipLaunchKernelGGL(k_sor_iters<1>, dim3(L.sorGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
-      it += 1;
-    }
+  // one iteration per launch, ping-pong between the two pressure buffers; an
+  // odd count leaves the result in pres2 and it is copied back
+  for (int it = 0; it < iters; it++) {
+    const float* pin = (it & 1) ? L.G.pres2 : L.G.pres;
+    float* pout = (it & 1) ? L.G.pres : L.G.pres2;
+    hipLaunchKernelGGL(k_sor_pipe, dim3(L.pipeGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
   }
-  if (launches & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
+  if (iters & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
 }
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {
