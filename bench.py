@@ -85,6 +85,20 @@ def run_gpu(sc, args):
     return wall, alive, stats, stage_ms
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM-side bytes per dispatch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
+    device copy of known size), or None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        k = d["kernels"][kernel]
+        return k["read_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def calib_copy(mb: int):
     """A plain device-to-device copy of `mb` MiB (HIP's blit kernel), so a counter
     pass can scale FETCH_SIZE / WRITE_SIZE by what they read for known traffic."""
@@ -162,7 +176,8 @@ def main():
         sor_bytes = 16 * fc
         ach = sor_bytes / (launch_us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_sor_pipe (one red-black SOR iteration per launch)",
+                "traffic": measured_traffic(args.workload, "k_sor_pipe"),
+                "kernel": "k_sor_pipe (one red-black SOR iteration per launch)",
                 "algorithmic_bytes_per_launch": sor_bytes, "launch_us": launch_us}
     out = {
         "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,

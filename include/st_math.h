@@ -60,18 +60,41 @@ ST_HD float st_smoothstepf(float lo, float hi, float x) {
 ST_HD float st_fractf(float x) { return x - floorf(x); }
 ST_HD uint32_t st_f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
 ST_HD float st_u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
-/* WGSL i32(f32): truncation toward zero, saturating, NaN -> 0 */
-ST_HD int32_t st_f2i_sat(float x) {
+/* WGSL i32(f32): truncation toward zero; out-of-range values clamp to
+ * [-2^31, 2^31 - 128] (the ends of the i32 range that f32 reaches); NaN -> 0 */
+ST_HD int32_t st_f2i_sat_portable(float x) {
   if (!(x == x)) return 0;
-  if (x >= 2147483520.0f) return 2147483647;
+  if (x >= 2147483520.0f) return 2147483520;
   if (x <= -2147483648.0f) return (-2147483647 - 1);
   return (int32_t)x;
 }
-/* WGSL u32(f32): truncation toward zero, saturating to [0, 2^32-1], NaN -> 0 */
-ST_HD uint32_t st_f2u_sat(float x) {
+/* WGSL u32(f32): truncation toward zero, clamped to [0, 2^32 - 256]; NaN -> 0 */
+ST_HD uint32_t st_f2u_sat_portable(float x) {
   if (!(x > 0.0f)) return 0u;
-  if (x >= 4294967040.0f) return 4294967295u;
+  if (x >= 4294967040.0f) return 4294967040u;
   return (uint32_t)x;
+}
+/* The same conversions.  On the GPU, v_cvt_i32_f32 / v_cvt_u32_f32 truncate,
+ * saturate to the full integer range and turn NaN into 0, so one instruction
+ * and an integer min give the result (checked against the portable form on the
+ * device for all 2^32 inputs by the test suite). */
+ST_HD int32_t st_f2i_sat(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  int32_t r;
+  __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r < 2147483520 ? r : 2147483520;
+#else
+  return st_f2i_sat_portable(x);
+#endif
+}
+ST_HD uint32_t st_f2u_sat(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  __asm__("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r < 4294967040u ? r : 4294967040u;
+#else
+  return st_f2u_sat_portable(x);
+#endif
 }
 
 /* a / n for n in 1..6, equal to the correctly rounded f32 division, given

@@ -73,6 +73,16 @@ def build_napi() -> str | None:
     return ADDON
 
 
+def build_checks() -> str:
+    """Device self-checks the GPU tests run (tests/c/*.hip), as a library beside them."""
+    src = os.path.join(ROOT, "tests", "c", "convert_check.hip")
+    out = os.path.join(ROOT, "tests", "c", "libconvert_check.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> str:
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-B", "-C", odir, "liboracle.so"])
@@ -82,4 +92,5 @@ def build_oracle() -> str:
 if __name__ == "__main__":
     print(build_hip(verbose=True))
     print(build_napi())
+    print(build_checks())
     print(build_oracle())

@@ -84,6 +84,9 @@ struct Sim {
   int iters = 1;
   uint32_t lcap = 0;
   uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
+  // need, binCount and binKey already hold the marks and bins of A's positions
+  // (the last g2p made them and nothing has moved a particle since)
+  bool binned = false;
   NbScratch nb{};
 
   LaunchCtx ctx() const {
@@ -97,6 +100,8 @@ struct Sim {
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, sorResident);  // whole groups of 8 (one per XCD)
+    L.binned = binned;
+    L.binNext = !nbOn;  // the narrow band's reseed moves particles after g2p
     L.nbOn = nbOn;
     L.initStep = initStep;
     return L;
@@ -138,7 +143,7 @@ int zeroSlots(Sim* s, uint32_t from, uint32_t to) {
   HIPCHK(hipMemsetAsync(s->G.pres + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.pres2 + a, 0, n * 4, s->st));
   HIPCHK(hipMemsetAsync(s->G.rhs + a, 0, n * 4, s->st));
-  HIPCHK(hipMemsetAsync(s->G.sten + a, 0, n * 4, s->st));
+  HIPCHK(hipMemsetAsync(s->G.sten + a, 0, n, s->st));
   HIPCHK(hipMemsetAsync(s->G.accV + a * 3, 0, n * 12, s->st));
   HIPCHK(hipMemsetAsync(s->G.accW + a * 3, 0, n * 12, s->st));
   if (s->G.velG) HIPCHK(hipMemsetAsync(s->G.velG + a * 3, 0, n * 12, s->st));
@@ -269,7 +274,7 @@ int runStage(Sim* s, int stage) {
   bool full = !s->onlyTiles;
   switch (stage) {
     case ST_STAGE_SORT: break;
-    case ST_STAGE_BLOCKS: launch_blocks(st, L); break;
+    case ST_STAGE_BLOCKS: launch_blocks(st, L); s->binned = false; break;
     case ST_STAGE_CLEAR: if (full) launch_clear(st, L); break;
     case ST_STAGE_P2G: if (full) launch_p2g(st, L); break;
     case ST_STAGE_NARROWBAND: if (full && s->nbOn) launch_narrowband(st, L); break;
@@ -283,9 +288,19 @@ int runStage(Sim* s, int stage) {
     case ST_STAGE_DIVERGENCE: if (full) launch_divergence(st, L); break;
     case ST_STAGE_PRESSURE: if (full) launch_pressure(st, L, s->iters); break;
     case ST_STAGE_PROJECT: if (full) launch_project(st, L); break;
-    case ST_STAGE_G2P: if (full) launch_g2p(st, L); break;
+    case ST_STAGE_G2P:
+      if (full) {
+        launch_g2p(st, L);
+        s->binned = L.binNext && L.n != 0;
+      }
+      break;
     case ST_STAGE_SPRAY: break;
-    case ST_STAGE_RESEED: if (full && s->nbOn) launch_reseed(st, L, s->nb); break;
+    case ST_STAGE_RESEED:
+      if (full && s->nbOn) {
+        launch_reseed(st, L, s->nb);
+        s->binned = false;
+      }
+      break;
     default: return fail(ST_ERR_ARG, "st_stage: unknown stage");
   }
   hipError_t e = hipGetLastError();
@@ -421,6 +436,7 @@ int st_upload_particles(void* p, uint32_t first, uint32_t count, const float* po
   HIPCHK(hipMemcpyAsync(s->G.vel + first, vel4, size_t(count) * 16, hipMemcpyHostToDevice, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   s->N = std::max(s->N, first + count);
+  s->binned = false;
   return ST_OK;
 }
 
@@ -429,6 +445,7 @@ int st_set_count(void* p, uint32_t n) {
   if (!s) return fail(ST_ERR_ARG, "st_set_count: null sim");
   if (n > s->maxn) return fail(ST_ERR_ARG, "st_set_count: beyond max_n");
   s->N = n;
+  s->binned = false;
   return ST_OK;
 }
 
@@ -446,6 +463,7 @@ int st_clear(void* p) {
   HIPCHK(hipStreamSynchronize(s->st));
   s->N = 0;
   s->tm = 0;
+  s->binned = false;
   std::vector<uint32_t> ids(s->maxn);
   for (size_t k = 0; k < ids.size(); k++) ids[k] = uint32_t(k);
   HIPCHK(hipMemcpy(s->G.pid, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
@@ -590,7 +608,7 @@ int st_grow(void* p, uint32_t n) {
   HIPCHK(hipMemcpy(s->G.pres, keep.pres, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.pres2, keep.pres2, C * 4, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.rhs, keep.rhs, C * 4, hipMemcpyDeviceToDevice));
-  HIPCHK(hipMemcpy(s->G.sten, keep.sten, C * 4, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(s->G.sten, keep.sten, C, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.accV, keep.accV, C * 12, hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(s->G.accW, keep.accW, C * 12, hipMemcpyDeviceToDevice));
   if (keep.velG) HIPCHK(hipMemcpy(s->G.velG, keep.velG, C * 12, hipMemcpyDeviceToDevice));

@@ -72,7 +72,7 @@ struct DGrid {
   int32_t* accV;       // 3 per cell: fixed-point sums, then new face velocity bits
   int32_t* accW;       // 3 per cell: fixed-point weights, then old face velocity bits
   float* velG;         // 3 per cell (narrow band)
-  uint32_t* sten;      // per cell: SOR stencil (fluid bit, non-solid count, fluid-neighbour mask)
+  uint8_t* sten;       // per cell: SOR stencil byte (bit 7 fluid, bits 0-2 non-solid neighbours)
   const float* sdf;
   const DCols* cols;
   float4* scratch4;    // readback scratch (id order)

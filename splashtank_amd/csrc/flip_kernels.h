@@ -16,6 +16,8 @@ struct LaunchCtx {
   uint32_t cellGrid;  // workgroups of grid-stride loops over active cells
   uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
   uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
+  bool binned;        // blocks: need, binCount and binKey already hold A's marks and bins
+  bool binNext;       // g2p: mark and bin the positions it writes, for the next substep
   bool nbOn;          // narrow band on this step
   uint32_t initStep;  // narrow-band start: 1 restart, 2 water added, 0 none
 };

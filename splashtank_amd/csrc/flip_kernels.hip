@@ -47,14 +47,14 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 // tileMark + markAround (wgsl-blocks.js:10-14, wgsl-core.js:239-246): every tile
 // within two cells of a live particle is needed.  The same pass counts each
 // particle into the bin of its cell's tile (dead particles into bin ntiles) and
-// keeps its rank in the bin.  Consecutive particles mostly share a bin (the
-// array is in last step's bin order), so the counting is one atomic per run of
-// equal bins in a wave.
-__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
-  const DParams& P = *PP;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < P.n;
-  float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
+// records the bin in binKey.  Consecutive particles mostly share a bin (the
+// array is in bin order), so the counting is one atomic per run of equal bins
+// in a wave.  k_mark_bin runs this over A at the start of a substep; g2p runs
+// it as it writes A, for the next substep, when nothing else moves particles
+// in between.
+// Particle i at position q (valid lanes only).  Every lane of the wave must
+// call this (lanes 0..26 do the marking).
+__device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint32_t i, bool valid, float4 q) {
   bool live = valid && !(q.y < ST_DEADY);
   uint32_t key = P.ntiles;
   // tiles [cc-2, cc+2] >> 3 relative to the particle's own tile, as bits
@@ -97,6 +97,14 @@ __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP,
     rem &= ~m;
   }
   if (valid) G.binKey[i] = key;
+}
+
+__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = i < P.n;
+  float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
+  bin_mark(P, G, i, valid, q);
 }
 
 // exclusive scan of n counts into out[0..n], out[n] = total; three launches
@@ -654,9 +662,8 @@ __global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __rest
 __device__ __forceinline__ float gvR(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accV[f]); }
 __device__ __forceinline__ float gv0R(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accW[f]); }
 
-// SOR stencil word: bit 15 fluid, bits 0-2 non-solid neighbours, bits 4-9 which
-// neighbours (x-, x+, y-, y+, z-, z+) are fluid.
-#define STEN_FLUID 0x8000u
+// SOR stencil byte: bit 7 fluid, bits 0-2 the count of non-solid neighbours
+#define STEN_FLUID 0x80u
 
 // divergence (wgsl-solver.js:106-123); also records the SOR stencil of the cell,
 // which depends only on the cell types fixed by classify.
@@ -667,7 +674,7 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
     uint32_t ct = G.ctype[c];
     if ((ct & 0xffu) != 1u) {
       G.rhs[c] = 0.0f;
-      G.sten[c] = 0u;
+      G.sten[c] = 0;
       continue;
     }
     int3 a = d_slotCoord(P, G.slotTile, s, local);
@@ -683,17 +690,16 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
     tys[4] = d_ctypeOf(P, G.tmap, G.ctype, i, j, k - 1);
     tys[5] = d_ctypeOf(P, G.tmap, G.ctype, i, j, k + 1);
     bool interior = true;
-    uint32_t nns = 0, mask = 0;
+    uint32_t nns = 0;
     for (int q = 0; q < 6; q++) {
       if (tys[q] == 0u) interior = false;
       if (tys[q] != 2u) nns++;
-      if (tys[q] == 1u) mask |= 1u << q;
     }
     float dens = float(G.cnt[c]) / P.rest - 1.0f;
     float tgt = P.comp * (interior ? dens : fmaxf(dens, 0.0f));
     if (P.nb != 0u && d_phiOfBits(ct) < -1.0f) tgt = 0.0f;
     G.rhs[c] = (tgt - div) * P.h * P.h / P.dt;
-    G.sten[c] = STEN_FLUID | nns | (mask << 4);
+    G.sten[c] = uint8_t(STEN_FLUID | nns);
   }
 }
 
@@ -745,18 +751,19 @@ __device__ __forceinline__ bool sor_missing(uint32_t slot) { return slot + 2u <=
 // independent within the launch (they read pin, write pout), so the pipeline
 // only reorders loads.
 // A thread holds either a pressure row (threads 0..143) or an rhs and stencil
-// row (threads 144..243) in the same 20 registers:
+// row (threads 144..243) in the same 13 registers:
 //   pressure: q0, q1 the 8 cells of the row's own tile, q2 = (two cells to the
 //             left, two to the right);
-//   rhs/sten: q0, q1 rhs of the own tile, q2, q3 its stencil words,
-//             q4 = (rhs left, rhs right, stencil left, stencil right).
+//   rhs/sten: q0, q1 rhs of the own tile, q2 = (rhs left, rhs right, stencil
+//             bytes of the own tile as two words), e = stencil left | right << 8.
 struct SorRows {
-  uint4 q0, q1, q2, q3, q4;
+  uint4 q0, q1, q2;
+  uint32_t e;
 };
 
 __device__ __forceinline__ SorRows sor_fetch(const DGrid& G, const float* __restrict__ pin, const uint32_t* ws, uint32_t tid) {
   const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-  SorRows R{z4, z4, z4, z4, z4};
+  SorRows R{z4, z4, z4, 0u};
   if (tid < 144u) {
     int32_t ly = int32_t(tid % 12u) - 2, lz = int32_t(tid / 12u) - 2;
     uint32_t rowoff = 8u * (uint32_t(ly & 7) + 8u * uint32_t(lz & 7));
@@ -765,13 +772,10 @@ __device__ __forceinline__ SorRows sor_fetch(const DGrid& G, const float* __rest
     const uint32_t* src = reinterpret_cast<const uint32_t*>(pin);
     uint2 l = sor_missing(sl) ? make_uint2(0u, 0u) : *(const uint2*)(src + sl * 512u + rowoff + 6u);
     uint2 h = sor_missing(sh) ? make_uint2(0u, 0u) : *(const uint2*)(src + sh * 512u + rowoff);
-    uint4 c0 = z4, c1 = z4;
     if (!sor_missing(sc)) {
-      c0 = *(const uint4*)(src + sc * 512u + rowoff);
-      c1 = *(const uint4*)(src + sc * 512u + rowoff + 4u);
+      R.q0 = *(const uint4*)(src + sc * 512u + rowoff);
+      R.q1 = *(const uint4*)(src + sc * 512u + rowoff + 4u);
     }
-    R.q0 = c0;
-    R.q1 = c1;
     R.q2 = make_uint4(l.x, l.y, h.x, h.y);
   } else if (tid < 244u) {
     uint32_t r = tid - 144u;
@@ -780,22 +784,23 @@ __device__ __forceinline__ SorRows sor_fetch(const DGrid& G, const float* __rest
     int32_t so = 3 * (((ly >> 3) + 1) + 3 * ((lz >> 3) + 1));
     uint32_t sl = ws[so], sc = ws[so + 1], sh = ws[so + 2];
     const uint32_t* rhs = reinterpret_cast<const uint32_t*>(G.rhs);
-    uint32_t rl = sor_missing(sl) ? 0u : rhs[sl * 512u + rowoff + 7u];
-    uint32_t tl = sor_missing(sl) ? 0u : G.sten[sl * 512u + rowoff + 7u];
-    uint32_t rh = sor_missing(sh) ? 0u : rhs[sh * 512u + rowoff];
-    uint32_t th = sor_missing(sh) ? 0u : G.sten[sh * 512u + rowoff];
-    uint4 r0 = z4, r1 = z4, t0 = z4, t1 = z4;
-    if (!sor_missing(sc)) {
-      r0 = *(const uint4*)(rhs + sc * 512u + rowoff);
-      r1 = *(const uint4*)(rhs + sc * 512u + rowoff + 4u);
-      t0 = *(const uint4*)(G.sten + sc * 512u + rowoff);
-      t1 = *(const uint4*)(G.sten + sc * 512u + rowoff + 4u);
+    uint32_t rl = 0u, rh = 0u, tl = 0u, th = 0u;
+    if (!sor_missing(sl)) {
+      rl = rhs[sl * 512u + rowoff + 7u];
+      tl = G.sten[sl * 512u + rowoff + 7u];
     }
-    R.q0 = r0;
-    R.q1 = r1;
-    R.q2 = t0;
-    R.q3 = t1;
-    R.q4 = make_uint4(rl, rh, tl, th);
+    if (!sor_missing(sh)) {
+      rh = rhs[sh * 512u + rowoff];
+      th = G.sten[sh * 512u + rowoff];
+    }
+    uint2 t = make_uint2(0u, 0u);
+    if (!sor_missing(sc)) {
+      R.q0 = *(const uint4*)(rhs + sc * 512u + rowoff);
+      R.q1 = *(const uint4*)(rhs + sc * 512u + rowoff + 4u);
+      t = *(const uint2*)(G.sten + sc * 512u + rowoff);
+    }
+    R.q2 = make_uint4(rl, rh, t.x, t.y);
+    R.e = tl | (th << 8);
   }
   return R;
 }
@@ -815,13 +820,13 @@ __device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, ui
     uint32_t r = tid - 144u;
     float* d = sr + 10u * r;
     uint32_t* e = ss + 10u * r;
-    d[0] = __uint_as_float(R.q4.x);
+    d[0] = __uint_as_float(R.q2.x);
     sor_row8(d + 1, R.q0, R.q1);
-    d[9] = __uint_as_float(R.q4.y);
-    e[0] = R.q4.z;
-    e[1] = R.q2.x; e[2] = R.q2.y; e[3] = R.q2.z; e[4] = R.q2.w;
-    e[5] = R.q3.x; e[6] = R.q3.y; e[7] = R.q3.z; e[8] = R.q3.w;
-    e[9] = R.q4.w;
+    d[9] = __uint_as_float(R.q2.y);
+    e[0] = R.e & 0xffu;
+    e[1] = R.q2.z & 0xffu; e[2] = (R.q2.z >> 8) & 0xffu; e[3] = (R.q2.z >> 16) & 0xffu; e[4] = R.q2.z >> 24;
+    e[5] = R.q2.w & 0xffu; e[6] = (R.q2.w >> 8) & 0xffu; e[7] = (R.q2.w >> 16) & 0xffu; e[8] = R.q2.w >> 24;
+    e[9] = R.e >> 8;
   }
 }
 
@@ -1116,7 +1121,19 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
 
 // g2p over the bins: one workgroup per tile (bin ntiles holds the dead
 // particles, which are copied).  Reads B, writes A in the same (bin) order.
-__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+// With `keys` set it also bins the new positions for the next substep
+// (bin_mark); the particle loops then run whole waves, so every lane calls it.
+__device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint32_t keys, uint32_t k, bool valid, float4 po,
+                                         float4 vo) {
+  if (valid) {
+    G.pos[k] = po;
+    G.vel[k] = vo;
+    G.pid[k] = G.pidB[k];
+  }
+  if (keys) bin_mark(P, G, k, valid, po);
+}
+
+__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G, uint32_t keys) {
   const DParams& P = *PP;
   __shared__ float sN[3][WIN3];
   __shared__ float sO[3][WIN3];
@@ -1126,22 +1143,27 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     if (beg == end) continue;
     if (t == P.ntiles) {
-      for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
-        G.pos[k] = G.posB[k];
-        G.vel[k] = G.velB[k];
-        G.pid[k] = G.pidB[k];
+      for (uint32_t k0 = beg; k0 < end; k0 += WG) {
+        uint32_t k = k0 + threadIdx.x;
+        bool valid = k < end;
+        float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
+        if (valid) {
+          po = G.posB[k];
+          vo = G.velB[k];
+        }
+        g2p_emit(P, G, keys, k, valid, po, vo);
       }
       continue;
     }
     bool storage = G.tmap[t] != ST_NONE;
     if (!storage) {  // its block had no room this step: falls freely (wgsl-solver.js:215)
       GlobalGrid gg{P, G};
-      for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
-        float4 po, vo;
-        g2p_particle(P, C, G.sdf, gg, false, G.posB[k], G.velB[k], po, vo);
-        G.pos[k] = po;
-        G.vel[k] = vo;
-        G.pid[k] = G.pidB[k];
+      for (uint32_t k0 = beg; k0 < end; k0 += WG) {
+        uint32_t k = k0 + threadIdx.x;
+        bool valid = k < end;
+        float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
+        if (valid) g2p_particle(P, C, G.sdf, gg, false, G.posB[k], G.velB[k], po, vo);
+        g2p_emit(P, G, keys, k, valid, po, vo);
       }
       continue;
     }
@@ -1163,12 +1185,12 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
     }
     __syncthreads();
     WindowGrid wg{P, G, sN, sO, sC, org};
-    for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
-      float4 po, vo;
-      g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
-      G.pos[k] = po;
-      G.vel[k] = vo;
-      G.pid[k] = G.pidB[k];
+    for (uint32_t k0 = beg; k0 < end; k0 += WG) {
+      uint32_t k = k0 + threadIdx.x;
+      bool valid = k < end;
+      float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
+      if (valid) g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
+      g2p_emit(P, G, keys, k, valid, po, vo);
     }
     __syncthreads();
   }
@@ -1484,9 +1506,11 @@ static void launch_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint3
 
 void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   uint32_t nt = L.ntiles;
-  launch_fill_u32(st, L.G.need, nt, 0u);
-  launch_fill_u32(st, L.G.binCount, size_t(nt) + 1, 0u);
-  if (L.n) hipLaunchKernelGGL(k_mark_bin, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
+  if (!L.binned) {  // else the last g2p already marked and counted A's positions
+    launch_fill_u32(st, L.G.need, nt, 0u);
+    launch_fill_u32(st, L.G.binCount, size_t(nt) + 1, 0u);
+    if (L.n) hipLaunchKernelGGL(k_mark_bin, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
+  }
   // narrow band: last step's tiles holding water by depth, and on a start the
   // tiles the starting water reaches (both before the counters reset)
   if (L.nbOn && L.initStep != 1) hipLaunchKernelGGL(k_nb_mark_phi, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
@@ -1577,8 +1601,12 @@ void launch_project(hipStream_t st, const LaunchCtx& L) {
 
 void launch_g2p(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
+  if (L.binNext) {  // the next substep's tile marks and bin counts start here
+    launch_fill_u32(st, L.G.need, L.ntiles, 0u);
+    launch_fill_u32(st, L.G.binCount, size_t(L.ntiles) + 1, 0u);
+  }
   uint32_t g = std::min<uint32_t>(L.ntiles + 1, 8192u);
-  hipLaunchKernelGGL(k_g2p_tile, dim3(g), dim3(WG), 0, st, L.P, L.C, L.G);
+  hipLaunchKernelGGL(k_g2p_tile, dim3(g), dim3(WG), 0, st, L.P, L.C, L.G, L.binNext ? 1u : 0u);
 }
 
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n) {
