@@ -179,7 +179,7 @@ void destroySim(Sim* s) {
   freeNb(s->nb);
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
   dfree(s->G.binKey); dfree(s->G.binCursor); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
-  dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part);
+  dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part); dfree(s->G.foamTab);
   dfree(s->G.scratch4);
   freeSlots(s);
   dfree(s->dSdf);
@@ -379,7 +379,7 @@ int st_create(const st_opts* o, void** out) {
       (rc = dalloc(&s->G.scanTmp, size_t(nbin + 1023) / 1024 + 1)) ||
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
-      (rc = dalloc(&s->G.part, 2 * 8192 + 64)) ||
+      (rc = dalloc(&s->G.part, 2 * 8192 + 64)) || (rc = dalloc(&s->G.foamTab, FOAM_TAB)) ||
       (rc = dalloc(&s->dSlots, RING))) {
     destroySim(s);
     return rc;

@@ -76,6 +76,7 @@ struct DGrid {
   const float* sdf;
   const DCols* cols;
   float4* scratch4;    // readback scratch (id order)
+  float4* foamTab;     // per step, for a density sum c < FOAM_TAB: the g2p factors of c (d_foamRow)
 };
 
 // ---------------------------------------------------------------------------

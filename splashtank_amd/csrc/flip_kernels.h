@@ -6,6 +6,9 @@
 
 // Everything one stage launch needs: the step's parameter block and collider
 // records (device copies), the sim's device pointers, and launch sizes.
+// entries of the per-step foam table (density sums 0 .. FOAM_TAB - 1)
+#define FOAM_TAB 1024
+
 struct LaunchCtx {
   const DParams* P;
   const DCols* C;

@@ -977,8 +977,7 @@ struct GlobalGrid {
 struct WindowGrid {
   const DParams& P;
   const DGrid& G;
-  const float (*sN)[WIN3];
-  const float (*sO)[WIN3];
+  const float2 (*sNO)[WIN3];  // per axis and window cell: (new, old) face velocity
   const int32_t* sC;
   int3 org;
   __device__ void sample(float3 p, float3& vn, float3& vo) const { d_sampleBothWin(P, *this, p, vn, vo); }
@@ -1010,8 +1009,9 @@ __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGr
       int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
       float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
       int32_t wi = wx[dx] + WIN * (wy[dy] + WIN * wz[dz]);
-      sn = sn + w * g.sN[axis][wi];
-      so = so + w * g.sO[axis][wi];
+      float2 e = g.sNO[axis][wi];
+      sn = sn + w * e.x;
+      so = so + w * e.y;
     }
     rn[axis] = sn;
     ro[axis] = so;
@@ -1048,9 +1048,31 @@ __device__ __forceinline__ void d_sampleBoth(const DParams& P, const GridT& g, f
 
 // one particle of g2p: FLIP/PIC blend, foam, air, speed cap, Euler advection,
 // collisions, open faces (wgsl-solver.js:208-275)
+// The part of g2p that depends only on the density sum csum (the count of the
+// 8 cells around the particle, or 8 x its own cell's count): air, shield, the
+// foam decay exp(-dt / tau) and the drag factor 1 - exp(-kd dt)
+// (wgsl-solver.js:221-245).  Per step, k_foam_table evaluates it for every
+// integer csum below FOAM_TAB; g2p reads the table, which gives the same bits
+// as evaluating it in place.
+__device__ __forceinline__ float4 d_foamRow(const DParams& P, float csum) {
+  float rho = csum / (8.0f * P.rest);
+  float air = 1.0f - st_smoothstepf(0.008f, 0.1f, rho);
+  float shield = st_smoothstepf(0.1f, 0.65f, rho);
+  float exposure = 1.0f - 0.97f * shield;
+  float tau = fmaxf(P.q5, 0.05f) * st_mixf(1.0f, 0.35f, st_smoothstepf(0.7f, 1.0f, rho)) * st_mixf(1.0f, 2.5f, air);
+  float kd = P.q4 * (0.04f + 1.5f * exposure * exposure);
+  return make_float4(air, shield, st_expf(-P.dt / tau), 1.0f - st_expf(-kd * P.dt));
+}
+
+__global__ void k_foam_table(const DParams* __restrict__ PP, DGrid G) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < FOAM_TAB; t += gridDim.x * blockDim.x)
+    G.foamTab[t] = d_foamRow(*PP, float(t));
+}
+
 template <class GridT>
-__device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, const GridT& g,
+__device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __restrict__ C, const DGrid& G, const GridT& g,
                                              bool storage, float4 p4, float4 v04, float4& po, float4& vo4) {
+  const float* __restrict__ sdf = G.sdf;
   float3 p = make_float3(p4.x, p4.y, p4.z);
   float3 v0 = make_float3(v04.x, v04.y, v04.z);
   int3 fcell = d_particleCell(P, p);
@@ -1073,10 +1095,9 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
           csum = csum + float(g.cnt(cx, cy, cz));
         }
   }
-  float rho = csum / (8.0f * P.rest);
-  float air = 1.0f - st_smoothstepf(0.008f, 0.1f, rho);
-  float shield = st_smoothstepf(0.1f, 0.65f, rho);
-  float exposure = 1.0f - 0.97f * shield;
+  // air, shield, foam decay and drag factor depend on the density sum alone
+  const float4 ft = (P.q9 < 0.5f && csum >= 0.0f && csum < float(FOAM_TAB)) ? G.foamTab[int32_t(csum)] : d_foamRow(P, csum);
+  const float air = ft.x, shield = ft.y;
   float spd0 = f3len(v);
   float sens = fmaxf(P.q6, 0.05f);
   float pacc = f3len(f3sub(vn, vo)) / P.dt;
@@ -1084,8 +1105,7 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   float impact = st_clampf((pacc / sens - 15.0f) / 80.0f, 0.0f, 1.0f) * settle;
   float turb = st_clampf((f3len(f3sub(v0, vo)) / sens - 0.35f) / 1.5f, 0.0f, 1.0f) * settle;
   float spray = air * (0.25f + 0.75f * st_clampf(spd0 / 4.0f, 0.0f, 1.0f));
-  float tau = fmaxf(P.q5, 0.05f) * st_mixf(1.0f, 0.35f, st_smoothstepf(0.7f, 1.0f, rho)) * st_mixf(1.0f, 2.5f, air);
-  float foam = v04.w * st_expf(-P.dt / tau);
+  float foam = v04.w * ft.z;
   foam = fmaxf(foam, fmaxf(spray * 0.9f, fmaxf(impact, turb * 0.8f)));
   float3 gust = make_float3(0.0f, 0.0f, 0.0f);
   if (P.q7 > 0.0f && P.q4 > 0.0f && (!fast || shield < 0.9f)) {
@@ -1095,9 +1115,8 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
                                    d_vnz(f3add(gp, make_float3(5.7f, 9.2f, tt)))), 2.0f),
                  make_float3(1.0f, 1.0f, 1.0f));
   }
-  float kd = P.q4 * (0.04f + 1.5f * exposure * exposure);
   float3 airv = f3add(make_float3(P.q1, P.q2, P.q3), f3mul(gust, P.q7));
-  v = f3add(v, f3mul(f3sub(airv, v), 1.0f - st_expf(-kd * P.dt)));
+  v = f3add(v, f3mul(f3sub(airv, v), ft.w));
   float sp = f3len(v);
   if (sp > P.vmax) v = f3mul(v, P.vmax / sp);
   float3 pt = f3add(p, f3mul(v, P.dt));
@@ -1135,8 +1154,7 @@ __device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint3
 
 __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G, uint32_t keys) {
   const DParams& P = *PP;
-  __shared__ float sN[3][WIN3];
-  __shared__ float sO[3][WIN3];
+  __shared__ float2 sNO[3][WIN3];
   __shared__ int32_t sC[WIN3];
   __shared__ uint32_t wslots[27];
   for (uint32_t t = blockIdx.x; t <= P.ntiles; t += gridDim.x) {
@@ -1162,7 +1180,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
         uint32_t k = k0 + threadIdx.x;
         bool valid = k < end;
         float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
-        if (valid) g2p_particle(P, C, G.sdf, gg, false, G.posB[k], G.velB[k], po, vo);
+        if (valid) g2p_particle(P, C, G, gg, false, G.posB[k], G.velB[k], po, vo);
         g2p_emit(P, G, keys, k, valid, po, vo);
       }
       continue;
@@ -1174,22 +1192,22 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
       uint32_t c = wcell(wslots, wx, wy, wz, org);
       if (c == ST_NONE) {
-        sN[0][q] = 0.0f; sN[1][q] = 0.0f; sN[2][q] = 0.0f;
-        sO[0][q] = 0.0f; sO[1][q] = 0.0f; sO[2][q] = 0.0f;
+        sNO[0][q] = make_float2(0.0f, 0.0f); sNO[1][q] = make_float2(0.0f, 0.0f); sNO[2][q] = make_float2(0.0f, 0.0f);
         sC[q] = 0;
         continue;
       }
-      sN[0][q] = gvR(G, c * 3); sN[1][q] = gvR(G, c * 3 + 1); sN[2][q] = gvR(G, c * 3 + 2);
-      sO[0][q] = gv0R(G, c * 3); sO[1][q] = gv0R(G, c * 3 + 1); sO[2][q] = gv0R(G, c * 3 + 2);
+      sNO[0][q] = make_float2(gvR(G, c * 3), gv0R(G, c * 3));
+      sNO[1][q] = make_float2(gvR(G, c * 3 + 1), gv0R(G, c * 3 + 1));
+      sNO[2][q] = make_float2(gvR(G, c * 3 + 2), gv0R(G, c * 3 + 2));
       sC[q] = G.cnt[c];
     }
     __syncthreads();
-    WindowGrid wg{P, G, sN, sO, sC, org};
+    WindowGrid wg{P, G, sNO, sC, org};
     for (uint32_t k0 = beg; k0 < end; k0 += WG) {
       uint32_t k = k0 + threadIdx.x;
       bool valid = k < end;
       float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
-      if (valid) g2p_particle(P, C, G.sdf, wg, true, G.posB[k], G.velB[k], po, vo);
+      if (valid) g2p_particle(P, C, G, wg, true, G.posB[k], G.velB[k], po, vo);
       g2p_emit(P, G, keys, k, valid, po, vo);
     }
     __syncthreads();
@@ -1605,6 +1623,7 @@ void launch_g2p(hipStream_t st, const LaunchCtx& L) {
     launch_fill_u32(st, L.G.need, L.ntiles, 0u);
     launch_fill_u32(st, L.G.binCount, size_t(L.ntiles) + 1, 0u);
   }
+  hipLaunchKernelGGL(k_foam_table, dim3(FOAM_TAB / WG), dim3(WG), 0, st, L.P, L.G);
   uint32_t g = std::min<uint32_t>(L.ntiles + 1, 8192u);
   hipLaunchKernelGGL(k_g2p_tile, dim3(g), dim3(WG), 0, st, L.P, L.C, L.G, L.binNext ? 1u : 0u);
 }
