@@ -177,8 +177,8 @@ void destroySim(Sim* s) {
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   freeNb(s->nb);
-  dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB);
-  dfree(s->G.binKey); dfree(s->G.binCursor); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
+  dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB); dfree(s->G.binIdx);
+  dfree(s->G.binKey); dfree(s->G.binRank); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part); dfree(s->G.foamTab);
   dfree(s->G.scratch4);
   freeSlots(s);
@@ -292,6 +292,11 @@ int runStage(Sim* s, int stage) {
       if (full) {
         launch_g2p(st, L);
         s->binned = L.binNext && L.n != 0;
+        if (L.n) {  // g2p wrote the new state into the other buffer
+          std::swap(s->G.pos, s->G.posB);
+          std::swap(s->G.vel, s->G.velB);
+          std::swap(s->G.pid, s->G.pidB);
+        }
       }
       break;
     case ST_STAGE_SPRAY: break;
@@ -374,8 +379,8 @@ int st_create(const st_opts* o, void** out) {
   size_t M = s->maxn;
   uint32_t nbin = s->ntiles + 1;
   if ((rc = dalloc(&s->G.pos, M)) || (rc = dalloc(&s->G.vel, M)) || (rc = dalloc(&s->G.pid, M)) || (rc = dalloc(&s->G.scratch4, M)) ||
-      (rc = dalloc(&s->G.posB, M)) || (rc = dalloc(&s->G.velB, M)) || (rc = dalloc(&s->G.pidB, M)) || (rc = dalloc(&s->G.binKey, M)) ||
-      (rc = dalloc(&s->G.binCount, nbin)) || (rc = dalloc(&s->G.binStart, size_t(nbin) + 1)) || (rc = dalloc(&s->G.binCursor, nbin)) ||
+      (rc = dalloc(&s->G.posB, M)) || (rc = dalloc(&s->G.velB, M)) || (rc = dalloc(&s->G.pidB, M)) || (rc = dalloc(&s->G.binKey, M)) || (rc = dalloc(&s->G.binIdx, M)) ||
+      (rc = dalloc(&s->G.binCount, nbin)) || (rc = dalloc(&s->G.binStart, size_t(nbin) + 1)) || (rc = dalloc(&s->G.binRank, M)) ||
       (rc = dalloc(&s->G.scanTmp, size_t(nbin + 1023) / 1024 + 1)) ||
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
@@ -390,7 +395,13 @@ int st_create(const st_opts* o, void** out) {
   // empty tile map, every slot but the spare slot 0 on the free stack (sim.js:94-98)
   std::vector<uint32_t> host(std::max<size_t>(M, std::max<size_t>(s->ntiles, s->scap)));
   for (size_t k = 0; k < M; k++) host[k] = uint32_t(k);
-  if ((e = hipMemcpy(s->G.pid, host.data(), M * 4, hipMemcpyHostToDevice)) != hipSuccess) { destroySim(s); return fail(ST_ERR_HIP, "pid upload failed"); }
+  // ids in both particle buffers: slots past the live count keep their own id,
+  // whichever buffer is current, so appended particles get id = index
+  if ((e = hipMemcpy(s->G.pid, host.data(), M * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(s->G.pidB, host.data(), M * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+    destroySim(s);
+    return fail(ST_ERR_HIP, "pid upload failed");
+  }
   std::vector<uint32_t> st(s->scap, 0u);
   for (uint32_t k = 0; k + 1 < s->scap; k++) st[k] = s->scap - 1 - k;
   std::vector<uint32_t> tm(s->ntiles, ST_NONE);
@@ -467,6 +478,7 @@ int st_clear(void* p) {
   std::vector<uint32_t> ids(s->maxn);
   for (size_t k = 0; k < ids.size(); k++) ids[k] = uint32_t(k);
   HIPCHK(hipMemcpy(s->G.pid, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->G.pidB, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
   return ST_OK;
 }
 

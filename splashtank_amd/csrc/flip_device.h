@@ -45,11 +45,13 @@ struct DGrid {
   float4* pos;
   float4* vel;
   uint32_t* pid;
-  float4* posB;  // bin-sorted copy read by p2g and g2p
+  float4* posB;  // the other particle buffer: g2p writes the next state here in bin order, then
+                 // the host swaps it with pos/vel/pid
   float4* velB;
   uint32_t* pidB;
   uint32_t* binKey;    // per particle (A order): bin = tile of its cell, ntiles if dead
-  uint32_t* binCursor; // per bin: next free place while scattering into bin order
+  uint32_t* binIdx;    // bin order: index into pos/vel/pid of the k-th particle
+  uint32_t* binRank;   // per particle (A order): its place within its bin
   uint32_t* binCount;  // per tile (+1 dead bin): particles whose cell is in the tile
   uint32_t* binStart;  // exclusive prefix of binCount (ntiles + 2 entries, last = N)
   uint32_t* scanTmp;   // per-1024-block sums of the bin scan

@@ -47,11 +47,11 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 // tileMark + markAround (wgsl-blocks.js:10-14, wgsl-core.js:239-246): every tile
 // within two cells of a live particle is needed.  The same pass counts each
 // particle into the bin of its cell's tile (dead particles into bin ntiles) and
-// records the bin in binKey.  Consecutive particles mostly share a bin (the
-// array is in bin order), so the counting is one atomic per run of equal bins
-// in a wave.  k_mark_bin runs this over A at the start of a substep; g2p runs
-// it as it writes A, for the next substep, when nothing else moves particles
-// in between.
+// records the bin in binKey and its place within the bin in binRank.
+// Consecutive particles mostly share a bin (the array is in bin order), so the
+// counting is one atomic per run of equal bins in a wave.  k_mark_bin runs this
+// over A at the start of a substep; g2p runs it as it writes the new state, for
+// the next substep, when nothing else moves particles in between.
 // Particle i at position q (valid lanes only).  Every lane of the wave must
 // call this (lanes 0..26 do the marking).
 __device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint32_t i, bool valid, float4 q) {
@@ -80,6 +80,7 @@ __device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint3
   // particle of the run needs it
   const uint32_t ln = lane_id();
   const int32_t mdx = int32_t(ln % 3u) - 1, mdy = int32_t((ln / 3u) % 3u) - 1, mdz = int32_t(ln / 9u) - 1;
+  uint32_t rank = 0u;
   uint64_t rem = __ballot(valid);
   while (rem) {
     int leader = __ffsll((unsigned long long)rem) - 1;
@@ -88,7 +89,11 @@ __device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint3
     bool inrun = (m >> ln) & 1ull;
     uint32_t um = inrun ? nmask : 0u;
     for (int off = 32; off > 0; off >>= 1) um |= (uint32_t)__shfl_xor((int)um, off);
-    if (int(ln) == leader) atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+    // the run's places in the bin: one returning atomic, read at the end
+    uint32_t base = 0u;
+    if (int(ln) == leader) base = atomicAdd(&G.binCount[k], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (inrun) rank = base + (uint32_t)__popcll(m & lanemask_lt());
     if (k < P.ntiles) {
       int32_t tx0 = __shfl(ox, leader), ty0 = __shfl(oy, leader), tz0 = __shfl(oz, leader);
       if (ln < 27u && ((um >> ln) & 1u))
@@ -96,7 +101,10 @@ __device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint3
     }
     rem &= ~m;
   }
-  if (valid) G.binKey[i] = key;
+  if (valid) {
+    G.binKey[i] = key;
+    G.binRank[i] = rank;
+  }
 }
 
 __global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
@@ -176,36 +184,13 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uin
     }
 }
 
-// scatter into bin order: A (pos, vel, pid) -> B.  Each run of equal bins in a
-// wave takes its places with one cursor atomic; the particle loads do not wait
-// for it.
+// bin order as an index list: binIdx[binStart[bin] + rank] = i.  The particles
+// themselves stay where they are; p2g and g2p read them through the list (runs
+// of a bin are mostly contiguous in A, which is in last step's bin order).
 __global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < PP->n;
-  uint32_t key = 0u, id = 0u;
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p;
-  if (valid) {
-    key = G.binKey[i];
-    p = G.pos[i];
-    v = G.vel[i];
-    id = G.pid[i];
-  }
-  uint64_t rem = __ballot(valid);
-  uint32_t dst = 0;
-  while (rem) {
-    int leader = __ffsll((unsigned long long)rem) - 1;
-    uint32_t k = (uint32_t)__shfl((int)key, leader);
-    uint64_t m = __ballot(valid && key == k) & rem;
-    uint32_t base = 0;
-    if (int(lane_id()) == leader) base = atomicAdd(&G.binCursor[k], (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, leader);
-    if ((m >> lane_id()) & 1ull) dst = base + (uint32_t)__popcll(m & lanemask_lt());
-    rem &= ~m;
-  }
-  if (!valid) return;
-  G.posB[dst] = p;
-  G.velB[dst] = v;
-  G.pidB[dst] = id;
+  if (i >= PP->n) return;
+  G.binIdx[G.binStart[G.binKey[i]] + G.binRank[i]] = i;
 }
 
 // Flags of one tile: freed (no longer needed, had a slot), new (needed, had
@@ -472,8 +457,9 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
     for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
-      float4 p4 = G.posB[k];
-      float4 v4 = G.velB[k];
+      uint32_t i = G.binIdx[k];
+      float4 p4 = G.pos[i];
+      float4 v4 = G.vel[i];
       float3 p = make_float3(p4.x, p4.y, p4.z);
       float3 v = make_float3(v4.x, v4.y, v4.z);
       int3 cc = d_particleCell(P, p);
@@ -1139,15 +1125,16 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
 }
 
 // g2p over the bins: one workgroup per tile (bin ntiles holds the dead
-// particles, which are copied).  Reads B, writes A in the same (bin) order.
+// particles, which are copied).  Reads A through the bin list and writes the new
+// state into B in bin order; the host then swaps A and B.
 // With `keys` set it also bins the new positions for the next substep
 // (bin_mark); the particle loops then run whole waves, so every lane calls it.
-__device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint32_t keys, uint32_t k, bool valid, float4 po,
-                                         float4 vo) {
+__device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint32_t keys, uint32_t k, uint32_t i, bool valid,
+                                         float4 po, float4 vo) {
   if (valid) {
-    G.pos[k] = po;
-    G.vel[k] = vo;
-    G.pid[k] = G.pidB[k];
+    G.posB[k] = po;
+    G.velB[k] = vo;
+    G.pidB[k] = G.pid[i];
   }
   if (keys) bin_mark(P, G, k, valid, po);
 }
@@ -1164,12 +1151,13 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       for (uint32_t k0 = beg; k0 < end; k0 += WG) {
         uint32_t k = k0 + threadIdx.x;
         bool valid = k < end;
+        uint32_t i = valid ? G.binIdx[k] : 0u;
         float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
         if (valid) {
-          po = G.posB[k];
-          vo = G.velB[k];
+          po = G.pos[i];
+          vo = G.vel[i];
         }
-        g2p_emit(P, G, keys, k, valid, po, vo);
+        g2p_emit(P, G, keys, k, i, valid, po, vo);
       }
       continue;
     }
@@ -1179,9 +1167,10 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
       for (uint32_t k0 = beg; k0 < end; k0 += WG) {
         uint32_t k = k0 + threadIdx.x;
         bool valid = k < end;
+        uint32_t i = valid ? G.binIdx[k] : 0u;
         float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
-        if (valid) g2p_particle(P, C, G, gg, false, G.posB[k], G.velB[k], po, vo);
-        g2p_emit(P, G, keys, k, valid, po, vo);
+        if (valid) g2p_particle(P, C, G, gg, false, G.pos[i], G.vel[i], po, vo);
+        g2p_emit(P, G, keys, k, i, valid, po, vo);
       }
       continue;
     }
@@ -1206,9 +1195,10 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
     for (uint32_t k0 = beg; k0 < end; k0 += WG) {
       uint32_t k = k0 + threadIdx.x;
       bool valid = k < end;
+      uint32_t i = valid ? G.binIdx[k] : 0u;
       float4 po = make_float4(0.f, -1.0e6f, 0.f, 0.f), vo = po;
-      if (valid) g2p_particle(P, C, G, wg, true, G.posB[k], G.velB[k], po, vo);
-      g2p_emit(P, G, keys, k, valid, po, vo);
+      if (valid) g2p_particle(P, C, G, wg, true, G.pos[i], G.vel[i], po, vo);
+      g2p_emit(P, G, keys, k, i, valid, po, vo);
     }
     __syncthreads();
   }
@@ -1542,7 +1532,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(L.scap) * 27, WG), 2048u))), dim3(WG), 0, st, L.P, L.G);
   if (L.initStep) hipLaunchKernelGGL(k_nb_phi_init, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   // particle bins: exclusive scan of the counts, then A -> B in bin order
-  launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp, L.G.binCursor);
+  launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp, nullptr);
   if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
 }
 
