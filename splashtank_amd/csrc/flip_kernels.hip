@@ -21,6 +21,7 @@
 #define WIN 10      // window edge: tile of 8 plus one cell each side
 #define WIN3 1000
 #define SW3_1 1728  // 12^3: pressure window of one SOR iteration
+#define SOR_NRED 448  // red cells one SOR tile relaxes: its own 256 and 6 x 32 across its faces
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
@@ -692,9 +693,9 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
 // relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
 // sim.js:291-293): one red-black iteration per launch, by tiles.  A workgroup
 // stages the pressure of a 12^3 window (the tile and two cells of halo) and the
-// rhs and stencil of the 10^3 window in LDS, relaxes the red cells of the 10^3
-// window, then the black cells of the tile, and writes the tile to the other
-// pressure buffer.  A red cell reads only black neighbours and a black cell only
+// rhs and stencil of the 10^3 window in LDS, relaxes the red cells of the tile
+// and of the one-cell slabs across its six faces, then the black cells of the
+// tile, and writes the tile to the other pressure buffer.  A red cell reads only black neighbours and a black cell only
 // red ones, so the black sweep sees exactly the red values the neighbouring
 // tiles compute, and the result is the reference's red-black iterate bit for bit.
 //
@@ -706,6 +707,7 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
 // of non-solid neighbours is st_div_small, equal to the division.
 template <int W>
 __device__ __forceinline__ void relax_lds(float* sp, int32_t p, uint32_t st, float rhs, float omega, const float* ytab) {
+  // st: the stencil byte (bit 7 fluid, bits 0-2 non-solid neighbours)
   uint32_t nns = st & 7u;
   if (nns == 0u) {
     sp[p] = 0.0f;
@@ -794,7 +796,8 @@ __device__ __forceinline__ void sor_row8(float* d, uint4 a, uint4 b) {
   d[0] = __uint_as_float(a.x); d[1] = __uint_as_float(a.y); d[2] = __uint_as_float(a.z); d[3] = __uint_as_float(a.w);
   d[4] = __uint_as_float(b.x); d[5] = __uint_as_float(b.y); d[6] = __uint_as_float(b.z); d[7] = __uint_as_float(b.w);
 }
-__device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, uint32_t tid, SorRows R) {
+// LDS image: sp the 12^3 pressure window; srs the 10^3 window of (rhs bits, stencil byte)
+__device__ __forceinline__ void sor_store(float* sp, uint2* srs, uint32_t tid, SorRows R) {
   if (tid < 144u) {
     float* d = sp + 12u * tid;
     d[0] = __uint_as_float(R.q2.x);
@@ -803,27 +806,27 @@ __device__ __forceinline__ void sor_store(float* sp, float* sr, uint32_t* ss, ui
     d[10] = __uint_as_float(R.q2.z);
     d[11] = __uint_as_float(R.q2.w);
   } else if (tid < 244u) {
-    uint32_t r = tid - 144u;
-    float* d = sr + 10u * r;
-    uint32_t* e = ss + 10u * r;
-    d[0] = __uint_as_float(R.q2.x);
-    sor_row8(d + 1, R.q0, R.q1);
-    d[9] = __uint_as_float(R.q2.y);
-    e[0] = R.e & 0xffu;
-    e[1] = R.q2.z & 0xffu; e[2] = (R.q2.z >> 8) & 0xffu; e[3] = (R.q2.z >> 16) & 0xffu; e[4] = R.q2.z >> 24;
-    e[5] = R.q2.w & 0xffu; e[6] = (R.q2.w >> 8) & 0xffu; e[7] = (R.q2.w >> 16) & 0xffu; e[8] = R.q2.w >> 24;
-    e[9] = R.e >> 8;
+    uint2* d = srs + 10u * (tid - 144u);
+    d[0] = make_uint2(R.q2.x, R.e & 0xffu);
+    d[1] = make_uint2(R.q0.x, R.q2.z & 0xffu);
+    d[2] = make_uint2(R.q0.y, (R.q2.z >> 8) & 0xffu);
+    d[3] = make_uint2(R.q0.z, (R.q2.z >> 16) & 0xffu);
+    d[4] = make_uint2(R.q0.w, R.q2.z >> 24);
+    d[5] = make_uint2(R.q1.x, R.q2.w & 0xffu);
+    d[6] = make_uint2(R.q1.y, (R.q2.w >> 8) & 0xffu);
+    d[7] = make_uint2(R.q1.z, (R.q2.w >> 16) & 0xffu);
+    d[8] = make_uint2(R.q1.w, R.q2.w >> 24);
+    d[9] = make_uint2(R.q2.y, R.e >> 8);
   }
 }
 
 __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
                                                  float* __restrict__ pout) {
   __shared__ float sp[SW3_1];
-  __shared__ float sr[WIN3];
-  __shared__ uint32_t ss[WIN3];
+  __shared__ uint2 srs[WIN3];
   __shared__ uint32_t wsl[2][27];
-  __shared__ uint32_t redIdx[500];  // red cells of the 10^3 region: 12^3 index | 10^3 index << 16
-  __shared__ float ytab[8];         // 1 / n, correctly rounded
+  __shared__ uint32_t redIdx[SOR_NRED];  // red cells relaxed per tile: 12^3 index | 10^3 index << 16
+  __shared__ float ytab[8];              // 1 / n, correctly rounded
   const float omega = PP->omega;
   const uint32_t tid = threadIdx.x;
   // Workgroup w runs on XCD w % 8.  Each XCD walks one contiguous eighth of the
@@ -835,11 +838,23 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
   const uint32_t sc = min(nact, (xcd + 1u) * per);
   uint32_t b = xcd * per + (blockIdx.x >> 3);
   if (b >= sc) return;
-  for (uint32_t m = tid; m < 500u; m += WG) {
-    // window parity = global parity (the window origin 8t - 2 is even)
-    uint32_t row = m / 5u, wy = 1u + row % 10u, wz = 1u + row / 10u;
-    uint32_t wx = 1u + 2u * (m % 5u) + ((wy + wz + 1u) & 1u);
-    redIdx[m] = (wx + 12u * (wy + 12u * wz)) | (((wx - 1u) + 10u * ((wy - 1u) + 10u * (wz - 1u))) << 16);
+  // The black sweep reads red cells of the tile and of the six one-cell slabs
+  // across its faces (never edges or corners): 256 + 6 x 32 red cells.  Tile
+  // coordinates (x, y, z) in [-1, 8]; red means x + y + z even (the tile origin
+  // is even).
+  for (uint32_t m = tid; m < SOR_NRED; m += WG) {
+    int32_t x, y, z;
+    if (m < 256u) {
+      y = int32_t((m >> 2) & 7u); z = int32_t(m >> 5);
+      x = 2 * int32_t(m & 3u) + ((y + z) & 1);
+    } else {
+      uint32_t f = (m - 256u) >> 5, j = (m - 256u) & 31u;
+      int32_t e = (f & 1u) ? 8 : -1, u = int32_t(j >> 2), v = 2 * int32_t(j & 3u);
+      if (f < 2u) { x = e; z = u; y = v + ((e + u) & 1); }
+      else if (f < 4u) { y = e; z = u; x = v + ((e + u) & 1); }
+      else { z = e; y = u; x = v + ((e + u) & 1); }
+    }
+    redIdx[m] = uint32_t((x + 2) + 12 * ((y + 2) + 12 * (z + 2))) | (uint32_t((x + 1) + 10 * ((y + 1) + 10 * (z + 1))) << 16);
   }
   if (tid < 8u) ytab[tid] = (tid >= 1u && tid <= 6u) ? 1.0f / float(tid) : 0.0f;
   // prologue: slots of tiles b and b + gs, window rows of tile b
@@ -855,22 +870,21 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
     const bool more = b + gs < sc;
     uint32_t ahead = 0u;
     if (tid < 27u && b + 2u * gs < sc) ahead = G.nbrL[(b + 2u * gs) * 27u + tid];
-    sor_store(sp, sr, ss, tid, R);
+    sor_store(sp, srs, tid, R);
     __syncthreads();
     if (more) R = sor_fetch(G, pin, wsl[cur ^ 1u], tid);
-    // red cells of the 10^3 region
-    for (uint32_t m = tid; m < 500u; m += WG) {
-      uint32_t e = redIdx[m], qr = e >> 16;
-      uint32_t st = ss[qr];
-      if (st & STEN_FLUID) relax_lds<12>(sp, int32_t(e & 0xffffu), st, sr[qr], omega, ytab);
+    // red cells the black sweep reads
+    for (uint32_t m = tid; m < SOR_NRED; m += WG) {
+      uint32_t e = redIdx[m];
+      uint2 rs = srs[e >> 16];
+      if (rs.y & STEN_FLUID) relax_lds<12>(sp, int32_t(e & 0xffffu), rs.y, __uint_as_float(rs.x), omega, ytab);
     }
     __syncthreads();
     {  // black cells of the tile, one per thread
       int32_t y = int32_t((tid >> 2) & 7u), z = int32_t(tid >> 5);
       int32_t x = 2 * int32_t(tid & 3u) + ((y + z + 1) & 1);
-      int32_t qr = (x + 1) + 10 * ((y + 1) + 10 * (z + 1));
-      uint32_t st = ss[qr];
-      if (st & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), st, sr[qr], omega, ytab);
+      uint2 rs = srs[(x + 1) + 10 * ((y + 1) + 10 * (z + 1))];
+      if (rs.y & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), rs.y, __uint_as_float(rs.x), omega, ytab);
     }
     __syncthreads();
     for (uint32_t q = tid; q < 512u; q += WG) {
