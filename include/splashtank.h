@@ -88,7 +88,13 @@ typedef struct st_params {
   uint32_t nfill;
   uint32_t nfv;
   float fills[64];
-  uint32_t reserved[4];
+  /* pressure solve: 0 (default) runs every iteration of a solve in one
+   * cooperative launch, with the pressure resident in LDS when the tiles in use
+   * fit and streaming tile windows otherwise; 1 one launch per red-black
+   * iteration; 2 the cooperative launch, always streaming.  All give the same
+   * bits.  No counterpart in the reference (an implementation choice here). */
+  uint32_t sor_mode;
+  uint32_t reserved[3];
 } st_params;
 
 /* Stages of one substep, named as in FlipSim.stages (sim.js:245-310). */

@@ -53,7 +53,7 @@ class StParams(C.Structure):
                 ("no_cull", C.c_uint32), ("fast_air", C.c_uint32), ("no_solid_cache", C.c_uint32),
                 ("nb_on", C.c_uint32), ("nb_band", C.c_float), ("only_tiles", C.c_uint32),
                 ("nb_init", C.c_uint32), ("nfill", C.c_uint32), ("nfv", C.c_uint32),
-                ("fills", C.c_float * 64), ("reserved", C.c_uint32 * 4)]
+                ("fills", C.c_float * 64), ("sor_mode", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 def make_opts(d, size=(3.0, 2.6, 3.0), ratio=2.0, open_=0, max_n=1, slots=0, device=0) -> StOpts:

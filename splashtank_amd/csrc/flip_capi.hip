@@ -84,6 +84,9 @@ struct Sim {
   int iters = 1;
   uint32_t lcap = 0;
   uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
+  uint32_t coopGrid = 0;        // resident-SOR workgroups (0: not available)
+  uint32_t sorSeq = 0;          // pressure solves so far
+  uint32_t sorMode = 0;         // this step's st_params.sor_mode
   // need, binCount and binKey already hold the marks and bins of A's positions
   // (the last g2p made them and nothing has moved a particle since)
   bool binned = false;
@@ -100,6 +103,9 @@ struct Sim {
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
     L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, sorResident);  // whole groups of 8 (one per XCD)
+    L.coopGrid = sorMode == 1u ? 0u : coopGrid;
+    L.sorStream = sorMode == 2u;
+    L.sorSeq = sorSeq;
     L.binned = binned;
     L.binNext = !nbOn;  // the narrow band's reseed moves particles after g2p
     L.nbOn = nbOn;
@@ -125,6 +131,8 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.slotList, n))) return rc;
   if ((rc = dalloc(&s->G.nbr, size_t(n) * 27))) return rc;
   if ((rc = dalloc(&s->G.nbrL, size_t(n) * 27))) return rc;
+  if ((rc = dalloc(&s->G.coopList, n))) return rc;
+  if ((rc = dalloc(&s->G.sorHalo, size_t(n) * 2 * 6 * 32))) return rc;
   return ST_OK;
 }
 
@@ -132,6 +140,7 @@ void freeSlotArrays(DGrid& g) {
   dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.pres2); dfree(g.rhs); dfree(g.sten);
   dfree(g.accV); dfree(g.accW); dfree(g.velG);
   dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr); dfree(g.nbrL);
+  dfree(g.coopList); dfree(g.sorHalo);
 }
 void freeSlots(Sim* s) { freeSlotArrays(s->G); }
 
@@ -180,6 +189,7 @@ void destroySim(Sim* s) {
   dfree(s->G.pos); dfree(s->G.vel); dfree(s->G.pid); dfree(s->G.posB); dfree(s->G.velB); dfree(s->G.pidB); dfree(s->G.binIdx);
   dfree(s->G.binKey); dfree(s->G.binRank); dfree(s->G.binCount); dfree(s->G.binStart); dfree(s->G.scanTmp);
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part); dfree(s->G.foamTab);
+  dfree(s->G.mortonTiles); dfree(s->G.coopFlags); dfree(s->G.coopOffs); dfree(s->G.sorBar);
   dfree(s->G.scratch4);
   freeSlots(s);
   dfree(s->dSdf);
@@ -224,6 +234,7 @@ int beginStep(Sim* s, const st_params* o) {
     }
   }
   s->onlyTiles = o->only_tiles != 0;
+  s->sorMode = o->sor_mode;
   DParams& P = s->hp;
   std::memset(&P, 0, sizeof P);
   P.n = s->N; P.nx = uint32_t(s->nx); P.ny = uint32_t(s->ny); P.nz = uint32_t(s->nz); P.ncols = nc; P.open = s->open; P.ncell = s->ncell; P.scap = s->scap;
@@ -286,7 +297,12 @@ int runStage(Sim* s, int stage) {
       break;
     case ST_STAGE_NORMFACES: if (full) launch_norm_faces(st, L); break;
     case ST_STAGE_DIVERGENCE: if (full) launch_divergence(st, L); break;
-    case ST_STAGE_PRESSURE: if (full) launch_pressure(st, L, s->iters); break;
+    case ST_STAGE_PRESSURE:
+      if (full) {
+        L.sorSeq = ++s->sorSeq;
+        if (launch_pressure(st, L, s->iters) != hipSuccess) s->coopGrid = 0;  // per-iteration launches from now on
+      }
+      break;
     case ST_STAGE_PROJECT: if (full) launch_project(st, L); break;
     case ST_STAGE_G2P:
       if (full) {
@@ -350,6 +366,7 @@ int st_create(const st_opts* o, void** out) {
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o->device));
     s->sorResident = sor_resident_grid(uint32_t(cus));
+    s->coopGrid = sor_coop_grid(uint32_t(cus));
   }
   double rt = o->ratio > 0 ? o->ratio : 2.0;
   rt = std::max(1.4, std::min(4.0, rt));
@@ -385,6 +402,8 @@ int st_create(const st_opts* o, void** out) {
       (rc = dalloc(&s->G.tmap, s->ntiles)) || (rc = dalloc(&s->G.need, s->ntiles)) || (rc = dalloc(&s->G.newList, size_t(s->ntiles) * 2)) ||
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
       (rc = dalloc(&s->G.part, 2 * 8192 + 64)) || (rc = dalloc(&s->G.foamTab, FOAM_TAB)) ||
+      (rc = dalloc(&s->G.mortonTiles, s->ntiles)) || (rc = dalloc(&s->G.coopFlags, s->ntiles)) ||
+      (rc = dalloc(&s->G.coopOffs, size_t(s->ntiles) + 1)) || (rc = dalloc(&s->G.sorBar, COOP_MAX_WG + 3)) ||
       (rc = dalloc(&s->dSlots, RING))) {
     destroySim(s);
     return rc;
@@ -405,9 +424,13 @@ int st_create(const st_opts* o, void** out) {
   std::vector<uint32_t> st(s->scap, 0u);
   for (uint32_t k = 0; k + 1 < s->scap; k++) st[k] = s->scap - 1 - k;
   std::vector<uint32_t> tm(s->ntiles, ST_NONE);
+  std::vector<uint32_t> mort(s->ntiles);
+  morton_tiles(s->tx, s->ty, s->tz, mort.data());
   uint32_t ctl[16] = {0, 0, s->scap - 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (hipMemcpy(s->G.fstack, st.data(), size_t(s->scap) * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->G.tmap, tm.data(), size_t(s->ntiles) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->G.mortonTiles, mort.data(), size_t(s->ntiles) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(s->G.sorBar, 0, (COOP_MAX_WG + 3) * 4) != hipSuccess ||
       hipMemcpy(s->G.ctl, ctl, 64, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(s->G.pos, 0, M * 16) != hipSuccess || hipMemset(s->G.vel, 0, M * 16) != hipSuccess ||
       hipMemset(s->G.need, 0, size_t(s->ntiles) * 4) != hipSuccess) {
@@ -610,7 +633,7 @@ int st_grow(void* p, uint32_t n) {
   DGrid keep = s->G;
   s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.pres2 = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
   s->G.accV = nullptr; s->G.accW = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
-  s->G.slotList = nullptr; s->G.nbr = nullptr; s->G.nbrL = nullptr;
+  s->G.slotList = nullptr; s->G.nbr = nullptr; s->G.nbrL = nullptr; s->G.coopList = nullptr; s->G.sorHalo = nullptr;
   int rc = allocSlots(s, n);
   if (rc) { freeSlots(s); s->G = keep; return rc; }
   // the wg scratch and grid sizes follow the pool
@@ -724,6 +747,9 @@ int st_sync(void* p) {
   if (!s) return fail(ST_ERR_ARG, "st_sync: null sim");
   HIPCHK(hipSetDevice(s->dev));
   HIPCHK(hipStreamSynchronize(s->st));
+  uint32_t timeouts = 0;
+  HIPCHK(hipMemcpy(&timeouts, s->G.sorBar + COOP_MAX_WG + 1, 4, hipMemcpyDeviceToHost));
+  if (timeouts) return fail(ST_ERR_HIP, "resident SOR: a grid barrier timed out (pressure of that solve is not valid)");
   return ST_OK;
 }
 

@@ -79,6 +79,13 @@ struct DGrid {
   const DCols* cols;
   float4* scratch4;    // readback scratch (id order)
   float4* foamTab;     // per step, for a density sum c < FOAM_TAB: the g2p factors of c (d_foamRow)
+  // resident SOR
+  uint32_t* mortonTiles; // tile indices in Morton (z-order) order of their tile coordinates
+  uint32_t* coopFlags;   // per Morton position: 1 when the tile is in use
+  uint32_t* coopOffs;    // exclusive prefix of coopFlags
+  uint32_t* coopList;    // slots in use, in Morton order (per slot)
+  float* sorHalo;        // per colour, slot and face: the 32 face cells of that colour (2 x slots x 6 x 32)
+  uint32_t* sorBar;      // per resident-SOR workgroup: the last barrier epoch it reached
 };
 
 // ---------------------------------------------------------------------------

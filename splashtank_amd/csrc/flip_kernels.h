@@ -9,6 +9,14 @@
 // entries of the per-step foam table (density sums 0 .. FOAM_TAB - 1)
 #define FOAM_TAB 1024
 
+// resident SOR: workgroup size, tiles resident per workgroup, and the largest
+// grid its barrier serves (sorBar holds COOP_MAX_WG epochs, then the sequence
+// number of the last solve it finished, a count of barrier time-outs and the
+// barrier's release word)
+#define COOP_WG 1024
+#define COOP_TMAX 56
+#define COOP_MAX_WG 1024
+
 struct LaunchCtx {
   const DParams* P;
   const DCols* C;
@@ -19,6 +27,9 @@ struct LaunchCtx {
   uint32_t cellGrid;  // workgroups of grid-stride loops over active cells
   uint32_t tileGrid;  // workgroups of loops with one workgroup per tile in use
   uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
+  uint32_t coopGrid;  // resident-SOR workgroups (one per CU), 0 when cooperative launches are unavailable
+  uint32_t sorSeq;    // sequence number of this pressure solve
+  bool sorStream;     // the cooperative solve streams tile windows even when the tiles fit in LDS
   bool binned;        // blocks: need, binCount and binKey already hold A's marks and bins
   bool binNext;       // g2p: mark and bin the positions it writes, for the next substep
   bool nbOn;          // narrow band on this step
@@ -48,10 +59,16 @@ void launch_narrowband(hipStream_t st, const LaunchCtx& L);
 void launch_reseed(hipStream_t st, const LaunchCtx& L, const NbScratch& S);
 void launch_norm_faces(hipStream_t st, const LaunchCtx& L);
 void launch_divergence(hipStream_t st, const LaunchCtx& L);
-void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters);
+// returns the error of the resident-SOR cooperative launch (the per-iteration
+// launches still solve when it failed)
+hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters);
 // workgroups of the persistent SOR launch that fit on `cus` CUs at once, a
 // multiple of 8 (the XCD count)
 uint32_t sor_resident_grid(uint32_t cus);
+// workgroups of the resident SOR (one per CU when it fits there), 0 if it does not
+uint32_t sor_coop_grid(uint32_t cus);
+// tile indices in Morton order of their (ti, tj, tk) coordinates
+void morton_tiles(uint32_t tx, uint32_t ty, uint32_t tz, uint32_t* out);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);

@@ -17,6 +17,10 @@
 #include "flip_device.h"
 #include "flip_kernels.h"
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #define WG 256
 #define WIN 10      // window edge: tile of 8 plus one cell each side
 #define WIN3 1000
@@ -820,24 +824,17 @@ __device__ __forceinline__ void sor_store(float* sp, uint2* srs, uint32_t tid, S
   }
 }
 
-__global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
-                                                 float* __restrict__ pout) {
-  __shared__ float sp[SW3_1];
-  __shared__ uint2 srs[WIN3];
-  __shared__ uint32_t wsl[2][27];
-  __shared__ uint32_t redIdx[SOR_NRED];  // red cells relaxed per tile: 12^3 index | 10^3 index << 16
-  __shared__ float ytab[8];              // 1 / n, correctly rounded
-  const float omega = PP->omega;
-  const uint32_t tid = threadIdx.x;
-  // Workgroup w runs on XCD w % 8.  Each XCD walks one contiguous eighth of the
-  // tile-ordered active list, so x and y neighbours of a tile are processed by
-  // the same XCD at about the same time and their halo rows hit its L2.
-  // The grid is a multiple of 8 workgroups (host side).
-  const uint32_t nact = G.ctl[4], gs = gridDim.x >> 3, xcd = blockIdx.x & 7u;
-  const uint32_t per = (nact + 7u) >> 3;
-  const uint32_t sc = min(nact, (xcd + 1u) * per);
-  uint32_t b = xcd * per + (blockIdx.x >> 3);
-  if (b >= sc) return;
+// LDS of one tile pipeline (256 threads)
+struct SorPipeLds {
+  float sp[SW3_1];
+  uint2 srs[WIN3];
+  uint32_t wsl[2][27];
+  uint32_t redIdx[SOR_NRED];  // red cells relaxed per tile: 12^3 index | 10^3 index << 16
+  float ytab[8];              // 1 / n, correctly rounded
+};
+
+// The tables of the pipeline (tid: 0..255 within its group of 256 threads).
+__device__ __forceinline__ void sor_pipe_init(SorPipeLds& L, uint32_t tid) {
   // The black sweep reads red cells of the tile and of the six one-cell slabs
   // across its faces (never edges or corners): 256 + 6 x 32 red cells.  Tile
   // coordinates (x, y, z) in [-1, 8]; red means x + y + z even (the tile origin
@@ -854,51 +851,396 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
       else if (f < 4u) { y = e; z = u; x = v + ((e + u) & 1); }
       else { z = e; y = u; x = v + ((e + u) & 1); }
     }
-    redIdx[m] = uint32_t((x + 2) + 12 * ((y + 2) + 12 * (z + 2))) | (uint32_t((x + 1) + 10 * ((y + 1) + 10 * (z + 1))) << 16);
+    L.redIdx[m] = uint32_t((x + 2) + 12 * ((y + 2) + 12 * (z + 2))) | (uint32_t((x + 1) + 10 * ((y + 1) + 10 * (z + 1))) << 16);
   }
-  if (tid < 8u) ytab[tid] = (tid >= 1u && tid <= 6u) ? 1.0f / float(tid) : 0.0f;
+  if (tid < 8u) L.ytab[tid] = (tid >= 1u && tid <= 6u) ? 1.0f / float(tid) : 0.0f;
+}
+
+// One red-black iteration over the tiles of virtual workgroup `vid` of `vgrid`
+// (a multiple of 8; vid runs on XCD vid % 8).  Each XCD walks one contiguous
+// eighth of the tile-ordered active list, so x and y neighbours of a tile are
+// processed by the same XCD at about the same time and their halo rows hit its
+// L2.  The loop runs `kmax` rounds for every group of the workgroup (so all
+// groups meet at each __syncthreads); a group past its share idles.
+__device__ __forceinline__ void sor_pipe_pass(const DGrid& G, float omega, const float* __restrict__ pin, float* __restrict__ pout,
+                                              uint32_t vid, uint32_t vgrid, uint32_t tid, SorPipeLds& L) {
+  const uint32_t nact = G.ctl[4], gs = vgrid >> 3, xcd = vid & 7u;
+  const uint32_t per = (nact + 7u) >> 3;
+  const uint32_t sc = min(nact, (xcd + 1u) * per);
+  const uint32_t kmax = (per + gs - 1u) / gs;
+  uint32_t b = xcd * per + (vid >> 3);
   // prologue: slots of tiles b and b + gs, window rows of tile b
   if (tid < 27u) {
-    wsl[0][tid] = G.nbrL[b * 27u + tid];
-    wsl[1][tid] = b + gs < sc ? G.nbrL[(b + gs) * 27u + tid] : 0u;
+    L.wsl[0][tid] = b < sc ? G.nbrL[b * 27u + tid] : 0u;
+    L.wsl[1][tid] = b + gs < sc ? G.nbrL[(b + gs) * 27u + tid] : 0u;
   }
   __syncthreads();
-  SorRows R = sor_fetch(G, pin, wsl[0], tid);
+  SorRows R{};
+  if (b < sc) R = sor_fetch(G, pin, L.wsl[0], tid);
   uint32_t cur = 0;
-  for (; b < sc; b += gs, cur ^= 1u) {
-    const uint32_t s = wsl[cur][13];
-    const bool more = b + gs < sc;
+  for (uint32_t k = 0; k < kmax; k++, b += gs, cur ^= 1u) {
+    const bool act = b < sc, more = b + gs < sc;
+    const uint32_t s = L.wsl[cur][13];
     uint32_t ahead = 0u;
     if (tid < 27u && b + 2u * gs < sc) ahead = G.nbrL[(b + 2u * gs) * 27u + tid];
-    sor_store(sp, srs, tid, R);
+    if (act) sor_store(L.sp, L.srs, tid, R);
     __syncthreads();
-    if (more) R = sor_fetch(G, pin, wsl[cur ^ 1u], tid);
-    // red cells the black sweep reads
-    for (uint32_t m = tid; m < SOR_NRED; m += WG) {
-      uint32_t e = redIdx[m];
-      uint2 rs = srs[e >> 16];
-      if (rs.y & STEN_FLUID) relax_lds<12>(sp, int32_t(e & 0xffffu), rs.y, __uint_as_float(rs.x), omega, ytab);
+    if (more) R = sor_fetch(G, pin, L.wsl[cur ^ 1u], tid);
+    if (act) {  // red cells the black sweep reads
+      for (uint32_t m = tid; m < SOR_NRED; m += WG) {
+        uint32_t e = L.redIdx[m];
+        uint2 rs = L.srs[e >> 16];
+        if (rs.y & STEN_FLUID) relax_lds<12>(L.sp, int32_t(e & 0xffffu), rs.y, __uint_as_float(rs.x), omega, L.ytab);
+      }
     }
     __syncthreads();
-    {  // black cells of the tile, one per thread
+    if (act) {  // black cells of the tile, one per thread
       int32_t y = int32_t((tid >> 2) & 7u), z = int32_t(tid >> 5);
       int32_t x = 2 * int32_t(tid & 3u) + ((y + z + 1) & 1);
-      uint2 rs = srs[(x + 1) + 10 * ((y + 1) + 10 * (z + 1))];
-      if (rs.y & STEN_FLUID) relax_lds<12>(sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), rs.y, __uint_as_float(rs.x), omega, ytab);
+      uint2 rs = L.srs[(x + 1) + 10 * ((y + 1) + 10 * (z + 1))];
+      if (rs.y & STEN_FLUID) relax_lds<12>(L.sp, (x + 2) + 12 * ((y + 2) + 12 * (z + 2)), rs.y, __uint_as_float(rs.x), omega, L.ytab);
     }
     __syncthreads();
-    for (uint32_t q = tid; q < 512u; q += WG) {
-      uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
-      pout[s * 512u + q] = sp[(x + 2) + 12 * ((y + 2) + 12 * (z + 2))];
+    if (act) {
+      for (uint32_t q = tid; q < 512u; q += WG) {
+        uint32_t x = q & 7u, y = (q >> 3) & 7u, z = q >> 6;
+        pout[s * 512u + q] = L.sp[(x + 2) + 12 * ((y + 2) + 12 * (z + 2))];
+      }
     }
     // slots of tile b + 2gs go where tile b's were (read above into s)
-    if (tid < 27u) wsl[cur][tid] = ahead;
+    if (tid < 27u) L.wsl[cur][tid] = ahead;
     __syncthreads();
   }
 }
 
+// One red-black iteration per launch (the grid a multiple of 8 workgroups, as
+// many as are resident), skipped when the resident solve has done this stage.
+__global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
+                                                 float* __restrict__ pout, uint32_t seq) {
+  __shared__ SorPipeLds L;
+  if (G.sorBar[COOP_MAX_WG] == seq) return;
+  sor_pipe_init(L, threadIdx.x);
+  sor_pipe_pass(G, PP->omega, pin, pout, blockIdx.x, gridDim.x, threadIdx.x, L);
+}
+
+// ---------------------------------------------------------------------------
+// Resident SOR: every iteration of one pressure solve in one cooperative launch,
+// one workgroup of COOP_WG threads per CU.  Workgroup w owns the w-th contiguous
+// range of the active tiles in Morton order (compact blobs of about nact / grid
+// tiles) and keeps, for the whole solve, their pressure in LDS, split by colour
+// (sP), and the rhs and stencil of the cells each thread relaxes in registers.
+// Only the face layers cross workgroups: after relaxing colour c a workgroup
+// writes the colour-c cells of every face whose neighbour tile another
+// workgroup owns to sorHalo[c], all workgroups meet at a grid barrier, and each
+// copies the colour-c cells across all six faces of its tiles into sH (from its
+// own LDS, from sorHalo, or zero where there is no tile) before relaxing the
+// other colour.  The arithmetic is relax_lds's, cell for cell, so the iterate is
+// the reference's bit for bit.  When the active tiles exceed COOP_TMAX per
+// workgroup the same launch streams tile windows instead (below).
+
+// index of tile-local cell (x, y, z) in the colour-split layout: colour
+// (x + y + z) & 1 (the tile origin is even, so this is the global colour), then
+// (z, y, x / 2)
+__device__ __forceinline__ uint32_t coop_cidx(int32_t x, int32_t y, int32_t z) {
+  return (uint32_t(x + y + z) & 1u) * 256u + uint32_t(z * 8 + y) * 4u + uint32_t(x >> 1);
+}
+// faces 0..5 = x-, x+, y-, y+, z-, z+; the nbr column of the tile across face f
+__device__ __forceinline__ uint32_t coop_faceq(uint32_t f) {
+  return f == 0u ? 12u : f == 1u ? 14u : f == 2u ? 10u : f == 3u ? 16u : f == 4u ? 4u : 22u;
+}
+// cell e (0..31) of colour c on tile-local face f: (u, v) are the in-plane
+// coordinates in axis order, e = 4 v + u / 2, and the normal coordinate is 0 on
+// a low face and 7 on a high one
+__device__ __forceinline__ int3 coop_face_cell(uint32_t f, uint32_t e, uint32_t c) {
+  int32_t n0 = (f & 1u) ? 7 : 0, v = int32_t(e >> 2);
+  int32_t u = 2 * int32_t(e & 3u) + ((int32_t(c) + n0 + v) & 1);
+  uint32_t a = f >> 1;
+  return a == 0u ? make_int3(n0, u, v) : (a == 1u ? make_int3(u, n0, v) : make_int3(u, v, n0));
+}
+
+#define COOP_NB_NONE 0xffffffffu
+#define COOP_NB_FOREIGN 0x80000000u
+#define COOP_JMAX (COOP_TMAX * 256 / COOP_WG)
+#define COOP_SPIN_LIMIT (1u << 21)
+
+// Grid barrier over the resident SOR's workgroups.  The halo cells cross it
+// through agent-scope (coherent) stores and loads.  Every wave waits until its
+// stores are done; workgroup w raises its flag sorBar[w] to `epoch` after an
+// agent-scope release fence; workgroup 0 waits for every flag (one thread per
+// flag) and raises the release word sorBar[COOP_MAX_WG + 2]; every other
+// workgroup waits for that word with a single thread; an agent-scope acquire
+// fence follows.  Each wait gives up after COOP_SPIN_LIMIT polls, so every wave
+// always leaves; a time-out is counted in sorBar[COOP_MAX_WG + 1] and returns
+// false.
+__device__ __forceinline__ bool coop_wait(const uint32_t* word, uint32_t epoch, uint32_t* sFail) {
+  uint32_t spins = 0;
+  while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+    if (++spins > COOP_SPIN_LIMIT) {
+      *sFail = 1u;
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+__device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
+  uint32_t* rel = G.sorBar + COOP_MAX_WG + 2;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(&G.sorBar[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < gridDim.x) coop_wait(&G.sorBar[threadIdx.x], epoch, sFail);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_store(rel, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else if (threadIdx.x == 0) {
+    coop_wait(rel, epoch, sFail);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+  if (*sFail) {
+    if (threadIdx.x == 0) atomicAdd(&G.sorBar[COOP_MAX_WG + 1], 1u);
+    return false;
+  }
+  return true;
+}
+
+// sH for relaxing the colour opposite c: the colour-c cells across each face,
+// copied in LDS from our own tiles, zero where there is no tile, and fetched
+// for the faces other workgroups own (listed in sFor, nFor of them; a thread's
+// fetches are in flight together).
+#define COOP_FOR_PER_THREAD 4
+__device__ __forceinline__ void coop_fill_halo(const DGrid& G, uint32_t scap, const float* __restrict__ pin, const float* sP,
+                                               float* sH, const uint32_t* sNb, const uint32_t* sFor, uint32_t nFor, uint32_t T,
+                                               uint32_t c, bool fromPres) {
+  for (uint32_t q = threadIdx.x; q < T * 192u; q += COOP_WG) {
+    uint32_t lt = q / 192u, r = q - lt * 192u, f = r >> 5, e = r & 31u;
+    uint32_t nb = sNb[lt * 6u + f];
+    if (nb & COOP_NB_FOREIGN) {
+      if (nb == COOP_NB_NONE) sH[q] = 0.0f;
+    } else {
+      int3 a = coop_face_cell(f ^ 1u, e, c);
+      sH[q] = sP[nb * 512u + coop_cidx(a.x, a.y, a.z)];
+    }
+  }
+  for (uint32_t q0 = 0; q0 < nFor * 32u; q0 += COOP_FOR_PER_THREAD * COOP_WG) {
+    float v[COOP_FOR_PER_THREAD];
+#pragma unroll
+    for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) {
+      uint32_t q = q0 + threadIdx.x + i * COOP_WG;
+      v[i] = 0.0f;
+      if (q < nFor * 32u) {
+        uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
+        uint32_t ns = sNb[hf] & ~COOP_NB_FOREIGN;
+        if (fromPres) {
+          int3 a = coop_face_cell(f ^ 1u, e, c);
+          v[i] = pin[ns * 512u + uint32_t(a.x + 8 * (a.y + 8 * a.z))];
+        } else {
+          v[i] = __hip_atomic_load(&G.sorHalo[((size_t(c) * scap + ns) * 6u + (f ^ 1u)) * 32u + e], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) {
+      uint32_t q = q0 + threadIdx.x + i * COOP_WG;
+      if (q < nFor * 32u) sH[sFor[q >> 5] * 32u + (q & 31u)] = v[i];
+    }
+  }
+}
+
+// LDS of the resident mode
+struct CoopLds {
+  float sP[COOP_TMAX * 512];
+  float sH[COOP_TMAX * 192];
+  uint32_t sNb[COOP_TMAX * 6];
+  uint32_t sSlot[COOP_TMAX];
+  uint32_t sFor[COOP_TMAX * 6];  // (tile, face) pairs whose neighbour another workgroup owns
+};
+constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
+
+__global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
+                                                          uint32_t seq, uint32_t stream) {
+  __shared__ __align__(16) unsigned char smem[kCoopLdsBytes];
+  __shared__ uint32_t nFor;
+  __shared__ float ytab[8];
+  __shared__ uint32_t sFail;
+  const uint32_t nact = G.ctl[4], ng = gridDim.x, tid = threadIdx.x, w = blockIdx.x;
+  if (nact == 0u || ng > COOP_MAX_WG) return;
+  const float omega = PP->omega;
+  if (tid == 0u) sFail = 0u;
+  if (stream || nact > ng * COOP_TMAX) {
+    // Streaming mode, for more tiles than fit in LDS: four tile pipelines of 256
+    // threads per workgroup run the per-iteration pass (as k_sor_pipe, virtual
+    // workgroup g * grid + w), with a grid barrier between iterations.
+    const uint32_t g = tid >> 8, gt = tid & 255u;
+    SorPipeLds& L = reinterpret_cast<SorPipeLds*>(smem)[g];
+    sor_pipe_init(L, gt);
+    uint32_t epoch = 0;
+    for (uint32_t it = 0; it < iters; it++) {
+      const float* pin = (it & 1u) ? G.pres2 : G.pres;
+      float* pout = (it & 1u) ? G.pres : G.pres2;
+      sor_pipe_pass(G, omega, pin, pout, g * ng + w, 4u * ng, gt, L);
+      if (it + 1u < iters && !coop_barrier(G, ++epoch, &sFail)) return;
+    }
+    if (iters & 1u) {  // the result sits in pres2: copy back the tiles this pipeline wrote
+      const uint32_t vid = g * ng + w, gs = (4u * ng) >> 3, xcd = vid & 7u, per = (nact + 7u) >> 3;
+      const uint32_t sc = min(nact, (xcd + 1u) * per);
+      for (uint32_t b = xcd * per + (vid >> 3); b < sc; b += gs) {
+        uint32_t sl = G.slotList[b];
+        for (uint32_t q = gt; q < 512u; q += WG) G.pres[sl * 512u + q] = G.pres2[sl * 512u + q];
+      }
+    }
+    if (w == 0u && tid == 0u) G.sorBar[COOP_MAX_WG] = seq;
+    return;
+  }
+  CoopLds& CL = *reinterpret_cast<CoopLds*>(smem);
+  float* sP = CL.sP;
+  float* sH = CL.sH;
+  uint32_t* sNb = CL.sNb;
+  uint32_t* sSlot = CL.sSlot;
+  uint32_t* sFor = CL.sFor;
+  const uint32_t scap = PP->scap;
+  const uint32_t beg = uint32_t(uint64_t(nact) * w / ng), T = uint32_t(uint64_t(nact) * (w + 1u) / ng) - beg;
+  float* pres = G.pres;
+  if (tid == 0u) nFor = 0u;
+  if (tid < 8u) ytab[tid] = (tid >= 1u && tid <= 6u) ? 1.0f / float(tid) : 0.0f;
+  if (tid < T) sSlot[tid] = G.coopList[beg + tid];
+  __syncthreads();
+  // the tile across each face: one of ours (its local index), another
+  // workgroup's (its slot), or none
+  for (uint32_t q = tid; q < T * 6u; q += COOP_WG) {
+    uint32_t lt = q / 6u, f = q - lt * 6u;
+    uint32_t ns = G.nbr[sSlot[lt] * 27u + coop_faceq(f)], code = COOP_NB_NONE;
+    if (!sor_missing(ns)) {
+      code = COOP_NB_FOREIGN | ns;
+      for (uint32_t j = 0; j < T; j++)
+        if (sSlot[j] == ns) code = j;
+    }
+    sNb[q] = code;
+  }
+  __syncthreads();
+  if (tid == 0u) {  // in (tile, face) order
+    uint32_t k = 0;
+    for (uint32_t q = 0; q < T * 6u; q++)
+      if (sNb[q] != COOP_NB_NONE && (sNb[q] & COOP_NB_FOREIGN)) sFor[k++] = q;
+    nFor = k;
+  }
+  for (uint32_t q = tid; q < T * 512u; q += COOP_WG) {
+    uint32_t lt = q >> 9, l = q & 511u;
+    sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))] = pres[sSlot[lt] * 512u + l];
+  }
+  // rhs and stencil of the cells this thread relaxes: colour c, cell k of tile
+  // lt for m = tid + j COOP_WG = 256 lt + k
+  float rr[2][COOP_JMAX];
+  uint32_t rs[2][(COOP_JMAX + 3) / 4];  // stencil bytes, four to a register
+#pragma unroll
+  for (uint32_t c = 0; c < 2u; c++) {
+#pragma unroll
+    for (uint32_t j = 0; j < (COOP_JMAX + 3) / 4; j++) rs[c][j] = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < COOP_JMAX; j++) {
+      uint32_t m = tid + j * COOP_WG;
+      rr[c][j] = 0.0f;
+      if (m < T * 256u) {
+        uint32_t lt = m >> 8, k = m & 255u;
+        int32_t z = int32_t(k >> 5), y = int32_t((k >> 2) & 7u), x = 2 * int32_t(k & 3u) + ((y + z + int32_t(c)) & 1);
+        uint32_t cell = sSlot[lt] * 512u + uint32_t(x + 8 * (y + 8 * z));
+        rr[c][j] = G.rhs[cell];
+        rs[c][j >> 2] |= uint32_t(G.sten[cell]) << (8u * (j & 3u));
+      }
+    }
+  }
+  __syncthreads();
+  coop_fill_halo(G, scap, pres, sP, sH, sNb, sFor, nFor, T, 1u, true);
+  __syncthreads();
+  uint32_t epoch = 0;
+  for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+    for (uint32_t c = 0; c < 2u; c++) {
+      const uint32_t oc = c ^ 1u;
+      // the cell addresses are recomputed every sweep rather than kept in
+      // registers for the whole solve (an opaque copy of the thread index)
+      uint32_t tv = tid;
+      asm volatile("" : "+v"(tv));
+#pragma unroll
+      for (uint32_t j = 0; j < COOP_JMAX; j++) {
+        uint32_t m = tv + j * COOP_WG;
+        uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
+        if (m < T * 256u && (st & STEN_FLUID)) {
+          uint32_t lt = m >> 8, k = m & 255u;
+          int32_t z = int32_t(k >> 5), y = int32_t((k >> 2) & 7u), x = 2 * int32_t(k & 3u) + ((y + z + int32_t(c)) & 1);
+          float* cen = sP + lt * 512u + c * 256u + k;
+          uint32_t nns = st & 7u;
+          if (nns == 0u) {
+            *cen = 0.0f;
+          } else {
+            // neighbours of the other colour: inside the tile from ob (indices
+            // clamped into the tile, so both arms of each choice stay in LDS),
+            // across a face from the halo
+            const float* ob = sP + lt * 512u + oc * 256u;
+            const float* hb = sH + lt * 192u;
+            const int32_t xm = max(x - 1, 0), xp = min(x + 1, 7), ym = max(y - 1, 0), yp = min(y + 1, 7);
+            const int32_t zm = max(z - 1, 0), zp = min(z + 1, 7), xh = x >> 1;
+            float n0 = x > 0 ? ob[(z * 8 + y) * 4 + (xm >> 1)] : hb[0 * 32 + z * 4 + (y >> 1)];
+            float n1 = x < 7 ? ob[(z * 8 + y) * 4 + (xp >> 1)] : hb[1 * 32 + z * 4 + (y >> 1)];
+            float n2 = y > 0 ? ob[(z * 8 + ym) * 4 + xh] : hb[2 * 32 + z * 4 + xh];
+            float n3 = y < 7 ? ob[(z * 8 + yp) * 4 + xh] : hb[3 * 32 + z * 4 + xh];
+            float n4 = z > 0 ? ob[(zm * 8 + y) * 4 + xh] : hb[4 * 32 + y * 4 + xh];
+            float n5 = z < 7 ? ob[(zp * 8 + y) * 4 + xh] : hb[5 * 32 + y * 4 + xh];
+            float yv = ytab[nns], pc = *cen;
+            float sum = 0.0f;
+            sum = sum + n0;
+            sum = sum + n1;
+            sum = sum + n2;
+            sum = sum + n3;
+            sum = sum + n4;
+            sum = sum + n5;
+            float pg = st_div_small(rr[c][j] + sum, float(nns), yv);
+            *cen = pc + omega * (pg - pc);
+          }
+        }
+      }
+      __syncthreads();
+      if (it + 1u == iters && c == 1u) break;  // the last sweep: nothing more to exchange
+      // colour-c face cells other workgroups read
+      for (uint32_t q = tid; q < nFor * 32u; q += COOP_WG) {
+        uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
+        int3 a = coop_face_cell(f, e, c);
+        __hip_atomic_store(&G.sorHalo[((size_t(c) * scap + sSlot[lt]) * 6u + f) * 32u + e],
+                           sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (!coop_barrier(G, ++epoch, &sFail)) return;
+      coop_fill_halo(G, scap, pres, sP, sH, sNb, sFor, nFor, T, c, false);
+      __syncthreads();
+    }
+  }
+  for (uint32_t q = tid; q < T * 512u; q += COOP_WG) {
+    uint32_t lt = q >> 9, l = q & 511u;
+    pres[sSlot[lt] * 512u + l] = sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))];
+  }
+  if (w == 0u && tid == 0u) G.sorBar[COOP_MAX_WG] = seq;
+}
+
+// Morton-ordered list of the slots in use (coopList) for the resident SOR
+__global__ void k_coop_flags(const DParams* __restrict__ PP, DGrid G) {
+  uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < PP->ntiles) G.coopFlags[m] = G.tmap[G.mortonTiles[m]] != ST_NONE ? 1u : 0u;
+}
+__global__ void k_coop_compact(const DParams* __restrict__ PP, DGrid G) {
+  uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < PP->ntiles && G.coopFlags[m]) G.coopList[G.coopOffs[m]] = G.tmap[G.mortonTiles[m]];
+}
+
 // after an odd number of iterations the result sits in the partner buffer
-__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst) {
+__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq) {
+  if (G.sorBar[COOP_MAX_WG] == seq) return;
   uint32_t tot = G.ctl[4] * 512u;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
     uint32_t c = G.slotList[t >> 9] * 512u + (t & 511u);
@@ -1605,15 +1947,60 @@ uint32_t sor_resident_grid(uint32_t cus) {
   return g < 8u ? 8u : g;
 }
 
-void launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
+uint32_t sor_coop_grid(uint32_t cus) {
+  int dev = 0, coop = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
+    return 0u;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sor_resident, COOP_WG, 0) != hipSuccess || nb < 1) return 0u;
+  return std::min<uint32_t>(cus, COOP_MAX_WG);
+}
+
+void morton_tiles(uint32_t tx, uint32_t ty, uint32_t tz, uint32_t* out) {
+  auto spread = [](uint64_t v) {  // bit i of v to bit 3i
+    uint64_t r = 0;
+    for (int b = 0; b < 21; b++) r |= ((v >> b) & 1ull) << (3 * b);
+    return r;
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> key;
+  key.reserve(size_t(tx) * ty * tz);
+  for (uint32_t k = 0; k < tz; k++)
+    for (uint32_t j = 0; j < ty; j++)
+      for (uint32_t i = 0; i < tx; i++)
+        key.emplace_back(spread(i) | (spread(j) << 1) | (spread(k) << 2), i + tx * (j + ty * k));
+  std::sort(key.begin(), key.end());
+  for (size_t m = 0; m < key.size(); m++) out[m] = key[m].second;
+}
+
+hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
+  hipError_t coopErr = hipSuccess;
+  if (L.coopGrid && iters > 0) {
+    // active slots in Morton order, cleared barrier flags, then the whole solve
+    // in one cooperative launch (resident when the tiles fit in LDS, streaming
+    // tile windows otherwise)
+    uint32_t nt = L.ntiles, g = gridFor(nt, WG);
+    hipLaunchKernelGGL(k_coop_flags, dim3(g), dim3(WG), 0, st, L.P, L.G);
+    launch_scan(st, L.G.coopFlags, L.G.coopOffs, nt, L.G.scanTmp, nullptr);
+    hipLaunchKernelGGL(k_coop_compact, dim3(g), dim3(WG), 0, st, L.P, L.G);
+    launch_fill_u32(st, L.G.sorBar, COOP_MAX_WG, 0u);
+    launch_fill_u32(st, L.G.sorBar + COOP_MAX_WG + 2, 1, 0u);
+    const DParams* P = L.P;
+    DGrid G = L.G;
+    uint32_t it = uint32_t(iters), seq = L.sorSeq, stream = L.sorStream ? 1u : 0u;
+    void* args[] = {(void*)&P, (void*)&G, (void*)&it, (void*)&seq, (void*)&stream};
+    coopErr = hipLaunchCooperativeKernel((const void*)k_sor_resident, dim3(L.coopGrid), dim3(COOP_WG), args, 0, st);
+    if (coopErr == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();  // the per-iteration launches below do the solve
+  }
   // one iteration per launch, ping-pong between the two pressure buffers; an
-  // odd count leaves the result in pres2 and it is copied back
+  // odd count leaves the result in pres2 and it is copied back.
   for (int it = 0; it < iters; it++) {
     const float* pin = (it & 1) ? L.G.pres2 : L.G.pres;
     float* pout = (it & 1) ? L.G.pres : L.G.pres2;
-    hipLaunchKernelGGL(k_sor_pipe, dim3(L.pipeGrid), dim3(WG), 0, st, L.P, L.G, pin, pout);
+    hipLaunchKernelGGL(k_sor_pipe, dim3(L.pipeGrid), dim3(WG), 0, st, L.P, L.G, pin, pout, L.sorSeq);
   }
-  if (iters & 1) hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres);
+  if (iters & 1)
+    hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres, L.sorSeq);
+  return coopErr;
 }
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {

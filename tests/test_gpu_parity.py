@@ -109,6 +109,42 @@ def test_dam_break_64_ten_steps():
     assert_f32_close(pg, po, "positions after 10 substeps")
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["per_iteration", "streaming"])
+def test_dam_break_64_other_sor_paths(mode):
+    """the 64^3 case with one launch per red-black iteration, and with the
+    cooperative solve streaming tile windows, instead of the resident solve"""
+    sc = scenes.dam_break(d=1.0 / 128)
+    sc.params["sor_mode"] = mode
+    g, o, log = run_stagewise(sc, 3)
+    assert_f32_close(g.dump("pos"), o.dump("pos"), "positions after 3 substeps")
+
+
+def test_sor_paths_identical_128():
+    """At 128^3 (4.3M particles, too large for the oracle in a test) the three
+    pressure paths must give the same bits: the resident solve, one launch per
+    iteration, and the cooperative streaming solve, from the same state, over
+    five substeps."""
+    sc = scenes.sparse_splash(d=1.0 / 256, iters=80, dt=1.0 / 480)
+    out = []
+    for mode in (0, 1, 2):
+        g = FlipSimC(opts_for(sc))
+        upload_scene(g, sc)
+        prm = capi.make_params(**dict(sc.params, sor_mode=mode))
+        t = 0.0
+        for step in range(5):
+            t += sc.params["dt"]
+            set_colliders_at(g, sc, t)
+            print(f"[parity] sparse_splash 128^3 sor mode {mode} step {step}", flush=True)
+            g.step(prm)
+        g.sync()
+        out.append((g.dump("pres"), g.dump("pos"), g.dump("vel")))
+        g.close()
+    for other in (1, 2):
+        for a, b, name in zip(out[0], out[other], ("pres", "pos", "vel")):
+            assert a.shape == b.shape
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"{name}: SOR path {other} differs from the resident one"
+
+
 def _multiset(a, b):
     """rows of (pos, vel) sorted, so the same particles in any order compare equal"""
     ka = a[np.lexsort(a.T[::-1])]
