@@ -14,7 +14,7 @@
 // number of the last solve it finished, a count of barrier time-outs and the
 // barrier's release word)
 #define COOP_WG 1024
-#define COOP_TMAX 56
+#define COOP_TMAX 54
 #define COOP_MAX_WG 1024
 
 struct LaunchCtx {

@@ -960,7 +960,8 @@ __device__ __forceinline__ int3 coop_face_cell(uint32_t f, uint32_t e, uint32_t 
 
 #define COOP_NB_NONE 0xffffffffu
 #define COOP_NB_FOREIGN 0x80000000u
-#define COOP_JMAX (COOP_TMAX * 256 / COOP_WG)
+#define COOP_JS ((COOP_TMAX * COOP_NSHELL + COOP_WG - 1) / COOP_WG)
+#define COOP_JC ((COOP_TMAX * COOP_NCORE + COOP_WG - 1) / COOP_WG)
 #define COOP_SPIN_LIMIT (1u << 21)
 
 // Grid barrier over the resident SOR's workgroups.  The halo cells cross it
@@ -983,14 +984,16 @@ __device__ __forceinline__ bool coop_wait(const uint32_t* word, uint32_t epoch, 
   }
   return true;
 }
-__device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
-  uint32_t* rel = G.sorBar + COOP_MAX_WG + 2;
+__device__ __forceinline__ void coop_arrive(const DGrid& G, uint32_t epoch) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_store(&G.sorBar[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+__device__ __forceinline__ bool coop_depart(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
+  uint32_t* rel = G.sorBar + COOP_MAX_WG + 2;
   if (blockIdx.x == 0) {
     if (threadIdx.x < gridDim.x) coop_wait(&G.sorBar[threadIdx.x], epoch, sFail);
     __syncthreads();
@@ -1009,6 +1012,10 @@ __device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uin
     return false;
   }
   return true;
+}
+__device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
+  coop_arrive(G, epoch);
+  return coop_depart(G, epoch, sFail);
 }
 
 // sH for relaxing the colour opposite c: the colour-c cells across each face,
@@ -1055,6 +1062,76 @@ __device__ __forceinline__ void coop_fill_halo(const DGrid& G, uint32_t scap, co
   }
 }
 
+// Cells of one colour of a tile, as local indices x + 8y + 64z: the 148 on
+// the tile's faces (some coordinate 0 or 7) first, then the 108 inside.  The
+// face cells are relaxed first so their values can leave for the neighbouring
+// workgroups while the inside cells are relaxed.
+#define COOP_NSHELL 148
+#define COOP_NCORE 108
+struct CoopCellTable {
+  uint16_t v[2][256];
+};
+constexpr CoopCellTable coop_cell_table() {
+  CoopCellTable t{};
+  for (uint32_t c = 0; c < 2; c++) {
+    uint32_t ns = 0, nc = COOP_NSHELL;
+    for (uint32_t l = 0; l < 512; l++) {
+      uint32_t x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
+      if (((x + y + z) & 1u) != c) continue;
+      bool shell = x == 0 || x == 7 || y == 0 || y == 7 || z == 0 || z == 7;
+      t.v[c][shell ? ns++ : nc++] = uint16_t(l);
+    }
+  }
+  return t;
+}
+__constant__ CoopCellTable kCoopCells = coop_cell_table();
+
+// one relaxation of cell l (colour c) of resident tile lt; a face cell takes
+// its neighbours across the face from the halo sH, an inside cell has all six
+// in the tile
+template <bool FACE>
+__device__ __forceinline__ void coop_relax(float* sP, const float* sH, uint32_t lt, uint32_t l, uint32_t c, float rhs, uint32_t st,
+                                           float omega, const float* ytab) {
+  const int32_t x = int32_t(l & 7u), y = int32_t((l >> 3) & 7u), z = int32_t(l >> 6), xh = x >> 1;
+  float* cen = sP + lt * 512u + c * 256u + uint32_t((z * 8 + y) * 4 + xh);
+  const uint32_t nns = st & 7u;
+  if (nns == 0u) {
+    *cen = 0.0f;
+    return;
+  }
+  const float* ob = sP + lt * 512u + (c ^ 1u) * 256u;
+  float n0, n1, n2, n3, n4, n5;
+  if (FACE) {
+    // inside-the-tile indices are clamped, so both arms of each choice stay in LDS
+    const float* hb = sH + lt * 192u;
+    const int32_t xm = max(x - 1, 0), xp = min(x + 1, 7), ym = max(y - 1, 0), yp = min(y + 1, 7);
+    const int32_t zm = max(z - 1, 0), zp = min(z + 1, 7);
+    n0 = x > 0 ? ob[(z * 8 + y) * 4 + (xm >> 1)] : hb[0 * 32 + z * 4 + (y >> 1)];
+    n1 = x < 7 ? ob[(z * 8 + y) * 4 + (xp >> 1)] : hb[1 * 32 + z * 4 + (y >> 1)];
+    n2 = y > 0 ? ob[(z * 8 + ym) * 4 + xh] : hb[2 * 32 + z * 4 + xh];
+    n3 = y < 7 ? ob[(z * 8 + yp) * 4 + xh] : hb[3 * 32 + z * 4 + xh];
+    n4 = z > 0 ? ob[(zm * 8 + y) * 4 + xh] : hb[4 * 32 + y * 4 + xh];
+    n5 = z < 7 ? ob[(zp * 8 + y) * 4 + xh] : hb[5 * 32 + y * 4 + xh];
+  } else {
+    n0 = ob[(z * 8 + y) * 4 + ((x - 1) >> 1)];
+    n1 = ob[(z * 8 + y) * 4 + ((x + 1) >> 1)];
+    n2 = ob[(z * 8 + y - 1) * 4 + xh];
+    n3 = ob[(z * 8 + y + 1) * 4 + xh];
+    n4 = ob[((z - 1) * 8 + y) * 4 + xh];
+    n5 = ob[((z + 1) * 8 + y) * 4 + xh];
+  }
+  float yv = ytab[nns], pc = *cen;
+  float sum = 0.0f;
+  sum = sum + n0;
+  sum = sum + n1;
+  sum = sum + n2;
+  sum = sum + n3;
+  sum = sum + n4;
+  sum = sum + n5;
+  float pg = st_div_small(rhs + sum, float(nns), yv);
+  *cen = pc + omega * (pg - pc);
+}
+
 // LDS of the resident mode
 struct CoopLds {
   float sP[COOP_TMAX * 512];
@@ -1062,6 +1139,7 @@ struct CoopLds {
   uint32_t sNb[COOP_TMAX * 6];
   uint32_t sSlot[COOP_TMAX];
   uint32_t sFor[COOP_TMAX * 6];  // (tile, face) pairs whose neighbour another workgroup owns
+  uint16_t sCell[2][256];        // kCoopCells
 };
 constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
 
@@ -1136,22 +1214,27 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     uint32_t lt = q >> 9, l = q & 511u;
     sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))] = pres[sSlot[lt] * 512u + l];
   }
-  // rhs and stencil of the cells this thread relaxes: colour c, cell k of tile
-  // lt for m = tid + j COOP_WG = 256 lt + k
-  float rr[2][COOP_JMAX];
-  uint32_t rs[2][(COOP_JMAX + 3) / 4];  // stencil bytes, four to a register
+  uint16_t* sCell = &CL.sCell[0][0];
+  for (uint32_t q = tid; q < 512u; q += COOP_WG) sCell[q] = kCoopCells.v[q >> 8][q & 255u];
+  __syncthreads();
+  // rhs and stencil of the cells this thread relaxes, colour c: face cells
+  // m = tid + j COOP_WG = 148 lt + i (j < COOP_JS), then inside cells
+  // m = 108 lt + i (j < COOP_JC)
+  float rr[2][COOP_JS + COOP_JC];
+  uint32_t rs[2][(COOP_JS + COOP_JC + 3) / 4];  // stencil bytes, four to a register
 #pragma unroll
   for (uint32_t c = 0; c < 2u; c++) {
 #pragma unroll
-    for (uint32_t j = 0; j < (COOP_JMAX + 3) / 4; j++) rs[c][j] = 0u;
+    for (uint32_t j = 0; j < (COOP_JS + COOP_JC + 3) / 4; j++) rs[c][j] = 0u;
 #pragma unroll
-    for (uint32_t j = 0; j < COOP_JMAX; j++) {
-      uint32_t m = tid + j * COOP_WG;
+    for (uint32_t j = 0; j < COOP_JS + COOP_JC; j++) {
+      const bool face = j < COOP_JS;
+      const uint32_t per = face ? COOP_NSHELL : COOP_NCORE;
+      uint32_t m = tid + (face ? j : j - COOP_JS) * COOP_WG;
       rr[c][j] = 0.0f;
-      if (m < T * 256u) {
-        uint32_t lt = m >> 8, k = m & 255u;
-        int32_t z = int32_t(k >> 5), y = int32_t((k >> 2) & 7u), x = 2 * int32_t(k & 3u) + ((y + z + int32_t(c)) & 1);
-        uint32_t cell = sSlot[lt] * 512u + uint32_t(x + 8 * (y + 8 * z));
+      if (m < T * per) {
+        uint32_t lt = m / per, i = m - lt * per + (face ? 0u : COOP_NSHELL);
+        uint32_t cell = sSlot[lt] * 512u + sCell[c * 256u + i];
         rr[c][j] = G.rhs[cell];
         rs[c][j >> 2] |= uint32_t(G.sten[cell]) << (8u * (j & 3u));
       }
@@ -1164,63 +1247,49 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
   for (uint32_t it = 0; it < iters; it++) {
 #pragma unroll
     for (uint32_t c = 0; c < 2u; c++) {
-      const uint32_t oc = c ^ 1u;
+      const bool last = it + 1u == iters && c == 1u;  // nothing more to exchange after it
       // the cell addresses are recomputed every sweep rather than kept in
       // registers for the whole solve (an opaque copy of the thread index)
       uint32_t tv = tid;
       asm volatile("" : "+v"(tv));
+      // face cells first
 #pragma unroll
-      for (uint32_t j = 0; j < COOP_JMAX; j++) {
+      for (uint32_t j = 0; j < COOP_JS; j++) {
         uint32_t m = tv + j * COOP_WG;
         uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
-        if (m < T * 256u && (st & STEN_FLUID)) {
-          uint32_t lt = m >> 8, k = m & 255u;
-          int32_t z = int32_t(k >> 5), y = int32_t((k >> 2) & 7u), x = 2 * int32_t(k & 3u) + ((y + z + int32_t(c)) & 1);
-          float* cen = sP + lt * 512u + c * 256u + k;
-          uint32_t nns = st & 7u;
-          if (nns == 0u) {
-            *cen = 0.0f;
-          } else {
-            // neighbours of the other colour: inside the tile from ob (indices
-            // clamped into the tile, so both arms of each choice stay in LDS),
-            // across a face from the halo
-            const float* ob = sP + lt * 512u + oc * 256u;
-            const float* hb = sH + lt * 192u;
-            const int32_t xm = max(x - 1, 0), xp = min(x + 1, 7), ym = max(y - 1, 0), yp = min(y + 1, 7);
-            const int32_t zm = max(z - 1, 0), zp = min(z + 1, 7), xh = x >> 1;
-            float n0 = x > 0 ? ob[(z * 8 + y) * 4 + (xm >> 1)] : hb[0 * 32 + z * 4 + (y >> 1)];
-            float n1 = x < 7 ? ob[(z * 8 + y) * 4 + (xp >> 1)] : hb[1 * 32 + z * 4 + (y >> 1)];
-            float n2 = y > 0 ? ob[(z * 8 + ym) * 4 + xh] : hb[2 * 32 + z * 4 + xh];
-            float n3 = y < 7 ? ob[(z * 8 + yp) * 4 + xh] : hb[3 * 32 + z * 4 + xh];
-            float n4 = z > 0 ? ob[(zm * 8 + y) * 4 + xh] : hb[4 * 32 + y * 4 + xh];
-            float n5 = z < 7 ? ob[(zp * 8 + y) * 4 + xh] : hb[5 * 32 + y * 4 + xh];
-            float yv = ytab[nns], pc = *cen;
-            float sum = 0.0f;
-            sum = sum + n0;
-            sum = sum + n1;
-            sum = sum + n2;
-            sum = sum + n3;
-            sum = sum + n4;
-            sum = sum + n5;
-            float pg = st_div_small(rr[c][j] + sum, float(nns), yv);
-            *cen = pc + omega * (pg - pc);
-          }
+        if (m < T * COOP_NSHELL && (st & STEN_FLUID)) {
+          uint32_t lt = m / COOP_NSHELL, i = m - lt * COOP_NSHELL;
+          coop_relax<true>(sP, sH, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
         }
       }
       __syncthreads();
-      if (it + 1u == iters && c == 1u) break;  // the last sweep: nothing more to exchange
-      // colour-c face cells other workgroups read
-      for (uint32_t q = tid; q < nFor * 32u; q += COOP_WG) {
-        uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
-        int3 a = coop_face_cell(f, e, c);
-        __hip_atomic_store(&G.sorHalo[((size_t(c) * scap + sSlot[lt]) * 6u + f) * 32u + e],
-                           sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!last) {
+        // colour-c face cells other workgroups read, then this workgroup's arrival
+        for (uint32_t q = tid; q < nFor * 32u; q += COOP_WG) {
+          uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
+          int3 a = coop_face_cell(f, e, c);
+          __hip_atomic_store(&G.sorHalo[((size_t(c) * scap + sSlot[lt]) * 6u + f) * 32u + e],
+                             sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        coop_arrive(G, ++epoch);
       }
-      if (!coop_barrier(G, ++epoch, &sFail)) return;
+      // inside cells while the faces travel
+#pragma unroll
+      for (uint32_t j = COOP_JS; j < COOP_JS + COOP_JC; j++) {
+        uint32_t m = tv + (j - COOP_JS) * COOP_WG;
+        uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
+        if (m < T * COOP_NCORE && (st & STEN_FLUID)) {
+          uint32_t lt = m / COOP_NCORE, i = m - lt * COOP_NCORE + COOP_NSHELL;
+          coop_relax<false>(sP, sH, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
+        }
+      }
+      if (last) break;
+      if (!coop_depart(G, epoch, &sFail)) return;
       coop_fill_halo(G, scap, pres, sP, sH, sNb, sFor, nFor, T, c, false);
       __syncthreads();
     }
   }
+  __syncthreads();
   for (uint32_t q = tid; q < T * 512u; q += COOP_WG) {
     uint32_t lt = q >> 9, l = q & 511u;
     pres[sSlot[lt] * 512u + l] = sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))];
