@@ -71,17 +71,29 @@ def run_gpu(sc, args):
     alive = int(np.sum(pos[:, 1] >= -1e5))
     ctl = sim.ctl()
     stats = dict(blocks=int(ctl[4]), fluid=int(ctl[0] + ctl[1]), vmax=float(ctl[6:7].view(np.float32)[0]))
-    # per-stage times of a few more substeps: HIP events around each stage on the
-    # sim's stream (the stream every kernel of the stage is launched on)
-    runs = []
-    for _ in range(args.stage_steps):
-        t += dt
-        if sc.colliders:
-            cf, sdf, sig = sc.collider_state(t, dt)
-            sim.set_colliders(cf, sdf, sig)
-        runs.append(sim.time_stages(prm))
-    stage_ms = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]} if runs else None
     sim.close()
+    # per-stage times of the same substeps: the scene again from its start, the
+    # same warm-up, then HIP events around each stage of the timed substeps on the
+    # sim's stream (the stream every kernel of the stage is launched on)
+    stage_ms = None
+    nst = min(args.stage_steps, args.steps) if args.stage_steps >= 0 else args.steps
+    if nst:
+        sim = capi.FlipSimC(opts)
+        sim.upload(sc.pos, sc.vel)
+        t = 0.0
+        for _ in range(args.warmup):
+            one()
+        runs = []
+        for _ in range(nst):
+            t += dt
+            if sc.colliders:
+                cf, sdf, sig = sc.collider_state(t, dt)
+                sim.set_colliders(cf, sdf, sig)
+            runs.append(sim.time_stages(prm))
+        stage_ms = {k: float(np.mean([r[k] for r in runs])) for k in runs[0]}
+        stats["sor_path"] = ["per-iteration", "resident", "streaming"][sim.sor_path()]
+        sim.close()
+        log(f"[bench] stage times over {nst} substeps")
     return wall, alive, stats, stage_ms
 
 
@@ -149,7 +161,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage-steps", type=int, default=3)
+    ap.add_argument("--stage-steps", type=int, default=-1, help="substeps timed stage by stage (default: all timed ones)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -170,14 +182,21 @@ def main():
     sub_bytes = 112 * alive + 224 * A + (20 + 16 * iters) * fc + 16 * T
     roof = None
     if stage_ms:
-        # dominant kernel: the SOR iteration (k_sor_pipe), one launch per red-black
-        # iteration; 16 B per fluid cell per iteration (rhs, type, pressure read+write)
-        launch_us = stage_ms["pressure"] * 1e3 / iters
-        sor_bytes = 16 * fc
+        # dominant kernel: the pressure solve, 16 B per fluid cell per red-black
+        # iteration (rhs, type, pressure read and write).  The cooperative solve
+        # runs all iterations in one launch (k_sor_resident; the stage also holds
+        # three small list kernels of a few us); the per-iteration path launches
+        # k_sor_pipe once per iteration.
+        coop = stats.get("sor_path") in ("resident", "streaming")
+        per_launch_iters = iters if coop else 1
+        launch_us = stage_ms["pressure"] * 1e3 / (iters // per_launch_iters)
+        sor_bytes = 16 * fc * per_launch_iters
         ach = sor_bytes / (launch_us * 1e-6) / 1e9
+        kname = "k_sor_resident" if coop else "k_sor_pipe"
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.workload, "k_sor_pipe"),
-                "kernel": "k_sor_pipe (one red-black SOR iteration per launch)",
+                "traffic": measured_traffic(args.workload, kname),
+                "kernel": (f"{kname} (all {iters} red-black SOR iterations in one launch, {stats['sor_path']})" if coop
+                           else "k_sor_pipe (one red-black SOR iteration per launch)"),
                 "algorithmic_bytes_per_launch": sor_bytes, "launch_us": launch_us}
     out = {
         "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,

@@ -198,6 +198,12 @@ int st_dump(void* sim, int32_t what, void* dst, size_t bytes);
 int st_sync(void* sim);
 /* Device memory held by the sim in bytes (memBytes, sim.js:492-495). */
 int st_mem_bytes(void* sim, uint64_t* bytes);
+/* Which path the last pressure solve took (waits for the stream): 0 one launch
+ * per red-black iteration, 1 one cooperative launch with the pressure resident
+ * in LDS, 2 one cooperative launch streaming tile windows (see
+ * st_params.sor_mode).  No counterpart in the reference; for benchmarks and
+ * tests. */
+int st_sor_path(void* sim, uint32_t* path);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,7 @@ PROTOS = {
     "st_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "st_sync": (C.c_int, [C.c_void_p]),
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -259,6 +260,12 @@ class FlipSimC(_Base):
         b = C.c_uint64(0)
         self._call("mem_bytes", self.h, C.byref(b))
         return b.value
+
+    def sor_path(self):
+        """path of the last pressure solve: 0 per-iteration launches, 1 resident, 2 streaming"""
+        v = C.c_uint32(0)
+        self._call("sor_path", self.h, C.byref(v))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):

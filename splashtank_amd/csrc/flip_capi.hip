@@ -403,7 +403,7 @@ int st_create(const st_opts* o, void** out) {
       (rc = dalloc(&s->G.wg, size_t((s->ntiles + 255) / 256) * 3 + 3)) || (rc = dalloc(&s->G.ctl, 16)) || (rc = allocSlots(s, s->scap)) ||
       (rc = dalloc(&s->G.part, 2 * 8192 + 64)) || (rc = dalloc(&s->G.foamTab, FOAM_TAB)) ||
       (rc = dalloc(&s->G.mortonTiles, s->ntiles)) || (rc = dalloc(&s->G.coopFlags, s->ntiles)) ||
-      (rc = dalloc(&s->G.coopOffs, size_t(s->ntiles) + 1)) || (rc = dalloc(&s->G.sorBar, COOP_MAX_WG + 3)) ||
+      (rc = dalloc(&s->G.coopOffs, size_t(s->ntiles) + 1)) || (rc = dalloc(&s->G.sorBar, COOP_MAX_WG + 4)) ||
       (rc = dalloc(&s->dSlots, RING))) {
     destroySim(s);
     return rc;
@@ -430,7 +430,7 @@ int st_create(const st_opts* o, void** out) {
   if (hipMemcpy(s->G.fstack, st.data(), size_t(s->scap) * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->G.tmap, tm.data(), size_t(s->ntiles) * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->G.mortonTiles, mort.data(), size_t(s->ntiles) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(s->G.sorBar, 0, (COOP_MAX_WG + 3) * 4) != hipSuccess ||
+      hipMemset(s->G.sorBar, 0, (COOP_MAX_WG + 4) * 4) != hipSuccess ||
       hipMemcpy(s->G.ctl, ctl, 64, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(s->G.pos, 0, M * 16) != hipSuccess || hipMemset(s->G.vel, 0, M * 16) != hipSuccess ||
       hipMemset(s->G.need, 0, size_t(s->ntiles) * 4) != hipSuccess) {
@@ -750,6 +750,15 @@ int st_sync(void* p) {
   uint32_t timeouts = 0;
   HIPCHK(hipMemcpy(&timeouts, s->G.sorBar + COOP_MAX_WG + 1, 4, hipMemcpyDeviceToHost));
   if (timeouts) return fail(ST_ERR_HIP, "resident SOR: a grid barrier timed out (pressure of that solve is not valid)");
+  return ST_OK;
+}
+
+int st_sor_path(void* p, uint32_t* path) {
+  Sim* s = getSim(p);
+  if (!s || !path) return fail(ST_ERR_ARG, "st_sor_path: null argument");
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpy(path, s->G.sorBar + COOP_MAX_WG + 3, 4, hipMemcpyDeviceToHost));
   return ST_OK;
 }
 

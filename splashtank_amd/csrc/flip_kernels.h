@@ -11,8 +11,9 @@
 
 // resident SOR: workgroup size, tiles resident per workgroup, and the largest
 // grid its barrier serves (sorBar holds COOP_MAX_WG epochs, then the sequence
-// number of the last solve it finished, a count of barrier time-outs and the
-// barrier's release word)
+// number of the last solve it finished, a count of barrier time-outs, the
+// barrier's release word and the path of the last solve: 0 one launch per
+// iteration, 1 resident, 2 streaming)
 #define COOP_WG 1024
 #define COOP_TMAX 54
 #define COOP_MAX_WG 1024

@@ -1175,7 +1175,10 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         for (uint32_t q = gt; q < 512u; q += WG) G.pres[sl * 512u + q] = G.pres2[sl * 512u + q];
       }
     }
-    if (w == 0u && tid == 0u) G.sorBar[COOP_MAX_WG] = seq;
+    if (w == 0u && tid == 0u) {
+      G.sorBar[COOP_MAX_WG] = seq;
+      G.sorBar[COOP_MAX_WG + 3] = 2u;
+    }
     return;
   }
   CoopLds& CL = *reinterpret_cast<CoopLds*>(smem);
@@ -1294,7 +1297,10 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     uint32_t lt = q >> 9, l = q & 511u;
     pres[sSlot[lt] * 512u + l] = sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))];
   }
-  if (w == 0u && tid == 0u) G.sorBar[COOP_MAX_WG] = seq;
+  if (w == 0u && tid == 0u) {
+    G.sorBar[COOP_MAX_WG] = seq;
+    G.sorBar[COOP_MAX_WG + 3] = 1u;
+  }
 }
 
 // Morton-ordered list of the slots in use (coopList) for the resident SOR
@@ -2069,6 +2075,7 @@ hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
   }
   if (iters & 1)
     hipLaunchKernelGGL(k_copy_tiles, dim3(L.cellGrid), dim3(WG), 0, st, L.G, (const float*)L.G.pres2, L.G.pres, L.sorSeq);
+  launch_fill_u32(st, L.G.sorBar + COOP_MAX_WG + 3, 1, 0u);
   return coopErr;
 }
 
