@@ -1,18 +1,20 @@
 // flip_kernels.hip - HIP kernels of the FLIP substep for gfx950 (wave64).
 //
 // Layout of the work:
-//  * particles are counting-sorted by the tile of their cell every substep (the
-//    tile marking pass also counts bins), so p2g and g2p run one workgroup per
-//    tile over a contiguous run of particles;
+//  * particles are binned by the tile of their cell every substep with a
+//    counting sort of indices (binIdx); g2p bins the positions it writes for
+//    the next substep, so p2g and g2p run one workgroup per tile over a
+//    contiguous run of bin entries;
 //  * p2g accumulates the tile's 10^3-cell window (the tile plus one cell all
 //    round) with LDS integer atomics and adds the window to HBM once; the sums
 //    are fixed point, so the order the atomics land in does not matter;
 //  * g2p stages the same window of face velocities and counts in LDS;
 //  * cell kernels are grid-stride loops over the device-side count of tiles in
 //    use (ctl[4]), so no launch needs a host readback and all can live in a graph;
-//  * the red-black SOR runs one workgroup per tile and colour, 256 threads for
-//    the 256 cells of the colour, with cross-tile neighbours from a per-slot
-//    table of the six face-neighbour slots.
+//  * the red-black SOR runs every iteration of a solve in one cooperative
+//    launch, each CU keeping the pressure of a Morton-ordered range of tiles in
+//    LDS and exchanging face halos with the others through a grid barrier; a
+//    per-iteration tile-window kernel is the other path.
 // The reference's kernels each of these replaces are cited at the kernel.
 #include "flip_device.h"
 #include "flip_kernels.h"
@@ -1962,7 +1964,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_init_new, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
   hipLaunchKernelGGL(k_nbr, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(L.scap) * 27, WG), 2048u))), dim3(WG), 0, st, L.P, L.G);
   if (L.initStep) hipLaunchKernelGGL(k_nb_phi_init, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.C, L.G);
-  // particle bins: exclusive scan of the counts, then A -> B in bin order
+  // particle bins: exclusive scan of the counts, then the bin list
   launch_scan(st, L.G.binCount, L.G.binStart, nt + 1, L.G.scanTmp, nullptr);
   if (L.n) hipLaunchKernelGGL(k_bin_scatter, dim3(gridFor(L.n, WG)), dim3(WG), 0, st, L.P, L.G);
 }
@@ -2014,7 +2016,6 @@ void launch_divergence(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_divergence, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
 }
 
-// iterations per SOR launch: 1 (12^3 window) costs the least work per iteration
 uint32_t sor_resident_grid(uint32_t cus) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sor_pipe, WG, 0) != hipSuccess || nb < 1) nb = 1;
