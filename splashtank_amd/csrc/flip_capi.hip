@@ -256,6 +256,7 @@ int beginStep(Sim* s, const st_params* o) {
   P.nfill = s->initStep ? std::min<uint32_t>(o->nfill, 8u) : 0u;
   P.nfv = s->initStep ? o->nfv : 0u;
   P.ntiles = s->ntiles;
+  P.settle = st_smoothstepf(0.1f, 0.5f, P.q8);
   for (int k = 0; k < 16; k++)
     for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
   s->iters = std::max(1, int(o->iters));
@@ -266,8 +267,11 @@ int beginStep(Sim* s, const st_params* o) {
   hs.P = P;
   size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
   std::memcpy(hs.C.c, s->cf.data(), ncAll * 32 * sizeof(float));
-  size_t bytes = offsetof(StepSlot, C) + ncAll * 32 * sizeof(float);
-  HIPCHK(hipMemcpyAsync(&s->dSlots[k], &hs, bytes, hipMemcpyHostToDevice, s->st));
+  for (size_t q = 0; q < ncAll; q++) {
+    float hq = s->hp.q0 / s->cf[q * 32 + 7];  // q0 / cell size of the collider grid
+    hs.C.near[q] = make_float4(-1.0f - hq, hq, 0.0f, 0.0f);
+  }
+  HIPCHK(hipMemcpyAsync(&s->dSlots[k], &hs, sizeof(StepSlot), hipMemcpyHostToDevice, s->st));
   if (!s->evUsed[k]) {
     HIPCHK(hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming));
     s->evUsed[k] = true;

@@ -30,7 +30,9 @@ struct DParams {
   float z1, q0, q1, q2, q3, q4, q5, q6;
   float q7, q8, q9, q10, q11, q12, sheetMin, band;
   uint32_t tx, ty, tz, lcap, seed, nb, nbInit, nfill;
-  uint32_t nfv, ntiles, pad0, pad1;
+  uint32_t nfv, ntiles;
+  float settle;  // smoothstep(0.1, 0.5, tm), the same for every particle of the step
+  uint32_t pad1;
   float fills[16][4];
 };
 
@@ -38,6 +40,9 @@ struct DParams {
 // (origin, cell), the three rows of R^T, position, velocity, angular velocity.
 struct DCols {
   float4 c[ST_MAX_COLS * 8];
+  // per collider, from its record: (lo, hi margin) of colNear, lo = -1 - q0 / cell,
+  // hq = q0 / cell (the same f32 divisions, done once on the host)
+  float4 near[ST_MAX_COLS];
 };
 
 // Device pointers of one sim, passed by value to every kernel.
@@ -206,8 +211,7 @@ __device__ __forceinline__ bool d_colNear(const DParams& P, const DCols* __restr
   float3 lp = make_float3(f3dot(c3(C->c[k * 8u + 2u]), d), f3dot(c3(C->c[k * 8u + 3u]), d), f3dot(c3(C->c[k * 8u + 4u]), d));
   float3 g = f3div(f3sub(lp, make_float3(b.x, b.y, b.z)), b.w);
   float4 a = C->c[k * 8u];
-  float lo = -1.0f - P.q0 / b.w;
-  float hq = P.q0 / b.w;
+  const float lo = C->near[k].x, hq = C->near[k].y;
   return g.x > lo && g.y > lo && g.z > lo && g.x < a.y + hq && g.y < a.z + hq && g.z < a.w + hq;
 }
 __device__ __forceinline__ int32_t d_colAt(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, float3 p) {

@@ -1520,7 +1520,7 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   float spd0 = f3len(v);
   float sens = fmaxf(P.q6, 0.05f);
   float pacc = f3len(f3sub(vn, vo)) / P.dt;
-  float settle = st_smoothstepf(0.1f, 0.5f, P.q8);
+  const float settle = P.settle;
   float impact = st_clampf((pacc / sens - 15.0f) / 80.0f, 0.0f, 1.0f) * settle;
   float turb = st_clampf((f3len(f3sub(v0, vo)) / sens - 0.35f) / 1.5f, 0.0f, 1.0f) * settle;
   float spray = air * (0.25f + 0.75f * st_clampf(spd0 / 4.0f, 0.0f, 1.0f));
@@ -1537,14 +1537,21 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   float3 airv = f3add(make_float3(P.q1, P.q2, P.q3), f3mul(gust, P.q7));
   v = f3add(v, f3mul(f3sub(airv, v), ft.w));
   float sp = f3len(v);
-  if (sp > P.vmax) v = f3mul(v, P.vmax / sp);
+  bool spIsLen = true;  // sp still equals |v| (v untouched since)
+  if (sp > P.vmax) {
+    v = f3mul(v, P.vmax / sp);
+    spIsLen = false;
+  }
   float3 pt = f3add(p, f3mul(v, P.dt));
   float3 pc = d_collideP(P, C, sdf, pt);
   float3 dsp = f3sub(pc, pt);
   if (f3dot(dsp, dsp) > 1.0e-12f) {
     float3 nrm = f3div(dsp, f3len(dsp));
     float vd = f3dot(v, nrm);
-    if (vd < 0.0f) v = f3sub(v, f3mul(nrm, vd));
+    if (vd < 0.0f) {
+      v = f3sub(v, f3mul(nrm, vd));
+      spIsLen = false;
+    }
   }
   uint32_t o = P.open;
   if (o != 0u && (((o & 1u) && pc.x < P.x0) || ((o & 2u) && pc.x > P.x1) || ((o & 4u) && pc.y < P.y0) || ((o & 8u) && pc.y > P.y1) ||
@@ -1553,7 +1560,9 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
     vo4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
-  po = make_float4(pc.x, pc.y, pc.z, fminf(f3len(v), 63.0f) + 64.0f * floorf(foam * 15.0f + 0.5f));
+  float vlen = sp;
+  if (!spIsLen) vlen = f3len(v);
+  po = make_float4(pc.x, pc.y, pc.z, fminf(vlen, 63.0f) + 64.0f * floorf(foam * 15.0f + 0.5f));
   vo4 = make_float4(v.x, v.y, v.z, foam);
 }
 
