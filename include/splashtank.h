@@ -205,6 +205,27 @@ int st_mem_bytes(void* sim, uint64_t* bytes);
  * tests. */
 int st_sor_path(void* sim, uint32_t* path);
 
+/* Batched tanks.  No counterpart in the reference, which steps one FlipSim at a
+ * time; these advance many independent tanks (each an st_sim of its own size,
+ * particles and colliders) with one kernel launch per stage for all of them.
+ * Each tank's results are the bits st_step gives it alone.  While in a batch the
+ * tanks share the batch's HIP stream (every st_* call on a tank still works and
+ * is ordered with the batched steps); st_destroy refuses a tank in a batch. */
+/* Group n tanks (at most 256, one device, none in another batch). */
+int st_batch_create(void* const* sims, uint32_t n, void** batch);
+/* Wait for the batch's work, give each tank its own stream back, free the batch. */
+int st_batch_destroy(void* batch);
+/* One full substep of every tank (st_step on each), enqueued on the batch's
+ * stream.  `p` holds one parameter block for all tanks (np = 1) or one per tank
+ * in batch order (np = n).  Tanks with no particles sit the substep out, as
+ * st_step does nothing for them.  The narrow band and tile-only steps are not
+ * batched (ST_ERR_ARG). */
+int st_batch_step(void* batch, const st_params* p, uint32_t np);
+/* Wait for all work of the batch. */
+int st_batch_sync(void* batch);
+/* Number of tanks. */
+int st_batch_size(void* batch, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

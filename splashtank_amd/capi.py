@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsplashtank_hip.so")
 
+ST_OK, ST_ERR_ARG, ST_ERR_HIP, ST_ERR_STATE, ST_ERR_NOMEM = 0, -1, -2, -3, -4
 ST_OPEN_X0, ST_OPEN_X1, ST_OPEN_Y0, ST_OPEN_Y1, ST_OPEN_Z0, ST_OPEN_Z1 = 1, 2, 4, 8, 16, 32
 
 STAGES = ["sort", "blocks", "clear", "p2g", "narrow band", "classify", "normFaces",
@@ -120,6 +121,11 @@ PROTOS = {
     "st_sync": (C.c_int, [C.c_void_p]),
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "st_batch_destroy": (C.c_int, [C.c_void_p]),
+    "st_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32]),
+    "st_batch_sync": (C.c_int, [C.c_void_p]),
+    "st_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -268,8 +274,8 @@ class FlipSimC(_Base):
         return v.value
 
     def close(self):
-        if getattr(self, "h", None):
-            self.lib.st_destroy(self.h)
+        # a tank still in a batch is refused; it stays open until the batch closes
+        if getattr(self, "h", None) and self.lib.st_destroy(self.h) == 0:
             self.h = None
 
     def __del__(self):
@@ -277,3 +283,47 @@ class FlipSimC(_Base):
             self.close()
         except Exception:
             pass
+
+
+class FlipBatchC:
+    """Many tanks (FlipSimC objects) advanced together, one launch per kernel
+    for all of them (st_batch_*).  The tanks stay usable on their own (upload,
+    read_particles, dump ...); while batched they share the batch's stream."""
+
+    def __init__(self, tanks):
+        self.lib = load()
+        self.tanks = list(tanks)
+        arr = (C.c_void_p * len(self.tanks))(*[t.h for t in self.tanks])
+        h = C.c_void_p()
+        rc = self.lib.st_batch_create(arr, len(self.tanks), C.byref(h))
+        if rc != 0:
+            raise StError(f"st_batch_create -> {rc}: {self.lib.st_last_error().decode()}")
+        self.h = h
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, f"st_batch_{name}")(*args)
+        if rc != 0:
+            raise StError(f"st_batch_{name} -> {rc}: {self.lib.st_last_error().decode()}")
+
+    def step(self, prm):
+        """One substep of every tank: `prm` is one StParams for all or a list, one per tank."""
+        if isinstance(prm, StParams):
+            self._call("step", self.h, C.byref(prm), 1)
+        else:
+            arr = (StParams * len(prm))(*prm)
+            self._call("step", self.h, arr, len(prm))
+
+    def sync(self):
+        self._call("sync", self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.st_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

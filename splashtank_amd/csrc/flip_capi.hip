@@ -91,6 +91,8 @@ struct Sim {
   // (the last g2p made them and nothing has moved a particle since)
   bool binned = false;
   NbScratch nb{};
+  void* batch = nullptr;          // the batch the tank belongs to (its stream is the batch's)
+  hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
 
   LaunchCtx ctx() const {
     LaunchCtx L;
@@ -203,8 +205,8 @@ void destroySim(Sim* s) {
 
 Sim* getSim(void* p) { return (Sim*)p; }
 
-// sets and uploads the step's parameter block and collider records (sim.js:179-231)
-int beginStep(Sim* s, const st_params* o) {
+// the step's parameter block and collider records (sim.js:179-231) into P and C
+int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   if (s->N == 0) return fail(ST_ERR_STATE, "st_begin_step: no particles (step() does nothing when N = 0)");
   s->nbOn = o->nb_on != 0;
   if (s->nbOn && !s->velGOn) {  // enableNB (sim.js:120-126)
@@ -260,17 +262,25 @@ int beginStep(Sim* s, const st_params* o) {
   for (int k = 0; k < 16; k++)
     for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
   s->iters = std::max(1, int(o->iters));
+  Pout = P;
+  size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
+  std::memcpy(C.c, s->cf.data(), ncAll * 32 * sizeof(float));
+  for (size_t q = 0; q < ncAll; q++) {
+    float hq = P.q0 / s->cf[q * 32 + 7];  // q0 / cell size of the collider grid
+    C.near[q] = make_float4(-1.0f - hq, hq, 0.0f, 0.0f);
+  }
+  return ST_OK;
+}
+
+// sets and uploads the step's parameter block and collider records
+int beginStep(Sim* s, const st_params* o) {
   // step slot: wait until the ring entry's last copy finished, fill, copy
-  uint32_t k = s->ringPos++ % RING;
+  uint32_t k = s->ringPos % RING;
   if (s->evUsed[k]) HIPCHK(hipEventSynchronize(s->ev[k]));
   StepSlot& hs = s->hSlots[k];
-  hs.P = P;
-  size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
-  std::memcpy(hs.C.c, s->cf.data(), ncAll * 32 * sizeof(float));
-  for (size_t q = 0; q < ncAll; q++) {
-    float hq = s->hp.q0 / s->cf[q * 32 + 7];  // q0 / cell size of the collider grid
-    hs.C.near[q] = make_float4(-1.0f - hq, hq, 0.0f, 0.0f);
-  }
+  int rc = prepStep(s, o, hs.P, hs.C);
+  if (rc) return rc;
+  s->ringPos++;
   HIPCHK(hipMemcpyAsync(&s->dSlots[k], &hs, sizeof(StepSlot), hipMemcpyHostToDevice, s->st));
   if (!s->evUsed[k]) {
     HIPCHK(hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming));
@@ -330,6 +340,65 @@ int runStage(Sim* s, int stage) {
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return ST_OK;
+}
+
+// Batched tanks (no counterpart in the reference, which steps one FlipSim at a
+// time): every stage of a substep for all tanks in one launch per kernel over
+// the concatenated workgroup ranges of the tanks.  The tanks keep their own
+// state; while in the batch they share the batch's stream.
+const int BRING = 8;
+struct Batch {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  std::vector<Sim*> tanks;
+  uint32_t sorResident = 2048;
+  // ring of step slots: tank records, parameter blocks, collider records, block
+  // tables (host pinned and device), each reusable once its upload is done
+  size_t slotBytes = 0, offP = 0, offC = 0, offTab = 0;
+  uint32_t itersCap = 0;
+  uint8_t* hRing = nullptr;
+  uint8_t* dRing = nullptr;
+  hipEvent_t ev[BRING];
+  bool evUsed[BRING];
+  uint32_t ringPos = 0;
+  std::vector<BTankPlan> plan;
+};
+
+size_t align256(size_t x) { return (x + 255u) & ~size_t(255u); }
+
+void freeRing(Batch* b) {
+  for (int k = 0; k < BRING; k++)
+    if (b->evUsed[k]) {
+      (void)hipEventSynchronize(b->ev[k]);
+      (void)hipEventDestroy(b->ev[k]);
+      b->evUsed[k] = false;
+    }
+  if (b->hRing) (void)hipHostFree(b->hRing);
+  dfree(b->dRing);
+  b->hRing = nullptr;
+  b->itersCap = 0;
+}
+
+// ring slots sized for nt tanks and up to `iters` SOR iterations
+int ensureRing(Batch* b, uint32_t iters) {
+  if (b->hRing && iters <= b->itersCap) return ST_OK;
+  HIPCHK(hipStreamSynchronize(b->st));
+  freeRing(b);
+  const size_t nt = b->tanks.size();
+  b->offP = align256(nt * sizeof(BTank));
+  b->offC = b->offP + align256(nt * sizeof(DParams));
+  b->offTab = b->offC + align256(nt * sizeof(DCols));
+  b->slotBytes = align256(b->offTab + batch_table_words(uint32_t(nt), iters) * 4u);
+  hipError_t e = hipHostMalloc((void**)&b->hRing, b->slotBytes * BRING, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    b->hRing = nullptr;
+    return fail(ST_ERR_NOMEM, "hipHostMalloc failed");
+  }
+  std::memset(b->hRing, 0, b->slotBytes * BRING);
+  int rc = dalloc(&b->dRing, b->slotBytes * BRING);
+  if (rc) return rc;
+  b->itersCap = iters;
   return ST_OK;
 }
 
@@ -450,6 +519,7 @@ int st_create(const st_opts* o, void** out) {
 
 int st_destroy(void* p) {
   if (!p) return fail(ST_ERR_ARG, "st_destroy: null sim");
+  if (getSim(p)->batch) return fail(ST_ERR_STATE, "st_destroy: the tank is in a batch (destroy the batch first)");
   destroySim(getSim(p));
   return ST_OK;
 }
@@ -771,6 +841,147 @@ int st_mem_bytes(void* p, uint64_t* bytes) {
   if (!s || !bytes) return fail(ST_ERR_ARG, "st_mem_bytes: null argument");
   uint64_t M = s->maxn, C = uint64_t(s->scap) * 512;
   *bytes = M * (2 * (16 + 16 + 4) + 4 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
+  return ST_OK;
+}
+
+int st_batch_create(void* const* sims, uint32_t n, void** out) {
+  if (!sims || !out || n == 0) return fail(ST_ERR_ARG, "st_batch_create: null argument or no tanks");
+  *out = nullptr;
+  if (n > ST_BATCH_MAX) return fail(ST_ERR_ARG, "st_batch_create: at most 256 tanks");
+  for (uint32_t k = 0; k < n; k++) {
+    Sim* s = getSim(sims[k]);
+    if (!s) return fail(ST_ERR_ARG, "st_batch_create: null tank");
+    if (s->batch) return fail(ST_ERR_STATE, "st_batch_create: a tank is already in a batch");
+    if (s->dev != getSim(sims[0])->dev) return fail(ST_ERR_ARG, "st_batch_create: tanks on different devices");
+    for (uint32_t q = 0; q < k; q++)
+      if (sims[q] == sims[k]) return fail(ST_ERR_ARG, "st_batch_create: a tank is listed twice");
+  }
+  Batch* b = new Batch();
+  for (int k = 0; k < BRING; k++) b->evUsed[k] = false;
+  b->dev = getSim(sims[0])->dev;
+  hipError_t e = hipSetDevice(b->dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(ST_ERR_HIP, std::string("st_batch_create: ") + hipGetErrorString(e));
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    Sim* s = getSim(sims[k]);
+    e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) {
+      for (uint32_t q = 0; q < k; q++) {
+        Sim* r = b->tanks[q];
+        r->st = r->ownStream;
+        r->ownStream = nullptr;
+        r->batch = nullptr;
+      }
+      (void)hipStreamDestroy(b->st);
+      delete b;
+      return fail(ST_ERR_HIP, std::string("st_batch_create: ") + hipGetErrorString(e));
+    }
+    s->ownStream = s->st;
+    s->st = b->st;
+    s->batch = b;
+    b->tanks.push_back(s);
+  }
+  b->sorResident = b->tanks[0]->sorResident;
+  b->plan.resize(n);
+  *out = b;
+  return ST_OK;
+}
+
+int st_batch_destroy(void* p) {
+  Batch* b = (Batch*)p;
+  if (!b) return fail(ST_ERR_ARG, "st_batch_destroy: null batch");
+  (void)hipSetDevice(b->dev);
+  (void)hipStreamSynchronize(b->st);
+  for (Sim* s : b->tanks) {
+    s->st = s->ownStream;
+    s->ownStream = nullptr;
+    s->batch = nullptr;
+  }
+  freeRing(b);
+  (void)hipStreamDestroy(b->st);
+  delete b;
+  return ST_OK;
+}
+
+int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
+  Batch* b = (Batch*)p;
+  if (!b || !prm) return fail(ST_ERR_ARG, "st_batch_step: null argument");
+  const uint32_t nt = uint32_t(b->tanks.size());
+  if (nprm != 1u && nprm != nt) return fail(ST_ERR_ARG, "st_batch_step: give one parameter block or one per tank");
+  HIPCHK(hipSetDevice(b->dev));
+  uint32_t itersMax = 0;
+  for (uint32_t t = 0; t < nt; t++) {
+    const st_params* o = prm + (nprm == 1u ? 0u : t);
+    if (o->nb_on || o->only_tiles) return fail(ST_ERR_ARG, "st_batch_step: the narrow band and tile-only steps are single-tank only");
+    itersMax = std::max<uint32_t>(itersMax, b->tanks[t]->N ? uint32_t(std::max(1, int(o->iters))) : 0u);
+  }
+  int rc = ensureRing(b, itersMax);
+  if (rc) return rc;
+  uint32_t k = b->ringPos % BRING;
+  if (b->evUsed[k]) HIPCHK(hipEventSynchronize(b->ev[k]));
+  uint8_t* h = b->hRing + k * b->slotBytes;
+  uint8_t* d = b->dRing + k * b->slotBytes;
+  BTank* hT = (BTank*)h;
+  DParams* hP = (DParams*)(h + b->offP);
+  DCols* hC = (DCols*)(h + b->offC);
+  uint32_t* hTab = (uint32_t*)(h + b->offTab);
+  for (uint32_t t = 0; t < nt; t++) {
+    Sim* s = b->tanks[t];
+    BTankPlan& pl = b->plan[t];
+    pl = BTankPlan{0u, s->ntiles, s->scap, 0u, false, false};
+    hT[t].P = (const DParams*)(d + b->offP) + t;
+    hT[t].C = (const DCols*)(d + b->offC) + t;
+    hT[t].G = s->G;
+    if (!s->N) continue;  // step() does nothing when N = 0
+    if ((rc = prepStep(s, prm + (nprm == 1u ? 0u : t), hP[t], hC[t]))) return rc;
+    pl.n = s->N;
+    pl.iters = uint32_t(s->iters);
+    pl.binned = s->binned;
+    pl.solid = s->solidDirty;
+  }
+  batch_tables(b->plan.data(), nt, itersMax, b->sorResident, hTab);
+  HIPCHK(hipMemcpyAsync(d, h, b->offTab + batch_table_words(nt, itersMax) * 4u, hipMemcpyHostToDevice, b->st));
+  if (!b->evUsed[k]) {
+    HIPCHK(hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming));
+    b->evUsed[k] = true;
+  }
+  HIPCHK(hipEventRecord(b->ev[k], b->st));
+  b->ringPos++;
+  launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+  // what the single-tank stages leave behind: g2p wrote the new state into the
+  // other buffers and binned it for the next substep
+  for (uint32_t t = 0; t < nt; t++) {
+    Sim* s = b->tanks[t];
+    if (!s->N) continue;
+    s->solidDirty = false;
+    s->sorSeq++;
+    s->binned = true;
+    s->cur = nullptr;
+    s->begun = false;
+    std::swap(s->G.pos, s->G.posB);
+    std::swap(s->G.vel, s->G.velB);
+    std::swap(s->G.pid, s->G.pidB);
+  }
+  return ST_OK;
+}
+
+int st_batch_sync(void* p) {
+  Batch* b = (Batch*)p;
+  if (!b) return fail(ST_ERR_ARG, "st_batch_sync: null batch");
+  HIPCHK(hipSetDevice(b->dev));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return ST_OK;
+}
+
+int st_batch_size(void* p, uint32_t* n) {
+  Batch* b = (Batch*)p;
+  if (!b || !n) return fail(ST_ERR_ARG, "st_batch_size: null argument");
+  *n = uint32_t(b->tanks.size());
   return ST_OK;
 }
 

@@ -51,6 +51,37 @@ struct NbScratch {
   uint32_t ncap, nmax;
 };
 
+// Batched tanks.  Per tank: the device copies of the step's parameter block and
+// collider records, and the tank's device pointers.
+struct BTank {
+  const DParams* P;
+  const DCols* C;
+  DGrid G;
+};
+#define ST_BATCH_MAX 256
+// per tank, what one batched substep launches for it
+struct BTankPlan {
+  uint32_t n;       // particles (0: the tank sits this substep out)
+  uint32_t ntiles;
+  uint32_t scap;
+  uint32_t iters;   // SOR iterations
+  bool binned;      // need, binCount and binKey already hold A's marks and bins
+  bool solid;       // rebuild the per-cell collider lookup
+};
+// Block tables of a batched substep: table k holds nt + 1 workgroup starts, one
+// range per tank (empty where the tank has no work in that launch).  Table
+// BT_PIPE0 + it belongs to SOR iteration it.
+enum BatchTable {
+  BT_FILL_MARK, BT_MARK, BT_NWG, BT_ONE, BT_ALLOC, BT_CELL, BT_NBR, BT_SCAN, BT_PART,
+  BT_TILE, BT_ONE_N, BT_SOLID, BT_COPY, BT_FOAM, BT_G2P, BT_FILL_NEXT, BT_PIPE0
+};
+size_t batch_table_words(uint32_t nt, uint32_t itersMax);
+// fills the tables (host copy); sorResident = SOR workgroups resident at once
+void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_t sorResident, uint32_t* tab);
+// every stage of one substep for all tanks, one launch per kernel: dT and dtab
+// are the device copies of the tank records and tables, htab the host copy
+void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax);
+
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_blocks(hipStream_t st, const LaunchCtx& L);
 void launch_clear(hipStream_t st, const LaunchCtx& L);

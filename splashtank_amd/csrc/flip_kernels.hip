@@ -114,19 +114,19 @@ __device__ __forceinline__ void bin_mark(const DParams& P, const DGrid& G, uint3
   }
 }
 
-__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void mark_bin_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = bid * blockDim.x + threadIdx.x;
   bool valid = i < P.n;
   float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
   bin_mark(P, G, i, valid, q);
 }
+__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) { mark_bin_body(PP, G, blockIdx.x, gridDim.x); }
 
 // exclusive scan of n counts into out[0..n], out[n] = total; three launches
-__global__ void __launch_bounds__(WG) k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
-                                                    uint32_t* __restrict__ sums) {
+__device__ __forceinline__ void scan_blocks_body(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, uint32_t* __restrict__ sums, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t wsum[WG / 64];
-  uint32_t base = blockIdx.x * (WG * 4) + threadIdx.x * 4;
+  uint32_t base = bid * (WG * 4) + threadIdx.x * 4;
   uint32_t v[4], s = 0;
   for (int q = 0; q < 4; q++) {
     v[q] = base + q < n ? in[base + q] : 0u;
@@ -150,9 +150,11 @@ __global__ void __launch_bounds__(WG) k_scan_blocks(const uint32_t* __restrict__
     if (base + q < n) out[base + q] = run;
     run += v[q];
   }
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+  if (threadIdx.x == 0) sums[bid] = tot;
 }
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n) {
+__global__ void __launch_bounds__(WG) k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+                                                    uint32_t* __restrict__ sums) { scan_blocks_body(in, out, n, sums, blockIdx.x, gridDim.x); }
+__device__ __forceinline__ void scan_top_body(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t part[1024 / 64];
   __shared__ uint32_t carry;
   if (threadIdx.x == 0) carry = 0;
@@ -179,10 +181,10 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, 
   }
   if (threadIdx.x == 0) out[n] = carry;
 }
-__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums,
-                                                 uint32_t* __restrict__ copy) {
-  uint32_t base = blockIdx.x * (WG * 4) + threadIdx.x * 4;
-  uint32_t add = sums[blockIdx.x];
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n) { scan_top_body(sums, nb, out, n, blockIdx.x, gridDim.x); }
+__device__ __forceinline__ void scan_add_body(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums, uint32_t* __restrict__ copy, uint32_t bid, uint32_t nbid) {
+  uint32_t base = bid * (WG * 4) + threadIdx.x * 4;
+  uint32_t add = sums[bid];
   for (int q = 0; q < 4; q++)
     if (base + q < n) {
       uint32_t v = out[base + q] + add;
@@ -190,15 +192,18 @@ __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uin
       if (copy) copy[base + q] = v;
     }
 }
+__global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums,
+                                                 uint32_t* __restrict__ copy) { scan_add_body(out, n, sums, copy, blockIdx.x, gridDim.x); }
 
 // bin order as an index list: binIdx[binStart[bin] + rank] = i.  The particles
 // themselves stay where they are; p2g and g2p read them through the list (runs
 // of a bin are mostly contiguous in A, which is in last step's bin order).
-__global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bin_scatter_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
+  uint32_t i = bid * blockDim.x + threadIdx.x;
   if (i >= PP->n) return;
   G.binIdx[G.binStart[G.binKey[i]] + G.binRank[i]] = i;
 }
+__global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) { bin_scatter_body(PP, G, blockIdx.x, gridDim.x); }
 
 // Flags of one tile: freed (no longer needed, had a slot), new (needed, had
 // none), kept (needed, had one).  tileFree in wgsl-blocks.js:16-21.
@@ -242,22 +247,23 @@ __device__ __forceinline__ Ranks3 wg_ranks3(bool a, bool b, bool c) {
   return R;
 }
 
-__global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ PP, DGrid G) {
-  uint32_t t = blockIdx.x * WG + threadIdx.x;
+__device__ __forceinline__ void tile_count_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
+  uint32_t t = bid * WG + threadIdx.x;
   TileFlags f = tile_flags(G, t, PP->ntiles);
   Ranks3 R = wg_ranks3(f.fr, f.nw, f.kp);
   if (threadIdx.x == 0) {
-    G.wg[blockIdx.x * 3 + 0] = R.tot[0];
-    G.wg[blockIdx.x * 3 + 1] = R.tot[1];
-    G.wg[blockIdx.x * 3 + 2] = R.tot[2];
+    G.wg[bid * 3 + 0] = R.tot[0];
+    G.wg[bid * 3 + 1] = R.tot[1];
+    G.wg[bid * 3 + 2] = R.tot[2];
   }
 }
+__global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ PP, DGrid G) { tile_count_body(PP, G, blockIdx.x, gridDim.x); }
 
 // One workgroup: exclusive scan of the per-workgroup counts, then the stack
 // arithmetic of tileFree/tileAlloc/tileFix (wgsl-blocks.js:16-35) in closed form:
 // the k-th new tile (tile order) pops stack entry top1-1-k, so the first
 // min(new, top1) new tiles get slots and the rest count as no room.
-__global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) {
+__device__ __forceinline__ void tile_scan_body(const DGrid& G, uint32_t nwg, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t part[3][1024 / 64];
   __shared__ uint32_t carry[3];
   if (threadIdx.x < 3) carry[threadIdx.x] = 0;
@@ -303,14 +309,15 @@ __global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) {
     G.ctl[14] = nalloc;
   }
 }
+__global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) { tile_scan_body(G, nwg, blockIdx.x, gridDim.x); }
 
 // Freed tiles push their slot (tile order), new tiles are listed (tile order),
 // kept tiles take their place in slotList (tile order).
-__global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__ PP, DGrid G) {
-  uint32_t t = blockIdx.x * WG + threadIdx.x;
+__device__ __forceinline__ void tile_scatter_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
+  uint32_t t = bid * WG + threadIdx.x;
   TileFlags f = tile_flags(G, t, PP->ntiles);
   Ranks3 R = wg_ranks3(f.fr, f.nw, f.kp);
-  uint32_t baseF = G.wg[blockIdx.x * 3 + 0], baseN = G.wg[blockIdx.x * 3 + 1], baseK = G.wg[blockIdx.x * 3 + 2];
+  uint32_t baseF = G.wg[bid * 3 + 0], baseN = G.wg[bid * 3 + 1], baseK = G.wg[bid * 3 + 2];
   uint32_t top0 = G.ctl[12], nalloc = G.ctl[14];
   if (f.fr) {
     G.fstack[top0 + baseF + R.r[0]] = G.tmap[t];
@@ -324,11 +331,12 @@ __global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__
     G.slotList[baseK + R.r[2] + min(nb, nalloc)] = G.tmap[t];
   }
 }
+__global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__ PP, DGrid G) { tile_scatter_body(PP, G, blockIdx.x, gridDim.x); }
 
 // tileAlloc (wgsl-blocks.js:26-33), deterministic pop order
-__global__ void k_tile_alloc(DGrid G) {
+__device__ __forceinline__ void tile_alloc_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
   uint32_t nn = G.ctl[3], top1 = G.ctl[13], nalloc = G.ctl[14];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += gridDim.x * blockDim.x) {
+  for (uint32_t k = bid * blockDim.x + threadIdx.x; k < nn; k += nbid * blockDim.x) {
     if (k < nalloc) {
       uint32_t s = G.fstack[top1 - 1u - k], t = G.newList[k * 2], idx = G.newList[k * 2 + 1];
       G.tmap[t] = s;
@@ -340,16 +348,17 @@ __global__ void k_tile_alloc(DGrid G) {
     }
   }
 }
+__global__ void k_tile_alloc(DGrid G) { tile_alloc_body(G, blockIdx.x, gridDim.x); }
 
 // Slots of the 27 tiles around every tile in use (itself included), index
 // (dx+1) + 3 (dy+1) + 9 (dz+1): 0 for a tile without storage (cellIdx maps it to
 // the spare slot), WSLOT_OUT outside the tile grid.  Built once per substep after
 // allocation so the window kernels need no tile-map lookups.
 #define WSLOT_OUT 0xfffffffeu
-__global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void nbr_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   uint32_t tot = G.ctl[4] * 27u;
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+  for (uint32_t e = bid * blockDim.x + threadIdx.x; e < tot; e += nbid * blockDim.x) {
     uint32_t b = e / 27u, q = e - b * 27u;
     uint32_t s = G.slotList[b], t = G.slotTile[s];
     int32_t x = int32_t(t % P.tx) + int32_t(q % 3u) - 1, y = int32_t((t / P.tx) % P.ty) + int32_t((q / 3u) % 3u) - 1,
@@ -363,12 +372,13 @@ __global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
     G.nbrL[e] = v;
   }
 }
+__global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) { nbr_body(PP, G, blockIdx.x, gridDim.x); }
 
 // initNew (wgsl-blocks.js:38-47): no pressure, collider lookup of each cell
-__global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+__device__ __forceinline__ void init_new_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   uint32_t tot = G.ctl[3] * 512u;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
     uint32_t s = G.newList[(t >> 9) * 2 + 1];
     if (s == ST_NONE) continue;
     uint32_t local = t & 511u, tl = G.newList[(t >> 9) * 2], tt = tl / P.tx;
@@ -383,17 +393,18 @@ __global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restri
     }
   }
 }
+__global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { init_new_body(PP, C, G, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // clearSlots (wgsl-solver.js:5-15): every slot in use plus the spare slot 0
-__global__ void k_clear(DGrid G) {
+__device__ __forceinline__ void clear_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
   uint32_t sc = G.ctl[4];
   uint32_t tot = (sc + 1u) * 512u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     G.ctl[0] = 0u;
     G.ctl[1] = 0u;
   }
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
     uint32_t blk = t >> 9;
     uint32_t s = blk < sc ? G.slotList[blk] : 0u;
     uint32_t c = s * 512u + (t & 511u);
@@ -403,6 +414,7 @@ __global__ void k_clear(DGrid G) {
     if (s == 0u) G.rhs[c] = 8.0f;
   }
 }
+__global__ void k_clear(DGrid G) { clear_body(G, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // Tile windows.  The window of tile (ti, tj, tk) is cells [8ti-1, 8ti+8] per axis.
@@ -443,7 +455,7 @@ __device__ __forceinline__ int32_t wface(const DParams& P, uint32_t axis, int32_
 
 // ---------------------------------------------------------------------------
 // p2g (wgsl-solver.js:16-40), one workgroup per tile in use
-__global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   // per face: weight in the low 32 bits, velocity in the high 32 bits of one
   // 64-bit sum; weights are >= 0 and their sum < 2^31, so the halves never mix
@@ -452,7 +464,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
   __shared__ uint32_t wslots[27];
   uint32_t vmaxBits = 0;
   uint32_t sc = G.ctl[4];
-  for (uint32_t b = blockIdx.x; b < sc; b += gridDim.x) {
+  for (uint32_t b = bid; b < sc; b += nbid) {
     uint32_t s = G.slotList[b], t = G.slotTile[s];
     int3 org;
     load_wslots(P, G, s, t, wslots, org);
@@ -524,13 +536,14 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
   if (threadIdx.x == 0) {
     uint32_t m = 0;
     for (int q = 0; q < WG / 64; q++) m = max(m, wmax[q]);
-    G.part[blockIdx.x] = m;
+    G.part[bid] = m;
   }
 }
+__global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) { p2g_tile_body(PP, G, blockIdx.x, gridDim.x); }
 
 // statistics partials -> control words: which 0: ctl[6] = max(ctl[6], vmax
 // partials); which 1: ctl[0], ctl[1] = sums of the per-colour fluid partials
-__global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint32_t which) {
+__device__ __forceinline__ void reduce_stats_body(const DGrid& G, uint32_t n, uint32_t which, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t ws[2][1024 / 64];
   uint32_t a = 0, b = 0;
   for (uint32_t i = threadIdx.x; i < n; i += 1024) {
@@ -551,16 +564,17 @@ __global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint
     else { G.ctl[0] = x; G.ctl[1] = y; }
   }
 }
+__global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint32_t which) { reduce_stats_body(G, n, which, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // per active cell kernels: t -> slot slotList[t >> 9], local t & 511 (acell,
 // wgsl-core.js:96-103)
 #define FOR_ACTIVE(G)                                                                          \
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, tot_ = (G).ctl[4] * 512u; t < tot_; \
-       t += gridDim.x * blockDim.x)
+  for (uint32_t t = bid * blockDim.x + threadIdx.x, tot_ = (G).ctl[4] * 512u; t < tot_; \
+       t += nbid * blockDim.x)
 
 // solidBuild (wgsl-solver.js:41-45)
-__global__ void k_solid_build(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+__device__ __forceinline__ void solid_build_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -568,17 +582,18 @@ __global__ void k_solid_build(const DParams* __restrict__ PP, const DCols* __res
     G.ctype[c] = (G.ctype[c] & 0xffff00ffu) | (uint32_t(d_colAt(P, C, G.sdf, d_cellCenter(P, p.x, p.y, p.z)) + 1) << 8);
   }
 }
+__global__ void k_solid_build(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { solid_build_body(PP, C, G, blockIdx.x, gridDim.x); }
 
 __device__ __forceinline__ float3 d_velGVec(const DParams& P, const DGrid& G, float3 x);
 
 // classify (wgsl-solver.js:46-74), plus the per-colour fluid counts of buildList
 // (wgsl-solver.js:152-158) as statistics
-__global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void classify_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   uint32_t tot = G.ctl[4] * 512u;
-  uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t stride = nbid * blockDim.x;
   uint32_t nred = 0, nblack = 0;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += stride) {
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += stride) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
     int3 a = d_slotCoord(P, G.slotTile, s, local);
     uint32_t ty = 0;
@@ -616,13 +631,14 @@ __global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP,
   if (threadIdx.x == 0) {
     uint32_t r = 0, b = 0;
     for (int q = 0; q < WG / 64; q++) { r += ws[0][q]; b += ws[1][q]; }
-    G.part[2 * blockIdx.x] = r;
-    G.part[2 * blockIdx.x + 1] = b;
+    G.part[2 * bid] = r;
+    G.part[2 * bid + 1] = b;
   }
 }
+__global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP, DGrid G) { classify_body(PP, G, blockIdx.x, gridDim.x); }
 
 // normBody / normFaces (wgsl-solver.js:75-105)
-__global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+__device__ __forceinline__ void norm_faces_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -651,6 +667,7 @@ __global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __rest
     }
   }
 }
+__global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { norm_faces_body(PP, C, G, blockIdx.x, gridDim.x); }
 
 __device__ __forceinline__ float gvR(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accV[f]); }
 __device__ __forceinline__ float gv0R(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accW[f]); }
@@ -660,7 +677,7 @@ __device__ __forceinline__ float gv0R(const DGrid& G, uint32_t f) { return st_u2
 
 // divergence (wgsl-solver.js:106-123); also records the SOR stencil of the cell,
 // which depends only on the cell types fixed by classify.
-__global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void divergence_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -695,6 +712,7 @@ __global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
     G.sten[c] = uint8_t(STEN_FLUID | nns);
   }
 }
+__global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) { divergence_body(PP, G, blockIdx.x, gridDim.x); }
 
 // relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
 // sim.js:291-293): one red-black iteration per launch, by tiles.  A workgroup
@@ -917,13 +935,14 @@ __device__ __forceinline__ void sor_pipe_pass(const DGrid& G, float omega, const
 
 // One red-black iteration per launch (the grid a multiple of 8 workgroups, as
 // many as are resident), skipped when the resident solve has done this stage.
-__global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
-                                                 float* __restrict__ pout, uint32_t seq) {
+__device__ __forceinline__ void sor_pipe_body(const DParams* __restrict__ PP, const DGrid& G, const float* __restrict__ pin, float* __restrict__ pout, uint32_t seq, uint32_t bid, uint32_t nbid) {
   __shared__ SorPipeLds L;
   if (G.sorBar[COOP_MAX_WG] == seq) return;
   sor_pipe_init(L, threadIdx.x);
-  sor_pipe_pass(G, PP->omega, pin, pout, blockIdx.x, gridDim.x, threadIdx.x, L);
+  sor_pipe_pass(G, PP->omega, pin, pout, bid, nbid, threadIdx.x, L);
 }
+__global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
+                                                 float* __restrict__ pout, uint32_t seq) { sor_pipe_body(PP, G, pin, pout, seq, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // Resident SOR: every iteration of one pressure solve in one cooperative launch,
@@ -1316,17 +1335,18 @@ __global__ void k_coop_compact(const DParams* __restrict__ PP, DGrid G) {
 }
 
 // after an odd number of iterations the result sits in the partner buffer
-__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq) {
+__device__ __forceinline__ void copy_tiles_body(const DGrid& G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq, uint32_t bid, uint32_t nbid) {
   if (G.sorBar[COOP_MAX_WG] == seq) return;
   uint32_t tot = G.ctl[4] * 512u;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
     uint32_t c = G.slotList[t >> 9] * 512u + (t & 511u);
     dst[c] = src[c];
   }
 }
+__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq) { copy_tiles_body(G, src, dst, seq, blockIdx.x, gridDim.x); }
 
 // projBody / project (wgsl-solver.js:165-180)
-__global__ void k_project(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void project_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1345,6 +1365,7 @@ __global__ void k_project(const DParams* __restrict__ PP, DGrid G) {
     }
   }
 }
+__global__ void k_project(const DParams* __restrict__ PP, DGrid G) { project_body(PP, G, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // g2p (wgsl-solver.js:185-275)
@@ -1483,10 +1504,11 @@ __device__ __forceinline__ float4 d_foamRow(const DParams& P, float csum) {
   return make_float4(air, shield, st_expf(-P.dt / tau), 1.0f - st_expf(-kd * P.dt));
 }
 
-__global__ void k_foam_table(const DParams* __restrict__ PP, DGrid G) {
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < FOAM_TAB; t += gridDim.x * blockDim.x)
+__device__ __forceinline__ void foam_table_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < FOAM_TAB; t += nbid * blockDim.x)
     G.foamTab[t] = d_foamRow(*PP, float(t));
 }
+__global__ void k_foam_table(const DParams* __restrict__ PP, DGrid G) { foam_table_body(PP, G, blockIdx.x, gridDim.x); }
 
 template <class GridT>
 __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __restrict__ C, const DGrid& G, const GridT& g,
@@ -1581,12 +1603,12 @@ __device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint3
   if (keys) bin_mark(P, G, k, valid, po);
 }
 
-__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G, uint32_t keys) {
+__device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t keys, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   __shared__ float2 sNO[3][WIN3];
   __shared__ int32_t sC[WIN3];
   __shared__ uint32_t wslots[27];
-  for (uint32_t t = blockIdx.x; t <= P.ntiles; t += gridDim.x) {
+  for (uint32_t t = bid; t <= P.ntiles; t += nbid) {
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     if (beg == end) continue;
     if (t == P.ntiles) {
@@ -1645,6 +1667,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP,
     __syncthreads();
   }
 }
+__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G, uint32_t keys) { g2p_tile_body(PP, C, G, keys, blockIdx.x, gridDim.x); }
 
 // kept grid velocity at a point, all three components (wgsl-core.js:185-197)
 __device__ __forceinline__ float d_velGAxis(const DParams& P, const DGrid& G, float3 x, uint32_t axis) {
@@ -1717,7 +1740,7 @@ __device__ __forceinline__ float d_phiSample(const DParams& P, const DGrid& G, f
 }
 
 // tileMarkPhi (wgsl-narrowband.js:6-10): last step's tiles holding water by depth
-__global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void nb_mark_phi_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u;
@@ -1731,6 +1754,7 @@ __global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) {
         for (int32_t ti = lx; ti <= hx; ti++) G.need[uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk))] = 1u;
   }
 }
+__global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) { nb_mark_phi_body(PP, G, blockIdx.x, gridDim.x); }
 // tileMarkFill (wgsl-narrowband.js:13-18): on a start, every tile the starting water reaches
 __global__ void k_nb_mark_fill(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
   const DParams& P = *PP;
@@ -1743,7 +1767,7 @@ __global__ void k_nb_mark_fill(const DParams* __restrict__ PP, const DCols* __re
   if (d_fillPhi(P, C, G.sdf, q) < 9.0f) G.need[t] = 1u;
 }
 // phiInitAll (wgsl-narrowband.js:21-35)
-__global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+__device__ __forceinline__ void nb_phi_init_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1761,8 +1785,9 @@ __global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __res
     }
   }
 }
+__global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { nb_phi_init_body(PP, C, G, blockIdx.x, gridDim.x); }
 // nbPhi (wgsl-narrowband.js:39-49): depth carried by the kept grid velocity
-__global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void nb_phi_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1775,8 +1800,9 @@ __global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) {
     G.rhs[c] = inside ? st_clampf(phA, -7.9f, -0.5f) : st_clampf(phA, 0.5f, 7.9f);
   }
 }
+__global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) { nb_phi_body(PP, G, blockIdx.x, gridDim.x); }
 // jacNb, nbJacA (fromRhs: phi <- from rhs) and nbJacB (rhs <- from phi) (wgsl-narrowband.js:51-68)
-__global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromRhs) {
+__device__ __forceinline__ void nb_jac_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t fromRhs, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1802,9 +1828,10 @@ __global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromR
     else G.rhs[c] = out;
   }
 }
+__global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromRhs) { nb_jac_body(PP, G, fromRhs, blockIdx.x, gridDim.x); }
 // nbVel (wgsl-narrowband.js:71-85): water faces no particle reached take the kept
 // grid velocity, carried along
-__global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) {
+__device__ __forceinline__ void nb_vel_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1824,8 +1851,9 @@ __global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) {
     }
   }
 }
+__global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) { nb_vel_body(PP, G, blockIdx.x, gridDim.x); }
 // saveVel (wgsl-narrowband.js:88-91)
-__global__ void k_save_vel(DGrid G) {
+__device__ __forceinline__ void save_vel_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
   FOR_ACTIVE(G) {
     uint32_t c = G.slotList[t >> 9] * 512u + (t & 511u);
     G.velG[c * 3] = gvR(G, c * 3);
@@ -1833,6 +1861,7 @@ __global__ void k_save_vel(DGrid G) {
     G.velG[c * 3 + 2] = gvR(G, c * 3 + 2);
   }
 }
+__global__ void k_save_vel(DGrid G) { save_vel_body(G, blockIdx.x, gridDim.x); }
 // nbKill (wgsl-narrowband.js:94-102): calm particles below the band are removed
 __global__ void k_nb_kill(const DParams* __restrict__ PP, DGrid G) {
   const DParams& P = *PP;
@@ -1935,6 +1964,107 @@ __global__ void k_unsort(const float4* __restrict__ src, const uint32_t* __restr
   if (i >= n) return;
   out[pid[i]] = src[i];
 }
+
+// ---------------------------------------------------------------------------
+// Batched tanks: one launch per stage for every tank of a batch.  The grid is
+// the concatenation of the tanks' own grids; start[t] is the first workgroup of
+// tank t (start[nt] the total), so a workgroup finds its tank with one ballot
+// over the start table (tanks whose range is empty are passed over) and runs the
+// same body as the single-tank kernel with its place within the tank's range.
+
+struct BPos {
+  uint32_t t, blk, nblk;
+};
+__device__ __forceinline__ BPos batch_pos(const uint32_t* __restrict__ start, uint32_t nt) {
+  const uint32_t b = blockIdx.x, ln = lane_id();
+  uint32_t cnt = 0;
+  for (uint32_t base = 0; base < nt; base += 64u) {
+    const uint32_t t = base + ln;
+    cnt += uint32_t(__popcll(__ballot(t < nt && start[t] <= b)));
+    if (cnt < base + 64u) break;  // the last tank starting at or before b is in this chunk
+  }
+  BPos r;
+  r.t = cnt - 1u;
+  r.blk = b - start[r.t];
+  r.nblk = start[r.t + 1u] - start[r.t];
+  return r;
+}
+
+// The tank records are read through the constant address space, as kernel
+// arguments are: the loads are then invariant, so the compiler reloads a
+// pointer where it is used instead of holding it in registers for the whole
+// kernel (the single-tank kernels get the same from their by-value DGrid).
+typedef const BTank __attribute__((address_space(4))) ConstBTank;
+typedef const DParams __attribute__((address_space(4))) ConstDParams;
+typedef const DCols __attribute__((address_space(4))) ConstDCols;
+#define BATCH_ARGS const BTank* __restrict__ TT, const uint32_t* __restrict__ start, uint32_t nt
+#define BATCH_TANK                     \
+  const BPos b = batch_pos(start, nt); \
+  const BTank& T = *(const BTank*)((ConstBTank*)TT + b.t); \
+  const DParams* TP = (const DParams*)(ConstDParams*)T.P;   \
+  const DCols* TC = (const DCols*)(ConstDCols*)T.C
+
+// need[0..ntiles) and binCount[0..ntiles] to zero
+__global__ void __launch_bounds__(WG) kb_fill_bins(BATCH_ARGS) {
+  BATCH_TANK;
+  const uint32_t nti = TP->ntiles;
+  for (uint32_t i = b.blk * WG + threadIdx.x; i <= nti; i += b.nblk * WG) {
+    if (i < nti) T.G.need[i] = 0u;
+    T.G.binCount[i] = 0u;
+  }
+}
+__global__ void __launch_bounds__(WG) kb_mark_bin(BATCH_ARGS) { BATCH_TANK; mark_bin_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_tile_count(BATCH_ARGS) { BATCH_TANK; tile_count_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(1024) kb_tile_scan(BATCH_ARGS) {
+  BATCH_TANK;
+  tile_scan_body(T.G, (TP->ntiles + WG - 1u) / WG, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(WG) kb_tile_scatter(BATCH_ARGS) { BATCH_TANK; tile_scatter_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_tile_alloc(BATCH_ARGS) { BATCH_TANK; tile_alloc_body(T.G, b.blk, b.nblk); }
+__global__ void kb_init_new(BATCH_ARGS) { BATCH_TANK; init_new_body(TP, TC, T.G, b.blk, b.nblk); }
+__global__ void kb_nbr(BATCH_ARGS) { BATCH_TANK; nbr_body(TP, T.G, b.blk, b.nblk); }
+// the bin scan: binCount (ntiles + 1 entries) to binStart
+__global__ void __launch_bounds__(WG) kb_scan_blocks(BATCH_ARGS) {
+  BATCH_TANK;
+  scan_blocks_body(T.G.binCount, T.G.binStart, TP->ntiles + 1u, T.G.scanTmp, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(1024) kb_scan_top(BATCH_ARGS) {
+  BATCH_TANK;
+  const uint32_t n = TP->ntiles + 1u;
+  scan_top_body(T.G.scanTmp, (n + WG * 4u - 1u) / (WG * 4u), T.G.binStart, n, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(WG) kb_scan_add(BATCH_ARGS) {
+  BATCH_TANK;
+  scan_add_body(T.G.binStart, TP->ntiles + 1u, T.G.scanTmp, nullptr, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(WG) kb_bin_scatter(BATCH_ARGS) { BATCH_TANK; bin_scatter_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_clear(BATCH_ARGS) { BATCH_TANK; clear_body(T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_p2g_tile(BATCH_ARGS) { BATCH_TANK; p2g_tile_body(TP, T.G, b.blk, b.nblk); }
+// reduces the partials of the launch whose block table is `of`
+__global__ void __launch_bounds__(1024) kb_reduce_stats(BATCH_ARGS, const uint32_t* __restrict__ of, uint32_t which) {
+  BATCH_TANK;
+  reduce_stats_body(T.G, of[b.t + 1u] - of[b.t], which, b.blk, b.nblk);
+}
+__global__ void kb_solid_build(BATCH_ARGS) { BATCH_TANK; solid_build_body(TP, TC, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_classify(BATCH_ARGS) { BATCH_TANK; classify_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_norm_faces(BATCH_ARGS) { BATCH_TANK; norm_faces_body(TP, TC, T.G, b.blk, b.nblk); }
+__global__ void kb_divergence(BATCH_ARGS) { BATCH_TANK; divergence_body(TP, T.G, b.blk, b.nblk); }
+// one SOR iteration, pres -> pres2 (odd = 0) or back; every tank's range is a
+// multiple of 8 workgroups starting at a multiple of 8, so a workgroup's place
+// within its tank keeps the XCD it runs on
+__global__ void __launch_bounds__(WG) kb_sor_pipe(BATCH_ARGS, uint32_t odd) {
+  BATCH_TANK;
+  sor_pipe_body(TP, T.G, odd ? T.G.pres2 : T.G.pres, odd ? T.G.pres : T.G.pres2, ST_NONE, b.blk, b.nblk);
+}
+__global__ void kb_copy_tiles(BATCH_ARGS) { BATCH_TANK; copy_tiles_body(T.G, T.G.pres2, T.G.pres, ST_NONE, b.blk, b.nblk); }
+// the path word of the solve just done: one launch per iteration
+__global__ void kb_sor_path(BATCH_ARGS) {
+  BATCH_TANK;
+  if (threadIdx.x == 0 && b.blk == 0) T.G.sorBar[COOP_MAX_WG + 3] = 0u;
+}
+__global__ void kb_project(BATCH_ARGS) { BATCH_TANK; project_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_foam_table(BATCH_ARGS) { BATCH_TANK; foam_table_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_g2p_tile(BATCH_ARGS) { BATCH_TANK; g2p_tile_body(TP, TC, T.G, 1u, b.blk, b.nblk); }
 
 // ---------------------------------------------------------------------------
 // launch wrappers
@@ -2108,4 +2238,101 @@ void launch_g2p(hipStream_t st, const LaunchCtx& L) {
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n) {
   if (!n) return;
   hipLaunchKernelGGL(k_unsort, dim3(gridFor(n, WG)), dim3(WG), 0, st, src, pid, out, n);
+}
+
+// ---------------------------------------------------------------------------
+// batched substep (host side)
+
+size_t batch_table_words(uint32_t nt, uint32_t itersMax) { return size_t(BT_PIPE0 + itersMax) * (nt + 1u); }
+
+void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_t sorResident, uint32_t* tab) {
+  uint64_t scapTot = 0;
+  for (uint32_t t = 0; t < nt; t++)
+    if (plan[t].n) scapTot += plan[t].scap;
+  const size_t w = size_t(nt) + 1u;
+  for (uint32_t k = 0; k < uint32_t(BT_PIPE0) + itersMax; k++) {
+    uint32_t* s = tab + k * w;
+    uint32_t acc = 0;
+    for (uint32_t t = 0; t < nt; t++) {
+      const BTankPlan& p = plan[t];
+      uint32_t c = 0;
+      if (p.n) {
+        const uint32_t cell = std::max<uint32_t>(1u, std::min<uint32_t>(p.scap * 2u, 4096u));
+        const uint32_t nwg = gridFor(p.ntiles, WG), bins = gridFor(size_t(p.ntiles) + 1u, WG);
+        switch (k) {
+          case BT_FILL_MARK: c = p.binned ? 0u : bins; break;
+          case BT_MARK: c = p.binned ? 0u : gridFor(p.n, WG); break;
+          case BT_NWG: c = nwg; break;
+          case BT_ONE: c = 1u; break;
+          case BT_ALLOC: c = std::max<uint32_t>(1u, std::min<uint32_t>(nwg, 256u)); break;
+          case BT_CELL: c = cell; break;
+          case BT_NBR: c = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(p.scap) * 27u, WG), 2048u)); break;
+          case BT_SCAN: c = gridFor(size_t(p.ntiles) + 1u, WG * 4u); break;
+          case BT_PART: c = gridFor(p.n, WG); break;
+          case BT_TILE: c = std::max<uint32_t>(1u, std::min<uint32_t>(p.scap, 8192u)); break;
+          case BT_ONE_N: c = 1u; break;
+          case BT_SOLID: c = p.solid ? cell : 0u; break;
+          case BT_COPY: c = (p.iters & 1u) ? cell : 0u; break;
+          case BT_FOAM: c = FOAM_TAB / WG; break;
+          case BT_G2P: c = std::min<uint32_t>(p.ntiles + 1u, 8192u); break;
+          case BT_FILL_NEXT: c = bins; break;
+          default: {
+            // SOR: the tank's share of the resident workgroups by pool size,
+            // whole groups of 8, at least one group and at most one workgroup
+            // per slot (rounded up to 8)
+            if (k - uint32_t(BT_PIPE0) >= p.iters) break;
+            const uint32_t most = (p.scap + 7u) & ~7u;
+            uint32_t share = uint32_t((uint64_t(sorResident) * p.scap + scapTot - 1u) / scapTot);
+            share = (share + 7u) & ~7u;
+            c = std::max<uint32_t>(8u, std::min<uint32_t>(share, most));
+          }
+        }
+      }
+      s[t] = acc;
+      acc += c;
+    }
+    s[nt] = acc;
+  }
+}
+
+void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax) {
+  const size_t w = size_t(nt) + 1u;
+  auto tab = [&](uint32_t k) { return dtab + k * w; };
+  auto tot = [&](uint32_t k) { return htab[k * w + nt]; };
+#define BLAUNCH(kern, k, threads, ...) \
+  if (tot(k)) hipLaunchKernelGGL(kern, dim3(tot(k)), dim3(threads), 0, st, dT, tab(k), nt, ##__VA_ARGS__)
+  // blocks: marks and bins (unless the last g2p made them), tile allocation,
+  // neighbour slots, bin scan and bin list
+  BLAUNCH(kb_fill_bins, BT_FILL_MARK, WG);
+  BLAUNCH(kb_mark_bin, BT_MARK, WG);
+  BLAUNCH(kb_tile_count, BT_NWG, WG);
+  BLAUNCH(kb_tile_scan, BT_ONE, 1024);
+  BLAUNCH(kb_tile_scatter, BT_NWG, WG);
+  BLAUNCH(kb_tile_alloc, BT_ALLOC, WG);
+  BLAUNCH(kb_init_new, BT_CELL, WG);
+  BLAUNCH(kb_nbr, BT_NBR, WG);
+  BLAUNCH(kb_scan_blocks, BT_SCAN, WG);
+  BLAUNCH(kb_scan_top, BT_ONE, 1024);
+  BLAUNCH(kb_scan_add, BT_SCAN, WG);
+  BLAUNCH(kb_bin_scatter, BT_PART, WG);
+  // clear, p2g
+  BLAUNCH(kb_clear, BT_CELL, WG);
+  BLAUNCH(kb_p2g_tile, BT_TILE, WG);
+  BLAUNCH(kb_reduce_stats, BT_ONE_N, 1024, tab(BT_TILE), 0u);
+  // classify, normFaces, divergence
+  BLAUNCH(kb_solid_build, BT_SOLID, WG);
+  BLAUNCH(kb_classify, BT_CELL, WG);
+  BLAUNCH(kb_reduce_stats, BT_ONE, 1024, tab(BT_CELL), 1u);
+  BLAUNCH(kb_norm_faces, BT_CELL, WG);
+  BLAUNCH(kb_divergence, BT_CELL, WG);
+  // pressure: one launch per iteration, ping-pong between pres and pres2
+  for (uint32_t it = 0; it < itersMax; it++) BLAUNCH(kb_sor_pipe, uint32_t(BT_PIPE0) + it, WG, it & 1u);
+  BLAUNCH(kb_copy_tiles, BT_COPY, WG);
+  BLAUNCH(kb_sor_path, BT_ONE, 64);
+  // project, g2p (which marks and bins the next substep)
+  BLAUNCH(kb_project, BT_CELL, WG);
+  BLAUNCH(kb_fill_bins, BT_FILL_NEXT, WG);
+  BLAUNCH(kb_foam_table, BT_FOAM, WG);
+  BLAUNCH(kb_g2p_tile, BT_G2P, WG);
+#undef BLAUNCH
 }

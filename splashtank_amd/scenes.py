@@ -245,3 +245,33 @@ def nb_pool(d=1.0 / 64, depth=0.5, band=3, seed=4, iters=30, dt=1.0 / 240, drop=
     sc = Scene("nb_pool", d, (1.0, 1.0, 1.0), 0, pos, vel, prm)
     sc.nb_fills = packed
     return sc
+
+
+def batched_tanks(count=64, seed=3, nmin=32, nmax=128, iters=40, dt=1.0 / 240):
+    """Independent tanks for batched stepping: per tank a unit box of n^3 cells
+    with n log-uniform in [nmin, nmax] rounded to a multiple of 8, a cell/spacing
+    ratio in [1.59, 2] (4 to 8 particles per cell) and fluid fractions fx, fy
+    from U(0.2, 0.6).  Even tanks are dam breaks (fluid x in [-0.5, -0.5 + fx],
+    y in [0, fy], full z); odd tanks drop a ball of radius fx / 3 from y = 0.7
+    into a pool of depth fy / 2.  All walls closed, zero initial velocity."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax))) / 8.0)) * 8
+        n = max(8, n)
+        ratio = float(rng.uniform(1.59, 2.0))
+        fx, fy = (float(v) for v in rng.uniform(0.2, 0.6, 2))
+        h = 1.0 / n
+        d = h / ratio
+        prng = np.random.default_rng(int(rng.integers(1 << 31)))
+        if k % 2 == 0:
+            pos, vel = fill_box(d, -0.5, -0.5 + fx, 0.0, fy, -0.5, 0.5, prng)
+            name = f"tank{k}_dam_n{n}"
+        else:
+            p1, v1 = fill_box(d, -0.5, 0.5, 0.0, 0.5 * fy, -0.5, 0.5, prng)
+            p2, v2 = fill_sphere(d, 0.0, 0.7, 0.0, fx / 3.0, prng)
+            pos, vel = np.concatenate([p1, p2]), np.concatenate([v1, v2])
+            name = f"tank{k}_drop_n{n}"
+        prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0)
+        out.append(Scene(name, d, (1.0, 1.0, 1.0), 0, pos, vel, prm, ratio=ratio))
+    return out
