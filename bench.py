@@ -7,7 +7,8 @@ iterations, dt = 1/480, f32.  Particles and grid stay in HBM; the timed region
 is K full substeps (every stage, colliders updated each substep).
 
 Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
-(measured with HIP events on the sim's stream), and the CPU oracle timed on a
+(measured with HIP events on the sim's stream), the batched-tanks throughput
+next to the same tanks stepped one at a time, and the CPU oracle timed on a
 bounded sample of the same scene on this host.
 """
 from __future__ import annotations
@@ -97,6 +98,75 @@ def run_gpu(sc, args):
     return wall, alive, stats, stage_ms
 
 
+def run_batched(args):
+    """Batched tanks: 64 independent tanks (n^3 cells, n log-uniform in 32..128,
+    4-8 particles per cell, 40 SOR iterations) advanced (a) one tank after
+    another with st_step, each on its own stream, and (b) together with
+    st_batch_step, one launch per kernel for all tanks.  Same scenes, same
+    warm-up, same number of timed substeps; fresh tanks for each way."""
+    from splashtank_amd import capi, scenes
+    tanks = scenes.batched_tanks(count=args.batch_tanks, seed=3)
+    ntot = sum(sc.n for sc in tanks)
+    prm = capi.make_params(**tanks[0].params)
+
+    def make():
+        sims = []
+        for sc in tanks:
+            s = capi.FlipSimC(capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n, device=0))
+            s.upload(sc.pos, sc.vel)
+            sims.append(s)
+        return sims
+
+    def sync(sims):
+        for s in sims:
+            s.sync()
+
+    K, W = args.batch_steps, args.batch_warmup
+    log(f"[bench] batched tanks: {len(tanks)} tanks, {ntot} particles; one tank at a time")
+    sims = make()
+    for _ in range(W):
+        for s in sims:
+            s.step(prm)
+    sync(sims)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        for s in sims:
+            s.step(prm)
+    sync(sims)
+    seq = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    log(f"[bench] batched tanks: one launch per kernel for all tanks")
+    sims = make()
+    batch = capi.FlipBatchC(sims)
+    for _ in range(W):
+        batch.step(prm)
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        batch.step(prm)
+    batch.sync()
+    bat = time.perf_counter() - t0
+    # algorithmic bytes of one substep summed over the tanks (the model of the
+    # single-tank line, each tank with its own blocks, fluid cells and tiles)
+    byts = 0
+    for sc, s in zip(tanks, sims):
+        ctl = s.ctl()
+        n = int(round(1.0 / (sc.d * sc.ratio)))
+        T = ((n + 1 + 7) // 8) ** 3
+        byts += 112 * sc.n + 224 * int(ctl[4]) * 512 + (20 + 16 * int(sc.params["iters"])) * int(ctl[0] + ctl[1]) + 16 * T
+    batch.close()
+    for s in sims:
+        s.close()
+    return {"workload": f"{len(tanks)}_tanks_32-128^3", "tanks": len(tanks), "particles": ntot,
+            "sor_iters": int(tanks[0].params["iters"]), "steps": K, "warmup": W,
+            "value": ntot * K / bat, "unit": "particle-substeps/s", "ms_per_step": bat * 1e3 / K,
+            "sequential_value": ntot * K / seq, "sequential_ms_per_step": seq * 1e3 / K,
+            "speedup_vs_sequential": seq / bat,
+            "substep_roofline": {"algorithmic_bytes": byts, "achieved": byts / (bat / K) / 1e9, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": byts / (bat / K) / 1e9 / HBM_PEAK_GBS}}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM-side bytes per dispatch of `kernel` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
@@ -163,6 +233,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=-1, help="substeps timed stage by stage (default: all timed ones)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-tanks line")
+    ap.add_argument("--batch-tanks", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-warmup", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the substep runs on a single MI355X; "
@@ -210,6 +284,8 @@ def main():
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": sub_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    if not args.no_batch:
+        out["batched"] = run_batched(args)
     if not args.no_cpu:
         v, cores, cw = cpu_baseline(sc, args)
         out["cpu_baseline"] = {"value": v, "unit": "particle-substeps/s", "cores": cores, "kind": "port",

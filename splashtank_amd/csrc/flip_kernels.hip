@@ -2002,7 +2002,10 @@ typedef const DCols __attribute__((address_space(4))) ConstDCols;
   const BPos b = batch_pos(start, nt); \
   const BTank& T = *(const BTank*)((ConstBTank*)TT + b.t); \
   const DParams* TP = (const DParams*)(ConstDParams*)T.P;   \
-  const DCols* TC = (const DCols*)(ConstDCols*)T.C
+  const DCols* TC = (const DCols*)(ConstDCols*)T.C;         \
+  __builtin_assume_dereferenceable(&T, sizeof(BTank));      \
+  __builtin_assume_dereferenceable(TP, sizeof(DParams));    \
+  __builtin_assume_dereferenceable(TC, sizeof(DCols))
 
 // need[0..ntiles) and binCount[0..ntiles] to zero
 __global__ void __launch_bounds__(WG) kb_fill_bins(BATCH_ARGS) {
@@ -2245,10 +2248,25 @@ void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float
 
 size_t batch_table_words(uint32_t nt, uint32_t itersMax) { return size_t(BT_PIPE0 + itersMax) * (nt + 1u); }
 
+// Grid-stride kernels take any number of workgroups per tank.  A single tank
+// launches up to 2 per slot (cells), 1 per slot (p2g) or 1 per bin (g2p); across
+// a batch that many would mostly find no tile in use and exit, so the batch
+// splits a fixed total over the tanks by pool size (at least one each, never
+// more than the single-tank grid).
+#define BATCH_CELL_WG 4096u
+#define BATCH_TILE_WG 16384u
+static uint32_t batch_share(uint64_t total, uint64_t part, uint64_t whole, uint32_t most) {
+  uint64_t c = whole ? (total * part + whole - 1u) / whole : 1u;
+  return uint32_t(std::max<uint64_t>(1u, std::min<uint64_t>(c, most)));
+}
+
 void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_t sorResident, uint32_t* tab) {
-  uint64_t scapTot = 0;
+  uint64_t scapTot = 0, binTot = 0;
   for (uint32_t t = 0; t < nt; t++)
-    if (plan[t].n) scapTot += plan[t].scap;
+    if (plan[t].n) {
+      scapTot += plan[t].scap;
+      binTot += plan[t].ntiles + 1u;
+    }
   const size_t w = size_t(nt) + 1u;
   for (uint32_t k = 0; k < uint32_t(BT_PIPE0) + itersMax; k++) {
     uint32_t* s = tab + k * w;
@@ -2257,7 +2275,7 @@ void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_
       const BTankPlan& p = plan[t];
       uint32_t c = 0;
       if (p.n) {
-        const uint32_t cell = std::max<uint32_t>(1u, std::min<uint32_t>(p.scap * 2u, 4096u));
+        const uint32_t cell = batch_share(BATCH_CELL_WG, p.scap, scapTot, std::max<uint32_t>(1u, std::min<uint32_t>(p.scap * 2u, 4096u)));
         const uint32_t nwg = gridFor(p.ntiles, WG), bins = gridFor(size_t(p.ntiles) + 1u, WG);
         switch (k) {
           case BT_FILL_MARK: c = p.binned ? 0u : bins; break;
@@ -2269,12 +2287,12 @@ void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_
           case BT_NBR: c = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(size_t(p.scap) * 27u, WG), 2048u)); break;
           case BT_SCAN: c = gridFor(size_t(p.ntiles) + 1u, WG * 4u); break;
           case BT_PART: c = gridFor(p.n, WG); break;
-          case BT_TILE: c = std::max<uint32_t>(1u, std::min<uint32_t>(p.scap, 8192u)); break;
+          case BT_TILE: c = batch_share(BATCH_TILE_WG, p.scap, scapTot, std::max<uint32_t>(1u, std::min<uint32_t>(p.scap, 8192u))); break;
           case BT_ONE_N: c = 1u; break;
           case BT_SOLID: c = p.solid ? cell : 0u; break;
           case BT_COPY: c = (p.iters & 1u) ? cell : 0u; break;
           case BT_FOAM: c = FOAM_TAB / WG; break;
-          case BT_G2P: c = std::min<uint32_t>(p.ntiles + 1u, 8192u); break;
+          case BT_G2P: c = batch_share(BATCH_TILE_WG, p.ntiles + 1u, binTot, std::min<uint32_t>(p.ntiles + 1u, 8192u)); break;
           case BT_FILL_NEXT: c = bins; break;
           default: {
             // SOR: the tank's share of the resident workgroups by pool size,

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "splashtank.h"
 
@@ -327,12 +328,66 @@ napi_value Version(napi_env env, napi_callback_info) {
   return v;
 }
 
+// batchCreate([handles]) -> batch handle (st_batch_create)
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  bool isArr = false;
+  napi_is_array(env, argv[0], &isArr);
+  if (!isArr) return throwErr(env, "batchCreate: expects an array of sim handles");
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[0], &n);
+  std::vector<void*> sims(n);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value e;
+    NAPI_OK(napi_get_element(env, argv[0], k, &e));
+    sims[k] = handleOf(env, e);
+  }
+  void* b = nullptr;
+  ST_OK_OR_THROW(st_batch_create(sims.data(), n, &b));
+  napi_value h;
+  NAPI_OK(napi_create_external(env, b, nullptr, nullptr, &h));
+  return h;
+}
+
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  void* b = handleOf(env, argv[0]);
+  if (b) ST_OK_OR_THROW(st_batch_destroy(b));
+  return nullptr;
+}
+
+// batchStep(batch, [params per tank]) (st_batch_step)
+napi_value BatchStep(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  std::vector<st_params> p(n);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value e;
+    NAPI_OK(napi_get_element(env, argv[1], k, &e));
+    readParams(env, e, &p[k]);
+  }
+  ST_OK_OR_THROW(st_batch_step(handleOf(env, argv[0]), p.data(), n));
+  return nullptr;
+}
+
+napi_value BatchSync(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  ST_OK_OR_THROW(st_batch_sync(handleOf(env, argv[0])));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   const struct { const char* n; napi_callback f; } fns[] = {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"version", Version}};
+      {"memBytes", MemBytes}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;
     napi_create_function(env, e.n, NAPI_AUTO_LENGTH, e.f, nullptr, &f);

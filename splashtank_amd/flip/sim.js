@@ -269,4 +269,31 @@ class FlipSim {
   pump() {}
 }
 
-module.exports = { FlipSim, initHip, packColliders, STAGE_IDS };
+// Many tanks stepped together, one kernel launch per stage for all of them
+// (st_batch_*; the reference steps one FlipSim at a time).  Each tank keeps its
+// FlipSim API; step(dt, o) takes one `o` for all tanks or an array, one per tank,
+// and gives every tank the result its own step(dt, o) would.
+class FlipBatch {
+  constructor(sims) {
+    this.sims = sims.slice();
+    this.handle = loadAddon().batchCreate(this.sims.map((s) => s.handle));
+  }
+  step(dt, o) {
+    const os = Array.isArray(o) ? o : this.sims.map(() => o);
+    if (os.length !== this.sims.length) throw new Error('FlipBatch.step: one o per tank');
+    const prms = [];
+    let any = null;
+    this.sims.forEach((s, k) => {
+      const t = s.prepare(dt, os[k]);
+      prms.push(t ? t.prm : null);
+      if (t) { any = t.prm; s.statsCache = null; s.stepped = true; }
+    });
+    if (!any) return;
+    // tanks without particles sit the step out; any valid block stands in for theirs
+    loadAddon().batchStep(this.handle, prms.map((p) => p || any));
+  }
+  sync() { loadAddon().batchSync(this.handle); }
+  destroy() { if (this.handle) { loadAddon().batchDestroy(this.handle); this.handle = null; } }
+}
+
+module.exports = { FlipSim, FlipBatch, initHip, packColliders, STAGE_IDS };
