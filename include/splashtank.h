@@ -226,6 +226,34 @@ int st_batch_sync(void* batch);
 /* Number of tanks. */
 int st_batch_size(void* batch, uint32_t* n);
 
+/* Snapshots: a copy of the state a substep starts from (particles, tile tables,
+ * pool pressure and cell types, time and seed), for re-running substeps from
+ * it (the checkpoints of a gradient computation).  No counterpart in the
+ * reference.  The pool must not grow between create and restore. */
+int st_snapshot_create(void* sim, void** snap);
+int st_snapshot_restore(void* sim, void* snap);
+int st_snapshot_destroy(void* snap);
+
+/* Adjoint of the substep (the north-star differentiation extension; no
+ * counterpart in the reference).  st_adj_record runs one substep exactly as
+ * st_step does and keeps what its backward pass reads (a stack of up to
+ * max_steps tapes).  st_adj_seed sets dL/d(positions, velocities) of the
+ * current state (host float4 arrays in particle-id order, w ignored) and clears
+ * the pressure carry; each st_adj_backward then reverses the last recorded
+ * substep still on the stack (g2p, projection, the red-black SOR transposed in
+ * reverse sweep order, divergence, normFaces, p2g), turning the gradient into
+ * that of the state before it; st_adj_grad reads it back.  The derivative is
+ * taken at fixed topology (tiles, cell types, counts and branches as the forward
+ * pass took them; the fixed-point sums differentiated as the sums they round).
+ * Covered: closed boxes without colliders, drag or the narrow band
+ * (ST_ERR_ARG otherwise). */
+int st_adj_create(void* sim, uint32_t max_steps, void** adj);
+int st_adj_destroy(void* adj);
+int st_adj_record(void* adj, const st_params* p);
+int st_adj_seed(void* adj, const float* g_pos4, const float* g_vel4);
+int st_adj_backward(void* adj);
+int st_adj_grad(void* adj, float* g_pos4, float* g_vel4);
+
 #ifdef __cplusplus
 }
 #endif

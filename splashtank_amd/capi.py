@@ -126,6 +126,15 @@ PROTOS = {
     "st_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32]),
     "st_batch_sync": (C.c_int, [C.c_void_p]),
     "st_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "st_snapshot_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "st_snapshot_restore": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "st_snapshot_destroy": (C.c_int, [C.c_void_p]),
+    "st_adj_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "st_adj_destroy": (C.c_int, [C.c_void_p]),
+    "st_adj_record": (C.c_int, [C.c_void_p, C.POINTER(StParams)]),
+    "st_adj_seed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "st_adj_backward": (C.c_int, [C.c_void_p]),
+    "st_adj_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -247,6 +256,11 @@ class FlipSimC(_Base):
         self._call("time_stages", self.h, C.byref(prm), ms.ctypes.data)
         return {STAGES[k]: float(ms[k]) for k in range(ST_STAGE_COUNT)}
 
+    def count(self):
+        cnt = C.c_uint32(0)
+        self._call("get_count", self.h, C.byref(cnt))
+        return cnt.value
+
     def read_particles(self, n=None):
         cnt = C.c_uint32(0)
         self._call("get_count", self.h, C.byref(cnt))
@@ -319,6 +333,84 @@ class FlipBatchC:
     def close(self):
         if getattr(self, "h", None):
             self.lib.st_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SnapshotC:
+    """A copy of a sim's state to restart substeps from (st_snapshot_*)."""
+
+    def __init__(self, sim):
+        self.lib = load()
+        self.sim = sim
+        h = C.c_void_p()
+        if self.lib.st_snapshot_create(sim.h, C.byref(h)) != 0:
+            raise StError(f"st_snapshot_create: {self.lib.st_last_error().decode()}")
+        self.h = h
+
+    def restore(self):
+        if self.lib.st_snapshot_restore(self.sim.h, self.h) != 0:
+            raise StError(f"st_snapshot_restore: {self.lib.st_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.st_snapshot_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AdjointC:
+    """Adjoint of the substep for one sim (st_adj_*): record substeps, seed the
+    gradient of the state after them, reverse them one at a time, read the
+    gradient of the state before.  Gradients are (N, 4) float32 in particle-id
+    order (w ignored)."""
+
+    def __init__(self, sim, max_steps):
+        self.lib = load()
+        self.sim = sim
+        h = C.c_void_p()
+        if self.lib.st_adj_create(sim.h, int(max_steps), C.byref(h)) != 0:
+            raise StError(f"st_adj_create: {self.lib.st_last_error().decode()}")
+        self.h = h
+
+    def _call(self, name, *args):
+        if getattr(self.lib, f"st_adj_{name}")(self.h, *args) != 0:
+            raise StError(f"st_adj_{name}: {self.lib.st_last_error().decode()}")
+
+    def record(self, prm):
+        self._call("record", C.byref(prm))
+
+    def seed(self, g_pos, g_vel):
+        n = self.sim.count()
+        gp = np.zeros((n, 4), np.float32)
+        gv = np.zeros((n, 4), np.float32)
+        gp[:, :3] = np.asarray(g_pos, np.float32).reshape(n, -1)[:, :3]
+        gv[:, :3] = np.asarray(g_vel, np.float32).reshape(n, -1)[:, :3]
+        self._call("seed", gp.ctypes.data, gv.ctypes.data)
+
+    def backward(self):
+        self._call("backward")
+
+    def grad(self):
+        n = self.sim.count()
+        gp = np.zeros((max(n, 1), 4), np.float32)
+        gv = np.zeros((max(n, 1), 4), np.float32)
+        self._call("grad", gp.ctypes.data, gv.ctypes.data)
+        return gp[:n, :3], gv[:n, :3]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.st_adj_destroy(self.h)
             self.h = None
 
     def __del__(self):

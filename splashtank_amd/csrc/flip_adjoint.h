@@ -1,0 +1,49 @@
+// flip_adjoint.h - adjoint (vector-Jacobian product) of the FLIP substep: the
+// recorded state of one substep and the launch of its backward pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "flip_device.h"
+
+// What the backward pass of one substep reads, recorded while the substep ran
+// forward.  Per-slot arrays cover the whole pool (scap * 512 cells).
+struct AdjTape {
+  float4* pos;          // particle state at the start of the substep (its A order)
+  float4* vel;
+  uint32_t* pid;
+  uint32_t* tmap0;      // tile map at the start of the substep
+  uint32_t* tmap;       // tile map, slot tiles and slot list after blocks
+  uint32_t* slotTile;
+  uint32_t* slotList;
+  uint32_t* ctl;        // control words at the end of the substep (ctl[4] = tiles in use)
+  uint32_t* ctype;      // cell types after classify
+  uint8_t* sten;        // SOR stencil byte (bit 7 fluid, bits 0-2 non-solid neighbours)
+  int32_t* accWraw;     // fixed-point weight sums after p2g (3 per cell)
+  int32_t* accVfin;     // new face velocity after project (f32 bits)
+  int32_t* accWfin;     // old face velocity (f32 bits)
+  DParams* P;           // the substep's parameter block (device copy)
+  uint32_t n;           // particles
+  uint32_t iters;       // SOR iterations
+};
+
+// Gradient buffers of one adjoint context (device).
+struct AdjGrad {
+  float4* gPos;   // dL/d(particle position), id order (N)
+  float4* gVel;   // dL/d(particle velocity), id order (N)
+  float4* gPos2;  // the same for the state before the substep being reversed
+  float4* gVel2;
+  float* gFN;     // per face: dL/d(new face velocity), then dL/d(velocity sum)
+  float* gFO;     // per face: dL/d(old face velocity), then dL/d(weight sum)
+  float* gPres;   // per cell: dL/d(pressure), SOR ping-pong with gPres2
+  float* gPres2;
+  float* gRhs;    // per cell: dL/d(rhs)
+  float* carry;   // per tile of the box (ntiles * 512): dL/d(pressure at the end of the
+                  // previous substep), through the next substep's warm start
+  uint32_t ncell; // scap * 512
+  uint32_t ncarry;
+};
+
+// the backward pass of one recorded substep: reads G.gPos/gVel (gradient of the
+// state after it), writes G.gPos2/gVel2 (gradient of the state before it) and the
+// pressure carry for the substep before
+void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles);

@@ -1,0 +1,320 @@
+// flip_adjoint.hip - hand-written adjoint kernels of the FLIP substep.
+//
+// The backward pass of one substep at fixed topology: the tile set, cell types,
+// particle counts and every branch (clamps, wall contacts) are those the forward
+// substep took, and the fixed-point sums of p2g are differentiated as the real
+// sums they round (the rounding is a step function; its derivative is taken as
+// one).  Stages, reversed:
+//   g2p       per particle: gradient of (position, velocity) after the substep ->
+//             velocity before it (FLIP term), position (advection, wall clamp,
+//             trilinear weights) and the new/old face velocities it sampled
+//             (scattered with f32 atomics; order does not matter at the f32 bar)
+//   project   face gradient passes through; the pressure gets -+dt/h of it
+//   SOR       the red-black sweeps transposed, in reverse order (black then red
+//             per iteration): relaxing colour c, p_c <- (1-w) p_c + w/n (rhs_c +
+//             sum of fluid neighbours), has the transpose g_c <- (1-w) g_c,
+//             g_rhs_c += w/n g_c, g_m += w/n_c g_c for every fluid neighbour m
+//   (the gradient left on the warm start goes to the previous substep's pressure)
+//   divergence rhs = (tgt - div) h^2/dt  ->  face gradients
+//   normFaces v = V / W on faces with no solid side (old velocity the same,
+//             new minus g dt on y faces)  ->  gradients of the sums V and W
+//   p2g       per particle: V = sum w v, W = sum w over its 8 faces per axis
+//             -> velocity and position gradients
+// References: p2g wgsl-solver.js:16-40, normFaces 75-105, divergence 106-123,
+// relax 124-140, project 165-180, g2p 208-275, collideP 185-197.
+#include "flip_adjoint.h"
+
+#define AWG 256
+
+// sparse view of a recorded substep for the addressing helpers
+struct AView {
+  const DParams& P;
+  const AdjTape& T;
+  __device__ __forceinline__ uint32_t face(uint32_t axis, int32_t i, int32_t j, int32_t k) const {
+    return d_faceIndex(P, T.tmap, axis, i, j, k);
+  }
+  __device__ __forceinline__ uint32_t type(int32_t i, int32_t j, int32_t k) const { return d_ctypeOf(P, T.tmap, T.ctype, i, j, k); }
+  __device__ __forceinline__ uint32_t cell(int32_t i, int32_t j, int32_t k) const { return d_cellIdx(P, T.tmap, i, j, k); }
+};
+
+__device__ __forceinline__ float3 axisGrid(const DParams& P, float3 p, uint32_t axis) {
+  float3 g = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
+  if (axis == 0) { g.y -= 0.5f; g.z -= 0.5f; }
+  else if (axis == 1) { g.x -= 0.5f; g.z -= 0.5f; }
+  else { g.x -= 0.5f; g.y -= 0.5f; }
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// g2p (wgsl-solver.js:208-275) reversed.  Needs drag 0, no colliders, closed walls.
+__global__ void __launch_bounds__(AWG) k_adj_g2p(AdjTape T, AdjGrad G) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  const uint32_t i = blockIdx.x * AWG + threadIdx.x;
+  if (i >= T.n) return;
+  const float4 p4 = T.pos[i], v4 = T.vel[i];
+  const uint32_t id = T.pid[i];
+  if (p4.y < ST_DEADY) {
+    G.gPos2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    G.gVel2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const float3 p = c3(p4), v0 = c3(v4);
+  const int3 fc = d_particleCell(P, p);
+  const bool storage = T.tmap[d_tileOf(P, fc.x, fc.y, fc.z)] != ST_NONE;
+  // forward again: sampled new / old velocity, FLIP blend
+  float3 vn = make_float3(v0.x, v0.y - P.gravity * P.dt, v0.z), vo = v0;
+  if (storage) {
+    float s_n[3], s_o[3];
+    for (uint32_t a = 0; a < 3; a++) {
+      const float3 g = axisGrid(P, p, a);
+      const float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
+      const float3 fr = f3sub(g, f0);
+      const int3 i0 = make_int3(int32_t(f0.x), int32_t(f0.y), int32_t(f0.z));
+      float sn = 0.f, so = 0.f;
+      for (uint32_t c = 0; c < 8; c++) {
+        const int32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const float w = (dx ? fr.x : 1.f - fr.x) * (dy ? fr.y : 1.f - fr.y) * (dz ? fr.z : 1.f - fr.z);
+        const uint32_t f = V.face(a, i0.x + dx, i0.y + dy, i0.z + dz);
+        sn += w * __int_as_float(T.accVfin[f]);
+        so += w * __int_as_float(T.accWfin[f]);
+      }
+      s_n[a] = sn;
+      s_o[a] = so;
+    }
+    vn = make_float3(s_n[0], s_n[1], s_n[2]);
+    vo = make_float3(s_o[0], s_o[1], s_o[2]);
+  }
+  const float3 vm = f3add(vn, f3mul(f3sub(v0, vo), P.flip));
+  const float sp = f3len(vm);
+  const bool capped = sp > P.vmax;
+  const float3 vc = capped ? f3mul(vm, P.vmax / sp) : vm;
+  const float3 pt = f3add(p, f3mul(vc, P.dt));
+  const float m = P.margin;
+  const float3 lo = make_float3(P.x0 + m, P.y0 + m, P.z0 + m), hi = make_float3(P.x1 - m, P.y1 - m, P.z1 - m);
+  const float3 pc = make_float3(st_clampf(pt.x, lo.x, hi.x), st_clampf(pt.y, lo.y, hi.y), st_clampf(pt.z, lo.z, hi.z));
+  const float3 keep = make_float3(pt.x > lo.x && pt.x < hi.x ? 1.f : 0.f, pt.y > lo.y && pt.y < hi.y ? 1.f : 0.f,
+                                  pt.z > lo.z && pt.z < hi.z ? 1.f : 0.f);
+  const float3 dsp = f3sub(pc, pt);
+  bool wall = false;
+  float3 nrm = make_float3(0.f, 0.f, 0.f);
+  if (f3dot(dsp, dsp) > 1.0e-12f) {
+    nrm = f3div(dsp, f3len(dsp));
+    wall = f3dot(vc, nrm) < 0.f;
+  }
+  // backward
+  const float3 gpc = c3(G.gPos[id]), gvo = c3(G.gVel[id]);
+  float3 gvc = wall ? f3sub(gvo, f3mul(nrm, f3dot(nrm, gvo))) : gvo;
+  gvc = f3add(gvc, f3mul(make_float3(keep.x * gpc.x, keep.y * gpc.y, keep.z * gpc.z), P.dt));
+  float3 gp = make_float3(keep.x * gpc.x, keep.y * gpc.y, keep.z * gpc.z);
+  float3 gvm = gvc;
+  if (capped) {
+    const float3 u = f3div(vm, sp);
+    gvm = f3mul(f3sub(gvc, f3mul(u, f3dot(u, gvc))), P.vmax / sp);
+  }
+  float3 gv0 = gvm;  // without storage vn = v0 - g dt and vo = v0
+  if (storage) {
+    gv0 = f3mul(gvm, P.flip);
+    const float gvmA[3] = {gvm.x, gvm.y, gvm.z};
+    for (uint32_t a = 0; a < 3; a++) {
+      const float3 g = axisGrid(P, p, a);
+      const float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
+      const float3 fr = f3sub(g, f0);
+      const int3 i0 = make_int3(int32_t(f0.x), int32_t(f0.y), int32_t(f0.z));
+      const float gn = gvmA[a], go = -P.flip * gvmA[a];
+      for (uint32_t c = 0; c < 8; c++) {
+        const int32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const float wx = dx ? fr.x : 1.f - fr.x, wy = dy ? fr.y : 1.f - fr.y, wz = dz ? fr.z : 1.f - fr.z;
+        const float sx = dx ? P.ih : -P.ih, sy = dy ? P.ih : -P.ih, sz = dz ? P.ih : -P.ih;
+        const uint32_t f = V.face(a, i0.x + dx, i0.y + dy, i0.z + dz);
+        const float w = wx * wy * wz;
+        atomicAdd(&G.gFN[f], w * gn);
+        atomicAdd(&G.gFO[f], w * go);
+        const float s = gn * __int_as_float(T.accVfin[f]) + go * __int_as_float(T.accWfin[f]);
+        gp = f3add(gp, make_float3(sx * wy * wz * s, wx * sy * wz * s, wx * wy * sz * s));
+      }
+    }
+  }
+  G.gPos2[id] = make_float4(gp.x, gp.y, gp.z, 0.f);
+  G.gVel2[id] = make_float4(gv0.x, gv0.y, gv0.z, 0.f);
+}
+
+#define FOR_CELLS(T)                                                                            \
+  for (uint32_t t = blockIdx.x * AWG + threadIdx.x, tot_ = (T).ctl[4] * 512u; t < tot_; t += gridDim.x * AWG)
+
+// project (wgsl-solver.js:165-180) reversed: a face that was projected took
+// -dt/h (p_hi - p_lo); fluid cells gather from their six faces.  The carry from
+// the next substep's warm start is added here (fluid cells only: classify zeroes
+// the others before the warm start).
+__global__ void __launch_bounds__(AWG) k_adj_project(AdjTape T, AdjGrad G, uint32_t hasCarry) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  FOR_CELLS(T) {
+    const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
+    if ((T.ctype[c] & 0xffu) != 1u) {
+      G.gPres[c] = 0.f;
+      continue;
+    }
+    const int3 at = d_slotCoord(P, T.slotTile, s, l);
+    const float k = P.dt * P.ih;
+    float g = 0.f;
+    for (uint32_t a = 0; a < 3; a++) {
+      int3 lo = at, up = at;
+      if (a == 0) { lo.x -= 1; up.x += 1; } else if (a == 1) { lo.y -= 1; up.y += 1; } else { lo.z -= 1; up.z += 1; }
+      // own lower face: this cell is its high side
+      const uint32_t tl = V.type(lo.x, lo.y, lo.z);
+      if (tl != 2u) g -= k * G.gFN[c * 3u + a];
+      // upper face (the lower face of the cell above): this cell is its low side
+      const uint32_t tu = V.type(up.x, up.y, up.z);
+      if (tu != 2u) g += k * G.gFN[V.cell(up.x, up.y, up.z) * 3u + a];
+    }
+    if (hasCarry) g += G.carry[T.slotTile[s] * 512u + l];
+    G.gPres[c] = g;
+  }
+}
+
+// one colour of one red-black iteration, transposed: gin -> gout
+__global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const float* __restrict__ gin, float* __restrict__ gout,
+                                                 uint32_t colour) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  const float w = P.omega;
+  FOR_CELLS(T) {
+    const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
+    const uint32_t st = T.sten[c];
+    if (!(st & 0x80u)) {
+      gout[c] = 0.f;
+      continue;
+    }
+    const int3 at = d_slotCoord(P, T.slotTile, s, l);
+    if (uint32_t((at.x + at.y + at.z) & 1) == colour) {
+      const uint32_t n = st & 7u;
+      if (n == 0u) {
+        gout[c] = 0.f;
+        continue;
+      }
+      const float g = gin[c];
+      G.gRhs[c] += w / float(n) * g;
+      gout[c] = (1.f - w) * g;
+    } else {
+      float g = gin[c];
+      for (uint32_t q = 0; q < 6; q++) {
+        int3 r = at;
+        if (q == 0) r.x -= 1; else if (q == 1) r.x += 1; else if (q == 2) r.y -= 1; else if (q == 3) r.y += 1; else if (q == 4) r.z -= 1; else r.z += 1;
+        if (V.type(r.x, r.y, r.z) != 1u) continue;
+        const uint32_t cm = V.cell(r.x, r.y, r.z);
+        const uint32_t nm = T.sten[cm] & 7u;
+        if (nm) g += w / float(nm) * gin[cm];
+      }
+      gout[c] = g;
+    }
+  }
+}
+
+// the gradient left on the warm start of this substep's solve, by tile of the
+// box, for tiles that kept their slot from the previous substep (a new tile's
+// pressure starts at zero)
+__global__ void __launch_bounds__(AWG) k_adj_carry(AdjTape T, AdjGrad G, const float* __restrict__ g0) {
+  FOR_CELLS(T) {
+    const uint32_t s = T.slotList[t >> 9], l = t & 511u, tile = T.slotTile[s];
+    if (T.tmap0[tile] == s) G.carry[tile * 512u + l] = g0[s * 512u + l];
+  }
+}
+
+// divergence (106-123) reversed into the face gradient before projection, then
+// normFaces (75-105) reversed into gradients of the sums V (gFN) and W (gFO)
+__global__ void __launch_bounds__(AWG) k_adj_faces(AdjTape T, AdjGrad G) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  const float k = P.h * P.h / P.dt * P.ih;
+  FOR_CELLS(T) {
+    const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
+    const int3 at = d_slotCoord(P, T.slotTile, s, l);
+    const bool fluid = (T.ctype[c] & 0xffu) == 1u && d_inDom(P, at);
+    const float gr = fluid ? G.gRhs[c] : 0.f;
+    for (uint32_t a = 0; a < 3; a++) {
+      int3 lo = at;
+      if (a == 0) lo.x -= 1; else if (a == 1) lo.y -= 1; else lo.z -= 1;
+      const uint32_t tl = V.type(lo.x, lo.y, lo.z), th = V.type(at.x, at.y, at.z);
+      const uint32_t f = c * 3u + a;
+      float gn = G.gFN[f] + k * gr;  // this face is the cell's lower face: div has -v/h
+      if (tl == 1u && d_inDom(P, lo)) gn -= k * G.gRhs[V.cell(lo.x, lo.y, lo.z)];
+      float gV = 0.f, gW = 0.f;
+      const int32_t ws = T.accWraw[f];
+      if (tl != 2u && th != 2u && ws > 0) {
+        const float W = float(ws) * (1.0f / 65536.0f);
+        const float gs = gn + G.gFO[f];
+        gV = gs / W;
+        gW = -gs * __int_as_float(T.accWfin[f]) / W;
+      }
+      G.gFN[f] = gV;
+      G.gFO[f] = gW;
+    }
+  }
+}
+
+// p2g (16-40) reversed: per particle, the 8 faces per axis it added to
+__global__ void __launch_bounds__(AWG) k_adj_p2g(AdjTape T, AdjGrad G) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  const uint32_t i = blockIdx.x * AWG + threadIdx.x;
+  if (i >= T.n) return;
+  const float4 p4 = T.pos[i], v4 = T.vel[i];
+  if (p4.y < ST_DEADY) return;
+  const float3 p = c3(p4);
+  const int3 fc = d_particleCell(P, p);
+  if (T.tmap[d_tileOf(P, fc.x, fc.y, fc.z)] == ST_NONE) return;
+  const uint32_t id = T.pid[i];
+  const float va[3] = {v4.x, v4.y, v4.z};
+  float gv[3] = {0.f, 0.f, 0.f};
+  float3 gp = make_float3(0.f, 0.f, 0.f);
+  for (uint32_t a = 0; a < 3; a++) {
+    const float3 g = axisGrid(P, p, a);
+    const float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
+    const float3 fr = f3sub(g, f0);
+    const int3 i0 = make_int3(int32_t(f0.x), int32_t(f0.y), int32_t(f0.z));
+    for (uint32_t c = 0; c < 8; c++) {
+      const int32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+      const float wx = dx ? fr.x : 1.f - fr.x, wy = dy ? fr.y : 1.f - fr.y, wz = dz ? fr.z : 1.f - fr.z;
+      const float sx = dx ? P.ih : -P.ih, sy = dy ? P.ih : -P.ih, sz = dz ? P.ih : -P.ih;
+      const uint32_t f = V.face(a, i0.x + dx, i0.y + dy, i0.z + dz);
+      const float gV = G.gFN[f], gW = G.gFO[f];
+      gv[a] += wx * wy * wz * gV;
+      const float s = gV * va[a] + gW;
+      gp = f3add(gp, make_float3(sx * wy * wz * s, wx * sy * wz * s, wx * wy * sz * s));
+    }
+  }
+  const float4 a = G.gPos2[id], b = G.gVel2[id];
+  G.gPos2[id] = make_float4(a.x + gp.x, a.y + gp.y, a.z + gp.z, 0.f);
+  G.gVel2[id] = make_float4(b.x + gv[0], b.y + gv[1], b.z + gv[2], 0.f);
+}
+
+__global__ void k_adj_zero(float* __restrict__ p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * AWG + threadIdx.x; i < n; i += size_t(gridDim.x) * AWG) p[i] = 0.f;
+}
+
+static uint32_t grid_for(size_t n) { return uint32_t(std::min<size_t>((n + AWG - 1) / AWG, 8192)); }
+
+void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles) {
+  const size_t C = size_t(scap) * 512u;
+  const uint32_t cg = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
+  const uint32_t pg = uint32_t((T.n + AWG - 1) / AWG);
+  hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFN, C * 3);
+  hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFO, C * 3);
+  hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C)), dim3(AWG), 0, st, G.gRhs, C);
+  if (pg) hipLaunchKernelGGL(k_adj_g2p, dim3(pg), dim3(AWG), 0, st, T, G);
+  hipLaunchKernelGGL(k_adj_project, dim3(cg), dim3(AWG), 0, st, T, G, 1u);
+  // the solve reversed: per iteration, black then red, ping-pong gPres <-> gPres2
+  float* a = G.gPres;
+  float* b = G.gPres2;
+  for (uint32_t it = 0; it < T.iters; it++)
+    for (int colour = 1; colour >= 0; colour--) {
+      hipLaunchKernelGGL(k_adj_sor, dim3(cg), dim3(AWG), 0, st, T, G, (const float*)a, b, uint32_t(colour));
+      std::swap(a, b);
+    }
+  // a holds the gradient of the warm start: the carry for the substep before
+  hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(size_t(ntiles) * 512u)), dim3(AWG), 0, st, G.carry, size_t(ntiles) * 512u);
+  hipLaunchKernelGGL(k_adj_carry, dim3(cg), dim3(AWG), 0, st, T, G, (const float*)a);
+  hipLaunchKernelGGL(k_adj_faces, dim3(cg), dim3(AWG), 0, st, T, G);
+  if (pg) hipLaunchKernelGGL(k_adj_p2g, dim3(pg), dim3(AWG), 0, st, T, G);
+}

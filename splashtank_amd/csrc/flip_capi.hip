@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "flip_adjoint.h"
 #include "flip_device.h"
 #include "flip_kernels.h"
 #include "splashtank.h"
@@ -401,6 +402,55 @@ int ensureRing(Batch* b, uint32_t iters) {
   b->itersCap = iters;
   return ST_OK;
 }
+
+// A copy of the state a substep starts from (particles, tile tables, pool
+// pressure and cell types, host scalars), to re-run substeps from it.
+struct Snap {
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  uint32_t* pid = nullptr;
+  uint32_t *tmap = nullptr, *slotTile = nullptr, *fstack = nullptr, *slotList = nullptr, *ctl = nullptr, *ctype = nullptr;
+  float* pres = nullptr;
+  uint32_t N = 0, scap = 0, seed = 0;
+  double tm = 0;
+  bool solidDirty = true;
+};
+void freeSnap(Snap* q) {
+  dfree(q->pos); dfree(q->vel); dfree(q->pid); dfree(q->tmap); dfree(q->slotTile); dfree(q->fstack); dfree(q->slotList);
+  dfree(q->ctl); dfree(q->ctype); dfree(q->pres);
+  delete q;
+}
+
+// Adjoint context: the tapes of recorded substeps (a stack) and the gradients.
+struct Adj {
+  Sim* s = nullptr;
+  uint32_t maxSteps = 0, depth = 0, scap = 0;
+  std::vector<AdjTape> tapes;
+  AdjGrad G{};
+};
+void freeTape(AdjTape& T) {
+  dfree(T.pos); dfree(T.vel); dfree(T.pid); dfree(T.tmap0); dfree(T.tmap); dfree(T.slotTile); dfree(T.slotList); dfree(T.ctl);
+  dfree(T.ctype); dfree(T.sten); dfree(T.accWraw); dfree(T.accVfin); dfree(T.accWfin); dfree(T.P);
+}
+void freeAdj(Adj* a) {
+  for (AdjTape& T : a->tapes) freeTape(T);
+  AdjGrad& G = a->G;
+  dfree(G.gPos); dfree(G.gVel); dfree(G.gPos2); dfree(G.gVel2); dfree(G.gFN); dfree(G.gFO); dfree(G.gPres); dfree(G.gPres2);
+  dfree(G.gRhs); dfree(G.carry);
+  delete a;
+}
+int allocTape(Sim* s, AdjTape& T) {
+  size_t M = s->maxn, C = size_t(s->scap) * 512u;
+  int rc;
+  if ((rc = dalloc(&T.pos, M)) || (rc = dalloc(&T.vel, M)) || (rc = dalloc(&T.pid, M)) || (rc = dalloc(&T.tmap0, s->ntiles)) ||
+      (rc = dalloc(&T.tmap, s->ntiles)) || (rc = dalloc(&T.slotTile, s->scap)) || (rc = dalloc(&T.slotList, s->scap)) ||
+      (rc = dalloc(&T.ctl, 16)) || (rc = dalloc(&T.ctype, C)) || (rc = dalloc(&T.sten, C)) || (rc = dalloc(&T.accWraw, C * 3)) ||
+      (rc = dalloc(&T.accVfin, C * 3)) || (rc = dalloc(&T.accWfin, C * 3)) || (rc = dalloc(&T.P, 1)))
+    return rc;
+  return ST_OK;
+}
+
+#define D2D(dst, src, n) HIPCHK(hipMemcpyAsync((dst), (src), (n) * sizeof(*(src)), hipMemcpyDeviceToDevice, s->st))
 
 }  // namespace
 
@@ -982,6 +1032,157 @@ int st_batch_size(void* p, uint32_t* n) {
   Batch* b = (Batch*)p;
   if (!b || !n) return fail(ST_ERR_ARG, "st_batch_size: null argument");
   *n = uint32_t(b->tanks.size());
+  return ST_OK;
+}
+
+int st_snapshot_create(void* p, void** out) {
+  Sim* s = getSim(p);
+  if (!s || !out) return fail(ST_ERR_ARG, "st_snapshot_create: null argument");
+  *out = nullptr;
+  HIPCHK(hipSetDevice(s->dev));
+  Snap* q = new Snap();
+  size_t M = s->maxn, C = size_t(s->scap) * 512u;
+  int rc;
+  if ((rc = dalloc(&q->pos, M)) || (rc = dalloc(&q->vel, M)) || (rc = dalloc(&q->pid, M)) || (rc = dalloc(&q->tmap, s->ntiles)) ||
+      (rc = dalloc(&q->slotTile, s->scap)) || (rc = dalloc(&q->fstack, s->scap)) || (rc = dalloc(&q->slotList, s->scap)) ||
+      (rc = dalloc(&q->ctl, 16)) || (rc = dalloc(&q->ctype, C)) || (rc = dalloc(&q->pres, C))) {
+    freeSnap(q);
+    return rc;
+  }
+  D2D(q->pos, s->G.pos, M); D2D(q->vel, s->G.vel, M); D2D(q->pid, s->G.pid, M); D2D(q->tmap, s->G.tmap, s->ntiles);
+  D2D(q->slotTile, s->G.slotTile, s->scap); D2D(q->fstack, s->G.fstack, s->scap); D2D(q->slotList, s->G.slotList, s->scap);
+  D2D(q->ctl, s->G.ctl, 16); D2D(q->ctype, s->G.ctype, C); D2D(q->pres, s->G.pres, C);
+  HIPCHK(hipStreamSynchronize(s->st));
+  q->N = s->N; q->scap = s->scap; q->seed = s->seed; q->tm = s->tm; q->solidDirty = s->solidDirty;
+  *out = q;
+  return ST_OK;
+}
+
+int st_snapshot_restore(void* p, void* snap) {
+  Sim* s = getSim(p);
+  Snap* q = (Snap*)snap;
+  if (!s || !q) return fail(ST_ERR_ARG, "st_snapshot_restore: null argument");
+  if (q->scap != s->scap) return fail(ST_ERR_STATE, "st_snapshot_restore: the pool has grown since the snapshot");
+  HIPCHK(hipSetDevice(s->dev));
+  size_t M = s->maxn, C = size_t(s->scap) * 512u;
+  D2D(s->G.pos, q->pos, M); D2D(s->G.vel, q->vel, M); D2D(s->G.pid, q->pid, M); D2D(s->G.tmap, q->tmap, s->ntiles);
+  D2D(s->G.slotTile, q->slotTile, s->scap); D2D(s->G.fstack, q->fstack, s->scap); D2D(s->G.slotList, q->slotList, s->scap);
+  D2D(s->G.ctl, q->ctl, 16); D2D(s->G.ctype, q->ctype, C); D2D(s->G.pres, q->pres, C);
+  s->N = q->N; s->seed = q->seed; s->tm = q->tm; s->solidDirty = q->solidDirty;
+  s->binned = false;  // marks and bins are rebuilt from the restored positions
+  s->begun = false;
+  return ST_OK;
+}
+
+int st_snapshot_destroy(void* snap) {
+  if (!snap) return fail(ST_ERR_ARG, "st_snapshot_destroy: null snapshot");
+  freeSnap((Snap*)snap);
+  return ST_OK;
+}
+
+int st_adj_create(void* p, uint32_t max_steps, void** out) {
+  Sim* s = getSim(p);
+  if (!s || !out || max_steps == 0) return fail(ST_ERR_ARG, "st_adj_create: null argument or no steps");
+  *out = nullptr;
+  HIPCHK(hipSetDevice(s->dev));
+  Adj* a = new Adj();
+  a->s = s;
+  a->maxSteps = max_steps;
+  a->scap = s->scap;
+  size_t M = s->maxn, C = size_t(s->scap) * 512u;
+  AdjGrad& G = a->G;
+  int rc;
+  if ((rc = dalloc(&G.gPos, M)) || (rc = dalloc(&G.gVel, M)) || (rc = dalloc(&G.gPos2, M)) || (rc = dalloc(&G.gVel2, M)) ||
+      (rc = dalloc(&G.gFN, C * 3)) || (rc = dalloc(&G.gFO, C * 3)) || (rc = dalloc(&G.gPres, C)) || (rc = dalloc(&G.gPres2, C)) ||
+      (rc = dalloc(&G.gRhs, C)) || (rc = dalloc(&G.carry, size_t(s->ntiles) * 512u))) {
+    freeAdj(a);
+    return rc;
+  }
+  G.ncell = uint32_t(C);
+  G.ncarry = s->ntiles * 512u;
+  *out = a;
+  return ST_OK;
+}
+
+int st_adj_destroy(void* adj) {
+  Adj* a = (Adj*)adj;
+  if (!a) return fail(ST_ERR_ARG, "st_adj_destroy: null context");
+  (void)hipStreamSynchronize(a->s->st);
+  freeAdj(a);
+  return ST_OK;
+}
+
+int st_adj_record(void* adj, const st_params* prm) {
+  Adj* a = (Adj*)adj;
+  if (!a || !prm) return fail(ST_ERR_ARG, "st_adj_record: null argument");
+  Sim* s = a->s;
+  if (a->depth >= a->maxSteps) return fail(ST_ERR_STATE, "st_adj_record: every tape is in use (reverse some substeps first)");
+  if (s->scap != a->scap) return fail(ST_ERR_STATE, "st_adj_record: the pool has grown since the context was made");
+  if (prm->nb_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u || s->ncTotal > prm->nfv)
+    return fail(ST_ERR_ARG, "st_adj_record: the adjoint covers closed boxes without colliders, drag or the narrow band");
+  HIPCHK(hipSetDevice(s->dev));
+  if (a->tapes.size() <= a->depth) {
+    a->tapes.emplace_back();
+    std::memset(&a->tapes.back(), 0, sizeof(AdjTape));
+    int rc = allocTape(s, a->tapes.back());
+    if (rc) return rc;
+  }
+  AdjTape& T = a->tapes[a->depth];
+  size_t M = s->maxn, C = size_t(s->scap) * 512u;
+  D2D(T.pos, s->G.pos, M); D2D(T.vel, s->G.vel, M); D2D(T.pid, s->G.pid, M); D2D(T.tmap0, s->G.tmap, s->ntiles);
+  int rc = st_begin_step(s, prm);
+  if (rc) return rc;
+  T.n = s->N;
+  T.iters = uint32_t(s->iters);
+  D2D(T.P, &s->cur->P, 1);
+  for (int k = 0; k < ST_STAGE_COUNT; k++) {
+    if ((rc = runStage(s, k))) return rc;
+    if (k == ST_STAGE_BLOCKS) { D2D(T.tmap, s->G.tmap, s->ntiles); D2D(T.slotTile, s->G.slotTile, s->scap); D2D(T.slotList, s->G.slotList, s->scap); }
+    if (k == ST_STAGE_P2G) D2D(T.accWraw, s->G.accW, C * 3);
+    if (k == ST_STAGE_CLASSIFY) D2D(T.ctype, s->G.ctype, C);
+    if (k == ST_STAGE_DIVERGENCE) D2D(T.sten, s->G.sten, C);
+    if (k == ST_STAGE_PROJECT) { D2D(T.accVfin, s->G.accV, C * 3); D2D(T.accWfin, s->G.accW, C * 3); D2D(T.ctl, s->G.ctl, 16); }
+  }
+  a->depth++;
+  return ST_OK;
+}
+
+int st_adj_seed(void* adj, const float* g_pos4, const float* g_vel4) {
+  Adj* a = (Adj*)adj;
+  if (!a || !g_pos4 || !g_vel4) return fail(ST_ERR_ARG, "st_adj_seed: null argument");
+  Sim* s = a->s;
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipMemcpyAsync(a->G.gPos, g_pos4, size_t(s->N) * 16, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(a->G.gVel, g_vel4, size_t(s->N) * 16, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemsetAsync(a->G.carry, 0, size_t(a->G.ncarry) * 4, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ST_OK;
+}
+
+int st_adj_backward(void* adj) {
+  Adj* a = (Adj*)adj;
+  if (!a) return fail(ST_ERR_ARG, "st_adj_backward: null context");
+  if (a->depth == 0) return fail(ST_ERR_STATE, "st_adj_backward: no recorded substep left to reverse");
+  Sim* s = a->s;
+  HIPCHK(hipSetDevice(s->dev));
+  AdjTape& T = a->tapes[a->depth - 1];
+  launch_adjoint_step(s->st, T, a->G, a->scap, s->ntiles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("adjoint launch: ") + hipGetErrorString(e));
+  std::swap(a->G.gPos, a->G.gPos2);
+  std::swap(a->G.gVel, a->G.gVel2);
+  a->depth--;
+  return ST_OK;
+}
+
+int st_adj_grad(void* adj, float* g_pos4, float* g_vel4) {
+  Adj* a = (Adj*)adj;
+  if (!a || !g_pos4 || !g_vel4) return fail(ST_ERR_ARG, "st_adj_grad: null argument");
+  Sim* s = a->s;
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipMemcpyAsync(g_pos4, a->G.gPos, size_t(s->N) * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(g_vel4, a->G.gVel, size_t(s->N) * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
   return ST_OK;
 }
 

@@ -1,0 +1,204 @@
+"""The hand-written adjoint of the substep (st_adj_*, splashtank_amd/adjoint.py)
+against the FP64 reference gradient (torch autograd of tests/adjoint_ref.py,
+itself pinned against central differences and the oracle on the CPU), and
+against central differences of that FP64 forward with the discrete choices
+frozen.  Bar: relative error <= 1e-3, in norm and along 16 random directions."""
+import numpy as np
+import pytest
+import torch
+
+from splashtank_amd import adjoint, capi, scenes
+from splashtank_amd.capi import StError
+from oracle.oracle import Oracle
+from tests.adjoint_ref import RefSim
+from tests.helpers import opts_for
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-3
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _dirs_check(ga, gb, rng, k=16):
+    """Directional derivatives along k random directions: the worst difference,
+    relative to the RMS of the reference's directional derivatives (a random
+    direction can be nearly orthogonal to the gradient, which would make a
+    per-direction ratio meaningless)."""
+    xs, ys = [], []
+    for _ in range(k):
+        d = rng.standard_normal(ga.shape)
+        xs.append(float((ga * d).sum()))
+        ys.append(float((gb * d).sum()))
+    xs, ys = np.array(xs), np.array(ys)
+    return float(np.abs(xs - ys).max() / max(np.sqrt(np.mean(ys * ys)), 1e-30))
+
+
+def _gpu_clamps(sc, ref, dims, pos0, vel0, prm, K):
+    """The reference follows the box-clamp branches of the f32 run: a particle
+    resting on a wall has a normal velocity within rounding of zero, and which
+    side of the bound it lands on is decided by f32 rounding (where the
+    derivative itself jumps)."""
+    g = capi.FlipSimC(opts_for(sc))
+    try:
+        p4 = np.zeros((len(pos0), 4), np.float32)
+        v4 = np.zeros_like(p4)
+        p4[:, :3] = pos0
+        v4[:, :3] = vel0
+        g.upload(p4, v4)
+        out = []
+        for _ in range(K):
+            g.step(prm)
+            out.append(ref.wall_branches(dims, g.dump("pos")))
+        return out
+    finally:
+        g.close()
+
+
+def test_adjoint_matches_fp64_reference_linear_loss():
+    sc = scenes.dam_break(d=1.0 / 32, iters=20)  # 16^3 cells
+    rng = np.random.default_rng(1)
+    n = sc.n
+    vel0 = (sc.vel[:, :3] + 0.05 * rng.standard_normal((n, 3))).astype(np.float32)
+    pos0 = sc.pos[:, :3].astype(np.float32)
+    A = rng.standard_normal((n, 3))
+    B = rng.standard_normal((n, 3))
+    K = 5
+    prm = capi.make_params(**sc.params)
+    ro = adjoint.Rollout(opts_for(sc), prm, K, every=2)
+    try:
+        print("[adjoint] forward + backward, 16^3, 5 substeps, snapshots every 2", flush=True)
+        _, _, snaps = ro.forward_raw(pos0, vel0, keep_snaps=True)
+        gp, gv = ro.backward_raw(snaps, A.astype(np.float32), B.astype(np.float32))
+    finally:
+        ro.close()
+    dims = Oracle(opts_for(sc), fp64=True).dims
+    ref = RefSim(dims, sc.params)
+    clamps = _gpu_clamps(sc, ref, dims, pos0, vel0, prm, K)
+    p0 = torch.tensor(pos0, dtype=torch.float64, requires_grad=True)
+    v0 = torch.tensor(vel0, dtype=torch.float64, requires_grad=True)
+    pK, vK, _ = ref.rollout(p0, v0, K, clamps=clamps)
+    ((torch.tensor(A) * pK).sum() + (torch.tensor(B) * vK).sum()).backward()
+    rp, rv = p0.grad.numpy(), v0.grad.numpy()
+    ev, ep = _rel(gv, rv), _rel(gp, rp)
+    dv, dp = _dirs_check(gv, rv, rng), _dirs_check(gp, rp, rng)
+    print(f"[adjoint] rel err vel {ev:.2e} pos {ep:.2e}; 16 directions vel {dv:.2e} pos {dp:.2e}", flush=True)
+    assert ev <= RTOL and ep <= RTOL and dv <= RTOL and dp <= RTOL
+
+
+def test_adjoint_checkpoint_spacing_does_not_change_the_gradient():
+    sc = scenes.dam_break(d=1.0 / 32, iters=11)
+    rng = np.random.default_rng(2)
+    n = sc.n
+    pos0 = sc.pos[:, :3].astype(np.float32)
+    vel0 = (0.05 * rng.standard_normal((n, 3))).astype(np.float32)
+    A = rng.standard_normal((n, 3)).astype(np.float32)
+    prm = capi.make_params(**sc.params)
+    out = []
+    for every in (1, 3, 4):
+        ro = adjoint.Rollout(opts_for(sc), prm, 4, every=every)
+        try:
+            _, _, snaps = ro.forward_raw(pos0, vel0, keep_snaps=True)
+            out.append(ro.backward_raw(snaps, A, np.zeros_like(A)))
+        finally:
+            ro.close()
+    for gp, gv in out[1:]:
+        # the g2p face gradients are f32 atomics: order-dependent roundings only
+        assert _rel(gv, out[0][1]) < 1e-5 and _rel(gp, out[0][0]) < 1e-5
+
+
+def test_adjoint_density_loss_block_velocities():
+    """The inverse problem's gradient: per-tile initial velocities, a dam break,
+    the L2 distance of the particle density after K substeps to Gaussian blobs;
+    GPU adjoint through torch autograd (adjoint.Rollout) against the FP64
+    reference gradient and central differences of the FP64 forward."""
+    sc = scenes.dam_break(d=1.0 / 64, iters=20)  # 32^3 cells, 4^3 tiles
+    o = Oracle(opts_for(sc), fp64=True)
+    dims = o.dims
+    nx, ny, nz = dims.nx, dims.ny, dims.nz
+    origin = (dims.x0, dims.y0, dims.z0)
+    h = float(dims.h)
+    rng = np.random.default_rng(3)
+    cent = [(rng.uniform(-0.3, 0.3), rng.uniform(0.1, 0.4), rng.uniform(-0.3, 0.3)) for _ in range(3)]
+    target = torch.tensor(adjoint.gaussian_target((nx, ny, nz), origin, h, cent, 4 * h, float(dims.rest)))
+    cells = np.floor((sc.pos[:, :3] - np.array(origin)) / h).astype(np.int64)
+    tx, ty = dims.tx, dims.ty
+    blk = torch.tensor((cells[:, 0] >> 3) + tx * ((cells[:, 1] >> 3) + ty * (cells[:, 2] >> 3)))
+    nb = dims.tx * dims.ty * dims.tz
+    u0 = 0.2 * rng.standard_normal((nb, 3))
+    K = 6
+    prm = capi.make_params(**sc.params)
+    pos0 = torch.tensor(sc.pos[:, :3], dtype=torch.float64)
+
+    def loss(pK):
+        return ((adjoint.splat_density(pK, origin, h, (nx, ny, nz)) - target) ** 2).sum()
+
+    ro = adjoint.Rollout(opts_for(sc), prm, K, every=3)
+    try:
+        print("[adjoint] density loss, 32^3, 6 substeps", flush=True)
+        u = torch.tensor(u0, requires_grad=True)
+        pK, _ = ro(pos0.to(torch.float32), u[blk].to(torch.float32))
+        L = loss(pK.to(torch.float64))
+        L.backward()
+        g_gpu = u.grad.numpy().copy()
+    finally:
+        ro.close()
+    ref = RefSim(dims, sc.params)
+    clamps = _gpu_clamps(sc, ref, dims, sc.pos[:, :3].astype(np.float32),
+                         torch.tensor(u0)[blk].to(torch.float32).numpy(), prm, K)
+    ur = torch.tensor(u0, requires_grad=True)
+    pr, _, rec = ref.rollout(pos0, ur[blk], K, clamps=clamps)
+    Lr = loss(pr)
+    Lr.backward()
+    g_ref = ur.grad.numpy()
+    e = _rel(g_gpu, g_ref)
+    print(f"[adjoint] loss gpu {float(L):.6e} ref {float(Lr):.6e}; grad rel err {e:.2e}", flush=True)
+    assert abs(float(L) - float(Lr)) <= 1e-4 * abs(float(Lr))
+    assert e <= RTOL
+    # central differences of the FP64 forward (discrete choices frozen) along 16 directions
+    eps = 1e-6
+    fds, ans = [], []
+    for _ in range(16):
+        d = torch.tensor(rng.standard_normal((nb, 3)))
+        with torch.no_grad():
+            a, _, _ = ref.rollout(pos0, (torch.tensor(u0) + eps * d)[blk], K, frozen=rec)
+            b, _, _ = ref.rollout(pos0, (torch.tensor(u0) - eps * d)[blk], K, frozen=rec)
+            fds.append((float(loss(a)) - float(loss(b))) / (2 * eps))
+        ans.append(float((torch.tensor(g_gpu) * d).sum()))
+    fds, ans = np.array(fds), np.array(ans)
+    worst = float(np.abs(ans - fds).max() / np.sqrt(np.mean(fds * fds)))
+    worst_own = float((np.abs(ans - fds) / np.abs(fds)).max())
+    print(f"[adjoint] 16 directions against central differences: worst {worst:.2e} of the RMS, "
+          f"worst per direction {worst_own:.2e}", flush=True)
+    assert worst <= RTOL
+
+
+def test_adjoint_refuses_what_it_does_not_cover():
+    sc = scenes.sparse_splash(d=1.0 / 64, iters=5)  # has a box collider
+    g = capi.FlipSimC(opts_for(sc))
+    try:
+        g.upload(sc.pos, sc.vel)
+        cf, sdf, sig = sc.collider_state(0.0, sc.params["dt"])
+        g.set_colliders(cf, sdf, sig)
+        a = capi.AdjointC(g, 2)
+        with pytest.raises(StError):
+            a.record(capi.make_params(**sc.params))
+        a.close()
+    finally:
+        g.close()
+    sc = scenes.dam_break(d=1.0 / 32)
+    g = capi.FlipSimC(opts_for(sc))
+    try:
+        g.upload(sc.pos, sc.vel)
+        a = capi.AdjointC(g, 1)
+        with pytest.raises(StError):
+            a.record(capi.make_params(**dict(sc.params, drag=0.5)))
+        with pytest.raises(StError):
+            a.backward()  # nothing recorded
+        a.record(capi.make_params(**sc.params))
+        with pytest.raises(StError):
+            a.record(capi.make_params(**sc.params))  # one tape only
+        a.close()
+    finally:
+        g.close()
