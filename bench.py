@@ -8,7 +8,8 @@ is K full substeps (every stage, colliders updated each substep).
 
 Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
 (measured with HIP events on the sim's stream), the batched-tanks throughput
-next to the same tanks stepped one at a time, and the CPU oracle timed on a
+next to the same tanks stepped one at a time, the cost of the adjoint
+(gradient) pass against the forward pass, and the CPU oracle timed on a
 bounded sample of the same scene on this host.
 """
 from __future__ import annotations
@@ -167,6 +168,45 @@ def run_batched(args):
                                  "unit": "GB/s", "frac": byts / (bat / K) / 1e9 / HBM_PEAK_GBS}}
 
 
+def run_adjoint(args):
+    """The inverse problem's gradient at 64^3: a dam break (502,656 particles,
+    40 SOR iterations), 60 substeps with a snapshot every 10; the forward pass
+    (st_step, snapshots taken) and the backward pass (each segment re-run with
+    its tapes recorded, then reversed) timed on the GPU, per substep."""
+    from splashtank_amd import adjoint, capi, scenes
+    sc = scenes.dam_break(d=1.0 / 128, iters=40, dt=1.0 / 240)
+    K, every = args.adj_steps, 10
+    prm = capi.make_params(**sc.params)
+    opts = capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n, device=0)
+    ro = adjoint.Rollout(opts, prm, K, every=every)
+    rng = np.random.default_rng(4)
+    g_pos = rng.standard_normal((sc.n, 3)).astype(np.float32)
+    try:
+        log(f"[bench] adjoint: 64^3 dam break, {K} substeps, snapshot every {every}")
+        ro.forward_raw(sc.pos[:, :3], sc.vel[:, :3])  # warm-up
+        ro.sim.sync()
+        t0 = time.perf_counter()
+        ro.forward_raw(sc.pos[:, :3], sc.vel[:, :3])
+        ro.sim.sync()
+        plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, snaps = ro.forward_raw(sc.pos[:, :3], sc.vel[:, :3], keep_snaps=True)
+        ro.sim.sync()
+        fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ro.backward_raw(snaps, g_pos, np.zeros_like(g_pos))
+        bwd = time.perf_counter() - t0
+        for q in snaps:
+            q.close()
+    finally:
+        ro.close()
+    return {"workload": "dam_break_64^3", "particles": sc.n, "substeps": K, "checkpoint_every": every,
+            "forward_ms_per_substep": plain * 1e3 / K, "forward_with_snapshots_ms_per_substep": fwd * 1e3 / K,
+            "backward_ms_per_substep": bwd * 1e3 / K,
+            "backward_over_forward": bwd / plain,
+            "note": "backward = each segment re-run forward with its tapes recorded, then reversed (st_adj_backward)"}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM-side bytes per dispatch of `kernel` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
@@ -237,6 +277,8 @@ def main():
     ap.add_argument("--batch-tanks", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=5)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint line")
+    ap.add_argument("--adj-steps", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the substep runs on a single MI355X; "
@@ -286,6 +328,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = run_batched(args)
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint(args)
     if not args.no_cpu:
         v, cores, cw = cpu_baseline(sc, args)
         out["cpu_baseline"] = {"value": v, "unit": "particle-substeps/s", "cores": cores, "kind": "port",

@@ -11,6 +11,8 @@ struct AdjTape {
   float4* pos;          // particle state at the start of the substep (its A order)
   float4* vel;
   uint32_t* pid;
+  uint32_t* binStart;   // the substep's particle bins by tile (ntiles + 2 starts into binIdx)
+  uint32_t* binIdx;
   uint32_t* tmap0;      // tile map at the start of the substep
   uint32_t* tmap;       // tile map, slot tiles and slot list after blocks
   uint32_t* slotTile;

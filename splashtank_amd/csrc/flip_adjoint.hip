@@ -47,21 +47,22 @@ __device__ __forceinline__ float3 axisGrid(const DParams& P, float3 p, uint32_t 
 
 // ---------------------------------------------------------------------------
 // g2p (wgsl-solver.js:208-275) reversed.  Needs drag 0, no colliders, closed walls.
-__global__ void __launch_bounds__(AWG) k_adj_g2p(AdjTape T, AdjGrad G) {
-  const DParams& P = *T.P;
-  const AView V{P, T};
-  const uint32_t i = blockIdx.x * AWG + threadIdx.x;
-  if (i >= T.n) return;
+// One workgroup per tile bin (the bins the forward substep used): the face
+// gradients of the tile's particles gather in a 10^3-cell LDS window (the tile
+// plus one cell all round holds every face they sampled) and go to HBM once per
+// tile, so particles of one tile do not queue on the same global addresses.
+
+#define AWIN 10
+#define AWIN3 1000
+
+// per particle: the gradient of its state before the substep from that after
+// it; the face gradients go to add(axis, face, d new, d old)
+template <class Fetch, class Add>
+__device__ __forceinline__ void adj_g2p_particle(const DParams& P, const AdjTape& T, const AdjGrad& G, uint32_t i, bool storage,
+                                                 Fetch fetch, Add add) {
   const float4 p4 = T.pos[i], v4 = T.vel[i];
   const uint32_t id = T.pid[i];
-  if (p4.y < ST_DEADY) {
-    G.gPos2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    G.gVel2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
   const float3 p = c3(p4), v0 = c3(v4);
-  const int3 fc = d_particleCell(P, p);
-  const bool storage = T.tmap[d_tileOf(P, fc.x, fc.y, fc.z)] != ST_NONE;
   // forward again: sampled new / old velocity, FLIP blend
   float3 vn = make_float3(v0.x, v0.y - P.gravity * P.dt, v0.z), vo = v0;
   if (storage) {
@@ -75,9 +76,9 @@ __global__ void __launch_bounds__(AWG) k_adj_g2p(AdjTape T, AdjGrad G) {
       for (uint32_t c = 0; c < 8; c++) {
         const int32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
         const float w = (dx ? fr.x : 1.f - fr.x) * (dy ? fr.y : 1.f - fr.y) * (dz ? fr.z : 1.f - fr.z);
-        const uint32_t f = V.face(a, i0.x + dx, i0.y + dy, i0.z + dz);
-        sn += w * __int_as_float(T.accVfin[f]);
-        so += w * __int_as_float(T.accWfin[f]);
+        const float2 f = fetch(a, i0.x + dx, i0.y + dy, i0.z + dz);
+        sn += w * f.x;
+        so += w * f.y;
       }
       s_n[a] = sn;
       s_o[a] = so;
@@ -126,17 +127,86 @@ __global__ void __launch_bounds__(AWG) k_adj_g2p(AdjTape T, AdjGrad G) {
         const int32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
         const float wx = dx ? fr.x : 1.f - fr.x, wy = dy ? fr.y : 1.f - fr.y, wz = dz ? fr.z : 1.f - fr.z;
         const float sx = dx ? P.ih : -P.ih, sy = dy ? P.ih : -P.ih, sz = dz ? P.ih : -P.ih;
-        const uint32_t f = V.face(a, i0.x + dx, i0.y + dy, i0.z + dz);
+        const float2 f = fetch(a, i0.x + dx, i0.y + dy, i0.z + dz);
         const float w = wx * wy * wz;
-        atomicAdd(&G.gFN[f], w * gn);
-        atomicAdd(&G.gFO[f], w * go);
-        const float s = gn * __int_as_float(T.accVfin[f]) + go * __int_as_float(T.accWfin[f]);
+        add(a, i0.x + dx, i0.y + dy, i0.z + dz, w * gn, w * go);
+        const float s = gn * f.x + go * f.y;
         gp = f3add(gp, make_float3(sx * wy * wz * s, wx * sy * wz * s, wx * wy * sz * s));
       }
     }
   }
   G.gPos2[id] = make_float4(gp.x, gp.y, gp.z, 0.f);
   G.gVel2[id] = make_float4(gv0.x, gv0.y, gv0.z, 0.f);
+}
+
+// face (axis, i, j, k) as faceIndex clamps it, relative to the window origin
+__device__ __forceinline__ int32_t adj_window_face(const DParams& P, uint32_t a, int32_t i, int32_t j, int32_t k, int3 org) {
+  const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
+  i = d_iclamp(i, 0, a == 0 ? X : X - 1);
+  j = d_iclamp(j, 0, a == 1 ? Y : Y - 1);
+  k = d_iclamp(k, 0, a == 2 ? Z : Z - 1);
+  return (i - org.x) + AWIN * ((j - org.y) + AWIN * (k - org.z));
+}
+
+#define AG2P_WG 512
+__global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
+  const DParams& P = *T.P;
+  const AView V{P, T};
+  __shared__ float2 sF[3][AWIN3];  // (new, old) face velocity of the window
+  __shared__ float sN[3][AWIN3];   // gradient gathered for the new face velocity
+  __shared__ float sO[3][AWIN3];   // and for the old one
+  const int32_t gx = int32_t(P.tx) * 8, gy = int32_t(P.ty) * 8, gz = int32_t(P.tz) * 8;
+  auto none = [](uint32_t, int32_t, int32_t, int32_t) { return make_float2(0.f, 0.f); };
+  auto drop = [](uint32_t, int32_t, int32_t, int32_t, float, float) {};
+  for (uint32_t t = blockIdx.x; t <= P.ntiles; t += gridDim.x) {
+    const uint32_t beg = T.binStart[t], end = T.binStart[t + 1];
+    if (beg == end) continue;
+    if (t == P.ntiles) {  // dead particles: no gradient
+      for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG) {
+        const uint32_t id = T.pid[T.binIdx[k]];
+        G.gPos2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+        G.gVel2[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      continue;
+    }
+    if (T.tmap[t] == ST_NONE) {  // no storage this substep: free fall, no grid
+      for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG) adj_g2p_particle(P, T, G, T.binIdx[k], false, none, drop);
+      continue;
+    }
+    const int3 org = make_int3(int32_t(t % P.tx) * 8 - 1, int32_t((t / P.tx) % P.ty) * 8 - 1, int32_t(t / (P.tx * P.ty)) * 8 - 1);
+    for (uint32_t q = threadIdx.x; q < AWIN3; q += AG2P_WG) {
+      const int32_t gi = org.x + int32_t(q % AWIN), gj = org.y + int32_t((q / AWIN) % AWIN), gk = org.z + int32_t(q / (AWIN * AWIN));
+      const bool in = gi >= 0 && gj >= 0 && gk >= 0 && gi < gx && gj < gy && gk < gz;
+      const uint32_t c = in ? V.cell(gi, gj, gk) * 3u : 0u;
+      for (uint32_t a = 0; a < 3; a++) {
+        sF[a][q] = in ? make_float2(__int_as_float(T.accVfin[c + a]), __int_as_float(T.accWfin[c + a])) : make_float2(0.f, 0.f);
+        sN[a][q] = 0.f;
+        sO[a][q] = 0.f;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG)
+      adj_g2p_particle(
+          P, T, G, T.binIdx[k], true,
+          [&](uint32_t a, int32_t i, int32_t j, int32_t kk) { return sF[a][adj_window_face(P, a, i, j, kk, org)]; },
+          [&](uint32_t a, int32_t i, int32_t j, int32_t kk, float dn, float dol) {
+            const int32_t q = adj_window_face(P, a, i, j, kk, org);
+            atomicAdd(&sN[a][q], dn);
+            atomicAdd(&sO[a][q], dol);
+          });
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < AWIN3; q += AG2P_WG) {
+      const int32_t gi = org.x + int32_t(q % AWIN), gj = org.y + int32_t((q / AWIN) % AWIN), gk = org.z + int32_t(q / (AWIN * AWIN));
+      for (uint32_t a = 0; a < 3; a++) {
+        const float dn = sN[a][q], dol = sO[a][q];
+        if (dn == 0.f && dol == 0.f) continue;  // only faces particles sampled, all inside the grid
+        const uint32_t f = V.cell(gi, gj, gk) * 3u + a;
+        atomicAdd(&G.gFN[f], dn);
+        atomicAdd(&G.gFO[f], dol);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 #define FOR_CELLS(T)                                                                            \
@@ -302,7 +372,7 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFN, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFO, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C)), dim3(AWG), 0, st, G.gRhs, C);
-  if (pg) hipLaunchKernelGGL(k_adj_g2p, dim3(pg), dim3(AWG), 0, st, T, G);
+  if (pg) hipLaunchKernelGGL(k_adj_g2p, dim3(std::min<uint32_t>(ntiles + 1u, 8192u)), dim3(AG2P_WG), 0, st, T, G);
   hipLaunchKernelGGL(k_adj_project, dim3(cg), dim3(AWG), 0, st, T, G, 1u);
   // the solve reversed: per iteration, black then red, ping-pong gPres <-> gPres2
   float* a = G.gPres;

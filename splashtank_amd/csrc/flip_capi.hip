@@ -429,7 +429,7 @@ struct Adj {
   AdjGrad G{};
 };
 void freeTape(AdjTape& T) {
-  dfree(T.pos); dfree(T.vel); dfree(T.pid); dfree(T.tmap0); dfree(T.tmap); dfree(T.slotTile); dfree(T.slotList); dfree(T.ctl);
+  dfree(T.pos); dfree(T.vel); dfree(T.pid); dfree(T.binStart); dfree(T.binIdx); dfree(T.tmap0); dfree(T.tmap); dfree(T.slotTile); dfree(T.slotList); dfree(T.ctl);
   dfree(T.ctype); dfree(T.sten); dfree(T.accWraw); dfree(T.accVfin); dfree(T.accWfin); dfree(T.P);
 }
 void freeAdj(Adj* a) {
@@ -443,6 +443,7 @@ int allocTape(Sim* s, AdjTape& T) {
   size_t M = s->maxn, C = size_t(s->scap) * 512u;
   int rc;
   if ((rc = dalloc(&T.pos, M)) || (rc = dalloc(&T.vel, M)) || (rc = dalloc(&T.pid, M)) || (rc = dalloc(&T.tmap0, s->ntiles)) ||
+      (rc = dalloc(&T.binStart, size_t(s->ntiles) + 2)) || (rc = dalloc(&T.binIdx, M)) ||
       (rc = dalloc(&T.tmap, s->ntiles)) || (rc = dalloc(&T.slotTile, s->scap)) || (rc = dalloc(&T.slotList, s->scap)) ||
       (rc = dalloc(&T.ctl, 16)) || (rc = dalloc(&T.ctype, C)) || (rc = dalloc(&T.sten, C)) || (rc = dalloc(&T.accWraw, C * 3)) ||
       (rc = dalloc(&T.accVfin, C * 3)) || (rc = dalloc(&T.accWfin, C * 3)) || (rc = dalloc(&T.P, 1)))
@@ -1137,7 +1138,10 @@ int st_adj_record(void* adj, const st_params* prm) {
   D2D(T.P, &s->cur->P, 1);
   for (int k = 0; k < ST_STAGE_COUNT; k++) {
     if ((rc = runStage(s, k))) return rc;
-    if (k == ST_STAGE_BLOCKS) { D2D(T.tmap, s->G.tmap, s->ntiles); D2D(T.slotTile, s->G.slotTile, s->scap); D2D(T.slotList, s->G.slotList, s->scap); }
+    if (k == ST_STAGE_BLOCKS) {
+      D2D(T.tmap, s->G.tmap, s->ntiles); D2D(T.slotTile, s->G.slotTile, s->scap); D2D(T.slotList, s->G.slotList, s->scap);
+      D2D(T.binStart, s->G.binStart, size_t(s->ntiles) + 2); D2D(T.binIdx, s->G.binIdx, M);
+    }
     if (k == ST_STAGE_P2G) D2D(T.accWraw, s->G.accW, C * 3);
     if (k == ST_STAGE_CLASSIFY) D2D(T.ctype, s->G.ctype, C);
     if (k == ST_STAGE_DIVERGENCE) D2D(T.sten, s->G.sten, C);
