@@ -1,3 +1,4 @@
 'use strict';
-// The MI355X FlipSim for Node: see sim.js.
-module.exports = require('./sim.js');
+// The MI355X FlipSim for Node: the simulator (sim.js) and its scene driving
+// (runner.js: SceneRunner, keyframed poses, the adaptive-substep bake loop).
+module.exports = Object.assign({}, require('./sim.js'), require('./runner.js'));

@@ -1,0 +1,39 @@
+'use strict';
+// A short bake through runner.bake on the GPU: a dam break, a box collider
+// moving along x on keyframes, a water emitter and a dropped ball.  Prints one
+// JSON line: per frame the particle count, substeps, and whether every particle
+// is finite and inside the box.
+const path = require('path');
+const M = require(path.join(__dirname, '..', '..', 'splashtank_amd', 'flip'));
+
+// distance grid of a box of half-extent hb sampled on a cell-h grid with a margin
+function boxSdf(hb, h, margin) {
+  const lo = -hb - margin, n = Math.ceil((2 * (hb + margin)) / h) + 1, data = new Float32Array(n * n * n);
+  for (let k = 0; k < n; k++) for (let j = 0; j < n; j++) for (let i = 0; i < n; i++) {
+    const p = [lo + i * h, lo + j * h, lo + k * h], q = p.map((v) => Math.abs(v) - hb);
+    const out = Math.hypot(Math.max(q[0], 0), Math.max(q[1], 0), Math.max(q[2], 0)), inside = Math.min(Math.max(q[0], q[1], q[2]), 0);
+    data[i + n * (j + n * k)] = out + inside;
+  }
+  return { id: 'box', dims: [n, n, n], origin: [lo, lo, lo], cell: h, data };
+}
+const d = 1 / 64, h = 2 * d;
+const desc = {
+  d, maxN: 400000, size: [1, 1, 1], open: 0, ratio: 2, fps: 60, frames: 4, substeps: 2, maxSub: 12, cfl: 1.5, g: 9.81, iters: 20,
+  fills: [{ t: 'box', x0: -0.5, x1: -0.2, y0: 0, y1: 0.4, z0: -0.5, z1: 0.5 }],
+  objs: [
+    { id: 1, kind: 'mesh', sdf: boxSdf(0.08, h, 4 * h), smooth: false, keys: [{ t: 0, x: 0.1, y: 0.15, z: 0 }, { t: 1, x: 0.6, y: 0.15, z: 0 }] },
+    { id: 2, kind: 'emitter', on: true, x: 0.2, y: 0.8, z: 0, az: 0, el: -90, radius: 0.05, speed: 1.5 },
+  ],
+  events: [{ t: 0.03, x: 0.25, y: 0.6, z: 0.2, r: 0.06 }],
+};
+const frames = [];
+M.bake(desc, M.initHip(0), (fr, info) => {
+  let fin = true, inside = true;
+  for (let i = 0; i < fr.n; i++) {
+    const x = fr.pos[i * 4], y = fr.pos[i * 4 + 1], z = fr.pos[i * 4 + 2];
+    if (!(Number.isFinite(x) && Number.isFinite(y) && Number.isFinite(z))) fin = false;
+    else if (x < -0.5 || x > 0.5 || y < 0 || y > 1 || z < -0.5 || z > 0.5) inside = false;
+  }
+  frames.push({ f: info.f, n: fr.n, sub: info.sub, finite: fin, inside, objectSpeed: info.objectSpeed || 0 });
+}).then((ok) => console.log(JSON.stringify({ ok, frames })))
+  .catch((e) => { console.error(e.stack || String(e)); process.exit(1); });

@@ -1,0 +1,52 @@
+"""Scene driving on the JS facade (splashtank_amd/flip/runner.js): SceneRunner's
+collider velocities, emitters and keyframed poses against a recording stand-in
+sim (CPU), and a short bake through FlipSim on the GPU."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "splashtank_amd", "flip", "splashtank_hip.node")
+NODE = shutil.which("node")
+needs_node = pytest.mark.skipif(NODE is None, reason="node is missing")
+
+
+def _run(script):
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", script)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@needs_node
+def test_runner_collider_velocities_emitters_keyframes():
+    o = _run("runner_unit.js")
+    # no previous pose on the first step: no velocity; then the pose difference
+    assert o["firstVel"] == [0, 0]
+    assert abs(o["vel"][0] - 0.5) < 1e-9 and o["vel"][1] == 0 and o["vel"][2] == 0
+    # 2 rad/s about y from R Rprev^T: sin(w dt) / dt
+    assert abs(o["omega"][1] - o["omegaExpect"]) < 1e-12 and abs(o["omega"][0]) < 1e-12 and abs(o["omega"][2]) < 1e-12
+    assert o["jumpVel"] == 0  # 120 m/s between steps is a jump, not a motion
+    # a smooth emitter releases one lattice disc per spacing travelled, at the emitter speed
+    assert o["emitted"] == o["emittedExpect"] > 0
+    assert abs(o["emitVel"][1] + 2) < 1e-12
+    assert o["keyHeld"] == [0, 3] and o["keyLinear"] == 2
+    assert o["keySpline"][0] == 1 and o["keySpline"][1] == 3 and abs(o["keySpline"][2] - 2.0625) < 1e-12
+    assert abs(o["objSpeed"] - 1.0) < 1e-9
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.isfile(ADDON), reason="the N-API addon is not built")
+def test_bake_drives_flipsim():
+    o = _run("bake_run.js")
+    assert o["ok"]
+    fr = o["frames"]
+    assert len(fr) == 5 and fr[0]["sub"] == 0
+    assert all(f["finite"] and f["inside"] for f in fr)
+    # the emitter and the dropped ball add water; substeps follow the speeds
+    assert fr[-1]["n"] > fr[0]["n"]
+    assert all(2 <= f["sub"] <= 12 for f in fr[1:])
+    assert any(f["objectSpeed"] > 0.4 for f in fr[1:])
