@@ -50,3 +50,22 @@ def test_bake_drives_flipsim():
     assert fr[-1]["n"] > fr[0]["n"]
     assert all(2 <= f["sub"] <= 12 for f in fr[1:])
     assert any(f["objectSpeed"] > 0.4 for f in fr[1:])
+    # the frames as an SPLC cache: written, read back unchanged, positions within
+    # half a 16-bit step of the readback
+    assert o["cacheRoundTrip"] and o["cacheBytes"] > 0
+    assert o["worstQuant"] <= 0.5 / 65535 + 1e-7
+
+
+@needs_node
+def test_splc_cache_layout():
+    """The SPLC cache (cache.js) on hand-made frames: version 2 bytes at fixed
+    offsets, the round trip, and a hand-built version 1 file."""
+    o = _run("cache_unit.js")
+    assert o["n"] == 2  # the two dead slots after the last water point are dropped
+    assert o["pos"] == [32768, 32768, 49151, 0, 65535, 65535]
+    assert o["spd"] == [40, 191] and o["fm"] == [51, 0]  # 2.5 m/s in 16ths; 63.5 capped at 191; foam level 3 * 17
+    assert o["objs"] == [1, 2, 3, 0, 0.5, 0]
+    assert o["magic"] == "SPLC" and o["version"] == 2 and o["jsonLen"][0] == o["jsonLen"][1]
+    assert o["frames"] == 2 and o["frame0"] == {"n": 2, "flags": 1, "yaw": 0.5, "pos0": 32768, "spd1": 191, "fm0": 51}
+    assert o["length"][0] == o["length"][1] and o["round"]
+    assert o["v1"] == {"fps": 30, "n": 2, "yaw": -0.75, "pos": [1, 2, 3, 4, 5, 6], "spd": [33, 44], "ids": [9]}
