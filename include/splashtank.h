@@ -169,8 +169,9 @@ int st_set_colliders(void* sim, const float* cf32, uint32_t nc, const float* sdf
 
 /* FlipSim.step(dt, o): one full substep enqueued on the sim's stream (sim.js:179-241). */
 int st_step(void* sim, const st_params* p);
-/* Run several substeps with the same settings (captured into one HIP graph
- * when `use_graph` is set); used by benchmarks and bakes. */
+/* Run several substeps with the same settings, as many st_step calls; used by
+ * benchmarks and bakes.  `use_graph` must be 0: HIP graph capture is refused
+ * (ST_ERR_ARG), because the pressure solve is a cooperative launch. */
 int st_step_n(void* sim, const st_params* p, int32_t nsteps, int32_t use_graph);
 /* The parameter upload of step() without running a stage (stepTimed, sim.js:312-319). */
 int st_begin_step(void* sim, const st_params* p);

@@ -247,8 +247,8 @@ class FlipSimC(_Base):
         if rc != 0:
             raise StError(f"st_create -> {rc}: {self.lib.st_last_error().decode()}")
 
-    def step_n(self, prm: StParams, n: int, graph: bool = False):
-        self._call("step_n", self.h, C.byref(prm), int(n), 1 if graph else 0)
+    def step_n(self, prm: StParams, n: int):
+        self._call("step_n", self.h, C.byref(prm), int(n), 0)
 
     def time_stages(self, prm: StParams):
         """One substep with HIP events around each stage: {stage name: ms}."""

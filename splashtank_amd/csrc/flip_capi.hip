@@ -693,7 +693,7 @@ int st_step(void* p, const st_params* prm) {
 }
 
 int st_step_n(void* p, const st_params* prm, int32_t nsteps, int32_t use_graph) {
-  (void)use_graph;
+  if (use_graph) return fail(ST_ERR_ARG, "st_step_n: graph capture is not supported (the pressure solve is a cooperative launch)");
   for (int32_t k = 0; k < nsteps; k++) {
     int rc = st_step(p, prm);
     if (rc) return rc;
