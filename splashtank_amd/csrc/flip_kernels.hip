@@ -475,7 +475,13 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
-    for (uint32_t k = beg + threadIdx.x; k < end; k += WG) {
+    // the four waves take every fourth entry of each run of WG bin entries:
+    // neighbouring entries tend to be neighbouring particles (last substep's
+    // order), and lanes adding to the same LDS word in one instruction queue
+    static_assert(WG == 256, "four waves of 64 cover a run of WG entries");
+    for (uint32_t k0 = beg; k0 < end; k0 += WG) {
+      const uint32_t k = k0 + (threadIdx.x & 63u) * 4u + (threadIdx.x >> 6);
+      if (k >= end) continue;
       uint32_t i = G.binIdx[k];
       float4 p4 = G.pos[i];
       float4 v4 = G.vel[i];
