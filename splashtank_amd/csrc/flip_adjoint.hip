@@ -185,7 +185,12 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
       }
     }
     __syncthreads();
-    for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG)
+    // the waves interleave over each run of AG2P_WG bin entries (neighbouring
+    // entries are mostly neighbouring particles, whose LDS adds would queue)
+    static_assert(AG2P_WG == 512, "eight waves of 64 cover a run of AG2P_WG entries");
+    for (uint32_t k0 = beg; k0 < end; k0 += AG2P_WG) {
+      const uint32_t k = k0 + (threadIdx.x & 63u) * 8u + (threadIdx.x >> 6);
+      if (k >= end) continue;
       adj_g2p_particle(
           P, T, G, T.binIdx[k], true,
           [&](uint32_t a, int32_t i, int32_t j, int32_t kk) { return sF[a][adj_window_face(P, a, i, j, kk, org)]; },
@@ -194,6 +199,7 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
             atomicAdd(&sN[a][q], dn);
             atomicAdd(&sO[a][q], dol);
           });
+    }
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < AWIN3; q += AG2P_WG) {
       const int32_t gi = org.x + int32_t(q % AWIN), gj = org.y + int32_t((q / AWIN) % AWIN), gk = org.z + int32_t(q / (AWIN * AWIN));
