@@ -121,10 +121,13 @@ __device__ __forceinline__ void mark_bin_body(const DParams* __restrict__ PP, co
   float4 q = valid ? G.pos[i] : make_float4(0.f, -1.0e6f, 0.f, 0.f);
   bin_mark(P, G, i, valid, q);
 }
-__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) { mark_bin_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_mark_bin(const DParams* __restrict__ PP, DGrid G) {
+  mark_bin_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // exclusive scan of n counts into out[0..n], out[n] = total; three launches
-__device__ __forceinline__ void scan_blocks_body(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, uint32_t* __restrict__ sums, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void scan_blocks_body(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+                                                 uint32_t* __restrict__ sums, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t wsum[WG / 64];
   uint32_t base = bid * (WG * 4) + threadIdx.x * 4;
   uint32_t v[4], s = 0;
@@ -153,8 +156,11 @@ __device__ __forceinline__ void scan_blocks_body(const uint32_t* __restrict__ in
   if (threadIdx.x == 0) sums[bid] = tot;
 }
 __global__ void __launch_bounds__(WG) k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
-                                                    uint32_t* __restrict__ sums) { scan_blocks_body(in, out, n, sums, blockIdx.x, gridDim.x); }
-__device__ __forceinline__ void scan_top_body(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n, uint32_t bid, uint32_t nbid) {
+                                                    uint32_t* __restrict__ sums) {
+  scan_blocks_body(in, out, n, sums, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void scan_top_body(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out,
+                                              uint32_t n, uint32_t bid, uint32_t nbid) {
   __shared__ uint32_t part[1024 / 64];
   __shared__ uint32_t carry;
   if (threadIdx.x == 0) carry = 0;
@@ -181,8 +187,12 @@ __device__ __forceinline__ void scan_top_body(uint32_t* __restrict__ sums, uint3
   }
   if (threadIdx.x == 0) out[n] = carry;
 }
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out, uint32_t n) { scan_top_body(sums, nb, out, n, blockIdx.x, gridDim.x); }
-__device__ __forceinline__ void scan_add_body(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums, uint32_t* __restrict__ copy, uint32_t bid, uint32_t nbid) {
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ out,
+                                                   uint32_t n) {
+  scan_top_body(sums, nb, out, n, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void scan_add_body(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums,
+                                              uint32_t* __restrict__ copy, uint32_t bid, uint32_t nbid) {
   uint32_t base = bid * (WG * 4) + threadIdx.x * 4;
   uint32_t add = sums[bid];
   for (int q = 0; q < 4; q++)
@@ -193,7 +203,9 @@ __device__ __forceinline__ void scan_add_body(uint32_t* __restrict__ out, uint32
     }
 }
 __global__ void __launch_bounds__(WG) k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ sums,
-                                                 uint32_t* __restrict__ copy) { scan_add_body(out, n, sums, copy, blockIdx.x, gridDim.x); }
+                                                 uint32_t* __restrict__ copy) {
+  scan_add_body(out, n, sums, copy, blockIdx.x, gridDim.x);
+}
 
 // bin order as an index list: binIdx[binStart[bin] + rank] = i.  The particles
 // themselves stay where they are; p2g and g2p read them through the list (runs
@@ -203,7 +215,9 @@ __device__ __forceinline__ void bin_scatter_body(const DParams* __restrict__ PP,
   if (i >= PP->n) return;
   G.binIdx[G.binStart[G.binKey[i]] + G.binRank[i]] = i;
 }
-__global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) { bin_scatter_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_bin_scatter(const DParams* __restrict__ PP, DGrid G) {
+  bin_scatter_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // Flags of one tile: freed (no longer needed, had a slot), new (needed, had
 // none), kept (needed, had one).  tileFree in wgsl-blocks.js:16-21.
@@ -257,7 +271,9 @@ __device__ __forceinline__ void tile_count_body(const DParams* __restrict__ PP, 
     G.wg[bid * 3 + 2] = R.tot[2];
   }
 }
-__global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ PP, DGrid G) { tile_count_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_tile_count(const DParams* __restrict__ PP, DGrid G) {
+  tile_count_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // One workgroup: exclusive scan of the per-workgroup counts, then the stack
 // arithmetic of tileFree/tileAlloc/tileFix (wgsl-blocks.js:16-35) in closed form:
@@ -309,7 +325,9 @@ __device__ __forceinline__ void tile_scan_body(const DGrid& G, uint32_t nwg, uin
     G.ctl[14] = nalloc;
   }
 }
-__global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) { tile_scan_body(G, nwg, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) k_tile_scan(DGrid G, uint32_t nwg) {
+  tile_scan_body(G, nwg, blockIdx.x, gridDim.x);
+}
 
 // Freed tiles push their slot (tile order), new tiles are listed (tile order),
 // kept tiles take their place in slotList (tile order).
@@ -331,7 +349,9 @@ __device__ __forceinline__ void tile_scatter_body(const DParams* __restrict__ PP
     G.slotList[baseK + R.r[2] + min(nb, nalloc)] = G.tmap[t];
   }
 }
-__global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__ PP, DGrid G) { tile_scatter_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_tile_scatter(const DParams* __restrict__ PP, DGrid G) {
+  tile_scatter_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // tileAlloc (wgsl-blocks.js:26-33), deterministic pop order
 __device__ __forceinline__ void tile_alloc_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
@@ -348,7 +368,9 @@ __device__ __forceinline__ void tile_alloc_body(const DGrid& G, uint32_t bid, ui
     }
   }
 }
-__global__ void k_tile_alloc(DGrid G) { tile_alloc_body(G, blockIdx.x, gridDim.x); }
+__global__ void k_tile_alloc(DGrid G) {
+  tile_alloc_body(G, blockIdx.x, gridDim.x);
+}
 
 // Slots of the 27 tiles around every tile in use (itself included), index
 // (dx+1) + 3 (dy+1) + 9 (dz+1): 0 for a tile without storage (cellIdx maps it to
@@ -372,10 +394,13 @@ __device__ __forceinline__ void nbr_body(const DParams* __restrict__ PP, const D
     G.nbrL[e] = v;
   }
 }
-__global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) { nbr_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_nbr(const DParams* __restrict__ PP, DGrid G) {
+  nbr_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // initNew (wgsl-blocks.js:38-47): no pressure, collider lookup of each cell
-__device__ __forceinline__ void init_new_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void init_new_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G,
+                                              uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   uint32_t tot = G.ctl[3] * 512u;
   for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
@@ -393,7 +418,9 @@ __device__ __forceinline__ void init_new_body(const DParams* __restrict__ PP, co
     }
   }
 }
-__global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { init_new_body(PP, C, G, blockIdx.x, gridDim.x); }
+__global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  init_new_body(PP, C, G, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // clearSlots (wgsl-solver.js:5-15): every slot in use plus the spare slot 0
@@ -414,7 +441,9 @@ __device__ __forceinline__ void clear_body(const DGrid& G, uint32_t bid, uint32_
     if (s == 0u) G.rhs[c] = 8.0f;
   }
 }
-__global__ void k_clear(DGrid G) { clear_body(G, blockIdx.x, gridDim.x); }
+__global__ void k_clear(DGrid G) {
+  clear_body(G, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // Tile windows.  The window of tile (ti, tj, tk) is cells [8ti-1, 8ti+8] per axis.
@@ -545,7 +574,9 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
     G.part[bid] = m;
   }
 }
-__global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) { p2g_tile_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) {
+  p2g_tile_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // statistics partials -> control words: which 0: ctl[6] = max(ctl[6], vmax
 // partials); which 1: ctl[0], ctl[1] = sums of the per-colour fluid partials
@@ -570,7 +601,9 @@ __device__ __forceinline__ void reduce_stats_body(const DGrid& G, uint32_t n, ui
     else { G.ctl[0] = x; G.ctl[1] = y; }
   }
 }
-__global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint32_t which) { reduce_stats_body(G, n, which, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint32_t which) {
+  reduce_stats_body(G, n, which, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // per active cell kernels: t -> slot slotList[t >> 9], local t & 511 (acell,
@@ -580,7 +613,8 @@ __global__ void __launch_bounds__(1024) k_reduce_stats(DGrid G, uint32_t n, uint
        t += nbid * blockDim.x)
 
 // solidBuild (wgsl-solver.js:41-45)
-__device__ __forceinline__ void solid_build_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void solid_build_body(const DParams* __restrict__ PP, const DCols* __restrict__ C,
+                                                 const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -588,7 +622,9 @@ __device__ __forceinline__ void solid_build_body(const DParams* __restrict__ PP,
     G.ctype[c] = (G.ctype[c] & 0xffff00ffu) | (uint32_t(d_colAt(P, C, G.sdf, d_cellCenter(P, p.x, p.y, p.z)) + 1) << 8);
   }
 }
-__global__ void k_solid_build(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { solid_build_body(PP, C, G, blockIdx.x, gridDim.x); }
+__global__ void k_solid_build(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  solid_build_body(PP, C, G, blockIdx.x, gridDim.x);
+}
 
 __device__ __forceinline__ float3 d_velGVec(const DParams& P, const DGrid& G, float3 x);
 
@@ -641,10 +677,13 @@ __device__ __forceinline__ void classify_body(const DParams* __restrict__ PP, co
     G.part[2 * bid + 1] = b;
   }
 }
-__global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP, DGrid G) { classify_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_classify(const DParams* __restrict__ PP, DGrid G) {
+  classify_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // normBody / normFaces (wgsl-solver.js:75-105)
-__device__ __forceinline__ void norm_faces_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void norm_faces_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G,
+                                                uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -673,7 +712,9 @@ __device__ __forceinline__ void norm_faces_body(const DParams* __restrict__ PP, 
     }
   }
 }
-__global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { norm_faces_body(PP, C, G, blockIdx.x, gridDim.x); }
+__global__ void k_norm_faces(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  norm_faces_body(PP, C, G, blockIdx.x, gridDim.x);
+}
 
 __device__ __forceinline__ float gvR(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accV[f]); }
 __device__ __forceinline__ float gv0R(const DGrid& G, uint32_t f) { return st_u2f((uint32_t)G.accW[f]); }
@@ -718,7 +759,9 @@ __device__ __forceinline__ void divergence_body(const DParams* __restrict__ PP, 
     G.sten[c] = uint8_t(STEN_FLUID | nns);
   }
 }
-__global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) { divergence_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_divergence(const DParams* __restrict__ PP, DGrid G) {
+  divergence_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // relax / rbRed / rbBlack (wgsl-solver.js:124-140, 161-164; loop at
 // sim.js:291-293): one red-black iteration per launch, by tiles.  A workgroup
@@ -941,14 +984,17 @@ __device__ __forceinline__ void sor_pipe_pass(const DGrid& G, float omega, const
 
 // One red-black iteration per launch (the grid a multiple of 8 workgroups, as
 // many as are resident), skipped when the resident solve has done this stage.
-__device__ __forceinline__ void sor_pipe_body(const DParams* __restrict__ PP, const DGrid& G, const float* __restrict__ pin, float* __restrict__ pout, uint32_t seq, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void sor_pipe_body(const DParams* __restrict__ PP, const DGrid& G, const float* __restrict__ pin,
+                                              float* __restrict__ pout, uint32_t seq, uint32_t bid, uint32_t nbid) {
   __shared__ SorPipeLds L;
   if (G.sorBar[COOP_MAX_WG] == seq) return;
   sor_pipe_init(L, threadIdx.x);
   sor_pipe_pass(G, PP->omega, pin, pout, bid, nbid, threadIdx.x, L);
 }
 __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP, DGrid G, const float* __restrict__ pin,
-                                                 float* __restrict__ pout, uint32_t seq) { sor_pipe_body(PP, G, pin, pout, seq, blockIdx.x, gridDim.x); }
+                                                 float* __restrict__ pout, uint32_t seq) {
+  sor_pipe_body(PP, G, pin, pout, seq, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // Resident SOR: every iteration of one pressure solve in one cooperative launch,
@@ -1341,7 +1387,8 @@ __global__ void k_coop_compact(const DParams* __restrict__ PP, DGrid G) {
 }
 
 // after an odd number of iterations the result sits in the partner buffer
-__device__ __forceinline__ void copy_tiles_body(const DGrid& G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void copy_tiles_body(const DGrid& G, const float* __restrict__ src, float* __restrict__ dst,
+                                                uint32_t seq, uint32_t bid, uint32_t nbid) {
   if (G.sorBar[COOP_MAX_WG] == seq) return;
   uint32_t tot = G.ctl[4] * 512u;
   for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
@@ -1349,7 +1396,9 @@ __device__ __forceinline__ void copy_tiles_body(const DGrid& G, const float* __r
     dst[c] = src[c];
   }
 }
-__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq) { copy_tiles_body(G, src, dst, seq, blockIdx.x, gridDim.x); }
+__global__ void k_copy_tiles(DGrid G, const float* __restrict__ src, float* __restrict__ dst, uint32_t seq) {
+  copy_tiles_body(G, src, dst, seq, blockIdx.x, gridDim.x);
+}
 
 // projBody / project (wgsl-solver.js:165-180)
 __device__ __forceinline__ void project_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
@@ -1371,7 +1420,9 @@ __device__ __forceinline__ void project_body(const DParams* __restrict__ PP, con
     }
   }
 }
-__global__ void k_project(const DParams* __restrict__ PP, DGrid G) { project_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_project(const DParams* __restrict__ PP, DGrid G) {
+  project_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // g2p (wgsl-solver.js:185-275)
@@ -1514,7 +1565,9 @@ __device__ __forceinline__ void foam_table_body(const DParams* __restrict__ PP, 
   for (uint32_t t = bid * blockDim.x + threadIdx.x; t < FOAM_TAB; t += nbid * blockDim.x)
     G.foamTab[t] = d_foamRow(*PP, float(t));
 }
-__global__ void k_foam_table(const DParams* __restrict__ PP, DGrid G) { foam_table_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_foam_table(const DParams* __restrict__ PP, DGrid G) {
+  foam_table_body(PP, G, blockIdx.x, gridDim.x);
+}
 
 template <class GridT>
 __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __restrict__ C, const DGrid& G, const GridT& g,
@@ -1609,7 +1662,8 @@ __device__ __forceinline__ void g2p_emit(const DParams& P, const DGrid& G, uint3
   if (keys) bin_mark(P, G, k, valid, po);
 }
 
-__device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t keys, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G,
+                                              uint32_t keys, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   __shared__ float2 sNO[3][WIN3];
   __shared__ int32_t sC[WIN3];
@@ -1673,7 +1727,10 @@ __device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, co
     __syncthreads();
   }
 }
-__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G, uint32_t keys) { g2p_tile_body(PP, C, G, keys, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(WG) k_g2p_tile(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G,
+                                                 uint32_t keys) {
+  g2p_tile_body(PP, C, G, keys, blockIdx.x, gridDim.x);
+}
 
 // kept grid velocity at a point, all three components (wgsl-core.js:185-197)
 __device__ __forceinline__ float d_velGAxis(const DParams& P, const DGrid& G, float3 x, uint32_t axis) {
@@ -1760,7 +1817,9 @@ __device__ __forceinline__ void nb_mark_phi_body(const DParams* __restrict__ PP,
         for (int32_t ti = lx; ti <= hx; ti++) G.need[uint32_t(ti) + P.tx * (uint32_t(tj) + P.ty * uint32_t(tk))] = 1u;
   }
 }
-__global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) { nb_mark_phi_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_nb_mark_phi(const DParams* __restrict__ PP, DGrid G) {
+  nb_mark_phi_body(PP, G, blockIdx.x, gridDim.x);
+}
 // tileMarkFill (wgsl-narrowband.js:13-18): on a start, every tile the starting water reaches
 __global__ void k_nb_mark_fill(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
   const DParams& P = *PP;
@@ -1773,7 +1832,8 @@ __global__ void k_nb_mark_fill(const DParams* __restrict__ PP, const DCols* __re
   if (d_fillPhi(P, C, G.sdf, q) < 9.0f) G.need[t] = 1u;
 }
 // phiInitAll (wgsl-narrowband.js:21-35)
-__device__ __forceinline__ void nb_phi_init_body(const DParams* __restrict__ PP, const DCols* __restrict__ C, const DGrid& G, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void nb_phi_init_body(const DParams* __restrict__ PP, const DCols* __restrict__ C,
+                                                 const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1791,7 +1851,9 @@ __device__ __forceinline__ void nb_phi_init_body(const DParams* __restrict__ PP,
     }
   }
 }
-__global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) { nb_phi_init_body(PP, C, G, blockIdx.x, gridDim.x); }
+__global__ void k_nb_phi_init(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  nb_phi_init_body(PP, C, G, blockIdx.x, gridDim.x);
+}
 // nbPhi (wgsl-narrowband.js:39-49): depth carried by the kept grid velocity
 __device__ __forceinline__ void nb_phi_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
   const DParams& P = *PP;
@@ -1806,9 +1868,12 @@ __device__ __forceinline__ void nb_phi_body(const DParams* __restrict__ PP, cons
     G.rhs[c] = inside ? st_clampf(phA, -7.9f, -0.5f) : st_clampf(phA, 0.5f, 7.9f);
   }
 }
-__global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) { nb_phi_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_nb_phi(const DParams* __restrict__ PP, DGrid G) {
+  nb_phi_body(PP, G, blockIdx.x, gridDim.x);
+}
 // jacNb, nbJacA (fromRhs: phi <- from rhs) and nbJacB (rhs <- from phi) (wgsl-narrowband.js:51-68)
-__device__ __forceinline__ void nb_jac_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t fromRhs, uint32_t bid, uint32_t nbid) {
+__device__ __forceinline__ void nb_jac_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t fromRhs, uint32_t bid,
+                                            uint32_t nbid) {
   const DParams& P = *PP;
   FOR_ACTIVE(G) {
     uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
@@ -1834,7 +1899,9 @@ __device__ __forceinline__ void nb_jac_body(const DParams* __restrict__ PP, cons
     else G.rhs[c] = out;
   }
 }
-__global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromRhs) { nb_jac_body(PP, G, fromRhs, blockIdx.x, gridDim.x); }
+__global__ void k_nb_jac(const DParams* __restrict__ PP, DGrid G, uint32_t fromRhs) {
+  nb_jac_body(PP, G, fromRhs, blockIdx.x, gridDim.x);
+}
 // nbVel (wgsl-narrowband.js:71-85): water faces no particle reached take the kept
 // grid velocity, carried along
 __device__ __forceinline__ void nb_vel_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t bid, uint32_t nbid) {
@@ -1857,7 +1924,9 @@ __device__ __forceinline__ void nb_vel_body(const DParams* __restrict__ PP, cons
     }
   }
 }
-__global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) { nb_vel_body(PP, G, blockIdx.x, gridDim.x); }
+__global__ void k_nb_vel(const DParams* __restrict__ PP, DGrid G) {
+  nb_vel_body(PP, G, blockIdx.x, gridDim.x);
+}
 // saveVel (wgsl-narrowband.js:88-91)
 __device__ __forceinline__ void save_vel_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
   FOR_ACTIVE(G) {
@@ -1867,7 +1936,9 @@ __device__ __forceinline__ void save_vel_body(const DGrid& G, uint32_t bid, uint
     G.velG[c * 3 + 2] = gvR(G, c * 3 + 2);
   }
 }
-__global__ void k_save_vel(DGrid G) { save_vel_body(G, blockIdx.x, gridDim.x); }
+__global__ void k_save_vel(DGrid G) {
+  save_vel_body(G, blockIdx.x, gridDim.x);
+}
 // nbKill (wgsl-narrowband.js:94-102): calm particles below the band are removed
 __global__ void k_nb_kill(const DParams* __restrict__ PP, DGrid G) {
   const DParams& P = *PP;
