@@ -1091,14 +1091,13 @@ __device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uin
   return coop_depart(G, epoch, sFail);
 }
 
-// sH for relaxing the colour opposite c: the colour-c cells across each face,
-// copied in LDS from our own tiles, zero where there is no tile, and fetched
-// for the faces other workgroups own (listed in sFor, nFor of them; a thread's
-// fetches are in flight together).
+// sH for relaxing the colour opposite c: the colour-c cells across each face.
+// coop_fill_own copies them in LDS from our own tiles (zero where there is no
+// tile); it needs only our own colour-c face cells, so it runs while the grid
+// barrier is pending.  coop_fill_foreign fetches the faces other workgroups own
+// (listed in sFor, nFor of them; a thread's fetches are in flight together).
 #define COOP_FOR_PER_THREAD 4
-__device__ __forceinline__ void coop_fill_halo(const DGrid& G, uint32_t scap, const float* __restrict__ pin, const float* sP,
-                                               float* sH, const uint32_t* sNb, const uint32_t* sFor, uint32_t nFor, uint32_t T,
-                                               uint32_t c, bool fromPres) {
+__device__ __forceinline__ void coop_fill_own(const float* sP, float* sH, const uint32_t* sNb, uint32_t T, uint32_t c) {
   for (uint32_t q = threadIdx.x; q < T * 192u; q += COOP_WG) {
     uint32_t lt = q / 192u, r = q - lt * 192u, f = r >> 5, e = r & 31u;
     uint32_t nb = sNb[lt * 6u + f];
@@ -1109,6 +1108,10 @@ __device__ __forceinline__ void coop_fill_halo(const DGrid& G, uint32_t scap, co
       sH[q] = sP[nb * 512u + coop_cidx(a.x, a.y, a.z)];
     }
   }
+}
+__device__ __forceinline__ void coop_fill_foreign(const DGrid& G, uint32_t scap, const float* __restrict__ pin, float* sH,
+                                                  const uint32_t* sNb, const uint32_t* sFor, uint32_t nFor, uint32_t c,
+                                                  bool fromPres) {
   for (uint32_t q0 = 0; q0 < nFor * 32u; q0 += COOP_FOR_PER_THREAD * COOP_WG) {
     float v[COOP_FOR_PER_THREAD];
 #pragma unroll
@@ -1317,7 +1320,8 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     }
   }
   __syncthreads();
-  coop_fill_halo(G, scap, pres, sP, sH, sNb, sFor, nFor, T, 1u, true);
+  coop_fill_own(sP, sH, sNb, T, 1u);
+  coop_fill_foreign(G, scap, pres, sH, sNb, sFor, nFor, 1u, true);
   __syncthreads();
   uint32_t epoch = 0;
   for (uint32_t it = 0; it < iters; it++) {
@@ -1360,8 +1364,11 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         }
       }
       if (last) break;
+      // our own faces of colour c are final and the face relaxation that read
+      // sH is done: copy them while the other workgroups arrive
+      coop_fill_own(sP, sH, sNb, T, c);
       if (!coop_depart(G, epoch, &sFail)) return;
-      coop_fill_halo(G, scap, pres, sP, sH, sNb, sFor, nFor, T, c, false);
+      coop_fill_foreign(G, scap, pres, sH, sNb, sFor, nFor, c, false);
       __syncthreads();
     }
   }
