@@ -73,6 +73,11 @@ def run_gpu(sc, args):
     alive = int(np.sum(pos[:, 1] >= -1e5))
     ctl = sim.ctl()
     stats = dict(blocks=int(ctl[4]), fluid=int(ctl[0] + ctl[1]), vmax=float(ctl[6:7].view(np.float32)[0]))
+    # FP64 norms (on the GPU) of the last timed substep's solve and projection
+    nrm = sim.solve_norms()
+    stats["solve_f64"] = dict(rel_residual=nrm["res_l2"] / max(nrm["rhs_l2"], 1e-300), res_max=nrm["res_max"],
+                              div_l2=nrm["div_l2"], div_minus_target_l2=nrm["err_l2"],
+                              div_minus_target_max=nrm["err_max"], fluid_cells=int(nrm["fluid"]))
     sim.close()
     # per-stage times of the same substeps: the scene again from its start, the
     # same warm-up, then HIP events around each stage of the timed substeps on the
@@ -288,6 +293,7 @@ def main():
         calib_copy(args.calib_copy_mb)
     sc = make_scene(args)
     wall, alive, stats, stage_ms = run_gpu(sc, args)
+    solve_f64 = stats.pop("solve_f64", None)
     value = alive * args.steps / wall
     ms_per_step = wall * 1e3 / args.steps
     iters = int(sc.params["iters"])
@@ -321,6 +327,7 @@ def main():
         "config": {"workload": sc.name + ("_256^3" if args.workload == "splash256" else "_64^3"), "particles": sc.n,
                    "sor_iters": sc.params["iters"], "dt": sc.params["dt"], **stats},
         "stage_ms": stage_ms,
+        "solve_f64": solve_f64,
         "roofline": roof,
         "substep_roofline": {"algorithmic_bytes": sub_bytes, "achieved": sub_bytes / (ms_per_step * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -206,6 +206,26 @@ int st_mem_bytes(void* sim, uint64_t* bytes);
  * tests. */
 int st_sor_path(void* sim, uint32_t* path);
 
+/* FP64 norms of the last substep's pressure solve and projection, over its fluid
+ * cells, computed on the GPU in f64 from the f32 state the substep left (the
+ * only f64 arithmetic on the GPU).  No counterpart in the reference, which
+ * computes no residual (its solve runs a fixed count of red-black iterations,
+ * src/flip/wgsl-solver.js:124-164).  With r = rhs + sum of fluid-neighbour
+ * pressures - nns * p (the SOR system of relax), div = sum of (v_hi - v_lo) / h
+ * over the projected faces, and tgt the density target of the divergence stage
+ * (wgsl-solver.js:114-121), the projection makes div - tgt = -(dt/h^2) r up to
+ * f32 rounding. */
+typedef struct st_norms {
+  double res_l2, res_max;  /* ||r||_2, max |r| */
+  double rhs_l2;           /* ||rhs||_2 (res_l2 / rhs_l2 = relative residual) */
+  double div_l2, div_max;  /* projected divergence */
+  double err_l2, err_max;  /* div - tgt */
+  double fluid;            /* fluid cells counted */
+} st_norms;
+/* Enqueues the two reduction kernels on the sim's stream after its last
+ * substep and waits for them.  ST_ERR_STATE before the first substep. */
+int st_solve_norms(void* sim, st_norms* out);
+
 /* Batched tanks.  No counterpart in the reference, which steps one FlipSim at a
  * time; these advance many independent tanks (each an st_sim of its own size,
  * particles and colliders) with one kernel launch per stage for all of them.

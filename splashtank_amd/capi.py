@@ -57,6 +57,11 @@ class StParams(C.Structure):
                 ("fills", C.c_float * 64), ("sor_mode", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
+class StNorms(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("res_l2", "res_max", "rhs_l2", "div_l2", "div_max",
+                                          "err_l2", "err_max", "fluid")]
+
+
 def make_opts(d, size=(3.0, 2.6, 3.0), ratio=2.0, open_=0, max_n=1, slots=0, device=0) -> StOpts:
     o = StOpts()
     o.d = float(d)
@@ -121,6 +126,7 @@ PROTOS = {
     "st_sync": (C.c_int, [C.c_void_p]),
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
     "st_batch_destroy": (C.c_int, [C.c_void_p]),
     "st_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32]),
@@ -286,6 +292,12 @@ class FlipSimC(_Base):
         v = C.c_uint32(0)
         self._call("sor_path", self.h, C.byref(v))
         return v.value
+
+    def solve_norms(self):
+        """FP64 norms of the last substep's solve and projection (st_solve_norms), as a dict."""
+        n = StNorms()
+        self._call("solve_norms", self.h, C.byref(n))
+        return {f: float(getattr(n, f)) for f, _ in StNorms._fields_}
 
     def close(self):
         # a tank still in a batch is refused; it stays open until the batch closes

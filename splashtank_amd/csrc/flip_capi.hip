@@ -93,6 +93,8 @@ struct Sim {
   bool binned = false;
   NbScratch nb{};
   void* batch = nullptr;          // the batch the tank belongs to (its stream is the batch's)
+  double* normPart = nullptr;     // st_solve_norms: per-workgroup partials, then the result
+  uint32_t normCap = 0;           // workgroups normPart has room for
   hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
 
   LaunchCtx ctx() const {
@@ -194,6 +196,7 @@ void destroySim(Sim* s) {
   dfree(s->G.tmap); dfree(s->G.need); dfree(s->G.newList); dfree(s->G.wg); dfree(s->G.ctl); dfree(s->G.part); dfree(s->G.foamTab);
   dfree(s->G.mortonTiles); dfree(s->G.coopFlags); dfree(s->G.coopOffs); dfree(s->G.sorBar);
   dfree(s->G.scratch4);
+  dfree(s->normPart);
   freeSlots(s);
   dfree(s->dSdf);
   dfree(s->dSlots);
@@ -875,6 +878,37 @@ int st_sync(void* p) {
   uint32_t timeouts = 0;
   HIPCHK(hipMemcpy(&timeouts, s->G.sorBar + COOP_MAX_WG + 1, 4, hipMemcpyDeviceToHost));
   if (timeouts) return fail(ST_ERR_HIP, "resident SOR: a grid barrier timed out (pressure of that solve is not valid)");
+  return ST_OK;
+}
+
+int st_solve_norms(void* p, st_norms* out) {
+  Sim* s = getSim(p);
+  if (!s || !out) return fail(ST_ERR_ARG, "st_solve_norms: null argument");
+  if (!s->cur) return fail(ST_ERR_STATE, "st_solve_norms: no substep of this sim to measure (none yet, or the last one was batched)");
+  HIPCHK(hipSetDevice(s->dev));
+  LaunchCtx L = s->ctx();
+  if (s->normCap < L.cellGrid) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->normPart);
+    s->normCap = 0;
+    int rc = dalloc(&s->normPart, size_t(L.cellGrid + 1) * ST_NORM_WORDS);
+    if (rc) return rc;
+    s->normCap = L.cellGrid;
+  }
+  double* res = s->normPart + size_t(s->normCap) * ST_NORM_WORDS;
+  launch_norms(s->st, L, s->normPart, res);
+  HIPCHK(hipGetLastError());
+  double h[ST_NORM_WORDS];
+  HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  out->res_l2 = std::sqrt(h[0]);
+  out->res_max = h[1];
+  out->rhs_l2 = std::sqrt(h[2]);
+  out->div_l2 = std::sqrt(h[3]);
+  out->div_max = h[4];
+  out->err_l2 = std::sqrt(h[5]);
+  out->err_max = h[6];
+  out->fluid = h[7];
   return ST_OK;
 }
 

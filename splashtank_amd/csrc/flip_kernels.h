@@ -104,3 +104,7 @@ void morton_tiles(uint32_t tx, uint32_t ty, uint32_t tz, uint32_t* out);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);
+// FP64 norms of the last pressure solve and projection (flip_norms.hip): part
+// holds L.cellGrid x ST_NORM_WORDS doubles, out ST_NORM_WORDS (the st_norms fields)
+#define ST_NORM_WORDS 8
+void launch_norms(hipStream_t st, const LaunchCtx& L, double* part, double* out);

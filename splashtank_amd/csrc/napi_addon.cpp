@@ -322,6 +322,21 @@ napi_value MemBytes(napi_env env, napi_callback_info info) {
   return mkNum(env, double(b));
 }
 
+// solveNorms(h) -> {resL2, resMax, rhsL2, divL2, divMax, errL2, errMax, fluid} (st_solve_norms)
+napi_value SolveNorms(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  st_norms n;
+  ST_OK_OR_THROW(st_solve_norms(handleOf(env, argv[0]), &n));
+  napi_value o;
+  napi_create_object(env, &o);
+  const struct { const char* k; double v; } f[] = {{"resL2", n.res_l2}, {"resMax", n.res_max}, {"rhsL2", n.rhs_l2},
+                                                    {"divL2", n.div_l2}, {"divMax", n.div_max}, {"errL2", n.err_l2},
+                                                    {"errMax", n.err_max}, {"fluid", n.fluid}};
+  for (const auto& e : f) napi_set_named_property(env, o, e.k, mkNum(env, e.v));
+  return o;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   napi_create_string_utf8(env, st_version(), NAPI_AUTO_LENGTH, &v);
@@ -386,7 +401,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;

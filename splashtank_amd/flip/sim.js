@@ -237,6 +237,9 @@ class FlipSim {
   }
   async activeCells() { const s = this.stats; this.blocksUsed = s.sc; return s.fluid; }
   memBytes() { return loadAddon().memBytes(this.handle); }
+  // FP64 norms of the last substep's pressure solve and projection (no counterpart in the
+  // reference, whose solve reports nothing): {resL2, resMax, rhsL2, divL2, divMax, errL2, errMax, fluid}
+  solveNorms() { return loadAddon().solveNorms(this.handle); }
   dump(what) { return loadAddon().dump(this.handle, DUMP[what]); }
   // one column / row of cells for debugging (sim.js:457-482)
   cellsAt(list) {

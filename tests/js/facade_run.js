@@ -30,7 +30,7 @@ sim.stepTimed(cfg.params.dt, o).then((stageMs) => {
     fs.writeFileSync(cfg.out, Buffer.from(dst.buffer));
     const st = sim.stats;
     console.log(JSON.stringify({ n: m, N: sim.N, nx: sim.nx, tiles: sim.ntiles, scap: sim.scap, stats: st, stages: Object.keys(stageMs),
-      fluidCells: st.fluid, mem: sim.memBytes() }));
+      fluidCells: st.fluid, mem: sim.memBytes(), norms: sim.solveNorms() }));
     sim.destroy();
   });
 }).catch((e) => { console.error(e.stack || String(e)); process.exit(1); });

@@ -60,6 +60,10 @@ def test_facade_matches_c_abi(tmp_path):
     ref = g.read_particles()
     np.testing.assert_array_equal(js.view(np.uint32), ref.view(np.uint32))
     assert info["stats"]["sc"] == int(g.ctl()[4])
+    # the FP64 norms: same state, same fixed reduction order, so the same doubles
+    nrm = g.solve_norms()
+    assert info["norms"]["resL2"] == nrm["res_l2"] and info["norms"]["errMax"] == nrm["err_max"]
+    assert info["norms"]["fluid"] == nrm["fluid"] > 0
 
 
 @needs_node
