@@ -91,11 +91,15 @@ typedef struct st_params {
   /* pressure solve: 0 (default) runs every iteration of a solve in one
    * cooperative launch, with the pressure resident in LDS when the tiles in use
    * fit and streaming tile windows otherwise; 1 one launch per red-black
-   * iteration; 2 the cooperative launch, always streaming.  All give the same
-   * bits.  No counterpart in the reference (an implementation choice here). */
+   * iteration; 2 the cooperative launch, always streaming; 3 (ST_SOR_TUNED) the
+   * path and launch width st_tune_sor chose for this sim (0 until it ran).  All
+   * give the same bits.  No counterpart in the reference (an implementation
+   * choice here). */
   uint32_t sor_mode;
   uint32_t reserved[3];
 } st_params;
+
+#define ST_SOR_TUNED 3u
 
 /* Stages of one substep, named as in FlipSim.stages (sim.js:245-310). */
 enum {
@@ -225,6 +229,31 @@ typedef struct st_norms {
 /* Enqueues the two reduction kernels on the sim's stream after its last
  * substep and waits for them.  ST_ERR_STATE before the first substep. */
 int st_solve_norms(void* sim, st_norms* out);
+
+/* Autotuner of the pressure-solve schedule.  No counterpart in the reference
+ * (its solve has one fixed dispatch pattern, src/flip/sim.js:291-293).  Every
+ * candidate gives the same bits, so tuning changes only speed.
+ * st_tune_sor: from the sim's current state, runs one substep's stages up to
+ * the divergence with `p` (narrow band off), then times the pressure stage for
+ * each candidate -- the resident cooperative solve, the streaming cooperative
+ * solve, and one launch per iteration at 1, 1/2 and 1/4 of the resident
+ * workgroup count -- `reps` times each after one warm-up, keeps the fastest for
+ * steps with sor_mode = ST_SOR_TUNED, and restores the state it started from
+ * (the next substep gives the bits it would have given without the tuning).
+ * st_batch_tune_sor: the same for a batch, whose solves run one launch per
+ * iteration: it times whole batched substeps for SOR widths of 1/4, 1/2, 1 and 2
+ * times the resident workgroup count and keeps the fastest for the batch. */
+#define ST_TUNE_MAX 8
+typedef struct st_sor_tune {
+  uint32_t mode;               /* chosen path (0 resident, 1 per iteration, 2 streaming) */
+  uint32_t width;              /* workgroups of the chosen per-iteration launch (0: cooperative) */
+  uint32_t n;                  /* candidates timed */
+  uint32_t cand_mode[ST_TUNE_MAX];
+  uint32_t cand_width[ST_TUNE_MAX];
+  float cand_ms[ST_TUNE_MAX];  /* mean time of each candidate (pressure stage, or batched substep) */
+} st_sor_tune;
+int st_tune_sor(void* sim, const st_params* p, int32_t reps, st_sor_tune* out);
+int st_batch_tune_sor(void* batch, const st_params* p, uint32_t np, int32_t reps, st_sor_tune* out);
 
 /* Batched tanks.  No counterpart in the reference, which steps one FlipSim at a
  * time; these advance many independent tanks (each an st_sim of its own size,

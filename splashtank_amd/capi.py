@@ -62,6 +62,21 @@ class StNorms(C.Structure):
                                           "err_l2", "err_max", "fluid")]
 
 
+ST_SOR_TUNED = 3
+ST_TUNE_MAX = 8
+
+
+class StSorTune(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("width", C.c_uint32), ("n", C.c_uint32),
+                ("cand_mode", C.c_uint32 * ST_TUNE_MAX), ("cand_width", C.c_uint32 * ST_TUNE_MAX),
+                ("cand_ms", C.c_float * ST_TUNE_MAX)]
+
+    def as_dict(self):
+        return {"mode": self.mode, "width": self.width,
+                "candidates": [{"mode": self.cand_mode[k], "width": self.cand_width[k], "ms": self.cand_ms[k]}
+                               for k in range(self.n)]}
+
+
 def make_opts(d, size=(3.0, 2.6, 3.0), ratio=2.0, open_=0, max_n=1, slots=0, device=0) -> StOpts:
     o = StOpts()
     o.d = float(d)
@@ -127,6 +142,8 @@ PROTOS = {
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
+    "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
+    "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
     "st_batch_destroy": (C.c_int, [C.c_void_p]),
     "st_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32]),
@@ -299,6 +316,14 @@ class FlipSimC(_Base):
         self._call("solve_norms", self.h, C.byref(n))
         return {f: float(getattr(n, f)) for f, _ in StNorms._fields_}
 
+    def tune_sor(self, prm: StParams, reps: int = 3):
+        """Time the pressure-solve candidates from the current state and keep the
+        fastest for steps with sor_mode = ST_SOR_TUNED (st_tune_sor); the state is
+        left as it was."""
+        t = StSorTune()
+        self._call("tune_sor", self.h, C.byref(prm), int(reps), C.byref(t))
+        return t.as_dict()
+
     def close(self):
         # a tank still in a batch is refused; it stays open until the batch closes
         if getattr(self, "h", None) and self.lib.st_destroy(self.h) == 0:
@@ -341,6 +366,19 @@ class FlipBatchC:
 
     def sync(self):
         self._call("sync", self.h)
+
+    def tune_sor(self, prm, reps: int = 3):
+        """Time whole batched substeps at several SOR launch widths and keep the
+        fastest (st_batch_tune_sor); the tanks are left as they were."""
+        t = StSorTune()
+        if isinstance(prm, StParams):
+            rc = self.lib.st_batch_tune_sor(self.h, C.byref(prm), 1, int(reps), C.byref(t))
+        else:
+            arr = (StParams * len(prm))(*prm)
+            rc = self.lib.st_batch_tune_sor(self.h, arr, len(prm), int(reps), C.byref(t))
+        if rc != 0:
+            raise StError(f"st_batch_tune_sor -> {rc}: {self.lib.st_last_error().decode()}")
+        return t.as_dict()
 
     def close(self):
         if getattr(self, "h", None):

@@ -51,6 +51,19 @@ template <class T> void dfree(T*& p) {
   p = nullptr;
 }
 
+// A captured plain substep and what it was captured for: the device pointers
+// (by value in every kernel's arguments), the host flags that select kernels,
+// and the sizes that set launch grids.  A replay is valid exactly when all of
+// them are the same again.
+struct StepGraph {
+  DGrid G;
+  bool binned, solid;
+  int iters;
+  uint32_t N, scap, pipe;
+  hipGraphExec_t exec;
+};
+const size_t GRAPH_CACHE = 4;
+
 struct Sim {
   int dev = 0;
   hipStream_t st = nullptr;
@@ -87,7 +100,12 @@ struct Sim {
   uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
   uint32_t coopGrid = 0;        // resident-SOR workgroups (0: not available)
   uint32_t sorSeq = 0;          // pressure solves so far
-  uint32_t sorMode = 0;         // this step's st_params.sor_mode
+  uint32_t sorMode = 0;         // this step's pressure path (st_params.sor_mode, a tuned choice for 3)
+  uint32_t pipeOverride = 0;    // workgroups of the per-iteration SOR launch (0: the default)
+  bool tuned = false;           // st_tune_sor has chosen a path
+  StepSlot* dGraphSlot = nullptr;  // the step slot every captured substep reads
+  std::vector<struct StepGraph> graphs;  // captured substeps (st_step_n with use_graph)
+  uint32_t tunedMode = 0, tunedPipe = 0;
   // need, binCount and binKey already hold the marks and bins of A's positions
   // (the last g2p made them and nothing has moved a particle since)
   bool binned = false;
@@ -107,7 +125,7 @@ struct Sim {
     L.scap = scap;
     L.cellGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
     L.tileGrid = std::max<uint32_t>(1u, std::min<uint32_t>(scap, 8192u));
-    L.pipeGrid = std::min<uint32_t>((scap + 7u) & ~7u, sorResident);  // whole groups of 8 (one per XCD)
+    L.pipeGrid = pipeOverride ? pipeOverride : std::min<uint32_t>((scap + 7u) & ~7u, sorResident);  // whole groups of 8 (one per XCD)
     L.coopGrid = sorMode == 1u ? 0u : coopGrid;
     L.sorStream = sorMode == 2u;
     L.sorSeq = sorSeq;
@@ -203,6 +221,9 @@ void destroySim(Sim* s) {
   if (s->hSlots) (void)hipHostFree(s->hSlots);
   for (int k = 0; k < RING; k++)
     if (s->evUsed[k]) (void)hipEventDestroy(s->ev[k]);
+  for (StepGraph& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+  dfree(s->dGraphSlot);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
@@ -240,7 +261,13 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
     }
   }
   s->onlyTiles = o->only_tiles != 0;
-  s->sorMode = o->sor_mode;
+  if (o->sor_mode == ST_SOR_TUNED) {  // the path st_tune_sor chose (the resident solve until then)
+    s->sorMode = s->tuned ? s->tunedMode : 0u;
+    s->pipeOverride = s->tuned && s->tunedMode == 1u ? s->tunedPipe : 0u;
+  } else {
+    s->sorMode = o->sor_mode;
+    s->pipeOverride = 0u;
+  }
   DParams& P = s->hp;
   std::memset(&P, 0, sizeof P);
   P.n = s->N; P.nx = uint32_t(s->nx); P.ny = uint32_t(s->ny); P.nz = uint32_t(s->nz); P.ncols = nc; P.open = s->open; P.ncell = s->ncell; P.scap = s->scap;
@@ -347,6 +374,91 @@ int runStage(Sim* s, int stage) {
   return ST_OK;
 }
 
+// Graph replay of a plain substep (st_step_n with use_graph).  The step's
+// parameter block and collider records go to one fixed device slot, which the
+// captured kernels read; the first substep of a kind is captured (stream
+// capture of every stage) and later ones replay it.  Cooperative launches
+// cannot be captured, so a graphed substep solves with one launch per
+// iteration (the same bits).  The host-side effects of the stages (the particle
+// buffers swapped, the bins of the next substep made, the collider lookup
+// rebuilt) are applied again on each replay.
+void dropGraphs(Sim* s) {
+  if (s->graphs.empty()) return;
+  (void)hipStreamSynchronize(s->st);
+  for (StepGraph& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
+int graphStep(Sim* s, const st_params* o) {
+  if (o->nb_on || o->only_tiles || o->nb_init)
+    return fail(ST_ERR_ARG, "st_step_n: graph replay covers plain substeps (no narrow band, no tile-only step)");
+  if (s->batch) return fail(ST_ERR_STATE, "st_step_n: graph replay of a tank in a batch (use st_batch_step_n)");
+  if (!s->dGraphSlot) {
+    int rc = dalloc(&s->dGraphSlot, 1);
+    if (rc) return rc;
+  }
+  uint32_t k = s->ringPos % RING;
+  if (s->evUsed[k]) HIPCHK(hipEventSynchronize(s->ev[k]));
+  StepSlot& hs = s->hSlots[k];
+  st_params q = *o;
+  q.sor_mode = 1;
+  int rc = prepStep(s, &q, hs.P, hs.C);
+  if (rc) return rc;
+  if (o->sor_mode == ST_SOR_TUNED && s->tuned && s->tunedMode == 1u) s->pipeOverride = s->tunedPipe;
+  s->ringPos++;
+  HIPCHK(hipMemcpyAsync(s->dGraphSlot, &hs, sizeof(StepSlot), hipMemcpyHostToDevice, s->st));
+  if (!s->evUsed[k]) {
+    HIPCHK(hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming));
+    s->evUsed[k] = true;
+  }
+  HIPCHK(hipEventRecord(s->ev[k], s->st));
+  s->cur = s->dGraphSlot;
+  s->begun = true;
+  const uint32_t pipe = s->ctx().pipeGrid;
+  for (StepGraph& g : s->graphs) {
+    if (std::memcmp(&g.G, &s->G, sizeof(DGrid)) != 0 || g.binned != s->binned || g.solid != s->solidDirty ||
+        g.iters != s->iters || g.N != s->N || g.scap != s->scap || g.pipe != pipe)
+      continue;
+    HIPCHK(hipGraphLaunch(g.exec, s->st));
+    // what runStage does on the host for these stages
+    s->solidDirty = false;
+    s->sorSeq++;
+    s->binned = true;
+    std::swap(s->G.pos, s->G.posB);
+    std::swap(s->G.vel, s->G.velB);
+    std::swap(s->G.pid, s->G.pidB);
+    return ST_OK;
+  }
+  StepGraph e;
+  e.G = s->G;
+  e.binned = s->binned;
+  e.solid = s->solidDirty;
+  e.iters = s->iters;
+  e.N = s->N;
+  e.scap = s->scap;
+  e.pipe = pipe;
+  e.exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  for (int st = 0; st < ST_STAGE_COUNT && !rc; st++) rc = runStage(s, st);
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(s->st, &g);
+  if (rc || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : fail(ST_ERR_HIP, std::string("st_step_n: stream capture: ") + hipGetErrorString(ce));
+  }
+  hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return fail(ST_ERR_HIP, std::string("st_step_n: hipGraphInstantiate: ") + hipGetErrorString(ie));
+  HIPCHK(hipGraphLaunch(e.exec, s->st));
+  if (s->graphs.size() >= GRAPH_CACHE) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    (void)hipGraphExecDestroy(s->graphs.front().exec);
+    s->graphs.erase(s->graphs.begin());
+  }
+  s->graphs.push_back(e);
+  return ST_OK;
+}
+
 // Batched tanks (no counterpart in the reference, which steps one FlipSim at a
 // time): every stage of a substep for all tanks in one launch per kernel over
 // the concatenated workgroup ranges of the tanks.  The tanks keep their own
@@ -356,7 +468,7 @@ struct Batch {
   int dev = 0;
   hipStream_t st = nullptr;
   std::vector<Sim*> tanks;
-  uint32_t sorResident = 2048;
+  uint32_t sorResident = 2048;   // SOR workgroups of a batched iteration, over all tanks
   // ring of step slots: tank records, parameter blocks, collider records, block
   // tables (host pinned and device), each reusable once its upload is done
   size_t slotBytes = 0, offP = 0, offC = 0, offTab = 0;
@@ -696,9 +808,11 @@ int st_step(void* p, const st_params* prm) {
 }
 
 int st_step_n(void* p, const st_params* prm, int32_t nsteps, int32_t use_graph) {
-  if (use_graph) return fail(ST_ERR_ARG, "st_step_n: graph capture is not supported (the pressure solve is a cooperative launch)");
+  Sim* s = getSim(p);
+  if (!s || !prm) return fail(ST_ERR_ARG, "st_step_n: null argument");
+  if (use_graph) HIPCHK(hipSetDevice(s->dev));
   for (int32_t k = 0; k < nsteps; k++) {
-    int rc = st_step(p, prm);
+    int rc = use_graph ? graphStep(s, prm) : st_step(p, prm);
     if (rc) return rc;
   }
   return ST_OK;
@@ -1112,6 +1226,127 @@ int st_snapshot_restore(void* p, void* snap) {
 int st_snapshot_destroy(void* snap) {
   if (!snap) return fail(ST_ERR_ARG, "st_snapshot_destroy: null snapshot");
   freeSnap((Snap*)snap);
+  return ST_OK;
+}
+
+int st_tune_sor(void* p, const st_params* prm, int32_t reps, st_sor_tune* out) {
+  Sim* s = getSim(p);
+  if (!s || !prm || !out) return fail(ST_ERR_ARG, "st_tune_sor: null argument");
+  if (s->batch) return fail(ST_ERR_STATE, "st_tune_sor: the tank is in a batch (use st_batch_tune_sor)");
+  if (!s->N) return fail(ST_ERR_STATE, "st_tune_sor: no particles");
+  reps = std::max(1, std::min(reps, 50));
+  HIPCHK(hipSetDevice(s->dev));
+  std::memset(out, 0, sizeof *out);
+  // candidates: the two cooperative paths, then one launch per iteration at
+  // 1, 1/2 and 1/4 of the resident width (whole groups of 8, one per XCD)
+  uint32_t def = std::min<uint32_t>((s->scap + 7u) & ~7u, s->sorResident);
+  uint32_t n = 0;
+  if (s->coopGrid) {
+    out->cand_mode[n] = 0; out->cand_width[n++] = 0;
+    out->cand_mode[n] = 2; out->cand_width[n++] = 0;
+  }
+  for (uint32_t div = 1; div <= 4; div *= 2) {
+    uint32_t w = std::max<uint32_t>(8u, (def / div) & ~7u);
+    bool dup = false;
+    for (uint32_t q = 0; q < n; q++) dup |= out->cand_mode[q] == 1u && out->cand_width[q] == w;
+    if (!dup) { out->cand_mode[n] = 1; out->cand_width[n++] = w; }
+  }
+  out->n = n;
+  void* snap = nullptr;
+  int rc = st_snapshot_create(p, &snap);
+  if (rc) return rc;
+  st_params q = *prm;
+  q.nb_on = 0; q.nb_init = 0; q.only_tiles = 0; q.sor_mode = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = st_begin_step(p, &q);
+  for (int k = 0; !rc && k <= ST_STAGE_DIVERGENCE; k++) rc = runStage(s, k);
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail(ST_ERR_HIP, "st_tune_sor: hipEventCreate");
+  for (uint32_t c = 0; !rc && c < n; c++) {
+    s->sorMode = out->cand_mode[c];
+    s->pipeOverride = out->cand_width[c];
+    rc = runStage(s, ST_STAGE_PRESSURE);  // warm-up
+    if (!rc && hipEventRecord(e0, s->st) != hipSuccess) rc = fail(ST_ERR_HIP, "st_tune_sor: hipEventRecord");
+    for (int32_t r = 0; !rc && r < reps; r++) rc = runStage(s, ST_STAGE_PRESSURE);
+    if (!rc && (hipEventRecord(e1, s->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = fail(ST_ERR_HIP, "st_tune_sor: timing");
+    float ms = 0.0f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(ST_ERR_HIP, "st_tune_sor: hipEventElapsedTime");
+    out->cand_ms[c] = ms / float(reps);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  // back to the state the tuning started from; the collider lookup is rebuilt
+  // on the next substep (the tuning step may have taken the records as seen)
+  s->pipeOverride = 0;
+  int rc2 = st_snapshot_restore(p, snap);
+  (void)hipStreamSynchronize(s->st);  // the restore copies read the snapshot
+  (void)st_snapshot_destroy(snap);
+  s->solidDirty = true;
+  s->cur = nullptr;
+  if (!rc) rc = rc2;
+  if (!rc) rc = st_sync(p);
+  if (rc) return rc;
+  uint32_t best = 0;
+  for (uint32_t c = 1; c < n; c++)
+    if (out->cand_ms[c] < out->cand_ms[best]) best = c;
+  out->mode = out->cand_mode[best];
+  out->width = out->cand_width[best];
+  s->tuned = true;
+  s->tunedMode = out->mode;
+  s->tunedPipe = out->width;
+  return ST_OK;
+}
+
+int st_batch_tune_sor(void* p, const st_params* prm, uint32_t nprm, int32_t reps, st_sor_tune* out) {
+  Batch* b = (Batch*)p;
+  if (!b || !prm || !out) return fail(ST_ERR_ARG, "st_batch_tune_sor: null argument");
+  reps = std::max(1, std::min(reps, 50));
+  HIPCHK(hipSetDevice(b->dev));
+  std::memset(out, 0, sizeof *out);
+  const uint32_t nt = uint32_t(b->tanks.size());
+  uint32_t base = b->tanks[0]->sorResident, n = 0;
+  for (uint32_t mul = 1; mul <= 8; mul *= 2) {  // 1/4, 1/2, 1, 2 of the resident width
+    out->cand_mode[n] = 1;
+    out->cand_width[n++] = std::max<uint32_t>(8u, (base * mul / 4u) & ~7u);
+  }
+  out->n = n;
+  std::vector<void*> snaps(nt, nullptr);
+  int rc = ST_OK;
+  for (uint32_t t = 0; !rc && t < nt; t++) rc = st_snapshot_create(b->tanks[t], &snaps[t]);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: hipEventCreate");
+  uint32_t keep = b->sorResident;
+  for (uint32_t c = 0; !rc && c < n; c++) {
+    b->sorResident = out->cand_width[c];
+    rc = st_batch_step(p, prm, nprm);  // warm-up
+    if (!rc && hipEventRecord(e0, b->st) != hipSuccess) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: hipEventRecord");
+    for (int32_t r = 0; !rc && r < reps; r++) rc = st_batch_step(p, prm, nprm);
+    if (!rc && (hipEventRecord(e1, b->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: timing");
+    float ms = 0.0f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: hipEventElapsedTime");
+    out->cand_ms[c] = ms / float(reps);
+    for (uint32_t t = 0; !rc && t < nt; t++) {
+      rc = st_snapshot_restore(b->tanks[t], snaps[t]);
+      b->tanks[t]->solidDirty = true;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (uint32_t t = 0; t < nt; t++)
+    if (snaps[t]) {
+      // the snapshot copies run on the batch stream: wait before freeing them
+      (void)hipStreamSynchronize(b->st);
+      (void)st_snapshot_destroy(snaps[t]);
+    }
+  if (rc) {
+    b->sorResident = keep;
+    return rc;
+  }
+  uint32_t best = 0;
+  for (uint32_t c = 1; c < n; c++)
+    if (out->cand_ms[c] < out->cand_ms[best]) best = c;
+  out->mode = 1;
+  out->width = out->cand_width[best];
+  b->sorResident = out->width;
   return ST_OK;
 }
 
