@@ -174,8 +174,15 @@ int st_set_colliders(void* sim, const float* cf32, uint32_t nc, const float* sdf
 /* FlipSim.step(dt, o): one full substep enqueued on the sim's stream (sim.js:179-241). */
 int st_step(void* sim, const st_params* p);
 /* Run several substeps with the same settings, as many st_step calls; used by
- * benchmarks and bakes.  `use_graph` must be 0: HIP graph capture is refused
- * (ST_ERR_ARG), because the pressure solve is a cooperative launch. */
+ * benchmarks and bakes.  With use_graph != 0 each substep is a HIP graph
+ * replay: the first substep of a kind (particle buffer parity, bins made or
+ * not, collider lookup rebuilt or not, sizes) is stream-captured and later
+ * ones replay it, with the step's parameters copied into the one device slot
+ * the graph reads.  Graphed substeps solve with one launch per SOR iteration
+ * (cooperative launches cannot be captured; the bits are the same), and cover
+ * plain substeps only (ST_ERR_ARG with the narrow band or a tile-only step).
+ * No counterpart in the reference, which records its dispatches into one
+ * compute pass per step (src/flip/sim.js:245-310). */
 int st_step_n(void* sim, const st_params* p, int32_t nsteps, int32_t use_graph);
 /* The parameter upload of step() without running a stage (stepTimed, sim.js:312-319). */
 int st_begin_step(void* sim, const st_params* p);
@@ -271,6 +278,10 @@ int st_batch_destroy(void* batch);
  * st_step does nothing for them.  The narrow band and tile-only steps are not
  * batched (ST_ERR_ARG). */
 int st_batch_step(void* batch, const st_params* p, uint32_t np);
+/* nsteps batched substeps with the same settings; with use_graph != 0 each is
+ * a HIP graph replay, captured the first time its block tables (which fix
+ * every launch of the substep) are seen and replayed after that. */
+int st_batch_step_n(void* batch, const st_params* p, uint32_t np, int32_t nsteps, int32_t use_graph);
 /* Wait for all work of the batch. */
 int st_batch_sync(void* batch);
 /* Number of tanks. */

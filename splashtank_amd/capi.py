@@ -143,10 +143,11 @@ PROTOS = {
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
     "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
-    "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
+    "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
     "st_batch_destroy": (C.c_int, [C.c_void_p]),
     "st_batch_step": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32]),
+    "st_batch_step_n": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.c_int32]),
     "st_batch_sync": (C.c_int, [C.c_void_p]),
     "st_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_snapshot_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -270,8 +271,9 @@ class FlipSimC(_Base):
         if rc != 0:
             raise StError(f"st_create -> {rc}: {self.lib.st_last_error().decode()}")
 
-    def step_n(self, prm: StParams, n: int):
-        self._call("step_n", self.h, C.byref(prm), int(n), 0)
+    def step_n(self, prm: StParams, n: int, graph: bool = False):
+        """n substeps with the same settings; graph=True replays captured HIP graphs."""
+        self._call("step_n", self.h, C.byref(prm), int(n), 1 if graph else 0)
 
     def time_stages(self, prm: StParams):
         """One substep with HIP events around each stage: {stage name: ms}."""
@@ -363,6 +365,14 @@ class FlipBatchC:
         else:
             arr = (StParams * len(prm))(*prm)
             self._call("step", self.h, arr, len(prm))
+
+    def step_n(self, prm, n: int, graph: bool = False):
+        """n batched substeps; graph=True replays captured HIP graphs."""
+        if isinstance(prm, StParams):
+            self._call("step_n", self.h, C.byref(prm), 1, int(n), 1 if graph else 0)
+        else:
+            arr = (StParams * len(prm))(*prm)
+            self._call("step_n", self.h, arr, len(prm), int(n), 1 if graph else 0)
 
     def sync(self):
         self._call("sync", self.h)

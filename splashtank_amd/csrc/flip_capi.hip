@@ -100,6 +100,7 @@ struct Sim {
   uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
   uint32_t coopGrid = 0;        // resident-SOR workgroups (0: not available)
   uint32_t sorSeq = 0;          // pressure solves so far
+  uint32_t sorTag = 1;          // tag of the next resident solve's first halo exchange (0 is never used)
   uint32_t sorMode = 0;         // this step's pressure path (st_params.sor_mode, a tuned choice for 3)
   uint32_t pipeOverride = 0;    // workgroups of the per-iteration SOR launch (0: the default)
   bool tuned = false;           // st_tune_sor has chosen a path
@@ -129,6 +130,7 @@ struct Sim {
     L.coopGrid = sorMode == 1u ? 0u : coopGrid;
     L.sorStream = sorMode == 2u;
     L.sorSeq = sorSeq;
+    L.sorTag = sorTag;
     L.binned = binned;
     L.binNext = !nbOn;  // the narrow band's reseed moves particles after g2p
     L.nbOn = nbOn;
@@ -156,6 +158,8 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.nbrL, size_t(n) * 27))) return rc;
   if ((rc = dalloc(&s->G.coopList, n))) return rc;
   if ((rc = dalloc(&s->G.sorHalo, size_t(n) * 2 * 6 * 32))) return rc;
+  // tag 0 is never a half-sweep's tag, so no entry looks written before it is
+  HIPCHK(hipMemset(s->G.sorHalo, 0, size_t(n) * 2 * 6 * 32 * sizeof(uint64_t)));
   return ST_OK;
 }
 
@@ -345,6 +349,17 @@ int runStage(Sim* s, int stage) {
     case ST_STAGE_PRESSURE:
       if (full) {
         L.sorSeq = ++s->sorSeq;
+        if (L.coopGrid) {
+          // every halo exchange of a resident solve has its own tag; when the
+          // counter would wrap, the entries are cleared so no old tag can match
+          const uint32_t need = 2u * uint32_t(s->iters) + 2u;
+          if (s->sorTag > 0xffffffffu - need) {
+            HIPCHK(hipMemsetAsync(s->G.sorHalo, 0, size_t(s->scap) * 2 * 6 * 32 * sizeof(uint64_t), st));
+            s->sorTag = 1;
+          }
+          L.sorTag = s->sorTag;
+          s->sorTag += need;
+        }
         if (launch_pressure(st, L, s->iters) != hipSuccess) s->coopGrid = 0;  // per-iteration launches from now on
       }
       break;
@@ -479,6 +494,14 @@ struct Batch {
   bool evUsed[BRING];
   uint32_t ringPos = 0;
   std::vector<BTankPlan> plan;
+  // graph replay (st_batch_step_n with use_graph): a fixed device slot the
+  // captured kernels read, and the captured substeps by their block tables
+  uint8_t* dGraph = nullptr;
+  struct Captured {
+    std::vector<uint32_t> tab;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
 };
 
 size_t align256(size_t x) { return (x + 255u) & ~size_t(255u); }
@@ -492,6 +515,10 @@ void freeRing(Batch* b) {
     }
   if (b->hRing) (void)hipHostFree(b->hRing);
   dfree(b->dRing);
+  if (!b->graphs.empty() || b->dGraph) (void)hipStreamSynchronize(b->st);
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+  dfree(b->dGraph);
   b->hRing = nullptr;
   b->itersCap = 0;
 }
@@ -871,6 +898,7 @@ int st_grow(void* p, uint32_t n) {
   if (n <= s->scap) return ST_OK;
   HIPCHK(hipSetDevice(s->dev));
   HIPCHK(hipStreamSynchronize(s->st));
+  dropGraphs(s);
   uint32_t old = s->scap;
   DGrid keep = s->G;
   s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.pres2 = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
@@ -1105,9 +1133,7 @@ int st_batch_destroy(void* p) {
   return ST_OK;
 }
 
-int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
-  Batch* b = (Batch*)p;
-  if (!b || !prm) return fail(ST_ERR_ARG, "st_batch_step: null argument");
+static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) {
   const uint32_t nt = uint32_t(b->tanks.size());
   if (nprm != 1u && nprm != nt) return fail(ST_ERR_ARG, "st_batch_step: give one parameter block or one per tank");
   HIPCHK(hipSetDevice(b->dev));
@@ -1119,10 +1145,11 @@ int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
   }
   int rc = ensureRing(b, itersMax);
   if (rc) return rc;
+  if (graph && !b->dGraph && (rc = dalloc(&b->dGraph, b->slotBytes))) return rc;
   uint32_t k = b->ringPos % BRING;
   if (b->evUsed[k]) HIPCHK(hipEventSynchronize(b->ev[k]));
   uint8_t* h = b->hRing + k * b->slotBytes;
-  uint8_t* d = b->dRing + k * b->slotBytes;
+  uint8_t* d = graph ? b->dGraph : b->dRing + k * b->slotBytes;
   BTank* hT = (BTank*)h;
   DParams* hP = (DParams*)(h + b->offP);
   DCols* hC = (DCols*)(h + b->offC);
@@ -1149,7 +1176,41 @@ int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
   }
   HIPCHK(hipEventRecord(b->ev[k], b->st));
   b->ringPos++;
-  launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+  if (!graph) {
+    launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+  } else {
+    // the block tables fix every launch's grid and kernel choice; the tank
+    // records, parameters and tables are read from the graph slot at run time
+    const size_t words = batch_table_words(nt, itersMax);
+    Batch::Captured* hit = nullptr;
+    for (auto& g : b->graphs)
+      if (g.tab.size() == words && std::memcmp(g.tab.data(), hTab, words * 4u) == 0) hit = &g;
+    if (hit) {
+      HIPCHK(hipGraphLaunch(hit->exec, b->st));
+    } else {
+      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+      hipError_t le = hipGetLastError();
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(b->st, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(ST_ERR_HIP, std::string("st_batch_step_n: capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      Batch::Captured c;
+      c.tab.assign(hTab, hTab + words);
+      hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(ST_ERR_HIP, std::string("st_batch_step_n: hipGraphInstantiate: ") + hipGetErrorString(ie));
+      HIPCHK(hipGraphLaunch(c.exec, b->st));
+      if (b->graphs.size() >= GRAPH_CACHE) {
+        HIPCHK(hipStreamSynchronize(b->st));
+        (void)hipGraphExecDestroy(b->graphs.front().exec);
+        b->graphs.erase(b->graphs.begin());
+      }
+      b->graphs.push_back(std::move(c));
+    }
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e));
   // what the single-tank stages leave behind: g2p wrote the new state into the
@@ -1165,6 +1226,22 @@ int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
     std::swap(s->G.pos, s->G.posB);
     std::swap(s->G.vel, s->G.velB);
     std::swap(s->G.pid, s->G.pidB);
+  }
+  return ST_OK;
+}
+
+int st_batch_step(void* p, const st_params* prm, uint32_t nprm) {
+  Batch* b = (Batch*)p;
+  if (!b || !prm) return fail(ST_ERR_ARG, "st_batch_step: null argument");
+  return batchStep(b, prm, nprm, false);
+}
+
+int st_batch_step_n(void* p, const st_params* prm, uint32_t nprm, int32_t nsteps, int32_t use_graph) {
+  Batch* b = (Batch*)p;
+  if (!b || !prm) return fail(ST_ERR_ARG, "st_batch_step_n: null argument");
+  for (int32_t k = 0; k < nsteps; k++) {
+    int rc = batchStep(b, prm, nprm, use_graph != 0);
+    if (rc) return rc;
   }
   return ST_OK;
 }

@@ -89,7 +89,8 @@ struct DGrid {
   uint32_t* coopFlags;   // per Morton position: 1 when the tile is in use
   uint32_t* coopOffs;    // exclusive prefix of coopFlags
   uint32_t* coopList;    // slots in use, in Morton order (per slot)
-  float* sorHalo;        // per colour, slot and face: the 32 face cells of that colour (2 x slots x 6 x 32)
+  uint64_t* sorHalo;     // per colour, slot and face: the 32 face cells of that colour, each with the tag of
+                         // the half-sweep that wrote it (2 x slots x 6 x 32 pairs: value bits low, tag high)
   uint32_t* sorBar;      // per resident-SOR workgroup: the last barrier epoch it reached
 };
 

@@ -1093,9 +1093,16 @@ __device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uin
 
 // sH for relaxing the colour opposite c: the colour-c cells across each face.
 // coop_fill_own copies them in LDS from our own tiles (zero where there is no
-// tile); it needs only our own colour-c face cells, so it runs while the grid
-// barrier is pending.  coop_fill_foreign fetches the faces other workgroups own
-// (listed in sFor, nFor of them; a thread's fetches are in flight together).
+// tile); it needs only our own colour-c face cells, so it runs while the other
+// workgroups' faces are on their way.  coop_fill_start reads the faces other
+// workgroups own (listed in sFor, nFor of them) from the pressure the solve
+// starts from; coop_fill_foreign takes them from the halo entries of the
+// half-sweep tagged `tag`: each entry is one 64-bit word, the f32 value in the
+// low half and the tag of the half-sweep that wrote it in the high half, stored
+// and loaded as a whole, so an entry with the right tag holds that half-sweep's
+// value and no flag or fence is needed.  A thread keeps polling its entries
+// (COOP_FOR_PER_THREAD in flight together) until their tags match, at most
+// COOP_SPIN_LIMIT rounds; a time-out sets *sFail.
 #define COOP_FOR_PER_THREAD 4
 __device__ __forceinline__ void coop_fill_own(const float* sP, float* sH, const uint32_t* sNb, uint32_t T, uint32_t c) {
   for (uint32_t q = threadIdx.x; q < T * 192u; q += COOP_WG) {
@@ -1109,31 +1116,56 @@ __device__ __forceinline__ void coop_fill_own(const float* sP, float* sH, const 
     }
   }
 }
-__device__ __forceinline__ void coop_fill_foreign(const DGrid& G, uint32_t scap, const float* __restrict__ pin, float* sH,
-                                                  const uint32_t* sNb, const uint32_t* sFor, uint32_t nFor, uint32_t c,
-                                                  bool fromPres) {
+__device__ __forceinline__ void coop_fill_start(const float* __restrict__ pin, float* sH, const uint32_t* sNb, const uint32_t* sFor,
+                                                uint32_t nFor, uint32_t c) {
+  for (uint32_t q = threadIdx.x; q < nFor * 32u; q += COOP_WG) {
+    uint32_t hf = sFor[q >> 5], e = q & 31u, f = hf % 6u;
+    uint32_t ns = sNb[hf] & ~COOP_NB_FOREIGN;
+    int3 a = coop_face_cell(f ^ 1u, e, c);
+    sH[hf * 32u + e] = pin[ns * 512u + uint32_t(a.x + 8 * (a.y + 8 * a.z))];
+  }
+}
+__device__ __forceinline__ size_t coop_halo_idx(uint32_t c, uint32_t scap, uint32_t slot, uint32_t f, uint32_t e) {
+  return ((size_t(c) * scap + slot) * 6u + f) * 32u + e;
+}
+__device__ __forceinline__ void coop_halo_put(const DGrid& G, size_t idx, float v, uint32_t tag) {
+  const uint64_t w = (uint64_t(tag) << 32) | uint64_t(__float_as_uint(v));
+  __hip_atomic_store(&G.sorHalo[idx], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void coop_fill_foreign(const DGrid& G, uint32_t scap, float* sH, const uint32_t* sNb, const uint32_t* sFor,
+                                                  uint32_t nFor, uint32_t c, uint32_t tag, uint32_t* sFail) {
   for (uint32_t q0 = 0; q0 < nFor * 32u; q0 += COOP_FOR_PER_THREAD * COOP_WG) {
-    float v[COOP_FOR_PER_THREAD];
+    uint64_t v[COOP_FOR_PER_THREAD];
+    size_t at[COOP_FOR_PER_THREAD];
 #pragma unroll
     for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) {
       uint32_t q = q0 + threadIdx.x + i * COOP_WG;
-      v[i] = 0.0f;
+      at[i] = 0;
+      v[i] = uint64_t(tag) << 32;
       if (q < nFor * 32u) {
         uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
-        uint32_t ns = sNb[hf] & ~COOP_NB_FOREIGN;
-        if (fromPres) {
-          int3 a = coop_face_cell(f ^ 1u, e, c);
-          v[i] = pin[ns * 512u + uint32_t(a.x + 8 * (a.y + 8 * a.z))];
-        } else {
-          v[i] = __hip_atomic_load(&G.sorHalo[((size_t(c) * scap + ns) * 6u + (f ^ 1u)) * 32u + e], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+        at[i] = coop_halo_idx(c, scap, sNb[hf] & ~COOP_NB_FOREIGN, f ^ 1u, e);
+        v[i] = __hip_atomic_load(&G.sorHalo[at[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+    }
+    for (uint32_t spins = 0;; spins++) {
+      bool ready = true;
+#pragma unroll
+      for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) ready = ready && uint32_t(v[i] >> 32) == tag;
+      if (ready) break;
+      if (spins >= COOP_SPIN_LIMIT) {
+        *sFail = 1u;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++)
+        if (uint32_t(v[i] >> 32) != tag) v[i] = __hip_atomic_load(&G.sorHalo[at[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) {
       uint32_t q = q0 + threadIdx.x + i * COOP_WG;
-      if (q < nFor * 32u) sH[sFor[q >> 5] * 32u + (q & 31u)] = v[i];
+      if (q < nFor * 32u) sH[sFor[q >> 5] * 32u + (q & 31u)] = __uint_as_float(uint32_t(v[i]));
     }
   }
 }
@@ -1220,7 +1252,7 @@ struct CoopLds {
 constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
 
 __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
-                                                          uint32_t seq, uint32_t stream) {
+                                                          uint32_t seq, uint32_t stream, uint32_t tag0) {
   __shared__ __align__(16) unsigned char smem[kCoopLdsBytes];
   __shared__ uint32_t nFor;
   __shared__ float ytab[8];
@@ -1321,7 +1353,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
   }
   __syncthreads();
   coop_fill_own(sP, sH, sNb, T, 1u);
-  coop_fill_foreign(G, scap, pres, sH, sNb, sFor, nFor, 1u, true);
+  coop_fill_start(pres, sH, sNb, sFor, nFor, 1u);
   __syncthreads();
   uint32_t epoch = 0;
   for (uint32_t it = 0; it < iters; it++) {
@@ -1344,14 +1376,13 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
       }
       __syncthreads();
       if (!last) {
-        // colour-c face cells other workgroups read, then this workgroup's arrival
+        // colour-c face cells other workgroups read, tagged with this half-sweep
+        ++epoch;
         for (uint32_t q = tid; q < nFor * 32u; q += COOP_WG) {
           uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
           int3 a = coop_face_cell(f, e, c);
-          __hip_atomic_store(&G.sorHalo[((size_t(c) * scap + sSlot[lt]) * 6u + f) * 32u + e],
-                             sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          coop_halo_put(G, coop_halo_idx(c, scap, sSlot[lt], f, e), sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], tag0 + epoch);
         }
-        coop_arrive(G, ++epoch);
       }
       // inside cells while the faces travel
 #pragma unroll
@@ -1367,9 +1398,12 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
       // our own faces of colour c are final and the face relaxation that read
       // sH is done: copy them while the other workgroups arrive
       coop_fill_own(sP, sH, sNb, T, c);
-      if (!coop_depart(G, epoch, &sFail)) return;
-      coop_fill_foreign(G, scap, pres, sH, sNb, sFor, nFor, c, false);
+      coop_fill_foreign(G, scap, sH, sNb, sFor, nFor, c, tag0 + epoch, &sFail);
       __syncthreads();
+      if (sFail) {  // a neighbour's faces never came: give up the solve (counted)
+        if (tid == 0u) atomicAdd(&G.sorBar[COOP_MAX_WG + 1], 1u);
+        return;
+      }
     }
   }
   __syncthreads();
@@ -2287,8 +2321,8 @@ hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
     launch_fill_u32(st, L.G.sorBar + COOP_MAX_WG + 2, 1, 0u);
     const DParams* P = L.P;
     DGrid G = L.G;
-    uint32_t it = uint32_t(iters), seq = L.sorSeq, stream = L.sorStream ? 1u : 0u;
-    void* args[] = {(void*)&P, (void*)&G, (void*)&it, (void*)&seq, (void*)&stream};
+    uint32_t it = uint32_t(iters), seq = L.sorSeq, stream = L.sorStream ? 1u : 0u, tag0 = L.sorTag;
+    void* args[] = {(void*)&P, (void*)&G, (void*)&it, (void*)&seq, (void*)&stream, (void*)&tag0};
     coopErr = hipLaunchCooperativeKernel((const void*)k_sor_resident, dim3(L.coopGrid), dim3(COOP_WG), args, 0, st);
     if (coopErr == hipSuccess) return hipSuccess;
     (void)hipGetLastError();  // the per-iteration launches below do the solve
