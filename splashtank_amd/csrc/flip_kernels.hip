@@ -1004,11 +1004,14 @@ __global__ void __launch_bounds__(WG) k_sor_pipe(const DParams* __restrict__ PP,
 // (sP), and the rhs and stencil of the cells each thread relaxes in registers.
 // Only the face layers cross workgroups: after relaxing colour c a workgroup
 // writes the colour-c cells of every face whose neighbour tile another
-// workgroup owns to sorHalo[c], all workgroups meet at a grid barrier, and each
-// copies the colour-c cells across all six faces of its tiles into sH (from its
-// own LDS, from sorHalo, or zero where there is no tile) before relaxing the
-// other colour.  The arithmetic is relax_lds's, cell for cell, so the iterate is
-// the reference's bit for bit.  When the active tiles exceed COOP_TMAX per
+// workgroup owns to sorHalo[c], each tagged with the half-sweep, and before
+// relaxing the other colour it waits for and copies the colour-c cells across
+// the faces toward other workgroups' tiles into sH (faces toward its own tiles
+// are read in place).  No grid barrier: a workgroup overwrites its colour-c
+// entries two half-sweeps later, only after it has read the next half-sweep's
+// entries of every workgroup that reads it (tiles that touch read each other),
+// which those write after their reads are done.  The arithmetic is
+// relax_lds's, cell for cell, so the iterate is the reference's bit for bit.  When the active tiles exceed COOP_TMAX per
 // workgroup the same launch streams tile windows instead (below).
 
 // index of tile-local cell (x, y, z) in the colour-split layout: colour
@@ -1091,30 +1094,21 @@ __device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uin
   return coop_depart(G, epoch, sFail);
 }
 
-// sH for relaxing the colour opposite c: the colour-c cells across each face.
-// coop_fill_own copies them in LDS from our own tiles (zero where there is no
-// tile); it needs only our own colour-c face cells, so it runs while the other
-// workgroups' faces are on their way.  coop_fill_start reads the faces other
-// workgroups own (listed in sFor, nFor of them) from the pressure the solve
-// starts from; coop_fill_foreign takes them from the halo entries of the
-// half-sweep tagged `tag`: each entry is one 64-bit word, the f32 value in the
-// low half and the tag of the half-sweep that wrote it in the high half, stored
-// and loaded as a whole, so an entry with the right tag holds that half-sweep's
-// value and no flag or fence is needed.  A thread keeps polling its entries
-// (COOP_FOR_PER_THREAD in flight together) until their tags match, at most
-// COOP_SPIN_LIMIT rounds; a time-out sets *sFail.
+// sH holds, per resident tile and face, the 32 cells across that face that the
+// face relaxation reads from another workgroup's tile (or zeros, set once by
+// coop_zero_none, where there is no tile; faces toward our own tiles are read
+// from those tiles directly).  coop_fill_start reads the foreign faces from the
+// pressure the solve starts from; coop_fill_foreign takes them from the halo
+// entries of the half-sweep tagged `tag`: each entry is one 64-bit word, the f32
+// value in the low half and the tag of the half-sweep that wrote it in the high
+// half, stored and loaded as a whole, so an entry with the right tag holds that
+// half-sweep's value and no flag or fence is needed.  A thread keeps polling its
+// entries (COOP_FOR_PER_THREAD in flight together) until their tags match, at
+// most COOP_SPIN_LIMIT rounds; a time-out sets *sFail.
 #define COOP_FOR_PER_THREAD 4
-__device__ __forceinline__ void coop_fill_own(const float* sP, float* sH, const uint32_t* sNb, uint32_t T, uint32_t c) {
-  for (uint32_t q = threadIdx.x; q < T * 192u; q += COOP_WG) {
-    uint32_t lt = q / 192u, r = q - lt * 192u, f = r >> 5, e = r & 31u;
-    uint32_t nb = sNb[lt * 6u + f];
-    if (nb & COOP_NB_FOREIGN) {
-      if (nb == COOP_NB_NONE) sH[q] = 0.0f;
-    } else {
-      int3 a = coop_face_cell(f ^ 1u, e, c);
-      sH[q] = sP[nb * 512u + coop_cidx(a.x, a.y, a.z)];
-    }
-  }
+__device__ __forceinline__ void coop_zero_none(float* sH, const uint32_t* sNb, uint32_t T) {
+  for (uint32_t q = threadIdx.x; q < T * 192u; q += COOP_WG)
+    if (sNb[q >> 5] == COOP_NB_NONE) sH[q] = 0.0f;
 }
 __device__ __forceinline__ void coop_fill_start(const float* __restrict__ pin, float* sH, const uint32_t* sNb, const uint32_t* sFor,
                                                 uint32_t nFor, uint32_t c) {
@@ -1194,12 +1188,15 @@ constexpr CoopCellTable coop_cell_table() {
 }
 __constant__ CoopCellTable kCoopCells = coop_cell_table();
 
-// one relaxation of cell l (colour c) of resident tile lt; a face cell takes
-// its neighbours across the face from the halo sH, an inside cell has all six
-// in the tile
+// one relaxation of cell l (colour c) of resident tile lt.  A neighbour inside
+// the tile is read from its other-colour array; across a face it is read from
+// the tile there when this workgroup holds it (the same index with the
+// coordinate wrapped into that tile), else from the halo copy sH of that face
+// (another workgroup's faces, or zeros where there is no tile).  One LDS load
+// per neighbour, from a chosen address.
 template <bool FACE>
-__device__ __forceinline__ void coop_relax(float* sP, const float* sH, uint32_t lt, uint32_t l, uint32_t c, float rhs, uint32_t st,
-                                           float omega, const float* ytab) {
+__device__ __forceinline__ void coop_relax(float* sP, const float* sH, const uint32_t* sNb, uint32_t lt, uint32_t l, uint32_t c,
+                                           float rhs, uint32_t st, float omega, const float* ytab) {
   const int32_t x = int32_t(l & 7u), y = int32_t((l >> 3) & 7u), z = int32_t(l >> 6), xh = x >> 1;
   float* cen = sP + lt * 512u + c * 256u + uint32_t((z * 8 + y) * 4 + xh);
   const uint32_t nns = st & 7u;
@@ -1207,19 +1204,31 @@ __device__ __forceinline__ void coop_relax(float* sP, const float* sH, uint32_t 
     *cen = 0.0f;
     return;
   }
-  const float* ob = sP + lt * 512u + (c ^ 1u) * 256u;
+  const uint32_t oc = (c ^ 1u) * 256u;
+  const float* ob = sP + lt * 512u + oc;
   float n0, n1, n2, n3, n4, n5;
   if (FACE) {
-    // inside-the-tile indices are clamped, so both arms of each choice stay in LDS
     const float* hb = sH + lt * 192u;
-    const int32_t xm = max(x - 1, 0), xp = min(x + 1, 7), ym = max(y - 1, 0), yp = min(y + 1, 7);
-    const int32_t zm = max(z - 1, 0), zp = min(z + 1, 7);
-    n0 = x > 0 ? ob[(z * 8 + y) * 4 + (xm >> 1)] : hb[0 * 32 + z * 4 + (y >> 1)];
-    n1 = x < 7 ? ob[(z * 8 + y) * 4 + (xp >> 1)] : hb[1 * 32 + z * 4 + (y >> 1)];
-    n2 = y > 0 ? ob[(z * 8 + ym) * 4 + xh] : hb[2 * 32 + z * 4 + xh];
-    n3 = y < 7 ? ob[(z * 8 + yp) * 4 + xh] : hb[3 * 32 + z * 4 + xh];
-    n4 = z > 0 ? ob[(zm * 8 + y) * 4 + xh] : hb[4 * 32 + y * 4 + xh];
-    n5 = z < 7 ? ob[(zp * 8 + y) * 4 + xh] : hb[5 * 32 + y * 4 + xh];
+    const uint32_t* nbs = sNb + lt * 6u;
+    // the address of the neighbour across face f: wrapped index w in the tile
+    // there, or index h in the face's halo copy
+    auto across = [&](uint32_t f, uint32_t w, uint32_t h) -> const float* {
+      const uint32_t nb = nbs[f];
+      return (nb & COOP_NB_FOREIGN) ? hb + f * 32u + h : sP + nb * 512u + oc + w;
+    };
+    const uint32_t row = uint32_t(z * 8 + y) * 4u;
+    const float* a0 = x > 0 ? ob + row + uint32_t((x - 1) >> 1) : across(0u, row + 3u, uint32_t(z * 4 + (y >> 1)));
+    const float* a1 = x < 7 ? ob + row + uint32_t((x + 1) >> 1) : across(1u, row, uint32_t(z * 4 + (y >> 1)));
+    const float* a2 = y > 0 ? ob + row - 4u + uint32_t(xh) : across(2u, uint32_t(z * 8 + 7) * 4u + uint32_t(xh), uint32_t(z * 4 + xh));
+    const float* a3 = y < 7 ? ob + row + 4u + uint32_t(xh) : across(3u, uint32_t(z * 8) * 4u + uint32_t(xh), uint32_t(z * 4 + xh));
+    const float* a4 = z > 0 ? ob + row - 32u + uint32_t(xh) : across(4u, uint32_t(56 + y) * 4u + uint32_t(xh), uint32_t(y * 4 + xh));
+    const float* a5 = z < 7 ? ob + row + 32u + uint32_t(xh) : across(5u, uint32_t(y) * 4u + uint32_t(xh), uint32_t(y * 4 + xh));
+    n0 = *a0;
+    n1 = *a1;
+    n2 = *a2;
+    n3 = *a3;
+    n4 = *a4;
+    n5 = *a5;
   } else {
     n0 = ob[(z * 8 + y) * 4 + ((x - 1) >> 1)];
     n1 = ob[(z * 8 + y) * 4 + ((x + 1) >> 1)];
@@ -1352,7 +1361,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     }
   }
   __syncthreads();
-  coop_fill_own(sP, sH, sNb, T, 1u);
+  coop_zero_none(sH, sNb, T);
   coop_fill_start(pres, sH, sNb, sFor, nFor, 1u);
   __syncthreads();
   uint32_t epoch = 0;
@@ -1371,7 +1380,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
         if (m < T * COOP_NSHELL && (st & STEN_FLUID)) {
           uint32_t lt = m / COOP_NSHELL, i = m - lt * COOP_NSHELL;
-          coop_relax<true>(sP, sH, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
+          coop_relax<true>(sP, sH, sNb, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
         }
       }
       __syncthreads();
@@ -1391,13 +1400,10 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
         if (m < T * COOP_NCORE && (st & STEN_FLUID)) {
           uint32_t lt = m / COOP_NCORE, i = m - lt * COOP_NCORE + COOP_NSHELL;
-          coop_relax<false>(sP, sH, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
+          coop_relax<false>(sP, sH, sNb, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
         }
       }
       if (last) break;
-      // our own faces of colour c are final and the face relaxation that read
-      // sH is done: copy them while the other workgroups arrive
-      coop_fill_own(sP, sH, sNb, T, c);
       coop_fill_foreign(G, scap, sH, sNb, sFor, nFor, c, tag0 + epoch, &sFail);
       __syncthreads();
       if (sFail) {  // a neighbour's faces never came: give up the solve (counted)
