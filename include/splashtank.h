@@ -96,6 +96,26 @@ typedef struct st_params {
    * give the same bits.  No counterpart in the reference (an implementation
    * choice here). */
   uint32_t sor_mode;
+  /* whitewater (o.spr, o.spr.ww, o.air; sim.js:203-206, 221-230).  The spray
+   * stage runs while the sim has room for whitewater points
+   * (st_set_spray_cap): the points there move every substep, and new ones are
+   * thrown off when spr_on. */
+  uint32_t spr_on;      /* o.spr given */
+  float spr_rate;       /* o.spr.rate: points per second per unit of emission */
+  float spr_min;        /* o.spr.min: crest speed where spray starts (at least 0.05) */
+  float spr_life;       /* o.spr.life: seconds a spray point lives */
+  float spr_mist;       /* o.spr.mist: share of fast spray that is mist */
+  float spr_drag;       /* o.spr.drag: spray drag toward the air, 1/s */
+  float spr_mdrag;      /* o.spr.mdrag: mist drag, 1/s */
+  float vs;             /* o.vs: metres per scene unit (0 -> 1) */
+  float ww_foam_life;   /* o.spr.ww.foamLife, s (0 -> 3) */
+  float ww_rise;        /* o.spr.ww.rise: bubble rise speed (0 -> 0.3) */
+  float ww_crest;       /* o.spr.ww.crest: weight of crest emission (1) */
+  float ww_impact;      /* o.spr.ww.impact: weight of churn emission (0.5) */
+  uint32_t air_on;      /* o.air.on: a coarse air field the spray drags toward (with spr_on) */
+  float air_couple;     /* o.air.couple, 1/s */
+  float air_decay;      /* o.air.decay, 1/s */
+  float air_blur;       /* o.air.blur (0.35) */
   uint32_t reserved[3];
 } st_params;
 
@@ -216,6 +236,17 @@ int st_mem_bytes(void* sim, uint64_t* bytes);
  * st_params.sor_mode).  No counterpart in the reference; for benchmarks and
  * tests. */
 int st_sor_path(void* sim, uint32_t* path);
+
+/* setSprayCap(n) (sim.js:333-340): room for n whitewater points (0 turns the
+ * spray stage off).  The new pool holds no points; the ring counter of handed
+ * out slots carries on. */
+int st_set_spray_cap(void* sim, uint32_t n);
+/* readSpray (sim.js:423-437): the first n = max(1, min(cap, slots handed out))
+ * whitewater slots, positions with the seconds of life left in w and
+ * velocities with the kind in w (0 spray, 1 mist, 2 foam, 3 bubble; the
+ * fraction is how long the point has had that kind); *n = 0 without a pool.
+ * Writes at most `room` points; blocks. */
+int st_read_spray(void* sim, float* pos4, float* vel4, uint32_t room, uint32_t* n);
 
 /* FP64 norms of the last substep's pressure solve and projection, over its fluid
  * cells, computed on the GPU in f64 from the f32 state the substep left (the

@@ -50,6 +50,38 @@ ST_HD float st_expf(float x) {
   return ldexpf(ldexpf(p, k1), k2);
 }
 
+/* sin and cos of x in f32 (used for |x| < 8).  k = rint(x * 2/pi); r = x - k*pi/2
+ * with pi/2 in three parts whose products with small k are exact (Cody-Waite);
+ * Taylor polynomials of sin (to r^11) and cos (to r^12) on |r| <= pi/4, whose
+ * truncation is below 2e-9; then the signs and swap of quadrant k mod 4. */
+ST_HD void st_sincosf(float x, float* sn, float* cs) {
+  const float two_over_pi = 0.636619772367581343f;
+  const float dp1 = 1.5703125f;
+  const float dp2 = 4.837512969970703125e-4f;
+  const float dp3 = 7.54978995489188216e-8f;
+  float kf = rintf(x * two_over_pi);
+  float r = x - kf * dp1;
+  r = r - kf * dp2;
+  r = r - kf * dp3;
+  float z = r * r;
+  float sp = -2.5052108385441718e-8f;
+  sp = sp * z + 2.7557319223985893e-6f;
+  sp = sp * z - 1.9841269841269841e-4f;
+  sp = sp * z + 8.3333333333333332e-3f;
+  sp = sp * z - 1.6666666666666666e-1f;
+  float s = r + r * (z * sp);
+  float cp = 2.0876756987868099e-9f;
+  cp = cp * z - 2.7557319223985888e-7f;
+  cp = cp * z + 2.4801587301587302e-5f;
+  cp = cp * z - 1.3888888888888889e-3f;
+  cp = cp * z + 4.1666666666666664e-2f;
+  cp = cp * z - 0.5f;
+  float c = 1.0f + z * cp;
+  int q = ((int)kf) & 3;
+  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
 ST_HD float st_mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 ST_HD float st_clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 /* WGSL smoothstep: t = clamp((x - lo) / (hi - lo), 0, 1); t * t * (3 - 2 t) */

@@ -39,6 +39,7 @@ const uint32_t NONE = 0xffffffffu;
 template <class R> struct M;
 template <> struct M<float> {
   static float exp(float x) { return st_expf(x); }
+  static void sincos(float x, float* sn, float* cs) { st_sincosf(x, sn, cs); }
   static float rnd(float x) { return rintf(x); }
   static float flo(float x) { return floorf(x); }
   static float sq(float x) { return sqrtf(x); }
@@ -46,6 +47,7 @@ template <> struct M<float> {
 };
 template <> struct M<double> {
   static double exp(double x) { return std::exp(x); }
+  static void sincos(double x, double* sn, double* cs) { *sn = std::sin(x); *cs = std::cos(x); }
   static double rnd(double x) { return std::nearbyint(x); }
   static double flo(double x) { return std::floor(x); }
   static double sq(double x) { return std::sqrt(x); }
@@ -104,6 +106,14 @@ template <class R> struct Params {
   uint32_t ocean;
   uint32_t nbInit, nfill, nfv;
   R fills[16][4];
+  // whitewater and air field (wgsl-core.js:10-16)
+  R sprRate, sprMin, sprLife, mistFrac, sprDrag, mistDrag, vs;
+  uint32_t smax;
+  R wwFoamLife, wwRise, wwCrest, wwImpact;
+  uint32_t airOn;
+  R airCouple, airDecay, airH;
+  uint32_t aNx, aNy, aNz;
+  R airBlur;
 };
 
 template <class R> struct Sim {
@@ -135,6 +145,14 @@ template <class R> struct Sim {
   bool nbOn = false;
   uint32_t initStep = 0;
   bool onlyTiles = false;
+  // whitewater (setSprayCap, airAlloc; sim.js:333-351): 4 values per point / air cell
+  uint32_t sprayCap = 0;
+  std::vector<R> sPos, sVel, sAux, air, air2;
+  std::vector<int32_t> airAcc;
+  uint32_t sctl[4] = {0, 0, 0, 0};
+  uint32_t airN = 0, aNx = 1, aNy = 1, aNz = 1;
+  double airH = 1.0;
+  bool airOnStep = false, sprOnStep = false;
 
   // Face values: from normFaces on, the reference keeps the new and old face
   // velocity as f32 bits in the same words that held the fixed-point sums
@@ -838,6 +856,201 @@ template <class R> struct Sim {
     x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
     return R(float(x)) * R(2.3283064e-10);
   }
+  // ---- whitewater and the air field (wgsl-whitewater.js:1-163) ----
+  R wetCell(int32_t i, int32_t j, int32_t k) const {  // 7-13
+    if (!inDom(I3{i, j, k}) || tmap[tileOf(i, j, k)] == NONE) return R(0);
+    uint32_t c = cellIdx(i, j, k);
+    if (P.nb != 0u && (ctype[c] & 0xffu) == 1u && phiOf(c) < R(-1.0)) return R(1);
+    return fmn(R(cnt[c]) / P.rest, R(1));
+  }
+  R wetAt(V3<R> p) const {  // 14-22
+    V3<R> g = v3((p.x - P.x0) * P.ih - R(0.5), (p.y - P.y0) * P.ih - R(0.5), (p.z - P.z0) * P.ih - R(0.5));
+    V3<R> f0 = v3(M<R>::flo(g.x), M<R>::flo(g.y), M<R>::flo(g.z));
+    V3<R> fr = g - f0;
+    I3 i0{toi(f0.x), toi(f0.y), toi(f0.z)};
+    R s = R(0);
+    for (uint32_t c = 0; c < 8; c++) {
+      int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+      R w = (dx == 1 ? fr.x : R(1) - fr.x) * (dy == 1 ? fr.y : R(1) - fr.y) * (dz == 1 ? fr.z : R(1) - fr.z);
+      if (w > R(0)) s = s + w * wetCell(i0.x + dx, i0.y + dy, i0.z + dz);
+    }
+    return s;
+  }
+  uint32_t airIdx(int32_t i, int32_t j, int32_t k) const {  // 24-27
+    i = iclamp(i, 0, int32_t(P.aNx) - 1);
+    j = iclamp(j, 0, int32_t(P.aNy) - 1);
+    k = iclamp(k, 0, int32_t(P.aNz) - 1);
+    return uint32_t(i) + P.aNx * (uint32_t(j) + P.aNy * uint32_t(k));
+  }
+  V3<R> airCellV(uint32_t c) const { return v3(air[c * 4], air[c * 4 + 1], air[c * 4 + 2]); }
+  V3<R> airAt(V3<R> p) const {  // 29-39
+    V3<R> g = v3((p.x - P.x0) / P.airH - R(0.5), (p.y - P.y0) / P.airH - R(0.5), (p.z - P.z0) / P.airH - R(0.5));
+    V3<R> f0 = v3(M<R>::flo(g.x), M<R>::flo(g.y), M<R>::flo(g.z));
+    V3<R> fr = g - f0;
+    I3 i0{toi(f0.x), toi(f0.y), toi(f0.z)};
+    V3<R> s = v3(R(0), R(0), R(0));
+    for (uint32_t c = 0; c < 8; c++) {
+      int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+      R w = (dx == 1 ? fr.x : R(1) - fr.x) * (dy == 1 ? fr.y : R(1) - fr.y) * (dz == 1 ? fr.z : R(1) - fr.z);
+      s = s + airCellV(airIdx(i0.x + dx, i0.y + dy, i0.z + dz)) * w;
+    }
+    return s;
+  }
+  void airPush(V3<R> p, V3<R> v, R w) {  // 40-45
+    uint32_t c = airIdx(toi(M<R>::flo((p.x - P.x0) / P.airH)), toi(M<R>::flo((p.y - P.y0) / P.airH)), toi(M<R>::flo((p.z - P.z0) / P.airH))) * 4u;
+    airAcc[c] += toi(v.x * w * R(256));
+    airAcc[c + 1] += toi(v.y * w * R(256));
+    airAcc[c + 2] += toi(v.z * w * R(256));
+    airAcc[c + 3] += toi(w * R(256));
+  }
+  void stageSpray() {  // the 'spray' stage (sim.js:296-305)
+    const uint32_t used = std::min(sctl[0], P.smax);
+    if (airOnStep) {
+      std::fill(airAcc.begin(), airAcc.end(), 0);  // airClear
+      for (uint32_t i = 0; i < P.n; i++) {  // airSplatW 54-59
+        if (pos[i * 4 + 1] < R(-1.0e5)) continue;
+        V3<R> p = v3(pos[i * 4], pos[i * 4 + 1], pos[i * 4 + 2]);
+        I3 c = particleCell(p);
+        if (hasStorage(c) && R(cnt[cellIdx(c.x, c.y, c.z)]) > R(0.85) * P.rest) continue;
+        airPush(p, v3(vel[i * 4], vel[i * 4 + 1], vel[i * 4 + 2]), R(1));
+      }
+      for (uint32_t j = 0; j < used; j++) {  // airSplatS 61-66
+        if (sPos[j * 4 + 3] <= R(0) || sVel[j * 4 + 3] > R(1.5)) continue;
+        airPush(v3(sPos[j * 4], sPos[j * 4 + 1], sPos[j * 4 + 2]), v3(sVel[j * 4], sVel[j * 4 + 1], sVel[j * 4 + 2]), R(0.25));
+      }
+      for (uint32_t c = 0; c < airN; c++) {  // airUpdate 68-86
+        int32_t i = int32_t(c % P.aNx), j = int32_t((c / P.aNx) % P.aNy), k = int32_t(c / (P.aNx * P.aNy));
+        V3<R> a = airCellV(c);
+        V3<R> nb = airCellV(airIdx(i + 1, j, k)) + airCellV(airIdx(i - 1, j, k)) + airCellV(airIdx(i, j + 1, k)) +
+                   airCellV(airIdx(i, j - 1, k)) + airCellV(airIdx(i, j, k + 1)) + airCellV(airIdx(i, j, k - 1));
+        nb = nb / R(6);
+        a = v3(mixs(a.x, nb.x, P.airBlur), mixs(a.y, nb.y, P.airBlur), mixs(a.z, nb.z, P.airBlur));
+        R w = R(airAcc[c * 4 + 3]) / R(256);
+        if (w > R(0)) {
+          V3<R> vt = v3(R(airAcc[c * 4]), R(airAcc[c * 4 + 1]), R(airAcc[c * 4 + 2])) / (w * R(256));
+          R t = R(1) - M<R>::exp(-P.airCouple * fmn(w, R(8)) * P.dt);
+          a = v3(mixs(a.x, vt.x, t), mixs(a.y, vt.y, t), mixs(a.z, vt.z, t));
+        }
+        V3<R> x = v3(P.x0 + (R(i) + R(0.5)) * P.airH, P.y0 + (R(j) + R(0.5)) * P.airH, P.z0 + (R(k) + R(0.5)) * P.airH);
+        for (uint32_t q = 0; q < P.ncols; q++) {
+          if (!colNear(q, x)) continue;
+          R phi;
+          V3<R> wn;
+          sdfEval(q, x, phi, wn);
+          if (phi < R(0.6) * P.airH) { a = colSurfVel(q, x); break; }
+        }
+        R t = R(1) - M<R>::exp(-P.airDecay * P.dt);
+        a = v3(mixs(a.x, P.q1, t), mixs(a.y, P.q2, t), mixs(a.z, P.q3, t));
+        air2[c * 4] = a.x; air2[c * 4 + 1] = a.y; air2[c * 4 + 2] = a.z; air2[c * 4 + 3] = R(0);
+      }
+      air = air2;  // airCopy
+    }
+    for (uint32_t j = 0; j < used; j++) {  // wwSample 88-95
+      if (sPos[j * 4 + 3] <= R(0)) continue;
+      V3<R> p = v3(sPos[j * 4], sPos[j * 4 + 1], sPos[j * 4 + 2]), vf = v3(R(0), R(0), R(0)), vo;
+      if (hasStorage(particleCell(p))) sampleBoth(p, vf, vo);
+      sAux[j * 4] = vf.x; sAux[j * 4 + 1] = vf.y; sAux[j * 4 + 2] = vf.z; sAux[j * 4 + 3] = wetAt(p);
+    }
+    const R hold = R(0.2f);  // WWHOLD
+    for (uint32_t j = 0; j < used; j++) {  // sprayStep 129-181
+      if (sPos[j * 4 + 3] <= R(0)) continue;
+      V3<R> v = v3(sVel[j * 4], sVel[j * 4 + 1], sVel[j * 4 + 2]);
+      R kw = sVel[j * 4 + 3];
+      uint32_t kind = tou(kw + R(0.5));
+      R tk = (kw - R(kind)) / R(0.4) * hold;
+      const bool settled = tk >= hold * R(0.999f);
+      const uint32_t k0 = kind;
+      R wet = sAux[j * 4 + 3];
+      V3<R> vf = v3(sAux[j * 4], sAux[j * 4 + 1], sAux[j * 4 + 2]);
+      R life = sPos[j * 4 + 3];
+      uint32_t h0 = j * 2654435761u + P.seed * 974711u;
+      if (kind == 0u && v.y < vf.y && wet > R(0.55) && tk > R(0.05)) { kind = 2u; life = P.wwFoamLife * (R(0.6) + R(0.8) * hash1(h0)); v = vf; }
+      else if (kind == 1u && wet > R(0.55)) { life = R(0); }
+      else if (kind == 2u && wet > R(0.97) && settled) { kind = 3u; }
+      else if (kind == 2u && wet < R(0.04) && settled) { kind = 0u; life = fmn(life, P.sprLife); }
+      else if (kind == 3u && wet < R(0.6) && settled) { kind = 2u; life = fmx(life, R(0.5) * P.wwFoamLife); }
+      if (kind != k0) tk = R(0); else tk = fmn(tk + P.dt, hold);
+      V3<R> p = v3(sPos[j * 4], sPos[j * 4 + 1], sPos[j * 4 + 2]);
+      if (kind <= 1u) {
+        const bool mist = kind == 1u;
+        V3<R> gust = v3(R(0), R(0), R(0));
+        if (P.q7 > R(0)) {
+          V3<R> gp = p * R(0.55) * P.vs;
+          R tt = P.q8 * R(0.6);
+          gust = v3(vnz(gp + v3(tt, R(0), R(0))), vnz(gp + v3(R(17.3), tt, R(4.1))), vnz(gp + v3(R(5.7), R(9.2), tt))) * R(2) - v3(R(1), R(1), R(1));
+        }
+        V3<R> base = v3(P.q1, P.q2, P.q3);
+        if (P.airOn != 0u) base = airAt(p);
+        V3<R> airv = base + gust * P.q7 * (mist ? R(2.5) : R(1));
+        v.y = v.y - P.gravity * P.dt;
+        v = v + (airv - v) * (R(1) - M<R>::exp(-(mist ? P.mistDrag : P.sprDrag) * P.dt));
+        life = life - P.dt;
+      } else if (kind == 2u) {
+        R vy = vf.y + clampr((wet - R(0.5)) * R(4), R(-1), R(1)) * P.h * R(4);
+        R t = R(1) - M<R>::exp(R(-10) * P.dt);
+        v = v3(mixs(v.x, vf.x, t), mixs(v.y, vy, t), mixs(v.z, vf.z, t));
+        life = life - P.dt;
+      } else {
+        V3<R> tv = vf + v3(R(0), P.wwRise, R(0));
+        R t = R(1) - M<R>::exp(R(-6) * P.dt);
+        v = v3(mixs(v.x, tv.x, t), mixs(v.y, tv.y, t), mixs(v.z, tv.z, t));
+        life = life - R(0.3) * P.dt;
+      }
+      V3<R> np = p + v * P.dt;
+      if (np.x < P.x0 || np.x > P.x1 || np.y < P.y0 || np.y > P.y1 || np.z < P.z0 || np.z > P.z1) life = R(0);
+      else if (P.ncols > 0u && colAt(np) >= 0) life = R(0);
+      sPos[j * 4] = np.x; sPos[j * 4 + 1] = np.y; sPos[j * 4 + 2] = np.z; sPos[j * 4 + 3] = life;
+      sVel[j * 4] = v.x; sVel[j * 4 + 1] = v.y; sVel[j * 4 + 2] = v.z; sVel[j * 4 + 3] = R(kind) + R(0.4) * tk / hold;
+    }
+    if (!sprOnStep) return;
+    uint32_t e = 0;  // sprayEmit 98-127: points in particle order, the ring's next slots
+    const uint32_t first = sctl[0];
+    for (uint32_t i = 0; i < P.n; i++) {
+      if (pos[i * 4 + 1] < R(-1.0e5)) continue;
+      V3<R> q = v3(pos[i * 4], pos[i * 4 + 1], pos[i * 4 + 2]), v = v3(vel[i * 4], vel[i * 4 + 1], vel[i * 4 + 2]);
+      R foam = vel[i * 4 + 3], spd = len(v);
+      R fast = clampr((spd * P.vs - P.sprMin) / (R(2) * P.sprMin), R(0), R(1));
+      I3 bc{toi(M<R>::flo((q.x - P.x0) * P.ih - R(0.5))), toi(M<R>::flo((q.y - P.y0) * P.ih - R(0.5))), toi(M<R>::flo((q.z - P.z0) * P.ih - R(0.5)))};
+      R rho = R(0);
+      for (int32_t dz = 0; dz <= 1; dz++)
+        for (int32_t dy = 0; dy <= 1; dy++)
+          for (int32_t dx = 0; dx <= 1; dx++) rho = rho + wetCell(bc.x + dx, bc.y + dy, bc.z + dz);
+      rho = rho / R(8);
+      R crest = (R(1) - smooth(R(0.3), R(0.75), rho)) * fast * (R(0.3) + R(0.7) * foam);
+      R churn = (R(1) - smooth(R(0.8), R(1), rho)) * clampr((foam - R(0.55)) / R(0.45), R(0), R(1)) * smooth(R(0.5), R(1.5), spd * P.vs);
+      R em = (P.wwCrest * crest + P.wwImpact * churn) * smooth(R(0.1), R(0.5), P.q8);
+      if (em <= R(0)) continue;
+      uint32_t h0 = i * 747796405u + P.seed * 2891336453u;
+      uint32_t k = std::min(tou(M<R>::flo(P.sprRate * em * P.dt + hash1(h0))), 4u);
+      for (uint32_t m = 0; m < k; m++, e++) {
+        if (e >= P.smax) continue;  // the ring came round to points placed this substep
+        uint32_t hh = h0 ^ ((m + 1u) * 0x9E3779B9u);
+        R z = hash1(hh + 1u) * R(2) - R(1), a = hash1(hh + 2u) * R(6.2831853f), sr = M<R>::sq(fmx(R(0), R(1) - z * z));
+        R sa, ca;
+        M<R>::sincos(a, &sa, &ca);
+        V3<R> dir = v3(sr * ca, sr * sa, z);
+        bool under = rho > R(0.7) && hash1(hh + 7u) < R(0.7);
+        bool mist = !under && hash1(hh + 3u) < P.mistFrac * fast;
+        V3<R> nv = v * (R(0.9) + R(0.2) * hash1(hh + 4u)) + dir * spd * (mist ? R(0.2) : R(0.06));
+        R life = P.sprLife * (R(0.6) + R(0.8) * hash1(hh + 5u)) * (mist ? R(1.6) : R(1));
+        R kind = mist ? R(1) : R(0);
+        if (under) { nv = v + dir * spd * R(0.05); life = P.wwFoamLife * (R(0.6) + R(0.8) * hash1(hh + 5u)); kind = R(3); }
+        uint32_t slot = (first + e) % P.smax;
+        if (sPos[slot * 4 + 3] > R(0)) continue;
+        V3<R> np = q + nv * P.dt + dir * P.h * R(0.3) * hash1(hh + 6u);
+        sPos[slot * 4] = np.x; sPos[slot * 4 + 1] = np.y; sPos[slot * 4 + 2] = np.z; sPos[slot * 4 + 3] = life;
+        sVel[slot * 4] = nv.x; sVel[slot * 4 + 1] = nv.y; sVel[slot * 4 + 2] = nv.z; sVel[slot * 4 + 3] = kind;
+      }
+    }
+    sctl[0] = first + e;
+  }
+  void setSprayCap(uint32_t n) {  // sim.js:333-340
+    if (n == sprayCap) return;
+    sprayCap = n;
+    sPos.assign(size_t(n) * 4, R(0));
+    sVel.assign(size_t(n) * 4, R(0));
+    sAux.assign(size_t(n) * 4, R(0));
+  }
   void stageReseed() {  // nbKill .. nbSpawn (wgsl-narrowband.js:93-149)
     for (uint32_t i = 0; i < P.n; i++) {  // nbKill
       if (pos[i * 4 + 1] < R(-1.0e5)) continue;
@@ -947,6 +1160,43 @@ template <class R> struct Sim {
     p.nfv = nfv;
     for (int k = 0; k < 16; k++) for (int q = 0; q < 4; q++) p.fills[k][q] = R(o->fills[k * 4 + q]);
     iters = std::max(1, int(o->iters));
+    {  // whitewater, air field (sim.js:203-206, 221-230)
+      const bool on = o->spr_on != 0 && sprayCap > 0, haveSp = o->spr_on != 0;
+      p.sprRate = on ? R(o->spr_rate) : R(0);
+      p.sprMin = on ? R(std::max(0.05f, o->spr_min)) : R(1);
+      p.sprLife = on ? R(o->spr_life) : R(1);
+      p.mistFrac = on ? R(o->spr_mist) : R(0);
+      p.sprDrag = on ? R(o->spr_drag) : R(1);
+      p.mistDrag = on ? R(o->spr_mdrag) : R(4);
+      p.vs = o->vs != 0.0f ? R(o->vs) : R(1);
+      p.smax = sprayCap;
+      p.wwFoamLife = haveSp && o->ww_foam_life != 0.0f ? R(o->ww_foam_life) : R(3);
+      p.wwRise = haveSp && o->ww_rise != 0.0f ? R(o->ww_rise) : R(0.3);
+      p.wwCrest = haveSp ? R(o->ww_crest) : R(1);
+      p.wwImpact = haveSp ? R(o->ww_impact) : R(0.5);
+      const bool airo = on && o->air_on != 0;
+      if (airo && !airN) {  // airAlloc (sim.js:343-351)
+        const double r = std::max(2.0, std::ceil(std::cbrt(double(ncell) / 260000.0)));
+        aNx = uint32_t(std::ceil(double(nx) / r)) + 1u;
+        aNy = uint32_t(std::ceil(double(ny) / r)) + 1u;
+        aNz = uint32_t(std::ceil(double(nz) / r)) + 1u;
+        airN = aNx * aNy * aNz;
+        airH = hd_ * r;
+        air.assign(size_t(airN) * 4, R(0));
+        air2.assign(size_t(airN) * 4, R(0));
+        airAcc.assign(size_t(airN) * 4, 0);
+      }
+      p.airOn = airo && airN ? 1u : 0u;
+      p.airCouple = airo ? R(o->air_couple) : R(0);
+      p.airDecay = airo ? R(o->air_decay) : R(0);
+      p.airH = airN ? R(airH) : R(1);
+      p.aNx = airN ? aNx : 1u;
+      p.aNy = airN ? aNy : 1u;
+      p.aNz = airN ? aNz : 1u;
+      p.airBlur = airo ? R(o->air_blur) : R(0);
+      airOnStep = p.airOn != 0u;
+      sprOnStep = on;
+    }
     return ST_OK;
   }
   int runStage(int st) {
@@ -962,7 +1212,7 @@ template <class R> struct Sim {
       case ST_STAGE_PRESSURE: if (!onlyTiles) stagePressure(); break;
       case ST_STAGE_PROJECT: if (!onlyTiles) { project(); if (nbOn) saveVel(); } break;
       case ST_STAGE_G2P: if (!onlyTiles) stageG2P(); break;
-      case ST_STAGE_SPRAY: break;
+      case ST_STAGE_SPRAY: if (!onlyTiles && sprayCap) stageSpray(); break;
       case ST_STAGE_RESEED: if (!onlyTiles && nbOn) stageReseed(); break;
       default: return ST_ERR_ARG;
     }
@@ -1032,6 +1282,14 @@ struct Handle {
 
 thread_local std::string g_err;
 
+// the first max(1, min(cap, handed out)) slots, as f64 (4 per point)
+template <class R> static uint32_t readSpray(const Sim<R>* s, double* p4, double* v4, uint32_t room) {
+  if (!s->sprayCap) return 0;
+  uint32_t k = std::max(1u, std::min(s->sprayCap, s->sctl[0]));
+  if (k > room) return ~0u;
+  for (size_t q = 0; q < size_t(k) * 4; q++) { p4[q] = double(s->sPos[q]); v4[q] = double(s->sVel[q]); }
+  return k;
+}
 }  // namespace
 
 extern "C" {
@@ -1097,6 +1355,44 @@ int orc_set_colliders(void* hp, const float* cf, uint32_t nc, const float* sdf, 
     s->ncolsTotal = nc;
   };
   if (h->fp64) set(h->d); else set(h->f);
+  return ST_OK;
+}
+int orc_set_spray_cap(void* hp, uint32_t n) {
+  Handle* h = (Handle*)hp;
+  if (h->fp64) h->d->setSprayCap(n); else h->f->setSprayCap(n);
+  return ST_OK;
+}
+// test hooks: put whitewater points into slots 0..n-1 and set the ring counter;
+// read the air field (4 values per air cell)
+int orc_spray_put(void* hp, const double* p4, const double* v4, uint32_t n, uint32_t handed) {
+  Handle* h = (Handle*)hp;
+  auto put = [&](auto* s) {
+    if (n > s->sprayCap) return ST_ERR_ARG;
+    for (size_t q = 0; q < size_t(n) * 4; q++) { s->sPos[q] = p4[q]; s->sVel[q] = v4[q]; }
+    s->sctl[0] = handed;
+    return ST_OK;
+  };
+  int rc = h->fp64 ? put(h->d) : put(h->f);
+  if (rc) g_err = "orc_spray_put: more points than the pool holds";
+  return rc;
+}
+int orc_read_air(void* hp, double* out, uint32_t room, uint32_t* n, double* cell) {
+  Handle* h = (Handle*)hp;
+  auto rd = [&](auto* s) {
+    *n = s->airN;
+    *cell = s->airH;
+    if (s->airN > room) return ST_ERR_ARG;
+    for (size_t q = 0; q < size_t(s->airN) * 4; q++) out[q] = double(s->air[q]);
+    return ST_OK;
+  };
+  return h->fp64 ? rd(h->d) : rd(h->f);
+}
+void orc_sincosf(float x, float* sn, float* cs) { st_sincosf(x, sn, cs); }
+int orc_read_spray(void* hp, double* p4, double* v4, uint32_t room, uint32_t* n) {
+  Handle* h = (Handle*)hp;
+  uint32_t k = h->fp64 ? readSpray(h->d, p4, v4, room) : readSpray(h->f, p4, v4, room);
+  if (k == ~0u) { g_err = "orc_read_spray: output arrays too small"; return ST_ERR_ARG; }
+  *n = k;
   return ST_OK;
 }
 int orc_begin_step(void* hp, const st_params* p) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, beginStep(p)); }

@@ -31,10 +31,22 @@ _PROTOS = {
     "orc_dump_size": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_size_t)]),
     "orc_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "orc_threads": (C.c_int, []),
+    "orc_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "orc_spray_put": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "orc_read_air": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "orc_sincosf": (None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "orc_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "orc_expf": (C.c_float, [C.c_float]),
 }
 
 _lib = None
+
+
+def sincosf(x: float):
+    """the shared f32 sin/cos definition (include/st_math.h)"""
+    s, c = C.c_float(0), C.c_float(0)
+    load().orc_sincosf(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def load(path: str = LIB_PATH):
@@ -77,6 +89,32 @@ class Oracle(_Base):
 
     def read_particles(self, n=None):
         return self.dump("pos")
+
+    def set_spray_cap(self, n):
+        self._call("set_spray_cap", self.h, int(n))
+
+    def spray_put(self, p4, v4, handed=None):
+        """test hook: whitewater points into slots 0..n-1, ring counter = handed (default n)"""
+        p4 = np.ascontiguousarray(p4, np.float64).reshape(-1, 4)
+        v4 = np.ascontiguousarray(v4, np.float64).reshape(-1, 4)
+        n = p4.shape[0]
+        self._call("spray_put", self.h, p4.ctypes.data, v4.ctypes.data, n, n if handed is None else int(handed))
+
+    def read_air(self, room=1 << 20):
+        """the air field, (cells, 4) and the cell size"""
+        a = np.zeros((room, 4), np.float64)
+        n = C.c_uint32(0)
+        cell = C.c_double(0)
+        self._call("read_air", self.h, a.ctypes.data, room, C.byref(n), C.byref(cell))
+        return a[:n.value], cell.value
+
+    def read_spray(self, room=1 << 22):
+        """(positions with life in w, velocities with kind in w) of the slots handed out, f64 arrays."""
+        p = np.zeros((room, 4), np.float64)
+        v = np.zeros((room, 4), np.float64)
+        n = C.c_uint32(0)
+        self._call("read_spray", self.h, p.ctypes.data, v.ctypes.data, room, C.byref(n))
+        return p[:n.value].astype(self.fdtype), v[:n.value].astype(self.fdtype)
 
     def close(self):
         if getattr(self, "h", None):

@@ -54,7 +54,12 @@ class StParams(C.Structure):
                 ("no_cull", C.c_uint32), ("fast_air", C.c_uint32), ("no_solid_cache", C.c_uint32),
                 ("nb_on", C.c_uint32), ("nb_band", C.c_float), ("only_tiles", C.c_uint32),
                 ("nb_init", C.c_uint32), ("nfill", C.c_uint32), ("nfv", C.c_uint32),
-                ("fills", C.c_float * 64), ("sor_mode", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("fills", C.c_float * 64), ("sor_mode", C.c_uint32),
+                ("spr_on", C.c_uint32), ("spr_rate", C.c_float), ("spr_min", C.c_float), ("spr_life", C.c_float),
+                ("spr_mist", C.c_float), ("spr_drag", C.c_float), ("spr_mdrag", C.c_float), ("vs", C.c_float),
+                ("ww_foam_life", C.c_float), ("ww_rise", C.c_float), ("ww_crest", C.c_float), ("ww_impact", C.c_float),
+                ("air_on", C.c_uint32), ("air_couple", C.c_float), ("air_decay", C.c_float), ("air_blur", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
 
 
 class StNorms(C.Structure):
@@ -104,6 +109,9 @@ def make_params(**kw) -> StParams:
     p.foam_tau = 1.2
     p.foam_sens = 1.0
     p.nb_band = 3.0
+    p.ww_crest = 1.0
+    p.ww_impact = 0.5
+    p.air_blur = 0.35
     for k, v in kw.items():
         if k == "wind":
             for q in range(3):
@@ -142,6 +150,8 @@ PROTOS = {
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
+    "st_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -311,6 +321,18 @@ class FlipSimC(_Base):
         v = C.c_uint32(0)
         self._call("sor_path", self.h, C.byref(v))
         return v.value
+
+    def set_spray_cap(self, n):
+        """setSprayCap: room for n whitewater points (0 turns the spray stage off)."""
+        self._call("set_spray_cap", self.h, int(n))
+
+    def read_spray(self, room=1 << 22):
+        """readSpray: (positions with life in w, velocities with kind in w) of the slots handed out."""
+        p = np.zeros((room, 4), np.float32)
+        v = np.zeros((room, 4), np.float32)
+        n = C.c_uint32(0)
+        self._call("read_spray", self.h, p.ctypes.data, v.ctypes.data, room, C.byref(n))
+        return p[:n.value], v[:n.value]
 
     def solve_norms(self):
         """FP64 norms of the last substep's solve and projection (st_solve_norms), as a dict."""

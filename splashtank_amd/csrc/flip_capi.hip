@@ -112,6 +112,11 @@ struct Sim {
   bool binned = false;
   NbScratch nb{};
   void* batch = nullptr;          // the batch the tank belongs to (its stream is the batch's)
+  // whitewater (setSprayCap, airAlloc; sim.js:333-351)
+  uint32_t sprayCap = 0;
+  uint32_t airN = 0, aNx = 1, aNy = 1, aNz = 1;  // air field cells (0: not made yet)
+  double airH = 1.0;
+  bool airOnStep = false, sprOnStep = false;
   double* normPart = nullptr;     // st_solve_norms: per-workgroup partials, then the result
   uint32_t normCap = 0;           // workgroups normPart has room for
   hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
@@ -135,6 +140,7 @@ struct Sim {
     L.binNext = !nbOn;  // the narrow band's reseed moves particles after g2p
     L.nbOn = nbOn;
     L.initStep = initStep;
+    L.smax = sprayCap;
     return L;
   }
 };
@@ -219,6 +225,8 @@ void destroySim(Sim* s) {
   dfree(s->G.mortonTiles); dfree(s->G.coopFlags); dfree(s->G.coopOffs); dfree(s->G.sorBar);
   dfree(s->G.scratch4);
   dfree(s->normPart);
+  dfree(s->G.sPos); dfree(s->G.sVel); dfree(s->G.sAux); dfree(s->G.sctl); dfree(s->G.sprCnt); dfree(s->G.sprOff); dfree(s->G.sprSums);
+  dfree(s->G.air); dfree(s->G.air2); dfree(s->G.airAcc);
   freeSlots(s);
   dfree(s->dSdf);
   dfree(s->dSlots);
@@ -233,6 +241,24 @@ void destroySim(Sim* s) {
 }
 
 Sim* getSim(void* p) { return (Sim*)p; }
+
+// airAlloc (sim.js:343-351): the air field's grid, cells a few solver cells
+// wide, at most about 260 thousand of them; made when first used, zeroed
+int airAlloc(Sim* s) {
+  const double r = std::max(2.0, std::ceil(std::cbrt(double(s->ncell) / 260000.0)));
+  const uint32_t nx = uint32_t(std::ceil(double(s->nx) / r)) + 1u, ny = uint32_t(std::ceil(double(s->ny) / r)) + 1u,
+                 nz = uint32_t(std::ceil(double(s->nz) / r)) + 1u, n = nx * ny * nz;
+  HIPCHK(hipStreamSynchronize(s->st));
+  int rc;
+  if ((rc = dalloc(&s->G.air, n)) || (rc = dalloc(&s->G.air2, n)) || (rc = dalloc(&s->G.airAcc, size_t(n) * 4))) return rc;
+  HIPCHK(hipMemset(s->G.air, 0, size_t(n) * 16));
+  HIPCHK(hipMemset(s->G.air2, 0, size_t(n) * 16));
+  HIPCHK(hipMemset(s->G.airAcc, 0, size_t(n) * 16));
+  s->airH = s->hd * r;
+  s->aNx = nx; s->aNy = ny; s->aNz = nz;
+  s->airN = n;
+  return ST_OK;
+}
 
 // the step's parameter block and collider records (sim.js:179-231) into P and C
 int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
@@ -294,6 +320,38 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   P.nfv = s->initStep ? o->nfv : 0u;
   P.ntiles = s->ntiles;
   P.settle = st_smoothstepf(0.1f, 0.5f, P.q8);
+  {  // whitewater, air field (sim.js:203-206, 221-230)
+    const bool on = o->spr_on != 0 && s->sprayCap > 0, haveSp = o->spr_on != 0;
+    P.sprRate = on ? o->spr_rate : 0.0f;
+    P.sprMin = on ? std::max(0.05f, o->spr_min) : 1.0f;
+    P.sprLife = on ? o->spr_life : 1.0f;
+    P.mistFrac = on ? o->spr_mist : 0.0f;
+    P.sprDrag = on ? o->spr_drag : 1.0f;
+    P.mistDrag = on ? o->spr_mdrag : 4.0f;
+    P.vs = o->vs != 0.0f ? o->vs : 1.0f;
+    P.smax = s->sprayCap;
+    // ww = (sp && sp.ww) || {}: without o.spr the defaults hold
+    P.wwFoamLife = haveSp && o->ww_foam_life != 0.0f ? o->ww_foam_life : 3.0f;
+    P.wwRise = haveSp && o->ww_rise != 0.0f ? o->ww_rise : 0.3f;
+    P.wwCrest = haveSp ? o->ww_crest : 1.0f;
+    P.wwImpact = haveSp ? o->ww_impact : 0.5f;
+    const bool airo = on && o->air_on != 0;
+    if (airo && !s->airN) {
+      int rc = airAlloc(s);
+      if (rc) return rc;
+    }
+    P.airOn = airo && s->airN ? 1u : 0u;
+    P.airCouple = airo ? o->air_couple : 0.0f;
+    P.airDecay = airo ? o->air_decay : 0.0f;
+    P.airH = s->airN ? float(s->airH) : 1.0f;
+    P.aNx = s->airN ? s->aNx : 1u;
+    P.aNy = s->airN ? s->aNy : 1u;
+    P.aNz = s->airN ? s->aNz : 1u;
+    P.airBlur = airo ? o->air_blur : 0.0f;
+    P.sprOn = on ? 1u : 0u;
+    s->airOnStep = P.airOn != 0u;
+    s->sprOnStep = on;
+  }
   for (int k = 0; k < 16; k++)
     for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
   s->iters = std::max(1, int(o->iters));
@@ -375,7 +433,9 @@ int runStage(Sim* s, int stage) {
         }
       }
       break;
-    case ST_STAGE_SPRAY: break;
+    case ST_STAGE_SPRAY:
+      if (full && s->sprayCap) launch_spray(st, L, s->airN, s->airOnStep, s->sprOnStep);
+      break;
     case ST_STAGE_RESEED:
       if (full && s->nbOn) {
         launch_reseed(st, L, s->nb);
@@ -615,6 +675,9 @@ void st_params_default(st_params* p) {
   p->foam_tau = 1.2f;
   p->foam_sens = 1.0f;
   p->nb_band = 3.0f;
+  p->ww_crest = 1.0f;
+  p->ww_impact = 0.5f;
+  p->air_blur = 0.35f;
 }
 
 int st_create(const st_opts* o, void** out) {
@@ -769,6 +832,11 @@ int st_clear(void* p) {
   for (size_t k = 0; k < ids.size(); k++) ids[k] = uint32_t(k);
   HIPCHK(hipMemcpy(s->G.pid, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(s->G.pidB, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  if (s->sprayCap) {  // clear() (sim.js:543-544): no whitewater, the ring starts again, calm air
+    HIPCHK(hipMemset(s->G.sPos, 0, size_t(s->sprayCap) * 16));
+    HIPCHK(hipMemset(s->G.sctl, 0, 16));
+  }
+  if (s->airN) HIPCHK(hipMemset(s->G.air, 0, size_t(s->airN) * 16));
   return ST_OK;
 }
 
@@ -1023,6 +1091,50 @@ int st_sync(void* p) {
   return ST_OK;
 }
 
+int st_set_spray_cap(void* p, uint32_t n) {
+  Sim* s = getSim(p);
+  if (!s) return fail(ST_ERR_ARG, "st_set_spray_cap: null sim");
+  if (n > (1u << 28)) return fail(ST_ERR_ARG, "st_set_spray_cap: at most 2^28 whitewater points");
+  if (n == s->sprayCap) return ST_OK;
+  HIPCHK(hipSetDevice(s->dev));
+  HIPCHK(hipStreamSynchronize(s->st));
+  dropGraphs(s);
+  dfree(s->G.sPos); dfree(s->G.sVel); dfree(s->G.sAux);
+  s->sprayCap = 0;
+  if (n == 0) return ST_OK;
+  int rc;
+  if ((rc = dalloc(&s->G.sPos, n)) || (rc = dalloc(&s->G.sVel, n)) || (rc = dalloc(&s->G.sAux, n))) return rc;
+  HIPCHK(hipMemset(s->G.sPos, 0, size_t(n) * 16));
+  HIPCHK(hipMemset(s->G.sVel, 0, size_t(n) * 16));
+  HIPCHK(hipMemset(s->G.sAux, 0, size_t(n) * 16));
+  if (!s->G.sctl) {  // the ring counter outlives the pool, as the reference's bSctl does
+    if ((rc = dalloc(&s->G.sctl, 4))) return rc;
+    HIPCHK(hipMemset(s->G.sctl, 0, 16));
+    const size_t M = size_t(s->maxn) + 1;
+    if ((rc = dalloc(&s->G.sprCnt, M)) || (rc = dalloc(&s->G.sprOff, M + 1)) || (rc = dalloc(&s->G.sprSums, M / 1024 + 2))) return rc;
+  }
+  s->sprayCap = n;
+  return ST_OK;
+}
+
+int st_read_spray(void* p, float* pos4, float* vel4, uint32_t room, uint32_t* n) {
+  Sim* s = getSim(p);
+  if (!s || !n) return fail(ST_ERR_ARG, "st_read_spray: null argument");
+  *n = 0;
+  if (!s->sprayCap) return ST_OK;
+  HIPCHK(hipSetDevice(s->dev));
+  uint32_t used = 0;
+  HIPCHK(hipMemcpyAsync(&used, s->G.sctl, 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  uint32_t k = std::max(1u, std::min(s->sprayCap, used));
+  if (k > room || !pos4 || !vel4) return fail(ST_ERR_ARG, "st_read_spray: output arrays too small");
+  HIPCHK(hipMemcpyAsync(pos4, s->G.sPos, size_t(k) * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(vel4, s->G.sVel, size_t(k) * 16, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  *n = k;
+  return ST_OK;
+}
+
 int st_solve_norms(void* p, st_norms* out) {
   Sim* s = getSim(p);
   if (!s || !out) return fail(ST_ERR_ARG, "st_solve_norms: null argument");
@@ -1141,6 +1253,7 @@ static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) 
   for (uint32_t t = 0; t < nt; t++) {
     const st_params* o = prm + (nprm == 1u ? 0u : t);
     if (o->nb_on || o->only_tiles) return fail(ST_ERR_ARG, "st_batch_step: the narrow band and tile-only steps are single-tank only");
+    if (b->tanks[t]->sprayCap) return fail(ST_ERR_ARG, "st_batch_step: whitewater is single-tank only (a tank has a spray pool)");
     itersMax = std::max<uint32_t>(itersMax, b->tanks[t]->N ? uint32_t(std::max(1, int(o->iters))) : 0u);
   }
   int rc = ensureRing(b, itersMax);

@@ -34,6 +34,16 @@ struct DParams {
   float settle;  // smoothstep(0.1, 0.5, tm), the same for every particle of the step
   uint32_t pad1;
   float fills[16][4];
+  // whitewater and the air field (wgsl-core.js:10-16; sim.js:203-206, 221-230)
+  float sprRate, sprMin, sprLife, mistFrac, sprDrag, mistDrag, vs;
+  uint32_t smax;      // whitewater slots (0: no spray stage)
+  float wwFoamLife, wwRise, wwCrest, wwImpact;
+  uint32_t airOn;
+  float airCouple, airDecay, airH;
+  uint32_t aNx, aNy, aNz;
+  float airBlur;
+  uint32_t sprOn;     // new points are thrown off this step
+  uint32_t pad2, pad3, pad4;
 };
 
 // Collider records: 8 float4 per collider (gpu.js:3-4): (sdf offset, nx, ny, nz),
@@ -92,6 +102,17 @@ struct DGrid {
   uint64_t* sorHalo;     // per colour, slot and face: the 32 face cells of that colour, each with the tag of
                          // the half-sweep that wrote it (2 x slots x 6 x 32 pairs: value bits low, tag high)
   uint32_t* sorBar;      // per resident-SOR workgroup: the last barrier epoch it reached
+  // whitewater (setSprayCap, sim.js:333-340): smax slots of points, control words, and the air field
+  float4* sPos;          // xyz, seconds of life left (0: free)
+  float4* sVel;          // velocity, kind (0 spray, 1 mist, 2 foam, 3 bubble) + 0.4 x how long it has had it
+  float4* sAux;          // water velocity at the point (xyz), how wet it is (w)
+  uint32_t* sctl;        // [0] slots handed out (a ring counter)
+  uint32_t* sprCnt;      // per particle id: points it throws off this step; then the exclusive prefix (N + 1)
+  uint32_t* sprOff;
+  uint32_t* sprSums;     // per-1024 block sums of that scan
+  float4* air;           // air field cell velocities, and the next step's
+  float4* air2;
+  int32_t* airAcc;       // per air cell: fixed-point (x 256) velocity and weight pushed into it
 };
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,7 @@ struct LaunchCtx {
   bool binNext;       // g2p: mark and bin the positions it writes, for the next substep
   bool nbOn;          // narrow band on this step
   uint32_t initStep;  // narrow-band start: 1 restart, 2 water added, 0 none
+  uint32_t smax;      // whitewater slots (host copy of P->smax)
 };
 
 // scratch of the narrow-band reseed: dead-slot compaction over particles and the
@@ -105,6 +106,9 @@ void morton_tiles(uint32_t tx, uint32_t ty, uint32_t tz, uint32_t* out);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);
+// the spray stage (wgsl-whitewater.js): the air field when `air` (airN cells),
+// then every whitewater point, then new points when `emit`
+void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit);
 // FP64 norms of the last pressure solve and projection (flip_norms.hip): part
 // holds L.cellGrid x ST_NORM_WORDS doubles, out ST_NORM_WORDS (the st_norms fields)
 #define ST_NORM_WORDS 8

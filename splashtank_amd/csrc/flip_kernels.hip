@@ -2478,3 +2478,326 @@ void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, co
   BLAUNCH(kb_g2p_tile, BT_G2P, WG);
 #undef BLAUNCH
 }
+
+// ---------------------------------------------------------------------------
+// Whitewater and the air field (src/flip/wgsl-whitewater.js:1-163): one pool of
+// smax points that never push the water -- spray, mist, foam and bubbles --
+// thrown off by fast exposed water and churn, moved by kind each substep; and a
+// coarse air field the spray drags toward.  Slots are handed out as a ring
+// (sctl[0] counts them); until the ring wraps only the first sctl[0] can be
+// alive, so the point kernels loop over min(sctl[0], smax) slots read on the
+// device.  New points take consecutive slots in particle-id order (an exclusive
+// scan of each particle's count), where the reference takes them in the order
+// its atomics land; a slot that still holds a living point is skipped, as there.
+
+#define WWHOLD 0.2f
+
+// wetCell / wetAt (wgsl-whitewater.js:7-22): how much water is at a cell, 0..1
+__device__ __forceinline__ float d_wetCell(const DParams& P, const DGrid& G, int32_t i, int32_t j, int32_t k) {
+  if (!d_inDom(P, make_int3(i, j, k))) return 0.0f;
+  uint32_t s = G.tmap[d_tileOf(P, i, j, k)];
+  if (s == ST_NONE) return 0.0f;
+  uint32_t c = d_cellIdx(P, G.tmap, i, j, k);
+  if (P.nb != 0u && (G.ctype[c] & 0xffu) == 1u && d_phiOfBits(G.ctype[c]) < -1.0f) return 1.0f;
+  return fminf(float(G.cnt[c]) / P.rest, 1.0f);
+}
+__device__ __forceinline__ float d_wetAt(const DParams& P, const DGrid& G, float3 p) {
+  float3 g = make_float3((p.x - P.x0) * P.ih - 0.5f, (p.y - P.y0) * P.ih - 0.5f, (p.z - P.z0) * P.ih - 0.5f);
+  float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
+  float3 fr = f3sub(g, f0);
+  int32_t i0 = st_f2i_sat(f0.x), j0 = st_f2i_sat(f0.y), k0 = st_f2i_sat(f0.z);
+  float s = 0.0f;
+  for (uint32_t c = 0; c < 8u; c++) {
+    int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+    float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+    if (w > 0.0f) s = s + w * d_wetCell(P, G, i0 + dx, j0 + dy, k0 + dz);
+  }
+  return s;
+}
+// air grid (wgsl-whitewater.js:23-50): cells of airH from the box's low corner
+__device__ __forceinline__ uint32_t d_airIdx(const DParams& P, int32_t i, int32_t j, int32_t k) {
+  i = d_iclamp(i, 0, int32_t(P.aNx) - 1);
+  j = d_iclamp(j, 0, int32_t(P.aNy) - 1);
+  k = d_iclamp(k, 0, int32_t(P.aNz) - 1);
+  return uint32_t(i) + P.aNx * (uint32_t(j) + P.aNy * uint32_t(k));
+}
+__device__ __forceinline__ float3 d_airAt(const DParams& P, const float4* __restrict__ air, float3 p) {
+  float3 g = make_float3((p.x - P.x0) / P.airH - 0.5f, (p.y - P.y0) / P.airH - 0.5f, (p.z - P.z0) / P.airH - 0.5f);
+  float3 f0 = make_float3(floorf(g.x), floorf(g.y), floorf(g.z));
+  float3 fr = f3sub(g, f0);
+  int32_t i0 = st_f2i_sat(f0.x), j0 = st_f2i_sat(f0.y), k0 = st_f2i_sat(f0.z);
+  float3 s = make_float3(0.0f, 0.0f, 0.0f);
+  for (uint32_t c = 0; c < 8u; c++) {
+    int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
+    float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+    s = f3add(s, f3mul(c3(air[d_airIdx(P, i0 + dx, j0 + dy, k0 + dz)]), w));
+  }
+  return s;
+}
+__device__ __forceinline__ void d_airPush(const DParams& P, const DGrid& G, float3 p, float3 v, float w) {
+  int32_t ci = st_f2i_sat(floorf((p.x - P.x0) / P.airH)), cj = st_f2i_sat(floorf((p.y - P.y0) / P.airH)),
+          ck = st_f2i_sat(floorf((p.z - P.z0) / P.airH));
+  uint32_t c = d_airIdx(P, ci, cj, ck) * 4u;
+  atomicAdd(&G.airAcc[c], st_f2i_sat(v.x * w * 256.0f));
+  atomicAdd(&G.airAcc[c + 1u], st_f2i_sat(v.y * w * 256.0f));
+  atomicAdd(&G.airAcc[c + 2u], st_f2i_sat(v.z * w * 256.0f));
+  atomicAdd(&G.airAcc[c + 3u], st_f2i_sat(w * 256.0f));
+}
+__device__ __forceinline__ uint32_t d_wwUsed(const DParams& P, const DGrid& G) { return min(G.sctl[0], P.smax); }
+
+// airSplatW (54-59): water near the surface drags the air
+__global__ void k_air_splat_w(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+    float4 q = G.pos[i];
+    if (q.y < ST_DEADY) continue;
+    float3 p = c3(q);
+    int3 c = d_particleCell(P, p);
+    if (G.tmap[d_tileOf(P, c.x, c.y, c.z)] != ST_NONE && float(G.cnt[d_cellIdx(P, G.tmap, c.x, c.y, c.z)]) > 0.85f * P.rest) continue;
+    d_airPush(P, G, p, c3(G.vel[i]), 1.0f);
+  }
+}
+// airSplatS (61-66): flying spray drags it too
+__global__ void k_air_splat_s(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t used = d_wwUsed(P, G);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < used; j += gridDim.x * blockDim.x) {
+    float4 sp = G.sPos[j];
+    if (sp.w <= 0.0f || G.sVel[j].w > 1.5f) continue;
+    d_airPush(P, G, c3(sp), c3(G.sVel[j]), 0.25f);
+  }
+}
+// airUpdate (68-86): spread, pulled toward what pushed it, moved by objects, settled to the wind
+__global__ void k_air_update(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t n = P.aNx * P.aNy * P.aNz;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    int32_t i = int32_t(c % P.aNx), j = int32_t((c / P.aNx) % P.aNy), k = int32_t(c / (P.aNx * P.aNy));
+    float3 a = c3(G.air[c]);
+    float3 nb = f3add(f3add(f3add(f3add(f3add(c3(G.air[d_airIdx(P, i + 1, j, k)]), c3(G.air[d_airIdx(P, i - 1, j, k)])),
+                                            c3(G.air[d_airIdx(P, i, j + 1, k)])),
+                                      c3(G.air[d_airIdx(P, i, j - 1, k)])),
+                                c3(G.air[d_airIdx(P, i, j, k + 1)])),
+                          c3(G.air[d_airIdx(P, i, j, k - 1)]));
+    nb = f3div(nb, 6.0f);
+    a = make_float3(st_mixf(a.x, nb.x, P.airBlur), st_mixf(a.y, nb.y, P.airBlur), st_mixf(a.z, nb.z, P.airBlur));
+    float w = float(G.airAcc[c * 4u + 3u]) / 256.0f;
+    if (w > 0.0f) {
+      float3 vt = f3div(make_float3(float(G.airAcc[c * 4u]), float(G.airAcc[c * 4u + 1u]), float(G.airAcc[c * 4u + 2u])), w * 256.0f);
+      float t = 1.0f - st_expf(-P.airCouple * fminf(w, 8.0f) * P.dt);
+      a = make_float3(st_mixf(a.x, vt.x, t), st_mixf(a.y, vt.y, t), st_mixf(a.z, vt.z, t));
+    }
+    float3 x = make_float3(P.x0 + (float(i) + 0.5f) * P.airH, P.y0 + (float(j) + 0.5f) * P.airH, P.z0 + (float(k) + 0.5f) * P.airH);
+    for (uint32_t q = 0; q < P.ncols; q++) {
+      if (!d_colNear(P, C, q, x)) continue;
+      float phi;
+      float3 wn;
+      d_sdfEval(C, G.sdf, q, x, phi, wn);
+      if (phi < 0.6f * P.airH) {
+        a = d_colSurfVel(C, q, x);
+        break;
+      }
+    }
+    float t = 1.0f - st_expf(-P.airDecay * P.dt);
+    a = make_float3(st_mixf(a.x, P.q1, t), st_mixf(a.y, P.q2, t), st_mixf(a.z, P.q3, t));
+    G.air2[c] = make_float4(a.x, a.y, a.z, 0.0f);
+  }
+}
+__global__ void k_air_copy(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t n = P.aNx * P.aNy * P.aNz;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) G.air[c] = G.air2[c];
+}
+// wwSample (88-95): how wet each point is and the water's velocity there
+__global__ void k_ww_sample(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t used = d_wwUsed(P, G);
+  GlobalGrid gg{P, G};
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < used; j += gridDim.x * blockDim.x) {
+    float4 sp = G.sPos[j];
+    if (sp.w <= 0.0f) continue;
+    float3 p = c3(sp), vf = make_float3(0.0f, 0.0f, 0.0f);
+    int3 pc = d_particleCell(P, p);
+    if (G.tmap[d_tileOf(P, pc.x, pc.y, pc.z)] != ST_NONE) {
+      float3 vo;
+      gg.sample(p, vf, vo);
+    }
+    G.sAux[j] = make_float4(vf.x, vf.y, vf.z, d_wetAt(P, G, p));
+  }
+}
+// sprayStep (129-181): every point moves by its kind; gone when its time is up,
+// when it hits an object or leaves the box
+__global__ void k_spray_step(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t used = d_wwUsed(P, G);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < used; j += gridDim.x * blockDim.x) {
+    float4 sp = G.sPos[j];
+    if (sp.w <= 0.0f) continue;
+    float4 sv = G.sVel[j];
+    float3 v = c3(sv);
+    float kw = sv.w;
+    uint32_t kind = st_f2u_sat(kw + 0.5f);
+    float tk = (kw - float(kind)) / 0.4f * WWHOLD;
+    const bool settled = tk >= WWHOLD * 0.999f;
+    const uint32_t k0 = kind;
+    float4 aux = G.sAux[j];
+    float wet = aux.w;
+    float3 vf = c3(aux);
+    float life = sp.w;
+    uint32_t h0 = j * 2654435761u + P.seed * 974711u;
+    if (kind == 0u && v.y < vf.y && wet > 0.55f && tk > 0.05f) {
+      kind = 2u;
+      life = P.wwFoamLife * (0.6f + 0.8f * d_hash1(h0));
+      v = vf;
+    } else if (kind == 1u && wet > 0.55f) {
+      life = 0.0f;
+    } else if (kind == 2u && wet > 0.97f && settled) {
+      kind = 3u;
+    } else if (kind == 2u && wet < 0.04f && settled) {
+      kind = 0u;
+      life = fminf(life, P.sprLife);
+    } else if (kind == 3u && wet < 0.6f && settled) {
+      kind = 2u;
+      life = fmaxf(life, 0.5f * P.wwFoamLife);
+    }
+    if (kind != k0) tk = 0.0f;
+    else tk = fminf(tk + P.dt, WWHOLD);
+    float3 p = c3(sp);
+    if (kind <= 1u) {
+      const bool mist = kind == 1u;
+      float3 gust = make_float3(0.0f, 0.0f, 0.0f);
+      if (P.q7 > 0.0f) {
+        float3 gp = f3mul(f3mul(p, 0.55f), P.vs);
+        float tt = P.q8 * 0.6f;
+        gust = f3sub(f3mul(make_float3(d_vnz(f3add(gp, make_float3(tt, 0.0f, 0.0f))), d_vnz(f3add(gp, make_float3(17.3f, tt, 4.1f))),
+                                       d_vnz(f3add(gp, make_float3(5.7f, 9.2f, tt)))),
+                           2.0f),
+                     make_float3(1.0f, 1.0f, 1.0f));
+      }
+      float3 base = make_float3(P.q1, P.q2, P.q3);
+      if (P.airOn != 0u) base = d_airAt(P, G.air, p);
+      float3 airv = f3add(base, f3mul(f3mul(gust, P.q7), mist ? 2.5f : 1.0f));
+      v.y = v.y - P.gravity * P.dt;
+      float t = 1.0f - st_expf(-(mist ? P.mistDrag : P.sprDrag) * P.dt);
+      v = f3add(v, f3mul(f3sub(airv, v), t));
+      life = life - P.dt;
+    } else if (kind == 2u) {
+      float vy = vf.y + st_clampf((wet - 0.5f) * 4.0f, -1.0f, 1.0f) * P.h * 4.0f;
+      float t = 1.0f - st_expf(-10.0f * P.dt);
+      v = make_float3(st_mixf(v.x, vf.x, t), st_mixf(v.y, vy, t), st_mixf(v.z, vf.z, t));
+      life = life - P.dt;
+    } else {
+      float3 tv = f3add(vf, make_float3(0.0f, P.wwRise, 0.0f));
+      float t = 1.0f - st_expf(-6.0f * P.dt);
+      v = make_float3(st_mixf(v.x, tv.x, t), st_mixf(v.y, tv.y, t), st_mixf(v.z, tv.z, t));
+      life = life - 0.3f * P.dt;
+    }
+    float3 np = f3add(p, f3mul(v, P.dt));
+    if (np.x < P.x0 || np.x > P.x1 || np.y < P.y0 || np.y > P.y1 || np.z < P.z0 || np.z > P.z1) life = 0.0f;
+    else if (P.ncols > 0u && d_colAt(P, C, G.sdf, np) >= 0) life = 0.0f;
+    G.sPos[j] = make_float4(np.x, np.y, np.z, life);
+    G.sVel[j] = make_float4(v.x, v.y, v.z, float(kind) + 0.4f * tk / WWHOLD);
+  }
+}
+// sprayEmit (98-127), the emission test and count of each particle: the count
+// goes to sprCnt at the particle's id (scanned next), and the per-particle
+// values the emission needs are recomputed by k_spray_emit
+__device__ __forceinline__ uint32_t d_sprayCount(const DParams& P, const DGrid& G, float4 q, float4 vw, uint32_t id, float& rho,
+                                                 float& fast, float& spd) {
+  float3 v = c3(vw);
+  float foam = vw.w;
+  spd = f3len(v);
+  fast = st_clampf((spd * P.vs - P.sprMin) / (2.0f * P.sprMin), 0.0f, 1.0f);
+  int32_t bi = st_f2i_sat(floorf((q.x - P.x0) * P.ih - 0.5f)), bj = st_f2i_sat(floorf((q.y - P.y0) * P.ih - 0.5f)),
+          bk = st_f2i_sat(floorf((q.z - P.z0) * P.ih - 0.5f));
+  rho = 0.0f;
+  for (int32_t dz = 0; dz <= 1; dz++)
+    for (int32_t dy = 0; dy <= 1; dy++)
+      for (int32_t dx = 0; dx <= 1; dx++) rho = rho + d_wetCell(P, G, bi + dx, bj + dy, bk + dz);
+  rho = rho / 8.0f;
+  float crest = (1.0f - st_smoothstepf(0.3f, 0.75f, rho)) * fast * (0.3f + 0.7f * foam);
+  float churn = (1.0f - st_smoothstepf(0.8f, 1.0f, rho)) * st_clampf((foam - 0.55f) / 0.45f, 0.0f, 1.0f) *
+                st_smoothstepf(0.5f, 1.5f, spd * P.vs);
+  float e = (P.wwCrest * crest + P.wwImpact * churn) * st_smoothstepf(0.1f, 0.5f, P.q8);
+  if (e <= 0.0f) return 0u;
+  uint32_t h0 = id * 747796405u + P.seed * 2891336453u;
+  return min(st_f2u_sat(floorf(P.sprRate * e * P.dt + d_hash1(h0))), 4u);
+}
+__global__ void k_spray_count(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+    float4 q = G.pos[i];
+    uint32_t id = G.pid[i], k = 0u;
+    if (q.y >= ST_DEADY) {
+      float rho, fast, spd;
+      k = d_sprayCount(P, G, q, G.vel[i], id, rho, fast, spd);
+    }
+    G.sprCnt[id] = k;
+  }
+}
+__global__ void k_spray_emit(const DParams* __restrict__ PP, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t first = G.sctl[0];  // k_spray_advance moves it on after this kernel
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+    float4 q = G.pos[i];
+    if (q.y < ST_DEADY) continue;
+    uint32_t id = G.pid[i];
+    float rho, fast, spd;
+    float4 vw = G.vel[i];
+    uint32_t k = d_sprayCount(P, G, q, vw, id, rho, fast, spd);
+    if (!k) continue;
+    float3 v = c3(vw);
+    uint32_t h0 = id * 747796405u + P.seed * 2891336453u, e0 = G.sprOff[id];
+    for (uint32_t m = 0; m < k; m++) {
+      // the e-th point thrown off this substep takes slot (first + e) mod smax;
+      // past smax points in one substep the ring has come round to slots this
+      // substep already filled, which hold living points
+      if (e0 + m >= P.smax) break;
+      uint32_t hh = h0 ^ ((m + 1u) * 0x9E3779B9u);
+      float z = d_hash1(hh + 1u) * 2.0f - 1.0f, a = d_hash1(hh + 2u) * 6.2831853f, s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+      float sa, ca;
+      st_sincosf(a, &sa, &ca);
+      float3 dir = make_float3(s * ca, s * sa, z);
+      bool under = rho > 0.7f && d_hash1(hh + 7u) < 0.7f;
+      bool mist = !under && d_hash1(hh + 3u) < P.mistFrac * fast;
+      float3 nv = f3add(f3mul(v, 0.9f + 0.2f * d_hash1(hh + 4u)), f3mul(f3mul(dir, spd), mist ? 0.2f : 0.06f));
+      float life = P.sprLife * (0.6f + 0.8f * d_hash1(hh + 5u)) * (mist ? 1.6f : 1.0f);
+      float kind = mist ? 1.0f : 0.0f;
+      if (under) {
+        nv = f3add(v, f3mul(f3mul(dir, spd), 0.05f));
+        life = P.wwFoamLife * (0.6f + 0.8f * d_hash1(hh + 5u));
+        kind = 3.0f;
+      }
+      uint32_t slot = (first + e0 + m) % P.smax;
+      if (G.sPos[slot].w > 0.0f) continue;
+      float3 np = f3add(f3add(c3(q), f3mul(nv, P.dt)), f3mul(f3mul(f3mul(dir, P.h), 0.3f), d_hash1(hh + 6u)));
+      G.sPos[slot] = make_float4(np.x, np.y, np.z, life);
+      G.sVel[slot] = make_float4(nv.x, nv.y, nv.z, kind);
+    }
+  }
+}
+// the ring counter moves on by every point asked for, placed or not
+__global__ void k_spray_advance(const DParams* __restrict__ PP, DGrid G) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) G.sctl[0] = G.sctl[0] + G.sprOff[PP->n];
+}
+
+void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit) {
+  const uint32_t pg = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.n, WG), 8192u));
+  const uint32_t sg = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.smax, WG), 8192u));
+  if (air && airN) {
+    launch_fill_u32(st, (uint32_t*)L.G.airAcc, size_t(airN) * 4u, 0u);
+    const uint32_t ag = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(airN, WG), 4096u));
+    if (L.n) hipLaunchKernelGGL(k_air_splat_w, dim3(pg), dim3(WG), 0, st, L.P, L.G);
+    hipLaunchKernelGGL(k_air_splat_s, dim3(sg), dim3(WG), 0, st, L.P, L.G);
+    hipLaunchKernelGGL(k_air_update, dim3(ag), dim3(WG), 0, st, L.P, L.C, L.G);
+    hipLaunchKernelGGL(k_air_copy, dim3(ag), dim3(WG), 0, st, L.P, L.G);
+  }
+  hipLaunchKernelGGL(k_ww_sample, dim3(sg), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_spray_step, dim3(sg), dim3(WG), 0, st, L.P, L.C, L.G);
+  if (emit && L.n) {
+    hipLaunchKernelGGL(k_spray_count, dim3(pg), dim3(WG), 0, st, L.P, L.G);
+    launch_scan(st, L.G.sprCnt, L.G.sprOff, L.n, L.G.sprSums, nullptr);
+    hipLaunchKernelGGL(k_spray_emit, dim3(pg), dim3(WG), 0, st, L.P, L.G);
+    hipLaunchKernelGGL(k_spray_advance, dim3(1), dim3(64), 0, st, L.P, L.G);
+  }
+}

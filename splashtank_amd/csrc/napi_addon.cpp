@@ -208,6 +208,24 @@ void readParams(napi_env env, napi_value o, st_params* p) {
   p->nb_init = uint32_t(numProp(env, o, "nbInit", 0));
   p->nfill = uint32_t(numProp(env, o, "nfill", 0));
   p->nfv = uint32_t(numProp(env, o, "nfv", 0));
+  p->sor_mode = uint32_t(numProp(env, o, "sorMode", 0));
+  // whitewater and air (o.spr, o.spr.ww, o.air, flattened by the facade)
+  p->spr_on = uint32_t(numProp(env, o, "sprOn", 0));
+  p->spr_rate = float(numProp(env, o, "sprRate", 0));
+  p->spr_min = float(numProp(env, o, "sprMin", 0));
+  p->spr_life = float(numProp(env, o, "sprLife", 0));
+  p->spr_mist = float(numProp(env, o, "sprMist", 0));
+  p->spr_drag = float(numProp(env, o, "sprDrag", 0));
+  p->spr_mdrag = float(numProp(env, o, "sprMdrag", 0));
+  p->vs = float(numProp(env, o, "vs", 0));
+  p->ww_foam_life = float(numProp(env, o, "wwFoamLife", 0));
+  p->ww_rise = float(numProp(env, o, "wwRise", 0));
+  p->ww_crest = float(numProp(env, o, "wwCrest", p->ww_crest));
+  p->ww_impact = float(numProp(env, o, "wwImpact", p->ww_impact));
+  p->air_on = uint32_t(numProp(env, o, "airOn", 0));
+  p->air_couple = float(numProp(env, o, "airCouple", 0));
+  p->air_decay = float(numProp(env, o, "airDecay", 0));
+  p->air_blur = float(numProp(env, o, "airBlur", p->air_blur));
   bool has = false;
   napi_has_named_property(env, o, "wind", &has);
   if (has) {
@@ -322,6 +340,45 @@ napi_value MemBytes(napi_env env, napi_callback_info info) {
   return mkNum(env, double(b));
 }
 
+// setSprayCap(h, n)
+napi_value SetSprayCap(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t n = 0;
+  napi_get_value_uint32(env, argv[1], &n);
+  ST_OK_OR_THROW(st_set_spray_cap(handleOf(env, argv[0]), n));
+  return nullptr;
+}
+
+// readSpray(h, room) -> {n, p: Float32Array(4n), v: Float32Array(4n)} or null without a pool
+napi_value ReadSpray(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t room = 0;
+  napi_get_value_uint32(env, argv[1], &room);
+  std::vector<float> p(size_t(std::max(room, 1u)) * 4), v(size_t(std::max(room, 1u)) * 4);
+  uint32_t n = 0;
+  ST_OK_OR_THROW(st_read_spray(handleOf(env, argv[0]), p.data(), v.data(), room, &n));
+  napi_value out;
+  if (!n) {
+    napi_get_null(env, &out);
+    return out;
+  }
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "n", mkNum(env, double(n)));
+  const char* names[2] = {"p", "v"};
+  float* src[2] = {p.data(), v.data()};
+  for (int q = 0; q < 2; q++) {
+    void* data = nullptr;
+    napi_value ab, ta;
+    napi_create_arraybuffer(env, size_t(n) * 16, &data, &ab);
+    std::memcpy(data, src[q], size_t(n) * 16);
+    napi_create_typedarray(env, napi_float32_array, size_t(n) * 4, ab, 0, &ta);
+    napi_set_named_property(env, out, names[q], ta);
+  }
+  return out;
+}
+
 // solveNorms(h) -> {resL2, resMax, rhsL2, divL2, divMax, errL2, errMax, fluid} (st_solve_norms)
 napi_value SolveNorms(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -401,7 +458,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;

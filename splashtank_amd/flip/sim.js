@@ -81,6 +81,7 @@ class FlipSim {
     this.size = [W, H, Dd];
     this.open = opt.open | 0;
     this.MAXN = opt.maxN; this.cap = opt.maxN; this.N = 0; this.ring = 0; this.recycle = true;
+    this.sprayCap = 0;
     this.lastMs = 0; this.slowMs = 0; this.inflight = 0; this.out = opt.out || null; this.outN = 0;
     this.ncell = D.ncell; this.tx = D.tx; this.ty = D.ty; this.tz = D.tz; this.ntiles = D.ntiles;
     this.scap = D.scap; this.scapMax = D.scapMax;
@@ -148,7 +149,20 @@ class FlipSim {
       foamTau: o.foamTau || 1.2, foamSens: o.foamSens || 1, gust: o.gust || 0,
       noFoam: this.opt.noFoam, noCull: this.opt.noCull, fastAir: this.opt.fastAir, noSolidCache: this.opt.noSolidCache,
       nbOn, nbBand: Math.max(2, (nbc && nbc.band) || 3), onlyTiles: this.only === 'tiles', nbInit: init, nfill: fl.length,
-      nfv: init ? this.nbFillVols.length : 0, fills };
+      nfv: init ? this.nbFillVols.length : 0, fills, sorMode: this.opt.sorMode || 0, ...this.sprayParams(o) };
+  }
+  // o.spr, o.spr.ww, o.air (sim.js:203-206, 221-230), flattened for the C ABI; the C side applies the
+  // reference's defaults and its "on only with room for points" rule
+  sprayParams(o) {
+    const sp = o.spr || null, ww = (sp && sp.ww) || {}, air = o.air || null;
+    const out = { vs: o.vs || 1 };
+    if (sp) {
+      Object.assign(out, { sprOn: 1, sprRate: sp.rate, sprMin: sp.min, sprLife: sp.life, sprMist: sp.mist, sprDrag: sp.drag, sprMdrag: sp.mdrag,
+        wwFoamLife: ww.foamLife || 0, wwRise: ww.rise || 0, wwCrest: ww.crest === undefined ? 1 : ww.crest,
+        wwImpact: ww.impact === undefined ? 0.5 : ww.impact });
+    }
+    if (air && air.on) Object.assign(out, { airOn: 1, airCouple: air.couple, airDecay: air.decay, airBlur: air.blur === undefined ? 0.35 : air.blur });
+    return out;
   }
   prepare(dt, o) {
     this.flushRun();
@@ -177,7 +191,7 @@ class FlipSim {
   stages() {
     const a = loadAddon(), h = this.handle;
     const names = ['blocks', 'clear', 'p2g'].concat(this.nbOn ? ['narrow band'] : [],
-      ['classify', 'normFaces', 'divergence', 'pressure', 'project', 'g2p'], this.nbOn ? ['reseed'] : []);
+      ['classify', 'normFaces', 'divergence', 'pressure', 'project', 'g2p'], this.sprayCap > 0 ? ['spray'] : [], this.nbOn ? ['reseed'] : []);
     if (this.only === 'tiles') return [{ name: 'blocks', fn: () => a.stage(h, STAGE_IDS.blocks) }];
     return names.map((name) => ({ name, fn: () => a.stage(h, STAGE_IDS[name]) }));
   }
@@ -265,8 +279,10 @@ class FlipSim {
   }
   destroy() { if (this.handle) { loadAddon().destroy(this.handle); this.handle = null; } this.dead = true; }
   // ---- outside the substep: whitewater and the live view ----
-  setSprayCap() {}
-  async readSpray() { return null; }
+  // room for n whitewater points, 0 turns the spray stage off (sim.js:333-340)
+  setSprayCap(n) { n = Math.max(0, n | 0); loadAddon().setSprayCap(this.handle, n); this.sprayCap = n; }
+  // every whitewater slot handed out so far: positions with life in w, velocities with kind in w (sim.js:423-437)
+  async readSpray() { return this.sprayCap ? loadAddon().readSpray(this.handle, this.sprayCap) : null; }
   setDeepLook() {}
   async readDeep() { return null; }
   pump() {}
