@@ -248,6 +248,22 @@ int st_set_spray_cap(void* sim, uint32_t n);
  * Writes at most `room` points; blocks. */
 int st_read_spray(void* sim, float* pos4, float* vel4, uint32_t room, uint32_t* n);
 
+/* buildSDF(mesh, cell, margin) (src/shared.js:98-181): the signed distance grid
+ * of a closed triangle mesh, the collider grids st_set_colliders takes.  v holds
+ * nv vertices (x, y, z doubles), tri nt triangles (vertex indices; the
+ * reference's meshTris fan triangulation of its faces).  cell <= 0 -> 0.025,
+ * margin < 0 -> 0.16; the cell grows by 1.25 until the grid has at most 2.6e6
+ * points.  Fills *grid (dims, origin = mesh bounds minus the margin, cell); with
+ * data != NULL also the grid (x fastest, negative inside), which needs cap >=
+ * dims[0]*dims[1]*dims[2] floats.  Host code: touches no GPU. */
+typedef struct st_sdf_grid {
+  int32_t dims[3];
+  double origin[3];
+  double cell;
+} st_sdf_grid;
+int st_build_sdf(const double* v, uint32_t nv, const uint32_t* tri, uint32_t nt, double cell, double margin, st_sdf_grid* grid,
+                 float* data, size_t cap);
+
 /* FP64 norms of the last substep's pressure solve and projection, over its fluid
  * cells, computed on the GPU in f64 from the f32 state the substep left (the
  * only f64 arithmetic on the GPU).  No counterpart in the reference, which

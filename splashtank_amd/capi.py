@@ -62,6 +62,29 @@ class StParams(C.Structure):
                 ("reserved", C.c_uint32 * 3)]
 
 
+class StSdfGrid(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_double * 3), ("cell", C.c_double)]
+
+
+def build_sdf(verts, tris, cell=0.025, margin=0.16):
+    """buildSDF (src/shared.js:98-181) through st_build_sdf: verts (n, 3) doubles,
+    tris (m, 3) vertex indices -> dict(dims, origin, cell, data (f32, x fastest))."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    g = StSdfGrid()
+    rc = lib.st_build_sdf(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0], float(cell), float(margin), C.byref(g), None, 0)
+    if rc != 0:
+        raise StError(f"st_build_sdf -> {rc}")
+    n = g.dims[0] * g.dims[1] * g.dims[2]
+    data = np.zeros(n, np.float32)
+    rc = lib.st_build_sdf(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0], float(cell), float(margin), C.byref(g),
+                          data.ctypes.data, n)
+    if rc != 0:
+        raise StError(f"st_build_sdf -> {rc}")
+    return {"dims": list(g.dims), "origin": list(g.origin), "cell": g.cell, "data": data}
+
+
 class StNorms(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("res_l2", "res_max", "rhs_l2", "div_l2", "div_max",
                                           "err_l2", "err_max", "fluid")]
@@ -151,6 +174,8 @@ PROTOS = {
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
     "st_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_build_sdf": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.POINTER(StSdfGrid),
+                               C.c_void_p, C.c_size_t]),
     "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),

@@ -340,6 +340,42 @@ napi_value MemBytes(napi_env env, napi_callback_info info) {
   return mkNum(env, double(b));
 }
 
+// buildSdf(Float64Array v (3 per vertex), Uint32Array tris (3 per triangle), cell, margin)
+//   -> {dims: [nx, ny, nz], origin: [x, y, z], cell, data: Float32Array}
+napi_value BuildSdf(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  getArgs(env, info, 4, argv);
+  void *v = nullptr, *t = nullptr;
+  size_t nv = 0, nt = 0;
+  if (!typed(env, argv[0], &v, &nv, napi_float64_array) || !typed(env, argv[1], &t, &nt, napi_uint32_array))
+    return throwErr(env, "buildSdf: vertices must be a Float64Array and triangles a Uint32Array");
+  double cell = 0.025, margin = 0.16;
+  napi_get_value_double(env, argv[2], &cell);
+  napi_get_value_double(env, argv[3], &margin);
+  st_sdf_grid g;
+  if (st_build_sdf((const double*)v, uint32_t(nv / 3), (const uint32_t*)t, uint32_t(nt / 3), cell, margin, &g, nullptr, 0) != ST_OK)
+    return throwErr(env, "buildSdf: bad mesh");
+  const size_t n = size_t(g.dims[0]) * g.dims[1] * g.dims[2];
+  void* data = nullptr;
+  napi_value ab, ta, out, dims, org;
+  napi_create_arraybuffer(env, n * 4, &data, &ab);
+  if (st_build_sdf((const double*)v, uint32_t(nv / 3), (const uint32_t*)t, uint32_t(nt / 3), cell, margin, &g, (float*)data, n) != ST_OK)
+    return throwErr(env, "buildSdf: failed");
+  napi_create_typedarray(env, napi_float32_array, n, ab, 0, &ta);
+  napi_create_object(env, &out);
+  napi_create_array_with_length(env, 3, &dims);
+  napi_create_array_with_length(env, 3, &org);
+  for (uint32_t k = 0; k < 3; k++) {
+    napi_set_element(env, dims, k, mkNum(env, double(g.dims[k])));
+    napi_set_element(env, org, k, mkNum(env, g.origin[k]));
+  }
+  napi_set_named_property(env, out, "dims", dims);
+  napi_set_named_property(env, out, "origin", org);
+  napi_set_named_property(env, out, "cell", mkNum(env, g.cell));
+  napi_set_named_property(env, out, "data", ta);
+  return out;
+}
+
 // setSprayCap(h, n)
 napi_value SetSprayCap(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -458,7 +494,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;
