@@ -116,7 +116,9 @@ typedef struct st_params {
   float air_couple;     /* o.air.couple, 1/s */
   float air_decay;      /* o.air.decay, 1/s */
   float air_blur;       /* o.air.blur (0.35) */
-  uint32_t reserved[3];
+  uint32_t drip_on;     /* o.drip.on: wet colliders drip (with spr_on and drip samples set) */
+  float drip_rate;      /* o.drip.rate: drops per second per sample when fully wet */
+  float drip_dry;       /* o.drip.dry: seconds a sample takes to dry */
 } st_params;
 
 #define ST_SOR_TUNED 3u
@@ -241,12 +243,37 @@ int st_sor_path(void* sim, uint32_t* path);
  * spray stage off).  The new pool holds no points; the ring counter of handed
  * out slots carries on. */
 int st_set_spray_cap(void* sim, uint32_t n);
+/* dripUpdate's upload (sim.js:353-363): n drip samples of 8 floats each -- the
+ * local position on a collider, the collider's number, the local normal, the
+ * wetness (the reference's dripSamples, src/flip/bake.js:19-40, gives them dry)
+ * -- replacing the ones before (n = 0 removes them).  The spray stage lets the
+ * samples drip when st_params.drip_on. */
+int st_set_drips(void* sim, const float* samples, uint32_t n);
 /* readSpray (sim.js:423-437): the first n = max(1, min(cap, slots handed out))
  * whitewater slots, positions with the seconds of life left in w and
  * velocities with the kind in w (0 spray, 1 mist, 2 foam, 3 bubble; the
  * fraction is how long the point has had that kind); *n = 0 without a pool.
  * Writes at most `room` points; blocks. */
 int st_read_spray(void* sim, float* pos4, float* vel4, uint32_t room, uint32_t* n);
+
+/* The live view (dispPack + pump, sim.js:364-404, 504-537; packDisp and
+ * sprayDisp, wgsl-whitewater.js:183-253): a sample of the water (every stride-th
+ * particle in id order, stride = max(1, ceil(N / disp_max)), disp_max 0 ->
+ * 300000) and of the whitewater (up to min(pool, 200000) sample slots, the
+ * living ones packed together in slot order) packed on the GPU into 32-bit
+ * words in the cache frame format: a 4-word header (whitewater points, deep
+ * points = 0, 0, 0), then the parts at the layout's word offsets.  With out ==
+ * NULL only the layout is filled; else out must hold layout->words words.
+ * Deep-water look points (setDeepLook) are not built (nD = 0).  Blocks. */
+typedef struct st_view_layout {
+  uint32_t nW, nS, nD;          /* water points, whitewater capacity, deep points */
+  uint32_t words;               /* words of the packed view */
+  uint32_t oWp, oWs, oWf;       /* water: positions (6 words per 4 points), speed bytes, white-water bytes */
+  uint32_t oSp, oSk;            /* whitewater: positions, 32-bit codes */
+  uint32_t oDp, oDs;            /* deep water: positions, speed bytes */
+  uint32_t stride, from_pos;    /* water sample stride; 1 when the water is read straight from the particles */
+} st_view_layout;
+int st_pack_view(void* sim, uint32_t disp_max, uint32_t* out, size_t cap_words, st_view_layout* layout);
 
 /* buildSDF(mesh, cell, margin) (src/shared.js:98-181): the signed distance grid
  * of a closed triangle mesh, the collider grids st_set_colliders takes.  v holds

@@ -59,7 +59,7 @@ class StParams(C.Structure):
                 ("spr_mist", C.c_float), ("spr_drag", C.c_float), ("spr_mdrag", C.c_float), ("vs", C.c_float),
                 ("ww_foam_life", C.c_float), ("ww_rise", C.c_float), ("ww_crest", C.c_float), ("ww_impact", C.c_float),
                 ("air_on", C.c_uint32), ("air_couple", C.c_float), ("air_decay", C.c_float), ("air_blur", C.c_float),
-                ("reserved", C.c_uint32 * 3)]
+                ("drip_on", C.c_uint32), ("drip_rate", C.c_float), ("drip_dry", C.c_float)]
 
 
 class StSdfGrid(C.Structure):
@@ -83,6 +83,14 @@ def build_sdf(verts, tris, cell=0.025, margin=0.16):
     if rc != 0:
         raise StError(f"st_build_sdf -> {rc}")
     return {"dims": list(g.dims), "origin": list(g.origin), "cell": g.cell, "data": data}
+
+
+class StViewLayout(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("nW", "nS", "nD", "words", "oWp", "oWs", "oWf", "oSp", "oSk", "oDp", "oDs",
+                                          "stride", "from_pos")]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 class StNorms(C.Structure):
@@ -174,6 +182,8 @@ PROTOS = {
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
     "st_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_set_drips": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "st_pack_view": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(StViewLayout)]),
     "st_build_sdf": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.POINTER(StSdfGrid),
                                C.c_void_p, C.c_size_t]),
     "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -351,6 +361,11 @@ class FlipSimC(_Base):
         """setSprayCap: room for n whitewater points (0 turns the spray stage off)."""
         self._call("set_spray_cap", self.h, int(n))
 
+    def set_drips(self, samples):
+        """dripUpdate: drip samples, 8 floats each (scenes.drip_samples gives them)"""
+        a = np.ascontiguousarray(samples, np.float32).reshape(-1, 8)
+        self._call("set_drips", self.h, a.ctypes.data if a.size else None, a.shape[0])
+
     def read_spray(self, room=1 << 22):
         """readSpray: (positions with life in w, velocities with kind in w) of the slots handed out."""
         p = np.zeros((room, 4), np.float32)
@@ -358,6 +373,14 @@ class FlipSimC(_Base):
         n = C.c_uint32(0)
         self._call("read_spray", self.h, p.ctypes.data, v.ctypes.data, room, C.byref(n))
         return p[:n.value], v[:n.value]
+
+    def pack_view(self, disp_max=0):
+        """the live view packed on the GPU (st_pack_view): (layout dict, uint32 words)"""
+        L = StViewLayout()
+        self._call("pack_view", self.h, int(disp_max), None, 0, C.byref(L))
+        out = np.zeros(max(L.words, 1), np.uint32)
+        self._call("pack_view", self.h, int(disp_max), out.ctypes.data, out.size, C.byref(L))
+        return L.as_dict(), out[:L.words]
 
     def solve_norms(self):
         """FP64 norms of the last substep's solve and projection (st_solve_norms), as a dict."""

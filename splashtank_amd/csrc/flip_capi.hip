@@ -117,6 +117,14 @@ struct Sim {
   uint32_t airN = 0, aNx = 1, aNy = 1, aNz = 1;  // air field cells (0: not made yet)
   double airH = 1.0;
   bool airOnStep = false, sprOnStep = false;
+  uint32_t ndrip = 0, dripCap = 0;  // drip samples set, room for them
+  bool dripOnStep = false;
+  // live view (st_pack_view): packed words, sample flags, their scan, the sampled whitewater
+  uint32_t* viewOut = nullptr;
+  uint32_t *viewFlag = nullptr, *viewOff = nullptr, *viewSums = nullptr;
+  float4* viewDisp = nullptr;
+  size_t viewWords = 0;
+  uint32_t viewSd = 0;
   double* normPart = nullptr;     // st_solve_norms: per-workgroup partials, then the result
   uint32_t normCap = 0;           // workgroups normPart has room for
   hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
@@ -227,6 +235,8 @@ void destroySim(Sim* s) {
   dfree(s->normPart);
   dfree(s->G.sPos); dfree(s->G.sVel); dfree(s->G.sAux); dfree(s->G.sctl); dfree(s->G.sprCnt); dfree(s->G.sprOff); dfree(s->G.sprSums);
   dfree(s->G.air); dfree(s->G.air2); dfree(s->G.airAcc);
+  dfree(s->G.drip); dfree(s->G.dripFlag); dfree(s->G.dripOff); dfree(s->G.dripSums);
+  dfree(s->viewOut); dfree(s->viewFlag); dfree(s->viewOff); dfree(s->viewSums); dfree(s->viewDisp);
   freeSlots(s);
   dfree(s->dSdf);
   dfree(s->dSlots);
@@ -349,8 +359,13 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
     P.aNz = s->airN ? s->aNz : 1u;
     P.airBlur = airo ? o->air_blur : 0.0f;
     P.sprOn = on ? 1u : 0u;
+    const bool dr = on && o->drip_on != 0;
+    P.ndrip = dr ? s->ndrip : 0u;
+    P.dripRate = dr ? o->drip_rate : 0.0f;
+    P.dripDry = dr ? o->drip_dry : 1.0f;
     s->airOnStep = P.airOn != 0u;
     s->sprOnStep = on;
+    s->dripOnStep = dr && s->ndrip > 0;
   }
   for (int k = 0; k < 16; k++)
     for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
@@ -434,7 +449,7 @@ int runStage(Sim* s, int stage) {
       }
       break;
     case ST_STAGE_SPRAY:
-      if (full && s->sprayCap) launch_spray(st, L, s->airN, s->airOnStep, s->sprOnStep);
+      if (full && s->sprayCap) launch_spray(st, L, s->airN, s->airOnStep, s->sprOnStep, s->dripOnStep ? s->ndrip : 0u);
       break;
     case ST_STAGE_RESEED:
       if (full && s->nbOn) {
@@ -1117,6 +1132,27 @@ int st_set_spray_cap(void* p, uint32_t n) {
   return ST_OK;
 }
 
+int st_set_drips(void* p, const float* samples, uint32_t n) {
+  Sim* s = getSim(p);
+  if (!s || (n && !samples)) return fail(ST_ERR_ARG, "st_set_drips: null argument");
+  HIPCHK(hipSetDevice(s->dev));
+  if (n > s->dripCap) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dropGraphs(s);
+    dfree(s->G.drip); dfree(s->G.dripFlag); dfree(s->G.dripOff); dfree(s->G.dripSums);
+    s->dripCap = 0;
+    int rc;
+    if ((rc = dalloc(&s->G.drip, size_t(n) * 2)) || (rc = dalloc(&s->G.dripFlag, n)) || (rc = dalloc(&s->G.dripOff, size_t(n) + 1)) ||
+        (rc = dalloc(&s->G.dripSums, size_t(n) / 1024 + 2)))
+      return rc;
+    s->dripCap = n;
+  }
+  if (n) HIPCHK(hipMemcpyAsync(s->G.drip, samples, size_t(n) * 32, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->ndrip = n;
+  return ST_OK;
+}
+
 int st_read_spray(void* p, float* pos4, float* vel4, uint32_t room, uint32_t* n) {
   Sim* s = getSim(p);
   if (!s || !n) return fail(ST_ERR_ARG, "st_read_spray: null argument");
@@ -1132,6 +1168,63 @@ int st_read_spray(void* p, float* pos4, float* vel4, uint32_t room, uint32_t* n)
   HIPCHK(hipMemcpyAsync(vel4, s->G.sVel, size_t(k) * 16, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   *n = k;
+  return ST_OK;
+}
+
+int st_pack_view(void* p, uint32_t disp_max, uint32_t* out, size_t cap_words, st_view_layout* Lo) {
+  Sim* s = getSim(p);
+  if (!s || !Lo) return fail(ST_ERR_ARG, "st_pack_view: null argument");
+  const uint32_t dispMax = disp_max ? disp_max : 300000u, N = s->N;
+  const uint32_t stride = std::max<uint32_t>(1u, uint32_t((uint64_t(N) + dispMax - 1) / dispMax));
+  const uint32_t sdMax = std::min<uint32_t>(s->sprayCap, 200000u);
+  const bool spr = sdMax > 0 && N > 0, fromPos = !spr && (stride <= 1 || !N);
+  const uint32_t nW = fromPos ? std::min<uint32_t>(N, std::min<uint32_t>(dispMax + 1, s->maxn)) : uint32_t((uint64_t(N) + stride - 1) / stride);
+  const uint32_t nS = spr ? sdMax : 0u;
+  // packLayout (sim.js:376-380)
+  const uint32_t gW = (nW + 3u) / 4u, gS = (nS + 3u) / 4u, gD = 0u;
+  st_view_layout L{};
+  L.nW = nW; L.nS = nS; L.nD = 0;
+  L.oWp = 4; L.oWs = L.oWp + gW * 6u; L.oWf = L.oWs + gW; L.oSp = L.oWf + gW; L.oSk = L.oSp + gS * 6u;
+  L.oDp = L.oSk + gS * 4u; L.oDs = L.oDp + gD * 6u; L.words = L.oDs + gD;
+  L.stride = fromPos ? 1u : stride;
+  L.from_pos = fromPos ? 1u : 0u;
+  *Lo = L;
+  if (!out) return ST_OK;
+  if (cap_words < L.words) return fail(ST_ERR_ARG, "st_pack_view: output shorter than layout.words");
+  if (spr && !s->cur) return fail(ST_ERR_STATE, "st_pack_view: the whitewater sample needs a substep of this sim first");
+  HIPCHK(hipSetDevice(s->dev));
+  int rc;
+  if (s->viewWords < L.words) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->viewOut);
+    s->viewWords = 0;
+    if ((rc = dalloc(&s->viewOut, L.words))) return rc;
+    s->viewWords = L.words;
+  }
+  if (!s->viewOff || s->viewSd < sdMax) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->viewFlag); dfree(s->viewOff); dfree(s->viewSums); dfree(s->viewDisp);
+    s->viewSd = 0;
+    if ((rc = dalloc(&s->viewFlag, std::max(sdMax, 1u))) || (rc = dalloc(&s->viewOff, size_t(sdMax) + 1)) ||
+        (rc = dalloc(&s->viewSums, size_t(sdMax) / 1024 + 2)) || (rc = dalloc(&s->viewDisp, std::max(sdMax, 1u))))
+      return rc;
+    s->viewSd = sdMax;
+  }
+  ViewParams V{};
+  V.x0 = s->x0; V.y0 = s->y0; V.z0 = s->z0;
+  V.kx = float(65535.0 / (double(s->x1) - double(s->x0)));
+  V.ky = float(65535.0 / (double(s->y1) - double(s->y0)));
+  V.kz = float(65535.0 / (double(s->z1) - double(s->z0)));
+  V.vs = s->hp.vs != 0.0f ? s->hp.vs : 1.0f;
+  V.foamLife = s->hp.wwFoamLife;
+  V.nW = nW; V.nS = nS; V.srcStride = L.stride; V.fromPos = L.from_pos;
+  V.oWp = L.oWp; V.oWs = L.oWs; V.oWf = L.oWf; V.oSp = L.oSp; V.oSk = L.oSk;
+  V.n = N;
+  LaunchCtx C = s->ctx();
+  launch_pack_view(s->st, C, V, spr ? sdMax : 0u, s->viewFlag, s->viewOff, s->viewSums, s->viewDisp, s->viewOut);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->viewOut, size_t(L.words) * 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
   return ST_OK;
 }
 

@@ -43,7 +43,8 @@ struct DParams {
   uint32_t aNx, aNy, aNz;
   float airBlur;
   uint32_t sprOn;     // new points are thrown off this step
-  uint32_t pad2, pad3, pad4;
+  uint32_t ndrip;     // drip samples on the colliders (0: no drips this step)
+  float dripRate, dripDry;
 };
 
 // Collider records: 8 float4 per collider (gpu.js:3-4): (sdf offset, nx, ny, nz),
@@ -113,6 +114,12 @@ struct DGrid {
   float4* air;           // air field cell velocities, and the next step's
   float4* air2;
   int32_t* airAcc;       // per air cell: fixed-point (x 256) velocity and weight pushed into it
+  // drips (dripUpdate, sim.js:353-363): per sample two float4 -- local position and collider number,
+  // local normal and wetness -- and the flags / scan that place this step's drops
+  float4* drip;
+  uint32_t* dripFlag;
+  uint32_t* dripOff;
+  uint32_t* dripSums;
 };
 
 // ---------------------------------------------------------------------------

@@ -106,9 +106,22 @@ void morton_tiles(uint32_t tx, uint32_t ty, uint32_t tz, uint32_t* out);
 void launch_project(hipStream_t st, const LaunchCtx& L);
 void launch_g2p(hipStream_t st, const LaunchCtx& L);
 void launch_unsort(hipStream_t st, const float4* src, const uint32_t* pid, float4* out, uint32_t n);
+// the live view: what one packing needs (positions quantised to 16 bits across the box)
+struct ViewParams {
+  float x0, y0, z0, kx, ky, kz, vs, foamLife;  // kx = 65535 / (x1 - x0), ...
+  uint32_t nW, nS, srcStride, fromPos;          // water points, whitewater capacity, water stride
+  uint32_t oWp, oWs, oWf, oSp, oSk;             // word offsets of the parts
+  uint32_t n;                                   // particles
+};
+// gathers the water (id order, every V.srcStride-th) and whitewater (sdMax sample slots,
+// living ones compacted) samples and packs them into `out` (V's word offsets); flag and
+// off hold sdMax (+1) words, sums sdMax / 1024 + 2, sdisp sdMax points
+void launch_pack_view(hipStream_t st, const LaunchCtx& L, const ViewParams& V, uint32_t sdMax, uint32_t* flag, uint32_t* off,
+                      uint32_t* sums, float4* sdisp, uint32_t* out);
 // the spray stage (wgsl-whitewater.js): the air field when `air` (airN cells),
-// then every whitewater point, then new points when `emit`
-void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit);
+// then every whitewater point, then new points when `emit`, then the drops of
+// ndrip drip samples (0: none)
+void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit, uint32_t ndrip);
 // FP64 norms of the last pressure solve and projection (flip_norms.hip): part
 // holds L.cellGrid x ST_NORM_WORDS doubles, out ST_NORM_WORDS (the st_norms fields)
 #define ST_NORM_WORDS 8

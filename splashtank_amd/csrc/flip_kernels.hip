@@ -2781,7 +2781,68 @@ __global__ void k_spray_advance(const DParams* __restrict__ PP, DGrid G) {
   if (threadIdx.x == 0 && blockIdx.x == 0) G.sctl[0] = G.sctl[0] + G.sprOff[PP->n];
 }
 
-void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit) {
+// dripStep (wgsl-whitewater.js:256-282): wet objects drip.  A sample over a
+// collider's surface is wet while water is at it and dries over dripDry seconds
+// once out; while drying it lets go of drops (more from surfaces facing down),
+// which fly as spray with the object's velocity.  The wetness is updated and
+// each sample's drop decided here; the drops take consecutive ring slots in
+// sample order (k_drip_emit).
+__device__ __forceinline__ void d_dripGeom(const DCols* __restrict__ C, float4 a, float4 b, uint32_t k, const DParams& P, float3& wp,
+                                           float3& wn, float3& probe) {
+  float3 r0 = c3(C->c[k * 8u + 2u]), r1 = c3(C->c[k * 8u + 3u]), r2 = c3(C->c[k * 8u + 4u]);
+  wp = f3add(f3add(f3add(c3(C->c[k * 8u + 5u]), f3mul(r0, a.x)), f3mul(r1, a.y)), f3mul(r2, a.z));
+  float3 n = f3add(f3add(f3mul(r0, b.x), f3mul(r1, b.y)), f3mul(r2, b.z));
+  wn = f3div(n, f3len(n));
+  probe = f3add(wp, f3mul(f3mul(wn, P.h), 0.75f));
+}
+__global__ void k_drip_flags(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.ndrip; s += gridDim.x * blockDim.x) {
+    G.dripFlag[s] = 0u;
+    float4 a = G.drip[s * 2u], b = G.drip[s * 2u + 1u];
+    uint32_t k = st_f2u_sat(a.w);
+    if (k >= P.ncols) continue;
+    float3 wp, wn, probe;
+    d_dripGeom(C, a, b, k, P, wp, wn, probe);
+    float here = d_wetAt(P, G, probe);
+    if (here > 0.4f) {
+      G.drip[s * 2u + 1u].w = 1.0f;
+      continue;
+    }
+    float wetness = fmaxf(0.0f, b.w - P.dt / fmaxf(P.dripDry, 0.05f));
+    G.drip[s * 2u + 1u].w = wetness;
+    if (wetness <= 0.0f || here > 0.1f || P.smax == 0u || P.dripRate <= 0.0f) continue;
+    float down = st_clampf(-wn.y, 0.0f, 1.0f);
+    float rate = P.dripRate * wetness * wetness * (0.15f + 0.85f * down);
+    uint32_t h0 = s * 747796405u + P.seed * 2891336453u + 17u;
+    if (d_hash1(h0) >= rate * P.dt) continue;
+    G.dripFlag[s] = 1u;
+  }
+}
+__global__ void k_drip_emit(const DParams* __restrict__ PP, const DCols* __restrict__ C, DGrid G) {
+  const DParams& P = *PP;
+  const uint32_t first = G.sctl[0];
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P.ndrip; s += gridDim.x * blockDim.x) {
+    if (!G.dripFlag[s]) continue;
+    const uint32_t e = G.dripOff[s];
+    if (e >= P.smax) continue;  // the ring came round to drops placed this substep
+    float4 a = G.drip[s * 2u], b = G.drip[s * 2u + 1u];
+    uint32_t k = st_f2u_sat(a.w);
+    float3 wp, wn, probe;
+    d_dripGeom(C, a, b, k, P, wp, wn, probe);
+    uint32_t slot = (first + e) % P.smax;
+    if (G.sPos[slot].w > 0.0f) continue;
+    uint32_t h0 = s * 747796405u + P.seed * 2891336453u + 17u;
+    float3 v = f3add(d_colSurfVel(C, k, wp), f3mul(f3mul(f3mul(wn, P.h), 2.0f), d_hash1(h0 + 1u)));
+    G.sPos[slot] = make_float4(probe.x, probe.y, probe.z, P.sprLife * 2.0f);
+    G.sVel[slot] = make_float4(v.x, v.y, v.z, 0.0f);
+  }
+}
+__global__ void k_drip_advance(const DParams* __restrict__ PP, DGrid G) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) G.sctl[0] = G.sctl[0] + G.dripOff[PP->ndrip];
+}
+
+void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, bool emit, uint32_t ndrip) {
   const uint32_t pg = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.n, WG), 8192u));
   const uint32_t sg = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(L.smax, WG), 8192u));
   if (air && airN) {
@@ -2800,4 +2861,149 @@ void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, b
     hipLaunchKernelGGL(k_spray_emit, dim3(pg), dim3(WG), 0, st, L.P, L.G);
     hipLaunchKernelGGL(k_spray_advance, dim3(1), dim3(64), 0, st, L.P, L.G);
   }
+  if (ndrip) {
+    const uint32_t dg = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(ndrip, WG), 1024u));
+    hipLaunchKernelGGL(k_drip_flags, dim3(dg), dim3(WG), 0, st, L.P, L.C, L.G);
+    launch_scan(st, L.G.dripFlag, L.G.dripOff, ndrip, L.G.dripSums, nullptr);
+    hipLaunchKernelGGL(k_drip_emit, dim3(dg), dim3(WG), 0, st, L.P, L.C, L.G);
+    hipLaunchKernelGGL(k_drip_advance, dim3(1), dim3(64), 0, st, L.P, L.G);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The live view (sim.js:364-404, 504-537; wgsl-whitewater.js:183-253): a sample
+// of the water and of the whitewater packed into 32-bit words in the cache
+// frame format -- a 4-word header, then per group of four water points six words
+// of 16-bit positions, a word of speed bytes and a word of white-water bytes,
+// then per group of four whitewater points six words of positions and four
+// words of 32-bit codes.  The water sample is every stride-th particle in id
+// order (the reference's buffer order); the whitewater sample is every s-th
+// slot of those handed out, the living ones packed together in slot order
+// (where the reference's atomics pack them in arrival order).
+
+// packSpray (wgsl-whitewater.js:184-195)
+__device__ __forceinline__ uint32_t d_packSpray(float3 v, uint32_t kind, float lifeFrac) {
+  float s = f3len(v);
+  uint32_t code = 0x80000000u | (min(st_f2u_sat(s * 4.0f), 255u) << 21);
+  if (kind == 1u) code |= 0x40000000u;
+  else if (kind == 2u) code |= 0x20000000u;
+  else if (kind == 3u) code |= 0x60000000u;
+  if (kind >= 2u) return code | st_f2u_sat(st_clampf(lifeFrac, 0.0f, 1.0f) * 1023.0f);
+  float3 n = make_float3(0.0f, 1.0f, 0.0f);
+  if (s > 1e-6f) n = f3div(v, s);
+  float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+  float px = n.x / l1, py = n.y / l1;
+  if (n.z < 0.0f) {
+    float qx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+    float qy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+    px = qx;
+    py = qy;
+  }
+  uint32_t u = st_f2u_sat(st_clampf(rintf((px * 0.5f + 0.5f) * 1023.0f), 0.0f, 1023.0f));
+  uint32_t w = st_f2u_sat(st_clampf(rintf((py * 0.5f + 0.5f) * 1023.0f), 0.0f, 1023.0f));
+  return code | (w << 10) | u;
+}
+// sprayDisp (wgsl-whitewater.js:201-211): which sample slots hold a living point
+__global__ void k_view_spray_flags(const DParams* __restrict__ PP, DGrid G, uint32_t sdMax, uint32_t* __restrict__ flag) {
+  const DParams& P = *PP;
+  const uint32_t used = d_wwUsed(P, G), stride = max(1u, (used + sdMax - 1u) / sdMax);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < sdMax; k += gridDim.x * blockDim.x) {
+    uint32_t j = k * stride;
+    flag[k] = (j < used && G.sPos[j].w > 0.0f) ? 1u : 0u;
+  }
+}
+__global__ void k_view_spray_put(const DParams* __restrict__ PP, DGrid G, uint32_t sdMax, const uint32_t* __restrict__ flag,
+                                 const uint32_t* __restrict__ off, float4* __restrict__ disp, float foamLife) {
+  const DParams& P = *PP;
+  const uint32_t used = d_wwUsed(P, G), stride = max(1u, (used + sdMax - 1u) / sdMax);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < sdMax; k += gridDim.x * blockDim.x) {
+    if (!flag[k]) continue;
+    uint32_t j = k * stride;
+    float4 sp = G.sPos[j], sv = G.sVel[j];
+    uint32_t code = d_packSpray(c3(sv), st_f2u_sat(sv.w + 0.5f), sp.w / fmaxf(foamLife, 0.01f));
+    disp[off[k]] = make_float4(sp.x, sp.y, sp.z, __uint_as_float(code));
+  }
+}
+__device__ __forceinline__ uint3 d_q16(const ViewParams& V, float3 p) {
+  return make_uint3(st_f2u_sat(st_clampf(rintf((p.x - V.x0) * V.kx), 0.0f, 65535.0f)),
+                    st_f2u_sat(st_clampf(rintf((p.y - V.y0) * V.ky), 0.0f, 65535.0f)),
+                    st_f2u_sat(st_clampf(rintf((p.z - V.z0) * V.kz), 0.0f, 65535.0f)));
+}
+__device__ __forceinline__ void d_put6(uint32_t* out, uint32_t o, const uint3* q) {
+  out[o] = q[0].x | (q[0].y << 16);
+  out[o + 1u] = q[0].z | (q[1].x << 16);
+  out[o + 2u] = q[1].y | (q[1].z << 16);
+  out[o + 3u] = q[2].x | (q[2].y << 16);
+  out[o + 4u] = q[2].z | (q[3].x << 16);
+  out[o + 5u] = q[3].y | (q[3].z << 16);
+}
+// packDisp (wgsl-whitewater.js:219-253) for the water and whitewater parts;
+// water points come from the id-ordered copy `wat` (every srcStride-th one)
+__global__ void k_pack_view(const ViewParams V, const float4* __restrict__ wat, const float4* __restrict__ sdisp,
+                            const uint32_t* __restrict__ nSp, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gW = (V.nW + 3u) / 4u, gS = (V.nS + 3u) / 4u;
+  const uint32_t nS = min(*nSp, V.nS);
+  if (t == 0u) {
+    out[0] = nS;
+    out[1] = 0u;
+    out[2] = 0u;
+    out[3] = 0u;
+  }
+  uint3 q[4];
+  if (t < gW) {
+    uint32_t b0 = 0u, b1 = 0u;
+    for (uint32_t m = 0; m < 4u; m++) {
+      const uint32_t i = t * 4u + m;
+      float4 v = make_float4(0.0f, -1.0e6f, 0.0f, 0.0f);
+      if (i < V.nW) v = wat[size_t(i) * V.srcStride];
+      uint32_t s = 255u, f = 0u;
+      q[m] = make_uint3(0u, 0u, 0u);
+      if (v.y > ST_DEADY) {
+        q[m] = d_q16(V, c3(v));
+        float lv = floorf(v.w / 64.0f);
+        s = st_f2u_sat(st_clampf((v.w - lv * 64.0f) * 16.0f * V.vs, 0.0f, 191.0f));
+        f = st_f2u_sat(st_clampf(lv, 0.0f, 15.0f)) * 17u;
+      }
+      b0 |= s << (8u * m);
+      b1 |= f << (8u * m);
+    }
+    d_put6(out, V.oWp + t * 6u, q);
+    out[V.oWs + t] = b0;
+    out[V.oWf + t] = b1;
+    return;
+  }
+  const uint32_t ts = t - gW;
+  if (ts < gS) {
+    for (uint32_t m = 0; m < 4u; m++) {
+      const uint32_t k = ts * 4u + m;
+      uint32_t code = 0u;
+      q[m] = make_uint3(0u, 0u, 0u);
+      if (k < nS) {
+        float4 v = sdisp[k];
+        if (v.y > ST_DEADY) {
+          q[m] = d_q16(V, c3(v));
+          code = __float_as_uint(v.w);
+        }
+      }
+      out[V.oSk + k] = code;
+    }
+    d_put6(out, V.oSp + ts * 6u, q);
+  }
+}
+
+void launch_pack_view(hipStream_t st, const LaunchCtx& L, const ViewParams& V, uint32_t sdMax, uint32_t* flag, uint32_t* off,
+                      uint32_t* sums, float4* sdisp, uint32_t* out) {
+  if (V.n) launch_unsort(st, L.G.pos, L.G.pid, L.G.scratch4, V.n);
+  if (sdMax) {
+    const uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(sdMax, WG), 4096u));
+    hipLaunchKernelGGL(k_view_spray_flags, dim3(g), dim3(WG), 0, st, L.P, L.G, sdMax, flag);
+    launch_scan(st, flag, off, sdMax, sums, nullptr);
+    hipLaunchKernelGGL(k_view_spray_put, dim3(g), dim3(WG), 0, st, L.P, L.G, sdMax, flag, off, sdisp, V.foamLife);
+  } else {
+    launch_fill_u32(st, off, 1, 0u);
+  }
+  const uint32_t groups = (V.nW + 3u) / 4u + (V.nS + 3u) / 4u;
+  hipLaunchKernelGGL(k_pack_view, dim3(std::max<uint32_t>(1u, gridFor(groups, WG))), dim3(WG), 0, st, V, L.G.scratch4, sdisp,
+                     sdMax ? off + sdMax : off, out);
 }

@@ -226,6 +226,9 @@ void readParams(napi_env env, napi_value o, st_params* p) {
   p->air_couple = float(numProp(env, o, "airCouple", 0));
   p->air_decay = float(numProp(env, o, "airDecay", 0));
   p->air_blur = float(numProp(env, o, "airBlur", p->air_blur));
+  p->drip_on = uint32_t(numProp(env, o, "dripOn", 0));
+  p->drip_rate = float(numProp(env, o, "dripRate", 0));
+  p->drip_dry = float(numProp(env, o, "dripDry", 0));
   bool has = false;
   napi_has_named_property(env, o, "wind", &has);
   if (has) {
@@ -376,6 +379,42 @@ napi_value BuildSdf(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// packView(h, dispMax) -> {layout: {nW, nS, nD, words, oWp, ...}, words: Uint32Array} (st_pack_view)
+napi_value PackView(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  uint32_t dispMax = 0;
+  napi_get_value_uint32(env, argv[1], &dispMax);
+  void* h = handleOf(env, argv[0]);
+  st_view_layout L;
+  ST_OK_OR_THROW(st_pack_view(h, dispMax, nullptr, 0, &L));
+  void* data = nullptr;
+  napi_value ab, ta, out, lay;
+  napi_create_arraybuffer(env, size_t(std::max(L.words, 1u)) * 4, &data, &ab);
+  ST_OK_OR_THROW(st_pack_view(h, dispMax, (uint32_t*)data, L.words, &L));
+  napi_create_typedarray(env, napi_uint32_array, L.words, ab, 0, &ta);
+  napi_create_object(env, &lay);
+  const struct { const char* k; uint32_t v; } f[] = {{"nW", L.nW}, {"nS", L.nS}, {"nD", L.nD}, {"words", L.words}, {"oWp", L.oWp},
+                                                      {"oWs", L.oWs}, {"oWf", L.oWf}, {"oSp", L.oSp}, {"oSk", L.oSk}, {"oDp", L.oDp},
+                                                      {"oDs", L.oDs}, {"stride", L.stride}, {"fromPos", L.from_pos}};
+  for (const auto& e : f) napi_set_named_property(env, lay, e.k, mkNum(env, double(e.v)));
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "layout", lay);
+  napi_set_named_property(env, out, "words", ta);
+  return out;
+}
+
+// setDrips(h, Float32Array samples (8 per sample))
+napi_value SetDrips(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  void* d = nullptr;
+  size_t n = 0;
+  if (!typed(env, argv[1], &d, &n, napi_float32_array)) return throwErr(env, "setDrips: samples must be a Float32Array");
+  ST_OK_OR_THROW(st_set_drips(handleOf(env, argv[0]), (const float*)d, uint32_t(n / 8)));
+  return nullptr;
+}
+
 // setSprayCap(h, n)
 napi_value SetSprayCap(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -494,7 +533,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;

@@ -20,6 +20,21 @@
 const path = require('path');
 
 let addon = null;
+// the packed live view (packLayout, sim.js:376-380) as a frame: one copy into a buffer just big enough
+// for what is there (unpackLive, sim.js:526-537)
+function unpackLive(words, L, b) {
+  const range = words.buffer, base = words.byteOffset;
+  const hdr = new Uint32Array(range, base, 4), nW = L.nW, nS = Math.min(hdr[0], L.nS), nD = Math.min(hdr[1], L.nD);
+  const sz = [nS * 4, nW * 6, nS * 6, nD * 6, nW, nW, nD], off = []; let o = 0;
+  for (const z of sz) { off.push(o); o += Math.ceil(z / 4) * 4; }
+  const ab = new ArrayBuffer(Math.max(o, 4)), src = new Uint8Array(range, base, words.byteLength);
+  const cp = (k, from) => { if (sz[k]) new Uint8Array(ab, off[k], sz[k]).set(src.subarray(from, from + sz[k])); };
+  cp(0, L.oSk * 4); cp(1, L.oWp * 4); cp(2, L.oSp * 4); cp(3, L.oDp * 4); cp(4, L.oWs * 4); cp(5, L.oWf * 4); cp(6, L.oDs * 4);
+  const fr = { n: nW, pos: new Uint16Array(ab, off[1], nW * 3), spd: new Uint8Array(ab, off[4], nW), fm: new Uint8Array(ab, off[5], nW), b, bytes: ab.byteLength };
+  if (nS) fr.sp = { n: nS, pos: new Uint16Array(ab, off[2], nS * 3), pk: new Uint32Array(ab, off[0], nS), alive: nS };
+  if (nD) fr.dp = { n: nD, pos: new Uint16Array(ab, off[3], nD * 3), spd: new Uint8Array(ab, off[6], nD) };
+  return fr;
+}
 function loadAddon() {
   if (!addon) addon = require(path.join(__dirname, 'splashtank_hip.node'));
   return addon;
@@ -285,7 +300,15 @@ class FlipSim {
   async readSpray() { return this.sprayCap ? loadAddon().readSpray(this.handle, this.sprayCap) : null; }
   setDeepLook() {}
   async readDeep() { return null; }
-  pump() {}
+  // the live view (sim.js:504-525): packed on the GPU (st_pack_view), here read back at once; outFrame is a
+  // frame in the cache format (unpackLive) that the drawing uploads as it is
+  pump() {
+    if (!this.handle || this.N === 0) return;
+    this.flushRun();
+    const r = loadAddon().packView(this.handle, this.dispMax || 300000);
+    this.outFrame = unpackLive(r.words, r.layout, Object.assign({}, this.b));
+    this.outN = r.layout.nW; this.gen = (this.gen || 0) + 1;
+  }
 }
 
 // Many tanks stepped together, one kernel launch per stage for all of them

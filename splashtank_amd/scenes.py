@@ -275,3 +275,59 @@ def batched_tanks(count=64, seed=3, nmin=32, nmax=128, iters=40, dt=1.0 / 240):
         prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0)
         out.append(Scene(name, d, (1.0, 1.0, 1.0), 0, pos, vel, prm, ratio=ratio))
     return out
+
+
+def _js_hypot3(a, b, c):
+    """Math.hypot of three finite numbers as JS engines evaluate it: scaled by the
+    largest magnitude, squares summed with Kahan compensation, root, rescale."""
+    m = max(abs(a), abs(b), abs(c))
+    if m == 0.0:
+        return 0.0
+    s = comp = 0.0
+    for x in (a, b, c):
+        n = x / m
+        summand = n * n - comp
+        prelim = s + summand
+        comp = (prelim - s) - summand
+        s = prelim
+    return math.sqrt(s) * m
+
+
+def drip_samples(sdf, h, k):
+    """dripSamples (src/flip/bake.js:19-40): points over a collider's surface from
+    its distance grid -- grid points within 0.6 cells of the surface moved onto it
+    along the gradient, thinned to about one per solver cell of surface (100 to
+    6000) by a seeded stride -- as 8 f32 each: local position, collider number k,
+    local normal, wetness 0.  `sdf` = dict(id, dims, origin, cell, data)."""
+    nx, ny, nz = (int(x) for x in sdf["dims"])
+    c, o = float(sdf["cell"]), [float(x) for x in sdf["origin"]]
+    d = np.asarray(sdf["data"], np.float32).reshape(-1)
+
+    def at(i, j, q):
+        return float(d[min(nx - 1, max(0, i)) + nx * (min(ny - 1, max(0, j)) + ny * min(nz - 1, max(0, q)))])
+
+    pts = []
+    for q in range(1, nz - 1):
+        for j in range(1, ny - 1):
+            for i in range(1, nx - 1):
+                v = float(d[i + nx * (j + ny * q)])
+                if abs(v) > 0.6 * c:
+                    continue
+                gx, gy, gz = at(i + 1, j, q) - at(i - 1, j, q), at(i, j + 1, q) - at(i, j - 1, q), at(i, j, q + 1) - at(i, j, q - 1)
+                gl = _js_hypot3(gx, gy, gz)
+                if gl < 1e-9:
+                    continue
+                gx, gy, gz = gx / gl, gy / gl, gz / gl
+                pts.append((o[0] + i * c - gx * v, o[1] + j * c - gy * v, o[2] + q * c - gz * v, gx, gy, gz))
+    nsurf = len(pts)
+    want = max(100, min(6000, math.floor(nsurf * c * c / (h * h) + 0.5)))
+    step = max(1.0, nsurf / want)
+    sd = 7 + int(sdf["id"])
+    out = []
+    sd = (sd * 16807) % 2147483647
+    t = sd / 2147483647 * step
+    while t < nsurf:
+        p = pts[math.floor(t)]
+        out.append((p[0], p[1], p[2], float(k), p[3], p[4], p[5], 0.0))
+        t += step
+    return np.array(out, np.float32).reshape(-1, 8)

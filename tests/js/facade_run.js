@@ -29,8 +29,11 @@ sim.stepTimed(cfg.params.dt, o).then((stageMs) => {
   return sim.readbackInto(dst).then((m) => {
     fs.writeFileSync(cfg.out, Buffer.from(dst.buffer));
     const st = sim.stats;
+    sim.pump();
+    const fr = sim.outFrame, sum = (a) => a.reduce((x, y) => x + y, 0);
+    const view = { n: fr.n, pos: sum(fr.pos), spd: sum(fr.spd), fm: sum(fr.fm) };
     console.log(JSON.stringify({ n: m, N: sim.N, nx: sim.nx, tiles: sim.ntiles, scap: sim.scap, stats: st, stages: Object.keys(stageMs),
-      fluidCells: st.fluid, mem: sim.memBytes(), norms: sim.solveNorms() }));
+      fluidCells: st.fluid, mem: sim.memBytes(), norms: sim.solveNorms(), view }));
     sim.destroy();
   });
 }).catch((e) => { console.error(e.stack || String(e)); process.exit(1); });

@@ -64,6 +64,14 @@ def test_facade_matches_c_abi(tmp_path):
     nrm = g.solve_norms()
     assert info["norms"]["resL2"] == nrm["res_l2"] and info["norms"]["errMax"] == nrm["err_max"]
     assert info["norms"]["fluid"] == nrm["fluid"] > 0
+    # the live view: the facade's pump() frame against the C ABI's packed words
+    from oracle import pack
+    L, words = g.pack_view()
+    u = pack.unpack(words, L)
+    view = info["view"]
+    assert view["n"] == L["nW"] == sc.n
+    assert view["pos"] == int(u["pos"].astype(np.int64).sum()) and view["spd"] == int(u["spd"].astype(np.int64).sum())
+    assert view["fm"] == int(u["fm"].astype(np.int64).sum())
 
 
 @needs_node
