@@ -114,6 +114,8 @@ template <class R> struct Params {
   R airCouple, airDecay, airH;
   uint32_t aNx, aNy, aNz;
   R airBlur;
+  uint32_t ndrip;
+  R dripRate, dripDry;
 };
 
 template <class R> struct Sim {
@@ -153,6 +155,9 @@ template <class R> struct Sim {
   uint32_t airN = 0, aNx = 1, aNy = 1, aNz = 1;
   double airH = 1.0;
   bool airOnStep = false, sprOnStep = false;
+  std::vector<R> drip;  // 8 per sample: local position, collider, local normal, wetness
+  uint32_t ndrip = 0;
+  bool dripOnStep = false;
 
   // Face values: from normFaces on, the reference keeps the new and old face
   // velocity as f32 bits in the same words that held the fixed-point sums
@@ -1002,7 +1007,10 @@ template <class R> struct Sim {
       sPos[j * 4] = np.x; sPos[j * 4 + 1] = np.y; sPos[j * 4 + 2] = np.z; sPos[j * 4 + 3] = life;
       sVel[j * 4] = v.x; sVel[j * 4 + 1] = v.y; sVel[j * 4 + 2] = v.z; sVel[j * 4 + 3] = R(kind) + R(0.4) * tk / hold;
     }
-    if (!sprOnStep) return;
+    if (sprOnStep) sprayEmit();
+    if (dripOnStep) stageDrips();
+  }
+  void sprayEmit() {
     uint32_t e = 0;  // sprayEmit 98-127: points in particle order, the ring's next slots
     const uint32_t first = sctl[0];
     for (uint32_t i = 0; i < P.n; i++) {
@@ -1043,6 +1051,43 @@ template <class R> struct Sim {
       }
     }
     sctl[0] = first + e;
+  }
+  void stageDrips() {  // dripStep (wgsl-whitewater.js:256-282), samples in order, drops in the ring's next slots
+    const uint32_t first = sctl[0];
+    uint32_t e = 0;
+    for (uint32_t sIdx = 0; sIdx < P.ndrip; sIdx++) {
+      R* a = &drip[size_t(sIdx) * 8];
+      R* b = a + 4;
+      uint32_t k = tou(a[3]);
+      if (k >= P.ncols) continue;
+      V3<R> r0 = colRow(k, 2), r1 = colRow(k, 3), r2 = colRow(k, 4), c = colRow(k, 5);
+      V3<R> wp = c + r0 * a[0] + r1 * a[1] + r2 * a[2];
+      V3<R> n = r0 * b[0] + r1 * b[1] + r2 * b[2];
+      V3<R> wn = n / len(n);
+      V3<R> probe = wp + wn * P.h * R(0.75);
+      R here = wetAt(probe);
+      if (here > R(0.4)) { b[3] = R(1); continue; }
+      R wetness = fmx(R(0), b[3] - P.dt / fmx(P.dripDry, R(0.05)));
+      b[3] = wetness;
+      if (wetness <= R(0) || here > R(0.1) || P.smax == 0u || P.dripRate <= R(0)) continue;
+      R down = clampr(-wn.y, R(0), R(1));
+      R rate = P.dripRate * wetness * wetness * (R(0.15) + R(0.85) * down);
+      uint32_t h0 = sIdx * 747796405u + P.seed * 2891336453u + 17u;
+      if (hash1(h0) >= rate * P.dt) continue;
+      uint32_t eh = e++;
+      if (eh >= P.smax) continue;
+      uint32_t slot = (first + eh) % P.smax;
+      if (sPos[slot * 4 + 3] > R(0)) continue;
+      V3<R> v = colSurfVel(k, wp) + wn * P.h * R(2) * hash1(h0 + 1u);
+      sPos[slot * 4] = probe.x; sPos[slot * 4 + 1] = probe.y; sPos[slot * 4 + 2] = probe.z; sPos[slot * 4 + 3] = P.sprLife * R(2);
+      sVel[slot * 4] = v.x; sVel[slot * 4 + 1] = v.y; sVel[slot * 4 + 2] = v.z; sVel[slot * 4 + 3] = R(0);
+    }
+    sctl[0] = first + e;
+  }
+  void setDrips(const float* a, uint32_t n) {
+    drip.assign(size_t(n) * 8, R(0));
+    for (size_t q = 0; q < size_t(n) * 8; q++) drip[q] = R(a[q]);
+    ndrip = n;
   }
   void setSprayCap(uint32_t n) {  // sim.js:333-340
     if (n == sprayCap) return;
@@ -1196,6 +1241,11 @@ template <class R> struct Sim {
       p.airBlur = airo ? R(o->air_blur) : R(0);
       airOnStep = p.airOn != 0u;
       sprOnStep = on;
+      const bool dr = on && o->drip_on != 0;
+      p.ndrip = dr ? ndrip : 0u;
+      p.dripRate = dr ? R(o->drip_rate) : R(0);
+      p.dripDry = dr ? R(o->drip_dry) : R(1);
+      dripOnStep = dr && ndrip > 0;
     }
     return ST_OK;
   }
@@ -1355,6 +1405,11 @@ int orc_set_colliders(void* hp, const float* cf, uint32_t nc, const float* sdf, 
     s->ncolsTotal = nc;
   };
   if (h->fp64) set(h->d); else set(h->f);
+  return ST_OK;
+}
+int orc_set_drips(void* hp, const float* a, uint32_t n) {
+  Handle* h = (Handle*)hp;
+  if (h->fp64) h->d->setDrips(a, n); else h->f->setDrips(a, n);
   return ST_OK;
 }
 int orc_set_spray_cap(void* hp, uint32_t n) {

@@ -32,6 +32,7 @@ _PROTOS = {
     "orc_dump": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "orc_threads": (C.c_int, []),
     "orc_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "orc_set_drips": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "orc_spray_put": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "orc_read_air": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "orc_sincosf": (None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -92,6 +93,10 @@ class Oracle(_Base):
 
     def set_spray_cap(self, n):
         self._call("set_spray_cap", self.h, int(n))
+
+    def set_drips(self, samples):
+        a = np.ascontiguousarray(samples, np.float32).reshape(-1, 8)
+        self._call("set_drips", self.h, a.ctypes.data if a.size else None, a.shape[0])
 
     def spray_put(self, p4, v4, handed=None):
         """test hook: whitewater points into slots 0..n-1, ring counter = handed (default n)"""

@@ -18,6 +18,7 @@
 // the substep this library provides; their members are stubs.
 
 const path = require('path');
+const { dripSamples } = require('./sdf.js');
 
 let addon = null;
 // the packed live view (packLayout, sim.js:376-380) as a frame: one copy into a buffer just big enough
@@ -148,6 +149,7 @@ class FlipSim {
     this.flushRun(); this.N = 0; this.ring = 0; this.devN = 0;
     loadAddon().clear(this.handle);
     this.tm = 0; this.nbInit = 1; this.nbFills = []; this.nbFillVols = []; this.outFrame = null;
+    this.dripSig = '';   // drip samples come back dry
   }
   // ---- stepping (sim.js:179-243) ----
   omega() { return Math.min(1.9, 2 / (1 + Math.sin(Math.PI / Math.max(this.nx, this.ny, this.nz)))); }
@@ -177,13 +179,26 @@ class FlipSim {
         wwImpact: ww.impact === undefined ? 0.5 : ww.impact });
     }
     if (air && air.on) Object.assign(out, { airOn: 1, airCouple: air.couple, airDecay: air.decay, airBlur: air.blur === undefined ? 0.35 : air.blur });
+    const dr = o.drip || null;
+    if (sp && dr && dr.on) Object.assign(out, { dripOn: 1, dripRate: dr.rate, dripDry: dr.dry });
     return out;
+  }
+  // drip samples on the colliders, rebuilt when the set of collider grids changes (dripUpdate, sim.js:353-363)
+  dripUpdate(cols) {
+    const sig = cols.map((c) => (c.sdf ? c.sdf.id : 'x')).join(',');
+    if (sig === this.dripSig) return;
+    this.dripSig = sig;
+    const parts = []; let n = 0;
+    cols.forEach((c, k) => { if (!c.sdf) return; const a = dripSamples(c.sdf, this.h, k); parts.push(a); n += a.length / 8; });
+    const all = new Float32Array(n * 8); let o = 0; parts.forEach((a) => { all.set(a, o); o += a.length; });
+    loadAddon().setDrips(this.handle, all);
   }
   prepare(dt, o) {
     this.flushRun();
     if (this.N === 0) return null;
     if (!this.only) this.maybeGrow();
     const prm = this.params(dt, o);
+    if (prm.dripOn && this.sprayCap > 0) this.dripUpdate((o.cols || []).slice(0, (o.cols || []).length));
     const init = prm.nbInit;
     const cols = init && this.nbFillVols.length ? (o.cols || []).concat(this.nbFillVols) : (o.cols || []);
     const nc = Math.min((o.cols || []).length, packColliders(this, cols));

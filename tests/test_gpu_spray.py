@@ -92,3 +92,34 @@ def test_spray_pool_resize_and_clear():
     p, _ = g.read_spray()
     assert p.shape == (1, 4) and np.all(p == 0)  # clear() starts the ring again
     g.close()
+
+
+def test_drips_from_a_box_lifted_out_of_the_water_match_oracle():
+    from splashtank_amd import scenes as sc_mod
+    d = 1 / 64
+    pos, vel = lattice(d, (-0.5, 0.0, -0.5), (0.5, 0.25, 0.5))
+    opts = capi.make_opts(d, (1, 1, 1), max_n=len(pos))
+    g, o = FlipSimC(opts), Oracle(opts)
+    h = 2 * d
+    dims, lo, data = sc_mod.box_sdf((0.1,) * 3, h, 4 * h)
+    samples = sc_mod.drip_samples(dict(id=1, dims=dims, origin=lo, cell=h, data=data), h, 0)
+    path = lambda t: np.array([0.0, 0.15 + 1.2 * t, 0.0])  # noqa: E731  (rises out of the pool)
+    for s in (g, o):
+        s.upload(pos, vel)
+        s.set_spray_cap(1 << 15)
+        s.set_drips(samples)
+    prm = capi.make_params(dt=1 / 120, g=9.81, iters=15, **dict(SPR, spr_rate=0.0, drip_on=1, drip_rate=40.0, drip_dry=1.5))
+    print(f"[drips] box lifted from a pool, {len(samples)} drip samples, {len(pos)} particles", flush=True)
+    t = 0.0
+    for step in range(36):
+        t += 1 / 120
+        p1, p0 = path(t), path(t - 1 / 120)
+        cf = sc_mod.collider_record(0, dims, lo, h, p1, vel=(p1 - p0) * 120)
+        for s in (g, o):
+            s.set_colliders(cf, data, 1)
+            s.step(prm)
+        if step % 6 == 5:
+            alive = _compare(g, o, f"step {step}")
+            print(f"[drips] step {step}: {alive} living whitewater points", flush=True)
+    assert alive > 20
+    g.close()

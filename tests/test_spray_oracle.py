@@ -119,3 +119,39 @@ def test_emission_waits_for_settling_and_needs_spr():
     kinds = v[alive, 3]
     assert set(np.unique(kinds)) <= {0.0, 1.0, 3.0}
     assert np.all(p[alive, 3] <= 1.6 * 1.4 + 1e-9)      # life <= spray life x (0.6 + 0.8) x mist factor
+
+
+def _drip_box(o, pos, vel=(0.0, 0.0, 0.0)):
+    """a 0.08 half-size box collider at `pos` in oracle or GPU sim `o`, and its drip samples"""
+    from splashtank_amd import scenes
+    h = 2 / 32
+    dims, lo, data = scenes.box_sdf((0.08,) * 3, h, 4 * h)
+    cf = scenes.collider_record(0, dims, lo, h, pos, vel=vel)
+    o.set_colliders(cf, data, 1)
+    return scenes.drip_samples(dict(id=1, dims=dims, origin=lo, cell=h, data=data), h, 0)
+
+
+def test_drips_dry_out_and_let_go_of_drops():
+    o = _pool_sim(True, cap=8192)
+    samples = _drip_box(o, (0.0, 0.6, 0.0))        # well above the water: every sample exposed
+    samples[:, 7] = 1.0                             # and wet, as if just lifted out
+    o.set_drips(samples)
+    prm = _prm(spr_on=1, spr_rate=0.0, spr_min=1.0, spr_life=1.5, spr_drag=1.0, spr_mdrag=4.0,
+               drip_on=1, drip_rate=1.0e7, drip_dry=2.0)
+    o.step(prm)
+    p, v = o.read_spray()
+    n = len(samples)
+    assert n > 20 and p.shape[0] == n               # every sample let go of one drop (rate x dt >> 1)
+    assert np.all(p[:, 3] == pytest.approx(3.0))   # spray life x 2
+    assert np.all(v[:, 3] == 0.0)                  # spray
+    h = 1 / 16
+    speed = np.linalg.norm(v[:, :3], axis=1)
+    assert speed.max() <= 2 * h + 1e-12            # a static box: only the push off its surface, h x 2 x hash
+    o.step(prm)                                     # wetness fell to 1 - dt / dry: fewer, still every sample
+    p2, _ = o.read_spray()
+    assert p2.shape[0] == min(2 * n, 8192)
+    dry = _pool_sim(True, cap=64)
+    samples2 = _drip_box(dry, (0.0, 0.6, 0.0))      # dry samples never drip
+    dry.set_drips(samples2)
+    dry.step(prm)
+    assert np.all(dry.read_spray()[0][:, 3] == 0.0)
