@@ -41,6 +41,9 @@ struct AdjGrad {
   float* gRhs;    // per cell: dL/d(rhs)
   float* carry;   // per tile of the box (ntiles * 512): dL/d(pressure at the end of the
                   // previous substep), through the next substep's warm start
+  uint32_t* sorNb;  // per cell, 6 per cell: the fluid neighbour a transposed sweep gathers from
+                    // (x-, x+, y-, y+, z-, z+), ST_NONE where there is none or it has no non-solid side
+  float* sorCoef;   // per cell, 6 per cell: omega / (non-solid neighbours of that neighbour)
   uint32_t ncell; // scap * 512
   uint32_t ncarry;
 };

@@ -249,12 +249,43 @@ __global__ void __launch_bounds__(AWG) k_adj_project(AdjTape T, AdjGrad G, uint3
   }
 }
 
-// one colour of one red-black iteration, transposed: gin -> gout
-__global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const float* __restrict__ gin, float* __restrict__ gout,
-                                                 uint32_t colour) {
+// The gather of a transposed sweep for every cell, fixed for the whole reversed
+// solve (the topology is the forward pass's): the fluid neighbours a cell
+// gathers from and their weights omega / n, in the order x-, x+, y-, y+, z-, z+.
+__global__ void __launch_bounds__(AWG) k_adj_sor_prep(AdjTape T, AdjGrad G) {
   const DParams& P = *T.P;
   const AView V{P, T};
   const float w = P.omega;
+  FOR_CELLS(T) {
+    const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
+    const bool fluid = (T.sten[c] & 0x80u) != 0u;
+    const int3 at = d_slotCoord(P, T.slotTile, s, l);
+    for (uint32_t q = 0; q < 6; q++) {
+      uint32_t nb = ST_NONE;
+      float coef = 0.f;
+      if (fluid) {
+        int3 r = at;
+        if (q == 0) r.x -= 1; else if (q == 1) r.x += 1; else if (q == 2) r.y -= 1; else if (q == 3) r.y += 1; else if (q == 4) r.z -= 1; else r.z += 1;
+        if (V.type(r.x, r.y, r.z) == 1u) {
+          const uint32_t cm = V.cell(r.x, r.y, r.z);
+          const uint32_t nm = T.sten[cm] & 7u;
+          if (nm) {
+            nb = cm;
+            coef = w / float(nm);
+          }
+        }
+      }
+      G.sorNb[size_t(c) * 6u + q] = nb;
+      G.sorCoef[size_t(c) * 6u + q] = coef;
+    }
+  }
+}
+
+// one colour of one red-black iteration, transposed: gin -> gout.  A cell's
+// colour is that of its place in the tile (tile origins are even).
+__global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const float* __restrict__ gin, float* __restrict__ gout,
+                                                 uint32_t colour) {
+  const float w = T.P->omega;
   FOR_CELLS(T) {
     const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
     const uint32_t st = T.sten[c];
@@ -262,8 +293,7 @@ __global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const flo
       gout[c] = 0.f;
       continue;
     }
-    const int3 at = d_slotCoord(P, T.slotTile, s, l);
-    if (uint32_t((at.x + at.y + at.z) & 1) == colour) {
+    if ((((l & 7u) + ((l >> 3) & 7u) + (l >> 6)) & 1u) == colour) {
       const uint32_t n = st & 7u;
       if (n == 0u) {
         gout[c] = 0.f;
@@ -274,14 +304,10 @@ __global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const flo
       gout[c] = (1.f - w) * g;
     } else {
       float g = gin[c];
-      for (uint32_t q = 0; q < 6; q++) {
-        int3 r = at;
-        if (q == 0) r.x -= 1; else if (q == 1) r.x += 1; else if (q == 2) r.y -= 1; else if (q == 3) r.y += 1; else if (q == 4) r.z -= 1; else r.z += 1;
-        if (V.type(r.x, r.y, r.z) != 1u) continue;
-        const uint32_t cm = V.cell(r.x, r.y, r.z);
-        const uint32_t nm = T.sten[cm] & 7u;
-        if (nm) g += w / float(nm) * gin[cm];
-      }
+      const uint32_t* nb = G.sorNb + size_t(c) * 6u;
+      const float* cf = G.sorCoef + size_t(c) * 6u;
+      for (uint32_t q = 0; q < 6; q++)
+        if (nb[q] != ST_NONE) g += cf[q] * gin[nb[q]];
       gout[c] = g;
     }
   }
@@ -383,6 +409,7 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   // the solve reversed: per iteration, black then red, ping-pong gPres <-> gPres2
   float* a = G.gPres;
   float* b = G.gPres2;
+  hipLaunchKernelGGL(k_adj_sor_prep, dim3(cg), dim3(AWG), 0, st, T, G);
   for (uint32_t it = 0; it < T.iters; it++)
     for (int colour = 1; colour >= 0; colour--) {
       hipLaunchKernelGGL(k_adj_sor, dim3(cg), dim3(AWG), 0, st, T, G, (const float*)a, b, uint32_t(colour));

@@ -653,7 +653,7 @@ void freeAdj(Adj* a) {
   for (AdjTape& T : a->tapes) freeTape(T);
   AdjGrad& G = a->G;
   dfree(G.gPos); dfree(G.gVel); dfree(G.gPos2); dfree(G.gVel2); dfree(G.gFN); dfree(G.gFO); dfree(G.gPres); dfree(G.gPres2);
-  dfree(G.gRhs); dfree(G.carry);
+  dfree(G.gRhs); dfree(G.carry); dfree(G.sorNb); dfree(G.sorCoef);
   delete a;
 }
 int allocTape(Sim* s, AdjTape& T) {
@@ -1647,7 +1647,8 @@ int st_adj_create(void* p, uint32_t max_steps, void** out) {
   int rc;
   if ((rc = dalloc(&G.gPos, M)) || (rc = dalloc(&G.gVel, M)) || (rc = dalloc(&G.gPos2, M)) || (rc = dalloc(&G.gVel2, M)) ||
       (rc = dalloc(&G.gFN, C * 3)) || (rc = dalloc(&G.gFO, C * 3)) || (rc = dalloc(&G.gPres, C)) || (rc = dalloc(&G.gPres2, C)) ||
-      (rc = dalloc(&G.gRhs, C)) || (rc = dalloc(&G.carry, size_t(s->ntiles) * 512u))) {
+      (rc = dalloc(&G.gRhs, C)) || (rc = dalloc(&G.carry, size_t(s->ntiles) * 512u)) || (rc = dalloc(&G.sorNb, C * 6)) ||
+      (rc = dalloc(&G.sorCoef, C * 6))) {
     freeAdj(a);
     return rc;
   }
