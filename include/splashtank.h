@@ -380,14 +380,27 @@ int st_snapshot_destroy(void* snap);
  * that of the state before it; st_adj_grad reads it back.  The derivative is
  * taken at fixed topology (tiles, cell types, counts and branches as the forward
  * pass took them; the fixed-point sums differentiated as the sums they round).
- * Covered: closed boxes without colliders, drag or the narrow band
- * (ST_ERR_ARG otherwise). */
+ * Covered: closed boxes without drag or the narrow band, with at most
+ * ST_ADJ_MAX_COLS colliders (ST_ERR_ARG otherwise). */
 int st_adj_create(void* sim, uint32_t max_steps, void** adj);
 int st_adj_destroy(void* adj);
 int st_adj_record(void* adj, const st_params* p);
 int st_adj_seed(void* adj, const float* g_pos4, const float* g_vel4);
 int st_adj_backward(void* adj);
 int st_adj_grad(void* adj, float* g_pos4, float* g_vel4);
+/* Colliders (at most ST_ADJ_MAX_COLS, rigid records as st_set_colliders takes
+ * them; their distance grids must not change between a record and its
+ * reversal) are differentiated through the particle push of g2p and the solid
+ * faces of normFaces.  After each st_adj_backward, st_adj_col_grad gives, per
+ * collider of the substep just reversed, 9 floats: dL/d(centre x, y, z),
+ * dL/d(linear velocity), dL/d(angular velocity) of that substep's record
+ * (words 20-22, 24-26, 28-30 of the 32; the rotation rows get none).
+ * ncols must equal that substep's collider count.  No counterpart in the
+ * reference; a keyframed collider's gradient follows from these by the chain
+ * rule through its pose curve (shared.js:608-621) and the pose-difference
+ * velocity (shared.js:531-537). */
+#define ST_ADJ_MAX_COLS 4
+int st_adj_col_grad(void* adj, float* g, uint32_t ncols);
 
 #ifdef __cplusplus
 }

@@ -204,6 +204,7 @@ PROTOS = {
     "st_adj_seed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "st_adj_backward": (C.c_int, [C.c_void_p]),
     "st_adj_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "st_adj_col_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
 }
 
 _lib = None
@@ -537,6 +538,13 @@ class AdjointC:
         gv = np.zeros((max(n, 1), 4), np.float32)
         self._call("grad", gp.ctypes.data, gv.ctypes.data)
         return gp[:n, :3], gv[:n, :3]
+
+    def col_grad(self, ncols):
+        """(ncols, 9) float32 of the substep reversed last: dL/d(centre),
+        dL/d(linear velocity), dL/d(angular velocity) per collider."""
+        g = np.zeros((max(int(ncols), 1), 9), np.float32)
+        self._call("col_grad", g.ctypes.data, int(ncols))
+        return g[: int(ncols)]
 
     def close(self):
         if getattr(self, "h", None):

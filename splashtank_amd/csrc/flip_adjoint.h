@@ -24,6 +24,8 @@ struct AdjTape {
   int32_t* accVfin;     // new face velocity after project (f32 bits)
   int32_t* accWfin;     // old face velocity (f32 bits)
   DParams* P;           // the substep's parameter block (device copy)
+  DCols* C;             // the substep's collider records (device copy)
+  const float* sdf;     // collider distance grids (the sim's, unchanged since the record)
   uint32_t n;           // particles
   uint32_t iters;       // SOR iterations
 };
@@ -44,9 +46,14 @@ struct AdjGrad {
   uint32_t* sorNb;  // per cell, 6 per cell: the fluid neighbour a transposed sweep gathers from
                     // (x-, x+, y-, y+, z-, z+), ST_NONE where there is none or it has no non-solid side
   float* sorCoef;   // per cell, 6 per cell: omega / (non-solid neighbours of that neighbour)
+  double* gCol;     // per collider of the substep reversed last, 9 words: dL/d(centre),
+                    // dL/d(linear velocity), dL/d(angular velocity)
   uint32_t ncell; // scap * 512
   uint32_t ncarry;
 };
+
+// colliders the adjoint follows (pushes of one particle are kept in registers)
+#define ADJ_MAX_COLS 4
 
 // the backward pass of one recorded substep: reads G.gPos/gVel (gradient of the
 // state after it), writes G.gPos2/gVel2 (gradient of the state before it) and the

@@ -20,6 +20,11 @@
 //             new minus g dt on y faces)  ->  gradients of the sums V and W
 //   p2g       per particle: V = sum w v, W = sum w over its 8 faces per axis
 //             -> velocity and position gradients
+// Colliders (at most ADJ_MAX_COLS) take part through the particle push of g2p,
+// p += n (q0 - phi) with phi and n the trilinear distance and its unit gradient
+// (so their derivatives need the trilinear Hessian), and through the solid faces
+// of normFaces, whose velocity is v + w x (x - c); both give the collider's
+// centre, linear and angular velocity a gradient (dL/dc, dL/dv, dL/dw).
 // References: p2g wgsl-solver.js:16-40, normFaces 75-105, divergence 106-123,
 // relax 124-140, project 165-180, g2p 208-275, collideP 185-197.
 #include "flip_adjoint.h"
@@ -46,7 +51,79 @@ __device__ __forceinline__ float3 axisGrid(const DParams& P, float3 p, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// g2p (wgsl-solver.js:208-275) reversed.  Needs drag 0, no colliders, closed walls.
+// collider push (collideP, wgsl-solver.js:185-197; sdfEval, gpu.js:6-30) reversed.
+// In the collider's grid coordinates g = (R^T (p - c) - origin) / cell the
+// distance is trilinear inside the grid, with gradient graw (the forward's
+// unnormalised normal) and a Hessian with a zero diagonal; outside it, phi =
+// trilinear(clamped g) + |g - clamped g| cell and the normal is the unit offset.
+// For the push p' = p + wn (q0 - phi) and a gradient u of p', returns the
+// gradient of p; that of the centre is minus the part beyond u.
+__device__ __forceinline__ float3 adj_push(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, uint32_t k,
+                                           float3 p, float3 u) {
+  const float4 a = C->c[k * 8u], b = C->c[k * 8u + 1u];
+  const float3 r0 = c3(C->c[k * 8u + 2u]), r1 = c3(C->c[k * 8u + 3u]), r2 = c3(C->c[k * 8u + 4u]);
+  const float3 d = f3sub(p, c3(C->c[k * 8u + 5u]));
+  const float3 lp = make_float3(f3dot(r0, d), f3dot(r1, d), f3dot(r2, d));
+  const float3 g = f3div(f3sub(lp, make_float3(b.x, b.y, b.z)), b.w);
+  const float e = 1.0001f;
+  const float3 gc = make_float3(st_clampf(g.x, 0.0f, a.y - e), st_clampf(g.y, 0.0f, a.z - e), st_clampf(g.z, 0.0f, a.w - e));
+  const float3 fl = make_float3(floorf(gc.x), floorf(gc.y), floorf(gc.z));
+  const float3 f = f3sub(gc, fl);
+  const uint32_t snx = st_f2u_sat(a.y), sny = st_f2u_sat(a.z);
+  const uint32_t base = st_f2u_sat(a.x) + st_f2u_sat(fl.x) + snx * (st_f2u_sat(fl.y) + sny * st_f2u_sat(fl.z));
+  const uint32_t s = snx * sny;
+  const float v000 = sdf[base], v100 = sdf[base + 1u], v010 = sdf[base + snx], v110 = sdf[base + snx + 1u];
+  const float v001 = sdf[base + s], v101 = sdf[base + s + 1u], v011 = sdf[base + s + snx], v111 = sdf[base + s + snx + 1u];
+  const float c00 = st_mixf(v000, v100, f.x), c10 = st_mixf(v010, v110, f.x), c01 = st_mixf(v001, v101, f.x), c11 = st_mixf(v011, v111, f.x);
+  const float c0 = st_mixf(c00, c10, f.y), c1 = st_mixf(c01, c11, f.y);
+  float phi = st_mixf(c0, c1, f.z);
+  const float3 graw = make_float3(st_mixf(st_mixf(v100 - v000, v110 - v010, f.y), st_mixf(v101 - v001, v111 - v011, f.y), f.z),
+                                  st_mixf(c10 - c00, c11 - c01, f.z), c1 - c0);
+  const float3 out = f3sub(g, gc);
+  float3 nl, dphi;   // unit normal and dphi/dg, grid coordinates
+  float J[3][3];     // d nl / d g
+  if (f3dot(out, out) > 0.0f) {
+    const float l = f3len(out);
+    phi = phi + l * b.w;
+    nl = f3div(out, l);
+    const float m[3] = {g.x != gc.x ? 1.f : 0.f, g.y != gc.y ? 1.f : 0.f, g.z != gc.z ? 1.f : 0.f};
+    const float n3[3] = {nl.x, nl.y, nl.z}, gr[3] = {graw.x, graw.y, graw.z};
+    float dp[3];
+    for (int i = 0; i < 3; i++) {
+      dp[i] = m[i] > 0.f ? n3[i] * b.w : gr[i];
+      for (int j = 0; j < 3; j++) J[i][j] = ((i == j ? 1.f : 0.f) - n3[i] * n3[j]) / l * m[j];
+    }
+    dphi = make_float3(dp[0], dp[1], dp[2]);
+  } else {
+    const float hxy = st_mixf((v110 - v010) - (v100 - v000), (v111 - v011) - (v101 - v001), f.z);
+    const float hxz = st_mixf(v101 - v001, v111 - v011, f.y) - st_mixf(v100 - v000, v110 - v010, f.y);
+    const float hyz = (c11 - c01) - (c10 - c00);
+    const float H[3][3] = {{0.f, hxy, hxz}, {hxy, 0.f, hyz}, {hxz, hyz, 0.f}};
+    const float l = f3len(graw), ld = fmaxf(l, 1e-9f);
+    nl = f3div(graw, ld);
+    const float n3[3] = {nl.x, nl.y, nl.z};
+    const bool proj = l > 1e-9f;  // below it the forward divides by a constant
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        float v = 0.f;
+        for (int q = 0; q < 3; q++) v += ((i == q ? 1.f : 0.f) - (proj ? n3[i] * n3[q] : 0.f)) * H[q][j];
+        J[i][j] = v / ld;
+      }
+    dphi = graw;
+  }
+  const float3 wn = f3add(f3add(f3mul(r0, nl.x), f3mul(r1, nl.y)), f3mul(r2, nl.z));
+  const float sh = P.q0 - phi;
+  // (d wn / d p)^T u = R J^T R^T u / cell, d phi / d p = R dphi / cell (R rows r0..r2 map world to grid)
+  const float3 Mu = make_float3(f3dot(r0, u), f3dot(r1, u), f3dot(r2, u));
+  const float3 JtMu = make_float3(J[0][0] * Mu.x + J[1][0] * Mu.y + J[2][0] * Mu.z, J[0][1] * Mu.x + J[1][1] * Mu.y + J[2][1] * Mu.z,
+                                  J[0][2] * Mu.x + J[1][2] * Mu.y + J[2][2] * Mu.z);
+  const float3 ga = f3div(f3add(f3add(f3mul(r0, JtMu.x), f3mul(r1, JtMu.y)), f3mul(r2, JtMu.z)), b.w);
+  const float3 dphiP = f3div(f3add(f3add(f3mul(r0, dphi.x), f3mul(r1, dphi.y)), f3mul(r2, dphi.z)), b.w);
+  return f3add(u, f3sub(f3mul(ga, sh), f3mul(dphiP, f3dot(wn, u))));
+}
+
+// ---------------------------------------------------------------------------
+// g2p (wgsl-solver.js:208-275) reversed.  Needs drag 0 and closed walls.
 // One workgroup per tile bin (the bins the forward substep used): the face
 // gradients of the tile's particles gather in a 10^3-cell LDS window (the tile
 // plus one cell all round holds every face they sampled) and go to HBM once per
@@ -59,7 +136,7 @@ __device__ __forceinline__ float3 axisGrid(const DParams& P, float3 p, uint32_t 
 // it; the face gradients go to add(axis, face, d new, d old)
 template <class Fetch, class Add>
 __device__ __forceinline__ void adj_g2p_particle(const DParams& P, const AdjTape& T, const AdjGrad& G, uint32_t i, bool storage,
-                                                 Fetch fetch, Add add) {
+                                                 float* sCol, Fetch fetch, Add add) {
   const float4 p4 = T.pos[i], v4 = T.vel[i];
   const uint32_t id = T.pid[i];
   const float3 p = c3(p4), v0 = c3(v4);
@@ -91,23 +168,60 @@ __device__ __forceinline__ void adj_g2p_particle(const DParams& P, const AdjTape
   const bool capped = sp > P.vmax;
   const float3 vc = capped ? f3mul(vm, P.vmax / sp) : vm;
   const float3 pt = f3add(p, f3mul(vc, P.dt));
+  // collider pushes in order (the positions they start from are kept), then the box clamp
+  float3 q = pt;
+  float3 qk[ADJ_MAX_COLS];
+  uint32_t hit = 0u;
+  const uint32_t ncol = min(P.ncols, uint32_t(ADJ_MAX_COLS));  // the record refuses more
+#pragma unroll
+  for (uint32_t k = 0; k < ADJ_MAX_COLS; k++) {  // unrolled: qk stays in registers
+    if (k >= ncol || !d_colNear(P, T.C, k, q)) continue;
+    float phi;
+    float3 wn;
+    d_sdfEval(T.C, T.sdf, k, q, phi, wn);
+    if (phi < P.q0) {
+      qk[k] = q;
+      hit |= 1u << k;
+      q = f3add(q, f3mul(wn, P.q0 - phi));
+    }
+  }
   const float m = P.margin;
   const float3 lo = make_float3(P.x0 + m, P.y0 + m, P.z0 + m), hi = make_float3(P.x1 - m, P.y1 - m, P.z1 - m);
-  const float3 pc = make_float3(st_clampf(pt.x, lo.x, hi.x), st_clampf(pt.y, lo.y, hi.y), st_clampf(pt.z, lo.z, hi.z));
-  const float3 keep = make_float3(pt.x > lo.x && pt.x < hi.x ? 1.f : 0.f, pt.y > lo.y && pt.y < hi.y ? 1.f : 0.f,
-                                  pt.z > lo.z && pt.z < hi.z ? 1.f : 0.f);
+  const float3 pc = make_float3(st_clampf(q.x, lo.x, hi.x), st_clampf(q.y, lo.y, hi.y), st_clampf(q.z, lo.z, hi.z));
+  const float3 keep = make_float3(q.x > lo.x && q.x < hi.x ? 1.f : 0.f, q.y > lo.y && q.y < hi.y ? 1.f : 0.f,
+                                  q.z > lo.z && q.z < hi.z ? 1.f : 0.f);
   const float3 dsp = f3sub(pc, pt);
   bool wall = false;
   float3 nrm = make_float3(0.f, 0.f, 0.f);
+  float dl = 0.f;
   if (f3dot(dsp, dsp) > 1.0e-12f) {
-    nrm = f3div(dsp, f3len(dsp));
+    dl = f3len(dsp);
+    nrm = f3div(dsp, dl);
     wall = f3dot(vc, nrm) < 0.f;
   }
-  // backward
+  // backward: the contact removes the normal velocity, v' = vc - n (n . vc), with
+  // n = dsp / |dsp| a function of the displacement the pushes and the clamp made
   const float3 gpc = c3(G.gPos[id]), gvo = c3(G.gVel[id]);
-  float3 gvc = wall ? f3sub(gvo, f3mul(nrm, f3dot(nrm, gvo))) : gvo;
-  gvc = f3add(gvc, f3mul(make_float3(keep.x * gpc.x, keep.y * gpc.y, keep.z * gpc.z), P.dt));
-  float3 gp = make_float3(keep.x * gpc.x, keep.y * gpc.y, keep.z * gpc.z);
+  float3 gvc = gvo, gdsp = make_float3(0.f, 0.f, 0.f);
+  if (wall) {
+    gvc = f3sub(gvo, f3mul(nrm, f3dot(nrm, gvo)));
+    const float3 gn = f3mul(f3add(f3mul(gvo, f3dot(nrm, vc)), f3mul(vc, f3dot(gvo, nrm))), -1.f);
+    gdsp = f3div(f3sub(gn, f3mul(nrm, f3dot(nrm, gn))), dl);
+  }
+  const float3 gpt0 = f3add(gpc, gdsp);
+  float3 gq = make_float3(keep.x * gpt0.x, keep.y * gpt0.y, keep.z * gpt0.z);
+#pragma unroll
+  for (int k = ADJ_MAX_COLS - 1; k >= 0; k--) {
+    if (!((hit >> k) & 1u)) continue;
+    const float3 gb = adj_push(P, T.C, T.sdf, uint32_t(k), qk[k], gq);
+    const float3 gcen = f3sub(gq, gb);  // the centre enters as p - c
+    atomicAdd(&sCol[k * 3 + 0], gcen.x);
+    atomicAdd(&sCol[k * 3 + 1], gcen.y);
+    atomicAdd(&sCol[k * 3 + 2], gcen.z);
+    gq = gb;
+  }
+  float3 gp = f3sub(gq, gdsp);
+  gvc = f3add(gvc, f3mul(gp, P.dt));
   float3 gvm = gvc;
   if (capped) {
     const float3 u = f3div(vm, sp);
@@ -155,6 +269,9 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
   __shared__ float2 sF[3][AWIN3];  // (new, old) face velocity of the window
   __shared__ float sN[3][AWIN3];   // gradient gathered for the new face velocity
   __shared__ float sO[3][AWIN3];   // and for the old one
+  __shared__ float sCol[ADJ_MAX_COLS * 3];  // centre gradients of the colliders
+  if (threadIdx.x < ADJ_MAX_COLS * 3) sCol[threadIdx.x] = 0.f;
+  __syncthreads();
   const int32_t gx = int32_t(P.tx) * 8, gy = int32_t(P.ty) * 8, gz = int32_t(P.tz) * 8;
   auto none = [](uint32_t, int32_t, int32_t, int32_t) { return make_float2(0.f, 0.f); };
   auto drop = [](uint32_t, int32_t, int32_t, int32_t, float, float) {};
@@ -170,7 +287,7 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
       continue;
     }
     if (T.tmap[t] == ST_NONE) {  // no storage this substep: free fall, no grid
-      for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG) adj_g2p_particle(P, T, G, T.binIdx[k], false, none, drop);
+      for (uint32_t k = beg + threadIdx.x; k < end; k += AG2P_WG) adj_g2p_particle(P, T, G, T.binIdx[k], false, sCol, none, drop);
       continue;
     }
     const int3 org = make_int3(int32_t(t % P.tx) * 8 - 1, int32_t((t / P.tx) % P.ty) * 8 - 1, int32_t(t / (P.tx * P.ty)) * 8 - 1);
@@ -192,7 +309,7 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
       const uint32_t k = k0 + (threadIdx.x & 63u) * 8u + (threadIdx.x >> 6);
       if (k >= end) continue;
       adj_g2p_particle(
-          P, T, G, T.binIdx[k], true,
+          P, T, G, T.binIdx[k], true, sCol,
           [&](uint32_t a, int32_t i, int32_t j, int32_t kk) { return sF[a][adj_window_face(P, a, i, j, kk, org)]; },
           [&](uint32_t a, int32_t i, int32_t j, int32_t kk, float dn, float dol) {
             const int32_t q = adj_window_face(P, a, i, j, kk, org);
@@ -213,6 +330,9 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
     }
     __syncthreads();
   }
+  __syncthreads();
+  if (threadIdx.x < min(P.ncols, uint32_t(ADJ_MAX_COLS)) * 3u && sCol[threadIdx.x] != 0.f)
+    atomicAdd(&G.gCol[(threadIdx.x / 3u) * 9u + threadIdx.x % 3u], double(sCol[threadIdx.x]));
 }
 
 #define FOR_CELLS(T)                                                                            \
@@ -324,11 +444,17 @@ __global__ void __launch_bounds__(AWG) k_adj_carry(AdjTape T, AdjGrad G, const f
 }
 
 // divergence (106-123) reversed into the face gradient before projection, then
-// normFaces (75-105) reversed into gradients of the sums V (gFN) and W (gFO)
+// normFaces (75-105) reversed into gradients of the sums V (gFN) and W (gFO); a
+// face with a solid side held the collider's surface velocity v + w x (x - c)
+// (new and old alike), whose gradient goes to that collider
 __global__ void __launch_bounds__(AWG) k_adj_faces(AdjTape T, AdjGrad G) {
   const DParams& P = *T.P;
   const AView V{P, T};
   const float k = P.h * P.h / P.dt * P.ih;
+  __shared__ float sCol[ADJ_MAX_COLS * 9];
+  if (threadIdx.x < ADJ_MAX_COLS * 9) sCol[threadIdx.x] = 0.f;
+  __syncthreads();
+  const uint32_t ncol = min(P.ncols, uint32_t(ADJ_MAX_COLS));
   FOR_CELLS(T) {
     const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
     const int3 at = d_slotCoord(P, T.slotTile, s, l);
@@ -348,11 +474,29 @@ __global__ void __launch_bounds__(AWG) k_adj_faces(AdjTape T, AdjGrad G) {
         const float gs = gn + G.gFO[f];
         gV = gs / W;
         gW = -gs * __int_as_float(T.accWfin[f]) / W;
+      } else if ((tl == 2u || th == 2u) && ncol > 0u) {
+        int32_t ci = -1;
+        if (tl == 2u) ci = d_colAtCell(P, T.tmap, T.ctype, lo.x, lo.y, lo.z);
+        if (ci < 0 && th == 2u) ci = d_colAtCell(P, T.tmap, T.ctype, at.x, at.y, at.z);
+        const float gs = gn + G.gFO[f];
+        if (ci >= 0 && ci < int32_t(ncol) && gs != 0.f) {
+          // sv_a = v_a + (w x q)_a, q = x - c: d/dv = e_a, d/dw = q x e_a, d/dc = w x e_a
+          const float3 fp = d_facePos(P, at, a);
+          const float3 q = f3sub(fp, c3(T.C->c[ci * 8 + 5])), w = c3(T.C->c[ci * 8 + 7]);
+          const float3 qe = a == 0 ? make_float3(0.f, q.z, -q.y) : (a == 1 ? make_float3(-q.z, 0.f, q.x) : make_float3(q.y, -q.x, 0.f));
+          const float3 we = a == 0 ? make_float3(0.f, w.z, -w.y) : (a == 1 ? make_float3(-w.z, 0.f, w.x) : make_float3(w.y, -w.x, 0.f));
+          float* o = sCol + ci * 9;
+          atomicAdd(&o[0], gs * we.x); atomicAdd(&o[1], gs * we.y); atomicAdd(&o[2], gs * we.z);
+          atomicAdd(&o[3 + a], gs);
+          atomicAdd(&o[6], gs * qe.x); atomicAdd(&o[7], gs * qe.y); atomicAdd(&o[8], gs * qe.z);
+        }
       }
       G.gFN[f] = gV;
       G.gFO[f] = gW;
     }
   }
+  __syncthreads();
+  if (threadIdx.x < ncol * 9u && sCol[threadIdx.x] != 0.f) atomicAdd(&G.gCol[threadIdx.x], double(sCol[threadIdx.x]));
 }
 
 // p2g (16-40) reversed: per particle, the 8 faces per axis it added to
@@ -404,6 +548,7 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFN, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFO, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C)), dim3(AWG), 0, st, G.gRhs, C);
+  hipLaunchKernelGGL(k_adj_zero, dim3(1), dim3(AWG), 0, st, (float*)G.gCol, size_t(ADJ_MAX_COLS * 9 * 2));  // doubles, all bits zero
   if (pg) hipLaunchKernelGGL(k_adj_g2p, dim3(std::min<uint32_t>(ntiles + 1u, 8192u)), dim3(AG2P_WG), 0, st, T, G);
   hipLaunchKernelGGL(k_adj_project, dim3(cg), dim3(AWG), 0, st, T, G, 1u);
   // the solve reversed: per iteration, black then red, ping-pong gPres <-> gPres2

@@ -630,7 +630,6 @@ struct Snap {
   float* pres = nullptr;
   uint32_t N = 0, scap = 0, seed = 0;
   double tm = 0;
-  bool solidDirty = true;
 };
 void freeSnap(Snap* q) {
   dfree(q->pos); dfree(q->vel); dfree(q->pid); dfree(q->tmap); dfree(q->slotTile); dfree(q->fstack); dfree(q->slotList);
@@ -643,17 +642,21 @@ struct Adj {
   Sim* s = nullptr;
   uint32_t maxSteps = 0, depth = 0, scap = 0;
   std::vector<AdjTape> tapes;
+  std::vector<uint64_t> sdfSig;   // per tape: the collider grids it was recorded with
+  std::vector<const float*> sdfAt;
+  std::vector<uint32_t> ncols;    // per tape: colliders of its substep
+  uint32_t lastCols = 0;          // colliders of the substep reversed last
   AdjGrad G{};
 };
 void freeTape(AdjTape& T) {
   dfree(T.pos); dfree(T.vel); dfree(T.pid); dfree(T.binStart); dfree(T.binIdx); dfree(T.tmap0); dfree(T.tmap); dfree(T.slotTile); dfree(T.slotList); dfree(T.ctl);
-  dfree(T.ctype); dfree(T.sten); dfree(T.accWraw); dfree(T.accVfin); dfree(T.accWfin); dfree(T.P);
+  dfree(T.ctype); dfree(T.sten); dfree(T.accWraw); dfree(T.accVfin); dfree(T.accWfin); dfree(T.P); dfree(T.C);
 }
 void freeAdj(Adj* a) {
   for (AdjTape& T : a->tapes) freeTape(T);
   AdjGrad& G = a->G;
   dfree(G.gPos); dfree(G.gVel); dfree(G.gPos2); dfree(G.gVel2); dfree(G.gFN); dfree(G.gFO); dfree(G.gPres); dfree(G.gPres2);
-  dfree(G.gRhs); dfree(G.carry); dfree(G.sorNb); dfree(G.sorCoef);
+  dfree(G.gRhs); dfree(G.carry); dfree(G.sorNb); dfree(G.sorCoef); dfree(G.gCol);
   delete a;
 }
 int allocTape(Sim* s, AdjTape& T) {
@@ -663,7 +666,7 @@ int allocTape(Sim* s, AdjTape& T) {
       (rc = dalloc(&T.binStart, size_t(s->ntiles) + 2)) || (rc = dalloc(&T.binIdx, M)) ||
       (rc = dalloc(&T.tmap, s->ntiles)) || (rc = dalloc(&T.slotTile, s->scap)) || (rc = dalloc(&T.slotList, s->scap)) ||
       (rc = dalloc(&T.ctl, 16)) || (rc = dalloc(&T.ctype, C)) || (rc = dalloc(&T.sten, C)) || (rc = dalloc(&T.accWraw, C * 3)) ||
-      (rc = dalloc(&T.accVfin, C * 3)) || (rc = dalloc(&T.accWfin, C * 3)) || (rc = dalloc(&T.P, 1)))
+      (rc = dalloc(&T.accVfin, C * 3)) || (rc = dalloc(&T.accWfin, C * 3)) || (rc = dalloc(&T.P, 1)) || (rc = dalloc(&T.C, 1)))
     return rc;
   return ST_OK;
 }
@@ -1485,7 +1488,7 @@ int st_snapshot_create(void* p, void** out) {
   D2D(q->slotTile, s->G.slotTile, s->scap); D2D(q->fstack, s->G.fstack, s->scap); D2D(q->slotList, s->G.slotList, s->scap);
   D2D(q->ctl, s->G.ctl, 16); D2D(q->ctype, s->G.ctype, C); D2D(q->pres, s->G.pres, C);
   HIPCHK(hipStreamSynchronize(s->st));
-  q->N = s->N; q->scap = s->scap; q->seed = s->seed; q->tm = s->tm; q->solidDirty = s->solidDirty;
+  q->N = s->N; q->scap = s->scap; q->seed = s->seed; q->tm = s->tm;
   *out = q;
   return ST_OK;
 }
@@ -1500,7 +1503,8 @@ int st_snapshot_restore(void* p, void* snap) {
   D2D(s->G.pos, q->pos, M); D2D(s->G.vel, q->vel, M); D2D(s->G.pid, q->pid, M); D2D(s->G.tmap, q->tmap, s->ntiles);
   D2D(s->G.slotTile, q->slotTile, s->scap); D2D(s->G.fstack, q->fstack, s->scap); D2D(s->G.slotList, q->slotList, s->scap);
   D2D(s->G.ctl, q->ctl, 16); D2D(s->G.ctype, q->ctype, C); D2D(s->G.pres, q->pres, C);
-  s->N = q->N; s->seed = q->seed; s->tm = q->tm; s->solidDirty = q->solidDirty;
+  s->N = q->N; s->seed = q->seed; s->tm = q->tm;
+  s->solidDirty = true;  // the restored cell types hold the collider bits of the records then
   s->binned = false;  // marks and bins are rebuilt from the restored positions
   s->begun = false;
   return ST_OK;
@@ -1648,7 +1652,7 @@ int st_adj_create(void* p, uint32_t max_steps, void** out) {
   if ((rc = dalloc(&G.gPos, M)) || (rc = dalloc(&G.gVel, M)) || (rc = dalloc(&G.gPos2, M)) || (rc = dalloc(&G.gVel2, M)) ||
       (rc = dalloc(&G.gFN, C * 3)) || (rc = dalloc(&G.gFO, C * 3)) || (rc = dalloc(&G.gPres, C)) || (rc = dalloc(&G.gPres2, C)) ||
       (rc = dalloc(&G.gRhs, C)) || (rc = dalloc(&G.carry, size_t(s->ntiles) * 512u)) || (rc = dalloc(&G.sorNb, C * 6)) ||
-      (rc = dalloc(&G.sorCoef, C * 6))) {
+      (rc = dalloc(&G.sorCoef, C * 6)) || (rc = dalloc(&G.gCol, size_t(ADJ_MAX_COLS) * 9))) {
     freeAdj(a);
     return rc;
   }
@@ -1672,12 +1676,18 @@ int st_adj_record(void* adj, const st_params* prm) {
   Sim* s = a->s;
   if (a->depth >= a->maxSteps) return fail(ST_ERR_STATE, "st_adj_record: every tape is in use (reverse some substeps first)");
   if (s->scap != a->scap) return fail(ST_ERR_STATE, "st_adj_record: the pool has grown since the context was made");
-  if (prm->nb_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u || s->ncTotal > prm->nfv)
-    return fail(ST_ERR_ARG, "st_adj_record: the adjoint covers closed boxes without colliders, drag or the narrow band");
+  if (prm->nb_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u)
+    return fail(ST_ERR_ARG, "st_adj_record: the adjoint covers closed boxes without drag or the narrow band");
+  const uint32_t ncol = s->ncTotal >= prm->nfv ? s->ncTotal - prm->nfv : 0u;
+  if (ncol > ADJ_MAX_COLS)
+    return fail(ST_ERR_ARG, "st_adj_record: the adjoint follows at most " + std::to_string(ADJ_MAX_COLS) + " colliders");
   HIPCHK(hipSetDevice(s->dev));
   if (a->tapes.size() <= a->depth) {
     a->tapes.emplace_back();
     std::memset(&a->tapes.back(), 0, sizeof(AdjTape));
+    a->sdfSig.push_back(0);
+    a->sdfAt.push_back(nullptr);
+    a->ncols.push_back(0);
     int rc = allocTape(s, a->tapes.back());
     if (rc) return rc;
   }
@@ -1689,6 +1699,11 @@ int st_adj_record(void* adj, const st_params* prm) {
   T.n = s->N;
   T.iters = uint32_t(s->iters);
   D2D(T.P, &s->cur->P, 1);
+  D2D(T.C, &s->cur->C, 1);
+  T.sdf = s->G.sdf;
+  a->sdfSig[a->depth] = s->sdfSig;
+  a->sdfAt[a->depth] = s->G.sdf;
+  a->ncols[a->depth] = ncol;
   for (int k = 0; k < ST_STAGE_COUNT; k++) {
     if ((rc = runStage(s, k))) return rc;
     if (k == ST_STAGE_BLOCKS) {
@@ -1723,12 +1738,31 @@ int st_adj_backward(void* adj) {
   Sim* s = a->s;
   HIPCHK(hipSetDevice(s->dev));
   AdjTape& T = a->tapes[a->depth - 1];
+  if (a->ncols[a->depth - 1] && (a->sdfSig[a->depth - 1] != s->sdfSig || a->sdfAt[a->depth - 1] != s->G.sdf))
+    return fail(ST_ERR_STATE, "st_adj_backward: the collider grids changed since the substep was recorded");
   launch_adjoint_step(s->st, T, a->G, a->scap, s->ntiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("adjoint launch: ") + hipGetErrorString(e));
   std::swap(a->G.gPos, a->G.gPos2);
   std::swap(a->G.gVel, a->G.gVel2);
+  a->lastCols = a->ncols[a->depth - 1];
   a->depth--;
+  return ST_OK;
+}
+
+static_assert(ADJ_MAX_COLS == ST_ADJ_MAX_COLS, "one collider limit for the kernels and the C ABI");
+int st_adj_col_grad(void* adj, float* g, uint32_t ncols) {
+  Adj* a = (Adj*)adj;
+  if (!a || (ncols && !g)) return fail(ST_ERR_ARG, "st_adj_col_grad: null argument");
+  if (ncols != a->lastCols)
+    return fail(ST_ERR_ARG, "st_adj_col_grad: the substep reversed last had " + std::to_string(a->lastCols) + " colliders");
+  if (!ncols) return ST_OK;
+  Sim* s = a->s;
+  HIPCHK(hipSetDevice(s->dev));
+  double h[ADJ_MAX_COLS * 9];
+  HIPCHK(hipMemcpyAsync(h, a->G.gCol, size_t(ncols) * 9 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (uint32_t q = 0; q < ncols * 9; q++) g[q] = float(h[q]);
   return ST_OK;
 }
 

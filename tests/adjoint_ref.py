@@ -7,9 +7,13 @@ divergence, relax, project, stageG2P, collideP), with the fixed-point p2g sums
 replaced by the real sums they round and every discrete choice (cell types,
 counts, tiles kept for the warm start, face weight test, speed cap, box clamp,
 wall contact and its normal) either taken from the state or, for finite
-differences, frozen from a base run.  Closed box, no colliders, drag 0, tear 0
-(the adjoint's domain).  torch autograd of it is the FP64 reference gradient;
-central differences of it pin that gradient.
+differences, frozen from a base run.  Closed box, drag 0, tear 0 (the adjoint's
+domain), with rigid colliders (RefCollider: the trilinear distance grid of
+sdfEval, gpu.js:6-30, its cells solid by colAt at cell centres, solid faces at
+the surface velocity v + w x (x - c), and the g2p push p += n (q0 - phi) of
+collideP); the centre and velocity of each collider may carry gradients.
+torch autograd of it is the FP64 reference gradient; central differences of it
+pin that gradient.
 """
 from __future__ import annotations
 
@@ -19,6 +23,72 @@ import numpy as np
 import torch
 
 T64 = torch.float64
+
+
+class RefCollider:
+    """One rigid collider of a substep: distance grid (dims, origin, cell, f32
+    data), rotation rows R (world to grid frame), centre c and velocity v (f64
+    tensors, possibly requiring grad), angular velocity w."""
+
+    def __init__(self, dims, origin, cell, data, c, v, w=(0.0, 0.0, 0.0), R=None):
+        self.dims = [int(x) for x in dims]
+        self.origin = torch.tensor(np.asarray(origin, np.float64)[:3], dtype=T64)
+        self.cell = float(cell)
+        self.data = torch.tensor(np.asarray(data, np.float32).astype(np.float64))
+        self.M = torch.tensor(np.eye(3) if R is None else np.asarray(R, np.float64).reshape(3, 3), dtype=T64)
+        self.c = c if torch.is_tensor(c) else torch.tensor(c, dtype=T64)
+        self.v = v if torch.is_tensor(v) else torch.tensor(v, dtype=T64)
+        self.w = w if torch.is_tensor(w) else torch.tensor(w, dtype=T64)
+
+    def grid(self, p):
+        return ((p - self.c) @ self.M.T - self.origin) / self.cell
+
+    def near(self, p, q0):
+        """colNear (wgsl-core.js:114-131): the grid box widened by q0"""
+        g = self.grid(p.detach())
+        lo = -1.0 - q0 / self.cell
+        hq = q0 / self.cell
+        ok = torch.ones(len(p), dtype=torch.bool)
+        for a in range(3):
+            ok &= (g[:, a] > lo) & (g[:, a] < self.dims[a] + hq)
+        return ok
+
+    def eval(self, p):
+        """sdfEval (gpu.js:6-30): (phi, unit world normal), trilinear inside the
+        grid, distance to it added outside"""
+        g = self.grid(p)
+        gc = torch.stack([g[:, a].clamp(0.0, self.dims[a] - 1.0001) for a in range(3)], 1)
+        fl = torch.floor(gc.detach())
+        f = gc - fl
+        i = fl.to(torch.int64)
+        nx, ny = self.dims[0], self.dims[1]
+        base = i[:, 0] + nx * (i[:, 1] + ny * i[:, 2])
+        s = nx * ny
+        D = self.data
+        v000, v100, v010, v110 = D[base], D[base + 1], D[base + nx], D[base + nx + 1]
+        v001, v101, v011, v111 = D[base + s], D[base + s + 1], D[base + s + nx], D[base + s + nx + 1]
+
+        def mix(a, b, t):
+            return a * (1.0 - t) + b * t
+
+        fx, fy, fz = f[:, 0], f[:, 1], f[:, 2]
+        c00, c10, c01, c11 = mix(v000, v100, fx), mix(v010, v110, fx), mix(v001, v101, fx), mix(v011, v111, fx)
+        c0, c1 = mix(c00, c10, fy), mix(c01, c11, fy)
+        phi = mix(c0, c1, fz)
+        graw = torch.stack([mix(mix(v100 - v000, v110 - v010, fy), mix(v101 - v001, v111 - v011, fy), fz),
+                            mix(c10 - c00, c11 - c01, fz), c1 - c0], 1)
+        out = g - gc
+        outside = (out.detach() ** 2).sum(1) > 0
+        l = torch.sqrt(torch.where(outside, (out * out).sum(1), torch.ones_like(phi)))
+        phi = torch.where(outside, phi + l * self.cell, phi)
+        lg = torch.sqrt((graw * graw).sum(1)).clamp(min=1e-9)
+        nl = torch.where(outside[:, None], out / l[:, None], graw / lg[:, None])
+        return phi, nl @ self.M
+
+    def surf_vel(self, x):
+        """colSurfVel (gpu.js:31-38): v + w x (x - c)"""
+        q = x - self.c
+        return self.v + torch.cross(self.w.expand_as(q), q, dim=1)
 
 
 class RefSim:
@@ -42,14 +112,10 @@ class RefSim:
         self.vmax = float(prm.get("vmax", 14.0)) or 14.0
         self.iters = max(1, int(prm.get("iters", 30)))
         self.shape = (self.nx, self.ny, self.nz)
+        self.q0 = float(np.float32(0.3 * self.h))
         I, J, K = np.meshgrid(np.arange(self.nx), np.arange(self.ny), np.arange(self.nz), indexing="ij")
+        self.centres = torch.from_numpy(np.stack([I, J, K], -1).reshape(-1, 3).astype(np.float64) + 0.5) * self.h + self.x0
         self.parity = torch.from_numpy(((I + J + K) & 1).astype(np.int64))
-        # non-wall neighbours of every cell (walls only at the box faces here)
-        nns = np.zeros(self.shape)
-        for a, n_a in enumerate(self.shape):
-            idx = (I, J, K)[a]
-            nns += (idx > 0).astype(float) + (idx < n_a - 1).astype(float)
-        self.nns = torch.from_numpy(nns)
 
     # ---- discrete state ----
     def cells(self, pos):
@@ -75,14 +141,25 @@ class RefSim:
         t = tiles.repeat_interleave(8, 0).repeat_interleave(8, 1).repeat_interleave(8, 2)
         return t[: self.nx, : self.ny, : self.nz]
 
-    def freeze(self, pos, vel, pres_prev, need_prev):
+    def solid_cells(self, cols):
+        """colAt at cell centres (solidBuild): the first collider within q0, -1 none"""
+        cid = torch.full((self.nx * self.ny * self.nz,), -1, dtype=torch.int64)
+        for k, c in enumerate(cols):
+            with torch.no_grad():
+                near = c.near(self.centres, self.q0)
+                phi, _ = c.eval(self.centres)
+            cid = torch.where((cid < 0) & near & (phi < self.q0), torch.full_like(cid, k), cid)
+        return cid.view(self.shape)
+
+    def freeze(self, pos, vel, pres_prev, need_prev, cols=()):
         """the discrete choices of one substep from its state"""
         cells = self.cells(pos)
         cnt = torch.zeros(self.shape, dtype=T64)
         cnt.index_put_((cells[:, 0], cells[:, 1], cells[:, 2]), torch.ones(len(pos), dtype=T64), accumulate=True)
         need = self.need(cells)
         kept = self.tile_cells(need & need_prev) if need_prev is not None else torch.zeros(self.shape, dtype=torch.bool)
-        return dict(cnt=cnt, fluid=cnt > 0, need=need, kept=kept)
+        cid = self.solid_cells(cols)
+        return dict(cnt=cnt, fluid=(cnt > 0) & (cid < 0), need=need, kept=kept, cid=cid)
 
     # ---- stages ----
     def _stencil(self, pos, axis):
@@ -109,11 +186,53 @@ class RefSim:
             out.append((w, flat))
         return out, fshape
 
-    def substep(self, pos, vel, pres_prev, fz, br=None):
+    def _solid_faces(self, cid, a, cols):
+        """normFaces' solid faces on axis a (a box wall or a collider cell on
+        either side) and their velocity: the surface velocity of the collider of
+        the low cell, else of the high one, else 0"""
+        n = self.shape[a]
+        pad = [(0, 0)] * 3
+        pad[a] = (1, 1)
+        cp = torch.from_numpy(np.pad(cid.numpy(), pad, constant_values=-2))
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        sl_lo[a] = slice(0, n + 1)
+        sl_hi[a] = slice(1, n + 2)
+        lo, hi = cp[tuple(sl_lo)], cp[tuple(sl_hi)]
+        solid = (lo != -1) | (hi != -1)
+        ci = torch.where(lo >= 0, lo, torch.where(hi >= 0, hi, torch.full_like(lo, -1)))
+        sv = torch.zeros(ci.shape, dtype=T64).reshape(-1)
+        cif = ci.reshape(-1)
+        for k, c in enumerate(cols):
+            idx = torch.nonzero(cif == k).flatten()
+            if len(idx) == 0:
+                continue
+            fs = list(ci.shape)
+            ii = torch.stack([idx // (fs[1] * fs[2]), (idx // fs[2]) % fs[1], idx % fs[2]], 1).to(T64)
+            off = torch.full((3,), 0.5, dtype=T64)
+            off[a] = 0.0
+            fp = self.x0 + (ii + off) * self.h
+            sv = sv.index_put((idx,), c.surf_vel(fp)[:, a])
+        return solid, sv.view(ci.shape)
+
+    def substep(self, pos, vel, pres_prev, fz, br=None, cols=()):
         """One substep at the discrete state fz; br: frozen g2p branches (else
-        taken here and returned)."""
+        taken here and returned); cols: the substep's RefColliders."""
         fluid, cnt = fz["fluid"], fz["cnt"]
+        cid = fz.get("cid")
+        if cid is None:
+            cid = torch.full(self.shape, -1, dtype=torch.int64)
+        solid = cid >= 0
         fl = fluid.to(T64)
+        # non-solid neighbours (box walls and collider cells are solid)
+        ok = (~solid).to(T64)
+        nns = torch.zeros(self.shape, dtype=T64)
+        nns[1:] += ok[:-1]
+        nns[:-1] += ok[1:]
+        nns[:, 1:] += ok[:, :-1]
+        nns[:, :-1] += ok[:, 1:]
+        nns[:, :, 1:] += ok[:, :, :-1]
+        nns[:, :, :-1] += ok[:, :, 1:]
         # p2g + normFaces
         VN, VO = [], []
         for a in range(3):
@@ -129,19 +248,13 @@ class RefSim:
             v = torch.where(ok, V / torch.where(ok, W, torch.ones_like(W)), torch.zeros_like(V)).view(fshape)
             old = v
             new = v - self.g * self.dt if a == 1 else v
-            # faces on the box walls (own-axis index 0 and n) hold 0
-            wall = torch.zeros(fshape, dtype=torch.bool)
-            sl0 = [slice(None)] * 3
-            sl1 = [slice(None)] * 3
-            sl0[a] = 0
-            sl1[a] = fshape[a] - 1
-            wall[tuple(sl0)] = True
-            wall[tuple(sl1)] = True
-            VN.append(torch.where(wall, torch.zeros_like(new), new))
-            VO.append(torch.where(wall, torch.zeros_like(old), old))
+            # faces with a solid side hold the collider's surface velocity (0 at walls)
+            sface, sv = self._solid_faces(cid, a, cols)
+            VN.append(torch.where(sface, sv, new))
+            VO.append(torch.where(sface, sv, old))
         # divergence
         div = ((VN[0][1:] - VN[0][:-1]) + (VN[1][:, 1:] - VN[1][:, :-1]) + (VN[2][:, :, 1:] - VN[2][:, :, :-1])) * self.ih
-        air = ~fluid
+        air = ~fluid & ~solid
         nb_air = torch.zeros(self.shape, dtype=torch.bool)
         nb_air[1:] |= air[:-1]
         nb_air[:-1] |= air[1:]
@@ -154,7 +267,7 @@ class RefSim:
         rhs = torch.where(fluid, (tgt - div) * self.h * self.h / self.dt, torch.zeros_like(div))
         # red-black SOR from the warm start
         p = pres_prev * (fz["kept"] & fluid).to(T64)
-        act = fluid & (self.nns > 0)
+        act = fluid & (nns > 0)
         for _ in range(self.iters):
             for colour in (0, 1):
                 q = p * fl
@@ -165,7 +278,7 @@ class RefSim:
                 s[:, :-1] = s[:, :-1] + q[:, 1:]
                 s[:, :, 1:] = s[:, :, 1:] + q[:, :, :-1]
                 s[:, :, :-1] = s[:, :, :-1] + q[:, :, 1:]
-                pg = (rhs + s) / torch.where(act, self.nns, torch.ones_like(self.nns))
+                pg = (rhs + s) / torch.where(act, nns, torch.ones_like(nns))
                 upd = act & (self.parity == colour)
                 p = torch.where(upd, p + self.omega * (pg - p), p)
         p = torch.where(act, p, torch.zeros_like(p))
@@ -173,13 +286,13 @@ class RefSim:
         pf = p * fl
         k = self.dt * self.ih
         V0 = VN[0].clone()
-        m = (fluid[1:] | fluid[:-1]).to(T64)
+        m = ((fluid[1:] | fluid[:-1]) & ~solid[1:] & ~solid[:-1]).to(T64)
         V0[1:-1] = VN[0][1:-1] - k * m * (pf[1:] - pf[:-1])
         V1 = VN[1].clone()
-        m = (fluid[:, 1:] | fluid[:, :-1]).to(T64)
+        m = ((fluid[:, 1:] | fluid[:, :-1]) & ~solid[:, 1:] & ~solid[:, :-1]).to(T64)
         V1[:, 1:-1] = VN[1][:, 1:-1] - k * m * (pf[:, 1:] - pf[:, :-1])
         V2 = VN[2].clone()
-        m = (fluid[:, :, 1:] | fluid[:, :, :-1]).to(T64)
+        m = ((fluid[:, :, 1:] | fluid[:, :, :-1]) & ~solid[:, :, 1:] & ~solid[:, :, :-1]).to(T64)
         V2[:, :, 1:-1] = VN[2][:, :, 1:-1] - k * m * (pf[:, :, 1:] - pf[:, :, :-1])
         VN = [V0, V1, V2]
         # g2p
@@ -197,42 +310,62 @@ class RefSim:
         sp = torch.linalg.norm(vm, dim=1)
         br = {} if br is None else dict(br)
         capped = br["capped"] if "capped" in br else sp.detach() > self.vmax
+        sp = torch.where(capped, sp, torch.ones_like(sp))  # no 1/|vm| where vm may vanish
         vc = torch.where(capped[:, None], vm * (self.vmax / sp)[:, None], vm)
         pt = pos + vc * self.dt
+        # collideP (wgsl-solver.js:185-197): the collider pushes in order, then the box clamp
+        q = pt
+        hits = br.get("hits")
+        new_hits = []
+        for kk, c in enumerate(cols):
+            phi, wn = c.eval(q)
+            hit = hits[kk] if hits is not None else c.near(q, self.q0) & (phi.detach() < self.q0)
+            new_hits.append(hit)
+            q = torch.where(hit[:, None], q + wn * (self.q0 - phi)[:, None], q)
+        br["hits"] = new_hits
         lo = self.x0 + self.margin
         hi = self.x1 - self.margin
-        keep = br["keep"] if "keep" in br else (pt.detach() > lo) & (pt.detach() < hi)
-        low = br["low"] if "low" in br else pt.detach() <= lo
-        pc = torch.where(keep, pt, torch.where(low, lo.expand_as(pt), hi.expand_as(pt)))
-        if "nrm" not in br:
-            dsp = (pc - pt).detach()
-            dd = (dsp * dsp).sum(1)
+        keep = br["keep"] if "keep" in br else (q.detach() > lo) & (q.detach() < hi)
+        low = br["low"] if "low" in br else q.detach() <= lo
+        pc = torch.where(keep, q, torch.where(low, lo.expand_as(q), hi.expand_as(q)))
+        # contact: v loses its component along n = (pc - pt) / |pc - pt| (n differentiated;
+        # the contact branches are the forward's)
+        dsp = pc - pt
+        if "wall" not in br:
+            dd = (dsp.detach() ** 2).sum(1)
             contact = dd > 1e-12
-            nrm = torch.where(contact[:, None], dsp / dd.clamp(min=1e-300).sqrt()[:, None], torch.zeros_like(dsp))
-            br["nrm"] = nrm
-            br["wall"] = contact & ((vc.detach() * nrm).sum(1) < 0)
+            n0 = dsp.detach() / dd.clamp(min=1e-300).sqrt()[:, None]
+            br["wall"] = contact & ((vc.detach() * n0).sum(1) < 0)
         br.update(capped=capped, keep=keep, low=low)
-        nrm, wall = br["nrm"], br["wall"]
+        wall = br["wall"]
+        dl = torch.sqrt(torch.where(wall, (dsp * dsp).sum(1), torch.ones_like(dsp[:, 0])))
+        nrm = dsp / dl[:, None]
         vd = (vc * nrm).sum(1)
         vout = torch.where(wall[:, None], vc - nrm * vd[:, None], vc)
         return pc, vout, p, br
 
-    def rollout(self, pos, vel, nsteps, frozen=None, clamps=None):
+    def rollout(self, pos, vel, nsteps, frozen=None, clamps=None, cols=None):
         """K substeps from an empty pool; returns final (pos, vel) and the frozen
         choices of every substep (pass them back to repeat the same branches).
-        clamps: per substep, g2p's box-clamp branches to take (wall_branches)."""
+        clamps: per substep, g2p's box-clamp branches to take (wall_branches);
+        cols: per substep, its list of RefColliders."""
         pres = torch.zeros(self.shape, dtype=T64)
         need_prev = None
         rec = [] if frozen is None else frozen
         for s in range(nsteps):
+            cs = cols[s] if cols else ()
             if frozen is None:
-                fz = self.freeze(pos, vel, pres, need_prev)
-                pos, vel, pres, br = self.substep(pos, vel, pres, fz, clamps[s] if clamps else None)
+                with torch.no_grad():
+                    cz = [RefCollider(c.dims, c.origin.numpy(), c.cell, c.data.numpy(), c.c.detach(), c.v.detach(),
+                                      c.w.detach(), c.M.numpy()) for c in cs]
+                fz = self.freeze(pos, vel, pres, need_prev, cz)
+                br0 = {k: v for k, v in clamps[s].items() if k in ("keep", "low")} if clamps else None
+                pos, vel, pres, br = self.substep(pos, vel, pres, fz, br0, cs)
                 rec.append((fz, br))
                 need_prev = fz["need"]
             else:
                 fz, br = frozen[s]
-                pos, vel, pres, _ = self.substep(pos, vel, pres, fz, br)
+                pos, vel, pres, _ = self.substep(pos, vel, pres, fz, br, cs)
         return pos, vel, rec
 
     def wall_branches(self, dims, pos_after):
@@ -254,3 +387,32 @@ class RefSim:
         nrm[contact] /= nn[contact, None]
         return dict(keep=torch.from_numpy(keep), low=torch.from_numpy(low), nrm=torch.from_numpy(nrm),
                     wall=torch.from_numpy(contact))
+
+
+def ref_cols(keyed, keys, nsteps, dt, t0=0.0):
+    """Per substep, the RefColliders of keyed colliders (splashtank_amd.adjoint.
+    KeyedCollider) whose key positions are f64 tensors: centre and velocity are
+    the same linear maps of the keys the rollout uses, so gradients reach them."""
+    out = []
+    for s in range(nsteps):
+        t = t0 + s * dt
+        cs = []
+        for kc, K in zip(keyed, keys):
+            _, _, wc, wv = kc.at(K.detach().numpy(), t, dt)
+            c = torch.from_numpy(wc) @ K
+            v = torch.from_numpy(wv) @ K
+            cs.append(RefCollider(kc.dims, kc.origin, kc.cell, kc.data, c, v, R=kc.R))
+        out.append(cs)
+    return out
+
+
+def keyed_sphere(dims, dt, r=0.12):
+    """A sphere collider with 4 keyframes (0, 2, 4, 6 substeps) moving down and
+    across into the dam break's water column; its distance grid has cell h."""
+    from splashtank_amd import adjoint, scenes
+
+    h = float(dims.h)
+    gd, lo, data = scenes.sphere_sdf(r, h, 2 * h)
+    kc = adjoint.KeyedCollider(gd, lo, h, data, [0.0, 2 * dt, 4 * dt, 6 * dt], smooth=True)
+    keys = np.array([[-0.28, 0.64, 0.00], [-0.28, 0.60, 0.01], [-0.27, 0.57, 0.00], [-0.25, 0.55, -0.01]])
+    return kc, keys

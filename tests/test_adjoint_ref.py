@@ -7,7 +7,7 @@ import torch
 
 from splashtank_amd import capi, scenes
 from oracle.oracle import Oracle
-from tests.adjoint_ref import RefSim
+from tests.adjoint_ref import RefSim, keyed_sphere, ref_cols
 from tests.helpers import opts_for
 
 
@@ -59,3 +59,74 @@ def test_reference_gradient_matches_central_differences():
         fd = (f(1) - f(-1)) / (2 * eps)
         an = float((pos0.grad * dp).sum() + (vel0.grad * dv).sum())
         assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
+
+
+def _oracle_with_sphere(sc, K):
+    o = Oracle(opts_for(sc), fp64=True)
+    o.upload(sc.pos, sc.vel)
+    dt = float(sc.params["dt"])
+    kc, keys = keyed_sphere(o.dims, dt)
+    prm = capi.make_params(**sc.params)
+    for s in range(K):
+        c, v, _, _ = kc.at(keys, s * dt, dt)
+        o.set_colliders(scenes.collider_record(0, kc.dims, kc.origin, kc.cell, c, vel=v), kc.data, 1)
+        o.step(prm)
+    return o, kc, keys
+
+
+def test_reference_forward_with_moving_sphere_matches_oracle_f64():
+    """Solid cells, solid-face velocities and the g2p push of a keyframed sphere
+    against the oracle's f64 substep (colAt, normFaces, collideP)."""
+    sc = _scene()
+    K = 3
+    o, kc, keys = _oracle_with_sphere(sc, K)
+    dt = float(sc.params["dt"])
+    ref = RefSim(o.dims, sc.params)
+    cols = ref_cols([kc], [torch.tensor(keys)], K, dt)
+    p, v, rec = ref.rollout(torch.tensor(sc.pos[:, :3], dtype=torch.float64),
+                            torch.tensor(sc.vel[:, :3], dtype=torch.float64), K, cols=cols)
+    hits = sum(int(br["hits"][0].sum()) for _, br in rec)
+    solid = sum(int((fz["cid"] >= 0).sum()) for fz, _ in rec)
+    assert hits > 20 and solid > 20, (hits, solid)
+    dp = np.abs(p.numpy() - o.dump("pos")[:, :3]).max()
+    dv = np.abs(v.numpy() - o.dump("vel")[:, :3]).max()
+    vs = np.abs(o.dump("vel")[:, :3]).max()
+    # the oracle rounds every p2g term to 1/65536 and the collider record to f32
+    assert dp < 1e-6, dp
+    assert dv < 2e-4 * vs, (dv, vs)
+
+
+def test_reference_key_gradient_matches_central_differences():
+    """d/d(sphere key positions) of a linear loss through the centres and
+    pose-difference velocities of 3 substeps, against central differences with
+    the discrete choices (solid cells, pushes, clamps) frozen."""
+    sc = _scene()
+    o = Oracle(opts_for(sc), fp64=True)
+    ref = RefSim(o.dims, sc.params)
+    dt = float(sc.params["dt"])
+    kc, keys = keyed_sphere(o.dims, dt)
+    K = 3
+    rng = np.random.default_rng(5)
+    n = sc.n
+    A = torch.tensor(rng.standard_normal((n, 3)))
+    B = torch.tensor(rng.standard_normal((n, 3)))
+    pos0 = torch.tensor(sc.pos[:, :3], dtype=torch.float64)
+    vel0 = torch.tensor(sc.vel[:, :3], dtype=torch.float64)
+    kt = torch.tensor(keys, requires_grad=True)
+    pK, vK, rec = ref.rollout(pos0, vel0, K, cols=ref_cols([kc], [kt], K, dt))
+    ((A * pK).sum() + (B * vK).sum()).backward()
+    g = kt.grad.numpy().copy()
+    assert np.abs(g).max() > 0
+    eps = 1e-6
+    for _ in range(4):
+        d = rng.standard_normal(keys.shape)
+
+        def f(s):
+            with torch.no_grad():
+                kk = torch.tensor(keys + s * eps * d)
+                a, b, _ = ref.rollout(pos0, vel0, K, frozen=rec, cols=ref_cols([kc], [kk], K, dt))
+                return float((A * a).sum() + (B * b).sum())
+
+        fd = (f(1) - f(-1)) / (2 * eps)
+        an = float((g * d).sum())
+        assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3), (fd, an)

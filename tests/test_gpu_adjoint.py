@@ -10,7 +10,7 @@ import torch
 from splashtank_amd import adjoint, capi, scenes
 from splashtank_amd.capi import StError
 from oracle.oracle import Oracle
-from tests.adjoint_ref import RefSim
+from tests.adjoint_ref import RefSim, keyed_sphere, ref_cols
 from tests.helpers import opts_for
 
 pytestmark = pytest.mark.gpu
@@ -35,11 +35,12 @@ def _dirs_check(ga, gb, rng, k=16):
     return float(np.abs(xs - ys).max() / max(np.sqrt(np.mean(ys * ys)), 1e-30))
 
 
-def _gpu_clamps(sc, ref, dims, pos0, vel0, prm, K):
+def _gpu_clamps(sc, ref, dims, pos0, vel0, prm, K, keyed=None):
     """The reference follows the box-clamp branches of the f32 run: a particle
     resting on a wall has a normal velocity within rounding of zero, and which
     side of the bound it lands on is decided by f32 rounding (where the
-    derivative itself jumps)."""
+    derivative itself jumps).  keyed: (KeyedCollider, keys) moved as the
+    rollout moves it."""
     g = capi.FlipSimC(opts_for(sc))
     try:
         p4 = np.zeros((len(pos0), 4), np.float32)
@@ -48,7 +49,12 @@ def _gpu_clamps(sc, ref, dims, pos0, vel0, prm, K):
         v4[:, :3] = vel0
         g.upload(p4, v4)
         out = []
-        for _ in range(K):
+        dt = float(prm.dt)
+        for s in range(K):
+            if keyed is not None:
+                kc, keys = keyed
+                c, v, _, _ = kc.at(keys, s * dt, dt)
+                g.set_colliders(scenes.collider_record(0, kc.dims, kc.origin, kc.cell, c, vel=v), kc.data, 1)
             g.step(prm)
             out.append(ref.wall_branches(dims, g.dump("pos")))
         return out
@@ -70,7 +76,7 @@ def test_adjoint_matches_fp64_reference_linear_loss():
     try:
         print("[adjoint] forward + backward, 16^3, 5 substeps, snapshots every 2", flush=True)
         _, _, snaps = ro.forward_raw(pos0, vel0, keep_snaps=True)
-        gp, gv = ro.backward_raw(snaps, A.astype(np.float32), B.astype(np.float32))
+        gp, gv, _ = ro.backward_raw(snaps, A.astype(np.float32), B.astype(np.float32))
     finally:
         ro.close()
     dims = Oracle(opts_for(sc), fp64=True).dims
@@ -100,7 +106,7 @@ def test_adjoint_checkpoint_spacing_does_not_change_the_gradient():
         ro = adjoint.Rollout(opts_for(sc), prm, 4, every=every)
         try:
             _, _, snaps = ro.forward_raw(pos0, vel0, keep_snaps=True)
-            out.append(ro.backward_raw(snaps, A, np.zeros_like(A)))
+            out.append(ro.backward_raw(snaps, A, np.zeros_like(A))[:2])
         finally:
             ro.close()
     for gp, gv in out[1:]:
@@ -174,16 +180,98 @@ def test_adjoint_density_loss_block_velocities():
     assert worst <= RTOL
 
 
+def test_adjoint_sphere_keyframes_and_block_velocities():
+    """The inverse problem with both parameter sets: per-tile initial
+    velocities and the 4 key positions of a sphere collider that moves into the
+    water (solid faces at its velocity, particles pushed out of it), density
+    loss after K substeps; GPU adjoint (st_adj_col_grad through the keyframe
+    map) against the FP64 reference gradient and central differences of the
+    FP64 forward along 16 joint directions."""
+    sc = scenes.dam_break(d=1.0 / 64, iters=20)  # 32^3 cells, 4^3 tiles
+    o = Oracle(opts_for(sc), fp64=True)
+    dims = o.dims
+    nx, ny, nz = dims.nx, dims.ny, dims.nz
+    origin = (dims.x0, dims.y0, dims.z0)
+    h = float(dims.h)
+    dt = float(sc.params["dt"])
+    rng = np.random.default_rng(4)
+    cent = [(rng.uniform(-0.3, 0.3), rng.uniform(0.1, 0.4), rng.uniform(-0.3, 0.3)) for _ in range(3)]
+    target = torch.tensor(adjoint.gaussian_target((nx, ny, nz), origin, h, cent, 4 * h, float(dims.rest)))
+    cells = np.floor((sc.pos[:, :3] - np.array(origin)) / h).astype(np.int64)
+    tx, ty = dims.tx, dims.ty
+    blk = torch.tensor((cells[:, 0] >> 3) + tx * ((cells[:, 1] >> 3) + ty * (cells[:, 2] >> 3)))
+    nb = dims.tx * dims.ty * dims.tz
+    u0 = 0.2 * rng.standard_normal((nb, 3))
+    kc, keys = keyed_sphere(dims, dt)
+    K = 6
+    prm = capi.make_params(**sc.params)
+    pos0 = torch.tensor(sc.pos[:, :3], dtype=torch.float64)
+
+    def loss(pK):
+        return ((adjoint.splat_density(pK, origin, h, (nx, ny, nz)) - target) ** 2).sum()
+
+    ro = adjoint.Rollout(opts_for(sc), prm, K, every=3, cols=[kc])
+    try:
+        print("[adjoint] sphere keyframes + tile velocities, 32^3, 6 substeps", flush=True)
+        u = torch.tensor(u0, requires_grad=True)
+        kt = torch.tensor(keys, requires_grad=True)
+        pK, _ = ro(pos0.to(torch.float32), u[blk].to(torch.float32), kt)
+        L = loss(pK.to(torch.float64))
+        L.backward()
+        gu, gk = u.grad.numpy().copy(), kt.grad.numpy().copy()
+    finally:
+        ro.close()
+    ref = RefSim(dims, sc.params)
+    vel0 = torch.tensor(u0)[blk].to(torch.float32).numpy()
+    clamps = _gpu_clamps(sc, ref, dims, sc.pos[:, :3].astype(np.float32), vel0, prm, K, keyed=(kc, keys))
+    ur = torch.tensor(u0, requires_grad=True)
+    kr = torch.tensor(keys, requires_grad=True)
+    pr, _, rec = ref.rollout(pos0, ur[blk], K, clamps=clamps, cols=ref_cols([kc], [kr], K, dt))
+    hits = sum(int(br["hits"][0].sum()) for _, br in rec)
+    Lr = loss(pr)
+    Lr.backward()
+    ru, rk = ur.grad.numpy(), kr.grad.numpy()
+    eu, ek = _rel(gu, ru), _rel(gk, rk)
+    print(f"[adjoint] {hits} pushes; loss gpu {float(L):.6e} ref {float(Lr):.6e}; "
+          f"grad rel err tiles {eu:.2e} keys {ek:.2e}", flush=True)
+    print(f"[adjoint] key gradient gpu {gk.ravel().round(4).tolist()}", flush=True)
+    print(f"[adjoint] key gradient ref {rk.ravel().round(4).tolist()}", flush=True)
+    assert hits > 100
+    assert abs(float(L) - float(Lr)) <= 1e-4 * abs(float(Lr))
+    assert eu <= RTOL and ek <= RTOL
+    eps = 1e-6
+    fds, ans = [], []
+    for _ in range(16):
+        du = rng.standard_normal((nb, 3))
+        dk = rng.standard_normal(keys.shape) * 0.05
+        with torch.no_grad():
+            a, _, _ = ref.rollout(pos0, torch.tensor(u0 + eps * du)[blk], K, frozen=rec,
+                                  cols=ref_cols([kc], [torch.tensor(keys + eps * dk)], K, dt))
+            b, _, _ = ref.rollout(pos0, torch.tensor(u0 - eps * du)[blk], K, frozen=rec,
+                                  cols=ref_cols([kc], [torch.tensor(keys - eps * dk)], K, dt))
+            fds.append((float(loss(a)) - float(loss(b))) / (2 * eps))
+        ans.append(float((gu * du).sum() + (gk * dk).sum()))
+    fds, ans = np.array(fds), np.array(ans)
+    worst = float(np.abs(ans - fds).max() / np.sqrt(np.mean(fds * fds)))
+    print(f"[adjoint] 16 joint directions against central differences: worst {worst:.2e} of the RMS", flush=True)
+    assert worst <= RTOL
+
+
 def test_adjoint_refuses_what_it_does_not_cover():
     sc = scenes.sparse_splash(d=1.0 / 64, iters=5)  # has a box collider
     g = capi.FlipSimC(opts_for(sc))
     try:
         g.upload(sc.pos, sc.vel)
         cf, sdf, sig = sc.collider_state(0.0, sc.params["dt"])
-        g.set_colliders(cf, sdf, sig)
+        g.set_colliders(np.tile(cf, 5), sdf, sig)  # five colliders: one more than the adjoint follows
         a = capi.AdjointC(g, 2)
         with pytest.raises(StError):
             a.record(capi.make_params(**sc.params))
+        g.set_colliders(cf, sdf, sig)
+        a.record(capi.make_params(**sc.params))
+        g.set_colliders(cf, sdf, sig + 1)  # new grids: the recorded substep cannot be reversed
+        with pytest.raises(StError):
+            a.backward()
         a.close()
     finally:
         g.close()
