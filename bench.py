@@ -296,6 +296,50 @@ def cpu_baseline(sc, args):
     return sc.n * args.cpu_steps / wall, threads(), wall
 
 
+REF_BAKER = os.path.join(ROOT, "oracle", "_ref", "splash")
+# the reference's native baker on a dam break of the 64^3 case's shape in its own
+# 3 x 2.6 x 3 box (fluid 0.4 x 0.6 x 1 of it): d 0.0224 gives 506,520 particles;
+# 60 fps with 4 substeps (dt 1/240), 4 density iterations
+REF_SCENE = """splash-scene 1
+d 0.0224
+fps 60
+frames {frames}
+substeps 4
+iters 4
+gravity 9.81
+maxn 600000
+fill box -1.5 -0.3 0 1.56 -1.5 1.5
+"""
+
+
+def reference_cpu(frames: int):
+    """The reference's CPU path (native/splash.cpp, PBF: a different algorithm,
+    a baseline only), built from its source into oracle/_ref by the oracle's
+    Makefile, timed by its own per-frame clock: sum of N x substeps over elapsed."""
+    import re
+    import subprocess
+    import tempfile
+    if not os.path.exists(REF_BAKER):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        scene = os.path.join(td, "scene.txt")
+        with open(scene, "w") as f:
+            f.write(REF_SCENE.format(frames=frames))
+        log(f"[bench] reference baker: {frames} frames")
+        r = subprocess.run([REF_BAKER, scene, os.devnull], capture_output=True, text=True, timeout=300)
+    thr = re.search(r"threads: (\d+)", r.stdout)
+    rows = re.findall(r"frame (\d+)/\d+\s+(\d+) points\s+([0-9.]+)s elapsed", r.stdout)
+    if r.returncode != 0 or not rows:
+        return {"error": f"exit {r.returncode}", "stdout_tail": r.stdout[-300:]}
+    work = sum(int(n) * 4 for _, n, _ in rows)
+    el = float(rows[-1][2])
+    return {"value": work / el, "unit": "particle-substeps/s", "cores": int(thr.group(1)) if thr else None,
+            "kind": "reference", "algorithm": "PBF (native/splash.cpp), not FLIP: baseline only",
+            "sample": f"dam break, {rows[0][1]} particles, {len(rows)} frames x 4 substeps, 4 density iterations; "
+                      f"its own elapsed clock ({el:.1f} s)",
+            "build": "g++ -O3 -fopenmp native/splash.cpp (oracle/Makefile target ref; -march=native left out)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +348,7 @@ def main():
     ap.add_argument("--workload", default="splash256", choices=["splash256", "dam64"])
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-warmup", type=int, default=1)
+    ap.add_argument("--ref-frames", type=int, default=12, help="frames of the reference's CPU baker to time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=-1, help="substeps timed stage by stage (default: all timed ones)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
@@ -349,6 +394,18 @@ def main():
                 "kernel": (f"{kname} (all {iters} red-black SOR iterations in one launch, {stats['sor_path']})" if coop
                            else "k_sor_pipe (one red-black SOR iteration per launch)"),
                 "algorithmic_bytes_per_launch": sor_bytes, "launch_us": launch_us}
+    p2g_atomics = None
+    if stage_ms:
+        # k_p2g_tile: per particle 24 packed 64-bit LDS adds (weight and weighted
+        # velocity of 8 faces on 3 axes) and one LDS count add; per tile at most
+        # 3 x 2 global i32 adds for each of the 10^3 window cells (non-zero sums
+        # only).  The reference makes 50 global atomics per particle (48 face
+        # sums, the count, the speed maximum).
+        p2g_s = stage_ms["p2g"] * 1e-3
+        p2g_atomics = {"reference_global_per_particle": 50, "lds_per_particle": 25,
+                       "global_per_launch_at_most": blocks * 6000,
+                       "lds_atomics_per_s": 25 * alive / p2g_s,
+                       "reference_atomics_per_s_equivalent": 50 * alive / p2g_s, "p2g_ms": stage_ms["p2g"]}
     out = {
         "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -358,6 +415,7 @@ def main():
         "stage_ms": stage_ms,
         "solve_f64": solve_f64,
         "roofline": roof,
+        "p2g_atomics": p2g_atomics,
         "substep_roofline": {"algorithmic_bytes": sub_bytes, "achieved": sub_bytes / (ms_per_step * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": sub_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -367,6 +425,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(args)
     if not args.no_cpu:
+        out["reference_cpu"] = reference_cpu(args.ref_frames)
         v, cores, cw = cpu_baseline(sc, args)
         out["cpu_baseline"] = {"value": v, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
                                "sample": f"{args.cpu_steps} substeps of the same scene after {args.cpu_warmup} warm-up, "

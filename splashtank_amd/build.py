@@ -89,8 +89,21 @@ def build_oracle() -> str:
     return os.path.join(odir, "liboracle.so")
 
 
+REF_SRC = "/root/reference/native/splash.cpp"
+
+
+def build_ref() -> str | None:
+    """The reference's CPU baker (oracle/_ref/splash), where its source is present."""
+    if not os.path.exists(REF_SRC):
+        return None
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-B", "-C", odir, "ref", f"REF_SRC={REF_SRC}"])
+    return os.path.join(odir, "_ref", "splash")
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))
     print(build_napi())
     print(build_checks())
     print(build_oracle())
+    print(build_ref())
