@@ -348,7 +348,7 @@ def main():
     ap.add_argument("--workload", default="splash256", choices=["splash256", "dam64"])
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-warmup", type=int, default=1)
-    ap.add_argument("--ref-frames", type=int, default=12, help="frames of the reference's CPU baker to time")
+    ap.add_argument("--ref-frames", type=int, default=36, help="frames of the reference's CPU baker to time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=-1, help="substeps timed stage by stage (default: all timed ones)")
     ap.add_argument("--calib-copy-mb", type=int, default=0)
