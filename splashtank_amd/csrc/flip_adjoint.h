@@ -46,7 +46,7 @@ struct AdjGrad {
   uint32_t* sorNb;  // per cell, 6 per cell: the fluid neighbour a transposed sweep gathers from
                     // (x-, x+, y-, y+, z-, z+), ST_NONE where there is none or it has no non-solid side
   float* sorCoef;   // per cell, 6 per cell: omega / (non-solid neighbours of that neighbour)
-  double* gCol;     // per collider of the substep reversed last, 9 words: dL/d(centre),
+  float* gCol;      // per collider of the substep reversed last, 9 words: dL/d(centre),
                     // dL/d(linear velocity), dL/d(angular velocity)
   uint32_t ncell; // scap * 512
   uint32_t ncarry;

@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(AG2P_WG) k_adj_g2p(AdjTape T, AdjGrad G) {
   }
   __syncthreads();
   if (threadIdx.x < min(P.ncols, uint32_t(ADJ_MAX_COLS)) * 3u && sCol[threadIdx.x] != 0.f)
-    atomicAdd(&G.gCol[(threadIdx.x / 3u) * 9u + threadIdx.x % 3u], double(sCol[threadIdx.x]));
+    atomicAdd(&G.gCol[(threadIdx.x / 3u) * 9u + threadIdx.x % 3u], sCol[threadIdx.x]);
 }
 
 #define FOR_CELLS(T)                                                                            \
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(AWG) k_adj_faces(AdjTape T, AdjGrad G) {
     }
   }
   __syncthreads();
-  if (threadIdx.x < ncol * 9u && sCol[threadIdx.x] != 0.f) atomicAdd(&G.gCol[threadIdx.x], double(sCol[threadIdx.x]));
+  if (threadIdx.x < ncol * 9u && sCol[threadIdx.x] != 0.f) atomicAdd(&G.gCol[threadIdx.x], sCol[threadIdx.x]);
 }
 
 // p2g (16-40) reversed: per particle, the 8 faces per axis it added to
@@ -548,7 +548,7 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFN, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C * 3)), dim3(AWG), 0, st, G.gFO, C * 3);
   hipLaunchKernelGGL(k_adj_zero, dim3(grid_for(C)), dim3(AWG), 0, st, G.gRhs, C);
-  hipLaunchKernelGGL(k_adj_zero, dim3(1), dim3(AWG), 0, st, (float*)G.gCol, size_t(ADJ_MAX_COLS * 9 * 2));  // doubles, all bits zero
+  hipLaunchKernelGGL(k_adj_zero, dim3(1), dim3(AWG), 0, st, G.gCol, size_t(ADJ_MAX_COLS * 9));
   if (pg) hipLaunchKernelGGL(k_adj_g2p, dim3(std::min<uint32_t>(ntiles + 1u, 8192u)), dim3(AG2P_WG), 0, st, T, G);
   hipLaunchKernelGGL(k_adj_project, dim3(cg), dim3(AWG), 0, st, T, G, 1u);
   // the solve reversed: per iteration, black then red, ping-pong gPres <-> gPres2

@@ -1759,10 +1759,8 @@ int st_adj_col_grad(void* adj, float* g, uint32_t ncols) {
   if (!ncols) return ST_OK;
   Sim* s = a->s;
   HIPCHK(hipSetDevice(s->dev));
-  double h[ADJ_MAX_COLS * 9];
-  HIPCHK(hipMemcpyAsync(h, a->G.gCol, size_t(ncols) * 9 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(g, a->G.gCol, size_t(ncols) * 9 * sizeof(float), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  for (uint32_t q = 0; q < ncols * 9; q++) g[q] = float(h[q]);
   return ST_OK;
 }
 
