@@ -275,6 +275,17 @@ typedef struct st_view_layout {
 } st_view_layout;
 int st_pack_view(void* sim, uint32_t disp_max, uint32_t* out, size_t cap_words, st_view_layout* layout);
 
+/* Deep water points of the narrow band look (setDeepLook/readDeep, sim.js:405-419;
+ * deepClear/deepPick/deepFin, wgsl-whitewater.js:279-297): the centre of every fluid
+ * cell in the domain that holds no particle after the last substep, as (x, y, z,
+ * 4096 + min(|kept grid velocity|, 63)).  With more than ST_DEEP_MAX such cells at
+ * the previous call, an even hash-chosen sample of them (the reference's rule).
+ * Points come in active-cell order (the reference's come in arrival order; compare
+ * as sets).  *n = 0 when the narrow band is off or the sim is empty.  At most
+ * min(room, ST_DEEP_MAX) points are written; *n is how many were. */
+#define ST_DEEP_MAX 200000u
+int st_read_deep(void* sim, float* out4, uint32_t room, uint32_t* n);
+
 /* buildSDF(mesh, cell, margin) (src/shared.js:98-181): the signed distance grid
  * of a closed triangle mesh, the collider grids st_set_colliders takes.  v holds
  * nv vertices (x, y, z doubles), tri nt triangles (vertex indices; the

@@ -1084,6 +1084,37 @@ template <class R> struct Sim {
     }
     sctl[0] = first + e;
   }
+  // deepClear/deepPick/deepFin (wgsl-whitewater.js:279-297): the centre of every fluid
+  // cell in the domain holding no particle, w = 4096 + min(|velG there|, 63); above
+  // 200000 such cells at the previous read, the cells with hash1(c 2654435761 + 7)
+  // prev < 200000.  In active-cell order (slot list, then cell), the first 200000.
+  uint32_t deepPrev = 0;
+  uint32_t readDeep(double* out, uint32_t room) {
+    const uint32_t DEEPMAX = 200000u;
+    if (!nbOn || !velGOn || !P.n) return 0;
+    uint32_t all = 0, k = 0;
+    for (uint32_t b = 0; b < ctl[4]; b++) {
+      const uint32_t s = slotList[b];
+      for (uint32_t local = 0; local < 512u; local++) {
+        const uint32_t c = s * 512u + local;
+        const I3 p = slotCoord(s, local);
+        if (!inDom(p) || (ctype[c] & 0xffu) != 1u || cnt[c] > 0) continue;
+        all++;
+        if (deepPrev > DEEPMAX && float(hash1(c * 2654435761u + 7u)) * float(deepPrev) >= float(DEEPMAX)) continue;
+        if (k >= DEEPMAX) continue;
+        const V3<R> x = cellCenter(p.x, p.y, p.z);
+        if (k < room) {
+          out[k * 4 + 0] = double(x.x);
+          out[k * 4 + 1] = double(x.y);
+          out[k * 4 + 2] = double(x.z);
+          out[k * 4 + 3] = double(R(4096.0) + std::min(len(velGVec(x)), R(63.0)));
+        }
+        k++;
+      }
+    }
+    deepPrev = all;
+    return std::min(k, room);
+  }
   void setDrips(const float* a, uint32_t n) {
     drip.assign(size_t(n) * 8, R(0));
     for (size_t q = 0; q < size_t(n) * 8; q++) drip[q] = R(a[q]);
@@ -1410,6 +1441,11 @@ int orc_set_colliders(void* hp, const float* cf, uint32_t nc, const float* sdf, 
 int orc_set_drips(void* hp, const float* a, uint32_t n) {
   Handle* h = (Handle*)hp;
   if (h->fp64) h->d->setDrips(a, n); else h->f->setDrips(a, n);
+  return ST_OK;
+}
+int orc_read_deep(void* hp, double* out, uint32_t room, uint32_t* n) {
+  Handle* h = (Handle*)hp;
+  *n = h->fp64 ? h->d->readDeep(out, room) : h->f->readDeep(out, room);
   return ST_OK;
 }
 int orc_set_spray_cap(void* hp, uint32_t n) {

@@ -37,6 +37,7 @@ _PROTOS = {
     "orc_read_air": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "orc_sincosf": (None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "orc_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "orc_read_deep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "orc_expf": (C.c_float, [C.c_float]),
 }
 
@@ -112,6 +113,13 @@ class Oracle(_Base):
         cell = C.c_double(0)
         self._call("read_air", self.h, a.ctypes.data, room, C.byref(n), C.byref(cell))
         return a[:n.value], cell.value
+
+    def read_deep(self, room=200000):
+        """Deep water points (x, y, z, 4096 + speed) in active-cell order, as readDeep."""
+        p = np.zeros((max(room, 1), 4), np.float64)
+        n = C.c_uint32(0)
+        self._call("read_deep", self.h, p.ctypes.data, room, C.byref(n))
+        return p[:n.value].astype(self.fdtype)
 
     def read_spray(self, room=1 << 22):
         """(positions with life in w, velocities with kind in w) of the slots handed out, f64 arrays."""

@@ -187,6 +187,7 @@ PROTOS = {
     "st_build_sdf": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.POINTER(StSdfGrid),
                                C.c_void_p, C.c_size_t]),
     "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "st_read_deep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -366,6 +367,14 @@ class FlipSimC(_Base):
         """dripUpdate: drip samples, 8 floats each (scenes.drip_samples gives them)"""
         a = np.ascontiguousarray(samples, np.float32).reshape(-1, 8)
         self._call("set_drips", self.h, a.ctypes.data if a.size else None, a.shape[0])
+
+    def read_deep(self, room=200000):
+        """readDeep: deep water points (x, y, z, 4096 + speed) of the narrow band look,
+        in active-cell order; empty when the narrow band is off."""
+        p = np.zeros((max(room, 1), 4), np.float32)
+        n = C.c_uint32(0)
+        self._call("read_deep", self.h, p.ctypes.data, room, C.byref(n))
+        return p[:n.value]
 
     def read_spray(self, room=1 << 22):
         """readSpray: (positions with life in w, velocities with kind in w) of the slots handed out."""

@@ -125,6 +125,11 @@ struct Sim {
   float4* viewDisp = nullptr;
   size_t viewWords = 0;
   uint32_t viewSd = 0;
+  // deep water points (st_read_deep): flags, offsets, scan sums and output,
+  // sized for deepCells cells; deepPrev = deep cells counted by the last read
+  uint32_t *deepFlag = nullptr, *deepOff = nullptr, *deepSums = nullptr, *deepAll = nullptr;
+  float4* deepOut = nullptr;
+  uint32_t deepCells = 0, deepPrev = 0;
   double* normPart = nullptr;     // st_solve_norms: per-workgroup partials, then the result
   uint32_t normCap = 0;           // workgroups normPart has room for
   hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
@@ -237,6 +242,7 @@ void destroySim(Sim* s) {
   dfree(s->G.air); dfree(s->G.air2); dfree(s->G.airAcc);
   dfree(s->G.drip); dfree(s->G.dripFlag); dfree(s->G.dripOff); dfree(s->G.dripSums);
   dfree(s->viewOut); dfree(s->viewFlag); dfree(s->viewOff); dfree(s->viewSums); dfree(s->viewDisp);
+  dfree(s->deepFlag); dfree(s->deepOff); dfree(s->deepSums); dfree(s->deepAll); dfree(s->deepOut);
   freeSlots(s);
   dfree(s->dSdf);
   dfree(s->dSlots);
@@ -1171,6 +1177,39 @@ int st_read_spray(void* p, float* pos4, float* vel4, uint32_t room, uint32_t* n)
   HIPCHK(hipMemcpyAsync(vel4, s->G.sVel, size_t(k) * 16, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   *n = k;
+  return ST_OK;
+}
+
+static_assert(DEEP_MAX == ST_DEEP_MAX, "one deep point limit for the kernels and the C ABI");
+int st_read_deep(void* p, float* out4, uint32_t room, uint32_t* n) {
+  Sim* s = getSim(p);
+  if (!s || !n || (room && !out4)) return fail(ST_ERR_ARG, "st_read_deep: null argument");
+  *n = 0;
+  if (!s->nbOn || !s->G.velG || !s->N || !s->cur) return ST_OK;  // readDeep gives null (sim.js:409)
+  HIPCHK(hipSetDevice(s->dev));
+  const uint32_t cells = s->scap * 512u;
+  int rc;
+  if (s->deepCells < cells || !s->deepOut) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->deepFlag); dfree(s->deepOff); dfree(s->deepSums); dfree(s->deepAll); dfree(s->deepOut);
+    s->deepCells = 0;
+    if ((rc = dalloc(&s->deepFlag, cells)) || (rc = dalloc(&s->deepOff, size_t(cells) + 1)) ||
+        (rc = dalloc(&s->deepSums, size_t(cells) / 1024 + 2)) || (rc = dalloc(&s->deepAll, 1)) ||
+        (rc = dalloc(&s->deepOut, size_t(ST_DEEP_MAX))))
+      return rc;
+    s->deepCells = cells;
+  }
+  launch_deep(s->st, s->ctx(), cells, s->deepPrev, s->deepFlag, s->deepOff, s->deepSums, s->deepAll, s->deepOut);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("st_read_deep: ") + hipGetErrorString(e));
+  uint32_t kept = 0, all = 0;
+  HIPCHK(hipMemcpyAsync(&kept, s->deepOff + cells, 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(&all, s->deepAll, 4, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->deepPrev = all;  // deepFin: this call's count picks the next call's sample
+  const uint32_t m = std::min<uint32_t>(std::min<uint32_t>(kept, ST_DEEP_MAX), room);
+  if (m) HIPCHK(hipMemcpy(out4, s->deepOut, size_t(m) * 16, hipMemcpyDeviceToHost));
+  *n = m;
   return ST_OK;
 }
 

@@ -116,6 +116,13 @@ struct ViewParams {
 // gathers the water (id order, every V.srcStride-th) and whitewater (sdMax sample slots,
 // living ones compacted) samples and packs them into `out` (V's word offsets); flag and
 // off hold sdMax (+1) words, sums sdMax / 1024 + 2, sdisp sdMax points
+// deep water points of the narrow band look (at most DEEP_MAX, the reference's
+// DEEPMAX); flags, scan and emit over the
+// ncell = slots * 512 cells of the pool; all[0] counts every deep cell, off[ncell]
+// the kept ones (at most ST_DEEP_MAX written to out)
+#define DEEP_MAX 200000u
+void launch_deep(hipStream_t st, const LaunchCtx& L, uint32_t ncell, uint32_t prev, uint32_t* flag, uint32_t* off, uint32_t* sums,
+                 uint32_t* all, float4* out);
 void launch_pack_view(hipStream_t st, const LaunchCtx& L, const ViewParams& V, uint32_t sdMax, uint32_t* flag, uint32_t* off,
                       uint32_t* sums, float4* sdisp, uint32_t* out);
 // the spray stage (wgsl-whitewater.js): the air field when `air` (airN cells),

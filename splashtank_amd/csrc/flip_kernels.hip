@@ -2992,6 +2992,50 @@ __global__ void k_pack_view(const ViewParams V, const float4* __restrict__ wat, 
   }
 }
 
+// deep water points (deepClear/deepPick/deepFin, wgsl-whitewater.js:279-297), narrow
+// band only: the centre of every fluid cell in the domain that holds no particle,
+// w = 4096 + min(|kept grid velocity there|, 63).  Above DEEP_MAX such cells
+// (counted by the previous read, `prev`) a cell is kept when hash1(c 2654435761 +
+// 7) prev < DEEP_MAX.  The points are written in active-cell order (slot list,
+// then cell) by an exclusive scan of the kept flags, where the reference appends
+// them with an atomic counter in arrival order; at most DEEP_MAX are written.
+__global__ void __launch_bounds__(WG) k_deep_flags(const DParams* __restrict__ PP, DGrid G, uint32_t prev, uint32_t* __restrict__ flag,
+                                                   uint32_t* __restrict__ all) {
+  const DParams& P = *PP;
+  const uint32_t tot = G.ctl[4] * 512u;  // a multiple of the block size: whole waves below
+  for (uint32_t t = blockIdx.x * WG + threadIdx.x; t < tot; t += gridDim.x * WG) {
+    const uint32_t s = G.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
+    const int3 at = d_slotCoord(P, G.slotTile, s, l);
+    const bool deep = d_inDom(P, at) && (G.ctype[c] & 0xffu) == 1u && G.cnt[c] <= 0;
+    bool keep = deep;
+    if (deep && prev > DEEP_MAX && d_hash1(c * 2654435761u + 7u) * float(prev) >= float(DEEP_MAX)) keep = false;
+    flag[t] = keep ? 1u : 0u;
+    const uint64_t m = __ballot(deep);
+    if ((threadIdx.x & 63u) == 0u && m) atomicAdd(all, uint32_t(__popcll(m)));
+  }
+}
+__global__ void __launch_bounds__(WG) k_deep_emit(const DParams* __restrict__ PP, DGrid G, const uint32_t* __restrict__ flag,
+                                                  const uint32_t* __restrict__ off, float4* __restrict__ out) {
+  const DParams& P = *PP;
+  const uint32_t tot = G.ctl[4] * 512u;
+  for (uint32_t t = blockIdx.x * WG + threadIdx.x; t < tot; t += gridDim.x * WG) {
+    if (!flag[t] || off[t] >= DEEP_MAX) continue;
+    const uint32_t s = G.slotList[t >> 9], l = t & 511u;
+    const int3 at = d_slotCoord(P, G.slotTile, s, l);
+    const float3 x = d_cellCenter(P, at.x, at.y, at.z);
+    out[off[t]] = make_float4(x.x, x.y, x.z, 4096.0f + fminf(f3len(d_velGVec(P, G, x)), 63.0f));
+  }
+}
+void launch_deep(hipStream_t st, const LaunchCtx& L, uint32_t ncell, uint32_t prev, uint32_t* flag, uint32_t* off, uint32_t* sums,
+                 uint32_t* all, float4* out) {
+  const uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(gridFor(ncell, WG), 8192u));
+  launch_fill_u32(st, all, 1, 0u);
+  launch_fill_u32(st, flag, ncell, 0u);  // cells past the slots in use stay unflagged
+  hipLaunchKernelGGL(k_deep_flags, dim3(g), dim3(WG), 0, st, L.P, L.G, prev, flag, all);
+  launch_scan(st, flag, off, ncell, sums, nullptr);
+  hipLaunchKernelGGL(k_deep_emit, dim3(g), dim3(WG), 0, st, L.P, L.G, flag, off, out);
+}
+
 void launch_pack_view(hipStream_t st, const LaunchCtx& L, const ViewParams& V, uint32_t sdMax, uint32_t* flag, uint32_t* off,
                       uint32_t* sums, float4* sdisp, uint32_t* out) {
   if (V.n) launch_unsort(st, L.G.pos, L.G.pid, L.G.scratch4, V.n);

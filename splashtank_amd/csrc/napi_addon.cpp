@@ -426,6 +426,27 @@ napi_value SetSprayCap(napi_env env, napi_callback_info info) {
 }
 
 // readSpray(h, room) -> {n, p: Float32Array(4n), v: Float32Array(4n)} or null without a pool
+napi_value ReadDeep(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  std::vector<float> p(size_t(ST_DEEP_MAX) * 4);
+  uint32_t n = 0;
+  ST_OK_OR_THROW(st_read_deep(handleOf(env, argv[0]), p.data(), ST_DEEP_MAX, &n));
+  napi_value out;
+  if (!n) {
+    napi_get_null(env, &out);
+    return out;
+  }
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "n", mkNum(env, double(n)));
+  void* data = nullptr;
+  napi_value ab, ta;
+  napi_create_arraybuffer(env, size_t(n) * 16, &data, &ab);
+  std::memcpy(data, p.data(), size_t(n) * 16);
+  napi_create_typedarray(env, napi_float32_array, size_t(n) * 4, ab, 0, &ta);
+  napi_set_named_property(env, out, "p", ta);
+  return out;
+}
 napi_value ReadSpray(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   getArgs(env, info, 2, argv);
@@ -533,7 +554,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;

@@ -313,8 +313,10 @@ class FlipSim {
   setSprayCap(n) { n = Math.max(0, n | 0); loadAddon().setSprayCap(this.handle, n); this.sprayCap = n; }
   // every whitewater slot handed out so far: positions with life in w, velocities with kind in w (sim.js:423-437)
   async readSpray() { return this.sprayCap ? loadAddon().readSpray(this.handle, this.sprayCap) : null; }
-  setDeepLook() {}
-  async readDeep() { return null; }
+  // narrow band look only (sim.js:405-419): deep water cells (water on the grid with no particles) as points
+  setDeepLook(on) { this.deepOn = !!on; }
+  // the deep water points of the last step (x, y, z, 4096 + speed), in active-cell order, or null
+  async readDeep() { return this.deepOn && this.handle && this.N ? loadAddon().readDeep(this.handle) : null; }
   // the live view (sim.js:504-525): packed on the GPU (st_pack_view), here read back at once; outFrame is a
   // frame in the cache format (unpackLive) that the drawing uploads as it is
   pump() {
