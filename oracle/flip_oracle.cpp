@@ -843,8 +843,8 @@ template <class R> struct Sim {
     }
     return clampr(m - R(1.0), R(-7.9), R(-0.5));
   }
-  void stageNarrowBand() {
-    forActive([&](uint32_t c, I3 p) {  // nbPhi 39-49
+  void nbPhi() {  // 39-49
+    forActive([&](uint32_t c, I3 p) {
       if (!inDom(p)) { rhs[c] = R(8.0); return; }
       if (colBits(c) != 0u) { rhs[c] = R(0.5); return; }
       V3<R> x = cellCenter(p.x, p.y, p.z);
@@ -852,9 +852,14 @@ template <class R> struct Sim {
       bool inside = cnt[c] > 0 || phA < R(-1.0);
       rhs[c] = inside ? clampr(phA, R(-7.9), R(-0.5)) : clampr(phA, R(0.5), R(7.9));
     });
-    forActive([&](uint32_t c, I3 p) { phiPut(c, jacNb(p, rhs[c], true)); });     // nbJacA
-    forActive([&](uint32_t c, I3 p) { rhs[c] = jacNb(p, phiOf(c), false); });    // nbJacB
-    forActive([&](uint32_t c, I3 p) { phiPut(c, jacNb(p, rhs[c], true)); });     // nbJacA
+  }
+  void nbJacA() { forActive([&](uint32_t c, I3 p) { phiPut(c, jacNb(p, rhs[c], true)); }); }   // 65-66
+  void nbJacB() { forActive([&](uint32_t c, I3 p) { rhs[c] = jacNb(p, phiOf(c), false); }); }  // 67-68
+  void stageNarrowBand() {  // the 'narrow band' stage (sim.js:281-283)
+    nbPhi();
+    nbJacA();
+    nbJacB();
+    nbJacA();
   }
   static R hash1(uint32_t n0) {  // wgsl-solver.js:181-184
     uint32_t x = n0;
@@ -1127,8 +1132,8 @@ template <class R> struct Sim {
     sVel.assign(size_t(n) * 4, R(0));
     sAux.assign(size_t(n) * 4, R(0));
   }
-  void stageReseed() {  // nbKill .. nbSpawn (wgsl-narrowband.js:93-149)
-    for (uint32_t i = 0; i < P.n; i++) {  // nbKill
+  void nbKill() {  // wgsl-narrowband.js:93-104
+    for (uint32_t i = 0; i < P.n; i++) {
       if (pos[i * 4 + 1] < R(-1.0e5)) continue;
       bool kill = false;
       V3<R> v = v3(vel[i * 4], vel[i * 4 + 1], vel[i * 4 + 2]);
@@ -1141,6 +1146,9 @@ template <class R> struct Sim {
         vel[i * 4] = R(0); vel[i * 4 + 1] = R(0); vel[i * 4 + 2] = R(0); vel[i * 4 + 3] = R(0);
       }
     }
+  }
+  void stageReseed() {  // nbKill .. nbSpawn (wgsl-narrowband.js:93-149)
+    nbKill();
     ctl[7] = ctl[8] = ctl[9] = ctl[10] = 0;  // nbReset
     for (uint32_t i = 0; i < P.n; i++) {      // nbDead, in index order
       if (pos[i * 4 + 1] < R(-1.0e5)) { uint32_t k = ctl[8]++; if (k < lcap) lst[k] = i; }
@@ -1325,6 +1333,47 @@ template <class R> struct Sim {
     }
     return 0;
   }
+  // test hook: the inverse of dump for the per-cell fields (same canonical order)
+  int poke(int what, const void* src, size_t bytes) {
+    if (bytes < dumpSize(what)) return ST_ERR_ARG;
+    std::vector<uint32_t> ts = tilesWithStorage();
+    const char* in = (const char*)src;
+    auto perCell = [&](void* dst, size_t esz, size_t per) {
+      for (size_t q = 0; q < ts.size(); q++) {
+        uint32_t s = tmap[ts[q]];
+        std::memcpy((char*)dst + size_t(s) * 512 * per * esz, in + q * 512 * per * esz, 512 * per * esz);
+      }
+    };
+    switch (what) {
+      case ST_DUMP_CNT: perCell(cnt.data(), 4, 1); break;
+      case ST_DUMP_CTYPE: perCell(ctype.data(), 4, 1); break;
+      case ST_DUMP_PRES: perCell(pres.data(), sizeof(R), 1); break;
+      case ST_DUMP_RHS: perCell(rhs.data(), sizeof(R), 1); break;
+      case ST_DUMP_ACCV: perCell(accVi.data(), 4, 3); break;
+      case ST_DUMP_ACCW: perCell(accWi.data(), 4, 3); break;
+      case ST_DUMP_FACEV: if constexpr (kF32) perCell(accVi.data(), 4, 3); else perCell(faceV.data(), sizeof(R), 3); break;
+      case ST_DUMP_FACEO: if constexpr (kF32) perCell(accWi.data(), 4, 3); else perCell(faceO.data(), sizeof(R), 3); break;
+      case ST_DUMP_VELG: perCell(velG.data(), sizeof(R), 3); break;
+      default: return ST_ERR_ARG;
+    }
+    return ST_OK;
+  }
+  // test hook: one kernel of the narrow band on the current state
+  int kernel(int which) {
+    switch (which) {
+      case 0: phiInitAll(); break;
+      case 1: nbPhi(); break;
+      case 2: nbJacA(); break;
+      case 3: nbJacB(); break;
+      case 4: nbVel(); break;
+      case 5: saveVel(); break;
+      case 6: nbKill(); break;
+      case 7: stageReseed(); break;
+      case 8: stageNarrowBand(); break;
+      default: return ST_ERR_ARG;
+    }
+    return ST_OK;
+  }
   int dump(int what, void* dst, size_t bytes) const {
     if (bytes < dumpSize(what)) return ST_ERR_ARG;
     std::vector<uint32_t> ts = tilesWithStorage();
@@ -1496,6 +1545,8 @@ int orc_read_ctl(void* hp, uint32_t ctl[16]) {
 }
 int orc_dump_size(void* hp, int32_t what, size_t* bytes) { Handle* h = (Handle*)hp; *bytes = ORC_DISPATCH(h, dumpSize(what)); return ST_OK; }
 int orc_dump(void* hp, int32_t what, void* dst, size_t bytes) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, dump(what, dst, bytes)); }
+int orc_poke(void* hp, int32_t what, const void* src, size_t bytes) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, poke(what, src, bytes)); }
+int orc_kernel(void* hp, int32_t which) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, kernel(which)); }
 float orc_expf(float x) { return st_expf(x); }
 int orc_threads(void) {
 #ifdef _OPENMP

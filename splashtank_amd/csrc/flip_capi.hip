@@ -60,6 +60,10 @@ struct StepGraph {
   bool binned, solid;
   int iters;
   uint32_t N, scap, pipe;
+  // which whitewater kernels the spray stage launches, and over how many
+  // drip samples and air cells (host flags of the step, not device state)
+  bool airOn, sprOn, dripOn;
+  uint32_t ndrip, airN, sprayCap;
   hipGraphExec_t exec;
 };
 const size_t GRAPH_CACHE = 4;
@@ -513,7 +517,9 @@ int graphStep(Sim* s, const st_params* o) {
   const uint32_t pipe = s->ctx().pipeGrid;
   for (StepGraph& g : s->graphs) {
     if (std::memcmp(&g.G, &s->G, sizeof(DGrid)) != 0 || g.binned != s->binned || g.solid != s->solidDirty ||
-        g.iters != s->iters || g.N != s->N || g.scap != s->scap || g.pipe != pipe)
+        g.iters != s->iters || g.N != s->N || g.scap != s->scap || g.pipe != pipe || g.airOn != s->airOnStep ||
+        g.sprOn != s->sprOnStep || g.dripOn != s->dripOnStep || g.ndrip != s->ndrip || g.airN != s->airN ||
+        g.sprayCap != s->sprayCap)
       continue;
     HIPCHK(hipGraphLaunch(g.exec, s->st));
     // what runStage does on the host for these stages
@@ -533,6 +539,12 @@ int graphStep(Sim* s, const st_params* o) {
   e.N = s->N;
   e.scap = s->scap;
   e.pipe = pipe;
+  e.airOn = s->airOnStep;
+  e.sprOn = s->sprOnStep;
+  e.dripOn = s->dripOnStep;
+  e.ndrip = s->ndrip;
+  e.airN = s->airN;
+  e.sprayCap = s->sprayCap;
   e.exec = nullptr;
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
   for (int st = 0; st < ST_STAGE_COUNT && !rc; st++) rc = runStage(s, st);
@@ -1111,7 +1123,14 @@ int st_sync(void* p) {
   HIPCHK(hipStreamSynchronize(s->st));
   uint32_t timeouts = 0;
   HIPCHK(hipMemcpy(&timeouts, s->G.sorBar + COOP_MAX_WG + 1, 4, hipMemcpyDeviceToHost));
-  if (timeouts) return fail(ST_ERR_HIP, "resident SOR: a grid barrier timed out (pressure of that solve is not valid)");
+  if (timeouts) {
+    // reported once: the counter starts again from zero, and later solves take
+    // the per-iteration launches, which have no waits between workgroups
+    HIPCHK(hipMemset(s->G.sorBar + COOP_MAX_WG + 1, 0, 4));
+    s->coopGrid = 0;
+    return fail(ST_ERR_HIP, "resident SOR: a wait between workgroups timed out (pressure of that solve is not valid); "
+                            "later solves use the per-iteration path");
+  }
   return ST_OK;
 }
 

@@ -145,7 +145,10 @@ class Scene:
             p1 = np.asarray(c["path"](t), np.float64)
             p0 = np.asarray(c["path"](t - dt), np.float64)
             v = (p1 - p0) / dt
-            cfs.append(collider_record(off, c["dims"], c["origin"], c["cell"], p1, vel=v))
+            # optional rotation: R(t) (local -> world) and a constant angular velocity
+            R = c["rot"](t) if "rot" in c else None
+            w = c.get("omega", (0.0, 0.0, 0.0))
+            cfs.append(collider_record(off, c["dims"], c["origin"], c["cell"], p1, R=R, vel=v, w=w))
             grids.append(c["data"])
             off += c["data"].size
         return np.concatenate(cfs), np.concatenate(grids), 1
@@ -181,6 +184,42 @@ def sparse_splash(d=1.0 / 512, seed=2, pool=0.25, sphere_r=0.12, sphere_y=0.7, s
                path=lambda t, bs=bs: bs + np.array([box_speed * t, 0.0, 0.0]))
     prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0)
     return Scene("sparse_splash", d, (1.0, 1.0, 1.0), 0, pos, vel, prm, colliders=[col])
+
+
+def axis_rotation(axis, angle):
+    """Rotation matrix (local -> world) by `angle` radians about the unit `axis`
+    (Rodrigues' formula)."""
+    k = np.asarray(axis, np.float64)
+    k = k / np.linalg.norm(k)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * (K @ K)
+
+
+def spinning_colliders(d=1.0 / 64, seed=5, pool=0.3, iters=20, dt=1.0 / 240, spin=3.0):
+    """A pool stirred by two overlapping colliders: a flat box tilted off the axes
+    and spinning about a tilted axis at `spin` rad/s while it moves down and
+    sideways, and a sphere grid (turned 40 degrees about x) sinking through the
+    box's end.  Exercises the collider-local transform, the spin term of the
+    surface velocity, the first-hit order of the cell lookup and the sequential
+    pushes of two colliders."""
+    rng = np.random.default_rng(seed)
+    pos, vel = fill_box(d, -0.5, 0.5, 0.0, pool, -0.5, 0.5, rng)
+    h = 2 * d
+    half = np.array([0.16, 0.05, 0.1])
+    dims, lo, data = box_sdf(half, h / 2, 4 * h / 2)
+    ax = np.array([0.3, 1.0, 0.2])
+    ax = ax / np.linalg.norm(ax)
+    R0 = axis_rotation((0, 0, 1), math.radians(25))
+    box = dict(dims=dims, origin=lo, cell=h / 2, data=data,
+               path=lambda t: np.array([-0.05 + 0.4 * t, 0.34 - 0.6 * t, 0.02]),
+               rot=lambda t: axis_rotation(ax, spin * t) @ R0, omega=tuple(spin * ax))
+    sd, slo, sdata = sphere_sdf(0.09, h / 2, 4 * h / 2)
+    Rs = axis_rotation((1, 0, 0), math.radians(40))
+    sph = dict(dims=sd, origin=slo, cell=h / 2, data=sdata,
+               path=lambda t: np.array([0.12 + 0.4 * t, 0.36 - 0.9 * t, 0.04]),
+               rot=lambda t: Rs, omega=(0.0, 0.0, 1.5))
+    prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0)
+    return Scene("spinning_colliders", d, (1.0, 1.0, 1.0), 0, pos, vel, prm, colliders=[box, sph])
 
 
 def nb_seed(d, h, size, fills, band=3, rng=None, cap=None):

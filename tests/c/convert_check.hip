@@ -33,15 +33,18 @@ extern "C" int st_check_conversions(unsigned long long* mismatches, uint32_t* fi
   uint32_t* dFirst = nullptr;
   const unsigned long long zero = 0;
   const uint32_t none = 0xffffffffu;
-  if (hipMalloc((void**)&dBad, sizeof *dBad) != hipSuccess) return 1;
-  if (hipMalloc((void**)&dFirst, sizeof *dFirst) != hipSuccess) return 1;
-  if (hipMemcpy(dBad, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) return 1;
-  if (hipMemcpy(dFirst, &none, sizeof none, hipMemcpyHostToDevice) != hipSuccess) return 1;
+  int rc = 1;
+  if (hipMalloc((void**)&dBad, sizeof *dBad) != hipSuccess) goto done;
+  if (hipMalloc((void**)&dFirst, sizeof *dFirst) != hipSuccess) goto done;
+  if (hipMemcpy(dBad, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) goto done;
+  if (hipMemcpy(dFirst, &none, sizeof none, hipMemcpyHostToDevice) != hipSuccess) goto done;
   hipLaunchKernelGGL(k_convert_check, dim3(4096), dim3(256), 0, 0, dBad, dFirst);
-  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return 1;
-  if (hipMemcpy(mismatches, dBad, sizeof *dBad, hipMemcpyDeviceToHost) != hipSuccess) return 1;
-  if (hipMemcpy(first_mismatch, dFirst, sizeof *dFirst, hipMemcpyDeviceToHost) != hipSuccess) return 1;
-  hipFree(dBad);
-  hipFree(dFirst);
-  return 0;
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) goto done;
+  if (hipMemcpy(mismatches, dBad, sizeof *dBad, hipMemcpyDeviceToHost) != hipSuccess) goto done;
+  if (hipMemcpy(first_mismatch, dFirst, sizeof *dFirst, hipMemcpyDeviceToHost) != hipSuccess) goto done;
+  rc = 0;
+done:  // every exit frees what was allocated
+  if (dBad) (void)hipFree(dBad);
+  if (dFirst) (void)hipFree(dFirst);
+  return rc;
 }

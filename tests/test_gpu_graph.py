@@ -113,3 +113,59 @@ def test_batch_graph_replay_matches_eager():
         batch.close()
         for s in solo + bt:
             s.close()
+
+
+def test_graph_replay_follows_whitewater_switches():
+    """Replays must follow the spray, air and drip switches of each substep and the
+    number of drip samples: a graph captured with one setting is never replayed
+    for another.  Eager and replayed sims get the same sequence of settings
+    (spray on, off, drips on, fewer drip samples) and must agree bit for bit,
+    whitewater slots included."""
+    from tests.test_gpu_spray import SPR
+    from tests.test_oracle_pins import lattice
+    d = 1 / 64
+    pos, vel = lattice(d, (-0.5, 0.0, -0.5), (0.5, 0.25, 0.5))
+    opts = capi.make_opts(d, (1, 1, 1), max_n=len(pos))
+    h = 2 * d
+    dims, lo, data = scenes.box_sdf((0.1,) * 3, h, 4 * h)
+    samples = scenes.drip_samples(dict(id=1, dims=dims, origin=lo, cell=h, data=data), h, 0)
+    a, b = FlipSimC(opts), FlipSimC(opts)
+    for s in (a, b):
+        s.upload(pos, vel)
+        s.set_spray_cap(1 << 14)
+        s.set_drips(samples)
+    base = dict(dt=1 / 120, g=9.81, iters=12)
+    plans = [("spray+air", dict(base, **SPR)),
+             ("off", dict(base)),
+             ("spray+drips", dict(base, **dict(SPR, spr_rate=500.0, drip_on=1, drip_rate=40.0, drip_dry=1.0))),
+             ("fewer drips", None),
+             ("spray+air again", dict(base, **SPR))]
+    path = lambda t: np.array([0.0, 0.15 + 1.2 * t, 0.0])  # noqa: E731  (a box rising out of the pool)
+    t = 0.0
+    last = None
+    for name, kw in plans:
+        if kw is None:  # fewer samples, same allocation: the replay must not scan the old count
+            for s in (a, b):
+                s.set_drips(samples[: len(samples) // 2])
+            kw = last
+        last = kw
+        prm = capi.make_params(**kw)
+        print(f"[graph] whitewater plan '{name}'", flush=True)
+        for _ in range(4):
+            t += 1 / 120
+            p1, p0 = path(t), path(t - 1 / 120)
+            cf = scenes.collider_record(0, dims, lo, h, p1, vel=(p1 - p0) * 120)
+            for s in (a, b):
+                s.set_colliders(cf, data, 1)
+            a.step(prm)
+            b.step_n(prm, 1, graph=True)
+        a.sync()
+        b.sync()
+        _same_bits(a, b, f"plan {name}")
+        pa, va = a.read_spray()
+        pb, vb = b.read_spray()
+        assert pa.shape == pb.shape, f"plan {name}: {pa.shape} vs {pb.shape} whitewater slots"
+        assert_int_equal(pa.view(np.uint32), pb.view(np.uint32), f"plan {name} whitewater positions")
+        assert_int_equal(va.view(np.uint32), vb.view(np.uint32), f"plan {name} whitewater velocities")
+    a.close()
+    b.close()
