@@ -119,7 +119,35 @@ typedef struct st_params {
   uint32_t drip_on;     /* o.drip.on: wet colliders drip (with spr_on and drip samples set) */
   float drip_rate;      /* o.drip.rate: drops per second per sample when fully wet */
   float drip_dry;       /* o.drip.dry: seconds a sample takes to dry */
+  /* ocean side boundary (o.ocean and o.time; sim.js:184-218, words 57-66 and
+   * 148 of the parameter block).  ocean_on turns the narrow band on too, with
+   * nb_band as its width (sim.js:185-186).  The wave list is the one
+   * st_ocean_waves gives for the ocean's settings, gravity and depth
+   * ocean_level - y0. */
+  uint32_t ocean_on;
+  uint32_t nwave;       /* waves in `waves` (at most ST_OCEAN_MAX_WAVES) */
+  float ocean_time;     /* o.time: seconds since the start */
+  float ocean_level;    /* o.ocean.level: height of the still water */
+  float ocean_layer;    /* o.ocean.layer: width of the boundary band at the side walls */
+  float waves[8 * 32];  /* per wave: kx, kz, amplitude, angular frequency, phase, wavenumber, 0, 0 */
 } st_params;
+
+#define ST_OCEAN_MAX_WAVES 32u
+
+/* The ocean's settings (o.ocean, src/app/*: height, length, dir, spread, seed). */
+typedef struct st_ocean {
+  double height;  /* significant wave height */
+  double length;  /* peak wavelength */
+  double dir;     /* main direction, degrees */
+  double spread;  /* directional spread, degrees */
+  int32_t seed;
+} st_ocean;
+/* oceanWaves(oc, g, depth) (src/shared.js:769-791): 24 linear waves around the
+ * peak wavelength with a Pierson-Moskowitz spectrum scaled to the significant
+ * height, spread around the direction, each with the finite-depth dispersion
+ * w^2 = g k tanh(k depth).  Writes 8 floats per wave (the layout of
+ * st_params.waves) for at most `cap` waves and sets *n to 24.  Host code. */
+int st_ocean_waves(const st_ocean* oc, double g, double depth, float* w, uint32_t cap, uint32_t* n);
 
 #define ST_SOR_TUNED 3u
 

@@ -82,6 +82,39 @@ ST_HD void st_sincosf(float x, float* sn, float* cs) {
   *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
 }
 
+/* sin and cos of x in f32 for |x| < 1.6e6 (ocean wave phases grow with time).
+ * The reduction x = k*pi/2 + r is done in f64 with pi/2 split into
+ * hi = 6746518852 / 2^32 (33 significant bits, so k * hi is exact for
+ * |k| < 2^20) and lo = pi/2 - hi; r rounded to f32 goes through the same
+ * Taylor polynomials as st_sincosf. */
+ST_HD void st_sincosf_wide(float x, float* sn, float* cs) {
+  const double two_over_pi = 0.63661977236758134308;
+  const double pio2_hi = 1.5707963267341256142;  /* 6746518852 / 2^32 */
+  const double pio2_lo = 6.077100506506192e-11;
+  double xd = (double)x;
+  double kd = rint(xd * two_over_pi);
+  double rd = xd - kd * pio2_hi;
+  rd = rd - kd * pio2_lo;
+  float r = (float)rd;
+  float z = r * r;
+  float sp = -2.5052108385441718e-8f;
+  sp = sp * z + 2.7557319223985893e-6f;
+  sp = sp * z - 1.9841269841269841e-4f;
+  sp = sp * z + 8.3333333333333332e-3f;
+  sp = sp * z - 1.6666666666666666e-1f;
+  float s = r + r * (z * sp);
+  float cp = 2.0876756987868099e-9f;
+  cp = cp * z - 2.7557319223985888e-7f;
+  cp = cp * z + 2.4801587301587302e-5f;
+  cp = cp * z - 1.3888888888888889e-3f;
+  cp = cp * z + 4.1666666666666664e-2f;
+  cp = cp * z - 0.5f;
+  float c = 1.0f + z * cp;
+  int q = (int)(((long long)kd) & 3);
+  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
 ST_HD float st_mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 ST_HD float st_clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 /* WGSL smoothstep: t = clamp((x - lo) / (hi - lo), 0, 1); t * t * (3 - 2 t) */

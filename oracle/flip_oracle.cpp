@@ -16,7 +16,9 @@
 // set of tiles with storage and every per-cell value do not depend on that order
 // when the pool does not overflow.
 //
-// Ocean waves (P.ocean) are not restated: the ocean branches are taken as off.
+// The ocean side boundary (oceanEta/oceanVel/layerW, wgsl-core.js:201-226) and its
+// branches in classify, normBody, g2p, the narrow band and the whitewater are
+// restated with the rest.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -40,6 +42,7 @@ template <class R> struct M;
 template <> struct M<float> {
   static float exp(float x) { return st_expf(x); }
   static void sincos(float x, float* sn, float* cs) { st_sincosf(x, sn, cs); }
+  static void sincosWide(float x, float* sn, float* cs) { st_sincosf_wide(x, sn, cs); }
   static float rnd(float x) { return rintf(x); }
   static float flo(float x) { return floorf(x); }
   static float sq(float x) { return sqrtf(x); }
@@ -48,6 +51,7 @@ template <> struct M<float> {
 template <> struct M<double> {
   static double exp(double x) { return std::exp(x); }
   static void sincos(double x, double* sn, double* cs) { *sn = std::sin(x); *cs = std::cos(x); }
+  static void sincosWide(double x, double* sn, double* cs) { *sn = std::sin(x); *cs = std::cos(x); }
   static double rnd(double x) { return std::nearbyint(x); }
   static double flo(double x) { return std::floor(x); }
   static double sq(double x) { return std::sqrt(x); }
@@ -116,6 +120,10 @@ template <class R> struct Params {
   R airBlur;
   uint32_t ndrip;
   R dripRate, dripDry;
+  // ocean (wgsl-core.js:12-13, 19; sim.js:207-218)
+  uint32_t nwave;
+  R otime, olevel, olayer, odepth, oy0;
+  R waves[64][4];
 };
 
 template <class R> struct Sim {
@@ -398,6 +406,42 @@ template <class R> struct Sim {
   }
   V3<R> velGVec(V3<R> x) const { return v3(velGAxis(x, 0), velGAxis(x, 1), velGAxis(x, 2)); }
   bool calmV(V3<R> v) const { return len(v) < R(9.0) * P.h; }
+  R oceanEta(R x, R z) const {  // wgsl-core.js:202-206
+    R e = R(0);
+    for (uint32_t k = 0; k < P.nwave; k++) {
+      const R* A = P.waves[k * 2];
+      const R* B = P.waves[k * 2 + 1];
+      R sn, cs;
+      M<R>::sincosWide(A[0] * x + A[1] * z - A[3] * P.otime + B[0], &sn, &cs);
+      e = e + A[2] * cs;
+    }
+    return P.olevel + e;
+  }
+  V3<R> oceanVel(V3<R> p) const {  // wgsl-core.js:208-220
+    const R D = fmx<R>(P.odepth, R(1.0e-3));
+    const R eta = oceanEta(p.x, p.z);
+    const R ys = clampr((p.y - P.oy0) * D / fmx<R>(D + eta - P.olevel, R(0.1) * D), R(0), D);
+    V3<R> v = v3(R(0), R(0), R(0));
+    for (uint32_t k = 0; k < P.nwave; k++) {
+      const R* A = P.waves[k * 2];
+      const R* B = P.waves[k * 2 + 1];
+      const R kk = fmx<R>(B[1], R(1.0e-4));
+      R sn, cs;
+      M<R>::sincosWide(A[0] * p.x + A[1] * p.z - A[3] * P.otime + B[0], &sn, &cs);
+      const R e1 = M<R>::exp(kk * (ys - D));
+      const R e2 = M<R>::exp(-kk * (ys + D));
+      const R dn = fmx<R>(R(1) - M<R>::exp(R(-2.0) * kk * D), R(1.0e-4));
+      const R aw = A[2] * A[3];
+      const R hc = aw * (e1 + e2) / dn * cs / kk;
+      v = v + v3(hc * A[0], aw * (e1 - e2) / dn * sn, hc * A[1]);
+    }
+    return v;
+  }
+  R layerW(V3<R> p) const {  // wgsl-core.js:222-226
+    if (P.ocean == 0u) return R(0);
+    const R dw = fmn<R>(fmn<R>(p.x - P.x0, P.x1 - p.x), fmn<R>(p.z - P.z0, P.z1 - p.z));
+    return clampr(R(1) - dw / fmx<R>(P.olayer, R(1.0e-4)), R(0), R(1));
+  }
   R fillPhi(V3<R> x) const {  // wgsl-core.js:228-238
     R dd = R(1.0e9);
     for (uint32_t k = 0; k < P.nfill; k++) {
@@ -416,6 +460,7 @@ template <class R> struct Sim {
       sdfEval(k, x, phi, wn);
       dd = fmn<R>(dd, phi);
     }
+    if (P.ocean != 0u) dd = fmn<R>(dd, x.y - oceanEta(x.x, x.z));
     return dd * P.ih;
   }
   void markAround(I3 cc) {  // wgsl-core.js:239-246
@@ -509,7 +554,13 @@ template <class R> struct Sim {
       if (!inDom(p)) ph = R(8.0);
       if (P.nbInit == 2u) ph = fmn<R>(ph, phiOf(c));
       phiPut(c, ph);
-      if (P.nbInit == 1u) for (uint32_t ax = 0; ax < 3; ax++) velG[c * 3 + ax] = R(0);
+      if (P.nbInit == 1u)
+        for (uint32_t ax = 0; ax < 3; ax++) {
+          V3<R> fp = facePos(p, ax);
+          R v = R(0);
+          if (P.ocean != 0u && fp.y < oceanEta(fp.x, fp.z)) v = oceanVel(fp)[int(ax)];
+          velG[c * 3 + ax] = v;
+        }
     });
   }
   // ---- solver (wgsl-solver.js) ----
@@ -582,6 +633,7 @@ template <class R> struct Sim {
       V3<R> cx = cellCenter(p.x, p.y, p.z);
       if (!inDom(p)) ty = 0;
       else if ((sb & 0xff00u) != 0u) ty = 2;
+      else if (P.ocean != 0u && layerW(cx) >= R(0.5)) ty = cx.y < oceanEta(cx.x, cx.z) ? 2u : 0u;
       else if (P.nb != 0u && phiOf(c) < R(-1.0) && (phiOf(c) < -P.band || calmV(velGVec(cx)))) ty = 1;
       else if (n > 0) {
         ty = 1;
@@ -658,6 +710,7 @@ template <class R> struct Sim {
           if (ci < 0 && th == 2u) ci = colAtCell(hi.x, hi.y, hi.z);
           V3<R> fp = facePos(at, axis);
           if (ci >= 0) sv = colSurfVel(uint32_t(ci), fp)[int(axis)];
+          else if (P.ocean != 0u && ((tl == 2u && inDom(lo)) || (th == 2u && inDom(hi)))) sv = oceanVel(fp)[int(axis)];
           v = sv; old = sv;
         }
         foW(f, old);
@@ -769,6 +822,15 @@ template <class R> struct Sim {
       if (hasStorage(fcell)) sampleBoth(p, vn, vo);
       V3<R> fl = v0 + (vn - vo);
       V3<R> v = v3(mixs(vn.x, fl.x, P.flip), mixs(vn.y, fl.y, P.flip), mixs(vn.z, fl.z, P.flip));
+      if (P.ocean != 0u) {  // wgsl-solver.js:218
+        R lw = layerW(p);
+        if (lw > R(0)) {
+          R sc = clampr(lw * R(2), R(0), R(1));
+          V3<R> ov = oceanVel(p);
+          R t = sc * sc * (R(1) - M<R>::exp(R(-8.0) * P.dt));
+          v = v3(mixs(v.x, ov.x, t), mixs(v.y, ov.y, t), mixs(v.z, ov.z, t));
+        }
+      }
       I3 bc{toi(M<R>::flo((p.x - P.x0) * P.ih - R(0.5))), toi(M<R>::flo((p.y - P.y0) * P.ih - R(0.5))), toi(M<R>::flo((p.z - P.z0) * P.ih - R(0.5)))};
       R csum = R(8.0) * P.rest;
       bool fast = P.q12 > R(0.5);
@@ -797,6 +859,7 @@ template <class R> struct Sim {
       R tau = fmx<R>(P.q5, R(0.05)) * mixs(R(1.0), R(0.35), smooth(R(0.7), R(1.0), rho)) * mixs(R(1.0), R(2.5), air);
       R foam = vel[i * 4 + 3] * M<R>::exp(-P.dt / tau);
       foam = fmx<R>(foam, fmx<R>(spray * R(0.9), fmx<R>(impact, turb * R(0.8))));
+      if (P.ocean != 0u) foam = foam * (R(1) - clampr(layerW(p) * R(2), R(0), R(1)));  // wgsl-solver.js:248
       V3<R> gust = v3(R(0), R(0), R(0));
       if (P.q7 > R(0.0) && P.q4 > R(0.0) && (!fast || shield < R(0.9))) {
         V3<R> gp = p * R(0.55);
@@ -850,6 +913,7 @@ template <class R> struct Sim {
       V3<R> x = cellCenter(p.x, p.y, p.z);
       R phA = phiSample(x - velGVec(x) * P.dt);
       bool inside = cnt[c] > 0 || phA < R(-1.0);
+      if (P.ocean != 0u && layerW(x) >= R(0.5)) inside = x.y < oceanEta(x.x, x.z);
       rhs[c] = inside ? clampr(phA, R(-7.9), R(-0.5)) : clampr(phA, R(0.5), R(7.9));
     });
   }
@@ -1009,6 +1073,7 @@ template <class R> struct Sim {
       V3<R> np = p + v * P.dt;
       if (np.x < P.x0 || np.x > P.x1 || np.y < P.y0 || np.y > P.y1 || np.z < P.z0 || np.z > P.z1) life = R(0);
       else if (P.ncols > 0u && colAt(np) >= 0) life = R(0);
+      else if (P.ocean != 0u && kind >= 2u && layerW(np) > R(0.6)) life = R(0);
       sPos[j * 4] = np.x; sPos[j * 4 + 1] = np.y; sPos[j * 4 + 2] = np.z; sPos[j * 4 + 3] = life;
       sVel[j * 4] = v.x; sVel[j * 4 + 1] = v.y; sVel[j * 4 + 2] = v.z; sVel[j * 4 + 3] = R(kind) + R(0.4) * tk / hold;
     }
@@ -1021,6 +1086,7 @@ template <class R> struct Sim {
     for (uint32_t i = 0; i < P.n; i++) {
       if (pos[i * 4 + 1] < R(-1.0e5)) continue;
       V3<R> q = v3(pos[i * 4], pos[i * 4 + 1], pos[i * 4 + 2]), v = v3(vel[i * 4], vel[i * 4 + 1], vel[i * 4 + 2]);
+      if (P.ocean != 0u && layerW(q) > R(0)) continue;  // wgsl-whitewater.js:105
       R foam = vel[i * 4 + 3], spd = len(v);
       R fast = clampr((spd * P.vs - P.sprMin) / (R(2) * P.sprMin), R(0), R(1));
       I3 bc{toi(M<R>::flo((q.x - P.x0) * P.ih - R(0.5))), toi(M<R>::flo((q.y - P.y0) * P.ih - R(0.5))), toi(M<R>::flo((q.z - P.z0) * P.ih - R(0.5)))};
@@ -1135,9 +1201,11 @@ template <class R> struct Sim {
   void nbKill() {  // wgsl-narrowband.js:93-104
     for (uint32_t i = 0; i < P.n; i++) {
       if (pos[i * 4 + 1] < R(-1.0e5)) continue;
-      bool kill = false;
+      V3<R> q = v3(pos[i * 4], pos[i * 4 + 1], pos[i * 4 + 2]);
+      R lw = layerW(q);
+      bool kill = lw >= R(0.5) || (lw > R(0.25) && q.y > oceanEta(q.x, q.z) + R(0.5) * P.h);
       V3<R> v = v3(vel[i * 4], vel[i * 4 + 1], vel[i * 4 + 2]);
-      if (P.nb != 0u && calmV(v)) {
+      if (!kill && P.nb != 0u && calmV(v)) {
         I3 cc = particleCell(v3(pos[i * 4], pos[i * 4 + 1], pos[i * 4 + 2]));
         if (hasStorage(cc) && phiOf(cellIdx(cc.x, cc.y, cc.z)) < -P.band) kill = true;
       }
@@ -1161,9 +1229,14 @@ template <class R> struct Sim {
         I3 p = slotCoord(s, local);
         if (!inDom(p) || colBits(c) != 0u) continue;
         V3<R> cx = cellCenter(p.x, p.y, p.z);
+        R lw = layerW(cx);
         R ph = phiOf(c);
-        if ((ctype[c] & 0xffu) != 1u || ph > R(-1.0) || ph < -P.band) continue;
-        if (!calmV(velGVec(cx))) continue;
+        if (lw > R(0.25) && lw < R(0.5)) {
+          if (cx.y > oceanEta(cx.x, cx.z) - R(0.25) * P.h || ph < -P.band) continue;
+        } else {
+          if ((ctype[c] & 0xffu) != 1u || ph > R(-1.0) || ph < -P.band) continue;
+          if (!calmV(velGVec(cx))) continue;
+        }
         uint32_t n = uint32_t(imax(cnt[c], 0));
         if (R(n) >= R(0.55) * P.rest) continue;
         uint32_t needc = std::min(tou(M<R>::rnd(R(0.85) * P.rest)) - n, 64u);
@@ -1188,6 +1261,7 @@ template <class R> struct Sim {
         uint32_t hh = (e * 9781u + m * 6271u + 17u) ^ (P.seed * 2654435761u);
         V3<R> x = v3(P.x0, P.y0, P.z0) + (cp + v3(hash1(hh + 1u), hash1(hh + 2u), hash1(hh + 3u))) * P.h;
         V3<R> v = velGVec(x);
+        if (layerW(x) > R(0.25)) v = oceanVel(x);
         pos[i * 4] = x.x; pos[i * 4 + 1] = x.y; pos[i * 4 + 2] = x.z; pos[i * 4 + 3] = fmn<R>(len(v), R(63.0));
         vel[i * 4] = v.x; vel[i * 4 + 1] = v.y; vel[i * 4 + 2] = v.z; vel[i * 4 + 3] = R(0);
       }
@@ -1197,7 +1271,7 @@ template <class R> struct Sim {
   // ---- host side of FlipSim.step (sim.js:179-241) ----
   int beginStep(const st_params* o) {
     if (N == 0) return ST_ERR_STATE;
-    nbOn = o->nb_on != 0;
+    nbOn = o->nb_on != 0 || o->ocean_on != 0;
     if (nbOn && !velGOn) {  // enableNB (sim.js:120-126): longer lists
       velGOn = true;
       lcap = scap * 512u;
@@ -1238,7 +1312,15 @@ template <class R> struct Sim {
     p.seed = seed;
     p.nb = nbOn ? 1u : 0u;
     p.band = R(std::max(2.0f, o->nb_band > 0 ? o->nb_band : 3.0f));
-    p.ocean = 0u;
+    p.ocean = o->ocean_on ? 1u : 0u;
+    p.nwave = o->ocean_on ? std::min(o->nwave, 32u) : 0u;
+    p.otime = R(o->ocean_time);
+    p.olevel = o->ocean_on ? R(o->ocean_level) : R(0);
+    p.olayer = o->ocean_on ? R(o->ocean_layer) : R(0);
+    p.oy0 = by0;
+    p.odepth = o->ocean_on ? R(float(std::max(double(o->ocean_level) - double(float(by0)), 1e-3))) : R(1);
+    for (uint32_t k = 0; k < p.nwave; k++)
+      for (int q = 0; q < 8; q++) p.waves[k * 2 + q / 4][q % 4] = R(o->waves[k * 8 + q]);
     p.nbInit = initStep;
     p.nfill = initStep ? std::min<uint32_t>(o->nfill, 8u) : 0u;
     p.nfv = nfv;
@@ -1548,6 +1630,34 @@ int orc_dump(void* hp, int32_t what, void* dst, size_t bytes) { Handle* h = (Han
 int orc_poke(void* hp, int32_t what, const void* src, size_t bytes) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, poke(what, src, bytes)); }
 int orc_kernel(void* hp, int32_t which) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, kernel(which)); }
 float orc_expf(float x) { return st_expf(x); }
+// test hooks: the gust noise of g2p (vnz) and its lattice hash (hsh3), in f32 or f64
+double orc_vnz(double x, double y, double z, int fp64) {
+  if (fp64) return Sim<double>::vnz(v3(x, y, z));
+  return double(Sim<float>::vnz(v3(float(x), float(y), float(z))));
+}
+double orc_hsh3(double x, double y, double z, int fp64) {
+  if (fp64) return Sim<double>::hsh3(v3(x, y, z));
+  return double(Sim<float>::hsh3(v3(float(x), float(y), float(z))));
+}
+// test hook: the ocean helpers of the begun step at n points (x, y, z, -) ->
+// (oceanEta(x, z), oceanVel(p) x, y, z, layerW(p)) per point
+int orc_ocean_eval(void* hp, const double* pts4, uint32_t n, double* out5) {
+  Handle* h = (Handle*)hp;
+  auto ev = [&](auto* s) {
+    using RR = typename std::remove_reference<decltype(s->h)>::type;
+    for (size_t q = 0; q < n; q++) {
+      V3<RR> p = v3(RR(pts4[q * 4]), RR(pts4[q * 4 + 1]), RR(pts4[q * 4 + 2]));
+      V3<RR> v = s->oceanVel(p);
+      out5[q * 5] = double(s->oceanEta(p.x, p.z));
+      out5[q * 5 + 1] = double(v.x);
+      out5[q * 5 + 2] = double(v.y);
+      out5[q * 5 + 3] = double(v.z);
+      out5[q * 5 + 4] = double(s->layerW(p));
+    }
+  };
+  if (h->fp64) ev(h->d); else ev(h->f);
+  return ST_OK;
+}
 int orc_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -39,6 +39,9 @@ _PROTOS = {
     "orc_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "orc_read_deep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "orc_expf": (C.c_float, [C.c_float]),
+    "orc_ocean_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "orc_vnz": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
+    "orc_hsh3": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
     "orc_poke": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "orc_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
 }
@@ -147,6 +150,15 @@ class Oracle(_Base):
         self._call("read_spray", self.h, p.ctypes.data, v.ctypes.data, room, C.byref(n))
         return p[:n.value].astype(self.fdtype), v[:n.value].astype(self.fdtype)
 
+    def ocean_eval(self, pts):
+        """test hook: the ocean helpers of the begun step at points (n, 3) ->
+        (n, 5): oceanEta(x, z), oceanVel (3), layerW"""
+        p = np.zeros((len(pts), 4), np.float64)
+        p[:, :3] = np.asarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros((len(pts), 5), np.float64)
+        self._call("ocean_eval", self.h, p.ctypes.data, len(pts), out.ctypes.data)
+        return out
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.orc_destroy(self.h)
@@ -165,3 +177,13 @@ def threads() -> int:
 
 def expf(x: float) -> float:
     return float(load().orc_expf(float(x)))
+
+
+def vnz(x, y, z, fp64=False) -> float:
+    """the oracle's gust noise (wgsl-solver.js:202-207)"""
+    return float(load().orc_vnz(float(x), float(y), float(z), 1 if fp64 else 0))
+
+
+def hsh3(x, y, z, fp64=False) -> float:
+    """the oracle's lattice hash of the gust noise (wgsl-solver.js:198-201)"""
+    return float(load().orc_hsh3(float(x), float(y), float(z), 1 if fp64 else 0))

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
     "-I", INCLUDE, "-I", CSRC,
 ]
-SOURCES = ["flip_kernels.hip", "flip_adjoint.hip", "flip_norms.hip", "flip_sdf.hip", "flip_capi.hip"]
+SOURCES = ["flip_kernels.hip", "flip_adjoint.hip", "flip_norms.hip", "flip_sdf.hip", "flip_ocean.hip", "flip_capi.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -59,7 +59,31 @@ class StParams(C.Structure):
                 ("spr_mist", C.c_float), ("spr_drag", C.c_float), ("spr_mdrag", C.c_float), ("vs", C.c_float),
                 ("ww_foam_life", C.c_float), ("ww_rise", C.c_float), ("ww_crest", C.c_float), ("ww_impact", C.c_float),
                 ("air_on", C.c_uint32), ("air_couple", C.c_float), ("air_decay", C.c_float), ("air_blur", C.c_float),
-                ("drip_on", C.c_uint32), ("drip_rate", C.c_float), ("drip_dry", C.c_float)]
+                ("drip_on", C.c_uint32), ("drip_rate", C.c_float), ("drip_dry", C.c_float),
+                ("ocean_on", C.c_uint32), ("nwave", C.c_uint32), ("ocean_time", C.c_float),
+                ("ocean_level", C.c_float), ("ocean_layer", C.c_float), ("waves", C.c_float * 256)]
+
+
+ST_OCEAN_MAX_WAVES = 32
+
+
+class StOcean(C.Structure):
+    _fields_ = [("height", C.c_double), ("length", C.c_double), ("dir", C.c_double), ("spread", C.c_double),
+                ("seed", C.c_int32)]
+
+
+def ocean_waves(height, length, dir=0.0, spread=0.0, seed=1, g=9.81, depth=1.0):
+    """oceanWaves (src/shared.js:769-791) through st_ocean_waves: the wave list
+    as an (n, 8) f32 array (kx, kz, amplitude, angular frequency, phase,
+    wavenumber, 0, 0)."""
+    lib = load()
+    oc = StOcean(float(height), float(length), float(dir), float(spread), int(seed))
+    w = np.zeros((ST_OCEAN_MAX_WAVES, 8), np.float32)
+    n = C.c_uint32(0)
+    rc = lib.st_ocean_waves(C.byref(oc), float(g), float(depth), w.ctypes.data, ST_OCEAN_MAX_WAVES, C.byref(n))
+    if rc != 0:
+        raise StError(f"st_ocean_waves -> {rc}")
+    return w[: n.value].copy()
 
 
 class StSdfGrid(C.Structure):
@@ -150,6 +174,13 @@ def make_params(**kw) -> StParams:
         elif k == "fills":
             for q, x in enumerate(v):
                 p.fills[q] = float(x)
+        elif k == "waves":
+            w = np.asarray(v, np.float32).reshape(-1)
+            if w.size > 8 * ST_OCEAN_MAX_WAVES:
+                raise ValueError("at most 32 waves")
+            for q, x in enumerate(w):
+                p.waves[q] = float(x)
+            p.nwave = w.size // 8
         else:
             setattr(p, k, v)
     return p
@@ -184,6 +215,8 @@ PROTOS = {
     "st_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
     "st_set_drips": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "st_pack_view": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(StViewLayout)]),
+    "st_ocean_waves": (C.c_int, [C.POINTER(StOcean), C.c_double, C.c_double, C.c_void_p, C.c_uint32,
+                                 C.POINTER(C.c_uint32)]),
     "st_build_sdf": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.POINTER(StSdfGrid),
                                C.c_void_p, C.c_size_t]),
     "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),

@@ -283,7 +283,8 @@ int airAlloc(Sim* s) {
 // the step's parameter block and collider records (sim.js:179-231) into P and C
 int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   if (s->N == 0) return fail(ST_ERR_STATE, "st_begin_step: no particles (step() does nothing when N = 0)");
-  s->nbOn = o->nb_on != 0;
+  if (o->ocean_on && o->nwave > ST_OCEAN_MAX_WAVES) return fail(ST_ERR_ARG, "st_begin_step: nwave above ST_OCEAN_MAX_WAVES");
+  s->nbOn = o->nb_on != 0 || o->ocean_on != 0;  // the ocean runs with the narrow band (sim.js:185-186)
   if (s->nbOn && !s->velGOn) {  // enableNB (sim.js:120-126)
     HIPCHK(hipStreamSynchronize(s->st));
     s->velGOn = true;
@@ -379,6 +380,19 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   }
   for (int k = 0; k < 16; k++)
     for (int q = 0; q < 4; q++) P.fills[k][q] = o->fills[k * 4 + q];
+  {  // ocean (sim.js:207-218): words 57-63 and 66, the waves from word 148
+    const bool oc = o->ocean_on != 0;
+    P.ocean = oc ? 1u : 0u;
+    P.nwave = oc ? o->nwave : 0u;
+    P.otime = o->ocean_time;
+    P.olevel = oc ? o->ocean_level : 0.0f;
+    P.olayer = oc ? o->ocean_layer : 0.0f;
+    P.oy0 = s->y0;
+    P.odepth = oc ? float(std::max(double(o->ocean_level) - double(s->y0), 1e-3)) : 1.0f;
+    P.oease = 1.0f - st_expf(-8.0f * P.dt);
+    for (uint32_t k = 0; k < P.nwave; k++)
+      for (int q = 0; q < 8; q++) P.waves[k * 2 + q / 4][q % 4] = o->waves[k * 8 + q];
+  }
   s->iters = std::max(1, int(o->iters));
   Pout = P;
   size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
@@ -490,8 +504,8 @@ void dropGraphs(Sim* s) {
 }
 
 int graphStep(Sim* s, const st_params* o) {
-  if (o->nb_on || o->only_tiles || o->nb_init)
-    return fail(ST_ERR_ARG, "st_step_n: graph replay covers plain substeps (no narrow band, no tile-only step)");
+  if (o->nb_on || o->ocean_on || o->only_tiles || o->nb_init)
+    return fail(ST_ERR_ARG, "st_step_n: graph replay covers plain substeps (no narrow band or ocean, no tile-only step)");
   if (s->batch) return fail(ST_ERR_STATE, "st_step_n: graph replay of a tank in a batch (use st_batch_step_n)");
   if (!s->dGraphSlot) {
     int rc = dalloc(&s->dGraphSlot, 1);
@@ -1406,7 +1420,8 @@ static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) 
   uint32_t itersMax = 0;
   for (uint32_t t = 0; t < nt; t++) {
     const st_params* o = prm + (nprm == 1u ? 0u : t);
-    if (o->nb_on || o->only_tiles) return fail(ST_ERR_ARG, "st_batch_step: the narrow band and tile-only steps are single-tank only");
+    if (o->nb_on || o->ocean_on || o->only_tiles)
+      return fail(ST_ERR_ARG, "st_batch_step: the narrow band, the ocean and tile-only steps are single-tank only");
     if (b->tanks[t]->sprayCap) return fail(ST_ERR_ARG, "st_batch_step: whitewater is single-tank only (a tank has a spray pool)");
     itersMax = std::max<uint32_t>(itersMax, b->tanks[t]->N ? uint32_t(std::max(1, int(o->iters))) : 0u);
   }
@@ -1601,7 +1616,7 @@ int st_tune_sor(void* p, const st_params* prm, int32_t reps, st_sor_tune* out) {
   int rc = st_snapshot_create(p, &snap);
   if (rc) return rc;
   st_params q = *prm;
-  q.nb_on = 0; q.nb_init = 0; q.only_tiles = 0; q.sor_mode = 0;
+  q.nb_on = 0; q.nb_init = 0; q.only_tiles = 0; q.sor_mode = 0; q.ocean_on = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   rc = st_begin_step(p, &q);
   for (int k = 0; !rc && k <= ST_STAGE_DIVERGENCE; k++) rc = runStage(s, k);
@@ -1734,7 +1749,7 @@ int st_adj_record(void* adj, const st_params* prm) {
   Sim* s = a->s;
   if (a->depth >= a->maxSteps) return fail(ST_ERR_STATE, "st_adj_record: every tape is in use (reverse some substeps first)");
   if (s->scap != a->scap) return fail(ST_ERR_STATE, "st_adj_record: the pool has grown since the context was made");
-  if (prm->nb_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u)
+  if (prm->nb_on || prm->ocean_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u)
     return fail(ST_ERR_ARG, "st_adj_record: the adjoint covers closed boxes without drag or the narrow band");
   const uint32_t ncol = s->ncTotal >= prm->nfv ? s->ncTotal - prm->nfv : 0u;
   if (ncol > ADJ_MAX_COLS)

@@ -45,6 +45,11 @@ struct DParams {
   uint32_t sprOn;     // new points are thrown off this step
   uint32_t ndrip;     // drip samples on the colliders (0: no drips this step)
   float dripRate, dripDry;
+  // the ocean side boundary (wgsl-core.js:12-13, 201-226; sim.js:207-218)
+  uint32_t ocean, nwave;
+  float otime, olevel, olayer, odepth, oy0;
+  float oease;        // 1 - exp(-8 dt): g2p's easing toward the waves per step
+  float waves[64][4]; // per wave two rows: (kx, kz, amplitude, angular frequency), (phase, wavenumber, 0, 0)
 };
 
 // Collider records: 8 float4 per collider (gpu.js:3-4): (sdf offset, nx, ny, nz),
@@ -267,3 +272,48 @@ __device__ __forceinline__ uint32_t d_phiPutBits(uint32_t ct, float v) {
   return (ct & 0xffffu) | (q << 16);
 }
 __device__ __forceinline__ bool d_calmV(const DParams& P, float3 v) { return f3len(v) < 9.0f * P.h; }
+
+// ---------------------------------------------------------------------------
+// ocean (wgsl-core.js:201-226): a sum of linear waves on water of depth odepth
+// over a floor at oy0, imposed through a layer at the side walls
+
+// surface height at (x, z)
+__device__ __forceinline__ float d_oceanEta(const DParams& P, float x, float z) {
+  float e = 0.0f;
+  for (uint32_t k = 0; k < P.nwave; k++) {
+    const float* A = P.waves[k * 2u];
+    const float* B = P.waves[k * 2u + 1u];
+    float sn, cs;
+    st_sincosf_wide(A[0] * x + A[1] * z - A[3] * P.otime + B[0], &sn, &cs);
+    e = e + A[2] * cs;
+  }
+  return P.olevel + e;
+}
+// water velocity under the waves; the height is stretched so the crest gets the
+// surface value (Wheeler stretching)
+__device__ __forceinline__ float3 d_oceanVel(const DParams& P, float3 p) {
+  const float D = fmaxf(P.odepth, 1.0e-3f);
+  const float eta = d_oceanEta(P, p.x, p.z);
+  const float ys = st_clampf((p.y - P.oy0) * D / fmaxf(D + eta - P.olevel, 0.1f * D), 0.0f, D);
+  float3 v = make_float3(0.0f, 0.0f, 0.0f);
+  for (uint32_t k = 0; k < P.nwave; k++) {
+    const float* A = P.waves[k * 2u];
+    const float* B = P.waves[k * 2u + 1u];
+    const float kk = fmaxf(B[1], 1.0e-4f);
+    float sn, cs;
+    st_sincosf_wide(A[0] * p.x + A[1] * p.z - A[3] * P.otime + B[0], &sn, &cs);
+    const float e1 = st_expf(kk * (ys - D));
+    const float e2 = st_expf(-kk * (ys + D));
+    const float dn = fmaxf(1.0f - st_expf(-2.0f * kk * D), 1.0e-4f);
+    const float aw = A[2] * A[3];
+    const float hc = aw * (e1 + e2) / dn * cs / kk;
+    v = f3add(v, make_float3(hc * A[0], aw * (e1 - e2) / dn * sn, hc * A[1]));
+  }
+  return v;
+}
+// 1 at the side walls, 0 at the layer's width in from them (0 without the ocean)
+__device__ __forceinline__ float d_layerW(const DParams& P, float3 p) {
+  if (P.ocean == 0u) return 0.0f;
+  const float dw = fminf(fminf(p.x - P.x0, P.x1 - p.x), fminf(p.z - P.z0, P.z1 - p.z));
+  return st_clampf(1.0f - dw / fmaxf(P.olayer, 1.0e-4f), 0.0f, 1.0f);
+}

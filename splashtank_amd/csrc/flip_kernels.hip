@@ -643,7 +643,11 @@ __device__ __forceinline__ void classify_body(const DParams* __restrict__ PP, co
     int32_t n = G.cnt[c];
     if (!d_inDom(P, a)) ty = 0;
     else if ((sb & 0xff00u) != 0u) ty = 2;
-    else if (P.nb != 0u && d_phiOfBits(sb) < -1.0f &&
+    else if (P.ocean != 0u && d_layerW(P, d_cellCenter(P, a.x, a.y, a.z)) >= 0.5f) {
+      // the ocean's boundary: a wall moving with the waves, under their surface
+      float3 cx = d_cellCenter(P, a.x, a.y, a.z);
+      ty = cx.y < d_oceanEta(P, cx.x, cx.z) ? 2u : 0u;
+    } else if (P.nb != 0u && d_phiOfBits(sb) < -1.0f &&
              (d_phiOfBits(sb) < -P.band || d_calmV(P, d_velGVec(P, G, d_cellCenter(P, a.x, a.y, a.z))))) ty = 1;
     else if (n > 0) {
       ty = 1;
@@ -704,6 +708,8 @@ __device__ __forceinline__ void norm_faces_body(const DParams* __restrict__ PP, 
         if (ci < 0 && th == 2u) ci = d_colAtCell(P, G.tmap, G.ctype, at.x, at.y, at.z);
         float3 fp = d_facePos(P, at, axis);
         if (ci >= 0) sv = f3get(d_colSurfVel(C, uint32_t(ci), fp), axis);
+        else if (P.ocean != 0u && ((tl == 2u && d_inDom(P, lo)) || (th == 2u && d_inDom(P, at))))
+          sv = f3get(d_oceanVel(P, fp), axis);  // the ocean's moving wall
         v = sv;
         old = sv;
       }
@@ -1628,6 +1634,16 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   if (storage) g.sample(p, vn, vo);
   float3 fl = f3add(v0, f3sub(vn, vo));
   float3 v = make_float3(st_mixf(vn.x, fl.x, P.flip), st_mixf(vn.y, fl.y, P.flip), st_mixf(vn.z, fl.z, P.flip));
+  float lw = 0.0f;
+  if (P.ocean != 0u) {  // near the ocean's boundary, ease into the waves (wgsl-solver.js:218)
+    lw = d_layerW(P, p);
+    if (lw > 0.0f) {
+      const float s = st_clampf(lw * 2.0f, 0.0f, 1.0f);
+      const float3 ov = d_oceanVel(P, p);
+      const float t = s * s * P.oease;
+      v = make_float3(st_mixf(v.x, ov.x, t), st_mixf(v.y, ov.y, t), st_mixf(v.z, ov.z, t));
+    }
+  }
   int3 bc = make_int3(st_f2i_sat(floorf((p.x - P.x0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.y - P.y0) * P.ih - 0.5f)), st_f2i_sat(floorf((p.z - P.z0) * P.ih - 0.5f)));
   float csum = 8.0f * P.rest;
   bool fast = P.q12 > 0.5f;
@@ -1654,6 +1670,7 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   float spray = air * (0.25f + 0.75f * st_clampf(spd0 / 4.0f, 0.0f, 1.0f));
   float foam = v04.w * ft.z;
   foam = fmaxf(foam, fmaxf(spray * 0.9f, fmaxf(impact, turb * 0.8f)));
+  if (P.ocean != 0u) foam = foam * (1.0f - st_clampf(lw * 2.0f, 0.0f, 1.0f));  // easing is not churning (wgsl-solver.js:248)
   float3 gust = make_float3(0.0f, 0.0f, 0.0f);
   if (P.q7 > 0.0f && P.q4 > 0.0f && (!fast || shield < 0.9f)) {
     float3 gp = f3mul(p, 0.55f);
@@ -1801,11 +1818,11 @@ __device__ __forceinline__ float3 d_velGVec(const DParams& P, const DGrid& G, fl
 }
 
 // ---------------------------------------------------------------------------
-// narrow band (wgsl-narrowband.js, wgsl-core.js:156-238); the ocean is not part
-// of this library, so its branches are the ocean-off ones
+// narrow band (wgsl-narrowband.js, wgsl-core.js:156-238), with the ocean's
+// branches (the ocean turns the narrow band on)
 
-// signed distance in cells to the starting water: boxes, spheres, water volumes
-// (wgsl-core.js:228-238)
+// signed distance in cells to the starting water: boxes, spheres, water volumes and
+// the ocean (wgsl-core.js:228-238)
 __device__ __forceinline__ float d_fillPhi(const DParams& P, const DCols* __restrict__ C, const float* __restrict__ sdf, float3 x) {
   float dd = 1.0e9f;
   for (uint32_t k = 0; k < P.nfill; k++) {
@@ -1825,6 +1842,7 @@ __device__ __forceinline__ float d_fillPhi(const DParams& P, const DCols* __rest
     d_sdfEval(C, sdf, k, x, phi, wn);
     dd = fminf(dd, phi);
   }
+  if (P.ocean != 0u) dd = fminf(dd, x.y - d_oceanEta(P, x.x, x.z));
   return dd * P.ih;
 }
 // depth at a point from the cell centres around it, collider cells left out
@@ -1891,10 +1909,15 @@ __device__ __forceinline__ void nb_phi_init_body(const DParams* __restrict__ PP,
     uint32_t ct = G.ctype[c];
     if (P.nbInit == 2u) ph = fminf(ph, d_phiOfBits(ct));
     G.ctype[c] = d_phiPutBits(ct, ph);
-    if (P.nbInit == 1u) {
-      G.velG[c * 3] = 0.0f;
-      G.velG[c * 3 + 1] = 0.0f;
-      G.velG[c * 3 + 2] = 0.0f;
+    if (P.nbInit == 1u) {  // at rest, or moving with the waves under the ocean's surface
+      for (uint32_t ax = 0; ax < 3; ax++) {
+        float v = 0.0f;
+        if (P.ocean != 0u) {
+          float3 fp = d_facePos(P, p, ax);
+          if (fp.y < d_oceanEta(P, fp.x, fp.z)) v = f3get(d_oceanVel(P, fp), ax);
+        }
+        G.velG[c * 3 + ax] = v;
+      }
     }
   }
 }
@@ -1912,6 +1935,7 @@ __device__ __forceinline__ void nb_phi_body(const DParams* __restrict__ PP, cons
     float3 x = d_cellCenter(P, p.x, p.y, p.z);
     float phA = d_phiSample(P, G, f3sub(x, f3mul(d_velGVec(P, G, x), P.dt)));
     bool inside = G.cnt[c] > 0 || phA < -1.0f;
+    if (P.ocean != 0u && d_layerW(P, x) >= 0.5f) inside = x.y < d_oceanEta(P, x.x, x.z);  // the ocean surface decides
     G.rhs[c] = inside ? st_clampf(phA, -7.9f, -0.5f) : st_clampf(phA, 0.5f, 7.9f);
   }
 }
@@ -1993,9 +2017,11 @@ __global__ void k_nb_kill(const DParams* __restrict__ PP, DGrid G) {
   if (i >= P.n) return;
   float4 q = G.pos[i];
   if (q.y < ST_DEADY) return;
-  bool kill = false;
+  // inside the ocean's boundary, or above the ocean surface in its inner half
+  const float lw = d_layerW(P, make_float3(q.x, q.y, q.z));
+  bool kill = lw >= 0.5f || (lw > 0.25f && q.y > d_oceanEta(P, q.x, q.z) + 0.5f * P.h);
   float4 v = G.vel[i];
-  if (P.nb != 0u && d_calmV(P, make_float3(v.x, v.y, v.z))) {
+  if (!kill && P.nb != 0u && d_calmV(P, make_float3(v.x, v.y, v.z))) {
     int3 cc = d_particleCell(P, make_float3(q.x, q.y, q.z));
     uint32_t sl = G.tmap[d_tileOf(P, cc.x, cc.y, cc.z)];
     if (sl != ST_NONE && d_phiOfBits(G.ctype[d_cellIdx(P, G.tmap, cc.x, cc.y, cc.z)]) < -P.band) kill = true;
@@ -2005,17 +2031,21 @@ __global__ void k_nb_kill(const DParams* __restrict__ PP, DGrid G) {
     G.vel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
-// nbDead (wgsl-narrowband.js:107-110) as a flag per particle, compacted in index order
+// nbDead (wgsl-narrowband.js:107-110) as a flag per particle id, compacted in id
+// order: the reference's particle index is its id (its arrays are never
+// reordered by default), and the particle arrays here are in bin order, so the
+// k-th dead slot listed is the storage of the k-th dead id
 __global__ void k_nb_dead_flags(const DParams* __restrict__ PP, DGrid G, uint32_t* __restrict__ flag) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= PP->n) return;
-  flag[i] = G.pos[i].y < ST_DEADY ? 1u : 0u;
+  flag[G.pid[i]] = G.pos[i].y < ST_DEADY ? 1u : 0u;
 }
 __global__ void k_nb_dead_list(const DParams* __restrict__ PP, DGrid G, const uint32_t* __restrict__ flag,
                                const uint32_t* __restrict__ start, uint32_t* __restrict__ list) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= PP->n) return;
-  if (flag[i]) list[start[i]] = i;
+  const uint32_t id = G.pid[i];
+  if (flag[id]) list[start[id]] = i;
 }
 // nbDeficit (wgsl-narrowband.js:113-130): per active cell (slotList order) how
 // many particles it is short; a scan of these gives each cell's entry number and
@@ -2032,8 +2062,16 @@ __global__ void k_nb_deficit(const DParams* __restrict__ PP, DGrid G, uint32_t* 
       int3 p = d_slotCoord(P, G.slotTile, s, local);
       uint32_t ct = G.ctype[c];
       float ph = d_phiOfBits(ct);
-      bool ok = d_inDom(P, p) && ((ct >> 8) & 0xffu) == 0u && (ct & 0xffu) == 1u && !(ph > -1.0f) && !(ph < -P.band);
-      if (ok && d_calmV(P, d_velGVec(P, G, d_cellCenter(P, p.x, p.y, p.z)))) {
+      bool ok = d_inDom(P, p) && ((ct >> 8) & 0xffu) == 0u;
+      if (ok) {
+        const float3 cx = d_cellCenter(P, p.x, p.y, p.z);
+        const float lw = d_layerW(P, cx);
+        if (lw > 0.25f && lw < 0.5f)  // the inner half of the ocean's boundary is always full below the surface
+          ok = !(cx.y > d_oceanEta(P, cx.x, cx.z) - 0.25f * P.h || ph < -P.band);
+        else
+          ok = (ct & 0xffu) == 1u && !(ph > -1.0f) && !(ph < -P.band) && d_calmV(P, d_velGVec(P, G, cx));
+      }
+      if (ok) {
         uint32_t n = uint32_t(max(G.cnt[c], 0));
         if (!(float(n) >= 0.55f * P.rest)) need = min(st_f2u_sat(rintf(0.85f * P.rest)) - n, 64u);
       }
@@ -2059,10 +2097,12 @@ __global__ void k_nb_spawn(const DParams* __restrict__ PP, DGrid G, const uint32
     if (!need) continue;
     uint32_t e = entIdx[t];
     if (e >= maxe) continue;
+    uint32_t base = needStart[t];
+    // nbReset / nbSpawn's ctl[10]: the needs of the entries the spawn served, summed
+    if (e + 1u == min(nent, maxe)) G.ctl[10] = base + need;
     uint32_t s = G.slotList[t >> 9], local = t & 511u;
     int3 p = d_slotCoord(P, G.slotTile, s, local);
     float3 cp = make_float3(float(uint32_t(p.x) & 0xffffu), float(uint32_t(p.y) & 0xffffu), float(uint32_t(p.z) & 0xffffu));
-    uint32_t base = needStart[t];
     for (uint32_t m = 0; m < need; m++) {
       uint32_t k = base + m;
       if (k >= avail) break;
@@ -2071,13 +2111,16 @@ __global__ void k_nb_spawn(const DParams* __restrict__ PP, DGrid G, const uint32
       float3 x = f3add(make_float3(P.x0, P.y0, P.z0),
                        f3mul(f3add(cp, make_float3(d_hash1(hh + 1u), d_hash1(hh + 2u), d_hash1(hh + 3u))), P.h));
       float3 v = d_velGVec(P, G, x);
+      if (d_layerW(P, x) > 0.25f) v = d_oceanVel(P, x);
       G.pos[i] = make_float4(x.x, x.y, x.z, fminf(f3len(v), 63.0f));
       G.vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    G.ctl[7] = 0u;
     G.ctl[8] = ndead;
     G.ctl[9] = nent;
+    if (nent == 0u) G.ctl[10] = 0u;
   }
 }
 
@@ -2695,6 +2738,7 @@ __global__ void k_spray_step(const DParams* __restrict__ PP, const DCols* __rest
     float3 np = f3add(p, f3mul(v, P.dt));
     if (np.x < P.x0 || np.x > P.x1 || np.y < P.y0 || np.y > P.y1 || np.z < P.z0 || np.z > P.z1) life = 0.0f;
     else if (P.ncols > 0u && d_colAt(P, C, G.sdf, np) >= 0) life = 0.0f;
+    else if (P.ocean != 0u && kind >= 2u && d_layerW(P, np) > 0.6f) life = 0.0f;  // foam and bubbles end in the ocean's boundary
     G.sPos[j] = make_float4(np.x, np.y, np.z, life);
     G.sVel[j] = make_float4(v.x, v.y, v.z, float(kind) + 0.4f * tk / WWHOLD);
   }
@@ -2704,6 +2748,7 @@ __global__ void k_spray_step(const DParams* __restrict__ PP, const DCols* __rest
 // values the emission needs are recomputed by k_spray_emit
 __device__ __forceinline__ uint32_t d_sprayCount(const DParams& P, const DGrid& G, float4 q, float4 vw, uint32_t id, float& rho,
                                                  float& fast, float& spd) {
+  if (P.ocean != 0u && d_layerW(P, c3(q)) > 0.0f) return 0u;  // none from the water easing into the waves
   float3 v = c3(vw);
   float foam = vw.w;
   spd = f3len(v);

@@ -249,6 +249,49 @@ void readParams(napi_env env, napi_value o, st_params* p) {
     size_t n = 0;
     if (typed(env, fv, &data, &n, napi_float32_array)) std::memcpy(p->fills, data, std::min<size_t>(n, 64) * 4);
   }
+  // the ocean (o.ocean, o.time): the facade passes the wave list oceanWaves gives
+  p->ocean_on = uint32_t(numProp(env, o, "oceanOn", 0));
+  p->ocean_time = float(numProp(env, o, "oceanTime", 0));
+  p->ocean_level = float(numProp(env, o, "oceanLevel", 0));
+  p->ocean_layer = float(numProp(env, o, "oceanLayer", 0));
+  has = false;
+  napi_has_named_property(env, o, "waves", &has);
+  if (has) {
+    napi_value wv;
+    napi_get_named_property(env, o, "waves", &wv);
+    void* data = nullptr;
+    size_t n = 0;
+    if (typed(env, wv, &data, &n, napi_float32_array)) {
+      n = std::min<size_t>(n, 8 * ST_OCEAN_MAX_WAVES);
+      std::memcpy(p->waves, data, n * 4);
+      p->nwave = uint32_t(n / 8);
+    }
+  }
+}
+
+// oceanWaves(oc {height, length, dir, spread, seed}, g, depth) -> Float32Array, 8 floats per wave
+napi_value OceanWaves(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  st_ocean oc{};
+  oc.height = numProp(env, argv[0], "height", 0);
+  oc.length = numProp(env, argv[0], "length", 1);
+  oc.dir = numProp(env, argv[0], "dir", 0);
+  oc.spread = numProp(env, argv[0], "spread", 0);
+  oc.seed = int32_t(numProp(env, argv[0], "seed", 0));
+  double g = 0, depth = 0;
+  napi_get_value_double(env, argv[1], &g);
+  napi_get_value_double(env, argv[2], &depth);
+  float w[8 * ST_OCEAN_MAX_WAVES];
+  uint32_t n = 0;
+  ST_OK_OR_THROW(st_ocean_waves(&oc, g, depth, w, ST_OCEAN_MAX_WAVES, &n));
+  n = std::min(n, ST_OCEAN_MAX_WAVES);
+  napi_value ab, arr;
+  void* data = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, size_t(n) * 32, &data, &ab));
+  std::memcpy(data, w, size_t(n) * 32);
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, size_t(n) * 8, ab, 0, &arr));
+  return arr;
 }
 
 napi_value Step(napi_env env, napi_callback_info info) {
@@ -554,7 +597,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"oceanWaves", OceanWaves}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}};
   for (const auto& e : fns) {
     napi_value f;

@@ -14,8 +14,9 @@
 //  - the block pool defaults to every tile of the box (no growth needed on a
 //    288 GB GPU); opt.slots still sets a smaller pool, with the reference's
 //    overflow counting and growth policy.
-// Whitewater, the live-view packing and deep-water display points are outside
-// the substep this library provides; their members are stubs.
+// Whitewater (setSprayCap/readSpray), drips, the live view (pump/outFrame) and the
+// deep water points of the narrow band (setDeepLook/readDeep) run on the GPU too,
+// as does the ocean side boundary (o.ocean with o.time).
 
 const path = require('path');
 const { dripSamples } = require('./sdf.js');
@@ -153,9 +154,24 @@ class FlipSim {
   }
   // ---- stepping (sim.js:179-243) ----
   omega() { return Math.min(1.9, 2 / (1 + Math.sin(Math.PI / Math.max(this.nx, this.ny, this.nz)))); }
+  // the ocean of this step: o.ocean when on, else (o.ocean not given) the one a narrow-band start set up
+  oceanOf(o) {
+    if (o.ocean && o.ocean.on) return o.ocean;
+    return o.ocean === undefined && this.oceanCfg && this.oceanCfg.on ? this.oceanCfg : null;
+  }
+  // its wave list, made again only when the settings or gravity change
+  oceanWaves(oc, g) {
+    const key = [oc.height, oc.length, oc.dir, oc.spread, oc.seed, oc.level, g].join('|');
+    if (key !== this.oceanKey) {
+      this.oceanKey = key;
+      this.oceanW = loadAddon().oceanWaves(oc, g, oc.level - this.b.y0);
+    }
+    return this.oceanW;
+  }
   params(dt, o) {
     const nbc = o.nb !== undefined ? o.nb : this.nbCfg;
-    const nbOn = !!(nbc && nbc.on);
+    const oc = this.oceanOf(o);
+    const nbOn = !!(nbc && nbc.on) || !!oc;   // the ocean runs with the narrow band
     const init = nbOn && this.nbInit ? this.nbInit : 0;
     const fl = init ? this.nbFills.slice(0, 8) : [];
     const fills = new Float32Array(64);
@@ -166,7 +182,9 @@ class FlipSim {
       foamTau: o.foamTau || 1.2, foamSens: o.foamSens || 1, gust: o.gust || 0,
       noFoam: this.opt.noFoam, noCull: this.opt.noCull, fastAir: this.opt.fastAir, noSolidCache: this.opt.noSolidCache,
       nbOn, nbBand: Math.max(2, (nbc && nbc.band) || 3), onlyTiles: this.only === 'tiles', nbInit: init, nfill: fl.length,
-      nfv: init ? this.nbFillVols.length : 0, fills, sorMode: this.opt.sorMode || 0, ...this.sprayParams(o) };
+      nfv: init ? this.nbFillVols.length : 0, fills, sorMode: this.opt.sorMode || 0, ...this.sprayParams(o),
+      oceanOn: oc ? 1 : 0, oceanTime: o.time || 0, oceanLevel: oc ? oc.level : 0, oceanLayer: oc ? oc.layer : 0,
+      waves: oc ? this.oceanWaves(oc, o.g) : new Float32Array(0) };
   }
   // o.spr, o.spr.ww, o.air (sim.js:203-206, 221-230), flattened for the C ABI; the C side applies the
   // reference's defaults and its "on only with room for points" rule

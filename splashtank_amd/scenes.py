@@ -135,6 +135,14 @@ class Scene:
     def n(self):
         return self.pos.shape[0]
 
+    def params_at(self, t: float, **kw) -> dict:
+        """Step settings at simulated time t (the ocean's waves move with o.time)."""
+        p = dict(self.params)
+        if p.get("ocean_on"):
+            p["ocean_time"] = t
+        p.update(kw)
+        return p
+
     def collider_state(self, t: float, dt: float):
         """Packed records and distance grids at time t; velocity from the pose
         change over the last dt as SceneRunner.step does (shared.js:531-537)."""
@@ -269,6 +277,112 @@ def nb_seed(d, h, size, fills, band=3, rng=None, cap=None):
     pos = np.concatenate([pos, park])
     vel = np.zeros_like(pos)
     return pos, vel, np.asarray(packed, np.float32).reshape(-1)
+
+
+def ocean_eta_vel(W, level, y0, x, y, z, t):
+    """Surface height and water velocity of the wave list W (n, 8) in f64 over
+    arrays of points: the sums of oceanEtaJS / oceanVelJS (src/shared.js:792-803),
+    a Wheeler-stretched Airy velocity under the surface."""
+    W = np.asarray(W, np.float64)
+    x, y, z = (np.asarray(q, np.float64) for q in (x, y, z))
+    kx, kz, a, w, ph, k = (W[:, q].reshape((-1,) + (1,) * x.ndim) for q in range(6))
+    th = kx * x + kz * z - w * t + ph
+    eta = level + (a * np.cos(th)).sum(0)
+    Dp = max(level - y0, 1e-3)
+    ys = np.clip((y - y0) * Dp / np.maximum(Dp + eta - level, 0.1 * Dp), 0.0, Dp)
+    kk = np.maximum(k, 1e-4)
+    e1, e2 = np.exp(kk * (ys - Dp)), np.exp(-kk * (ys + Dp))
+    dn = np.maximum(1.0 - np.exp(-2.0 * kk * Dp), 1e-4)
+    hc = a * w * (e1 + e2) / dn * np.cos(th) / kk
+    vel = np.stack([(hc * kx).sum(0), (a * w * (e1 - e2) / dn * np.sin(th)).sum(0), (hc * kz).sum(0)], -1)
+    return eta, vel
+
+
+def ocean_seed(d, h, size, oc, W, t=0.0, band=3, rng=None):
+    """The ocean part of nbSeed (shared.js:840-848): on every lattice column of
+    the box outside the outer half of the boundary layer, particles from a band
+    below the surface (eta capped a spacing under the lid) up to it, moving with
+    the waves.  y runs from y0 + d/2 + max(0, floor((eta - band - y0) / d)) d by
+    d below eta; the points are computed as start + k d rather than the JS loop's
+    running sum.  Jitter +-0.005 d / 0.12 on positions only."""
+    rng = rng or np.random.default_rng(0)
+    Wd, Hd, Dd = size
+    x0, x1, y0, y1, z0, z1 = -Wd / 2, Wd / 2, 0.0, Hd, -Dd / 2, Dd / 2
+    bandw = max(2, band) * h
+    xs, zs = _axis(x0, x1, d), _axis(z0, z1, d)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    X, Z = X.ravel(), Z.ravel()
+    dw = np.minimum(np.minimum(X - x0, x1 - X), np.minimum(Z - z0, z1 - Z))
+    keep = 1.0 - dw / max(oc["layer"], 1e-4) < 0.5
+    X, Z = X[keep], Z[keep]
+    eta, _ = ocean_eta_vel(W, oc["level"], y0, X, np.zeros_like(X), Z, t)
+    eta = np.minimum(eta, y1 - d)
+    ystart = y0 + d / 2 + np.maximum(0.0, np.floor((eta - bandw - y0) / d)) * d
+    m = int(math.ceil((bandw + d) / d)) + 2
+    Y = ystart[:, None] + np.arange(m)[None, :] * d
+    inside = Y < eta[:, None]
+    Xc = np.broadcast_to(X[:, None], Y.shape)[inside]
+    Zc = np.broadcast_to(Z[:, None], Y.shape)[inside]
+    Yc = Y[inside]
+    _, v = ocean_eta_vel(W, oc["level"], y0, Xc, Yc, Zc, t)
+    jit = 0.01 * d / 0.12
+    n = Xc.size
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = np.stack([Xc, Yc, Zc], 1) + (rng.random((n, 3)) - 0.5) * jit
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = v
+    pos[:, 3] = np.minimum(np.linalg.norm(v, axis=1), 63.0)
+    return pos, vel
+
+
+def stress(d=1.0 / 96, seed=9, level=0.35, layer=0.12, band=3, iters=30, dt=1.0 / 240,
+           ocean=(0.04, 0.45, 30.0, 40.0, 7), drop=(0.12, 0.72, -0.05, 0.09), spray_frac=0.25):
+    """Every feature at once: the ocean side boundary (24 Airy waves, the
+    boundary layer `layer` wide) over a deep pool of depth `level` in
+    narrow-band mode (band 3: only the band under the surface has particles), a
+    ball of water dropped into it, whitewater with the air field, and two SDF
+    colliders -- a box grid tilted and spinning as it ploughs along x through
+    the surface, and a sphere grid sinking and rising through it.  At d = 1/1024
+    this is the 512^3 full-feature stress case.  Step settings carry the ocean
+    and the whitewater; `ocean_time` must follow the simulated time
+    (Scene.params_at)."""
+    from splashtank_amd import capi
+    rng = np.random.default_rng(seed)
+    h = 2 * d
+    height, length, odir, spread, oseed = ocean
+    oc = dict(height=height, length=length, dir=odir, spread=spread, seed=oseed, level=level, layer=layer)
+    W = capi.ocean_waves(height, length, odir, spread, oseed, g=9.81, depth=level)
+    p1, v1 = ocean_seed(d, h, (1.0, 1.0, 1.0), oc, W, 0.0, band, rng)
+    fills = [dict(t="sphere", x=drop[0], y=drop[1], z=drop[2], r=drop[3])]
+    p2, v2, packed = nb_seed(d, h, (1.0, 1.0, 1.0), fills, band=band, rng=rng)
+    live2 = p2[:, 1] > -1e5
+    ocean_n = p1.shape[0]
+    spare = int(math.ceil(0.35 * (ocean_n + int(live2.sum())))) + 4000
+    park = np.zeros((spare, 4), np.float32)
+    park[:, 1] = -1e6
+    pos = np.concatenate([p1, p2[live2], park])
+    vel = np.concatenate([v1, v2[live2], np.zeros_like(park)])
+    bh = np.array([0.09, 0.05, 0.12])
+    dims, lo, data = box_sdf(bh, h / 2, 4 * h / 2)
+    ax = np.array([0.2, 1.0, -0.3])
+    ax = ax / np.linalg.norm(ax)
+    R0 = axis_rotation((0, 0, 1), math.radians(20))
+    box = dict(dims=dims, origin=lo, cell=h / 2, data=data,
+               path=lambda t: np.array([-0.28 + 0.5 * t, level + 0.02 - 0.2 * t, 0.1]),
+               rot=lambda t: axis_rotation(ax, 2.0 * t) @ R0, omega=tuple(2.0 * ax))
+    sd, slo, sdata = sphere_sdf(0.08, h / 2, 4 * h / 2)
+    sph = dict(dims=sd, origin=slo, cell=h / 2, data=sdata,
+               path=lambda t: np.array([-0.1, level + 0.12 - 0.8 * t + 2.0 * t * t, -0.18]),
+               rot=lambda t: axis_rotation((1, 0, 0), math.radians(30)), omega=(0.0, 0.0, 0.0))
+    prm = dict(dt=dt, g=9.81, iters=iters, omega=1.7, flip=0.95, comp=6.0, tear=0.0, vmax=14.0,
+               nb_on=1, nb_band=float(band), ocean_on=1, waves=W, ocean_level=level, ocean_layer=layer,
+               spr_on=1, spr_rate=40.0, spr_min=1.0, spr_life=1.2, spr_mist=0.3, spr_drag=1.0, spr_mdrag=5.0,
+               air_on=1, air_couple=3.0, air_decay=0.6, air_blur=0.35)
+    sc = Scene("stress", d, (1.0, 1.0, 1.0), 0, pos, vel, prm, colliders=[box, sph])
+    sc.nb_fills = packed
+    sc.spray_cap = max(1024, int(spray_frac * ocean_n))
+    sc.ocean = oc
+    return sc
 
 
 def nb_pool(d=1.0 / 64, depth=0.5, band=3, seed=4, iters=30, dt=1.0 / 240, drop=None):

@@ -69,3 +69,16 @@ def test_splc_cache_layout():
     assert o["frames"] == 2 and o["frame0"] == {"n": 2, "flags": 1, "yaw": 0.5, "pos0": 32768, "spd1": 191, "fm0": 51}
     assert o["length"][0] == o["length"][1] and o["round"]
     assert o["v1"] == {"fps": 30, "n": 2, "yaw": -0.75, "pos": [1, 2, 3, 4, 5, 6], "spd": [33, 44], "ids": [9]}
+
+
+@needs_node
+def test_cache_writers_match_reference_golden():
+    """packFrame (water, whitewater codes and deep points of a readback),
+    packDeepFrame, packSprayFrame, packSprayCode and writeCache give, byte for
+    byte, what the reference's packFrameBuf, packDeep, packSprayFrame,
+    packSprayJS and writeCacheParts wrote from the same inputs
+    (tests/golden/make_cache_golden.js), including the spray (flag 2) and
+    deep-water (flag 4) sections; readCache reads them back."""
+    o = _run("cache_golden.js")
+    assert all(o[k] for k in ("frame", "sprayFull", "sprayCapped", "codes", "deep", "file", "readBack")), o
+    assert o["fileBytes"][0] == o["fileBytes"][1]
