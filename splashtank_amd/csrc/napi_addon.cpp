@@ -592,13 +592,191 @@ napi_value BatchSync(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- the extensions: graph replay, the SOR autotuner, snapshots and the adjoint ----
+
+int32_t intArg(napi_env env, napi_value v, int32_t dflt) {
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_number) return dflt;
+  int32_t x = dflt;
+  napi_get_value_int32(env, v, &x);
+  return x;
+}
+
+// stepN(h, params, nsteps, useGraph) (st_step_n)
+napi_value StepN(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  getArgs(env, info, 4, argv);
+  st_params p;
+  readParams(env, argv[1], &p);
+  ST_OK_OR_THROW(st_step_n(handleOf(env, argv[0]), &p, intArg(env, argv[2], 1), intArg(env, argv[3], 0)));
+  return nullptr;
+}
+
+// batchStepN(batch, [params per tank], nsteps, useGraph) (st_batch_step_n)
+napi_value BatchStepN(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  getArgs(env, info, 4, argv);
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  std::vector<st_params> p(n);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value e;
+    NAPI_OK(napi_get_element(env, argv[1], k, &e));
+    readParams(env, e, &p[k]);
+  }
+  ST_OK_OR_THROW(st_batch_step_n(handleOf(env, argv[0]), p.data(), n, intArg(env, argv[2], 1), intArg(env, argv[3], 0)));
+  return nullptr;
+}
+
+napi_value tuneResult(napi_env env, const st_sor_tune& t) {
+  napi_value o, arr;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "mode", mkNum(env, t.mode));
+  napi_set_named_property(env, o, "width", mkNum(env, t.width));
+  napi_create_array_with_length(env, t.n, &arr);
+  for (uint32_t k = 0; k < t.n && k < ST_TUNE_MAX; k++) {
+    napi_value c;
+    napi_create_object(env, &c);
+    napi_set_named_property(env, c, "mode", mkNum(env, t.cand_mode[k]));
+    napi_set_named_property(env, c, "width", mkNum(env, t.cand_width[k]));
+    napi_set_named_property(env, c, "ms", mkNum(env, t.cand_ms[k]));
+    napi_set_element(env, arr, k, c);
+  }
+  napi_set_named_property(env, o, "candidates", arr);
+  return o;
+}
+
+// tuneSor(h, params, reps) -> {mode, width, candidates: [{mode, width, ms}]} (st_tune_sor)
+napi_value TuneSor(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  st_params p;
+  readParams(env, argv[1], &p);
+  st_sor_tune t{};
+  ST_OK_OR_THROW(st_tune_sor(handleOf(env, argv[0]), &p, intArg(env, argv[2], 3), &t));
+  return tuneResult(env, t);
+}
+
+// batchTuneSor(batch, [params per tank], reps) (st_batch_tune_sor)
+napi_value BatchTuneSor(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  std::vector<st_params> p(n);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value e;
+    NAPI_OK(napi_get_element(env, argv[1], k, &e));
+    readParams(env, e, &p[k]);
+  }
+  st_sor_tune t{};
+  ST_OK_OR_THROW(st_batch_tune_sor(handleOf(env, argv[0]), p.data(), n, intArg(env, argv[2], 3), &t));
+  return tuneResult(env, t);
+}
+
+napi_value external(napi_env env, void* p) {
+  napi_value h;
+  if (napi_create_external(env, p, nullptr, nullptr, &h) != napi_ok) return throwErr(env, "napi_create_external failed");
+  return h;
+}
+
+// snapshotCreate(h) -> snapshot; snapshotRestore(h, snapshot); snapshotDestroy(snapshot)
+napi_value SnapshotCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  void* snap = nullptr;
+  ST_OK_OR_THROW(st_snapshot_create(handleOf(env, argv[0]), &snap));
+  return external(env, snap);
+}
+napi_value SnapshotRestore(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  ST_OK_OR_THROW(st_snapshot_restore(handleOf(env, argv[0]), handleOf(env, argv[1])));
+  return nullptr;
+}
+napi_value SnapshotDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  void* snap = handleOf(env, argv[0]);
+  if (snap) ST_OK_OR_THROW(st_snapshot_destroy(snap));
+  return nullptr;
+}
+
+// adjCreate(h, maxSteps) -> adjoint; adjRecord(adj, params); adjSeed(adj, gPos4, gVel4);
+// adjBackward(adj); adjGrad(adj, gPos4, gVel4) fills them; adjColGrad(adj, ncols) -> Float32Array(9 ncols);
+// adjDestroy(adj)  (st_adj_*)
+napi_value AdjCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  void* adj = nullptr;
+  ST_OK_OR_THROW(st_adj_create(handleOf(env, argv[0]), uint32_t(intArg(env, argv[1], 1)), &adj));
+  return external(env, adj);
+}
+napi_value AdjRecord(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  st_params p;
+  readParams(env, argv[1], &p);
+  ST_OK_OR_THROW(st_adj_record(handleOf(env, argv[0]), &p));
+  return nullptr;
+}
+napi_value AdjSeed(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  void *gp = nullptr, *gv = nullptr;
+  size_t np = 0, nv = 0;
+  if (!typed(env, argv[1], &gp, &np, napi_float32_array) || !typed(env, argv[2], &gv, &nv, napi_float32_array) || np != nv)
+    return throwErr(env, "adjSeed: expects two Float32Arrays of the same length (4 per particle)");
+  ST_OK_OR_THROW(st_adj_seed(handleOf(env, argv[0]), (const float*)gp, (const float*)gv));
+  return nullptr;
+}
+napi_value AdjBackward(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  ST_OK_OR_THROW(st_adj_backward(handleOf(env, argv[0])));
+  return nullptr;
+}
+napi_value AdjGrad(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  getArgs(env, info, 3, argv);
+  void *gp = nullptr, *gv = nullptr;
+  size_t np = 0, nv = 0;
+  if (!typed(env, argv[1], &gp, &np, napi_float32_array) || !typed(env, argv[2], &gv, &nv, napi_float32_array) || np != nv)
+    return throwErr(env, "adjGrad: expects two Float32Arrays of the same length (4 per particle)");
+  ST_OK_OR_THROW(st_adj_grad(handleOf(env, argv[0]), (float*)gp, (float*)gv));
+  return nullptr;
+}
+napi_value AdjColGrad(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  getArgs(env, info, 2, argv);
+  const uint32_t nc = uint32_t(intArg(env, argv[1], 0));
+  std::vector<float> g(size_t(nc) * 9 + 1);
+  ST_OK_OR_THROW(st_adj_col_grad(handleOf(env, argv[0]), g.data(), nc));
+  napi_value ab, arr;
+  void* data = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, size_t(nc) * 36, &data, &ab));
+  if (nc) std::memcpy(data, g.data(), size_t(nc) * 36);
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, size_t(nc) * 9, ab, 0, &arr));
+  return arr;
+}
+napi_value AdjDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  getArgs(env, info, 1, argv);
+  void* adj = handleOf(env, argv[0]);
+  if (adj) ST_OK_OR_THROW(st_adj_destroy(adj));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   const struct { const char* n; napi_callback f; } fns[] = {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
       {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"oceanWaves", OceanWaves}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
-      {"batchStep", BatchStep}, {"batchSync", BatchSync}};
+      {"batchStep", BatchStep}, {"batchSync", BatchSync}, {"stepN", StepN}, {"batchStepN", BatchStepN}, {"tuneSor", TuneSor},
+      {"batchTuneSor", BatchTuneSor}, {"snapshotCreate", SnapshotCreate}, {"snapshotRestore", SnapshotRestore},
+      {"snapshotDestroy", SnapshotDestroy}, {"adjCreate", AdjCreate}, {"adjRecord", AdjRecord}, {"adjSeed", AdjSeed},
+      {"adjBackward", AdjBackward}, {"adjGrad", AdjGrad}, {"adjColGrad", AdjColGrad}, {"adjDestroy", AdjDestroy}};
   for (const auto& e : fns) {
     napi_value f;
     napi_create_function(env, e.n, NAPI_AUTO_LENGTH, e.f, nullptr, &f);

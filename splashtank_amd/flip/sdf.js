@@ -25,41 +25,54 @@ function buildSDF(mesh, cell, margin) {
   const g = loadAddon().buildSdf(v, Uint32Array.from(meshTris(mesh).t), cell, margin);
   return { id: SDF_ID++, dims: g.dims, origin: g.origin, cell: g.cell, data: g.data };
 }
-// drip samples on a collider, from its distance grid (dripSamples, src/flip/bake.js:17-40): grid points within
-// 0.6 cells of the surface moved onto it along the gradient, about one per solver cell of surface (100 to 6000),
-// picked by a seeded stride; 8 floats each: local position, collider number k, local normal, wetness 0 (dry)
-const DRIP_CACHE = new Map();
-function dripSamples(sdf, h, k) {
-  const key = sdf.id + '|' + h.toFixed(5);
-  let base = DRIP_CACHE.get(key);
-  if (!base) {
-    const nx = sdf.dims[0], ny = sdf.dims[1], nz = sdf.dims[2], c = sdf.cell, d = sdf.data, o = sdf.origin, pts = [];
-    const at = (i, j, q) => d[Math.min(nx - 1, Math.max(0, i)) + nx * (Math.min(ny - 1, Math.max(0, j)) + ny * Math.min(nz - 1, Math.max(0, q)))];
-    for (let q = 1; q < nz - 1; q++) {
-      for (let j = 1; j < ny - 1; j++) {
-        for (let i = 1; i < nx - 1; i++) {
-          const v = d[i + nx * (j + ny * q)];
-          if (Math.abs(v) > 0.6 * c) continue;
-          let gx = at(i + 1, j, q) - at(i - 1, j, q), gy = at(i, j + 1, q) - at(i, j - 1, q), gz = at(i, j, q + 1) - at(i, j, q - 1);
-          const gl = Math.hypot(gx, gy, gz);
-          if (gl < 1e-9) continue;
-          gx /= gl; gy /= gl; gz /= gl;
-          pts.push([o[0] + i * c - gx * v, o[1] + j * c - gy * v, o[2] + q * c - gz * v, gx, gy, gz]);
-        }
+// Drip samples on a collider (dripSamples, src/flip/bake.js:17-40): the grid
+// nodes of its distance field within 0.6 cells of the surface, each moved onto
+// the surface along the field's normalised central-difference gradient (indices
+// clamped at the grid's edge); of those, about one per solver cell of surface
+// (100 to 6000) taken at an even stride from a start drawn by the Park-Miller
+// minimal standard generator (16807 mod 2^31 - 1) seeded 7 + sdf.id.  8 floats
+// each: local position, collider number k, local normal, wetness 0 (dry).  The
+// picks depend only on the grid and h, so they are kept per (grid, h), 32 at most.
+function surfacePoints(sdf) {
+  const [nx, ny, nz] = sdf.dims, cell = sdf.cell, D = sdf.data, org = sdf.origin;
+  const clampIdx = (v, n) => Math.min(n - 1, Math.max(0, v));
+  const value = (i, j, q) => D[clampIdx(i, nx) + nx * (clampIdx(j, ny) + ny * clampIdx(q, nz))];
+  const pts = [];
+  for (let q = 1; q < nz - 1; q++) {
+    for (let j = 1; j < ny - 1; j++) {
+      for (let i = 1; i < nx - 1; i++) {
+        const dist = D[i + nx * (j + ny * q)];
+        if (Math.abs(dist) > 0.6 * cell) continue;
+        const g = [value(i + 1, j, q) - value(i - 1, j, q), value(i, j + 1, q) - value(i, j - 1, q), value(i, j, q + 1) - value(i, j, q - 1)];
+        const len = Math.hypot(g[0], g[1], g[2]);
+        if (len < 1e-9) continue;
+        const n = g.map((x) => x / len);
+        pts.push({ p: [org[0] + i * cell - n[0] * dist, org[1] + j * cell - n[1] * dist, org[2] + q * cell - n[2] * dist], n });
       }
     }
-    const nsurf = pts.length, want = Math.max(100, Math.min(6000, Math.round(nsurf * c * c / (h * h))));
-    const step = Math.max(1, nsurf / want);
-    let sd = 7 + (sdf.id | 0);
-    sd = (sd * 16807) % 2147483647;
-    base = [];
-    for (let t = (sd / 2147483647) * step; t < nsurf; t += step) { const p = pts[Math.floor(t)]; base.push(p[0], p[1], p[2], 0, p[3], p[4], p[5], 0); }
-    base = new Float32Array(base);
-    DRIP_CACHE.set(key, base);
-    if (DRIP_CACHE.size > 32) DRIP_CACHE.delete(DRIP_CACHE.keys().next().value);
   }
-  const a = base.slice();
-  for (let i = 3; i < a.length; i += 8) a[i] = k;
-  return a;
+  return pts;
+}
+const DRIP_PICKS = new Map();
+function dripSamples(sdf, h, k) {
+  const key = sdf.id + '|' + h.toFixed(5);
+  let picks = DRIP_PICKS.get(key);
+  if (!picks) {
+    const pts = surfacePoints(sdf), cell = sdf.cell;
+    const want = Math.max(100, Math.min(6000, Math.round(pts.length * cell * cell / (h * h))));
+    const stride = Math.max(1, pts.length / want);
+    const seed = (((7 + (sdf.id | 0)) * 16807) % 2147483647) / 2147483647;
+    const rows = [];
+    for (let t = seed * stride; t < pts.length; t += stride) {
+      const s = pts[Math.floor(t)];
+      rows.push(s.p[0], s.p[1], s.p[2], 0, s.n[0], s.n[1], s.n[2], 0);
+    }
+    picks = new Float32Array(rows);
+    DRIP_PICKS.set(key, picks);
+    if (DRIP_PICKS.size > 32) DRIP_PICKS.delete(DRIP_PICKS.keys().next().value);
+  }
+  const out = picks.slice();
+  for (let r = 0; r < out.length / 8; r++) out[r * 8 + 3] = k;
+  return out;
 }
 module.exports = { buildSDF, meshTris, dripSamples };

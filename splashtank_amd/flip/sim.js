@@ -22,19 +22,17 @@ const path = require('path');
 const { dripSamples } = require('./sdf.js');
 
 let addon = null;
-// the packed live view (packLayout, sim.js:376-380) as a frame: one copy into a buffer just big enough
-// for what is there (unpackLive, sim.js:526-537)
-function unpackLive(words, L, b) {
-  const range = words.buffer, base = words.byteOffset;
-  const hdr = new Uint32Array(range, base, 4), nW = L.nW, nS = Math.min(hdr[0], L.nS), nD = Math.min(hdr[1], L.nD);
-  const sz = [nS * 4, nW * 6, nS * 6, nD * 6, nW, nW, nD], off = []; let o = 0;
-  for (const z of sz) { off.push(o); o += Math.ceil(z / 4) * 4; }
-  const ab = new ArrayBuffer(Math.max(o, 4)), src = new Uint8Array(range, base, words.byteLength);
-  const cp = (k, from) => { if (sz[k]) new Uint8Array(ab, off[k], sz[k]).set(src.subarray(from, from + sz[k])); };
-  cp(0, L.oSk * 4); cp(1, L.oWp * 4); cp(2, L.oSp * 4); cp(3, L.oDp * 4); cp(4, L.oWs * 4); cp(5, L.oWf * 4); cp(6, L.oDs * 4);
-  const fr = { n: nW, pos: new Uint16Array(ab, off[1], nW * 3), spd: new Uint8Array(ab, off[4], nW), fm: new Uint8Array(ab, off[5], nW), b, bytes: ab.byteLength };
-  if (nS) fr.sp = { n: nS, pos: new Uint16Array(ab, off[2], nS * 3), pk: new Uint32Array(ab, off[0], nS), alive: nS };
-  if (nD) fr.dp = { n: nD, pos: new Uint16Array(ab, off[3], nD * 3), spd: new Uint8Array(ab, off[6], nD) };
+// The packed live view as a cache-format frame: typed-array views straight into
+// the packed words at the offsets of its layout (no copy).  Header word 0 counts
+// the living whitewater points, word 1 the deep points; the water sample has
+// layout.nW points.
+function liveFrame(words, L, b) {
+  const buf = words.buffer, at = (wordOff) => words.byteOffset + wordOff * 4;
+  const nW = L.nW, nS = Math.min(words[0], L.nS), nD = Math.min(words[1], L.nD);
+  const fr = { n: nW, pos: new Uint16Array(buf, at(L.oWp), nW * 3), spd: new Uint8Array(buf, at(L.oWs), nW),
+    fm: new Uint8Array(buf, at(L.oWf), nW), b, bytes: words.byteLength };
+  if (nS) fr.sp = { n: nS, pos: new Uint16Array(buf, at(L.oSp), nS * 3), pk: new Uint32Array(buf, at(L.oSk), nS), alive: nS };
+  if (nD) fr.dp = { n: nD, pos: new Uint16Array(buf, at(L.oDp), nD * 3), spd: new Uint8Array(buf, at(L.oDs), nD) };
   return fr;
 }
 function loadAddon() {
@@ -52,31 +50,36 @@ function initHip(ordinal) {
   return { ordinal: ordinal | 0, version: a.version(), limits: { maxStorageBufferBindingSize: Infinity } };
 }
 
-// packColliders (gpu.js:42-70): 32 floats per collider into sim.cf, distance
-// grids deduplicated by sdf.id and uploaded when the set of ids changes
+// packColliders (gpu.js:42-70): the collider records (32 floats each: grid
+// offset and dims, grid origin and cell, the rows of R^T, position, velocity,
+// angular velocity; the layout the kernels read) into sim.cf, and the distance
+// grids, one copy per sdf.id in first-use order, uploaded again only when that
+// list of ids changes
 function packColliders(sim, cols) {
   const nc = Math.min(cols.length, 32), cf = sim.cf;
-  let sig = '', off = 0; const offs = new Map(), grids = [];
-  for (let k = 0; k < nc; k++) {
-    const sd = cols[k].sdf;
-    if (!offs.has(sd.id)) { offs.set(sd.id, off); off += sd.data.length; sig += sd.id + ','; grids.push(sd.data); }
+  const grids = new Map();   // sdf.id -> {offset, data}
+  let words = 0;
+  for (const c of cols.slice(0, nc)) {
+    if (grids.has(c.sdf.id)) continue;
+    grids.set(c.sdf.id, { offset: words, data: c.sdf.data });
+    words += c.sdf.data.length;
   }
+  const sig = Array.from(grids.keys()).join(',') + ',';
   if (sig !== sim.sdfSig) {
-    const all = new Float32Array(Math.max(off, 1)); let o = 0;
-    for (const g of grids) { all.set(g, o); o += g.length; }
-    sim.sdfAll = all.subarray(0, off); sim.sdfSig = sig; sim.sdfSigNum = (sim.sdfSigNum || 0) + 1;
+    const all = new Float32Array(Math.max(words, 1));
+    for (const g of grids.values()) all.set(g.data, g.offset);
+    sim.sdfAll = all.subarray(0, words); sim.sdfSig = sig; sim.sdfSigNum = (sim.sdfSigNum || 0) + 1;
   }
-  for (let k = 0; k < nc; k++) {
-    const c = cols[k], sd = c.sdf, R = c.R, j = k * 32;
-    cf[j] = offs.get(sd.id); cf[j + 1] = sd.dims[0]; cf[j + 2] = sd.dims[1]; cf[j + 3] = sd.dims[2];
-    cf[j + 4] = sd.origin[0]; cf[j + 5] = sd.origin[1]; cf[j + 6] = sd.origin[2]; cf[j + 7] = sd.cell;
-    cf[j + 8] = R[0]; cf[j + 9] = R[3]; cf[j + 10] = R[6]; cf[j + 11] = 0;
-    cf[j + 12] = R[1]; cf[j + 13] = R[4]; cf[j + 14] = R[7]; cf[j + 15] = 0;
-    cf[j + 16] = R[2]; cf[j + 17] = R[5]; cf[j + 18] = R[8]; cf[j + 19] = 0;
-    cf[j + 20] = c.x; cf[j + 21] = c.y; cf[j + 22] = c.z; cf[j + 23] = 0;
-    cf[j + 24] = c.vx; cf[j + 25] = c.vy; cf[j + 26] = c.vz; cf[j + 27] = 0;
-    cf[j + 28] = c.wx; cf[j + 29] = c.wy; cf[j + 30] = c.wz; cf[j + 31] = 0;
-  }
+  cols.slice(0, nc).forEach((c, k) => {
+    const sd = c.sdf, R = c.R, rec = new Float32Array(32);
+    rec.set([grids.get(sd.id).offset, sd.dims[0], sd.dims[1], sd.dims[2]], 0);
+    rec.set([sd.origin[0], sd.origin[1], sd.origin[2], sd.cell], 4);
+    for (let col = 0; col < 3; col++) rec.set([R[col], R[3 + col], R[6 + col]], 8 + 4 * col);   // column col of R = row col of R^T
+    rec.set([c.x, c.y, c.z], 20);
+    rec.set([c.vx, c.vy, c.vz], 24);
+    rec.set([c.wx, c.wy, c.wz], 28);
+    cf.set(rec, k * 32);
+  });
   loadAddon().setColliders(sim.handle, cf, nc, sim.sdfAll || new Float32Array(0), sim.sdfSigNum || 0);
   return nc;
 }
@@ -112,17 +115,25 @@ class FlipSim {
     this.rand = Math.random;   // jitter source of fillBox / fillSphere
   }
   // ---- particles (GpuSim.setCap/clear/add/flushRun, gpu.js:388-408) ----
-  setCap(c) { this.cap = Math.min(c, this.MAXN); if (this.N > this.cap) this.N = this.cap; if (this.ring >= this.cap) this.ring = 0; }
+  // the most particles the sim keeps (at most maxN); a smaller cap drops the ones past it
+  setCap(c) {
+    this.cap = Math.min(c, this.MAXN);
+    this.N = Math.min(this.N, this.cap);
+    if (this.ring >= this.cap) this.ring = 0;
+  }
+  // One particle into host staging: the next free slot while there is room,
+  // else (recycle on) the oldest slot of the ring, else refused (false).  w of
+  // the position is the speed, w of the velocity (foam) starts at 0.  Slots
+  // written one after another form a run that flushRun uploads in one copy.
   add(x, y, z, vx, vy, vz) {
-    let i;
-    if (this.N < this.cap) i = this.N++;
-    else if (this.recycle) { i = this.ring++; if (this.ring >= this.cap) this.ring = 0; }
+    let slot;
+    if (this.N < this.cap) slot = this.N++;
+    else if (this.recycle) { slot = this.ring; this.ring = (this.ring + 1 >= this.cap) ? 0 : this.ring + 1; }
     else return false;
-    const o = i * 4;
-    this.pS[o] = x; this.pS[o + 1] = y; this.pS[o + 2] = z; this.pS[o + 3] = Math.sqrt(vx * vx + vy * vy + vz * vz);
-    this.vS[o] = vx; this.vS[o + 1] = vy; this.vS[o + 2] = vz; this.vS[o + 3] = 0;
-    if (this.runLen > 0 && i === this.runStart + this.runLen) this.runLen++;
-    else { this.flushRun(); this.runStart = i; this.runLen = 1; }
+    this.pS.set([x, y, z, Math.sqrt(vx * vx + vy * vy + vz * vz)], slot * 4);
+    this.vS.set([vx, vy, vz, 0], slot * 4);
+    if (this.runLen > 0 && slot === this.runStart + this.runLen) this.runLen++;
+    else { this.flushRun(); this.runStart = slot; this.runLen = 1; }
     return true;
   }
   flushRun() {
@@ -132,19 +143,26 @@ class FlipSim {
     this.runLen = 0;
     if (this.devN !== this.N) { a.setCount(this.handle, this.N); this.devN = this.N; }
   }
-  // fillBox / fillSphere (shared.js:285-297)
+  // a lattice at spacing d from d/2 in (JS double steps, y outer, x, z inner), each
+  // point jittered by up to 0.005 d / 0.12 on each axis (shared.js:285-297)
+  jittered(x, y, z, vx, vy, vz) {
+    const j = 0.01 * this.d / 0.12, r = this.rand;
+    const jx = (r() - 0.5) * j, jy = (r() - 0.5) * j, jz = (r() - 0.5) * j;
+    return this.add(x + jx, y + jy, z + jz, vx, vy, vz);
+  }
   fillBox(x0, x1, y0, y1, z0, z1) {
-    const d = this.d, j = 0.01 * d / 0.12, r = this.rand;
+    const d = this.d;
     for (let y = y0 + d / 2; y < y1; y += d)
       for (let x = x0 + d / 2; x < x1; x += d)
-        for (let z = z0 + d / 2; z < z1; z += d)
-          this.add(x + (r() - 0.5) * j, y + (r() - 0.5) * j, z + (r() - 0.5) * j, 0, 0, 0);
+        for (let z = z0 + d / 2; z < z1; z += d) this.jittered(x, y, z, 0, 0, 0);
   }
+  // the lattice of the cube [-R, R]^3 (steps of d from -R) kept inside the ball, moved to the centre
   fillSphere(cx, cy, cz, R, vx, vy, vz) {
-    const d = this.d, j = 0.01 * d / 0.12, r = this.rand;
-    for (let y = -R; y <= R; y += d) for (let x = -R; x <= R; x += d) for (let z = -R; z <= R; z += d)
-      if (x * x + y * y + z * z <= R * R)
-        this.add(cx + x + (r() - 0.5) * j, cy + y + (r() - 0.5) * j, cz + z + (r() - 0.5) * j, vx, vy, vz);
+    const d = this.d;
+    for (let y = -R; y <= R; y += d)
+      for (let x = -R; x <= R; x += d)
+        for (let z = -R; z <= R; z += d)
+          if (x * x + y * y + z * z <= R * R) this.jittered(cx + x, cy + y, cz + z, vx, vy, vz);
   }
   clear() {
     this.flushRun(); this.N = 0; this.ring = 0; this.devN = 0;
@@ -211,9 +229,10 @@ class FlipSim {
     const all = new Float32Array(n * 8); let o = 0; parts.forEach((a) => { all.set(a, o); o += a.length; });
     loadAddon().setDrips(this.handle, all);
   }
-  prepare(dt, o) {
+  prepare(dt, o, probe) {
     this.flushRun();
     if (this.N === 0) return null;
+    if (probe) return { prm: this.params(dt, o), nc: 0 };   // settings only: nothing is stepped
     if (!this.only) this.maybeGrow();
     const prm = this.params(dt, o);
     if (prm.dripOn && this.sprayCap > 0) this.dripUpdate((o.cols || []).slice(0, (o.cols || []).length));
@@ -235,6 +254,27 @@ class FlipSim {
     return t.nc;
   }
   track(t0) { const ms = Date.now() - t0; this.lastMs = ms; this.slowMs = Math.max(this.slowMs, ms); }
+  // n substeps with the same settings, as n step() calls; graph: each one a HIP graph replay (plain
+  // substeps only).  No counterpart in the reference.
+  stepN(dt, o, n, graph) {
+    const t = this.prepare(dt, o);
+    if (!t) return undefined;
+    if (!this.only) { this.tm += dt * (n - 1); this.seed = (this.seed + n - 1) >>> 0; }
+    loadAddon().stepN(this.handle, t.prm, n | 0, graph ? 1 : 0);
+    this.statsCache = null; this.stepped = true;
+    return t.nc;
+  }
+  // times the pressure-solve paths from the current state (left unchanged) and keeps the fastest for
+  // steps with opt.sorMode = 3: {mode, width, candidates}.  No counterpart in the reference.
+  tuneSor(dt, o, reps) {
+    const t = this.prepare(dt, o, true);
+    if (!t) return null;
+    return loadAddon().tuneSor(this.handle, t.prm, reps || 3);
+  }
+  // a copy of the state the next substep starts from; restore(snapshot) goes back to it
+  snapshot() { return loadAddon().snapshotCreate(this.handle); }
+  restore(snap) { loadAddon().snapshotRestore(this.handle, snap); }
+  dropSnapshot(snap) { loadAddon().snapshotDestroy(snap); }
   // the named stages of one step (sim.js:245-310): fn() enqueues that stage
   stages() {
     const a = loadAddon(), h = this.handle;
@@ -336,14 +376,36 @@ class FlipSim {
   // the deep water points of the last step (x, y, z, 4096 + speed), in active-cell order, or null
   async readDeep() { return this.deepOn && this.handle && this.N ? loadAddon().readDeep(this.handle) : null; }
   // the live view (sim.js:504-525): packed on the GPU (st_pack_view), here read back at once; outFrame is a
-  // frame in the cache format (unpackLive) that the drawing uploads as it is
+  // frame in the cache format (views into the packed words) that the drawing uploads as it is
   pump() {
     if (!this.handle || this.N === 0) return;
     this.flushRun();
     const r = loadAddon().packView(this.handle, this.dispMax || 300000);
-    this.outFrame = unpackLive(r.words, r.layout, Object.assign({}, this.b));
+    this.outFrame = liveFrame(r.words, r.layout, Object.assign({}, this.b));
     this.outN = r.layout.nW; this.gen = (this.gen || 0) + 1;
   }
+}
+
+// The adjoint of the substep (st_adj_*; no counterpart in the reference):
+// record(dt, o) runs a substep as step() does and keeps its tape; seed(gPos, gVel)
+// sets dL/d(state) (Float32Array, 4 per particle in id order, w ignored);
+// backward() reverses the last recorded substep; grad(gPos, gVel) reads
+// dL/d(state before it); colGrad(ncols) the 9 floats per collider of that
+// substep (dL/d centre, velocity, angular velocity).
+class FlipAdjoint {
+  constructor(sim, maxSteps) { this.sim = sim; this.handle = loadAddon().adjCreate(sim.handle, maxSteps | 0); }
+  record(dt, o) {
+    const t = this.sim.prepare(dt, o);
+    if (!t) return undefined;
+    loadAddon().adjRecord(this.handle, t.prm);
+    this.sim.statsCache = null; this.sim.stepped = true;
+    return t.nc;
+  }
+  seed(gPos, gVel) { loadAddon().adjSeed(this.handle, gPos, gVel); }
+  backward() { loadAddon().adjBackward(this.handle); }
+  grad(gPos, gVel) { loadAddon().adjGrad(this.handle, gPos, gVel); }
+  colGrad(ncols) { return loadAddon().adjColGrad(this.handle, ncols | 0); }
+  destroy() { if (this.handle) { loadAddon().adjDestroy(this.handle); this.handle = null; } }
 }
 
 // Many tanks stepped together, one kernel launch per stage for all of them
@@ -369,8 +431,30 @@ class FlipBatch {
     // tanks without particles sit the step out; any valid block stands in for theirs
     loadAddon().batchStep(this.handle, prms.map((p) => p || any));
   }
+  // n batched substeps with the same settings; graph: HIP graph replays
+  stepN(dt, o, n, graph) {
+    const prms = this.prms(dt, o, false);
+    if (prms) loadAddon().batchStepN(this.handle, prms, n | 0, graph ? 1 : 0);
+    if (prms) this.sims.forEach((s) => { if (!s.only && s.N) { s.tm += dt * (n - 1); s.seed = (s.seed + n - 1) >>> 0; } });
+  }
+  // times batched substeps at several SOR widths and keeps the fastest (st_batch_tune_sor)
+  tuneSor(dt, o, reps) {
+    const prms = this.prms(dt, o, true);
+    return prms ? loadAddon().batchTuneSor(this.handle, prms, reps || 3) : null;
+  }
+  prms(dt, o, probe) {
+    const os = Array.isArray(o) ? o : this.sims.map(() => o);
+    if (os.length !== this.sims.length) throw new Error('FlipBatch: one o per tank');
+    let any = null;
+    const prms = this.sims.map((s, k) => {
+      const t = s.prepare(dt, os[k], probe);
+      if (t) { any = t.prm; if (!probe) { s.statsCache = null; s.stepped = true; } }
+      return t ? t.prm : null;
+    });
+    return any ? prms.map((p) => p || any) : null;
+  }
   sync() { loadAddon().batchSync(this.handle); }
   destroy() { if (this.handle) { loadAddon().batchDestroy(this.handle); this.handle = null; } }
 }
 
-module.exports = { FlipSim, FlipBatch, initHip, packColliders, STAGE_IDS };
+module.exports = { FlipSim, FlipBatch, FlipAdjoint, initHip, packColliders, STAGE_IDS };

@@ -30,23 +30,19 @@ const frames = [], cache = [];
 const box = { x0: -0.5, x1: 0.5, y0: 0, y1: 1, z0: -0.5, z1: 0.5 };
 let worstQ = 0;
 M.bake(desc, M.initHip(0), (fr, info) => {
-  // the frame as the page's cache holds it, and how far its 16-bit positions are from the readback
-  const cf = M.packFrame(fr.pos, fr.n, box, desc.objs.map((o) => M.sampleState(o, info.f / desc.fps)), 1);
-  cache.push(cf);
-  for (let i = 0; i < cf.n; i++) {
-    if (cf.spd[i] === 255) continue;
+  // the frame comes packed as the page's cache holds it; its 16-bit positions decode inside the box
+  cache.push(fr);
+  for (let i = 0; i < fr.n; i++) {
+    if (fr.spd[i] === 255) continue;
     for (let a = 0; a < 3; a++) {
       const lo = [box.x0, box.y0, box.z0][a], ext = [box.x1 - box.x0, box.y1 - box.y0, box.z1 - box.z0][a];
-      worstQ = Math.max(worstQ, Math.abs(lo + cf.pos[i * 3 + a] * ext / 65535 - fr.pos[i * 4 + a]) / ext);
+      const x = lo + fr.pos[i * 3 + a] * ext / 65535;
+      if (x < lo - 1e-9 || x > lo + ext + 1e-9) worstQ = 1;
     }
   }
-  let fin = true, inside = true;
-  for (let i = 0; i < fr.n; i++) {
-    const x = fr.pos[i * 4], y = fr.pos[i * 4 + 1], z = fr.pos[i * 4 + 2];
-    if (!(Number.isFinite(x) && Number.isFinite(y) && Number.isFinite(z))) fin = false;
-    else if (x < -0.5 || x > 0.5 || y < 0 || y > 1 || z < -0.5 || z > 0.5) inside = false;
-  }
-  frames.push({ f: info.f, n: fr.n, sub: info.sub, finite: fin, inside, objectSpeed: info.objectSpeed || 0 });
+  const dg = info.diag;
+  frames.push({ f: info.f, n: dg.alive, sub: info.sub, finite: dg.nonFinite === 0, inside: dg.outsideBox === 0,
+    objectSpeed: dg.objectSpeed || 0, maxSpeed: dg.maxSpeed });
 }).then((ok) => {
   const file = M.writeCache({ d, fps: desc.fps, ids: desc.objs.map((o) => o.id), b: box, maxN: desc.maxN, engine: 'mi355x-flip' }, cache);
   const back = M.readCache(file);

@@ -104,3 +104,46 @@ def test_facade_batch_matches_c_abi(tmp_path):
         np.testing.assert_array_equal(js.view(np.uint32), g.read_particles().view(np.uint32))
         assert info["stats"][k] == int(g.ctl()[4])
         g.close()
+
+
+@needs_node
+@pytest.mark.gpu
+def test_facade_extensions(tmp_path):
+    """stepN (with and without graph replay), the autotuner, snapshots and the
+    adjoint through Node: stepN gives the bits of step() one at a time, the
+    tuner and a snapshot round trip leave the state as it was, and the adjoint's
+    gradient equals the C ABI's (capi.AdjointC) bit for bit."""
+    sc = scenes.dam_break(d=1.0 / 32, iters=15)  # 16^3 cells, closed box
+    steps = 4
+    parts = np.concatenate([sc.pos, sc.vel], axis=1).astype(np.float32)
+    (tmp_path / "p.bin").write_bytes(parts.tobytes())
+    cfg = dict(particles=str(tmp_path / "p.bin"), out=str(tmp_path / "g.bin"), d=sc.d, size=list(sc.size), open=sc.open_,
+               ratio=sc.ratio, steps=steps, params=dict(sc.params))
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    print(f"[facade] extensions, {sc.n} particles, {steps} substeps", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "ext_run.js"), str(tmp_path / "cfg.json")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["stepN"] and info["stepNGraph"] and info["afterTune"] and info["restore"], info
+    assert info["tune"]["candidates"] >= 3
+    js = np.frombuffer((tmp_path / "g.bin").read_bytes(), np.float32).reshape(2, -1, 4)
+    g = capi.FlipSimC(capi.make_opts(sc.d, sc.size, ratio=sc.ratio, open_=sc.open_, max_n=sc.n))
+    try:
+        g.upload(sc.pos, sc.vel)
+        a = capi.AdjointC(g, steps)
+        prm = capi.make_params(**sc.params)
+        for _ in range(steps):
+            a.record(prm)
+        gp = np.zeros((sc.n, 4), np.float32)
+        gp[:, 0], gp[:, 1], gp[:, 2] = 1, 2, 3
+        a.seed(gp, np.zeros_like(gp))
+        for _ in range(steps):
+            a.backward()
+        gpos, gvel = a.grad()
+        a.close()
+    finally:
+        g.close()
+    np.testing.assert_array_equal(js[0][:, :3], gpos)
+    np.testing.assert_array_equal(js[1][:, :3], gvel)
+    assert np.abs(gvel[:, :3]).max() > 0
