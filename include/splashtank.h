@@ -134,7 +134,7 @@ typedef struct st_params {
 
 #define ST_OCEAN_MAX_WAVES 32u
 
-/* The ocean's settings (o.ocean, src/app/*: height, length, dir, spread, seed). */
+/* The ocean's settings (o.ocean as the reference's page sets it: height, length, dir, spread, seed). */
 typedef struct st_ocean {
   double height;  /* significant wave height */
   double length;  /* peak wavelength */
@@ -349,6 +349,15 @@ typedef struct st_norms {
 /* Enqueues the two reduction kernels on the sim's stream after its last
  * substep and waits for them.  ST_ERR_STATE before the first substep. */
 int st_solve_norms(void* sim, st_norms* out);
+/* st_solve_norms without any device allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc).  Its per-workgroup partial sums and results
+ * live in one fixed array of 4097 x 8 doubles (256 KiB) per GPU, looked up on
+ * the first call on that GPU and held until the process ends; every call on a
+ * GPU shares it, so calls that overlap on one GPU race.  ST_ERR_ARG when the
+ * sim's launch needs more than the array holds (its reductions use at most
+ * 4096 workgroups).  The same results as st_solve_norms. */
+int st_solve_norms_static(void* sim, st_norms* out);
 
 /* Autotuner of the pressure-solve schedule.  No counterpart in the reference
  * (its solve has one fixed dispatch pattern, src/flip/sim.js:291-293).  Every

@@ -212,6 +212,7 @@ PROTOS = {
     "st_mem_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "st_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_solve_norms": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
+    "st_solve_norms_static": (C.c_int, [C.c_void_p, C.POINTER(StNorms)]),
     "st_set_spray_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
     "st_set_drips": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "st_pack_view": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(StViewLayout)]),
@@ -429,6 +430,12 @@ class FlipSimC(_Base):
         """FP64 norms of the last substep's solve and projection (st_solve_norms), as a dict."""
         n = StNorms()
         self._call("solve_norms", self.h, C.byref(n))
+        return {f: float(getattr(n, f)) for f, _ in StNorms._fields_}
+
+    def solve_norms_static(self):
+        """The same norms through st_solve_norms_static (no device allocation)."""
+        n = StNorms()
+        self._call("solve_norms_static", self.h, C.byref(n))
         return {f: float(getattr(n, f)) for f, _ in StNorms._fields_}
 
     def tune_sor(self, prm: StParams, reps: int = 3):

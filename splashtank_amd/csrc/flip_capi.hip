@@ -1303,6 +1303,24 @@ int st_pack_view(void* p, uint32_t disp_max, uint32_t* out, size_t cap_words, st
   return ST_OK;
 }
 
+// the norms kernels with partials in `part` and results in `res`, read back into *out
+static int normsInto(Sim* s, const LaunchCtx& L, double* part, double* res, st_norms* out) {
+  launch_norms(s->st, L, part, res);
+  HIPCHK(hipGetLastError());
+  double h[ST_NORM_WORDS];
+  HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  out->res_l2 = std::sqrt(h[0]);
+  out->res_max = h[1];
+  out->rhs_l2 = std::sqrt(h[2]);
+  out->div_l2 = std::sqrt(h[3]);
+  out->div_max = h[4];
+  out->err_l2 = std::sqrt(h[5]);
+  out->err_max = h[6];
+  out->fluid = h[7];
+  return ST_OK;
+}
+
 int st_solve_norms(void* p, st_norms* out) {
   Sim* s = getSim(p);
   if (!s || !out) return fail(ST_ERR_ARG, "st_solve_norms: null argument");
@@ -1317,21 +1335,23 @@ int st_solve_norms(void* p, st_norms* out) {
     if (rc) return rc;
     s->normCap = L.cellGrid;
   }
-  double* res = s->normPart + size_t(s->normCap) * ST_NORM_WORDS;
-  launch_norms(s->st, L, s->normPart, res);
-  HIPCHK(hipGetLastError());
-  double h[ST_NORM_WORDS];
-  HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  out->res_l2 = std::sqrt(h[0]);
-  out->res_max = h[1];
-  out->rhs_l2 = std::sqrt(h[2]);
-  out->div_l2 = std::sqrt(h[3]);
-  out->div_max = h[4];
-  out->err_l2 = std::sqrt(h[5]);
-  out->err_max = h[6];
-  out->fluid = h[7];
-  return ST_OK;
+  return normsInto(s, L, s->normPart, s->normPart + size_t(s->normCap) * ST_NORM_WORDS, out);
+}
+
+int st_solve_norms_static(void* p, st_norms* out) {
+  Sim* s = getSim(p);
+  if (!s || !out) return fail(ST_ERR_ARG, "st_solve_norms_static: null argument");
+  if (!s->cur) return fail(ST_ERR_STATE, "st_solve_norms_static: no substep of this sim to measure (none yet, or the last one was batched)");
+  HIPCHK(hipSetDevice(s->dev));
+  LaunchCtx L = s->ctx();
+  double* part = nullptr;
+  size_t words = 0;
+  HIPCHK(norms_static_scratch(s->dev, &part, &words));
+  const size_t need = size_t(L.cellGrid + 1) * ST_NORM_WORDS;
+  if (need > words)
+    return fail(ST_ERR_ARG, "st_solve_norms_static: the launch needs " + std::to_string(need) + " scratch words, the fixed array holds " +
+                                std::to_string(words));
+  return normsInto(s, L, part, part + size_t(L.cellGrid) * ST_NORM_WORDS, out);
 }
 
 int st_sor_path(void* p, uint32_t* path) {

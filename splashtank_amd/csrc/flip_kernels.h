@@ -133,3 +133,8 @@ void launch_spray(hipStream_t st, const LaunchCtx& L, uint32_t airN, bool air, b
 // holds L.cellGrid x ST_NORM_WORDS doubles, out ST_NORM_WORDS (the st_norms fields)
 #define ST_NORM_WORDS 8
 void launch_norms(hipStream_t st, const LaunchCtx& L, double* part, double* out);
+// the fixed scratch of the allocation-free norms on device `dev` (which must be
+// current): room for the partials of NORMS_STATIC_WG workgroups and the results
+#define NORMS_STATIC_WG 4096u
+#define NORMS_STATIC_DEVICES 64
+hipError_t norms_static_scratch(int dev, double** ptr, size_t* words);

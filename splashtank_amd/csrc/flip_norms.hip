@@ -20,6 +20,8 @@
 // a fixed butterfly, a workgroup its four waves in order, and one workgroup
 // combines the per-workgroup partials in a fixed tree: the result does not
 // depend on scheduling.
+#include <mutex>
+
 #include "flip_device.h"
 #include "flip_kernels.h"
 
@@ -125,4 +127,29 @@ __global__ void __launch_bounds__(NWG_T) k_norms_final(const double* __restrict_
 void launch_norms(hipStream_t st, const LaunchCtx& L, double* part, double* out) {
   hipLaunchKernelGGL(k_norms, dim3(L.cellGrid), dim3(NWG_T), 0, st, L.P, L.G, part);
   hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(NWG_T), 0, st, part, L.cellGrid, out);
+}
+
+// The allocation-free norms (st_solve_norms_static): the per-workgroup partials
+// and the results live in this fixed array, one copy per GPU for the life of the
+// process.  In host code `scratch` names a host-side stand-in, so its device
+// address on each GPU is looked up once (with that GPU current) and kept.
+static __device__ __align__(256) double scratch[(NORMS_STATIC_WG + 1) * ST_NORM_WORDS];
+
+namespace {
+std::mutex g_scratchMu;
+double* g_scratchDev[NORMS_STATIC_DEVICES] = {};
+}  // namespace
+
+hipError_t norms_static_scratch(int dev, double** ptr, size_t* words) {
+  if (dev < 0 || dev >= NORMS_STATIC_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (!g_scratchDev[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchDev[dev] = static_cast<double*>(p);
+  }
+  *ptr = g_scratchDev[dev];
+  *words = size_t(NORMS_STATIC_WG + 1) * ST_NORM_WORDS;
+  return hipSuccess;
 }
