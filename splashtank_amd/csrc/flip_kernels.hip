@@ -1042,8 +1042,6 @@ __device__ __forceinline__ int3 coop_face_cell(uint32_t f, uint32_t e, uint32_t 
 
 #define COOP_NB_NONE 0xffffffffu
 #define COOP_NB_FOREIGN 0x80000000u
-#define COOP_JS ((COOP_TMAX * COOP_NSHELL + COOP_WG - 1) / COOP_WG)
-#define COOP_JC ((COOP_TMAX * COOP_NCORE + COOP_WG - 1) / COOP_WG)
 #define COOP_SPIN_LIMIT (1u << 21)
 
 // Grid barrier over the resident SOR's workgroups.  The halo cells cross it
@@ -1170,80 +1168,24 @@ __device__ __forceinline__ void coop_fill_foreign(const DGrid& G, uint32_t scap,
   }
 }
 
-// Cells of one colour of a tile, as local indices x + 8y + 64z: the 148 on
-// the tile's faces (some coordinate 0 or 7) first, then the 108 inside.  The
-// face cells are relaxed first so their values can leave for the neighbouring
-// workgroups while the inside cells are relaxed.
-#define COOP_NSHELL 148
-#define COOP_NCORE 108
-struct CoopCellTable {
-  uint16_t v[2][256];
-};
-constexpr CoopCellTable coop_cell_table() {
-  CoopCellTable t{};
-  for (uint32_t c = 0; c < 2; c++) {
-    uint32_t ns = 0, nc = COOP_NSHELL;
-    for (uint32_t l = 0; l < 512; l++) {
-      uint32_t x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
-      if (((x + y + z) & 1u) != c) continue;
-      bool shell = x == 0 || x == 7 || y == 0 || y == 7 || z == 0 || z == 7;
-      t.v[c][shell ? ns++ : nc++] = uint16_t(l);
-    }
-  }
-  return t;
-}
-__constant__ CoopCellTable kCoopCells = coop_cell_table();
+// The resident solve relaxes rows: item lt * 64 + r is row r = 8 z + y of
+// resident tile lt, whose colour-c cells x = x0, x0 + 2, x0 + 4, x0 + 6
+// (x0 = (y + z + c) & 1) lie side by side in the colour-split layout, one
+// float4.  Their neighbours across y and z are the same four positions of the
+// other colour's rows r -+ 1 and r -+ 8 (the tile's origin is even, so the
+// other colour's cells in those rows sit at the same x), or of the tile or
+// halo copy across that face; across x they are the other colour's row r
+// itself shifted by one place, with one cell from across an x face.
+#define COOP_J ((COOP_TMAX * 64 + COOP_WG - 1) / COOP_WG)  // row items per thread and colour
+static_assert(COOP_TMAX * 64 <= 4 * COOP_WG, "the item scan gives each thread four items");
 
-// one relaxation of cell l (colour c) of resident tile lt.  A neighbour inside
-// the tile is read from its other-colour array; across a face it is read from
-// the tile there when this workgroup holds it (the same index with the
-// coordinate wrapped into that tile), else from the halo copy sH of that face
-// (another workgroup's faces, or zeros where there is no tile).  One LDS load
-// per neighbour, from a chosen address.
-template <bool FACE>
-__device__ __forceinline__ void coop_relax(float* sP, const float* sH, const uint32_t* sNb, uint32_t lt, uint32_t l, uint32_t c,
-                                           float rhs, uint32_t st, float omega, const float* ytab) {
-  const int32_t x = int32_t(l & 7u), y = int32_t((l >> 3) & 7u), z = int32_t(l >> 6), xh = x >> 1;
-  float* cen = sP + lt * 512u + c * 256u + uint32_t((z * 8 + y) * 4 + xh);
+// one cell of relax_lds: the same sum, quotient and over-relaxation, bit for
+// bit; a cell outside the fluid keeps its value, a fluid cell with no open
+// neighbour becomes 0.  The quotient's exact fallback runs only for the cells
+// that need it (st_div_small's range test), not for cells that keep a value.
+__device__ __forceinline__ float coop_cell(float pc, float rhs, uint32_t st, float n0, float n1, float n2, float n3, float n4,
+                                           float n5, float omega, const float* ytab) {
   const uint32_t nns = st & 7u;
-  if (nns == 0u) {
-    *cen = 0.0f;
-    return;
-  }
-  const uint32_t oc = (c ^ 1u) * 256u;
-  const float* ob = sP + lt * 512u + oc;
-  float n0, n1, n2, n3, n4, n5;
-  if (FACE) {
-    const float* hb = sH + lt * 192u;
-    const uint32_t* nbs = sNb + lt * 6u;
-    // the address of the neighbour across face f: wrapped index w in the tile
-    // there, or index h in the face's halo copy
-    auto across = [&](uint32_t f, uint32_t w, uint32_t h) -> const float* {
-      const uint32_t nb = nbs[f];
-      return (nb & COOP_NB_FOREIGN) ? hb + f * 32u + h : sP + nb * 512u + oc + w;
-    };
-    const uint32_t row = uint32_t(z * 8 + y) * 4u;
-    const float* a0 = x > 0 ? ob + row + uint32_t((x - 1) >> 1) : across(0u, row + 3u, uint32_t(z * 4 + (y >> 1)));
-    const float* a1 = x < 7 ? ob + row + uint32_t((x + 1) >> 1) : across(1u, row, uint32_t(z * 4 + (y >> 1)));
-    const float* a2 = y > 0 ? ob + row - 4u + uint32_t(xh) : across(2u, uint32_t(z * 8 + 7) * 4u + uint32_t(xh), uint32_t(z * 4 + xh));
-    const float* a3 = y < 7 ? ob + row + 4u + uint32_t(xh) : across(3u, uint32_t(z * 8) * 4u + uint32_t(xh), uint32_t(z * 4 + xh));
-    const float* a4 = z > 0 ? ob + row - 32u + uint32_t(xh) : across(4u, uint32_t(56 + y) * 4u + uint32_t(xh), uint32_t(y * 4 + xh));
-    const float* a5 = z < 7 ? ob + row + 32u + uint32_t(xh) : across(5u, uint32_t(y) * 4u + uint32_t(xh), uint32_t(y * 4 + xh));
-    n0 = *a0;
-    n1 = *a1;
-    n2 = *a2;
-    n3 = *a3;
-    n4 = *a4;
-    n5 = *a5;
-  } else {
-    n0 = ob[(z * 8 + y) * 4 + ((x - 1) >> 1)];
-    n1 = ob[(z * 8 + y) * 4 + ((x + 1) >> 1)];
-    n2 = ob[(z * 8 + y - 1) * 4 + xh];
-    n3 = ob[(z * 8 + y + 1) * 4 + xh];
-    n4 = ob[((z - 1) * 8 + y) * 4 + xh];
-    n5 = ob[((z + 1) * 8 + y) * 4 + xh];
-  }
-  float yv = ytab[nns], pc = *cen;
   float sum = 0.0f;
   sum = sum + n0;
   sum = sum + n1;
@@ -1251,8 +1193,61 @@ __device__ __forceinline__ void coop_relax(float* sP, const float* sH, const uin
   sum = sum + n3;
   sum = sum + n4;
   sum = sum + n5;
-  float pg = st_div_small(rhs + sum, float(nns), yv);
-  *cen = pc + omega * (pg - pc);
+  const float a = rhs + sum, n = float(nns), y = ytab[nns];
+  const float q = a * y;
+  const float rr = fmaf(-q, n, a);
+  float pg = fmaf(rr, y, q);
+  const uint32_t ua = st_f2u(a) & 0x7fffffffu;
+  if (nns != 0u && !(ua - 0x01800000u < 0x7e000000u)) pg = a / n;
+  float out = pc + omega * (pg - pc);
+  if (nns == 0u) out = (st & STEN_FLUID) ? 0.0f : pc;
+  return out;
+}
+
+// one row item (colour c): rhs and stencil bytes of its four cells in rhs4 / st4
+// (byte k for cell k).  Reads only the other colour and writes only this row.
+__device__ __forceinline__ void coop_relax_row(float* sP, const float* sH, const uint32_t* sNb, uint32_t item, uint32_t c,
+                                               float4 rhs4, uint32_t st4, float omega, const float* ytab) {
+  const uint32_t lt = item >> 6, r = item & 63u, y = r & 7u, z = r >> 3, x0 = (y + z + c) & 1u;
+  const uint32_t oc = (c ^ 1u) * 256u;
+  float* cen = sP + lt * 512u + c * 256u + r * 4u;
+  const float* ob = sP + lt * 512u + oc;
+  const float* hb = sH + lt * 192u;
+  const uint32_t* nbs = sNb + lt * 6u;
+  // the address of the neighbour across face f: index w in the tile there when
+  // this workgroup holds it, else index h in the face's halo copy
+  auto across = [&](uint32_t f, uint32_t w, uint32_t h) -> const float* {
+    const uint32_t nb = nbs[f];
+    return (nb & COOP_NB_FOREIGN) ? hb + f * 32u + h : sP + nb * 512u + oc + w;
+  };
+  const float4 O = *reinterpret_cast<const float4*>(ob + r * 4u);
+  const float A = *across(x0, r * 4u + (x0 ? 0u : 3u), z * 4u + (y >> 1));
+  const float4 Ym = *reinterpret_cast<const float4*>(y > 0u ? ob + (r - 1u) * 4u : across(2u, (z * 8u + 7u) * 4u, z * 4u));
+  const float4 Yp = *reinterpret_cast<const float4*>(y < 7u ? ob + (r + 1u) * 4u : across(3u, z * 32u, z * 4u));
+  const float4 Zm = *reinterpret_cast<const float4*>(z > 0u ? ob + (r - 8u) * 4u : across(4u, (56u + y) * 4u, y * 4u));
+  const float4 Zp = *reinterpret_cast<const float4*>(z < 7u ? ob + (r + 8u) * 4u : across(5u, y * 4u, y * 4u));
+  const float4 pc = *reinterpret_cast<const float4*>(cen);
+  // x neighbours of cell k: x0 = 0 has (k ? O[k-1] : A, O[k]); x0 = 1 has (O[k], k < 3 ? O[k+1] : A)
+  const float m0 = x0 ? O.x : A, m1 = x0 ? O.y : O.x, m2 = x0 ? O.z : O.y, m3 = x0 ? O.w : O.z;
+  const float p0 = x0 ? O.y : O.x, p1 = x0 ? O.z : O.y, p2 = x0 ? O.w : O.z, p3 = x0 ? A : O.w;
+  float4 o;
+  o.x = coop_cell(pc.x, rhs4.x, st4 & 0xffu, m0, p0, Ym.x, Yp.x, Zm.x, Zp.x, omega, ytab);
+  o.y = coop_cell(pc.y, rhs4.y, (st4 >> 8) & 0xffu, m1, p1, Ym.y, Yp.y, Zm.y, Zp.y, omega, ytab);
+  o.z = coop_cell(pc.z, rhs4.z, (st4 >> 16) & 0xffu, m2, p2, Ym.z, Yp.z, Zm.z, Zp.z, omega, ytab);
+  o.w = coop_cell(pc.w, rhs4.w, st4 >> 24, m3, p3, Ym.w, Yp.w, Zm.w, Zp.w, omega, ytab);
+  *reinterpret_cast<float4*>(cen) = o;
+}
+
+// whether row item (lt, r) of colour c has a cell on a face whose neighbour
+// tile another workgroup holds (those cells are sent after each half-sweep)
+__device__ __forceinline__ bool coop_row_early(const uint32_t* sNb, uint32_t lt, uint32_t r, uint32_t c) {
+  const uint32_t y = r & 7u, z = r >> 3, x0 = (y + z + c) & 1u;
+  auto foreign = [&](uint32_t f) {
+    const uint32_t nb = sNb[lt * 6u + f];
+    return nb != COOP_NB_NONE && (nb & COOP_NB_FOREIGN) != 0u;
+  };
+  return foreign(x0) || (y == 0u && foreign(2u)) || (y == 7u && foreign(3u)) || (z == 0u && foreign(4u)) ||
+         (z == 7u && foreign(5u));
 }
 
 // LDS of the resident mode
@@ -1262,8 +1257,8 @@ struct CoopLds {
   uint32_t sNb[COOP_TMAX * 6];
   uint32_t sSlot[COOP_TMAX];
   uint32_t sFor[COOP_TMAX * 6];  // (tile, face) pairs whose neighbour another workgroup owns
-  uint16_t sCell[2][256];        // kCoopCells
 };
+static_assert(sizeof(float) * COOP_TMAX * 192 >= 2 * COOP_TMAX * 64 * sizeof(uint16_t), "the row lists fit in the halo copy");
 constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
 
 __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
@@ -1340,32 +1335,73 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     uint32_t lt = q >> 9, l = q & 511u;
     sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))] = pres[sSlot[lt] * 512u + l];
   }
-  uint16_t* sCell = &CL.sCell[0][0];
-  for (uint32_t q = tid; q < 512u; q += COOP_WG) sCell[q] = kCoopCells.v[q >> 8][q & 255u];
-  __syncthreads();
-  // rhs and stencil of the cells this thread relaxes, colour c: face cells
-  // m = tid + j COOP_WG = 148 lt + i (j < COOP_JS), then inside cells
-  // m = 108 lt + i (j < COOP_JC)
-  float rr[2][COOP_JS + COOP_JC];
-  uint32_t rs[2][(COOP_JS + COOP_JC + 3) / 4];  // stencil bytes, four to a register
+  // Row items of each colour, those with a cell on a face toward another
+  // workgroup's tile ("early") first: they are relaxed and sent before the
+  // rest.  The lists are built in the halo copy's space, filled only after.
+  // Thread t classifies items 4t .. 4t + 3; an exclusive scan of the packed
+  // (early, late) counts places them.
+  uint16_t* sList = reinterpret_cast<uint16_t*>(sH);  // [2][COOP_TMAX * 64]
+  __shared__ uint32_t sWave[COOP_WG / 64];
+  __shared__ uint32_t sEarly[2];
+  const uint32_t nItems = T * 64u;
+  for (uint32_t c = 0; c < 2u; c++) {
+    uint32_t flags = 0u, cnt = 0u;
+    for (uint32_t i = 0; i < 4u; i++) {
+      const uint32_t q = tid * 4u + i;
+      if (q >= nItems) continue;
+      if (coop_row_early(sNb, q >> 6, q & 63u, c)) {
+        flags |= 1u << i;
+        cnt += 1u;
+      } else {
+        cnt += 1u << 16;
+      }
+    }
+    uint32_t inc = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t o = (uint32_t)__shfl_up((int)inc, off);
+      if (int(lane_id()) >= off) inc += o;
+    }
+    if (lane_id() == 63u) sWave[tid >> 6] = inc;
+    __syncthreads();
+    uint32_t before = 0u, tot = 0u;
+    for (uint32_t u = 0; u < COOP_WG / 64u; u++) {
+      if (u < (tid >> 6)) before += sWave[u];
+      tot += sWave[u];
+    }
+    const uint32_t ex = before + inc - cnt;
+    uint32_t e = ex & 0xffffu, l = (tot & 0xffffu) + (ex >> 16);
+    for (uint32_t i = 0; i < 4u; i++) {
+      const uint32_t q = tid * 4u + i;
+      if (q >= nItems) continue;
+      sList[c * (COOP_TMAX * 64u) + ((flags >> i) & 1u ? e++ : l++)] = uint16_t(q);
+    }
+    if (tid == 0u) sEarly[c] = tot & 0xffffu;
+    __syncthreads();
+  }
+  // this thread's items (positions tid + j COOP_WG of each list), with the rhs
+  // and stencil bytes of their cells in registers for the whole solve
+  uint32_t items[2][COOP_J];
+  float4 rr[2][COOP_J];
+  uint32_t rs[2][COOP_J];
 #pragma unroll
   for (uint32_t c = 0; c < 2u; c++) {
 #pragma unroll
-    for (uint32_t j = 0; j < (COOP_JS + COOP_JC + 3) / 4; j++) rs[c][j] = 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < COOP_JS + COOP_JC; j++) {
-      const bool face = j < COOP_JS;
-      const uint32_t per = face ? COOP_NSHELL : COOP_NCORE;
-      uint32_t m = tid + (face ? j : j - COOP_JS) * COOP_WG;
-      rr[c][j] = 0.0f;
-      if (m < T * per) {
-        uint32_t lt = m / per, i = m - lt * per + (face ? 0u : COOP_NSHELL);
-        uint32_t cell = sSlot[lt] * 512u + sCell[c * 256u + i];
-        rr[c][j] = G.rhs[cell];
-        rs[c][j >> 2] |= uint32_t(G.sten[cell]) << (8u * (j & 3u));
+    for (uint32_t j = 0; j < COOP_J; j++) {
+      const uint32_t pos = tid + j * COOP_WG;
+      items[c][j] = 0u;
+      rr[c][j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      rs[c][j] = 0u;
+      if (pos < nItems) {
+        const uint32_t it = sList[c * (COOP_TMAX * 64u) + pos], r = it & 63u, y = r & 7u, z = r >> 3;
+        const uint32_t base = sSlot[it >> 6] * 512u + 64u * z + 8u * y + ((y + z + c) & 1u);
+        items[c][j] = it;
+        rr[c][j] = make_float4(G.rhs[base], G.rhs[base + 2u], G.rhs[base + 4u], G.rhs[base + 6u]);
+        rs[c][j] = uint32_t(G.sten[base]) | (uint32_t(G.sten[base + 2u]) << 8) | (uint32_t(G.sten[base + 4u]) << 16) |
+                   (uint32_t(G.sten[base + 6u]) << 24);
       }
     }
   }
+  const uint32_t nEarly[2] = {sEarly[0], sEarly[1]};
   __syncthreads();
   coop_zero_none(sH, sNb, T);
   coop_fill_start(pres, sH, sNb, sFor, nFor, 1u);
@@ -1375,20 +1411,17 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
 #pragma unroll
     for (uint32_t c = 0; c < 2u; c++) {
       const bool last = it + 1u == iters && c == 1u;  // nothing more to exchange after it
-      // the cell addresses are recomputed every sweep rather than kept in
-      // registers for the whole solve (an opaque copy of the thread index)
+      // what derives from the thread index, the items and the stencil bytes
+      // (addresses, masks, divisors) is recomputed every half-sweep rather than
+      // kept in registers for the whole solve: opaque copies
       uint32_t tv = tid;
       asm volatile("" : "+v"(tv));
-      // face cells first
 #pragma unroll
-      for (uint32_t j = 0; j < COOP_JS; j++) {
-        uint32_t m = tv + j * COOP_WG;
-        uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
-        if (m < T * COOP_NSHELL && (st & STEN_FLUID)) {
-          uint32_t lt = m / COOP_NSHELL, i = m - lt * COOP_NSHELL;
-          coop_relax<true>(sP, sH, sNb, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
-        }
-      }
+      for (uint32_t j = 0; j < COOP_J; j++) asm volatile("" : "+v"(items[c][j]), "+v"(rs[c][j]));
+      // early rows first
+#pragma unroll
+      for (uint32_t j = 0; j < COOP_J; j++)
+        if (tv + j * COOP_WG < nEarly[c]) coop_relax_row(sP, sH, sNb, items[c][j], c, rr[c][j], rs[c][j], omega, ytab);
       __syncthreads();
       if (!last) {
         // colour-c face cells other workgroups read, tagged with this half-sweep
@@ -1399,15 +1432,11 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
           coop_halo_put(G, coop_halo_idx(c, scap, sSlot[lt], f, e), sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], tag0 + epoch);
         }
       }
-      // inside cells while the faces travel
+      // the other rows while the faces travel
 #pragma unroll
-      for (uint32_t j = COOP_JS; j < COOP_JS + COOP_JC; j++) {
-        uint32_t m = tv + (j - COOP_JS) * COOP_WG;
-        uint32_t st = (rs[c][j >> 2] >> (8u * (j & 3u))) & 0xffu;
-        if (m < T * COOP_NCORE && (st & STEN_FLUID)) {
-          uint32_t lt = m / COOP_NCORE, i = m - lt * COOP_NCORE + COOP_NSHELL;
-          coop_relax<false>(sP, sH, sNb, lt, sCell[c * 256u + i], c, rr[c][j], st, omega, ytab);
-        }
+      for (uint32_t j = 0; j < COOP_J; j++) {
+        const uint32_t pos = tv + j * COOP_WG;
+        if (pos >= nEarly[c] && pos < nItems) coop_relax_row(sP, sH, sNb, items[c][j], c, rr[c][j], rs[c][j], omega, ytab);
       }
       if (last) break;
       coop_fill_foreign(G, scap, sH, sNb, sFor, nFor, c, tag0 + epoch, &sFail);

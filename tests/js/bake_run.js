@@ -41,7 +41,7 @@ M.bake(desc, M.initHip(0), (fr, info) => {
     }
   }
   const dg = info.diag;
-  frames.push({ f: info.f, n: dg.alive, sub: info.sub, finite: dg.nonFinite === 0, inside: dg.outsideBox === 0,
+  frames.push({ f: info.f, n: dg.alive, sub: dg.sub, finite: dg.nonFinite === 0, inside: dg.outsideBox === 0,
     objectSpeed: dg.objectSpeed || 0, maxSpeed: dg.maxSpeed });
 }).then((ok) => {
   const file = M.writeCache({ d, fps: desc.fps, ids: desc.objs.map((o) => o.id), b: box, maxN: desc.maxN, engine: 'mi355x-flip' }, cache);

@@ -112,7 +112,9 @@ def test_facade_extensions(tmp_path):
     """stepN (with and without graph replay), the autotuner, snapshots and the
     adjoint through Node: stepN gives the bits of step() one at a time, the
     tuner and a snapshot round trip leave the state as it was, and the adjoint's
-    gradient equals the C ABI's (capi.AdjointC) bit for bit."""
+    gradient equals the C ABI's (capi.AdjointC) to float rounding: the backward
+    sums face gradients with float atomics, whose order varies from run to run
+    (two runs differ by at most a few ulps, 2.4e-7 relative seen)."""
     sc = scenes.dam_break(d=1.0 / 32, iters=15)  # 16^3 cells, closed box
     steps = 4
     parts = np.concatenate([sc.pos, sc.vel], axis=1).astype(np.float32)
@@ -144,6 +146,6 @@ def test_facade_extensions(tmp_path):
         a.close()
     finally:
         g.close()
-    np.testing.assert_array_equal(js[0][:, :3], gpos)
-    np.testing.assert_array_equal(js[1][:, :3], gvel)
+    for got, want in ((js[0][:, :3], gpos[:, :3]), (js[1][:, :3], gvel[:, :3])):
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6 * float(np.abs(want).max()))
     assert np.abs(gvel[:, :3]).max() > 0
