@@ -152,6 +152,17 @@ ST_HD int32_t st_f2i_sat(float x) {
   return st_f2i_sat_portable(x);
 #endif
 }
+/* st_f2i_sat for a value known to be below 2147483520 (or NaN): the conversion
+ * alone, without the clamp that cannot act */
+ST_HD int32_t st_f2i_nosat(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  int32_t r;
+  __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+#else
+  return st_f2i_sat_portable(x);
+#endif
+}
 ST_HD uint32_t st_f2u_sat(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint32_t r;

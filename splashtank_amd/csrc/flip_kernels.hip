@@ -463,6 +463,11 @@ __device__ __forceinline__ uint32_t wcell(const uint32_t* wslots, int32_t wx, in
   int32_t gx = org.x + wx, gy = org.y + wy, gz = org.z + wz;
   return s * 512u + uint32_t(gx & 7) + 8u * (uint32_t(gy & 7) + 8u * uint32_t(gz & 7));
 }
+// window cell index wx + WIN (wy + WIN wz) of offsets in [0, WIN): 24-bit
+// multiplies (full rate; a 32-bit multiply is a quarter-rate instruction)
+__device__ __forceinline__ uint32_t win_idx(int32_t wx, int32_t wy, int32_t wz) {
+  return uint32_t(wx) + __umul24(uint32_t(wy) + __umul24(uint32_t(wz), uint32_t(WIN)), uint32_t(WIN));
+}
 // window offset, clamped into the window: a guard for LDS only -- every face or
 // cell a particle of the tile touches already lies inside (faceIndex's clamp moves
 // an index toward the particle's own clamped cell, which is in the tile)
@@ -504,6 +509,7 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
+    const bool wscaleSmall = P.scale <= 2147483520.0f;
     // the four waves take every fourth entry of each run of WG bin entries:
     // neighbouring entries tend to be neighbouring particles (last substep's
     // order), and lanes adding to the same LDS word in one instruction queue
@@ -537,8 +543,11 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
           int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
           float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
           int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
-          int32_t aw = st_f2i_sat(rintf(w * P.scale));
-          int32_t wi = wx[dx] + WIN * (wy[dy] + WIN * wz[dz]);
+          // w is in [0, 1] (NaN for a NaN position), so w * scale <= scale and
+          // the saturating clamp of st_f2i_sat never acts when the scale is
+          // below it (wscaleSmall); the conversion alone gives the same bits
+          int32_t aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
+          const uint32_t wi = win_idx(wx[dx], wy[dy], wz[dz]);
           atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
         }
       }
@@ -1552,7 +1561,25 @@ struct GlobalGrid {
   __device__ float fnew(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gvR(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
   __device__ float fold(uint32_t axis, int32_t i, int32_t j, int32_t k) const { return gv0R(G, d_faceIndex(P, G.tmap, axis, i, j, k)); }
   __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const { return G.cnt[d_cellIdx(P, G.tmap, i, j, k)]; }
+  // the particle counts of the 2x2x2 cells from bc, each clamped to the box, summed
+  __device__ float dsum(int3 bc) const {
+    float csum = 0.0f;
+    for (int32_t dz = 0; dz <= 1; dz++)
+      for (int32_t dy = 0; dy <= 1; dy++)
+        for (int32_t dx = 0; dx <= 1; dx++) {
+          int32_t cx = d_iclamp(bc.x + dx, 0, int32_t(P.nx) - 1), cy = d_iclamp(bc.y + dy, 0, int32_t(P.ny) - 1),
+                  cz = d_iclamp(bc.z + dz, 0, int32_t(P.nz) - 1);
+          csum = csum + float(cnt(cx, cy, cz));
+        }
+    return csum;
+  }
 };
+// The 2x2x2 count sums of a tile window, for every base cell a particle of the
+// tile can have: base b = org + w, w in [0, 8] per axis, cells clamped to the box
+// as GlobalGrid::dsum clamps them.  Counts are integers far below 2^24, so their
+// f32 sum in any order is exact and equals the integer sum.
+#define DSUM_N 9
+#define DSUM3 (DSUM_N * DSUM_N * DSUM_N)
 struct WindowGrid {
   const DParams& P;
   const DGrid& G;
@@ -1563,6 +1590,17 @@ struct WindowGrid {
   // cells of the density sum come clamped to the box, so they lie in the window
   __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const {
     return sC[wclamp(i - org.x) + WIN * (wclamp(j - org.y) + WIN * wclamp(k - org.z))];
+  }
+  // GlobalGrid::dsum from the table sS.  A base below -1 (above n - 1) clamps its
+  // two cells to the ones of base -1 (n - 1), so the base is clamped to
+  // [-1, n - 1] first; for a particle of the tile that puts it in the window's
+  // [0, 8] (the outer clamp only keeps LDS reads in bounds)
+  const int32_t* sS;
+  __device__ float dsum(int3 bc) const {
+    const int32_t wx = min(max(d_iclamp(bc.x, -1, int32_t(P.nx) - 1) - org.x, 0), DSUM_N - 1);
+    const int32_t wy = min(max(d_iclamp(bc.y, -1, int32_t(P.ny) - 1) - org.y, 0), DSUM_N - 1);
+    const int32_t wz = min(max(d_iclamp(bc.z, -1, int32_t(P.nz) - 1) - org.z, 0), DSUM_N - 1);
+    return float(sS[wx + DSUM_N * (wy + DSUM_N * wz)]);
   }
 };
 
@@ -1587,7 +1625,7 @@ __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGr
     for (uint32_t c = 0; c < 8; c++) {
       int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
       float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-      int32_t wi = wx[dx] + WIN * (wy[dy] + WIN * wz[dz]);
+      const uint32_t wi = win_idx(wx[dx], wy[dy], wz[dz]);
       float2 e = g.sNO[axis][wi];
       sn = sn + w * e.x;
       so = so + w * e.y;
@@ -1679,13 +1717,7 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   if (P.q9 < 0.5f && fast) {
     csum = 8.0f * float(g.cnt(fcell.x, fcell.y, fcell.z));
   } else if (P.q9 < 0.5f) {
-    csum = 0.0f;
-    for (int32_t dz = 0; dz <= 1; dz++)
-      for (int32_t dy = 0; dy <= 1; dy++)
-        for (int32_t dx = 0; dx <= 1; dx++) {
-          int32_t cx = d_iclamp(bc.x + dx, 0, int32_t(P.nx) - 1), cy = d_iclamp(bc.y + dy, 0, int32_t(P.ny) - 1), cz = d_iclamp(bc.z + dz, 0, int32_t(P.nz) - 1);
-          csum = csum + float(g.cnt(cx, cy, cz));
-        }
+    csum = g.dsum(bc);
   }
   // air, shield, foam decay and drag factor depend on the density sum alone
   const float4 ft = (P.q9 < 0.5f && csum >= 0.0f && csum < float(FOAM_TAB)) ? G.foamTab[int32_t(csum)] : d_foamRow(P, csum);
@@ -1760,6 +1792,7 @@ __device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, co
   const DParams& P = *PP;
   __shared__ float2 sNO[3][WIN3];
   __shared__ int32_t sC[WIN3];
+  __shared__ int32_t sS[DSUM3];
   __shared__ uint32_t wslots[27];
   for (uint32_t t = bid; t <= P.ntiles; t += nbid) {
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
@@ -1808,7 +1841,22 @@ __device__ __forceinline__ void g2p_tile_body(const DParams* __restrict__ PP, co
       sC[q] = G.cnt[c];
     }
     __syncthreads();
-    WindowGrid wg{P, G, sNO, sC, org};
+    if (P.q9 < 0.5f && !(P.q12 > 0.5f)) {  // the density sums g2p reads
+      for (uint32_t q = threadIdx.x; q < DSUM3; q += WG) {
+        const int32_t bx = org.x + int32_t(q % DSUM_N), by = org.y + int32_t((q / DSUM_N) % DSUM_N), bz = org.z + int32_t(q / (DSUM_N * DSUM_N));
+        int32_t n = 0;
+        for (int32_t dz = 0; dz <= 1; dz++)
+          for (int32_t dy = 0; dy <= 1; dy++)
+            for (int32_t dx = 0; dx <= 1; dx++) {
+              const int32_t cx = d_iclamp(bx + dx, 0, int32_t(P.nx) - 1), cy = d_iclamp(by + dy, 0, int32_t(P.ny) - 1),
+                            cz = d_iclamp(bz + dz, 0, int32_t(P.nz) - 1);
+              n += sC[wclamp(cx - org.x) + WIN * (wclamp(cy - org.y) + WIN * wclamp(cz - org.z))];
+            }
+        sS[q] = n;
+      }
+      __syncthreads();
+    }
+    WindowGrid wg{P, G, sNO, sC, org, sS};
     for (uint32_t k0 = beg; k0 < end; k0 += WG) {
       uint32_t k = k0 + threadIdx.x;
       bool valid = k < end;
