@@ -248,18 +248,92 @@ def run_adjoint(args):
             "note": "backward = each segment re-run forward with its tapes recorded, then reversed (st_adj_backward)"}
 
 
+def counter_summary(workload: str):
+    """The committed counter summary of this workload (profiles/<workload>_traffic.json,
+    written by --summarize-counters from rocprofv3 --pmc passes over this bench), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM-side bytes per dispatch of `kernel` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
     device copy of known size), or None when there is none for this workload."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    d = counter_summary(workload)
     try:
-        with open(path) as f:
-            d = json.load(f)
         k = d["kernels"][kernel]
         return k["read_bytes"] + k["write_bytes"]
-    except (OSError, KeyError, ValueError):
+    except (TypeError, KeyError):
         return None
+
+
+# MI355X: 256 CUs; a CU issues at most one wave64 VALU instruction per clock (four
+# SIMDs, four clocks each); 2.4 GHz peak engine clock
+CUS, CLOCK_HZ = 256, 2.4e9
+
+
+def summarize_counters(src: str, workload: str, command: str, calib_mb: int):
+    """Per-kernel summary of rocprofv3 --pmc passes (one sub-directory per pass,
+    each with run_counter_collection.csv) into profiles/<workload>_traffic.json:
+    HBM-side read and write bytes per dispatch (FETCH_SIZE and WRITE_SIZE in KiB,
+    scaled by what they read for the calibration copy of --calib-copy-mb), and the
+    instruction, wait and LDS counters with the VALU issue share they imply."""
+    import csv
+    import glob
+    per = {}
+    durs = {}
+    calib_rows = {}
+    for f in sorted(glob.glob(os.path.join(src, "*", "run_counter_collection.csv"))):
+        rows = list(csv.DictReader(open(f)))
+        # the calibration copy: the copy dispatches of the largest grid (the small
+        # ones are the parameter uploads)
+        cg = max([int(r["Grid_Size"]) for r in rows if "copyBuffer" in r["Kernel_Name"]] or [0])
+        for r in rows:
+            k = r["Kernel_Name"].split("(")[0]
+            if "copyBuffer" in k:
+                if int(r["Grid_Size"]) == cg:
+                    calib_rows.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                continue
+            per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            durs.setdefault(k, {})[(f, r["Dispatch_Id"])] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+    copy = ["calibration copy"] if calib_rows else []
+    per.update({"calibration copy": calib_rows} if calib_rows else {})
+    mean = lambda v: sum(v) / len(v)
+    fs = ws = 1.0
+    calib = None
+    if copy:
+        c = per[copy[0]]
+        nbytes = calib_mb << 20
+        fs = nbytes / (mean(c["FETCH_SIZE"]) * 1024.0) if "FETCH_SIZE" in c else 1.0
+        ws = nbytes / (mean(c["WRITE_SIZE"]) * 1024.0) if "WRITE_SIZE" in c else 1.0
+        calib = {"copy_bytes": nbytes, "fetch_kib": mean(c.get("FETCH_SIZE", [0])), "write_kib": mean(c.get("WRITE_SIZE", [0]))}
+    out = {"workload": workload, "fetch_scale": fs, "write_scale": ws, "calibration": calib, "source": command, "kernels": {}}
+    for k, c in per.items():
+        if k in copy:
+            continue
+        e = {"dispatches": max(len(v) for v in c.values())}
+        if "FETCH_SIZE" in c:
+            e["read_bytes"] = mean(c["FETCH_SIZE"]) * 1024.0 * fs
+        if "WRITE_SIZE" in c:
+            e["write_bytes"] = mean(c["WRITE_SIZE"]) * 1024.0 * ws
+        for name, v in c.items():
+            if name.startswith("SQ_") or name.startswith("GRBM_"):
+                e[name] = mean(v)
+        d = mean(list(durs[k].values()))
+        e["dispatch_s"] = d
+        if "SQ_INSTS_VALU" in c:
+            e["valu_issue_share"] = e["SQ_INSTS_VALU"] / (CUS * CLOCK_HZ * d)
+        if "SQ_INSTS_LDS" in c:
+            e["lds_instr_per_cu_clock"] = e["SQ_INSTS_LDS"] / (CUS * CLOCK_HZ * d)
+        out["kernels"][k] = e
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(path)
 
 
 def calib_copy(mb: int):
@@ -301,6 +375,24 @@ def cpu_baseline(sc, args):
     wall = time.perf_counter() - t0
     o.close()
     return sc.n * args.cpu_steps / wall, threads(), wall
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), with the logical CPUs this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count()
+    return f"{name} ({n} logical CPUs available)"
 
 
 REF_BAKER = os.path.join(ROOT, "oracle", "_ref", "splash")
@@ -353,7 +445,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="splash256", choices=["splash256", "dam64"])
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-warmup", type=int, default=1)
     ap.add_argument("--ref-frames", type=int, default=36, help="frames of the reference's CPU baker to time")
     ap.add_argument("--no-cpu", action="store_true")
@@ -365,7 +457,14 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint line")
     ap.add_argument("--adj-steps", type=int, default=60)
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="summarize rocprofv3 --pmc passes under DIR (one sub-directory per pass) into "
+                         "profiles/<workload>_traffic.json and exit; --calib-copy-mb names the calibration copy")
+    ap.add_argument("--counter-command", default="", help="the rocprofv3 commands, recorded in the summary")
     args = ap.parse_args()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.workload, args.counter_command, args.calib_copy_mb or 256)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the substep runs on a single MI355X; "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
@@ -384,23 +483,50 @@ def main():
     # algorithmic bytes of one substep: 112 N + 224 A + (20 + 16 I) Fc + 16 T
     sub_bytes = 112 * alive + 224 * A + (20 + 16 * iters) * fc + 16 * T
     roof = None
+    kernels = None
     if stage_ms:
-        # dominant kernel: the pressure solve, 16 B per fluid cell per red-black
-        # iteration (rhs, type, pressure read and write).  The cooperative solve
-        # runs all iterations in one launch (k_sor_resident; the stage also holds
-        # three small list kernels of a few us); the per-iteration path launches
-        # k_sor_pipe once per iteration.
+        # the three large kernels, each with its algorithmic bytes per launch and
+        # its time per launch (HIP events around its stage on the sim's stream):
+        #  - pressure: 16 B per fluid cell per red-black iteration (rhs, type,
+        #    pressure read and write); the cooperative solve runs all iterations
+        #    in one launch (k_sor_resident; the stage also holds three small list
+        #    kernels of a few us), the per-iteration path one per iteration;
+        #  - g2p: 64 B per particle (state read and written) and 28 B per active
+        #    cell (new and old face velocities, count);
+        #  - p2g: 32 B per particle (state read) and 56 B per active cell (three
+        #    face sums of weight and velocity written, count).
+        # The roofline is the HBM one; `limiter` says what the counters in the
+        # committed summary show bounds the kernel instead, when it is not HBM.
         coop = stats.get("sor_path") in ("resident", "streaming")
         per_launch_iters = iters if coop else 1
-        launch_us = stage_ms["pressure"] * 1e3 / (iters // per_launch_iters)
-        sor_bytes = 16 * fc * per_launch_iters
-        ach = sor_bytes / (launch_us * 1e-6) / 1e9
-        kname = "k_sor_resident" if coop else "k_sor_pipe"
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.workload, kname),
-                "kernel": (f"{kname} (all {iters} red-black SOR iterations in one launch, {stats['sor_path']})" if coop
-                           else "k_sor_pipe (one red-black SOR iteration per launch)"),
-                "algorithmic_bytes_per_launch": sor_bytes, "launch_us": launch_us}
+        sor_name = "k_sor_resident" if coop else "k_sor_pipe"
+        cands = {
+            sor_name: (16 * fc * per_launch_iters, stage_ms["pressure"] * 1e3 / (iters // per_launch_iters),
+                       f"{sor_name} (all {iters} red-black SOR iterations in one launch, {stats['sor_path']})" if coop
+                       else "k_sor_pipe (one red-black SOR iteration per launch)"),
+            "k_g2p_tile": (64 * alive + 28 * A, stage_ms["g2p"] * 1e3, "k_g2p_tile (g2p and the next substep's bin marks)"),
+            "k_p2g_tile": (32 * alive + 56 * A, stage_ms["p2g"] * 1e3, "k_p2g_tile (p2g, one workgroup per tile)"),
+        }
+        summ = counter_summary(args.workload)
+        kernels = {}
+        for kname, (byts, us, desc) in cands.items():
+            ach = byts / (us * 1e-6) / 1e9
+            e = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                 "traffic": measured_traffic(args.workload, kname), "kernel": desc,
+                 "algorithmic_bytes_per_launch": byts, "launch_us": us}
+            k = (summ or {}).get("kernels", {}).get(kname)
+            if k:
+                e["traffic_source"] = f"profiles/{args.workload}_traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, calibrated by a device copy; committed, not measured in this run)"
+                if "valu_issue_share" in k:
+                    share = k["valu_issue_share"]
+                    e["valu_issue_share"] = share
+                    if share >= 0.5:
+                        e["limiter"] = (f"VALU issue: the counters put the kernel's VALU instructions at {share:.0%} of the "
+                                        f"CUs' issue rate (one wave64 instruction per CU per clock at 2.4 GHz), "
+                                        f"HBM traffic at {(e['traffic'] or 0) / (us * 1e-6) / 1e9 / HBM_PEAK_GBS:.0%} of peak")
+            kernels[kname] = e
+        dom = max(kernels, key=lambda q: kernels[q]["launch_us"])
+        roof = dict(kernels[dom])
     p2g_atomics = None
     if stage_ms:
         # k_p2g_tile: per particle 24 packed 64-bit LDS adds (weight and weighted
@@ -412,7 +538,12 @@ def main():
         p2g_atomics = {"reference_global_per_particle": 50, "lds_per_particle": 25,
                        "global_per_launch_at_most": blocks * 6000,
                        "lds_atomics_per_s": 25 * alive / p2g_s,
-                       "reference_atomics_per_s_equivalent": 50 * alive / p2g_s, "p2g_ms": stage_ms["p2g"]}
+                       "reference_atomics_per_s_equivalent": 50 * alive / p2g_s, "p2g_ms": stage_ms["p2g"],
+                       # no published figure: the LDS moves 128 B per clock per CU, so at most 16 lanes'
+                       # 64-bit adds per clock per CU, x 256 CUs x 2.4 GHz (bank conflicts only lower it)
+                       "lds_atomic_peak_per_s": 16 * CUS * CLOCK_HZ,
+                       "lds_atomic_peak_basis": "128 B/clock/CU of LDS = 16 64-bit adds/clock/CU x 256 CUs x 2.4 GHz (model)",
+                       "lds_atomic_frac": 25 * alive / p2g_s / (16 * CUS * CLOCK_HZ)}
     out = {
         "metric": "particle-substeps/s", "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -422,6 +553,7 @@ def main():
         "stage_ms": stage_ms,
         "solve_f64": solve_f64,
         "roofline": roof,
+        "kernel_rooflines": kernels,
         "p2g_atomics": p2g_atomics,
         "substep_roofline": {"algorithmic_bytes": sub_bytes, "achieved": sub_bytes / (ms_per_step * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -435,6 +567,7 @@ def main():
         out["reference_cpu"] = reference_cpu(args.ref_frames)
         v, cores, cw = cpu_baseline(sc, args)
         out["cpu_baseline"] = {"value": v, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+                               "cpu": cpu_model(),
                                "sample": f"{args.cpu_steps} substeps of the same scene after {args.cpu_warmup} warm-up, "
                                          f"{cw:.1f} s"}
     print(json.dumps(out))
