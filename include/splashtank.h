@@ -314,6 +314,33 @@ int st_pack_view(void* sim, uint32_t disp_max, uint32_t* out, size_t cap_words, 
 #define ST_DEEP_MAX 200000u
 int st_read_deep(void* sim, float* out4, uint32_t room, uint32_t* n);
 
+/* Asynchronous readback, for a caller that must not block: replaces the
+ * reference's mapAsync readbacks -- readbackInto (src/flip/sim.js:439-455),
+ * readStats (sim.js:160-168), readSpray (sim.js:423-437) and readDeep
+ * (sim.js:405-419) -- behind the N-API addon's Promises.
+ * st_read_begin enqueues on the sim's stream, behind all work enqueued before
+ * it, the copy of one result into pinned host memory that the ticket owns, and
+ * returns without waiting.  st_read_wait may be called from any thread: it
+ * waits for that copy alone (an event) and copies the result out, touching
+ * nothing of the sim, so the sim may step on meanwhile (the result is the state
+ * at st_read_begin).  st_read_free releases the ticket (any thread; it waits for
+ * the copy first).  A ticket outlives its sim.
+ *   ST_READ_PARTICLES: min(n, count) particles in id order (x, y, z, w) into
+ *     out0 (room >= that many); *n = how many.
+ *   ST_READ_CTL: the 16 control words (st_read_ctl) into out0 (room >= 16).
+ *   ST_READ_SPRAY: as st_read_spray, positions into out0 and velocities into
+ *     out1 (room points each); *n = 0 without a pool.
+ *   ST_READ_DEEP: as st_read_deep into out0 (room points); *n = 0 when it
+ *     gives none.  The count that picks the next read's sample is taken when
+ *     st_read_wait runs. */
+#define ST_READ_PARTICLES 0u
+#define ST_READ_CTL 1u
+#define ST_READ_SPRAY 2u
+#define ST_READ_DEEP 3u
+int st_read_begin(void* sim, uint32_t what, uint32_t n, void** ticket);
+int st_read_wait(void* ticket, void* out0, void* out1, uint32_t room, uint32_t* n);
+int st_read_free(void* ticket);
+
 /* buildSDF(mesh, cell, margin) (src/shared.js:98-181): the signed distance grid
  * of a closed triangle mesh, the collider grids st_set_colliders takes.  v holds
  * nv vertices (x, y, z doubles), tri nt triangles (vertex indices; the

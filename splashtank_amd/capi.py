@@ -7,13 +7,32 @@ Loading fails loudly when the HIP library is not built: there is no fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsplashtank_hip.so")
+
+# Handles still open when the interpreter exits are closed first thing at exit,
+# while the HIP runtime is still up, dependents before what they depend on:
+# adjoints and snapshots, then batches, then sims.
+_LIVE = {kind: weakref.WeakSet() for kind in ("adjoint", "snapshot", "batch", "sim")}
+
+
+def close_all():
+    for kind in ("adjoint", "snapshot", "batch", "sim"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(close_all)
 
 ST_OK, ST_ERR_ARG, ST_ERR_HIP, ST_ERR_STATE, ST_ERR_NOMEM = 0, -1, -2, -3, -4
 ST_OPEN_X0, ST_OPEN_X1, ST_OPEN_Y0, ST_OPEN_Y1, ST_OPEN_Z0, ST_OPEN_Z1 = 1, 2, 4, 8, 16, 32
@@ -222,6 +241,9 @@ PROTOS = {
                                C.c_void_p, C.c_size_t]),
     "st_read_spray": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "st_read_deep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "st_read_begin": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "st_read_wait": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "st_read_free": (C.c_int, [C.c_void_p]),
     "st_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_tune_sor": (C.c_int, [C.c_void_p, C.POINTER(StParams), C.c_uint32, C.c_int32, C.POINTER(StSorTune)]),
     "st_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -344,6 +366,7 @@ class FlipSimC(_Base):
         h = C.c_void_p()
         self._call_create(opts, h)
         self.h = h
+        _LIVE["sim"].add(self)
         self.dims = StDims()
         self._call("get_dims", self.h, C.byref(self.dims))
 
@@ -377,6 +400,14 @@ class FlipSimC(_Base):
 
     def sync(self):
         self._call("sync", self.h)
+
+    def read_particles_async(self, n=None):
+        """Start reading the particles (id order) without waiting: returns a
+        PendingRead whose wait() gives the (n, 4) array of the state at this call."""
+        cnt = C.c_uint32(0)
+        self._call("get_count", self.h, C.byref(cnt))
+        n = cnt.value if n is None else min(int(n), cnt.value)
+        return PendingRead(self.lib, self.h, ST_READ_PARTICLES, n)
 
     def grow(self, slots):
         self._call("grow", self.h, int(slots))
@@ -458,6 +489,46 @@ class FlipSimC(_Base):
             pass
 
 
+ST_READ_PARTICLES, ST_READ_CTL, ST_READ_SPRAY, ST_READ_DEEP = 0, 1, 2, 3
+
+
+class PendingRead:
+    """One asynchronous read (st_read_begin): the copy is enqueued at construction;
+    wait() blocks until it has landed and returns the result (any thread)."""
+
+    def __init__(self, lib, sim_h, what, n):
+        self.lib, self.what, self.n = lib, what, n
+        t = C.c_void_p()
+        if lib.st_read_begin(sim_h, what, n, C.byref(t)) != 0:
+            raise StError(f"st_read_begin: {lib.st_last_error().decode()}")
+        self.t = t
+
+    def wait(self):
+        room = self.n if self.what == ST_READ_PARTICLES else 16
+        out = np.zeros((max(room, 1), 4) if self.what == ST_READ_PARTICLES else 16,
+                       np.float32 if self.what == ST_READ_PARTICLES else np.uint32)
+        got = C.c_uint32(0)
+        try:
+            if self.what not in (ST_READ_PARTICLES, ST_READ_CTL):
+                raise StError("PendingRead.wait: particles and control words only")
+            if self.lib.st_read_wait(self.t, out.ctypes.data, None, room, C.byref(got)) != 0:
+                raise StError(f"st_read_wait: {self.lib.st_last_error().decode()}")
+        finally:
+            self.close()
+        return out[:got.value]
+
+    def close(self):
+        if getattr(self, "t", None):
+            self.lib.st_read_free(self.t)
+            self.t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FlipBatchC:
     """Many tanks (FlipSimC objects) advanced together, one launch per kernel
     for all of them (st_batch_*).  The tanks stay usable on their own (upload,
@@ -472,6 +543,7 @@ class FlipBatchC:
         if rc != 0:
             raise StError(f"st_batch_create -> {rc}: {self.lib.st_last_error().decode()}")
         self.h = h
+        _LIVE["batch"].add(self)
 
     def _call(self, name, *args):
         rc = getattr(self.lib, f"st_batch_{name}")(*args)
@@ -532,6 +604,7 @@ class SnapshotC:
         if self.lib.st_snapshot_create(sim.h, C.byref(h)) != 0:
             raise StError(f"st_snapshot_create: {self.lib.st_last_error().decode()}")
         self.h = h
+        _LIVE["snapshot"].add(self)
 
     def restore(self):
         if self.lib.st_snapshot_restore(self.sim.h, self.h) != 0:
@@ -562,6 +635,7 @@ class AdjointC:
         if self.lib.st_adj_create(sim.h, int(max_steps), C.byref(h)) != 0:
             raise StError(f"st_adj_create: {self.lib.st_last_error().decode()}")
         self.h = h
+        _LIVE["adjoint"].add(self)
 
     def _call(self, name, *args):
         if getattr(self.lib, f"st_adj_{name}")(self.h, *args) != 0:

@@ -9,7 +9,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <atomic>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -67,6 +70,18 @@ struct StepGraph {
   hipGraphExec_t exec;
 };
 const size_t GRAPH_CACHE = 4;
+
+// One pinned staging buffer per sim, lent to one read at a time; a read that
+// finds it lent, or too small while lent, gets a buffer of its own.
+struct PinCache {
+  std::mutex m;
+  char* ptr = nullptr;
+  size_t bytes = 0;
+  bool lent = false;
+  ~PinCache() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+};
 
 struct Sim {
   int dev = 0;
@@ -133,7 +148,10 @@ struct Sim {
   // sized for deepCells cells; deepPrev = deep cells counted by the last read
   uint32_t *deepFlag = nullptr, *deepOff = nullptr, *deepSums = nullptr, *deepAll = nullptr;
   float4* deepOut = nullptr;
-  uint32_t deepCells = 0, deepPrev = 0;
+  uint32_t deepCells = 0;
+  // shared with the asynchronous reads, which set it when they are waited for
+  std::shared_ptr<std::atomic<uint32_t>> deepPrev = std::make_shared<std::atomic<uint32_t>>(0u);
+  std::shared_ptr<PinCache> pin;  // pinned staging of the asynchronous reads
   double* normPart = nullptr;     // st_solve_norms: per-workgroup partials, then the result
   uint32_t normCap = 0;           // workgroups normPart has room for
   hipStream_t ownStream = nullptr;  // the tank's own stream while it is in a batch
@@ -1214,6 +1232,20 @@ int st_read_spray(void* p, float* pos4, float* vel4, uint32_t room, uint32_t* n)
 }
 
 static_assert(DEEP_MAX == ST_DEEP_MAX, "one deep point limit for the kernels and the C ABI");
+// the deep point pass's buffers, sized for `cells` cells
+static int deepBuffers(Sim* s, uint32_t cells) {
+  if (s->deepCells >= cells && s->deepOut) return ST_OK;
+  HIPCHK(hipStreamSynchronize(s->st));
+  dfree(s->deepFlag); dfree(s->deepOff); dfree(s->deepSums); dfree(s->deepAll); dfree(s->deepOut);
+  s->deepCells = 0;
+  int rc;
+  if ((rc = dalloc(&s->deepFlag, cells)) || (rc = dalloc(&s->deepOff, size_t(cells) + 1)) ||
+      (rc = dalloc(&s->deepSums, size_t(cells) / 1024 + 2)) || (rc = dalloc(&s->deepAll, 1)) ||
+      (rc = dalloc(&s->deepOut, size_t(ST_DEEP_MAX))))
+    return rc;
+  s->deepCells = cells;
+  return ST_OK;
+}
 int st_read_deep(void* p, float* out4, uint32_t room, uint32_t* n) {
   Sim* s = getSim(p);
   if (!s || !n || (room && !out4)) return fail(ST_ERR_ARG, "st_read_deep: null argument");
@@ -1222,27 +1254,169 @@ int st_read_deep(void* p, float* out4, uint32_t room, uint32_t* n) {
   HIPCHK(hipSetDevice(s->dev));
   const uint32_t cells = s->scap * 512u;
   int rc;
-  if (s->deepCells < cells || !s->deepOut) {
-    HIPCHK(hipStreamSynchronize(s->st));
-    dfree(s->deepFlag); dfree(s->deepOff); dfree(s->deepSums); dfree(s->deepAll); dfree(s->deepOut);
-    s->deepCells = 0;
-    if ((rc = dalloc(&s->deepFlag, cells)) || (rc = dalloc(&s->deepOff, size_t(cells) + 1)) ||
-        (rc = dalloc(&s->deepSums, size_t(cells) / 1024 + 2)) || (rc = dalloc(&s->deepAll, 1)) ||
-        (rc = dalloc(&s->deepOut, size_t(ST_DEEP_MAX))))
-      return rc;
-    s->deepCells = cells;
-  }
-  launch_deep(s->st, s->ctx(), cells, s->deepPrev, s->deepFlag, s->deepOff, s->deepSums, s->deepAll, s->deepOut);
+  if ((rc = deepBuffers(s, cells)) != ST_OK) return rc;
+  launch_deep(s->st, s->ctx(), cells, s->deepPrev->load(), s->deepFlag, s->deepOff, s->deepSums, s->deepAll, s->deepOut);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("st_read_deep: ") + hipGetErrorString(e));
   uint32_t kept = 0, all = 0;
   HIPCHK(hipMemcpyAsync(&kept, s->deepOff + cells, 4, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipMemcpyAsync(&all, s->deepAll, 4, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  s->deepPrev = all;  // deepFin: this call's count picks the next call's sample
+  s->deepPrev->store(all);  // deepFin: this call's count picks the next call's sample
   const uint32_t m = std::min<uint32_t>(std::min<uint32_t>(kept, ST_DEEP_MAX), room);
   if (m) HIPCHK(hipMemcpy(out4, s->deepOut, size_t(m) * 16, hipMemcpyDeviceToHost));
   *n = m;
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// asynchronous readback (st_read_begin / st_read_wait / st_read_free)
+
+struct ReadTicket {
+  int dev = 0;
+  uint32_t what = 0, n = 0;
+  hipEvent_t ev = nullptr;
+  char* buf = nullptr;
+  size_t bytes = 0;
+  std::shared_ptr<PinCache> cache;  // set when buf is the cache's
+  std::shared_ptr<std::atomic<uint32_t>> deepPrev;
+  ~ReadTicket() {
+    if (ev) {
+      (void)hipEventSynchronize(ev);  // the copy into buf has landed before buf is let go
+      (void)hipEventDestroy(ev);
+    }
+    if (buf && cache) {
+      std::lock_guard<std::mutex> g(cache->m);
+      cache->lent = false;
+    } else if (buf) {
+      (void)hipHostFree(buf);
+    }
+  }
+};
+
+static int ticketBuffer(Sim* s, ReadTicket* t, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 64);
+  if (!s->pin) s->pin = std::make_shared<PinCache>();
+  PinCache& c = *s->pin;
+  std::lock_guard<std::mutex> g(c.m);
+  if (!c.lent) {
+    if (c.bytes < bytes) {
+      if (c.ptr) (void)hipHostFree(c.ptr);
+      c.ptr = nullptr;
+      c.bytes = 0;
+      if (hipHostMalloc((void**)&c.ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        c.ptr = nullptr;
+        return fail(ST_ERR_NOMEM, "st_read_begin: hipHostMalloc failed");
+      }
+      c.bytes = bytes;
+    }
+    c.lent = true;
+    t->buf = c.ptr;
+    t->cache = s->pin;
+  } else if (hipHostMalloc((void**)&t->buf, bytes, hipHostMallocDefault) != hipSuccess) {
+    t->buf = nullptr;
+    return fail(ST_ERR_NOMEM, "st_read_begin: hipHostMalloc failed");
+  }
+  t->bytes = bytes;
+  return ST_OK;
+}
+
+int st_read_begin(void* p, uint32_t what, uint32_t n, void** ticket) {
+  Sim* s = getSim(p);
+  if (!s || !ticket) return fail(ST_ERR_ARG, "st_read_begin: null argument");
+  *ticket = nullptr;
+  if (what > ST_READ_DEEP) return fail(ST_ERR_ARG, "st_read_begin: unknown result");
+  HIPCHK(hipSetDevice(s->dev));
+  std::unique_ptr<ReadTicket> t(new ReadTicket());
+  t->dev = s->dev;
+  t->what = what;
+  int rc;
+  if (what == ST_READ_PARTICLES) {
+    t->n = std::min(n, s->N);
+    if ((rc = ticketBuffer(s, t.get(), size_t(t->n) * 16)) != ST_OK) return rc;
+    if (t->n) {
+      launch_unsort(s->st, s->G.pos, s->G.pid, s->G.scratch4, t->n);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(t->buf, s->G.scratch4, size_t(t->n) * 16, hipMemcpyDeviceToHost, s->st));
+    }
+  } else if (what == ST_READ_CTL) {
+    if ((rc = ticketBuffer(s, t.get(), 64)) != ST_OK) return rc;
+    HIPCHK(hipMemcpyAsync(t->buf, s->G.ctl, 64, hipMemcpyDeviceToHost, s->st));
+  } else if (what == ST_READ_SPRAY) {
+    // the slot counter, then every slot (positions, velocities): n is fixed at the wait
+    t->n = s->sprayCap;
+    if ((rc = ticketBuffer(s, t.get(), 16 + size_t(s->sprayCap) * 32)) != ST_OK) return rc;
+    if (s->sprayCap) {
+      HIPCHK(hipMemcpyAsync(t->buf, s->G.sctl, 4, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(t->buf + 16, s->G.sPos, size_t(s->sprayCap) * 16, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(t->buf + 16 + size_t(s->sprayCap) * 16, s->G.sVel, size_t(s->sprayCap) * 16, hipMemcpyDeviceToHost, s->st));
+    }
+  } else {  // ST_READ_DEEP: the kept and counted cells, then the points
+    const bool none = !s->nbOn || !s->G.velG || !s->N || !s->cur;  // readDeep gives null (sim.js:409)
+    t->n = none ? 0u : 1u;
+    if ((rc = ticketBuffer(s, t.get(), none ? 64 : 16 + size_t(ST_DEEP_MAX) * 16)) != ST_OK) return rc;
+    if (!none) {
+      const uint32_t cells = s->scap * 512u;
+      if ((rc = deepBuffers(s, cells)) != ST_OK) return rc;
+      launch_deep(s->st, s->ctx(), cells, s->deepPrev->load(), s->deepFlag, s->deepOff, s->deepSums, s->deepAll, s->deepOut);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(t->buf, s->deepOff + cells, 4, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(t->buf + 4, s->deepAll, 4, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(t->buf + 16, s->deepOut, size_t(ST_DEEP_MAX) * 16, hipMemcpyDeviceToHost, s->st));
+      t->deepPrev = s->deepPrev;
+    }
+  }
+  HIPCHK(hipEventCreateWithFlags(&t->ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(t->ev, s->st));
+  *ticket = t.release();
+  return ST_OK;
+}
+
+int st_read_wait(void* p, void* out0, void* out1, uint32_t room, uint32_t* n) {
+  ReadTicket* t = (ReadTicket*)p;
+  if (!t || !n) return fail(ST_ERR_ARG, "st_read_wait: null argument");
+  *n = 0;
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipEventSynchronize(t->ev));
+  switch (t->what) {
+    case ST_READ_PARTICLES:
+      if (t->n > room || (t->n && !out0)) return fail(ST_ERR_ARG, "st_read_wait: output array too small");
+      if (t->n) std::memcpy(out0, t->buf, size_t(t->n) * 16);
+      *n = t->n;
+      break;
+    case ST_READ_CTL:
+      if (room < 16 || !out0) return fail(ST_ERR_ARG, "st_read_wait: the control words need 16 entries");
+      std::memcpy(out0, t->buf, 64);
+      *n = 16;
+      break;
+    case ST_READ_SPRAY: {
+      if (!t->n) break;
+      uint32_t used = 0;
+      std::memcpy(&used, t->buf, 4);
+      const uint32_t k = std::max(1u, std::min(t->n, used));
+      if (k > room || !out0 || !out1) return fail(ST_ERR_ARG, "st_read_wait: output arrays too small");
+      std::memcpy(out0, t->buf + 16, size_t(k) * 16);
+      std::memcpy(out1, t->buf + 16 + size_t(t->n) * 16, size_t(k) * 16);
+      *n = k;
+      break;
+    }
+    default: {  // ST_READ_DEEP
+      if (!t->n) break;
+      uint32_t kept = 0, all = 0;
+      std::memcpy(&kept, t->buf, 4);
+      std::memcpy(&all, t->buf + 4, 4);
+      t->deepPrev->store(all);  // deepFin: this read's count picks the next read's sample
+      const uint32_t m = std::min<uint32_t>(std::min<uint32_t>(kept, ST_DEEP_MAX), room);
+      if (m && !out0) return fail(ST_ERR_ARG, "st_read_wait: null output");
+      if (m) std::memcpy(out0, t->buf + 16, size_t(m) * 16);
+      *n = m;
+    }
+  }
+  return ST_OK;
+}
+
+int st_read_free(void* p) {
+  delete (ReadTicket*)p;
   return ST_OK;
 }
 

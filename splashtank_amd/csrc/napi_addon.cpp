@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -468,6 +469,125 @@ napi_value SetSprayCap(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- asynchronous reads (st_read_begin / st_read_wait / st_read_free) ----
+// readAsync(h, what, dst, n) -> Promise.  The copy is enqueued on the sim's
+// stream at once (behind every step enqueued before); a libuv worker thread
+// waits for it and copies it out (napi_async_work), so the event loop runs
+// meanwhile; the Promise settles on the main thread.  what 0: particles into
+// dst (a Float32Array of >= 4n floats, kept alive until then), resolves n;
+// 1: the 16 control words, resolves a Uint32Array; 2: whitewater slots,
+// resolves {n, p, v} or null; 3: deep water points, resolves {n, p} or null.
+struct ReadWork {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref dstRef = nullptr;
+  void* ticket = nullptr;
+  uint32_t what = 0, room = 0, n = 0;
+  float* dst = nullptr;
+  std::vector<float> a, b;
+  uint32_t ctl[16] = {};
+  int rc = 0;
+  std::string err;
+};
+
+void readExecute(napi_env, void* data) {
+  ReadWork* w = (ReadWork*)data;
+  void *o0 = nullptr, *o1 = nullptr;
+  if (w->what == ST_READ_PARTICLES) {
+    o0 = w->dst;
+  } else if (w->what == ST_READ_CTL) {
+    o0 = w->ctl;
+    w->room = 16;
+  } else if (w->what == ST_READ_SPRAY) {
+    w->a.resize(size_t(std::max(w->room, 1u)) * 4);
+    w->b.resize(size_t(std::max(w->room, 1u)) * 4);
+    o0 = w->a.data();
+    o1 = w->b.data();
+  } else {
+    w->room = ST_DEEP_MAX;
+    w->a.resize(size_t(ST_DEEP_MAX) * 4);
+    o0 = w->a.data();
+  }
+  w->rc = st_read_wait(w->ticket, o0, o1, w->room, &w->n);
+  if (w->rc != ST_OK) w->err = st_last_error();  // this thread's error text
+}
+
+napi_value floats(napi_env env, const float* src, uint32_t n4) {
+  void* data = nullptr;
+  napi_value ab, ta;
+  napi_create_arraybuffer(env, size_t(n4) * 4, &data, &ab);
+  if (n4) std::memcpy(data, src, size_t(n4) * 4);
+  napi_create_typedarray(env, napi_float32_array, n4, ab, 0, &ta);
+  return ta;
+}
+
+void readComplete(napi_env env, napi_status status, void* data) {
+  ReadWork* w = (ReadWork*)data;
+  st_read_free(w->ticket);
+  if (w->dstRef) napi_delete_reference(env, w->dstRef);
+  napi_value out;
+  if (status != napi_ok || w->rc != ST_OK) {
+    napi_value msg;
+    const std::string m = "readAsync failed: " + (w->err.empty() ? std::string("cancelled") : w->err);
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &out);
+    napi_reject_deferred(env, w->deferred, out);
+  } else {
+    if (w->what == ST_READ_PARTICLES) {
+      out = mkNum(env, double(w->n));
+    } else if (w->what == ST_READ_CTL) {
+      void* d = nullptr;
+      napi_value ab;
+      napi_create_arraybuffer(env, 64, &d, &ab);
+      std::memcpy(d, w->ctl, 64);
+      napi_create_typedarray(env, napi_uint32_array, 16, ab, 0, &out);
+    } else if (!w->n) {
+      napi_get_null(env, &out);
+    } else {
+      napi_create_object(env, &out);
+      napi_set_named_property(env, out, "n", mkNum(env, double(w->n)));
+      napi_set_named_property(env, out, "p", floats(env, w->a.data(), w->n * 4));
+      if (w->what == ST_READ_SPRAY) napi_set_named_property(env, out, "v", floats(env, w->b.data(), w->n * 4));
+    }
+    napi_resolve_deferred(env, w->deferred, out);
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+napi_value ReadAsync(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  getArgs(env, info, 4, argv);
+  void* h = handleOf(env, argv[0]);
+  uint32_t what = 0, n = 0;
+  napi_get_value_uint32(env, argv[1], &what);
+  napi_get_value_uint32(env, argv[3], &n);
+  if (what > ST_READ_DEEP) return throwErr(env, "readAsync: unknown result");
+  std::unique_ptr<ReadWork> w(new ReadWork());
+  w->what = what;
+  w->room = n;
+  if (what == ST_READ_PARTICLES) {
+    void* dst = nullptr;
+    size_t len = 0;
+    if (!typed(env, argv[2], &dst, &len, napi_float32_array)) return throwErr(env, "readAsync: dst must be a Float32Array");
+    if (size_t(n) * 4 > len) return throwErr(env, "readAsync: dst shorter than n*4");
+    w->dst = (float*)dst;
+    NAPI_OK(napi_create_reference(env, argv[2], 1, &w->dstRef));
+  }
+  int rc = st_read_begin(h, what, n, &w->ticket);
+  if (rc != ST_OK) {
+    if (w->dstRef) napi_delete_reference(env, w->dstRef);
+    return throwErr(env, std::string("st_read_begin failed: ") + st_last_error());
+  }
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "splashtank.readAsync", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, readExecute, readComplete, w.get(), &w->work));
+  NAPI_OK(napi_queue_async_work(env, w->work));
+  w.release();
+  return promise;
+}
+
 // readSpray(h, room) -> {n, p: Float32Array(4n), v: Float32Array(4n)} or null without a pool
 napi_value ReadDeep(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -772,7 +892,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"getDims", GetDims}, {"upload", Upload}, {"setCount", SetCount},
       {"clear", Clear}, {"setColliders", SetColliders}, {"step", Step}, {"beginStep", BeginStep}, {"stage", Stage},
       {"readParticles", ReadParticles}, {"readCtl", ReadCtl}, {"grow", Grow}, {"dump", Dump}, {"sync", Sync},
-      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"oceanWaves", OceanWaves}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
+      {"memBytes", MemBytes}, {"solveNorms", SolveNorms}, {"setSprayCap", SetSprayCap}, {"buildSdf", BuildSdf}, {"oceanWaves", OceanWaves}, {"packView", PackView}, {"setDrips", SetDrips}, {"readSpray", ReadSpray}, {"readDeep", ReadDeep}, {"readAsync", ReadAsync}, {"version", Version}, {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy},
       {"batchStep", BatchStep}, {"batchSync", BatchSync}, {"stepN", StepN}, {"batchStepN", BatchStepN}, {"tuneSor", TuneSor},
       {"batchTuneSor", BatchTuneSor}, {"snapshotCreate", SnapshotCreate}, {"snapshotRestore", SnapshotRestore},
       {"snapshotDestroy", SnapshotDestroy}, {"adjCreate", AdjCreate}, {"adjRecord", AdjRecord}, {"adjSeed", AdjSeed},

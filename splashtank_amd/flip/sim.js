@@ -8,9 +8,13 @@
 // unchanged.  What differs, and why:
 //  - `device` is a HIP device handle ({ ordinal }) from initHip(), not a GPUDevice;
 //  - there is no pipeline compile step: `ready` is true and `whenReady` resolves at once;
-//  - step() enqueues on the sim's HIP stream; readbacks wait for the stream, then
-//    resolve their Promise (the reference maps buffers asynchronously);
-//  - `stats` is read when asked, so it is the last step's (the reference's lags one);
+//  - step() enqueues on the sim's HIP stream; readbackInto, readSpray, readDeep and
+//    readStats enqueue their copy at once and return a Promise that a worker
+//    thread settles when the copy has landed (the reference's mapAsync), so the
+//    event loop runs meanwhile and the result is the state at the call;
+//  - `stats` is the last finished readStats() result, started after each step
+//    while the pool can grow (the reference's lags one step the same way); read
+//    before any has finished, it reads the control words at once;
 //  - the block pool defaults to every tile of the box (no growth needed on a
 //    288 GB GPU); opt.slots still sets a smaller pool, with the reference's
 //    overflow counting and growth policy.
@@ -249,9 +253,17 @@ class FlipSim {
     if (!t) return undefined;
     const t0 = Date.now();
     loadAddon().step(this.handle, t.prm);
-    this.statsCache = null; this.stepped = true;
+    this.afterStep();
     if (!this.only) this.track(t0);
     return t.nc;
+  }
+  // after enqueueing steps: while the pool can grow, the control words are read without
+  // blocking (readStats in the reference's step, sim.js:237) and `stats` keeps the last
+  // landed read; with every tile pooled nothing needs them, so `stats` is read when asked
+  afterStep() {
+    this.stepped = true;
+    if (this.scap < this.scapMax && !this.only) this.readStats().catch(() => {});
+    else this.statsCache = null;
   }
   track(t0) { const ms = Date.now() - t0; this.lastMs = ms; this.slowMs = Math.max(this.slowMs, ms); }
   // n substeps with the same settings, as n step() calls; graph: each one a HIP graph replay (plain
@@ -261,7 +273,7 @@ class FlipSim {
     if (!t) return undefined;
     if (!this.only) { this.tm += dt * (n - 1); this.seed = (this.seed + n - 1) >>> 0; }
     loadAddon().stepN(this.handle, t.prm, n | 0, graph ? 1 : 0);
-    this.statsCache = null; this.stepped = true;
+    this.afterStep();
     return t.nc;
   }
   // times the pressure-solve paths from the current state (left unchanged) and keeps the fastest for
@@ -299,13 +311,24 @@ class FlipSim {
     return out;
   }
   // ---- pool (sim.js:137-178) ----
-  get stats() {
-    if (this.statsCache) return this.statsCache;
-    const a = loadAddon().readCtl(this.handle);
-    const vmax = new Float32Array(a.buffer)[6];
-    this.statsCache = { sc: a[4], ovf: a[5], fluid: a[0] + a[1], vmax, seen: false };
+  statsFrom(a) {
+    this.statsCache = { sc: a[4], ovf: a[5], fluid: a[0] + a[1], vmax: new Float32Array(a.buffer)[6], seen: false };
     this.peak = Math.max(this.peak, a[4]);
     return this.statsCache;
+  }
+  get stats() {
+    if (this.statsCache) return this.statsCache;
+    return this.statsFrom(loadAddon().readCtl(this.handle));
+  }
+  // the control words of the steps enqueued so far, read without blocking (sim.js:160-168):
+  // sets `stats` when they land; one read at a time
+  readStats() {
+    if (this.statBusy || !this.handle) return this.statPromise;
+    this.statBusy = true;
+    this.statPromise = loadAddon().readAsync(this.handle, 1, null, 16).then((a) => {
+      this.statsFrom(a); this.statBusy = false; return this.statsCache;
+    }, (e) => { this.statBusy = false; throw e; });
+    return this.statPromise;
   }
   grow(n) {
     n = Math.min(this.scapMax, Math.max(n, this.scap + 1)); if (n <= this.scap) return false;
@@ -317,15 +340,17 @@ class FlipSim {
   maybeGrow() {
     // a pool that holds every tile of the box never overflows: no stats read, no sync
     if (this.scap >= this.scapMax || !this.stepped) return;
-    const s = this.stats; if (s.seen) return;
+    const s = this.statsCache; if (!s || s.seen) return;
     s.seen = true; this.ovfTotal += s.ovf;
     const want = s.sc + s.ovf;
     if (s.ovf > 0 || want > 0.8 * this.scap) this.grow(Math.max(Math.ceil(this.scap * 1.5), Math.ceil(want * 1.4) + 8));
   }
   async prime(o) {
     for (let k = 0; k < 8; k++) {
+      if (this.statBusy) await this.statPromise;
       this.only = 'tiles'; this.step(1 / 60, o); this.only = null;
-      const s = this.stats; s.seen = true;
+      this.statsCache = null;
+      const s = await this.readStats(); s.seen = true;
       if (s.ovf === 0 && s.sc <= 0.8 * this.scap) return;
       this.grow(Math.max(Math.ceil(this.scap * 1.5), Math.ceil((s.sc + s.ovf) * 1.4) + 8));
     }
@@ -334,8 +359,8 @@ class FlipSim {
   readbackInto(dst) {
     this.flushRun();
     const n = this.N;
-    if (n > 0) loadAddon().readParticles(this.handle, dst, n);
-    return Promise.resolve(n);
+    if (n <= 0) return Promise.resolve(n);
+    return loadAddon().readAsync(this.handle, 0, dst, n);
   }
   async activeCells() { const s = this.stats; this.blocksUsed = s.sc; return s.fluid; }
   memBytes() { return loadAddon().memBytes(this.handle); }
@@ -370,11 +395,13 @@ class FlipSim {
   // room for n whitewater points, 0 turns the spray stage off (sim.js:333-340)
   setSprayCap(n) { n = Math.max(0, n | 0); loadAddon().setSprayCap(this.handle, n); this.sprayCap = n; }
   // every whitewater slot handed out so far: positions with life in w, velocities with kind in w (sim.js:423-437)
-  async readSpray() { return this.sprayCap ? loadAddon().readSpray(this.handle, this.sprayCap) : null; }
+  readSpray() { return this.sprayCap ? loadAddon().readAsync(this.handle, 2, null, this.sprayCap) : Promise.resolve(null); }
   // narrow band look only (sim.js:405-419): deep water cells (water on the grid with no particles) as points
   setDeepLook(on) { this.deepOn = !!on; }
   // the deep water points of the last step (x, y, z, 4096 + speed), in active-cell order, or null
-  async readDeep() { return this.deepOn && this.handle && this.N ? loadAddon().readDeep(this.handle) : null; }
+  readDeep() {
+    return this.deepOn && this.nbOn && this.handle && this.N ? loadAddon().readAsync(this.handle, 3, null, 0) : Promise.resolve(null);
+  }
   // the live view (sim.js:504-525): packed on the GPU (st_pack_view), here read back at once; outFrame is a
   // frame in the cache format (views into the packed words) that the drawing uploads as it is
   pump() {

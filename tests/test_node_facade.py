@@ -149,3 +149,27 @@ def test_facade_extensions(tmp_path):
     for got, want in ((js[0][:, :3], gpos[:, :3]), (js[1][:, :3], gvel[:, :3])):
         np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6 * float(np.abs(want).max()))
     assert np.abs(gvel[:, :3]).max() > 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_async_readback_keeps_the_event_loop_running(tmp_path):
+    """readbackInto returns a Promise at once; the event loop turns while the
+    copy is pending; the result is the state at the call although substeps were
+    enqueued right after it; readStats and readSpray resolve what the blocking
+    reads give."""
+    sc = scenes.dam_break(d=1.0 / 128, iters=40)  # the 64^3 case: 502,656 particles
+    parts = np.concatenate([sc.pos, sc.vel], axis=1).astype(np.float32)
+    (tmp_path / "p.bin").write_bytes(parts.tobytes())
+    cfg = dict(particles=str(tmp_path / "p.bin"), d=sc.d, size=list(sc.size), open=sc.open_, ratio=sc.ratio,
+               steps=20, after=10, sprayCap=20000, sprSteps=6, params=dict(sc.params))
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    print(f"[facade] async readback, {sc.n} particles", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "async_run.js"), str(tmp_path / "cfg.json")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout.strip().splitlines()[-1])
+    print(o)
+    assert o["pendingAfterCall"] and o["loopTurnsWhilePending"] >= 1
+    assert o["resolvedN"] == sc.n and o["stateAtCall"]
+    assert o["stats"] and o["spray"]
