@@ -198,6 +198,36 @@ template <class R> struct Sim {
     lst.assign(size_t(lcap) * 2, 0u);
   }
 
+  // grow (sim.js:137-152): a pool of n slots, the old slots' contents unchanged,
+  // the new slots pushed on the free stack (growFree, wgsl-blocks.js:57-60; the
+  // reference pushes them in atomic order, here in slot order, one of its orders)
+  int grow(uint32_t n) {
+    n = std::min(scapMax, std::max(n, scap + 1u));
+    if (n <= scap) return ST_OK;
+    const uint32_t old = scap;
+    const size_t C = 512;
+    cnt.resize(n * C, 0);
+    accVi.resize(n * C * 3, 0);
+    accWi.resize(n * C * 3, 0);
+    ctype.resize(n * C, 0u);
+    pres.resize(n * C, R(0));
+    rhs.resize(n * C, R(0));
+    if (!kF32) {
+      faceV.resize(n * C * 3, R(0));
+      faceO.resize(n * C * 3, R(0));
+    }
+    velG.resize(n * C * 3, R(0));
+    slotList.resize(n, 0u);
+    slotTile.resize(n, 0u);
+    fstack.resize(n, 0u);
+    for (uint32_t t = 0; t < n - old; t++) fstack[ctl[2] + t] = old + t;
+    ctl[2] += n - old;
+    scap = n;
+    lcap = velGOn ? n * 512u : std::min<uint32_t>(n * 512u, maxn);
+    lst.assign(size_t(lcap) * 2, 0u);  // the lists are rebuilt every step
+    return ST_OK;
+  }
+
   int init(const st_opts* o) {
     d = R(o->d);
     double rt = o->ratio > 0 ? o->ratio : 2.0;
@@ -1620,6 +1650,7 @@ int orc_read_spray(void* hp, double* p4, double* v4, uint32_t room, uint32_t* n)
 int orc_begin_step(void* hp, const st_params* p) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, beginStep(p)); }
 int orc_stage(void* hp, int32_t st) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, runStage(st)); }
 int orc_step(void* hp, const st_params* p) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, step(p)); }
+int orc_grow(void* hp, uint32_t n) { Handle* h = (Handle*)hp; return ORC_DISPATCH(h, grow(n)); }
 int orc_read_ctl(void* hp, uint32_t ctl[16]) {
   Handle* h = (Handle*)hp;
   std::memcpy(ctl, h->fp64 ? h->d->ctl : h->f->ctl, 64);

@@ -20,6 +20,7 @@ _PROTOS = {
     "orc_last_error": (C.c_char_p, []),
     "orc_create": (C.c_int, [C.POINTER(StOpts), C.c_int, C.POINTER(C.c_void_p)]),
     "orc_destroy": (C.c_int, [C.c_void_p]),
+    "orc_grow": (C.c_int, [C.c_void_p, C.c_uint32]),
     "orc_get_dims": (C.c_int, [C.c_void_p, C.POINTER(StDims)]),
     "orc_upload_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "orc_set_count": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -97,6 +98,11 @@ class Oracle(_Base):
     def step_n(self, prm, n):
         for _ in range(n):
             self.step(prm)
+
+    def grow(self, slots):
+        """grow (sim.js:137-152): more slots, the old ones unchanged, the new ones on the free stack."""
+        self._call("grow", self.h, int(slots))
+        self._call("get_dims", self.h, C.byref(self.dims))
 
     def read_particles(self, n=None):
         return self.dump("pos")

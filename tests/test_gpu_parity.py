@@ -47,15 +47,20 @@ def _compare(g, o, stage, step, log):
         assert ca[k] == cb[k], f"step {step} after {capi.STAGES[stage]}: ctl[{k}] {ca[k]} vs {cb[k]}"
 
 
-def run_stagewise(sc, nsteps, check_every_stage=True, **optkw):
-    opts = opts_for(sc, **optkw)
-    g = FlipSimC(opts)
-    o = Oracle(opts)
-    upload_scene(g, sc)
-    upload_scene(o, sc)
+def run_stagewise(sc, nsteps, check_every_stage=True, sims=None, t0=0.0, **optkw):
+    """nsteps substeps of the GPU and the oracle, compared stage by stage; `sims`
+    continues a pair (g, o) from time t0 instead of starting from the scene."""
+    if sims is None:
+        opts = opts_for(sc, **optkw)
+        g = FlipSimC(opts)
+        o = Oracle(opts)
+        upload_scene(g, sc)
+        upload_scene(o, sc)
+    else:
+        g, o = sims
     prm = capi.make_params(**sc.params)
     log = []
-    t = 0.0
+    t = t0
     for step in range(nsteps):
         t += sc.params["dt"]
         set_colliders_at(g, sc, t)
@@ -98,6 +103,37 @@ def test_small_pool_overflow_matches():
     sc = scenes.dam_break(d=1.0 / 32)
     g, o, _ = run_stagewise(sc, 2, slots=6)
     assert g.ctl()[5] > 0
+
+
+def test_pool_growth_matches_oracle():
+    """The block pool's growth (grow, sim.js:137-152, with maybeGrow's sizes): a
+    pool of 6 slots overflows (the tiles without room are counted and their
+    particles fall freely); grow() leaves every slot's contents as they were and
+    puts the new slots on the free stack (growFree, wgsl-blocks.js:57-60); the
+    substeps after it stay bit-identical to the oracle grown the same way, and
+    nothing overflows any more."""
+    sc = scenes.dam_break(d=1.0 / 32)
+    dt = sc.params["dt"]
+    g, o, _ = run_stagewise(sc, 2, slots=6)
+    assert g.ctl()[5] > 0 and g.ctl()[5] == o.ctl()[5]
+    fields = ("tiles", "tmap", "cnt", "ctype", "pres", "rhs", "facev", "faceo", "pos", "vel")
+    before = {f: (g.dump(f), o.dump(f)) for f in fields}
+    ctl = g.ctl()
+    scap = g.dims.scap
+    want = int(ctl[4]) + int(ctl[5])
+    n = max(-(-scap * 3 // 2), -(-want * 14 // 10) + 8)  # maybeGrow (sim.js:154-159)
+    print(f"[parity] pool of {scap} slots, {ctl[5]} tiles without room; grow to {n}", flush=True)
+    g.grow(n)
+    o.grow(n)
+    assert g.dims.scap == o.dims.scap == min(n, g.dims.scap_max)
+    for f in fields:
+        a, b = g.dump(f), o.dump(f)
+        assert np.array_equal(a.view(np.uint8), before[f][0].view(np.uint8)), f"grow changed the GPU's {f}"
+        assert np.array_equal(b.view(np.uint8), before[f][1].view(np.uint8)), f"grow changed the oracle's {f}"
+    assert g.ctl()[2] == o.ctl()[2]  # free stack height
+    run_stagewise(sc, 3, sims=(g, o), t0=2 * dt)
+    assert g.ctl()[5] == 0 and o.ctl()[5] == 0
+    assert_f32_close(g.dump("pos"), o.dump("pos"), "positions after growth")
 
 
 def test_dam_break_64_ten_steps():

@@ -173,3 +173,29 @@ def test_async_readback_keeps_the_event_loop_running(tmp_path):
     assert o["pendingAfterCall"] and o["loopTurnsWhilePending"] >= 1
     assert o["resolvedN"] == sc.n and o["stateAtCall"]
     assert o["stats"] and o["spray"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_facade_pool_prime_and_growth(tmp_path):
+    """A facade sim with a 6-slot pool and opt.sort on: prime() grows the pool
+    until the first step's tiles fit with the reference's headroom, maybeGrow
+    keeps it ahead while stepping, and the readback equals a sim with every tile
+    pooled and opt.sort off, bit for bit (the pool's size and the particle
+    order in memory change nothing of the result)."""
+    sc = scenes.dam_break(d=1.0 / 48)
+    parts = np.concatenate([sc.pos, sc.vel], axis=1).astype(np.float32)
+    (tmp_path / "p.bin").write_bytes(parts.tobytes())
+    cfg = dict(particles=str(tmp_path / "p.bin"), d=sc.d, size=list(sc.size), open=sc.open_, ratio=sc.ratio,
+               steps=12, params=dict(sc.params))
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    print(f"[facade] pool growth, {sc.n} particles", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "pool_run.js"), str(tmp_path / "cfg.json")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout.strip().splitlines()[-1])
+    print(o)
+    assert o["slotsAtStart"] == 6 and o["slotsAfterPrime"] > 6
+    assert o["primeStats"]["ovf"] == 0 and o["primeStats"]["sc"] <= 0.8 * o["slotsAfterPrime"]
+    assert o["ovfTotal"] == 0 and o["slotsAtEnd"] < o["fullPool"]
+    assert o["same"]
