@@ -1139,19 +1139,20 @@ __device__ __forceinline__ void coop_halo_put(const DGrid& G, size_t idx, float 
   const uint64_t w = (uint64_t(tag) << 32) | uint64_t(__float_as_uint(v));
   __hip_atomic_store(&G.sorHalo[idx], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void coop_fill_foreign(const DGrid& G, uint32_t scap, float* sH, const uint32_t* sNb, const uint32_t* sFor,
-                                                  uint32_t nFor, uint32_t c, uint32_t tag, uint32_t* sFail) {
+// k-th foreign face: its 32 entries are polled at sGetG[k] + cOff + e (cOff
+// the colour's offset) and copied to sH[sFor[k] * 32 + e]
+__device__ __forceinline__ void coop_fill_foreign(const DGrid& G, float* sH, const uint32_t* sFor, const uint32_t* sGetG,
+                                                  uint32_t nFor, uint32_t cOff, uint32_t tag, uint32_t* sFail) {
   for (uint32_t q0 = 0; q0 < nFor * 32u; q0 += COOP_FOR_PER_THREAD * COOP_WG) {
     uint64_t v[COOP_FOR_PER_THREAD];
-    size_t at[COOP_FOR_PER_THREAD];
+    uint32_t at[COOP_FOR_PER_THREAD];
 #pragma unroll
     for (uint32_t i = 0; i < COOP_FOR_PER_THREAD; i++) {
       uint32_t q = q0 + threadIdx.x + i * COOP_WG;
       at[i] = 0;
       v[i] = uint64_t(tag) << 32;
       if (q < nFor * 32u) {
-        uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
-        at[i] = coop_halo_idx(c, scap, sNb[hf] & ~COOP_NB_FOREIGN, f ^ 1u, e);
+        at[i] = sGetG[q >> 5] + cOff + (q & 31u);
         v[i] = __hip_atomic_load(&G.sorHalo[at[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
@@ -1266,9 +1267,16 @@ struct CoopLds {
   uint32_t sNb[COOP_TMAX * 6];
   uint32_t sSlot[COOP_TMAX];
   uint32_t sFor[COOP_TMAX * 6];  // (tile, face) pairs whose neighbour another workgroup owns
+  // per foreign face k (sFor order): lt * 512 | f << 16 of our face, the halo index of
+  // our face's entry 0 (colour 0), and of the entry 0 of the neighbour's face we read
+  uint32_t sPutL[COOP_TMAX * 6];
+  uint32_t sPutG[COOP_TMAX * 6];
+  uint32_t sGetG[COOP_TMAX * 6];
+  uint16_t sFC[6 * 2 * 32];  // colour-split index of cell e of colour c on face f, at (2 f + c) * 32 + e
 };
 static_assert(sizeof(float) * COOP_TMAX * 192 >= 2 * COOP_TMAX * 64 * sizeof(uint16_t), "the row lists fit in the halo copy");
 constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
+static_assert(kCoopLdsBytes + 256 <= 160 * 1024, "the resident solve's LDS fits a CU");
 
 __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
                                                           uint32_t seq, uint32_t stream, uint32_t tag0) {
@@ -1314,7 +1322,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
   uint32_t* sNb = CL.sNb;
   uint32_t* sSlot = CL.sSlot;
   uint32_t* sFor = CL.sFor;
-  const uint32_t scap = PP->scap;
+  const uint32_t scap = PP->scap, cStride = scap * 192u;  // halo entries of one colour
   const uint32_t beg = uint32_t(uint64_t(nact) * w / ng), T = uint32_t(uint64_t(nact) * (w + 1u) / ng) - beg;
   float* pres = G.pres;
   if (tid == 0u) nFor = 0u;
@@ -1344,47 +1352,64 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     uint32_t lt = q >> 9, l = q & 511u;
     sP[lt * 512u + coop_cidx(int32_t(l & 7u), int32_t((l >> 3) & 7u), int32_t(l >> 6))] = pres[sSlot[lt] * 512u + l];
   }
+  for (uint32_t q = tid; q < 6u * 2u * 32u; q += COOP_WG) {
+    const int3 a = coop_face_cell(q >> 6, q & 31u, (q >> 5) & 1u);
+    CL.sFC[q] = uint16_t(coop_cidx(a.x, a.y, a.z));
+  }
+  __syncthreads();  // sFor and nFor
+  // the exchange's addresses, fixed for the solve
+  for (uint32_t k = tid; k < nFor; k += COOP_WG) {
+    const uint32_t hf = sFor[k], lt = hf / 6u, f = hf - lt * 6u;
+    CL.sPutL[k] = lt * 512u | (f << 16);
+    CL.sPutG[k] = uint32_t(coop_halo_idx(0u, scap, sSlot[lt], f, 0u));
+    CL.sGetG[k] = uint32_t(coop_halo_idx(0u, scap, sNb[hf] & ~COOP_NB_FOREIGN, f ^ 1u, 0u));
+  }
   // Row items of each colour, those with a cell on a face toward another
   // workgroup's tile ("early") first: they are relaxed and sent before the
-  // rest.  The lists are built in the halo copy's space, filled only after.
-  // Thread t classifies items 4t .. 4t + 3; an exclusive scan of the packed
-  // (early, late) counts places them.
+  // rest.  Within each part the rows inside the tile in y and z come before
+  // the rows on its y or z faces, so that most waves hold rows of one kind and
+  // skip the reads across those faces.  The lists are built in the halo copy's
+  // space, filled only after.  Thread t classifies items 4t .. 4t + 3; an
+  // exclusive scan of the four packed 16-bit counts places them.
   uint16_t* sList = reinterpret_cast<uint16_t*>(sH);  // [2][COOP_TMAX * 64]
-  __shared__ uint32_t sWave[COOP_WG / 64];
+  __shared__ uint64_t sWave[COOP_WG / 64];
   __shared__ uint32_t sEarly[2];
   const uint32_t nItems = T * 64u;
   for (uint32_t c = 0; c < 2u; c++) {
-    uint32_t flags = 0u, cnt = 0u;
+    uint32_t kinds = 0u;  // 2 bits per item: early ? 0 : 2, + 1 on a y or z face
+    uint64_t cnt = 0u;
     for (uint32_t i = 0; i < 4u; i++) {
       const uint32_t q = tid * 4u + i;
       if (q >= nItems) continue;
-      if (coop_row_early(sNb, q >> 6, q & 63u, c)) {
-        flags |= 1u << i;
-        cnt += 1u;
-      } else {
-        cnt += 1u << 16;
-      }
+      const uint32_t r = q & 63u, y = r & 7u, z = r >> 3;
+      const uint32_t kind = (coop_row_early(sNb, q >> 6, r, c) ? 0u : 2u) + ((y == 0u || y == 7u || z == 0u || z == 7u) ? 1u : 0u);
+      kinds |= kind << (2u * i);
+      cnt += uint64_t(1) << (16u * kind);
     }
-    uint32_t inc = cnt;
+    uint64_t inc = cnt;
     for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t o = (uint32_t)__shfl_up((int)inc, off);
-      if (int(lane_id()) >= off) inc += o;
+      const uint32_t lo = (uint32_t)__shfl_up((int)uint32_t(inc), off), hi = (uint32_t)__shfl_up((int)uint32_t(inc >> 32), off);
+      if (int(lane_id()) >= off) inc += (uint64_t(hi) << 32) | lo;
     }
     if (lane_id() == 63u) sWave[tid >> 6] = inc;
     __syncthreads();
-    uint32_t before = 0u, tot = 0u;
+    uint64_t before = 0u, tot = 0u;
     for (uint32_t u = 0; u < COOP_WG / 64u; u++) {
       if (u < (tid >> 6)) before += sWave[u];
       tot += sWave[u];
     }
-    const uint32_t ex = before + inc - cnt;
-    uint32_t e = ex & 0xffffu, l = (tot & 0xffffu) + (ex >> 16);
+    const uint64_t ex = before + inc - cnt;
+    uint32_t at[4], base = 0u;
+    for (uint32_t kd = 0; kd < 4u; kd++) {
+      at[kd] = base + uint32_t((ex >> (16u * kd)) & 0xffffu);
+      base += uint32_t((tot >> (16u * kd)) & 0xffffu);
+    }
     for (uint32_t i = 0; i < 4u; i++) {
       const uint32_t q = tid * 4u + i;
       if (q >= nItems) continue;
-      sList[c * (COOP_TMAX * 64u) + ((flags >> i) & 1u ? e++ : l++)] = uint16_t(q);
+      sList[c * (COOP_TMAX * 64u) + at[(kinds >> (2u * i)) & 3u]++] = uint16_t(q);
     }
-    if (tid == 0u) sEarly[c] = tot & 0xffffu;
+    if (tid == 0u) sEarly[c] = uint32_t(tot & 0xffffu) + uint32_t((tot >> 16) & 0xffffu);
     __syncthreads();
   }
   // this thread's items (positions tid + j COOP_WG of each list), with the rhs
@@ -1436,9 +1461,9 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         // colour-c face cells other workgroups read, tagged with this half-sweep
         ++epoch;
         for (uint32_t q = tid; q < nFor * 32u; q += COOP_WG) {
-          uint32_t hf = sFor[q >> 5], e = q & 31u, lt = hf / 6u, f = hf - lt * 6u;
-          int3 a = coop_face_cell(f, e, c);
-          coop_halo_put(G, coop_halo_idx(c, scap, sSlot[lt], f, e), sP[lt * 512u + coop_cidx(a.x, a.y, a.z)], tag0 + epoch);
+          const uint32_t k = q >> 5, e = q & 31u, pl = CL.sPutL[k];
+          const float v = sP[(pl & 0xffffu) + CL.sFC[(((pl >> 16) << 1) + c) * 32u + e]];
+          coop_halo_put(G, size_t(CL.sPutG[k]) + c * cStride + e, v, tag0 + epoch);
         }
       }
       // the other rows while the faces travel
@@ -1448,7 +1473,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
         if (pos >= nEarly[c] && pos < nItems) coop_relax_row(sP, sH, sNb, items[c][j], c, rr[c][j], rs[c][j], omega, ytab);
       }
       if (last) break;
-      coop_fill_foreign(G, scap, sH, sNb, sFor, nFor, c, tag0 + epoch, &sFail);
+      coop_fill_foreign(G, sH, sFor, CL.sGetG, nFor, c * cStride, tag0 + epoch, &sFail);
       __syncthreads();
       if (sFail) {  // a neighbour's faces never came: give up the solve (counted)
         if (tid == 0u) atomicAdd(&G.sorBar[COOP_MAX_WG + 1], 1u);
