@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <atomic>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -298,6 +299,56 @@ int airAlloc(Sim* s) {
   return ST_OK;
 }
 
+// A world box around colNear's region of one collider record (32 floats: a =
+// sdf base and dims, b = grid origin and cell, rows of R, centre; colNear
+// takes g = (R (p - c) - origin) / cell and asks lo < g < dims + hq): the
+// local box [origin + lo cell, origin + (dims + hq) cell] through p = c + R^-1 l
+// in f64, padded by 1e-4 of the scene's size.  The f32 test differs from the
+// exact one by a few ulps of |p - c|, far inside the padding, so a point outside
+// the box is one colNear turns down.  Unbounded when R is singular or a value
+// is not finite.
+void nearBox(const float* r, float lo, float hq, float4& wl, float4& wh) {
+  const double inf = std::numeric_limits<double>::infinity();
+  double M[3][3], R[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) R[i][j] = r[8 + 4 * i + j];
+  const double det = R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1]) - R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0]) +
+                     R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]);
+  double l0[3], l1[3], ext = 1.0;
+  bool ok = std::isfinite(det) && std::fabs(det) > 1e-12;
+  for (int j = 0; j < 3; j++) {
+    l0[j] = double(r[4 + j]) + double(lo) * double(r[7]);
+    l1[j] = double(r[4 + j]) + (double(r[1 + j]) + double(hq)) * double(r[7]);
+    ok = ok && std::isfinite(l0[j]) && std::isfinite(l1[j]) && std::isfinite(double(r[20 + j]));
+    ext += std::fabs(l0[j]) + std::fabs(l1[j]) + std::fabs(double(r[20 + j]));
+  }
+  if (!ok) {
+    wl = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    wh = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    return;
+  }
+  // inverse by cofactors
+  M[0][0] = (R[1][1] * R[2][2] - R[1][2] * R[2][1]) / det; M[0][1] = (R[0][2] * R[2][1] - R[0][1] * R[2][2]) / det;
+  M[0][2] = (R[0][1] * R[1][2] - R[0][2] * R[1][1]) / det; M[1][0] = (R[1][2] * R[2][0] - R[1][0] * R[2][2]) / det;
+  M[1][1] = (R[0][0] * R[2][2] - R[0][2] * R[2][0]) / det; M[1][2] = (R[0][2] * R[1][0] - R[0][0] * R[1][2]) / det;
+  M[2][0] = (R[1][0] * R[2][1] - R[1][1] * R[2][0]) / det; M[2][1] = (R[0][1] * R[2][0] - R[0][0] * R[2][1]) / det;
+  M[2][2] = (R[0][0] * R[1][1] - R[0][1] * R[1][0]) / det;
+  const double pad = 1e-4 * ext;
+  double lo3[3], hi3[3];
+  for (int i = 0; i < 3; i++) {
+    double mid = double(r[20 + i]), half = 0.0;
+    for (int j = 0; j < 3; j++) {
+      mid += M[i][j] * 0.5 * (l0[j] + l1[j]);
+      half += std::fabs(M[i][j]) * 0.5 * std::fabs(l1[j] - l0[j]);
+    }
+    lo3[i] = mid - half - pad;
+    hi3[i] = mid + half + pad;
+    if (!std::isfinite(lo3[i]) || !std::isfinite(hi3[i])) { lo3[i] = -inf; hi3[i] = inf; }
+  }
+  wl = make_float4(float(lo3[0]), float(lo3[1]), float(lo3[2]), 0.0f);
+  wh = make_float4(float(hi3[0]), float(hi3[1]), float(hi3[2]), 0.0f);
+}
+
 // the step's parameter block and collider records (sim.js:179-231) into P and C
 int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   if (s->N == 0) return fail(ST_ERR_STATE, "st_begin_step: no particles (step() does nothing when N = 0)");
@@ -418,6 +469,7 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   for (size_t q = 0; q < ncAll; q++) {
     float hq = P.q0 / s->cf[q * 32 + 7];  // q0 / cell size of the collider grid
     C.near[q] = make_float4(-1.0f - hq, hq, 0.0f, 0.0f);
+    nearBox(&s->cf[q * 32], -1.0f - hq, hq, C.wlo[q], C.whi[q]);
   }
   return ST_OK;
 }

@@ -59,6 +59,12 @@ struct DCols {
   // per collider, from its record: (lo, hi margin) of colNear, lo = -1 - q0 / cell,
   // hq = q0 / cell (the same f32 divisions, done once on the host)
   float4 near[ST_MAX_COLS];
+  // per collider, a world box holding the whole colNear region with room to
+  // spare (the host maps the local near box through the inverse rotation, in
+  // f64, and pads it), or +-inf when it cannot be made: a point outside it
+  // is not near, which colNear then need not work out
+  float4 wlo[ST_MAX_COLS];
+  float4 whi[ST_MAX_COLS];
 };
 
 // Device pointers of one sim, passed by value to every kernel.
@@ -240,6 +246,8 @@ __device__ __forceinline__ float3 d_colSurfVel(const DCols* __restrict__ C, uint
 }
 __device__ __forceinline__ bool d_colNear(const DParams& P, const DCols* __restrict__ C, uint32_t k, float3 p) {
   if (P.q10 > 0.5f) return true;
+  const float4 wl = C->wlo[k], wh = C->whi[k];
+  if (p.x < wl.x || p.y < wl.y || p.z < wl.z || p.x > wh.x || p.y > wh.y || p.z > wh.z) return false;
   float4 b = C->c[k * 8u + 1u];
   float3 d = f3sub(p, c3(C->c[k * 8u + 5u]));
   float3 lp = make_float3(f3dot(c3(C->c[k * 8u + 2u]), d), f3dot(c3(C->c[k * 8u + 3u]), d), f3dot(c3(C->c[k * 8u + 4u]), d));

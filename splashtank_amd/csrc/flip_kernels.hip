@@ -1220,15 +1220,16 @@ __device__ __forceinline__ void coop_relax_row(float* sP, const float* sH, const
                                                float4 rhs4, uint32_t st4, float omega, const float* ytab) {
   const uint32_t lt = item >> 6, r = item & 63u, y = r & 7u, z = r >> 3, x0 = (y + z + c) & 1u;
   const uint32_t oc = (c ^ 1u) * 256u;
-  float* cen = sP + lt * 512u + c * 256u + r * 4u;
-  const float* ob = sP + lt * 512u + oc;
-  const float* hb = sH + lt * 192u;
-  const uint32_t* nbs = sNb + lt * 6u;
+  // offsets as 24-bit products (lt < 64): no 32- or 64-bit multiplies
+  float* cen = sP + ((lt << 9) + c * 256u + r * 4u);
+  const float* ob = sP + ((lt << 9) + oc);
+  const float* hb = sH + __umul24(lt, 192u);
+  const uint32_t* nbs = sNb + __umul24(lt, 6u);
   // the address of the neighbour across face f: index w in the tile there when
   // this workgroup holds it, else index h in the face's halo copy
   auto across = [&](uint32_t f, uint32_t w, uint32_t h) -> const float* {
     const uint32_t nb = nbs[f];
-    return (nb & COOP_NB_FOREIGN) ? hb + f * 32u + h : sP + nb * 512u + oc + w;
+    return (nb & COOP_NB_FOREIGN) ? hb + (f * 32u + h) : sP + ((nb << 9) + oc + w);
   };
   const float4 O = *reinterpret_cast<const float4*>(ob + r * 4u);
   const float A = *across(x0, r * 4u + (x0 ? 0u : 3u), z * 4u + (y >> 1));
