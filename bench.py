@@ -173,16 +173,28 @@ def run_batched(args):
         batch.step(prm)
     tune = batch.tune_sor(prm, reps=2)
     batch.sync()
+    # both legs start from the same state (snapshots of every tank after the tuner)
+    snaps = [capi.SnapshotC(s) for s in sims]
+
+    def restore():
+        for q in snaps:
+            q.restore()
+
+    restore()
     t0 = time.perf_counter()
     batch.step_n(prm, K, graph=False)
     batch.sync()
     tun = time.perf_counter() - t0
-    batch.step_n(prm, 1, graph=True)
+    restore()
+    batch.step_n(prm, 2, graph=True)  # captures the graphs (first substep after a restore, and the rest)
     batch.sync()
+    restore()
     t0 = time.perf_counter()
     batch.step_n(prm, K, graph=True)
     batch.sync()
     gra = time.perf_counter() - t0
+    for q in snaps:
+        q.close()
     batch.close()
     for s in sims:
         s.close()

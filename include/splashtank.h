@@ -130,7 +130,17 @@ typedef struct st_params {
   float ocean_level;    /* o.ocean.level: height of the still water */
   float ocean_layer;    /* o.ocean.layer: width of the boundary band at the side walls */
   float waves[8 * 32];  /* per wave: kx, kz, amplitude, angular frequency, phase, wavenumber, 0, 0 */
+  /* Opt-in modes the reference does not have.  0 (the default) is the
+   * reference's path, bit for bit. */
+  uint32_t rk2;           /* 1: a particle of a tile with storage moves by the midpoint rule
+                           * through the grid's new velocity, p + dt u(p + dt/2 u(p)), instead
+                           * of p + dt v (its velocity stays the FLIP/PIC blend) */
+  uint32_t extrap_layers; /* face velocities (new and old) carried from the faces next to the
+                           * fluid into the air around it after the projection, this many
+                           * layers (0 off, at most ST_EXTRAP_MAX) */
 } st_params;
+
+#define ST_EXTRAP_MAX 4u
 
 #define ST_OCEAN_MAX_WAVES 32u
 

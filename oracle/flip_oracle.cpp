@@ -108,6 +108,7 @@ template <class R> struct Params {
   uint32_t nb;
   R band;
   uint32_t ocean;
+  uint32_t rk2, extrap;  // opt-in modes (0: the reference's path)
   uint32_t nbInit, nfill, nfv;
   R fills[16][4];
   // whitewater and air field (wgsl-core.js:10-16)
@@ -808,6 +809,62 @@ template <class R> struct Sim {
       }
     });
   }
+  // Opt-in velocity extrapolation after project (no counterpart in the
+  // reference): per cell a mask, bit a a known face of axis a (fluid on one
+  // side, or set by a layer), bit 3 + a a face with air on both sides; each of
+  // P.extrap layers gives every such face not yet known the mean of the known
+  // faces of the same axis in the six cells around (inside the tile grid, in
+  // tiles with storage, summed x-, x+, y-, y+, z-, z+), new and old values alike.
+  void extrapolate() {
+    if (!P.extrap) return;
+    const size_t C = size_t(scap) * 512u;
+    std::vector<uint8_t> m0(C, 0), m1(C, 0);
+    forActive([&](uint32_t c, I3 at) {
+      const uint32_t th = ctypeOf(at.x, at.y, at.z);
+      uint32_t m = 0;
+      for (uint32_t axis = 0; axis < 3; axis++) {
+        I3 lo = at;
+        if (axis == 0) lo.x -= 1; else if (axis == 1) lo.y -= 1; else lo.z -= 1;
+        const uint32_t tl = ctypeOf(lo.x, lo.y, lo.z);
+        if (tl == 1u || th == 1u) m |= 1u << axis;
+        else if (tl == 0u && th == 0u) m |= 8u << axis;
+      }
+      m0[c] = uint8_t(m);
+    });
+    const int32_t gx = int32_t(tx) * 8, gy = int32_t(ty) * 8, gz = int32_t(tz) * 8;
+    for (uint32_t l = 0; l < P.extrap; l++) {
+      const std::vector<uint8_t>& mi = (l & 1u) ? m1 : m0;
+      std::vector<uint8_t>& mo = (l & 1u) ? m0 : m1;
+      forActive([&](uint32_t c, I3 at) {
+        const uint32_t m = mi[c];
+        uint32_t o = m;
+        for (uint32_t axis = 0; axis < 3; axis++) {
+          if (((m >> axis) & 1u) || !((m >> (3u + axis)) & 1u)) continue;
+          R sn = R(0), so = R(0);
+          uint32_t cnt = 0;
+          for (uint32_t q = 0; q < 6u; q++) {
+            I3 r = at;
+            const int32_t d = (q & 1u) ? 1 : -1;
+            if (q < 2u) r.x += d; else if (q < 4u) r.y += d; else r.z += d;
+            if (r.x < 0 || r.y < 0 || r.z < 0 || r.x >= gx || r.y >= gy || r.z >= gz) continue;
+            const uint32_t sl = tmap[tileOf(r.x, r.y, r.z)];
+            if (sl == NONE) continue;
+            const uint32_t cn = sl * 512u + uint32_t(r.x & 7) + 8u * (uint32_t(r.y & 7) + 8u * uint32_t(r.z & 7));
+            if (!((mi[cn] >> axis) & 1u)) continue;
+            sn = sn + fvR(cn * 3u + axis);
+            so = so + foR(cn * 3u + axis);
+            cnt++;
+          }
+          if (cnt) {
+            fvW(c * 3u + axis, sn / R(cnt));
+            foW(c * 3u + axis, so / R(cnt));
+            o |= 1u << axis;
+          }
+        }
+        mo[c] = uint8_t(o);
+      });
+    }
+  }
   void saveVel() {  // wgsl-narrowband.js:88-91
     forActive([&](uint32_t c, I3) { for (uint32_t ax = 0; ax < 3; ax++) velG[c * 3 + ax] = fvR(c * 3 + ax); });
   }
@@ -902,6 +959,14 @@ template <class R> struct Sim {
       R sp = len(v);
       if (sp > P.vmax) v = v * (P.vmax / sp);
       V3<R> pt = p + v * P.dt;
+      if (P.rk2 != 0u && hasStorage(fcell)) {
+        // opt-in midpoint rule through the grid's new velocity (no counterpart in
+        // the reference): pt = p + dt u(p + dt/2 u(p)), vn being u(p)
+        const R hdt = R(0.5) * P.dt;
+        V3<R> pm = v3(p.x + vn.x * hdt, p.y + vn.y * hdt, p.z + vn.z * hdt), um, uo;
+        sampleBoth(pm, um, uo);
+        pt = v3(p.x + um.x * P.dt, p.y + um.y * P.dt, p.z + um.z * P.dt);
+      }
       V3<R> pc = collideP(pt);
       V3<R> dsp = pc - pt;
       if (dot(dsp, dsp) > R(1.0e-12)) {
@@ -1343,6 +1408,8 @@ template <class R> struct Sim {
     p.nb = nbOn ? 1u : 0u;
     p.band = R(std::max(2.0f, o->nb_band > 0 ? o->nb_band : 3.0f));
     p.ocean = o->ocean_on ? 1u : 0u;
+    p.rk2 = o->rk2 ? 1u : 0u;
+    p.extrap = o->extrap_layers;
     p.nwave = o->ocean_on ? std::min(o->nwave, 32u) : 0u;
     p.otime = R(o->ocean_time);
     p.olevel = o->ocean_on ? R(o->ocean_level) : R(0);
@@ -1411,7 +1478,7 @@ template <class R> struct Sim {
       case ST_STAGE_NORMFACES: if (!onlyTiles) normFaces(); break;
       case ST_STAGE_DIVERGENCE: if (!onlyTiles) divergence(); break;
       case ST_STAGE_PRESSURE: if (!onlyTiles) stagePressure(); break;
-      case ST_STAGE_PROJECT: if (!onlyTiles) { project(); if (nbOn) saveVel(); } break;
+      case ST_STAGE_PROJECT: if (!onlyTiles) { project(); extrapolate(); if (nbOn) saveVel(); } break;
       case ST_STAGE_G2P: if (!onlyTiles) stageG2P(); break;
       case ST_STAGE_SPRAY: if (!onlyTiles && sprayCap) stageSpray(); break;
       case ST_STAGE_RESEED: if (!onlyTiles && nbOn) stageReseed(); break;
@@ -1482,6 +1549,7 @@ template <class R> struct Sim {
       case 6: nbKill(); break;
       case 7: stageReseed(); break;
       case 8: stageNarrowBand(); break;
+      case 9: extrapolate(); break;
       default: return ST_ERR_ARG;
     }
     return ST_OK;

@@ -49,7 +49,7 @@ _PROTOS = {
 
 # test hook: single narrow-band kernels of the oracle (Oracle.kernel)
 KERNELS = {"phiInitAll": 0, "nbPhi": 1, "nbJacA": 2, "nbJacB": 3, "nbVel": 4, "saveVel": 5, "nbKill": 6,
-           "reseed": 7, "narrow band": 8}
+           "reseed": 7, "narrow band": 8, "extrapolate": 9}
 
 _lib = None
 
@@ -119,7 +119,8 @@ class Oracle(_Base):
         self._call("poke", self.h, DUMP[what], a.ctypes.data, a.nbytes)
 
     def kernel(self, name):
-        """test hook: run one narrow-band kernel (KERNELS) on the current state"""
+        """test hook: run one kernel (KERNELS: the narrow band's, or the opt-in
+        extrapolation with the last step's settings) on the current state"""
         self._call("kernel", self.h, KERNELS[name])
 
     def set_drips(self, samples):

@@ -80,7 +80,8 @@ class StParams(C.Structure):
                 ("air_on", C.c_uint32), ("air_couple", C.c_float), ("air_decay", C.c_float), ("air_blur", C.c_float),
                 ("drip_on", C.c_uint32), ("drip_rate", C.c_float), ("drip_dry", C.c_float),
                 ("ocean_on", C.c_uint32), ("nwave", C.c_uint32), ("ocean_time", C.c_float),
-                ("ocean_level", C.c_float), ("ocean_layer", C.c_float), ("waves", C.c_float * 256)]
+                ("ocean_level", C.c_float), ("ocean_layer", C.c_float), ("waves", C.c_float * 256),
+                ("rk2", C.c_uint32), ("extrap_layers", C.c_uint32)]
 
 
 ST_OCEAN_MAX_WAVES = 32

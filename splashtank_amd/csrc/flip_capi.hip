@@ -68,6 +68,7 @@ struct StepGraph {
   // drip samples and air cells (host flags of the step, not device state)
   bool airOn, sprOn, dripOn;
   uint32_t ndrip, airN, sprayCap;
+  uint32_t extrap;  // layers of the opt-in velocity extrapolation
   hipGraphExec_t exec;
 };
 const size_t GRAPH_CACHE = 4;
@@ -137,6 +138,9 @@ struct Sim {
   uint32_t airN = 0, aNx = 1, aNy = 1, aNz = 1;  // air field cells (0: not made yet)
   double airH = 1.0;
   bool airOnStep = false, sprOnStep = false;
+  // opt-in velocity extrapolation: layers this step, masks for exCells cells
+  uint32_t extrapOnStep = 0, exCells = 0;
+  uint8_t* exMask = nullptr;
   uint32_t ndrip = 0, dripCap = 0;  // drip samples set, room for them
   bool dripOnStep = false;
   // live view (st_pack_view): packed words, sample flags, their scan, the sampled whitewater
@@ -177,6 +181,8 @@ struct Sim {
     L.nbOn = nbOn;
     L.initStep = initStep;
     L.smax = sprayCap;
+    L.extrap = extrapOnStep;
+    L.exMask = exMask;
     return L;
   }
 };
@@ -266,6 +272,7 @@ void destroySim(Sim* s) {
   dfree(s->G.drip); dfree(s->G.dripFlag); dfree(s->G.dripOff); dfree(s->G.dripSums);
   dfree(s->viewOut); dfree(s->viewFlag); dfree(s->viewOff); dfree(s->viewSums); dfree(s->viewDisp);
   dfree(s->deepFlag); dfree(s->deepOff); dfree(s->deepSums); dfree(s->deepAll); dfree(s->deepOut);
+  dfree(s->exMask);
   freeSlots(s);
   dfree(s->dSdf);
   dfree(s->dSlots);
@@ -462,6 +469,19 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
     for (uint32_t k = 0; k < P.nwave; k++)
       for (int q = 0; q < 8; q++) P.waves[k * 2 + q / 4][q % 4] = o->waves[k * 8 + q];
   }
+  // opt-in modes (no counterpart in the reference)
+  if (o->extrap_layers > ST_EXTRAP_MAX) return fail(ST_ERR_ARG, "st_params.extrap_layers: at most ST_EXTRAP_MAX layers");
+  P.rk2 = o->rk2 ? 1u : 0u;
+  P.extrap = o->extrap_layers;
+  if (o->extrap_layers && !o->only_tiles && (!s->exMask || s->exCells < s->scap * 512u)) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->exMask);
+    s->exCells = 0;
+    int rc = dalloc(&s->exMask, size_t(s->scap) * 512u * 2u);
+    if (rc) return rc;
+    s->exCells = s->scap * 512u;
+  }
+  s->extrapOnStep = o->only_tiles ? 0u : o->extrap_layers;
   s->iters = std::max(1, int(o->iters));
   Pout = P;
   size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
@@ -603,7 +623,7 @@ int graphStep(Sim* s, const st_params* o) {
     if (std::memcmp(&g.G, &s->G, sizeof(DGrid)) != 0 || g.binned != s->binned || g.solid != s->solidDirty ||
         g.iters != s->iters || g.N != s->N || g.scap != s->scap || g.pipe != pipe || g.airOn != s->airOnStep ||
         g.sprOn != s->sprOnStep || g.dripOn != s->dripOnStep || g.ndrip != s->ndrip || g.airN != s->airN ||
-        g.sprayCap != s->sprayCap)
+        g.sprayCap != s->sprayCap || g.extrap != s->extrapOnStep)
       continue;
     HIPCHK(hipGraphLaunch(g.exec, s->st));
     // what runStage does on the host for these stages
@@ -629,6 +649,7 @@ int graphStep(Sim* s, const st_params* o) {
   e.ndrip = s->ndrip;
   e.airN = s->airN;
   e.sprayCap = s->sprayCap;
+  e.extrap = s->extrapOnStep;
   e.exec = nullptr;
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
   for (int st = 0; st < ST_STAGE_COUNT && !rc; st++) rc = runStage(s, st);
@@ -1668,6 +1689,7 @@ static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) 
     const st_params* o = prm + (nprm == 1u ? 0u : t);
     if (o->nb_on || o->ocean_on || o->only_tiles)
       return fail(ST_ERR_ARG, "st_batch_step: the narrow band, the ocean and tile-only steps are single-tank only");
+    if (o->extrap_layers) return fail(ST_ERR_ARG, "st_batch_step: velocity extrapolation is single-tank only");
     if (b->tanks[t]->sprayCap) return fail(ST_ERR_ARG, "st_batch_step: whitewater is single-tank only (a tank has a spray pool)");
     itersMax = std::max<uint32_t>(itersMax, b->tanks[t]->N ? uint32_t(std::max(1, int(o->iters))) : 0u);
   }
@@ -1997,6 +2019,8 @@ int st_adj_record(void* adj, const st_params* prm) {
   if (s->scap != a->scap) return fail(ST_ERR_STATE, "st_adj_record: the pool has grown since the context was made");
   if (prm->nb_on || prm->ocean_on || prm->only_tiles || prm->drag != 0.0f || s->open != 0u)
     return fail(ST_ERR_ARG, "st_adj_record: the adjoint covers closed boxes without drag or the narrow band");
+  if (prm->rk2 || prm->extrap_layers)
+    return fail(ST_ERR_ARG, "st_adj_record: the adjoint follows the reference's Euler step without extrapolation");
   const uint32_t ncol = s->ncTotal >= prm->nfv ? s->ncTotal - prm->nfv : 0u;
   if (ncol > ADJ_MAX_COLS)
     return fail(ST_ERR_ARG, "st_adj_record: the adjoint follows at most " + std::to_string(ADJ_MAX_COLS) + " colliders");

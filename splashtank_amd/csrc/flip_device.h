@@ -47,6 +47,7 @@ struct DParams {
   float dripRate, dripDry;
   // the ocean side boundary (wgsl-core.js:12-13, 201-226; sim.js:207-218)
   uint32_t ocean, nwave;
+  uint32_t rk2, extrap;  // opt-in modes (0: the reference's path)
   float otime, olevel, olayer, odepth, oy0;
   float oease;        // 1 - exp(-8 dt): g2p's easing toward the waves per step
   float waves[64][4]; // per wave two rows: (kx, kz, amplitude, angular frequency), (phase, wavenumber, 0, 0)

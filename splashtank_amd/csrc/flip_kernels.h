@@ -37,6 +37,8 @@ struct LaunchCtx {
   bool nbOn;          // narrow band on this step
   uint32_t initStep;  // narrow-band start: 1 restart, 2 water added, 0 none
   uint32_t smax;      // whitewater slots (host copy of P->smax)
+  uint32_t extrap;    // opt-in face velocity extrapolation: layers (host copy of P->extrap)
+  uint8_t* exMask;    // its face masks, two per pool cell (ping-pong)
 };
 
 // scratch of the narrow-band reseed: dead-slot compaction over particles and the

@@ -1541,6 +1541,77 @@ __global__ void k_project(const DParams* __restrict__ PP, DGrid G) {
   project_body(PP, G, blockIdx.x, gridDim.x);
 }
 
+// Opt-in velocity extrapolation after the projection (no counterpart in the
+// reference, which leaves faces without fluid as normFaces set them).  A mask
+// byte per cell: bit a (axis a) marks a face whose value is known -- a face with
+// fluid on one side, or one a layer has set -- and bit 3 + a a face that may be
+// set, with air on both sides.  Each layer gives every settable face not yet
+// known the mean of its known neighbours of the same axis (the faces of the six
+// cells around, within the tile grid and in tiles with storage, summed in the
+// order x-, x+, y-, y+, z-, z+), new and old values alike.  A layer reads the
+// previous layer's mask and only faces known there, which it never writes, so
+// the values are updated in place.
+__device__ __forceinline__ void extrap_init_body(const DParams* __restrict__ PP, const DGrid& G, uint8_t* __restrict__ mask,
+                                                 uint32_t bid, uint32_t nbid) {
+  const DParams& P = *PP;
+  FOR_ACTIVE(G) {
+    const uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    const int3 at = d_slotCoord(P, G.slotTile, s, local);
+    const uint32_t th = d_ctypeOf(P, G.tmap, G.ctype, at.x, at.y, at.z);
+    uint32_t m = 0u;
+    for (uint32_t axis = 0; axis < 3; axis++) {
+      int3 lo = at;
+      if (axis == 0) lo.x -= 1; else if (axis == 1) lo.y -= 1; else lo.z -= 1;
+      const uint32_t tl = d_ctypeOf(P, G.tmap, G.ctype, lo.x, lo.y, lo.z);
+      if (tl == 1u || th == 1u) m |= 1u << axis;
+      else if (tl == 0u && th == 0u) m |= 8u << axis;
+    }
+    mask[c] = uint8_t(m);
+  }
+}
+__global__ void k_extrap_init(const DParams* __restrict__ PP, DGrid G, uint8_t* __restrict__ mask) {
+  extrap_init_body(PP, G, mask, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void extrap_layer_body(const DParams* __restrict__ PP, const DGrid& G, const uint8_t* __restrict__ min,
+                                                  uint8_t* __restrict__ mout, uint32_t bid, uint32_t nbid) {
+  const DParams& P = *PP;
+  const int32_t gx = int32_t(P.tx) * 8, gy = int32_t(P.ty) * 8, gz = int32_t(P.tz) * 8;
+  FOR_ACTIVE(G) {
+    const uint32_t s = G.slotList[t >> 9], local = t & 511u, c = s * 512u + local;
+    const int3 at = d_slotCoord(P, G.slotTile, s, local);
+    const uint32_t m = min[c];
+    uint32_t mo = m;
+    for (uint32_t axis = 0; axis < 3; axis++) {
+      if (((m >> axis) & 1u) || !((m >> (3u + axis)) & 1u)) continue;
+      float sn = 0.0f, so = 0.0f;
+      uint32_t cnt = 0u;
+      for (uint32_t q = 0; q < 6u; q++) {
+        int3 r = at;
+        const int32_t d = (q & 1u) ? 1 : -1;
+        if (q < 2u) r.x += d; else if (q < 4u) r.y += d; else r.z += d;
+        if (r.x < 0 || r.y < 0 || r.z < 0 || r.x >= gx || r.y >= gy || r.z >= gz) continue;
+        const uint32_t sl = G.tmap[d_tileOf(P, r.x, r.y, r.z)];
+        if (sl == ST_NONE) continue;
+        const uint32_t cn = sl * 512u + uint32_t(r.x & 7) + 8u * (uint32_t(r.y & 7) + 8u * uint32_t(r.z & 7));
+        if (!((min[cn] >> axis) & 1u)) continue;
+        sn = sn + gvR(G, cn * 3u + axis);
+        so = so + gv0R(G, cn * 3u + axis);
+        cnt++;
+      }
+      if (cnt) {
+        const uint32_t f = c * 3u + axis;
+        G.accV[f] = (int32_t)st_f2u(sn / float(cnt));
+        G.accW[f] = (int32_t)st_f2u(so / float(cnt));
+        mo |= 1u << axis;
+      }
+    }
+    mout[c] = uint8_t(mo);
+  }
+}
+__global__ void k_extrap_layer(const DParams* __restrict__ PP, DGrid G, const uint8_t* __restrict__ min, uint8_t* __restrict__ mout) {
+  extrap_layer_body(PP, G, min, mout, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // g2p (wgsl-solver.js:185-275)
 
@@ -1775,6 +1846,16 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
     spIsLen = false;
   }
   float3 pt = f3add(p, f3mul(v, P.dt));
+  if (P.rk2 != 0u && storage) {
+    // opt-in midpoint rule through the grid's new velocity u (vn is u at p); the
+    // midpoint may lie outside the tile's window, so it is sampled from the grid
+    const float hdt = 0.5f * P.dt;
+    const float3 pm = make_float3(p.x + vn.x * hdt, p.y + vn.y * hdt, p.z + vn.z * hdt);
+    float3 um, uo;
+    GlobalGrid gm{P, G};
+    d_sampleBoth(P, gm, pm, um, uo);
+    pt = make_float3(p.x + um.x * P.dt, p.y + um.y * P.dt, p.z + um.z * P.dt);
+  }
   float3 pc = d_collideP(P, C, sdf, pt);
   float3 dsp = f3sub(pc, pt);
   if (f3dot(dsp, dsp) > 1.0e-12f) {
@@ -2494,6 +2575,13 @@ hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
 
 void launch_project(hipStream_t st, const LaunchCtx& L) {
   hipLaunchKernelGGL(k_project, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G);
+  if (L.extrap && L.exMask) {
+    const size_t C = size_t(L.scap) * 512u;
+    hipLaunchKernelGGL(k_extrap_init, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, L.exMask);
+    for (uint32_t l = 0; l < L.extrap; l++)
+      hipLaunchKernelGGL(k_extrap_layer, dim3(L.cellGrid), dim3(WG), 0, st, L.P, L.G, (const uint8_t*)(L.exMask + (l & 1u) * C),
+                         L.exMask + ((l + 1u) & 1u) * C);
+  }
   if (L.nbOn) hipLaunchKernelGGL(k_save_vel, dim3(L.cellGrid), dim3(WG), 0, st, L.G);
 }
 

@@ -255,6 +255,8 @@ void readParams(napi_env env, napi_value o, st_params* p) {
   p->ocean_time = float(numProp(env, o, "oceanTime", 0));
   p->ocean_level = float(numProp(env, o, "oceanLevel", 0));
   p->ocean_layer = float(numProp(env, o, "oceanLayer", 0));
+  p->rk2 = uint32_t(numProp(env, o, "rk2", 0));
+  p->extrap_layers = uint32_t(numProp(env, o, "extrapLayers", 0));
   has = false;
   napi_has_named_property(env, o, "waves", &has);
   if (has) {

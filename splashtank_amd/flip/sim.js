@@ -205,6 +205,10 @@ class FlipSim {
       noFoam: this.opt.noFoam, noCull: this.opt.noCull, fastAir: this.opt.fastAir, noSolidCache: this.opt.noSolidCache,
       nbOn, nbBand: Math.max(2, (nbc && nbc.band) || 3), onlyTiles: this.only === 'tiles', nbInit: init, nfill: fl.length,
       nfv: init ? this.nbFillVols.length : 0, fills, sorMode: this.opt.sorMode || 0, ...this.sprayParams(o),
+      // opt-in modes with no counterpart in the reference (off unless asked for): o.rk2 moves
+      // particles by the midpoint rule through the grid velocity, o.extrap carries face
+      // velocities that many layers into the air after the projection
+      rk2: o.rk2 ? 1 : 0, extrapLayers: Math.max(0, o.extrap | 0),
       oceanOn: oc ? 1 : 0, oceanTime: o.time || 0, oceanLevel: oc ? oc.level : 0, oceanLayer: oc ? oc.layer : 0,
       waves: oc ? this.oceanWaves(oc, o.g) : new Float32Array(0) };
   }

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 def test_params_struct_layout_matches_header():
     # sizes the header implies: 7 floats + int + 3 + 4 floats + 4 u32 + u32 + f + 4 u32 + 64 f + u32
     # + 16 whitewater words + 3 drip words + 5 ocean words and 32 waves of 8 floats
-    assert ctypes.sizeof(capi.StParams) == 4 * (7 + 1 + 3 + 4 + 4 + 1 + 1 + 1 + 3 + 64 + 1 + 16 + 3 + 5 + 256)
+    assert ctypes.sizeof(capi.StParams) == 4 * (7 + 1 + 3 + 4 + 4 + 1 + 1 + 1 + 3 + 64 + 1 + 16 + 3 + 5 + 256 + 2)
     assert ctypes.sizeof(capi.StOpts) == 8 * 5 + 4 * 4
     assert ctypes.sizeof(capi.StDims) == 4 * (10 + 6 + 5)
 
