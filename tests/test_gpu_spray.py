@@ -3,6 +3,7 @@ the oracle, through the C ABI: every whitewater slot (position, life, velocity,
 kind) and every particle bit for bit after each checked substep, with the air
 field on, from a shear pool and from a ball splashing next to a moving box
 collider."""
+import math
 import numpy as np
 import pytest
 
@@ -115,6 +116,45 @@ def test_drips_from_a_box_lifted_out_of_the_water_match_oracle():
         t += 1 / 120
         p1, p0 = path(t), path(t - 1 / 120)
         cf = sc_mod.collider_record(0, dims, lo, h, p1, vel=(p1 - p0) * 120)
+        for s in (g, o):
+            s.set_colliders(cf, data, 1)
+            s.step(prm)
+        if step % 6 == 5:
+            alive = _compare(g, o, f"step {step}")
+            print(f"[drips] step {step}: {alive} living whitewater points", flush=True)
+    assert alive > 20
+    g.close()
+
+
+def test_drips_from_a_tilted_spinning_box_match_oracle():
+    """The drips of a collider turned off the axes and spinning as it rises out
+    of the pool: the drip samples' world positions and surface velocities go
+    through the collider's rotation and its spin term (d_dripGeom, colSurfVel)."""
+    from splashtank_amd import scenes as sc_mod
+    d = 1 / 64
+    pos, vel = lattice(d, (-0.5, 0.0, -0.5), (0.5, 0.25, 0.5))
+    opts = capi.make_opts(d, (1, 1, 1), max_n=len(pos))
+    g, o = FlipSimC(opts), Oracle(opts)
+    h = 2 * d
+    dims, lo, data = sc_mod.box_sdf((0.14, 0.05, 0.09), h, 4 * h)
+    samples = sc_mod.drip_samples(dict(id=1, dims=dims, origin=lo, cell=h, data=data), h, 0)
+    ax = np.array([0.3, 1.0, 0.2]) / np.linalg.norm([0.3, 1.0, 0.2])
+    spin = 2.5
+    R0 = sc_mod.axis_rotation((0, 0, 1), math.radians(25))
+    path = lambda t: np.array([0.05, 0.14 + 1.1 * t, -0.03])  # noqa: E731
+    for s in (g, o):
+        s.upload(pos, vel)
+        s.set_spray_cap(1 << 15)
+        s.set_drips(samples)
+    prm = capi.make_params(dt=1 / 120, g=9.81, iters=15, **dict(SPR, spr_rate=0.0, drip_on=1, drip_rate=40.0, drip_dry=1.5))
+    print(f"[drips] tilted spinning box from a pool, {len(samples)} drip samples", flush=True)
+    t = 0.0
+    alive = 0
+    for step in range(36):
+        t += 1 / 120
+        p1, p0 = path(t), path(t - 1 / 120)
+        R = sc_mod.axis_rotation(ax, spin * t) @ R0
+        cf = sc_mod.collider_record(0, dims, lo, h, p1, R=R, vel=(p1 - p0) * 120, w=tuple(spin * ax))
         for s in (g, o):
             s.set_colliders(cf, data, 1)
             s.step(prm)
