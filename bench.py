@@ -155,9 +155,10 @@ def run_batched(args):
     bat = time.perf_counter() - t0
     # algorithmic bytes of one substep summed over the tanks (the model of the
     # single-tank line, each tank with its own blocks, fluid cells and tiles)
-    byts = 0
+    byts = act = 0
     for sc, s in zip(tanks, sims):
         ctl = s.ctl()
+        act += int(ctl[4])
         n = int(round(1.0 / (sc.d * sc.ratio)))
         T = ((n + 1 + 7) // 8) ** 3
         byts += 112 * sc.n + 224 * int(ctl[4]) * 512 + (20 + 16 * int(sc.params["iters"])) * int(ctl[0] + ctl[1]) + 16 * T
@@ -185,8 +186,12 @@ def run_batched(args):
     batch.step_n(prm, K, graph=False)
     batch.sync()
     tun = time.perf_counter() - t0
+    # the graphs' warm-up: the first substep after a restore and the rest are
+    # captured on their first use, and the first replays of a new graph run
+    # slower (measured 7.3 against 5.7 ms over 20 substeps after 2 warm-up ones),
+    # so the graph leg warms up over as many substeps as it times
     restore()
-    batch.step_n(prm, 2, graph=True)  # captures the graphs (first substep after a restore, and the rest)
+    batch.step_n(prm, K, graph=True)
     batch.sync()
     restore()
     t0 = time.perf_counter()
@@ -199,7 +204,7 @@ def run_batched(args):
     for s in sims:
         s.close()
     return {"workload": f"{len(tanks)}_tanks_32-128^3", "tanks": len(tanks), "particles": ntot,
-            "sor_iters": int(tanks[0].params["iters"]), "steps": K, "warmup": W,
+            "sor_iters": int(tanks[0].params["iters"]), "steps": K, "warmup": W, "active_tiles": act,
             "value": ntot * K / bat, "unit": "particle-substeps/s", "ms_per_step": bat * 1e3 / K,
             "sequential_value": ntot * K / seq, "sequential_ms_per_step": seq * 1e3 / K,
             "speedup_vs_sequential": seq / bat,

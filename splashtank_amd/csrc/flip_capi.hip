@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <atomic>
@@ -357,6 +358,22 @@ void nearBox(const float* r, float lo, float hq, float4& wl, float4& wh) {
 }
 
 // the step's parameter block and collider records (sim.js:179-231) into P and C
+// The largest f32 T at or below (c dt sens)^2 (1 - 1e-5), or -1 when dt or
+// sens is not a positive finite number.  For a squared length s <= T the
+// quotient g2p clamps, RN(RN(RN(sqrt(s)) / dt) / sens), is at most
+// c sqrt(1 - 1e-5) (1 + 2^-24)^3 < c before its last rounding, hence at most c
+// (c is an f32) after it; g2p's impact (c = 15) and turb (c = 0.35f, dt = 1)
+// are then 0.
+static float quietBelow(double c, float dt, float sens) {
+  if (!(dt > 0.0f) || !(sens > 0.0f) || !std::isfinite(dt) || !std::isfinite(sens)) return -1.0f;
+  const double r = c * double(dt) * double(sens);
+  double t = r * r * (1.0 - 1e-5);
+  if (!(t < double(FLT_MAX))) t = double(FLT_MAX);
+  float f = float(t);
+  if (double(f) > t) f = std::nextafter(f, 0.0f);
+  return f;
+}
+
 int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   if (s->N == 0) return fail(ST_ERR_STATE, "st_begin_step: no particles (step() does nothing when N = 0)");
   if (o->ocean_on && o->nwave > ST_OCEAN_MAX_WAVES) return fail(ST_ERR_ARG, "st_begin_step: nwave above ST_OCEAN_MAX_WAVES");
@@ -417,6 +434,11 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   P.nfv = s->initStep ? o->nfv : 0u;
   P.ntiles = s->ntiles;
   P.settle = st_smoothstepf(0.1f, 0.5f, P.q8);
+  {
+    const float sens = std::fmax(P.q6, 0.05f);
+    P.impS2 = quietBelow(15.0, P.dt, sens);
+    P.turbS2 = quietBelow(double(0.35f), 1.0f, sens);
+  }
   {  // whitewater, air field (sim.js:203-206, 221-230)
     const bool on = o->spr_on != 0 && s->sprayCap > 0, haveSp = o->spr_on != 0;
     P.sprRate = on ? o->spr_rate : 0.0f;
@@ -850,6 +872,10 @@ int st_create(const st_opts* o, void** out) {
   s->nx = std::max(4, (int)std::floor(sz[0] / s->hd + 0.5));
   s->ny = std::max(4, (int)std::floor(sz[1] / s->hd + 0.5));
   s->nz = std::max(4, (int)std::floor(sz[2] / s->hd + 0.5));
+  if (double(s->nx) * s->ny * s->nz >= 4294967296.0 || uint64_t((s->nx + 8) / 8) * uint64_t((s->ny + 8) / 8) * uint64_t((s->nz + 8) / 8) >= (1ull << 24)) {
+    delete s;
+    return fail(ST_ERR_ARG, "st_create: grid too large (cells below 2^32 and 8^3 tiles below 2^24)");
+  }
   double W = s->nx * s->hd, H = s->ny * s->hd, D = s->nz * s->hd;
   s->x0 = float(-W / 2); s->x1 = float(W / 2); s->y0 = 0.0f; s->y1 = float(H); s->z0 = float(-D / 2); s->z1 = float(D / 2);
   s->open = o->open;

@@ -50,6 +50,9 @@ struct DParams {
   uint32_t rk2, extrap;  // opt-in modes (0: the reference's path)
   float otime, olevel, olayer, odepth, oy0;
   float oease;        // 1 - exp(-8 dt): g2p's easing toward the waves per step
+  // squared lengths of vn - vo (v0 - vo) at or below which g2p's impact (turb)
+  // quotient cannot exceed 15 (0.35), so both are 0 there
+  float impS2, turbS2;
   float waves[64][4]; // per wave two rows: (kx, kz, amplitude, angular frequency), (phase, wavenumber, 0, 0)
 };
 
@@ -138,7 +141,8 @@ struct DGrid {
 // sparse addressing (wgsl-core.js:76-112)
 
 __device__ __forceinline__ uint32_t d_tileOf(const DParams& P, int32_t i, int32_t j, int32_t k) {
-  return uint32_t(i >> 3) + P.tx * (uint32_t(j >> 3) + P.ty * uint32_t(k >> 3));
+  // every product stays below the tile count, which st_create keeps below 2^24
+  return uint32_t(i >> 3) + __umul24(P.tx, uint32_t(j >> 3) + __umul24(P.ty, uint32_t(k >> 3)));
 }
 __device__ __forceinline__ uint32_t d_cellIdx(const DParams& P, const uint32_t* __restrict__ tmap, int32_t i, int32_t j, int32_t k) {
   uint32_t s = tmap[d_tileOf(P, i, j, k)];

@@ -466,7 +466,7 @@ __device__ __forceinline__ uint32_t wcell(const uint32_t* wslots, int32_t wx, in
 // window cell index wx + WIN (wy + WIN wz) of offsets in [0, WIN): 24-bit
 // multiplies (full rate; a 32-bit multiply is a quarter-rate instruction)
 __device__ __forceinline__ uint32_t win_idx(int32_t wx, int32_t wy, int32_t wz) {
-  return uint32_t(wx) + __umul24(uint32_t(wy) + __umul24(uint32_t(wz), uint32_t(WIN)), uint32_t(WIN));
+  return uint32_t(wx) + __umul24(uint32_t(wy), uint32_t(WIN)) + __umul24(uint32_t(wz), uint32_t(WIN * WIN));
 }
 // window offset, clamped into the window: a guard for LDS only -- every face or
 // cell a particle of the tile touches already lies inside (faceIndex's clamp moves
@@ -1686,7 +1686,7 @@ struct WindowGrid {
   __device__ void sample(float3 p, float3& vn, float3& vo) const { d_sampleBothWin(P, *this, p, vn, vo); }
   // cells of the density sum come clamped to the box, so they lie in the window
   __device__ int32_t cnt(int32_t i, int32_t j, int32_t k) const {
-    return sC[wclamp(i - org.x) + WIN * (wclamp(j - org.y) + WIN * wclamp(k - org.z))];
+    return sC[win_idx(wclamp(i - org.x), wclamp(j - org.y), wclamp(k - org.z))];
   }
   // GlobalGrid::dsum from the table sS.  A base below -1 (above n - 1) clamps its
   // two cells to the ones of base -1 (n - 1), so the base is clamped to
@@ -1697,33 +1697,45 @@ struct WindowGrid {
     const int32_t wx = min(max(d_iclamp(bc.x, -1, int32_t(P.nx) - 1) - org.x, 0), DSUM_N - 1);
     const int32_t wy = min(max(d_iclamp(bc.y, -1, int32_t(P.ny) - 1) - org.y, 0), DSUM_N - 1);
     const int32_t wz = min(max(d_iclamp(bc.z, -1, int32_t(P.nz) - 1) - org.z, 0), DSUM_N - 1);
-    return float(sS[wx + DSUM_N * (wy + DSUM_N * wz)]);
+    return float(sS[uint32_t(wx) + __umul24(uint32_t(wy), uint32_t(DSUM_N)) + __umul24(uint32_t(wz), uint32_t(DSUM_N * DSUM_N))]);
   }
 };
 
-// sampleBoth (wgsl-core.js:138-154) from a staged window: the two clamped window
-// offsets per axis give the eight face indices of each trilinear stencil
+// sampleBoth (wgsl-core.js:138-154) from a staged window.  Per dimension the
+// three stencils use two coordinates, the unshifted one (a face's own axis) and
+// the one shifted by half a cell (the other two axes), so the floor, fraction
+// and the two clamped window offsets are worked out once for each of the six;
+// each stencil's arithmetic is the one of d_sampleBoth, term for term.
+struct WinAxis {
+  float f, g;    // fraction and 1 - fraction
+  int32_t w[2];  // the two clamped window offsets
+};
+__device__ __forceinline__ WinAxis win_axis(float q, int32_t hi, int32_t org) {
+  WinAxis a;
+  const float f0 = floorf(q);
+  a.f = q - f0;
+  a.g = 1.0f - a.f;
+  wpair(st_f2i_sat(f0), hi, org, a.w);
+  return a;
+}
 __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGrid& g, float3 p, float3& vn, float3& vo) {
-  const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
+  const float ux = (p.x - P.x0) * P.ih, uy = (p.y - P.y0) * P.ih, uz = (p.z - P.z0) * P.ih;
+  const WinAxis xu = win_axis(ux, int32_t(P.nx), g.org.x), xs = win_axis(ux - 0.5f, int32_t(P.nx) - 1, g.org.x);
+  const WinAxis yu = win_axis(uy, int32_t(P.ny), g.org.y), ys = win_axis(uy - 0.5f, int32_t(P.ny) - 1, g.org.y);
+  const WinAxis zu = win_axis(uz, int32_t(P.nz), g.org.z), zs = win_axis(uz - 0.5f, int32_t(P.nz) - 1, g.org.z);
   float rn[3], ro[3];
+#pragma unroll
   for (uint32_t axis = 0; axis < 3; axis++) {
-    float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
-    if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
-    else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
-    else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
-    float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
-    float3 fr = f3sub(q, f0);
-    int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
-    int32_t wx[2], wy[2], wz[2];
-    wpair(i0.x, axis == 0 ? X : X - 1, g.org.x, wx);
-    wpair(i0.y, axis == 1 ? Y : Y - 1, g.org.y, wy);
-    wpair(i0.z, axis == 2 ? Z : Z - 1, g.org.z, wz);
+    const WinAxis& ax = axis == 0 ? xu : xs;
+    const WinAxis& ay = axis == 1 ? yu : ys;
+    const WinAxis& az = axis == 2 ? zu : zs;
+    const float2* __restrict__ s = g.sNO[axis];
     float sn = 0.0f, so = 0.0f;
+#pragma unroll
     for (uint32_t c = 0; c < 8; c++) {
-      int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
-      float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
-      const uint32_t wi = win_idx(wx[dx], wy[dy], wz[dz]);
-      float2 e = g.sNO[axis][wi];
+      const uint32_t dx = c & 1u, dy = (c >> 1) & 1u, dz = (c >> 2) & 1u;
+      const float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
+      const float2 e = s[win_idx(ax.w[dx], ay.w[dy], az.w[dz])];
       sn = sn + w * e.x;
       so = so + w * e.y;
     }
@@ -1819,13 +1831,21 @@ __device__ __forceinline__ void g2p_particle(const DParams& P, const DCols* __re
   // air, shield, foam decay and drag factor depend on the density sum alone
   const float4 ft = (P.q9 < 0.5f && csum >= 0.0f && csum < float(FOAM_TAB)) ? G.foamTab[int32_t(csum)] : d_foamRow(P, csum);
   const float air = ft.x, shield = ft.y;
-  float spd0 = f3len(v);
   float sens = fmaxf(P.q6, 0.05f);
-  float pacc = f3len(f3sub(vn, vo)) / P.dt;
   const float settle = P.settle;
-  float impact = st_clampf((pacc / sens - 15.0f) / 80.0f, 0.0f, 1.0f) * settle;
-  float turb = st_clampf((f3len(f3sub(v0, vo)) / sens - 0.35f) / 1.5f, 0.0f, 1.0f) * settle;
-  float spray = air * (0.25f + 0.75f * st_clampf(spd0 / 4.0f, 0.0f, 1.0f));
+  // impact and turb are +0 wherever the quotient they clamp is not above 15
+  // (0.35); below the squared lengths impS2 (turbS2) it cannot be, so the
+  // roots and divisions are skipped there with the same result
+  float impact = 0.0f, turb = 0.0f;
+  const float3 dvp = f3sub(vn, vo), dvt = f3sub(v0, vo);
+  if (f3dot(dvp, dvp) > P.impS2) {
+    float pacc = f3len(dvp) / P.dt;
+    impact = st_clampf((pacc / sens - 15.0f) / 80.0f, 0.0f, 1.0f) * settle;
+  }
+  if (f3dot(dvt, dvt) > P.turbS2) turb = st_clampf((f3len(dvt) / sens - 0.35f) / 1.5f, 0.0f, 1.0f) * settle;
+  // +0 where air is +0 (the factor it scales is finite), without the root
+  float spray = 0.0f;
+  if (air != 0.0f) spray = air * (0.25f + 0.75f * st_clampf(f3len(v) / 4.0f, 0.0f, 1.0f));
   float foam = v04.w * ft.z;
   foam = fmaxf(foam, fmaxf(spray * 0.9f, fmaxf(impact, turb * 0.8f)));
   if (P.ocean != 0u) foam = foam * (1.0f - st_clampf(lw * 2.0f, 0.0f, 1.0f));  // easing is not churning (wgsl-solver.js:248)

@@ -44,7 +44,20 @@ def _compare(g, o, stage, step, log):
             log.append((tag, r))
     ca, cb = g.ctl(), o.ctl()
     for k in CTL_AFTER.get(stage, []):
+        if k == 6 and _nan_bits(ca[k]) and _nan_bits(cb[k]):
+            # ctl[6] is the largest f32 bit pattern of the particle speeds.  In an
+            # overflowed pool the stencils of p2g reach into tiles without
+            # storage, whose faces the spare slot holds as fixed-point sums that
+            # later stages read as f32 (as the reference does), so NaN can reach
+            # the speeds; which NaN (sign, payload) an operation on two NaN
+            # operands returns is left open by IEEE 754, so any two NaN patterns
+            # count as equal, as they do in the f32 comparisons
+            continue
         assert ca[k] == cb[k], f"step {step} after {capi.STAGES[stage]}: ctl[{k}] {ca[k]} vs {cb[k]}"
+
+
+def _nan_bits(u):
+    return (int(u) & 0x7F800000) == 0x7F800000 and (int(u) & 0x007FFFFF) != 0
 
 
 def run_stagewise(sc, nsteps, check_every_stage=True, sims=None, t0=0.0, **optkw):
