@@ -478,6 +478,41 @@ __device__ __forceinline__ void wpair(int32_t i0, int32_t hi, int32_t org, int32
   w[0] = wclamp(d_iclamp(i0, 0, hi) - org);
   w[1] = wclamp(d_iclamp(i0 + 1, 0, hi) - org);
 }
+// One dimension of a trilinear stencil over a window: the fraction of the
+// coordinate q, its complement and the two clamped window offsets.  Per
+// dimension the three face stencils of a point use two coordinates, the
+// unshifted one (a face's own axis, clamped to [0, n]) and the one shifted by
+// half a cell (the other two axes, clamped to [0, n-1]), so p2g and g2p work
+// these out once for each of the six and pick them per axis.
+struct WinAxis {
+  float f, g;    // fraction and 1 - fraction
+  int32_t w[2];  // the two clamped window offsets
+};
+__device__ __forceinline__ WinAxis win_axis(float q, int32_t hi, int32_t org) {
+  WinAxis a;
+  const float f0 = floorf(q);
+  a.f = q - f0;
+  a.g = 1.0f - a.f;
+  wpair(st_f2i_sat(f0), hi, org, a.w);
+  return a;
+}
+struct WinAxes {
+  WinAxis xu, xs, yu, ys, zu, zs;  // unshifted and shifted, per dimension
+  __device__ const WinAxis& x(uint32_t axis) const { return axis == 0 ? xu : xs; }
+  __device__ const WinAxis& y(uint32_t axis) const { return axis == 1 ? yu : ys; }
+  __device__ const WinAxis& z(uint32_t axis) const { return axis == 2 ? zu : zs; }
+};
+__device__ __forceinline__ WinAxes win_axes(const DParams& P, float3 p, int3 org) {
+  const float ux = (p.x - P.x0) * P.ih, uy = (p.y - P.y0) * P.ih, uz = (p.z - P.z0) * P.ih;
+  WinAxes a;
+  a.xu = win_axis(ux, int32_t(P.nx), org.x);
+  a.xs = win_axis(ux - 0.5f, int32_t(P.nx) - 1, org.x);
+  a.yu = win_axis(uy, int32_t(P.ny), org.y);
+  a.ys = win_axis(uy - 0.5f, int32_t(P.ny) - 1, org.y);
+  a.zu = win_axis(uz, int32_t(P.nz), org.z);
+  a.zs = win_axis(uz - 0.5f, int32_t(P.nz) - 1, org.z);
+  return a;
+}
 // face (axis, i, j, k) clamped as faceIndex does (wgsl-core.js:84-89), as a window index or -1
 __device__ __forceinline__ int32_t wface(const DParams& P, uint32_t axis, int32_t i, int32_t j, int32_t k, int3 org) {
   int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
@@ -508,7 +543,6 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) sC[q] = 0;
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
-    const int32_t X = int32_t(P.nx), Y = int32_t(P.ny), Z = int32_t(P.nz);
     const bool wscaleSmall = P.scale <= 2147483520.0f;
     // the four waves take every fourth entry of each run of WG bin entries:
     // neighbouring entries tend to be neighbouring particles (last substep's
@@ -524,30 +558,25 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
       float3 v = make_float3(v4.x, v4.y, v4.z);
       int3 cc = d_particleCell(P, p);
       vmaxBits = max(vmaxBits, st_f2u(f3len(v)));
+      const WinAxes wa = win_axes(P, p, org);
+#pragma unroll
       for (uint32_t axis = 0; axis < 3; axis++) {
-        float3 q = make_float3((p.x - P.x0) * P.ih, (p.y - P.y0) * P.ih, (p.z - P.z0) * P.ih);
-        if (axis == 0) { q.y = q.y - 0.5f; q.z = q.z - 0.5f; }
-        else if (axis == 1) { q.x = q.x - 0.5f; q.z = q.z - 0.5f; }
-        else { q.x = q.x - 0.5f; q.y = q.y - 0.5f; }
-        float3 f0 = make_float3(floorf(q.x), floorf(q.y), floorf(q.z));
-        float3 fr = f3sub(q, f0);
-        int3 i0 = make_int3(st_f2i_sat(f0.x), st_f2i_sat(f0.y), st_f2i_sat(f0.z));
+        const WinAxis& ax = wa.x(axis);
+        const WinAxis& ay = wa.y(axis);
+        const WinAxis& az = wa.z(axis);
         float vc = f3get(v, axis);
-        int32_t wx[2], wy[2], wz[2];
-        wpair(i0.x, axis == 0 ? X : X - 1, org.x, wx);
-        wpair(i0.y, axis == 1 ? Y : Y - 1, org.y, wy);
-        wpair(i0.z, axis == 2 ? Z : Z - 1, org.z, wz);
         // every corner is added: a zero weight adds round(0) = 0 to an integer sum,
         // exactly as skipping it (wgsl-solver.js:33)
+#pragma unroll
         for (uint32_t c = 0; c < 8; c++) {
-          int32_t dx = int32_t(c & 1u), dy = int32_t((c >> 1) & 1u), dz = int32_t((c >> 2) & 1u);
-          float w = (dx == 1 ? fr.x : 1.0f - fr.x) * (dy == 1 ? fr.y : 1.0f - fr.y) * (dz == 1 ? fr.z : 1.0f - fr.z);
+          const uint32_t dx = c & 1u, dy = (c >> 1) & 1u, dz = (c >> 2) & 1u;
+          float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
           int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
           // w is in [0, 1] (NaN for a NaN position), so w * scale <= scale and
           // the saturating clamp of st_f2i_sat never acts when the scale is
           // below it (wscaleSmall); the conversion alone gives the same bits
           int32_t aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
-          const uint32_t wi = win_idx(wx[dx], wy[dy], wz[dz]);
+          const uint32_t wi = win_idx(ax.w[dx], ay.w[dy], az.w[dz]);
           atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
         }
       }
@@ -1701,34 +1730,17 @@ struct WindowGrid {
   }
 };
 
-// sampleBoth (wgsl-core.js:138-154) from a staged window.  Per dimension the
-// three stencils use two coordinates, the unshifted one (a face's own axis) and
-// the one shifted by half a cell (the other two axes), so the floor, fraction
-// and the two clamped window offsets are worked out once for each of the six;
-// each stencil's arithmetic is the one of d_sampleBoth, term for term.
-struct WinAxis {
-  float f, g;    // fraction and 1 - fraction
-  int32_t w[2];  // the two clamped window offsets
-};
-__device__ __forceinline__ WinAxis win_axis(float q, int32_t hi, int32_t org) {
-  WinAxis a;
-  const float f0 = floorf(q);
-  a.f = q - f0;
-  a.g = 1.0f - a.f;
-  wpair(st_f2i_sat(f0), hi, org, a.w);
-  return a;
-}
+// sampleBoth (wgsl-core.js:138-154) from a staged window, with the six
+// per-dimension terms of win_axes; each stencil's arithmetic is the one of
+// d_sampleBoth, term for term.
 __device__ __forceinline__ void d_sampleBothWin(const DParams& P, const WindowGrid& g, float3 p, float3& vn, float3& vo) {
-  const float ux = (p.x - P.x0) * P.ih, uy = (p.y - P.y0) * P.ih, uz = (p.z - P.z0) * P.ih;
-  const WinAxis xu = win_axis(ux, int32_t(P.nx), g.org.x), xs = win_axis(ux - 0.5f, int32_t(P.nx) - 1, g.org.x);
-  const WinAxis yu = win_axis(uy, int32_t(P.ny), g.org.y), ys = win_axis(uy - 0.5f, int32_t(P.ny) - 1, g.org.y);
-  const WinAxis zu = win_axis(uz, int32_t(P.nz), g.org.z), zs = win_axis(uz - 0.5f, int32_t(P.nz) - 1, g.org.z);
+  const WinAxes wa = win_axes(P, p, g.org);
   float rn[3], ro[3];
 #pragma unroll
   for (uint32_t axis = 0; axis < 3; axis++) {
-    const WinAxis& ax = axis == 0 ? xu : xs;
-    const WinAxis& ay = axis == 1 ? yu : ys;
-    const WinAxis& az = axis == 2 ? zu : zs;
+    const WinAxis& ax = wa.x(axis);
+    const WinAxis& ay = wa.y(axis);
+    const WinAxis& az = wa.z(axis);
     const float2* __restrict__ s = g.sNO[axis];
     float sn = 0.0f, so = 0.0f;
 #pragma unroll
