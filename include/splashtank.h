@@ -486,6 +486,19 @@ int st_adj_grad(void* adj, float* g_pos4, float* g_vel4);
  * velocity (shared.js:531-537). */
 #define ST_ADJ_MAX_COLS 4
 int st_adj_col_grad(void* adj, float* g, uint32_t ncols);
+/* The transposed solve of st_adj_backward runs as one launch per sweep (the
+ * default), or, after st_adj_set_sor_path(adj, 1), as one cooperative launch
+ * with a grid barrier between its sweeps where the device allows it (the same
+ * sweeps in the same order; (adj, 0) goes back).  On MI355X the one launch is
+ * the slower: each barrier has to write back and invalidate the L2 of every
+ * XCD, which costs more than a launch boundary.  st_adj_sor_path reports
+ * whether the last st_adj_backward ran as one launch.  A barrier time-out (a
+ * workgroup that never arrived) is reported as ST_ERR_HIP by the next
+ * st_adj_grad or st_adj_col_grad, whose gradients are then not valid; the
+ * context falls back to one launch per sweep.  No counterpart in the
+ * reference (it has no adjoint). */
+int st_adj_set_sor_path(void* adj, uint32_t one_launch);
+int st_adj_sor_path(void* adj, uint32_t* one_launch);
 
 #ifdef __cplusplus
 }

@@ -263,6 +263,8 @@ PROTOS = {
     "st_adj_backward": (C.c_int, [C.c_void_p]),
     "st_adj_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "st_adj_col_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "st_adj_set_sor_path": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_adj_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
 _lib = None
@@ -669,6 +671,17 @@ class AdjointC:
         g = np.zeros((max(int(ncols), 1), 9), np.float32)
         self._call("col_grad", g.ctypes.data, int(ncols))
         return g[: int(ncols)]
+
+    def set_sor_path(self, one_launch):
+        """True: the transposed solve in one cooperative launch where the
+        device allows it; False (the default): one launch per sweep."""
+        self._call("set_sor_path", 1 if one_launch else 0)
+
+    def sor_path(self):
+        """Whether the last backward() ran its transposed solve as one launch."""
+        v = C.c_uint32()
+        self._call("sor_path", C.byref(v))
+        return bool(v.value)
 
     def close(self):
         if getattr(self, "h", None):

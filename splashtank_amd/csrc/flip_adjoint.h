@@ -48,6 +48,7 @@ struct AdjGrad {
   float* sorCoef;   // per cell, 6 per cell: omega / (non-solid neighbours of that neighbour)
   float* gCol;      // per collider of the substep reversed last, 9 words: dL/d(centre),
                     // dL/d(linear velocity), dL/d(angular velocity)
+  uint32_t* bar;    // ADJ_BAR_WORDS: the one-launch solve's grid barrier (flags, time-outs, release)
   uint32_t ncell; // scap * 512
   uint32_t ncarry;
 };
@@ -55,7 +56,23 @@ struct AdjGrad {
 // colliders the adjoint follows (pushes of one particle are kept in registers)
 #define ADJ_MAX_COLS 4
 
+// workgroups of the one-launch transposed solve: at most ADJ_SOR_MAX_WG, so
+// that workgroup 0 waits for each barrier flag with one thread; the barrier
+// words are the flags, then the time-out count (ADJ_BAR_FAILS) and the release
+// word (ADJ_BAR_RELEASE)
+#define ADJ_SOR_MAX_WG 256u
+#define ADJ_BAR_FAILS (ADJ_SOR_MAX_WG + 1u)
+#define ADJ_BAR_RELEASE (ADJ_SOR_MAX_WG + 2u)
+#define ADJ_BAR_WORDS (ADJ_SOR_MAX_WG + 4u)
+
+// resident workgroups of the one-launch transposed solve on the current device
+// (0 when cooperative launches are not available)
+uint32_t adj_sor_coop_grid(uint32_t cus);
+
 // the backward pass of one recorded substep: reads G.gPos/gVel (gradient of the
 // state after it), writes G.gPos2/gVel2 (gradient of the state before it) and the
-// pressure carry for the substep before
-void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles);
+// pressure carry for the substep before.  With coopGrid > 0 the transposed
+// solve runs as one cooperative launch of that many workgroups (the same
+// arithmetic as one launch per sweep, which runs when coopGrid is 0 or the
+// cooperative launch is refused); returns whether the one launch ran.
+bool launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles, uint32_t coopGrid);

@@ -28,6 +28,9 @@
 // References: p2g wgsl-solver.js:16-40, normFaces 75-105, divergence 106-123,
 // relax 124-140, project 165-180, g2p 208-275, collideP 185-197.
 #include "flip_adjoint.h"
+#include "grid_barrier.h"
+
+#include <algorithm>
 
 #define AWG 256
 
@@ -403,9 +406,8 @@ __global__ void __launch_bounds__(AWG) k_adj_sor_prep(AdjTape T, AdjGrad G) {
 
 // one colour of one red-black iteration, transposed: gin -> gout.  A cell's
 // colour is that of its place in the tile (tile origins are even).
-__global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const float* __restrict__ gin, float* __restrict__ gout,
-                                                 uint32_t colour) {
-  const float w = T.P->omega;
+__device__ __forceinline__ void adj_sor_body(const AdjTape& T, const AdjGrad& G, const float* __restrict__ gin,
+                                             float* __restrict__ gout, uint32_t colour, float w) {
   FOR_CELLS(T) {
     const uint32_t s = T.slotList[t >> 9], l = t & 511u, c = s * 512u + l;
     const uint32_t st = T.sten[c];
@@ -431,6 +433,48 @@ __global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const flo
       gout[c] = g;
     }
   }
+}
+__global__ void __launch_bounds__(AWG) k_adj_sor(AdjTape T, AdjGrad G, const float* __restrict__ gin, float* __restrict__ gout,
+                                                 uint32_t colour) {
+  adj_sor_body(T, G, gin, gout, colour, T.P->omega);
+}
+
+// The whole transposed solve in one cooperative launch: the sweeps of
+// k_adj_sor in the same order (per iteration black then red, ping-pong from
+// gPres), a grid barrier between consecutive sweeps (the gathers of one read
+// what the last wrote in other workgroups).  2 iters sweeps leave the result in
+// gPres, as the launches per sweep do.  A barrier time-out (counted in
+// bar[ADJ_BAR_FAILS]) ends the solve early; the host reports it.
+__global__ void __launch_bounds__(AWG) k_adj_sor_all(AdjTape T, AdjGrad G) {
+  __shared__ uint32_t sFail;
+  if (threadIdx.x == 0) sFail = 0u;
+  __syncthreads();
+  const float w = T.P->omega;
+  float* a = G.gPres;
+  float* b = G.gPres2;
+  uint32_t epoch = 0;
+  for (uint32_t it = 0; it < T.iters; it++)
+    for (int colour = 1; colour >= 0; colour--) {
+      adj_sor_body(T, G, a, b, uint32_t(colour), w);
+      if (it + 1u < T.iters || colour == 1)
+        if (!gb_barrier(G.bar, G.bar + ADJ_BAR_RELEASE, G.bar + ADJ_BAR_FAILS, ++epoch, &sFail)) return;
+      float* t = a;
+      a = b;
+      b = t;
+    }
+}
+
+__global__ void k_adj_clear_bar(uint32_t* __restrict__ bar) {
+  for (uint32_t q = threadIdx.x; q < ADJ_BAR_WORDS; q += blockDim.x)
+    if (q != ADJ_BAR_FAILS) bar[q] = 0u;
+}
+
+uint32_t adj_sor_coop_grid(uint32_t cus) {
+  int dev = 0, coop = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
+    return 0u;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj_sor_all, AWG, 0) != hipSuccess || nb < 1) return 0u;
+  return std::min<uint32_t>(cus * uint32_t(nb), ADJ_SOR_MAX_WG);
 }
 
 // the gradient left on the warm start of this substep's solve, by tile of the
@@ -541,7 +585,7 @@ __global__ void k_adj_zero(float* __restrict__ p, size_t n) {
 
 static uint32_t grid_for(size_t n) { return uint32_t(std::min<size_t>((n + AWG - 1) / AWG, 8192)); }
 
-void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles) {
+bool launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uint32_t scap, uint32_t ntiles, uint32_t coopGrid) {
   const size_t C = size_t(scap) * 512u;
   const uint32_t cg = std::max<uint32_t>(1u, std::min<uint32_t>(scap * 2u, 4096u));
   const uint32_t pg = uint32_t((T.n + AWG - 1) / AWG);
@@ -555,7 +599,16 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   float* a = G.gPres;
   float* b = G.gPres2;
   hipLaunchKernelGGL(k_adj_sor_prep, dim3(cg), dim3(AWG), 0, st, T, G);
-  for (uint32_t it = 0; it < T.iters; it++)
+  bool one = false;
+  if (coopGrid && T.iters) {
+    hipLaunchKernelGGL(k_adj_clear_bar, dim3(1), dim3(AWG), 0, st, G.bar);
+    AdjTape t = T;
+    AdjGrad g = G;
+    void* args[] = {(void*)&t, (void*)&g};
+    one = hipLaunchCooperativeKernel((const void*)k_adj_sor_all, dim3(std::min(coopGrid, ADJ_SOR_MAX_WG)), dim3(AWG), args, 0, st) == hipSuccess;
+    if (!one) (void)hipGetLastError();  // the launches per sweep below do the solve
+  }
+  for (uint32_t it = 0; !one && it < T.iters; it++)
     for (int colour = 1; colour >= 0; colour--) {
       hipLaunchKernelGGL(k_adj_sor, dim3(cg), dim3(AWG), 0, st, T, G, (const float*)a, b, uint32_t(colour));
       std::swap(a, b);
@@ -565,4 +618,5 @@ void launch_adjoint_step(hipStream_t st, const AdjTape& T, const AdjGrad& G, uin
   hipLaunchKernelGGL(k_adj_carry, dim3(cg), dim3(AWG), 0, st, T, G, (const float*)a);
   hipLaunchKernelGGL(k_adj_faces, dim3(cg), dim3(AWG), 0, st, T, G);
   if (pg) hipLaunchKernelGGL(k_adj_p2g, dim3(pg), dim3(AWG), 0, st, T, G);
+  return one;
 }

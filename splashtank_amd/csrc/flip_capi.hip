@@ -791,6 +791,9 @@ struct Adj {
   std::vector<const float*> sdfAt;
   std::vector<uint32_t> ncols;    // per tape: colliders of its substep
   uint32_t lastCols = 0;          // colliders of the substep reversed last
+  uint32_t coopGrid = 0;          // workgroups of the one-launch transposed solve (0: one launch per sweep)
+  bool oneLaunch = false;         // st_adj_set_sor_path: use the one launch when coopGrid > 0
+  bool lastOne = false;           // the last reversed substep's solve ran as one launch
   AdjGrad G{};
 };
 void freeTape(AdjTape& T) {
@@ -801,7 +804,7 @@ void freeAdj(Adj* a) {
   for (AdjTape& T : a->tapes) freeTape(T);
   AdjGrad& G = a->G;
   dfree(G.gPos); dfree(G.gVel); dfree(G.gPos2); dfree(G.gVel2); dfree(G.gFN); dfree(G.gFO); dfree(G.gPres); dfree(G.gPres2);
-  dfree(G.gRhs); dfree(G.carry); dfree(G.sorNb); dfree(G.sorCoef); dfree(G.gCol);
+  dfree(G.gRhs); dfree(G.carry); dfree(G.sorNb); dfree(G.sorCoef); dfree(G.gCol); dfree(G.bar);
   delete a;
 }
 int allocTape(Sim* s, AdjTape& T) {
@@ -2019,14 +2022,37 @@ int st_adj_create(void* p, uint32_t max_steps, void** out) {
   if ((rc = dalloc(&G.gPos, M)) || (rc = dalloc(&G.gVel, M)) || (rc = dalloc(&G.gPos2, M)) || (rc = dalloc(&G.gVel2, M)) ||
       (rc = dalloc(&G.gFN, C * 3)) || (rc = dalloc(&G.gFO, C * 3)) || (rc = dalloc(&G.gPres, C)) || (rc = dalloc(&G.gPres2, C)) ||
       (rc = dalloc(&G.gRhs, C)) || (rc = dalloc(&G.carry, size_t(s->ntiles) * 512u)) || (rc = dalloc(&G.sorNb, C * 6)) ||
-      (rc = dalloc(&G.sorCoef, C * 6)) || (rc = dalloc(&G.gCol, size_t(ADJ_MAX_COLS) * 9))) {
+      (rc = dalloc(&G.sorCoef, C * 6)) || (rc = dalloc(&G.gCol, size_t(ADJ_MAX_COLS) * 9)) || (rc = dalloc(&G.bar, ADJ_BAR_WORDS))) {
     freeAdj(a);
     return rc;
+  }
+  {
+    hipError_t e = hipMemsetAsync(G.bar, 0, ADJ_BAR_WORDS * sizeof(uint32_t), s->st);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->dev);
+    if (e != hipSuccess) {
+      freeAdj(a);
+      return fail(ST_ERR_HIP, std::string("st_adj_create: ") + hipGetErrorString(e));
+    }
+    a->coopGrid = adj_sor_coop_grid(uint32_t(cus));
   }
   G.ncell = uint32_t(C);
   G.ncarry = s->ntiles * 512u;
   *out = a;
   return ST_OK;
+}
+
+// After a synchronisation: a time-out of the one-launch solve's grid barrier
+// (a workgroup that never arrived) left gradients that are not valid.  It is
+// reported once, and the context falls back to one launch per sweep.
+static int adjBarrierCheck(Adj* a, const char* who) {
+  uint32_t fails = 0;
+  HIPCHK(hipMemcpy(&fails, a->G.bar + ADJ_BAR_FAILS, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (!fails) return ST_OK;
+  HIPCHK(hipMemset(a->G.bar + ADJ_BAR_FAILS, 0, sizeof(uint32_t)));
+  a->coopGrid = 0;
+  return fail(ST_ERR_HIP, std::string(who) + ": a grid barrier of the one-launch adjoint solve timed out " + std::to_string(fails) +
+                              " times; the gradients are not valid (later solves use one launch per sweep)");
 }
 
 int st_adj_destroy(void* adj) {
@@ -2109,7 +2135,7 @@ int st_adj_backward(void* adj) {
   AdjTape& T = a->tapes[a->depth - 1];
   if (a->ncols[a->depth - 1] && (a->sdfSig[a->depth - 1] != s->sdfSig || a->sdfAt[a->depth - 1] != s->G.sdf))
     return fail(ST_ERR_STATE, "st_adj_backward: the collider grids changed since the substep was recorded");
-  launch_adjoint_step(s->st, T, a->G, a->scap, s->ntiles);
+  a->lastOne = launch_adjoint_step(s->st, T, a->G, a->scap, s->ntiles, a->oneLaunch ? a->coopGrid : 0u);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ST_ERR_HIP, std::string("adjoint launch: ") + hipGetErrorString(e));
   std::swap(a->G.gPos, a->G.gPos2);
@@ -2130,7 +2156,7 @@ int st_adj_col_grad(void* adj, float* g, uint32_t ncols) {
   HIPCHK(hipSetDevice(s->dev));
   HIPCHK(hipMemcpyAsync(g, a->G.gCol, size_t(ncols) * 9 * sizeof(float), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  return ST_OK;
+  return adjBarrierCheck(a, "st_adj_col_grad");
 }
 
 int st_adj_grad(void* adj, float* g_pos4, float* g_vel4) {
@@ -2141,6 +2167,20 @@ int st_adj_grad(void* adj, float* g_pos4, float* g_vel4) {
   HIPCHK(hipMemcpyAsync(g_pos4, a->G.gPos, size_t(s->N) * 16, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipMemcpyAsync(g_vel4, a->G.gVel, size_t(s->N) * 16, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  return adjBarrierCheck(a, "st_adj_grad");
+}
+
+int st_adj_set_sor_path(void* adj, uint32_t one_launch) {
+  Adj* a = (Adj*)adj;
+  if (!a) return fail(ST_ERR_ARG, "st_adj_set_sor_path: null context");
+  a->oneLaunch = one_launch != 0u;
+  return ST_OK;
+}
+
+int st_adj_sor_path(void* adj, uint32_t* one_launch) {
+  Adj* a = (Adj*)adj;
+  if (!a || !one_launch) return fail(ST_ERR_ARG, "st_adj_sor_path: null argument");
+  *one_launch = a->lastOne ? 1u : 0u;
   return ST_OK;
 }
 

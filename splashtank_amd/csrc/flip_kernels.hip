@@ -18,6 +18,7 @@
 // The reference's kernels each of these replaces are cited at the kernel.
 #include "flip_device.h"
 #include "flip_kernels.h"
+#include "grid_barrier.h"
 
 #include <algorithm>
 #include <utility>
@@ -1080,60 +1081,13 @@ __device__ __forceinline__ int3 coop_face_cell(uint32_t f, uint32_t e, uint32_t 
 
 #define COOP_NB_NONE 0xffffffffu
 #define COOP_NB_FOREIGN 0x80000000u
-#define COOP_SPIN_LIMIT (1u << 21)
+#define COOP_SPIN_LIMIT GB_SPIN_LIMIT
 
-// Grid barrier over the resident SOR's workgroups.  The halo cells cross it
-// through agent-scope (coherent) stores and loads.  Every wave waits until its
-// stores are done; workgroup w raises its flag sorBar[w] to `epoch` after an
-// agent-scope release fence; workgroup 0 waits for every flag (one thread per
-// flag) and raises the release word sorBar[COOP_MAX_WG + 2]; every other
-// workgroup waits for that word with a single thread; an agent-scope acquire
-// fence follows.  Each wait gives up after COOP_SPIN_LIMIT polls, so every wave
-// always leaves; a time-out is counted in sorBar[COOP_MAX_WG + 1] and returns
-// false.
-__device__ __forceinline__ bool coop_wait(const uint32_t* word, uint32_t epoch, uint32_t* sFail) {
-  uint32_t spins = 0;
-  while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-    if (++spins > COOP_SPIN_LIMIT) {
-      *sFail = 1u;
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  return true;
-}
-__device__ __forceinline__ void coop_arrive(const DGrid& G, uint32_t epoch) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(&G.sorBar[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-__device__ __forceinline__ bool coop_depart(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
-  uint32_t* rel = G.sorBar + COOP_MAX_WG + 2;
-  if (blockIdx.x == 0) {
-    if (threadIdx.x < gridDim.x) coop_wait(&G.sorBar[threadIdx.x], epoch, sFail);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-      __hip_atomic_store(rel, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  } else if (threadIdx.x == 0) {
-    coop_wait(rel, epoch, sFail);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  __syncthreads();
-  if (*sFail) {
-    if (threadIdx.x == 0) atomicAdd(&G.sorBar[COOP_MAX_WG + 1], 1u);
-    return false;
-  }
-  return true;
-}
+// Grid barrier over the resident SOR's workgroups (grid_barrier.h): flags in
+// sorBar[0 .. grid), time-outs counted in sorBar[COOP_MAX_WG + 1], the release
+// word sorBar[COOP_MAX_WG + 2].
 __device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
-  coop_arrive(G, epoch);
-  return coop_depart(G, epoch, sFail);
+  return gb_barrier(G.sorBar, G.sorBar + COOP_MAX_WG + 2, G.sorBar + COOP_MAX_WG + 1, epoch, sFail);
 }
 
 // sH holds, per resident tile and face, the 32 cells across that face that the

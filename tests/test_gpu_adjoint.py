@@ -323,3 +323,43 @@ def test_adjoint_refuses_what_it_does_not_cover():
         a.close()
     finally:
         g.close()
+
+
+def test_adjoint_one_launch_solve_matches_launches_per_sweep():
+    """The transposed solve as one cooperative launch (a grid barrier between
+    its sweeps) against one launch per sweep: the same sweeps in the same
+    order, so the gradients after 3 reversed substeps of the 32^3 dam break
+    agree up to the order of the f32 atomics of the reversed g2p and p2g
+    (relative 1e-5 in norm; a sweep reading a stale neighbour would be far off);
+    the default path is one launch per sweep."""
+    sc = scenes.dam_break(d=1.0 / 64, iters=20)
+    prm = capi.make_params(**sc.params)
+    out = []
+    for one in (True, False):
+        g = capi.FlipSimC(opts_for(sc))
+        a = None
+        try:
+            g.upload(sc.pos, sc.vel)
+            a = capi.AdjointC(g, 3)
+            if one:
+                a.set_sor_path(True)
+            for _ in range(3):
+                a.record(prm)
+            rng = np.random.default_rng(5)
+            n = g.count()
+            a.seed(rng.standard_normal((n, 3)), rng.standard_normal((n, 3)))
+            paths = []
+            for _ in range(3):
+                a.backward()
+                paths.append(a.sor_path())
+            gp, gv = a.grad()
+            out.append((gp.copy(), gv.copy(), paths))
+        finally:
+            if a is not None:
+                a.close()
+            g.close()
+    print(f"[adjoint] one-launch solve paths {out[0][2]}, per-sweep paths {out[1][2]}", flush=True)
+    assert all(out[0][2]) and not any(out[1][2])
+    ep, ev = _rel(out[0][0], out[1][0]), _rel(out[0][1], out[1][1])
+    print(f"[adjoint] one launch vs per sweep: rel diff pos {ep:.2e} vel {ev:.2e}", flush=True)
+    assert ep <= 1e-5 and ev <= 1e-5
