@@ -239,8 +239,9 @@ int st_step(void* sim, const st_params* p);
  * not, collider lookup rebuilt or not, sizes) is stream-captured and later
  * ones replay it, with the step's parameters copied into the one device slot
  * the graph reads.  Graphed substeps solve with one launch per SOR iteration
- * (cooperative launches cannot be captured; the bits are the same), and cover
- * plain substeps only (ST_ERR_ARG with the narrow band or a tile-only step).
+ * (the same bits; a captured cooperative launch is not scheduled as one when
+ * other streams run at the same time).  Graphs cover plain substeps only
+ * (ST_ERR_ARG with the narrow band or a tile-only step).
  * No counterpart in the reference, which records its dispatches into one
  * compute pass per step (src/flip/sim.js:245-310). */
 int st_step_n(void* sim, const st_params* p, int32_t nsteps, int32_t use_graph);

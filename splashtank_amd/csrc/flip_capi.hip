@@ -122,7 +122,7 @@ struct Sim {
   uint32_t sorResident = 2048;  // SOR workgroups resident at once on this device
   uint32_t coopGrid = 0;        // resident-SOR workgroups (0: not available)
   uint32_t sorSeq = 0;          // pressure solves so far
-  uint32_t sorTag = 1;          // tag of the next resident solve's first halo exchange (0 is never used)
+  uint32_t sorTag = 1;          // tag of the next substep's first halo exchange (0 is never used)
   uint32_t sorMode = 0;         // this step's pressure path (st_params.sor_mode, a tuned choice for 3)
   uint32_t pipeOverride = 0;    // workgroups of the per-iteration SOR launch (0: the default)
   bool tuned = false;           // st_tune_sor has chosen a path
@@ -176,7 +176,6 @@ struct Sim {
     L.coopGrid = sorMode == 1u ? 0u : coopGrid;
     L.sorStream = sorMode == 2u;
     L.sorSeq = sorSeq;
-    L.sorTag = sorTag;
     L.binned = binned;
     L.binNext = !nbOn;  // the narrow band's reseed moves particles after g2p
     L.nbOn = nbOn;
@@ -505,6 +504,19 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
   }
   s->extrapOnStep = o->only_tiles ? 0u : o->extrap_layers;
   s->iters = std::max(1, int(o->iters));
+  {
+    // Every halo exchange of a resident solve carries its own tag, taken from
+    // this block (so each replay of a captured substep gets new ones): the
+    // step reserves 2 iters + 2.  When the counter would wrap, the entries are
+    // cleared first so that no old tag can match.
+    const uint32_t need = 2u * uint32_t(s->iters) + 2u;
+    if (s->sorTag > 0xffffffffu - need) {
+      launch_fill_u32(s->st, reinterpret_cast<uint32_t*>(s->G.sorHalo), size_t(s->scap) * 2 * 6 * 32 * 2, 0u);
+      s->sorTag = 1;
+    }
+    P.sorTag = s->sorTag;
+    s->sorTag += need;
+  }
   Pout = P;
   size_t ncAll = std::min<size_t>(s->ncTotal, ST_MAX_COLS);
   std::memcpy(C.c, s->cf.data(), ncAll * 32 * sizeof(float));
@@ -558,17 +570,6 @@ int runStage(Sim* s, int stage) {
     case ST_STAGE_PRESSURE:
       if (full) {
         L.sorSeq = ++s->sorSeq;
-        if (L.coopGrid) {
-          // every halo exchange of a resident solve has its own tag; when the
-          // counter would wrap, the entries are cleared so no old tag can match
-          const uint32_t need = 2u * uint32_t(s->iters) + 2u;
-          if (s->sorTag > 0xffffffffu - need) {
-            HIPCHK(hipMemsetAsync(s->G.sorHalo, 0, size_t(s->scap) * 2 * 6 * 32 * sizeof(uint64_t), st));
-            s->sorTag = 1;
-          }
-          L.sorTag = s->sorTag;
-          s->sorTag += need;
-        }
         if (launch_pressure(st, L, s->iters) != hipSuccess) s->coopGrid = 0;  // per-iteration launches from now on
       }
       break;
@@ -603,11 +604,16 @@ int runStage(Sim* s, int stage) {
 // Graph replay of a plain substep (st_step_n with use_graph).  The step's
 // parameter block and collider records go to one fixed device slot, which the
 // captured kernels read; the first substep of a kind is captured (stream
-// capture of every stage) and later ones replay it.  Cooperative launches
-// cannot be captured, so a graphed substep solves with one launch per
-// iteration (the same bits).  The host-side effects of the stages (the particle
-// buffers swapped, the bins of the next substep made, the collider lookup
-// rebuilt) are applied again on each replay.
+// capture of every stage) and later ones replay it.  A graphed substep solves
+// with one launch per SOR iteration (the same bits).  A captured cooperative
+// launch does replay, alone with the right bits, but with a second sim
+// stepping on its own stream at the same time the replays gave other bits
+// than the eager substeps (direct cooperative launches kept them): a graph
+// node is not scheduled as a cooperative launch, whose workgroups must all be
+// resident at once.
+// The host-side effects of the stages (the particle buffers swapped, the bins
+// of the next substep made, the collider lookup rebuilt) are applied again on
+// each replay.
 void dropGraphs(Sim* s) {
   if (s->graphs.empty()) return;
   (void)hipStreamSynchronize(s->st);

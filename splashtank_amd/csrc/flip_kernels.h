@@ -30,7 +30,6 @@ struct LaunchCtx {
   uint32_t pipeGrid;  // persistent SOR workgroups (about eight per CU)
   uint32_t coopGrid;  // resident-SOR workgroups (one per CU), 0 when cooperative launches are unavailable
   uint32_t sorSeq;    // sequence number of this pressure solve
-  uint32_t sorTag;    // resident solve: tag of its first halo exchange (the k-th carries sorTag + k)
   bool sorStream;     // the cooperative solve streams tile windows even when the tiles fit in LDS
   bool binned;        // blocks: need, binCount and binKey already hold A's marks and bins
   bool binNext;       // g2p: mark and bin the positions it writes, for the next substep

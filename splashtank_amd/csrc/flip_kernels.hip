@@ -1263,7 +1263,7 @@ constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? size
 static_assert(kCoopLdsBytes + 256 <= 160 * 1024, "the resident solve's LDS fits a CU");
 
 __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
-                                                          uint32_t seq, uint32_t stream, uint32_t tag0) {
+                                                          uint32_t seq, uint32_t stream) {
   __shared__ __align__(16) unsigned char smem[kCoopLdsBytes];
   __shared__ uint32_t nFor;
   __shared__ float ytab[8];
@@ -1271,6 +1271,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
   const uint32_t nact = G.ctl[4], ng = gridDim.x, tid = threadIdx.x, w = blockIdx.x;
   if (nact == 0u || ng > COOP_MAX_WG) return;
   const float omega = PP->omega;
+  const uint32_t tag0 = PP->sorTag;
   if (tid == 0u) sFail = 0u;
   if (stream || nact > ng * COOP_TMAX) {
     // Streaming mode, for more tiles than fit in LDS: four tile pipelines of 256
@@ -2540,8 +2541,8 @@ hipError_t launch_pressure(hipStream_t st, const LaunchCtx& L, int iters) {
     launch_fill_u32(st, L.G.sorBar + COOP_MAX_WG + 2, 1, 0u);
     const DParams* P = L.P;
     DGrid G = L.G;
-    uint32_t it = uint32_t(iters), seq = L.sorSeq, stream = L.sorStream ? 1u : 0u, tag0 = L.sorTag;
-    void* args[] = {(void*)&P, (void*)&G, (void*)&it, (void*)&seq, (void*)&stream, (void*)&tag0};
+    uint32_t it = uint32_t(iters), seq = L.sorSeq, stream = L.sorStream ? 1u : 0u;
+    void* args[] = {(void*)&P, (void*)&G, (void*)&it, (void*)&seq, (void*)&stream};
     coopErr = hipLaunchCooperativeKernel((const void*)k_sor_resident, dim3(L.coopGrid), dim3(COOP_WG), args, 0, st);
     if (coopErr == hipSuccess) return hipSuccess;
     (void)hipGetLastError();  // the per-iteration launches below do the solve
