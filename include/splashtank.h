@@ -407,9 +407,10 @@ int st_solve_norms_static(void* sim, st_norms* out);
  * workgroup count -- `reps` times each after one warm-up, keeps the fastest for
  * steps with sor_mode = ST_SOR_TUNED, and restores the state it started from
  * (the next substep gives the bits it would have given without the tuning).
- * st_batch_tune_sor: the same for a batch, whose solves run one launch per
- * iteration: it times whole batched substeps for SOR widths of 1/4, 1/2, 1 and 2
- * times the resident workgroup count and keeps the fastest for the batch. */
+ * st_batch_tune_sor: the same for a batch's launches per iteration (graph
+ * replays, or a batch whose resident solve is switched off): it times whole
+ * batched substeps for SOR widths of 1/4, 1/2, 1 and 2 times the resident
+ * workgroup count and keeps the fastest for the batch. */
 #define ST_TUNE_MAX 8
 typedef struct st_sor_tune {
   uint32_t mode;               /* chosen path (0 resident, 1 per iteration, 2 streaming) */
@@ -446,6 +447,17 @@ int st_batch_step_n(void* batch, const st_params* p, uint32_t np, int32_t nsteps
 int st_batch_sync(void* batch);
 /* Number of tanks. */
 int st_batch_size(void* batch, uint32_t* n);
+/* The batched pressure solve.  By default (st_batch_step, no graph) every
+ * tank's SOR runs resident in one cooperative launch of the device's resident
+ * workgroups (one per CU): each tank gets a range of them, enough that its
+ * tiles fit in LDS (a tank too big for all of them streams tile windows), in
+ * as few phases as the tiles allow, and the same bits as alone.  Graph replays
+ * and st_batch_sor_grid(batch, 0) use one launch per iteration instead;
+ * st_batch_sor_grid(batch, w) caps the cooperative launch at w workgroups (at
+ * least 2, at most the resident count; more phases).  st_batch_sor_path reports whether
+ * the last batched substep's solve ran in the one launch. */
+int st_batch_sor_grid(void* batch, uint32_t workgroups);
+int st_batch_sor_path(void* batch, uint32_t* resident);
 
 /* Snapshots: a copy of the state a substep starts from (particles, tile tables,
  * pool pressure and cell types, time and seed), for re-running substeps from

@@ -264,6 +264,8 @@ PROTOS = {
     "st_adj_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "st_adj_col_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "st_adj_set_sor_path": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_batch_sor_grid": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "st_batch_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "st_adj_sor_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
@@ -571,6 +573,17 @@ class FlipBatchC:
 
     def sync(self):
         self._call("sync", self.h)
+
+    def sor_grid(self, workgroups: int):
+        """Cap the one-launch batched SOR at `workgroups` resident workgroups
+        (0: one launch per iteration)."""
+        self._call("sor_grid", self.h, int(workgroups))
+
+    def sor_path(self) -> bool:
+        """Whether the last batched substep's SOR ran in the one launch."""
+        v = C.c_uint32()
+        self._call("sor_path", self.h, C.byref(v))
+        return bool(v.value)
 
     def tune_sor(self, prm, reps: int = 3):
         """Time whole batched substeps at several SOR launch widths and keep the

@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(AWG) k_adj_sor_all(AdjTape T, AdjGrad G) {
     for (int colour = 1; colour >= 0; colour--) {
       adj_sor_body(T, G, a, b, uint32_t(colour), w);
       if (it + 1u < T.iters || colour == 1)
-        if (!gb_barrier(G.bar, G.bar + ADJ_BAR_RELEASE, G.bar + ADJ_BAR_FAILS, ++epoch, &sFail)) return;
+        if (!gb_barrier(G.bar, G.bar + ADJ_BAR_RELEASE, G.bar + ADJ_BAR_FAILS, ++epoch, &sFail, blockIdx.x, gridDim.x)) return;
       float* t = a;
       a = b;
       b = t;

@@ -515,6 +515,7 @@ int prepStep(Sim* s, const st_params* o, DParams& Pout, DCols& C) {
       s->sorTag = 1;
     }
     P.sorTag = s->sorTag;
+    P.iters = uint32_t(s->iters);
     s->sorTag += need;
   }
   Pout = P;
@@ -710,6 +711,9 @@ struct Batch {
   hipStream_t st = nullptr;
   std::vector<Sim*> tanks;
   uint32_t sorResident = 2048;   // SOR workgroups of a batched iteration, over all tanks
+  uint32_t coopGrid = 0;         // resident workgroups of the one-launch batched SOR (0: launches per iteration)
+  uint32_t* dPlan = nullptr;     // its per-tank workgroup plan (3 nt + 1 words)
+  uint32_t lastResident = 0;     // the last substep's SOR ran in the one launch
   // ring of step slots: tank records, parameter blocks, collider records, block
   // tables (host pinned and device), each reusable once its upload is done
   size_t slotBytes = 0, offP = 0, offC = 0, offTab = 0;
@@ -1694,7 +1698,12 @@ int st_batch_create(void* const* sims, uint32_t n, void** out) {
     b->tanks.push_back(s);
   }
   b->sorResident = b->tanks[0]->sorResident;
+  b->coopGrid = b->tanks[0]->coopGrid;
   b->plan.resize(n);
+  if (b->coopGrid && dalloc(&b->dPlan, size_t(n) * 3 + 1) != ST_OK) {
+    (void)hipGetLastError();
+    b->coopGrid = 0;  // launches per iteration
+  }
   *out = b;
   return ST_OK;
 }
@@ -1710,6 +1719,7 @@ int st_batch_destroy(void* p) {
     s->batch = nullptr;
   }
   freeRing(b);
+  dfree(b->dPlan);
   (void)hipStreamDestroy(b->st);
   delete b;
   return ST_OK;
@@ -1762,10 +1772,14 @@ static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) 
   HIPCHK(hipEventRecord(b->ev[k], b->st));
   b->ringPos++;
   if (!graph) {
-    launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+    b->lastResident = launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax, b->coopGrid,
+                                        b->dPlan) ? 1u : 0u;
   } else {
     // the block tables fix every launch's grid and kernel choice; the tank
-    // records, parameters and tables are read from the graph slot at run time
+    // records, parameters and tables are read from the graph slot at run time.
+    // The SOR takes one launch per iteration in a graph: a captured
+    // cooperative launch is not scheduled as one when other streams run.
+    b->lastResident = 0u;
     const size_t words = batch_table_words(nt, itersMax);
     Batch::Captured* hit = nullptr;
     for (auto& g : b->graphs)
@@ -1774,7 +1788,7 @@ static int batchStep(Batch* b, const st_params* prm, uint32_t nprm, bool graph) 
       HIPCHK(hipGraphLaunch(hit->exec, b->st));
     } else {
       HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-      launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax);
+      launch_batch_step(b->st, (const BTank*)d, (const uint32_t*)(d + b->offTab), hTab, nt, itersMax, 0u, nullptr);
       hipError_t le = hipGetLastError();
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(b->st, &g);
@@ -1836,6 +1850,29 @@ int st_batch_sync(void* p) {
   if (!b) return fail(ST_ERR_ARG, "st_batch_sync: null batch");
   HIPCHK(hipSetDevice(b->dev));
   HIPCHK(hipStreamSynchronize(b->st));
+  return ST_OK;
+}
+
+int st_batch_sor_grid(void* p, uint32_t workgroups) {
+  Batch* b = (Batch*)p;
+  if (!b) return fail(ST_ERR_ARG, "st_batch_sor_grid: null batch");
+  const uint32_t most = b->tanks[0]->coopGrid;
+  if (workgroups && !b->dPlan) {
+    if (!most) return fail(ST_ERR_STATE, "st_batch_sor_grid: cooperative launches are not available on this device");
+    HIPCHK(hipSetDevice(b->dev));
+    int rc = dalloc(&b->dPlan, b->tanks.size() * 3 + 1);
+    if (rc) return rc;
+  }
+  // a tank that streams tile windows runs four pipelines per workgroup over
+  // eight XCD groups, which needs at least two workgroups
+  b->coopGrid = workgroups ? std::max(2u, std::min(workgroups, most)) : 0u;
+  return ST_OK;
+}
+
+int st_batch_sor_path(void* p, uint32_t* resident) {
+  Batch* b = (Batch*)p;
+  if (!b || !resident) return fail(ST_ERR_ARG, "st_batch_sor_path: null argument");
+  *resident = b->lastResident;
   return ST_OK;
 }
 

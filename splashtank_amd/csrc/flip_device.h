@@ -33,6 +33,7 @@ struct DParams {
   uint32_t nfv, ntiles;
   float settle;  // smoothstep(0.1, 0.5, tm), the same for every particle of the step
   uint32_t sorTag;  // resident SOR: tag of the substep's first halo exchange (the k-th carries sorTag + k)
+  uint32_t iters;   // SOR iterations of the substep
   float fills[16][4];
   // whitewater and the air field (wgsl-core.js:10-16; sim.js:203-206, 221-230)
   float sprRate, sprMin, sprLife, mistFrac, sprDrag, mistDrag, vs;

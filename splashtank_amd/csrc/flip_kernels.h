@@ -83,7 +83,11 @@ size_t batch_table_words(uint32_t nt, uint32_t itersMax);
 void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_t sorResident, uint32_t* tab);
 // every stage of one substep for all tanks, one launch per kernel: dT and dtab
 // are the device copies of the tank records and tables, htab the host copy
-void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax);
+// coopGrid > 0 (and a plan buffer of 3 nt + 1 words): the SOR of every tank in
+// one cooperative launch of that many resident workgroups, else (or when the
+// launch is refused) one launch per iteration; returns whether it ran resident
+bool launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax,
+                       uint32_t coopGrid, uint32_t* dPlan);
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_blocks(hipStream_t st, const LaunchCtx& L);

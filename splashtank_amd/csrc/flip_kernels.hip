@@ -1083,11 +1083,11 @@ __device__ __forceinline__ int3 coop_face_cell(uint32_t f, uint32_t e, uint32_t 
 #define COOP_NB_FOREIGN 0x80000000u
 #define COOP_SPIN_LIMIT GB_SPIN_LIMIT
 
-// Grid barrier over the resident SOR's workgroups (grid_barrier.h): flags in
-// sorBar[0 .. grid), time-outs counted in sorBar[COOP_MAX_WG + 1], the release
-// word sorBar[COOP_MAX_WG + 2].
-__device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail) {
-  return gb_barrier(G.sorBar, G.sorBar + COOP_MAX_WG + 2, G.sorBar + COOP_MAX_WG + 1, epoch, sFail);
+// Barrier over the resident SOR's ng workgroups of one sim (grid_barrier.h):
+// flags in sorBar[0 .. ng), time-outs counted in sorBar[COOP_MAX_WG + 1], the
+// release word sorBar[COOP_MAX_WG + 2].
+__device__ __forceinline__ bool coop_barrier(const DGrid& G, uint32_t epoch, uint32_t* sFail, uint32_t w, uint32_t ng) {
+  return gb_barrier(G.sorBar, G.sorBar + COOP_MAX_WG + 2, G.sorBar + COOP_MAX_WG + 1, epoch, sFail, w, ng);
 }
 
 // sH holds, per resident tile and face, the 32 cells across that face that the
@@ -1262,13 +1262,15 @@ static_assert(sizeof(float) * COOP_TMAX * 192 >= 2 * COOP_TMAX * 64 * sizeof(uin
 constexpr size_t kCoopLdsBytes = sizeof(CoopLds) > 4 * sizeof(SorPipeLds) ? sizeof(CoopLds) : 4 * sizeof(SorPipeLds);
 static_assert(kCoopLdsBytes + 256 <= 160 * 1024, "the resident solve's LDS fits a CU");
 
-__global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
-                                                          uint32_t seq, uint32_t stream) {
+// The resident solve of one sim by its workgroups w = 0 .. ng-1 (the whole
+// launch for one sim, or its range of a batched launch).
+__device__ __forceinline__ void sor_resident_body(const DParams* __restrict__ PP, const DGrid& G, uint32_t iters, uint32_t seq,
+                                                  uint32_t stream, uint32_t w, uint32_t ng) {
   __shared__ __align__(16) unsigned char smem[kCoopLdsBytes];
   __shared__ uint32_t nFor;
   __shared__ float ytab[8];
   __shared__ uint32_t sFail;
-  const uint32_t nact = G.ctl[4], ng = gridDim.x, tid = threadIdx.x, w = blockIdx.x;
+  const uint32_t nact = G.ctl[4], tid = threadIdx.x;
   if (nact == 0u || ng > COOP_MAX_WG) return;
   const float omega = PP->omega;
   const uint32_t tag0 = PP->sorTag;
@@ -1285,7 +1287,7 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
       const float* pin = (it & 1u) ? G.pres2 : G.pres;
       float* pout = (it & 1u) ? G.pres : G.pres2;
       sor_pipe_pass(G, omega, pin, pout, g * ng + w, 4u * ng, gt, L);
-      if (it + 1u < iters && !coop_barrier(G, ++epoch, &sFail)) return;
+      if (it + 1u < iters && !coop_barrier(G, ++epoch, &sFail, w, ng)) return;
     }
     if (iters & 1u) {  // the result sits in pres2: copy back the tiles this pipeline wrote
       const uint32_t vid = g * ng + w, gs = (4u * ng) >> 3, xcd = vid & 7u, per = (nact + 7u) >> 3;
@@ -1475,6 +1477,10 @@ __global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restr
     G.sorBar[COOP_MAX_WG] = seq;
     G.sorBar[COOP_MAX_WG + 3] = 1u;
   }
+}
+__global__ void __launch_bounds__(COOP_WG) k_sor_resident(const DParams* __restrict__ PP, DGrid G, uint32_t iters,
+                                                          uint32_t seq, uint32_t stream) {
+  sor_resident_body(PP, G, iters, seq, stream, blockIdx.x, gridDim.x);
 }
 
 // Morton-ordered list of the slots in use (coopList) for the resident SOR
@@ -2403,6 +2409,106 @@ __global__ void kb_sor_path(BATCH_ARGS) {
   BATCH_TANK;
   if (threadIdx.x == 0 && b.blk == 0) T.G.sorBar[COOP_MAX_WG + 3] = 0u;
 }
+// Batched resident SOR.  The tanks' solves run in one cooperative launch of
+// the resident workgroups (one per CU): each tank gets a range of them, in one
+// of a few phases when all tanks' tiles do not fit in LDS at once, and runs
+// sor_resident_body over its range (its own halo entries, tags and barrier
+// words).  A tank's workgroups only wait on each other, and every workgroup
+// goes through the phases in order, so a phase never waits on a later one.
+// Per tank active this substep (its range in the first iteration's table is
+// not empty): kb_coop_lists (one 1024-thread workgroup per tank) builds its
+// Morton-ordered slot list, as k_coop_flags / scan / k_coop_compact do, and
+// clears its barrier words; kb_sor_plan (one thread) assigns the workgroups.
+__global__ void __launch_bounds__(1024) kb_coop_lists(const BTank* __restrict__ TT, const uint32_t* __restrict__ act, uint32_t nt) {
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  if (t >= nt || act[t + 1] == act[t]) return;
+  const DGrid G = TT[t].G;
+  const uint32_t ntl = TT[t].P->ntiles;
+  __shared__ uint32_t wsum[1024 / 64];
+  for (uint32_t q = tid; q < COOP_MAX_WG; q += 1024u) G.sorBar[q] = 0u;
+  if (tid == 0u) G.sorBar[COOP_MAX_WG + 2] = 0u;
+  uint32_t base = 0;
+  for (uint32_t m0 = 0; m0 < ntl; m0 += 1024u) {
+    const uint32_t m = m0 + tid;
+    const uint32_t sl = m < ntl ? G.tmap[G.mortonTiles[m]] : ST_NONE;
+    const uint32_t f = sl != ST_NONE ? 1u : 0u;
+    const uint64_t bal = __ballot(f != 0u);
+    const uint32_t inWave = uint32_t(__popcll(bal & lanemask_lt()));
+    if (lane_id() == 0u) wsum[tid >> 6] = uint32_t(__popcll(bal));
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t u = 0; u < 1024u / 64u; u++) {
+      if (u < (tid >> 6)) before += wsum[u];
+      tot += wsum[u];
+    }
+    if (f) G.coopList[base + before + inWave] = sl;
+    base += tot;
+    __syncthreads();
+  }
+}
+
+// plan[3 t .. 3 t + 2] = (phase, first workgroup, workgroups) of tank t (phase
+// ST_NONE: no solve), plan[3 nt] = phases.  First fit in tank order, each tank
+// at least the workgroups its tiles need in LDS (all `grid` when even those
+// are too few: it then streams); a phase's spare workgroups go to its tanks in
+// proportion to their tiles.  A tank that streams is alone in its phase with
+// all `grid` workgroups (at least 2, which its tile pipelines need).
+__global__ void kb_sor_plan(const BTank* __restrict__ TT, const uint32_t* __restrict__ act, uint32_t nt, uint32_t grid,
+                            uint32_t* __restrict__ plan) {
+  if (threadIdx.x != 0u || blockIdx.x != 0u) return;
+  uint32_t phase = 0, used = 0, first = 0;
+  uint64_t tiles = 0;
+  auto close = [&](uint32_t endT) {  // spread the spare workgroups of the phase over tanks [first, endT)
+    const uint32_t spare = grid - used;
+    uint32_t at = 0;
+    for (uint32_t q = first; q < endT; q++) {
+      if (plan[3 * q] != phase) continue;
+      const uint32_t nact = TT[q].G.ctl[4];
+      const uint32_t extra = tiles ? uint32_t(uint64_t(spare) * nact / tiles) : 0u;
+      plan[3 * q + 1] = at;
+      plan[3 * q + 2] += extra;
+      at += plan[3 * q + 2];
+    }
+  };
+  for (uint32_t t = 0; t < nt; t++) {
+    plan[3 * t] = ST_NONE;
+    plan[3 * t + 1] = 0u;
+    plan[3 * t + 2] = 0u;
+    const uint32_t nact = act[t + 1] != act[t] ? TT[t].G.ctl[4] : 0u;
+    if (nact == 0u) continue;
+    const uint32_t need = min(grid, (nact + COOP_TMAX - 1u) / COOP_TMAX);
+    if (used + need > grid) {
+      close(t);
+      phase++;
+      used = 0;
+      tiles = 0;
+      first = t;
+    }
+    plan[3 * t] = phase;
+    plan[3 * t + 2] = need;
+    used += need;
+    tiles += nact;
+  }
+  if (used) {
+    close(nt);
+    phase++;
+  }
+  plan[3 * nt] = phase;
+}
+
+__global__ void __launch_bounds__(COOP_WG) kb_sor_resident(const BTank* __restrict__ TT, const uint32_t* __restrict__ plan, uint32_t nt) {
+  const uint32_t nph = plan[3 * nt], w = blockIdx.x;
+  for (uint32_t p = 0; p < nph; p++) {
+    uint32_t t = ST_NONE;
+    for (uint32_t q = 0; q < nt; q++)
+      if (plan[3 * q] == p && w >= plan[3 * q + 1] && w < plan[3 * q + 1] + plan[3 * q + 2]) t = q;
+    if (t == ST_NONE) continue;
+    const BTank& B = TT[t];
+    sor_resident_body(B.P, B.G, B.P->iters, 0u, 0u, w - plan[3 * t + 1], plan[3 * t + 2]);
+    __syncthreads();  // the LDS of this phase's tank is free again
+  }
+}
+
 __global__ void kb_project(BATCH_ARGS) { BATCH_TANK; project_body(TP, T.G, b.blk, b.nblk); }
 __global__ void kb_foam_table(BATCH_ARGS) { BATCH_TANK; foam_table_body(TP, T.G, b.blk, b.nblk); }
 __global__ void __launch_bounds__(WG) kb_g2p_tile(BATCH_ARGS) { BATCH_TANK; g2p_tile_body(TP, TC, T.G, 1u, b.blk, b.nblk); }
@@ -2658,7 +2764,8 @@ void batch_tables(const BTankPlan* plan, uint32_t nt, uint32_t itersMax, uint32_
   }
 }
 
-void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax) {
+bool launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, const uint32_t* htab, uint32_t nt, uint32_t itersMax,
+                       uint32_t coopGrid, uint32_t* dPlan) {
   const size_t w = size_t(nt) + 1u;
   auto tab = [&](uint32_t k) { return dtab + k * w; };
   auto tot = [&](uint32_t k) { return htab[k * w + nt]; };
@@ -2688,16 +2795,31 @@ void launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, co
   BLAUNCH(kb_reduce_stats, BT_ONE, 1024, tab(BT_CELL), 1u);
   BLAUNCH(kb_norm_faces, BT_CELL, WG);
   BLAUNCH(kb_divergence, BT_CELL, WG);
-  // pressure: one launch per iteration, ping-pong between pres and pres2
-  for (uint32_t it = 0; it < itersMax; it++) BLAUNCH(kb_sor_pipe, uint32_t(BT_PIPE0) + it, WG, it & 1u);
-  BLAUNCH(kb_copy_tiles, BT_COPY, WG);
-  BLAUNCH(kb_sor_path, BT_ONE, 64);
+  // pressure: the resident solves of all tanks in one cooperative launch, or
+  // one launch per iteration (ping-pong between pres and pres2)
+  bool resident = false;
+  if (coopGrid && dPlan && itersMax) {
+    hipLaunchKernelGGL(kb_coop_lists, dim3(nt), dim3(1024), 0, st, dT, tab(BT_PIPE0), nt);
+    hipLaunchKernelGGL(kb_sor_plan, dim3(1), dim3(64), 0, st, dT, tab(BT_PIPE0), nt, coopGrid, dPlan);
+    const BTank* a0 = dT;
+    const uint32_t* a1 = dPlan;
+    uint32_t a2 = nt;
+    void* args[] = {(void*)&a0, (void*)&a1, (void*)&a2};
+    resident = hipLaunchCooperativeKernel((const void*)kb_sor_resident, dim3(coopGrid), dim3(COOP_WG), args, 0, st) == hipSuccess;
+    if (!resident) (void)hipGetLastError();  // the launches per iteration below do the solve
+  }
+  if (!resident) {
+    for (uint32_t it = 0; it < itersMax; it++) BLAUNCH(kb_sor_pipe, uint32_t(BT_PIPE0) + it, WG, it & 1u);
+    BLAUNCH(kb_copy_tiles, BT_COPY, WG);
+    BLAUNCH(kb_sor_path, BT_ONE, 64);
+  }
   // project, g2p (which marks and bins the next substep)
   BLAUNCH(kb_project, BT_CELL, WG);
   BLAUNCH(kb_fill_bins, BT_FILL_NEXT, WG);
   BLAUNCH(kb_foam_table, BT_FOAM, WG);
   BLAUNCH(kb_g2p_tile, BT_G2P, WG);
 #undef BLAUNCH
+  return resident;
 }
 
 // ---------------------------------------------------------------------------

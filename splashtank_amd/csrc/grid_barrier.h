@@ -2,8 +2,10 @@
 // (every workgroup resident at once), shared by the solves that run all their
 // sweeps in a single launch.
 //
-// Words: flags[w] is workgroup w's arrival epoch (gridDim.x <= blockDim.x, so
-// workgroup 0 can wait for every flag with one thread each), *release the epoch
+// The n workgroups taking part are numbered w = 0 .. n-1 (blockIdx.x and
+// gridDim.x for a whole launch, or a range of it).  Words: flags[w] is
+// workgroup w's arrival epoch (n <= blockDim.x, so workgroup 0 can wait for
+// every flag with one thread each), *release the epoch
 // workgroup 0 has let go, *fails the count of time-outs.  Every wave waits
 // until its stores are done; workgroup w raises flags[w] to `epoch` after an
 // agent-scope release fence; workgroup 0 waits for every flag and raises
@@ -31,15 +33,16 @@ __device__ __forceinline__ bool gb_wait(const uint32_t* word, uint32_t epoch, ui
   return true;
 }
 
-__device__ __forceinline__ bool gb_barrier(uint32_t* flags, uint32_t* release, uint32_t* fails, uint32_t epoch, uint32_t* sFail) {
+__device__ __forceinline__ bool gb_barrier(uint32_t* flags, uint32_t* release, uint32_t* fails, uint32_t epoch, uint32_t* sFail,
+                                           uint32_t w, uint32_t n) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(&flags[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&flags[w], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (blockIdx.x == 0) {
-    if (threadIdx.x < gridDim.x) gb_wait(&flags[threadIdx.x], epoch, sFail);
+  if (w == 0) {
+    if (threadIdx.x < n) gb_wait(&flags[threadIdx.x], epoch, sFail);
     __syncthreads();
     if (threadIdx.x == 0) {
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");

@@ -85,6 +85,43 @@ def test_batch_matches_single_tanks():
         assert s.h is None
 
 
+@pytest.mark.parametrize("grid", [None, 0, 8, 2])
+def test_batch_resident_solve_phases(grid):
+    """The batched SOR in one cooperative launch (default), in several phases
+    and with tanks that stream tile windows when the launch is capped at 8 or
+    2 workgroups, and one launch per iteration (0): the same bits as each tank
+    stepped alone, over three substeps."""
+    tanks = _tank_set()
+    prms = _params(tanks)
+    solo = [FlipSimC(opts_for(sc)) for sc in tanks]
+    bt = [FlipSimC(opts_for(sc)) for sc in tanks]
+    for sc, a, b in zip(tanks, solo, bt):
+        upload_scene(a, sc)
+        upload_scene(b, sc)
+    batch = FlipBatchC(bt)
+    try:
+        if grid is not None:
+            batch.sor_grid(grid)
+        t = 0.0
+        for step in range(3):
+            t += tanks[0].params["dt"]
+            print(f"[batch] resident solve, grid cap {grid}, step {step}", flush=True)
+            for sc, a, b, p in zip(tanks, solo, bt, prms):
+                set_colliders_at(a, sc, t)
+                set_colliders_at(b, sc, t)
+                a.step(p)
+                a.sync()
+            batch.step(prms)
+            batch.sync()
+            assert batch.sor_path() == (grid != 0)
+            for sc, a, b in zip(tanks, solo, bt):
+                _same(a, b, f"grid {grid} step {step} {sc.name}")
+    finally:
+        batch.close()
+    for s in solo + bt:
+        s.close()
+
+
 def test_batch_one_param_block_and_oracle():
     """One parameter block for all tanks; the tanks' particles after two
     substeps against the oracle (f32 bar, bit-identical counted)."""
