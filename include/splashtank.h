@@ -407,10 +407,11 @@ int st_solve_norms_static(void* sim, st_norms* out);
  * workgroup count -- `reps` times each after one warm-up, keeps the fastest for
  * steps with sor_mode = ST_SOR_TUNED, and restores the state it started from
  * (the next substep gives the bits it would have given without the tuning).
- * st_batch_tune_sor: the same for a batch's launches per iteration (graph
- * replays, or a batch whose resident solve is switched off): it times whole
- * batched substeps for SOR widths of 1/4, 1/2, 1 and 2 times the resident
- * workgroup count and keeps the fastest for the batch. */
+ * st_batch_tune_sor: the same for a batch: it times whole batched substeps
+ * with the resident solves in one cooperative launch and with one launch per
+ * iteration at 1/4, 1/2, 1 and 2 times the resident workgroup count, keeps the
+ * fastest for stream steps (mode 0 or 1) and the fastest per-iteration width
+ * for graph replays, and restores the tanks. */
 #define ST_TUNE_MAX 8
 typedef struct st_sor_tune {
   uint32_t mode;               /* chosen path (0 resident, 1 per iteration, 2 streaming) */

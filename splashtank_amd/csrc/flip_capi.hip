@@ -2004,7 +2004,12 @@ int st_batch_tune_sor(void* p, const st_params* prm, uint32_t nprm, int32_t reps
   std::memset(out, 0, sizeof *out);
   const uint32_t nt = uint32_t(b->tanks.size());
   uint32_t base = b->tanks[0]->sorResident, n = 0;
-  for (uint32_t mul = 1; mul <= 8; mul *= 2) {  // 1/4, 1/2, 1, 2 of the resident width
+  const uint32_t coop = b->dPlan ? b->tanks[0]->coopGrid : 0u, keepCoop = b->coopGrid;
+  if (coop) {  // the resident solves in one cooperative launch
+    out->cand_mode[n] = 0;
+    out->cand_width[n++] = coop;
+  }
+  for (uint32_t mul = 1; mul <= 8; mul *= 2) {  // one launch per iteration, 1/4, 1/2, 1, 2 of the resident width
     out->cand_mode[n] = 1;
     out->cand_width[n++] = std::max<uint32_t>(8u, (base * mul / 4u) & ~7u);
   }
@@ -2016,7 +2021,12 @@ int st_batch_tune_sor(void* p, const st_params* prm, uint32_t nprm, int32_t reps
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: hipEventCreate");
   uint32_t keep = b->sorResident;
   for (uint32_t c = 0; !rc && c < n; c++) {
-    b->sorResident = out->cand_width[c];
+    if (out->cand_mode[c] == 0) {
+      b->coopGrid = out->cand_width[c];
+    } else {
+      b->coopGrid = 0u;
+      b->sorResident = out->cand_width[c];
+    }
     rc = st_batch_step(p, prm, nprm);  // warm-up
     if (!rc && hipEventRecord(e0, b->st) != hipSuccess) rc = fail(ST_ERR_HIP, "st_batch_tune_sor: hipEventRecord");
     for (int32_t r = 0; !rc && r < reps; r++) rc = st_batch_step(p, prm, nprm);
@@ -2039,14 +2049,20 @@ int st_batch_tune_sor(void* p, const st_params* prm, uint32_t nprm, int32_t reps
     }
   if (rc) {
     b->sorResident = keep;
+    b->coopGrid = keepCoop;
     return rc;
   }
-  uint32_t best = 0;
+  // the fastest overall for stream steps; the fastest per-iteration width for
+  // graph replays (which always launch per iteration)
+  uint32_t best = 0, bestPipe = n;
   for (uint32_t c = 1; c < n; c++)
     if (out->cand_ms[c] < out->cand_ms[best]) best = c;
-  out->mode = 1;
+  for (uint32_t c = 0; c < n; c++)
+    if (out->cand_mode[c] == 1u && (bestPipe == n || out->cand_ms[c] < out->cand_ms[bestPipe])) bestPipe = c;
+  out->mode = out->cand_mode[best];
   out->width = out->cand_width[best];
-  b->sorResident = out->width;
+  b->sorResident = out->cand_width[bestPipe];
+  b->coopGrid = out->mode == 0u ? out->width : 0u;
   return ST_OK;
 }
 
