@@ -198,6 +198,7 @@ int allocSlots(Sim* s, uint32_t n) {
   if ((rc = dalloc(&s->G.sten, C))) return rc;
   if ((rc = dalloc(&s->G.accV, C * 3))) return rc;
   if ((rc = dalloc(&s->G.accW, C * 3))) return rc;
+  if ((rc = dalloc(&s->G.p2gHalo, size_t(n) * P2G_HALO_WORDS))) return rc;
   if (s->velGOn && (rc = dalloc(&s->G.velG, C * 3))) return rc;
   if ((rc = dalloc(&s->G.slotTile, n))) return rc;
   if ((rc = dalloc(&s->G.fstack, n))) return rc;
@@ -213,7 +214,7 @@ int allocSlots(Sim* s, uint32_t n) {
 
 void freeSlotArrays(DGrid& g) {
   dfree(g.cnt); dfree(g.ctype); dfree(g.pres); dfree(g.pres2); dfree(g.rhs); dfree(g.sten);
-  dfree(g.accV); dfree(g.accW); dfree(g.velG);
+  dfree(g.accV); dfree(g.accW); dfree(g.p2gHalo); dfree(g.velG);
   dfree(g.slotTile); dfree(g.fstack); dfree(g.slotList); dfree(g.nbr); dfree(g.nbrL);
   dfree(g.coopList); dfree(g.sorHalo);
 }
@@ -1150,7 +1151,7 @@ int st_grow(void* p, uint32_t n) {
   uint32_t old = s->scap;
   DGrid keep = s->G;
   s->G.cnt = nullptr; s->G.ctype = nullptr; s->G.pres = nullptr; s->G.pres2 = nullptr; s->G.rhs = nullptr; s->G.sten = nullptr;
-  s->G.accV = nullptr; s->G.accW = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
+  s->G.accV = nullptr; s->G.accW = nullptr; s->G.p2gHalo = nullptr; s->G.velG = nullptr; s->G.slotTile = nullptr; s->G.fstack = nullptr;
   s->G.slotList = nullptr; s->G.nbr = nullptr; s->G.nbrL = nullptr; s->G.coopList = nullptr; s->G.sorHalo = nullptr;
   int rc = allocSlots(s, n);
   if (rc) { freeSlots(s); s->G = keep; return rc; }
@@ -1653,7 +1654,8 @@ int st_mem_bytes(void* p, uint64_t* bytes) {
   Sim* s = getSim(p);
   if (!s || !bytes) return fail(ST_ERR_ARG, "st_mem_bytes: null argument");
   uint64_t M = s->maxn, C = uint64_t(s->scap) * 512;
-  *bytes = M * (2 * (16 + 16 + 4) + 4 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) + uint64_t(s->scap) * 36 + s->sdfCap * 4;
+  *bytes = M * (2 * (16 + 16 + 4) + 4 + 16) + uint64_t(s->ntiles) * 24 + C * (4 * 5 + 24 + (s->velGOn ? 12 : 0)) +
+           uint64_t(s->scap) * (36 + P2G_HALO_WORDS * 4) + s->sdfCap * 4;
   return ST_OK;
 }
 

@@ -19,6 +19,10 @@
 #define ST_NONE 0xffffffffu
 #define ST_DEADY (-1.0e5f)
 #define ST_MAX_COLS 32
+// p2g's window around a tile is 10^3 cells; the 488 outside the tile (the halo
+// shell) carry 3 faces x (velocity sum, weight sum) each
+#define P2G_HALO_CELLS 488
+#define P2G_HALO_WORDS (P2G_HALO_CELLS * 6)
 
 // The uniform block of the reference (src/flip/wgsl-core.js:3-20, packed by
 // FlipSim.step at src/flip/sim.js:196-231), minus the display, spray, air and
@@ -105,6 +109,8 @@ struct DGrid {
   float* rhs;
   int32_t* accV;       // 3 per cell: fixed-point sums, then new face velocity bits
   int32_t* accW;       // 3 per cell: fixed-point weights, then old face velocity bits
+  int32_t* p2gHalo;    // per slot, P2G_HALO_WORDS: the sums p2g made for the halo shell of the
+                       // slot's window (cells of neighbour tiles), which the owners add in
   float* velG;         // 3 per cell (narrow band)
   uint8_t* sten;       // per cell: SOR stencil byte (bit 7 fluid, bits 0-2 non-solid neighbours)
   const float* sdf;

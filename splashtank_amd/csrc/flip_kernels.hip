@@ -469,6 +469,16 @@ __device__ __forceinline__ uint32_t wcell(const uint32_t* wslots, int32_t wx, in
 __device__ __forceinline__ uint32_t win_idx(int32_t wx, int32_t wy, int32_t wz) {
   return uint32_t(wx) + __umul24(uint32_t(wy), uint32_t(WIN)) + __umul24(uint32_t(wz), uint32_t(WIN * WIN));
 }
+// index 0 .. P2G_HALO_CELLS-1 of window cell (wx, wy, wz) of the halo shell (a
+// coordinate at 0 or WIN-1): the two full z planes, then per inner plane the
+// two full y rows and the x ends of the rows between
+__device__ __forceinline__ uint32_t shell_idx(uint32_t wx, uint32_t wy, uint32_t wz) {
+  if (wz == 0u || wz == WIN - 1u) return (wz == 0u ? 0u : WIN * WIN) + wx + WIN * wy;
+  const uint32_t ring = (wy == 0u || wy == WIN - 1u) ? (wy == 0u ? 0u : WIN) + wx : 2u * WIN + 2u * (wy - 1u) + (wx == 0u ? 0u : 1u);
+  return 2u * WIN * WIN + (wz - 1u) * (4u * WIN - 4u) + ring;
+}
+static_assert(2 * WIN * WIN + (WIN - 2) * (4 * WIN - 4) == P2G_HALO_CELLS, "the halo shell's cells");
+
 // window offset, clamped into the window: a guard for LDS only -- every face or
 // cell a particle of the tile touches already lies inside (faceIndex's clamp moves
 // an index toward the particle's own clamped cell, which is in the tile)
@@ -584,15 +594,36 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
       atomicAdd(&sC[uint32_t(cc.x & 7) + 8u * (uint32_t(cc.y & 7) + 8u * uint32_t(cc.z & 7))], 1);
     }
     __syncthreads();
-    // the window into HBM: counts are the tile's own (plain stores), faces add
+    // the window into HBM.  Counts and the faces of the tile's own cells are
+    // stores (no other tile writes them: the neighbours' sums for the tile's
+    // boundary cells wait in their halo records, which k_p2g_merge adds in);
+    // the halo shell goes to this tile's record, except for cells of tiles
+    // without storage (the spare slot), which every tile adds to directly.
     for (uint32_t q = threadIdx.x; q < 512u; q += WG) G.cnt[s * 512u + q] = sC[q];
+    int32_t* rec = G.p2gHalo + size_t(s) * P2G_HALO_WORDS;
     for (uint32_t q = threadIdx.x; q < WIN3; q += WG) {
-      int32_t wx = int32_t(q % WIN), wy = int32_t((q / WIN) % WIN), wz = int32_t(q / (WIN * WIN));
-      unsigned long long e0 = sVW[0][q], e1 = sVW[1][q], e2 = sVW[2][q];
-      if ((e0 | e1 | e2) == 0ull) continue;
-      uint32_t c = wcell(wslots, wx, wy, wz, org);
-      if (c == ST_NONE) continue;
-      unsigned long long e[3] = {e0, e1, e2};
+      const uint32_t wx = q % WIN, wy = (q / WIN) % WIN, wz = q / (WIN * WIN);
+      const unsigned long long e[3] = {sVW[0][q], sVW[1][q], sVW[2][q]};
+      const bool own = wx - 1u < 8u && wy - 1u < 8u && wz - 1u < 8u;
+      if (own) {
+        const uint32_t c = s * 512u + (wx - 1u) + 8u * ((wy - 1u) + 8u * (wz - 1u));
+        for (int a = 0; a < 3; a++) {
+          G.accV[c * 3 + a] = int32_t(uint32_t(e[a] >> 32));
+          G.accW[c * 3 + a] = int32_t(uint32_t(e[a]));
+        }
+        continue;
+      }
+      const uint32_t ns = wslots[woff(int32_t(wx)) + 3 * (woff(int32_t(wy)) + 3 * woff(int32_t(wz)))];
+      if (ns == WSLOT_OUT) continue;
+      if (ns != 0u) {
+        int32_t* r = rec + shell_idx(wx, wy, wz) * 6u;
+        for (int a = 0; a < 3; a++) {
+          r[2 * a] = int32_t(uint32_t(e[a] >> 32));
+          r[2 * a + 1] = int32_t(uint32_t(e[a]));
+        }
+        continue;
+      }
+      const uint32_t c = wcell(wslots, int32_t(wx), int32_t(wy), int32_t(wz), org);
       for (int a = 0; a < 3; a++) {
         int32_t ww = int32_t(uint32_t(e[a])), vv = int32_t(uint32_t(e[a] >> 32));
         if (vv) atomicAdd(&G.accV[c * 3 + a], vv);
@@ -616,6 +647,41 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
 __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP, DGrid G) {
   p2g_tile_body(PP, G, blockIdx.x, gridDim.x);
 }
+
+// After p2g: every tile in use adds to the faces of its boundary cells (a local
+// coordinate at 0 or 7) the sums its neighbours' windows made for them, from
+// the neighbours' halo records (tiles without storage ran no p2g).  A cell on
+// the tile's faces, edges or corner lies in the windows of 1, 3 or 7
+// neighbours.  Integer sums: any order gives the bits the atomics gave.
+__device__ __forceinline__ void p2g_merge_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
+  const uint32_t sc = G.ctl[4];
+  for (uint32_t b = bid; b < sc; b += nbid) {
+    const uint32_t s = G.slotList[b];
+    const uint32_t* nb = G.nbr + s * 27u;
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+      const uint32_t x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
+      const int32_t bx = x == 0u ? -1 : (x == 7u ? 1 : 0), by = y == 0u ? -1 : (y == 7u ? 1 : 0), bz = z == 0u ? -1 : (z == 7u ? 1 : 0);
+      if ((bx | by | bz) == 0) continue;
+      int32_t sum[6] = {0, 0, 0, 0, 0, 0};
+      for (uint32_t k = 1; k < 8u; k++) {  // the neighbour offsets (k&1 ? bx : 0, k&2 ? by : 0, k&4 ? bz : 0)
+        const int32_t dx = (k & 1u) ? bx : 0, dy = (k & 2u) ? by : 0, dz = (k & 4u) ? bz : 0;
+        if ((k & 1u && !bx) || (k & 2u && !by) || (k & 4u && !bz)) continue;
+        const uint32_t ns = nb[uint32_t(dx + 1) + 3u * uint32_t(dy + 1) + 9u * uint32_t(dz + 1)];
+        if (ns == 0u || ns == WSLOT_OUT) continue;
+        // this cell in the neighbour's window: local + 1 - 8 d per axis
+        const uint32_t h = shell_idx(uint32_t(int32_t(x) + 1 - 8 * dx), uint32_t(int32_t(y) + 1 - 8 * dy), uint32_t(int32_t(z) + 1 - 8 * dz));
+        const int32_t* r = G.p2gHalo + size_t(ns) * P2G_HALO_WORDS + h * 6u;
+        for (int q = 0; q < 6; q++) sum[q] += r[q];
+      }
+      const uint32_t c = s * 512u + l;
+      for (int a = 0; a < 3; a++) {
+        if (sum[2 * a]) G.accV[c * 3 + a] += sum[2 * a];
+        if (sum[2 * a + 1]) G.accW[c * 3 + a] += sum[2 * a + 1];
+      }
+    }
+  }
+}
+__global__ void __launch_bounds__(WG) k_p2g_merge(DGrid G) { p2g_merge_body(G, blockIdx.x, gridDim.x); }
 
 // statistics partials -> control words: which 0: ctl[6] = max(ctl[6], vmax
 // partials); which 1: ctl[0], ctl[1] = sums of the per-colour fluid partials
@@ -2387,6 +2453,7 @@ __global__ void __launch_bounds__(WG) kb_scan_add(BATCH_ARGS) {
 __global__ void __launch_bounds__(WG) kb_bin_scatter(BATCH_ARGS) { BATCH_TANK; bin_scatter_body(TP, T.G, b.blk, b.nblk); }
 __global__ void kb_clear(BATCH_ARGS) { BATCH_TANK; clear_body(T.G, b.blk, b.nblk); }
 __global__ void __launch_bounds__(WG) kb_p2g_tile(BATCH_ARGS) { BATCH_TANK; p2g_tile_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_p2g_merge(BATCH_ARGS) { BATCH_TANK; p2g_merge_body(T.G, b.blk, b.nblk); }
 // reduces the partials of the launch whose block table is `of`
 __global__ void __launch_bounds__(1024) kb_reduce_stats(BATCH_ARGS, const uint32_t* __restrict__ of, uint32_t which) {
   BATCH_TANK;
@@ -2562,6 +2629,7 @@ void launch_clear(hipStream_t st, const LaunchCtx& L) {
 void launch_p2g(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
   hipLaunchKernelGGL(k_p2g_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G);
+  hipLaunchKernelGGL(k_p2g_merge, dim3(L.tileGrid), dim3(WG), 0, st, L.G);
   hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, st, L.G, L.tileGrid, 0u);
 }
 
@@ -2788,6 +2856,7 @@ bool launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, co
   // clear, p2g
   BLAUNCH(kb_clear, BT_CELL, WG);
   BLAUNCH(kb_p2g_tile, BT_TILE, WG);
+  BLAUNCH(kb_p2g_merge, BT_TILE, WG);
   BLAUNCH(kb_reduce_stats, BT_ONE_N, 1024, tab(BT_TILE), 0u);
   // classify, normFaces, divergence
   BLAUNCH(kb_solid_build, BT_SOLID, WG);
