@@ -424,22 +424,21 @@ __global__ void k_init_new(const DParams* __restrict__ PP, const DCols* __restri
 }
 
 // ---------------------------------------------------------------------------
-// clearSlots (wgsl-solver.js:5-15): every slot in use plus the spare slot 0
+// clearSlots (wgsl-solver.js:5-15): the counts and face sums of the slots in use
+// and of the spare slot 0.  p2g stores the counts and the faces of every slot
+// in use (each tile's own cells), so only the spare, which p2g adds to with
+// atomics, is cleared here.
 __device__ __forceinline__ void clear_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
-  uint32_t sc = G.ctl[4];
-  uint32_t tot = (sc + 1u) * 512u;
   if (bid == 0 && threadIdx.x == 0) {
     G.ctl[0] = 0u;
     G.ctl[1] = 0u;
   }
-  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < tot; t += nbid * blockDim.x) {
-    uint32_t blk = t >> 9;
-    uint32_t s = blk < sc ? G.slotList[blk] : 0u;
-    uint32_t c = s * 512u + (t & 511u);
+  for (uint32_t t = bid * blockDim.x + threadIdx.x; t < 512u; t += nbid * blockDim.x) {
+    const uint32_t c = t;  // slot 0
     G.cnt[c] = 0;
     G.accV[c * 3] = 0; G.accV[c * 3 + 1] = 0; G.accV[c * 3 + 2] = 0;
     G.accW[c * 3] = 0; G.accW[c * 3 + 1] = 0; G.accW[c * 3 + 2] = 0;
-    if (s == 0u) G.rhs[c] = 8.0f;
+    G.rhs[c] = 8.0f;
   }
 }
 __global__ void k_clear(DGrid G) {
@@ -2623,7 +2622,7 @@ void launch_blocks(hipStream_t st, const LaunchCtx& L) {
 }
 
 void launch_clear(hipStream_t st, const LaunchCtx& L) {
-  hipLaunchKernelGGL(k_clear, dim3(L.cellGrid), dim3(WG), 0, st, L.G);
+  hipLaunchKernelGGL(k_clear, dim3(512 / WG), dim3(WG), 0, st, L.G);
 }
 
 void launch_p2g(hipStream_t st, const LaunchCtx& L) {
