@@ -652,15 +652,30 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const DParams* __restrict__ PP,
 // the neighbours' halo records (tiles without storage ran no p2g).  A cell on
 // the tile's faces, edges or corner lies in the windows of 1, 3 or 7
 // neighbours.  Integer sums: any order gives the bits the atomics gave.
+#define P2G_BOUNDARY 296  // cells of a tile with a local coordinate at 0 or 7
+#define P2G_MERGE_WG 320   // one thread per boundary cell (five waves)
+struct BoundaryCells {
+  uint16_t l[P2G_BOUNDARY];
+};
+constexpr BoundaryCells boundary_cells() {
+  BoundaryCells b{};
+  int n = 0;
+  for (int l = 0; l < 512; l++) {
+    const int x = l & 7, y = (l >> 3) & 7, z = l >> 6;
+    if (x == 0 || x == 7 || y == 0 || y == 7 || z == 0 || z == 7) b.l[n++] = uint16_t(l);
+  }
+  return b;
+}
+__constant__ BoundaryCells kBoundary = boundary_cells();
+
 __device__ __forceinline__ void p2g_merge_body(const DGrid& G, uint32_t bid, uint32_t nbid) {
   const uint32_t sc = G.ctl[4];
   for (uint32_t b = bid; b < sc; b += nbid) {
     const uint32_t s = G.slotList[b];
     const uint32_t* nb = G.nbr + s * 27u;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
-      const uint32_t x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
+    for (uint32_t q = threadIdx.x; q < P2G_BOUNDARY; q += blockDim.x) {
+      const uint32_t l = kBoundary.l[q], x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
       const int32_t bx = x == 0u ? -1 : (x == 7u ? 1 : 0), by = y == 0u ? -1 : (y == 7u ? 1 : 0), bz = z == 0u ? -1 : (z == 7u ? 1 : 0);
-      if ((bx | by | bz) == 0) continue;
       int32_t sum[6] = {0, 0, 0, 0, 0, 0};
       for (uint32_t k = 1; k < 8u; k++) {  // the neighbour offsets (k&1 ? bx : 0, k&2 ? by : 0, k&4 ? bz : 0)
         const int32_t dx = (k & 1u) ? bx : 0, dy = (k & 2u) ? by : 0, dz = (k & 4u) ? bz : 0;
@@ -680,7 +695,7 @@ __device__ __forceinline__ void p2g_merge_body(const DGrid& G, uint32_t bid, uin
     }
   }
 }
-__global__ void __launch_bounds__(WG) k_p2g_merge(DGrid G) { p2g_merge_body(G, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(P2G_MERGE_WG) k_p2g_merge(DGrid G) { p2g_merge_body(G, blockIdx.x, gridDim.x); }
 
 // statistics partials -> control words: which 0: ctl[6] = max(ctl[6], vmax
 // partials); which 1: ctl[0], ctl[1] = sums of the per-colour fluid partials
@@ -2452,7 +2467,7 @@ __global__ void __launch_bounds__(WG) kb_scan_add(BATCH_ARGS) {
 __global__ void __launch_bounds__(WG) kb_bin_scatter(BATCH_ARGS) { BATCH_TANK; bin_scatter_body(TP, T.G, b.blk, b.nblk); }
 __global__ void kb_clear(BATCH_ARGS) { BATCH_TANK; clear_body(T.G, b.blk, b.nblk); }
 __global__ void __launch_bounds__(WG) kb_p2g_tile(BATCH_ARGS) { BATCH_TANK; p2g_tile_body(TP, T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(WG) kb_p2g_merge(BATCH_ARGS) { BATCH_TANK; p2g_merge_body(T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(P2G_MERGE_WG) kb_p2g_merge(BATCH_ARGS) { BATCH_TANK; p2g_merge_body(T.G, b.blk, b.nblk); }
 // reduces the partials of the launch whose block table is `of`
 __global__ void __launch_bounds__(1024) kb_reduce_stats(BATCH_ARGS, const uint32_t* __restrict__ of, uint32_t which) {
   BATCH_TANK;
@@ -2628,7 +2643,7 @@ void launch_clear(hipStream_t st, const LaunchCtx& L) {
 void launch_p2g(hipStream_t st, const LaunchCtx& L) {
   if (!L.n) return;
   hipLaunchKernelGGL(k_p2g_tile, dim3(L.tileGrid), dim3(WG), 0, st, L.P, L.G);
-  hipLaunchKernelGGL(k_p2g_merge, dim3(L.tileGrid), dim3(WG), 0, st, L.G);
+  hipLaunchKernelGGL(k_p2g_merge, dim3(L.tileGrid), dim3(P2G_MERGE_WG), 0, st, L.G);
   hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, st, L.G, L.tileGrid, 0u);
 }
 
@@ -2855,7 +2870,7 @@ bool launch_batch_step(hipStream_t st, const BTank* dT, const uint32_t* dtab, co
   // clear, p2g
   BLAUNCH(kb_clear, BT_CELL, WG);
   BLAUNCH(kb_p2g_tile, BT_TILE, WG);
-  BLAUNCH(kb_p2g_merge, BT_TILE, WG);
+  BLAUNCH(kb_p2g_merge, BT_TILE, P2G_MERGE_WG);
   BLAUNCH(kb_reduce_stats, BT_ONE_N, 1024, tab(BT_TILE), 0u);
   // classify, normFaces, divergence
   BLAUNCH(kb_solid_build, BT_SOLID, WG);
