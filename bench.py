@@ -557,13 +557,14 @@ def main():
     p2g_atomics = None
     if stage_ms:
         # k_p2g_tile: per particle 24 packed 64-bit LDS adds (weight and weighted
-        # velocity of 8 faces on 3 axes) and one LDS count add; per tile at most
-        # 3 x 2 global i32 adds for each of the 10^3 window cells (non-zero sums
-        # only).  The reference makes 50 global atomics per particle (48 face
-        # sums, the count, the speed maximum).
+        # velocity of 8 faces on 3 axes) and one LDS count add; no global atomics
+        # for tiles with storage (own cells stored, the 488-cell halo shell
+        # written to the tile's record and added in by k_p2g_merge), only for
+        # cells of tiles without storage.  The reference makes 50 global atomics
+        # per particle (48 face sums, the count, the speed maximum).
         p2g_s = stage_ms["p2g"] * 1e-3
         p2g_atomics = {"reference_global_per_particle": 50, "lds_per_particle": 25,
-                       "global_per_launch_at_most": blocks * 6000,
+                       "global_per_particle": 0, "halo_record_words_per_launch": blocks * 2928,
                        "lds_atomics_per_s": 25 * alive / p2g_s,
                        "reference_atomics_per_s_equivalent": 50 * alive / p2g_s, "p2g_ms": stage_ms["p2g"],
                        # no published figure: the LDS moves 128 B per clock per CU, so at most 16 lanes'
