@@ -2422,13 +2422,23 @@ typedef const BTank __attribute__((address_space(4))) ConstBTank;
 typedef const DParams __attribute__((address_space(4))) ConstDParams;
 typedef const DCols __attribute__((address_space(4))) ConstDCols;
 #define BATCH_ARGS const BTank* __restrict__ TT, const uint32_t* __restrict__ start, uint32_t nt
+// The tank's parameter block is copied into LDS once per workgroup: read through
+// a pointer loaded from memory, the compiler cannot treat it as invariant and
+// reads its words with vector loads, per use; from LDS they are cheap.
+__device__ __forceinline__ void batch_stage_params(DParams& dst, const DParams* __restrict__ src) {
+  static_assert(sizeof(DParams) % 4 == 0, "word copy");
+  for (uint32_t q = threadIdx.x; q < sizeof(DParams) / 4; q += blockDim.x)
+    reinterpret_cast<uint32_t*>(&dst)[q] = reinterpret_cast<const uint32_t*>(src)[q];
+  __syncthreads();
+}
 #define BATCH_TANK                     \
   const BPos b = batch_pos(start, nt); \
   const BTank& T = *(const BTank*)((ConstBTank*)TT + b.t); \
-  const DParams* TP = (const DParams*)(ConstDParams*)T.P;   \
+  __shared__ DParams sBatchP;                               \
+  batch_stage_params(sBatchP, (const DParams*)(ConstDParams*)T.P); \
+  const DParams* TP = &sBatchP;                             \
   const DCols* TC = (const DCols*)(ConstDCols*)T.C;         \
   __builtin_assume_dereferenceable(&T, sizeof(BTank));      \
-  __builtin_assume_dereferenceable(TP, sizeof(DParams));    \
   __builtin_assume_dereferenceable(TC, sizeof(DCols))
 
 // need[0..ntiles) and binCount[0..ntiles] to zero
