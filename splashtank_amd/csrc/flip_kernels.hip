@@ -2422,24 +2422,24 @@ typedef const BTank __attribute__((address_space(4))) ConstBTank;
 typedef const DParams __attribute__((address_space(4))) ConstDParams;
 typedef const DCols __attribute__((address_space(4))) ConstDCols;
 #define BATCH_ARGS const BTank* __restrict__ TT, const uint32_t* __restrict__ start, uint32_t nt
-// The tank's parameter block is copied into LDS once per workgroup: read through
-// a pointer loaded from memory, the compiler cannot treat it as invariant and
-// reads its words with vector loads, per use; from LDS they are cheap.
+#define BATCH_TANK                     \
+  const BPos b = batch_pos(start, nt); \
+  const BTank& T = *(const BTank*)((ConstBTank*)TT + b.t); \
+  const DParams* TP = (const DParams*)(ConstDParams*)T.P;   \
+  const DCols* TC = (const DCols*)(ConstDCols*)T.C;         \
+  __builtin_assume_dereferenceable(&T, sizeof(BTank));      \
+  __builtin_assume_dereferenceable(TP, sizeof(DParams));    \
+  __builtin_assume_dereferenceable(TC, sizeof(DCols))
+// For g2p the tank's parameter block is copied into LDS once per workgroup:
+// read through the pointer above, the compiler cannot treat it as invariant
+// there and reads its words with vector loads at each use; from LDS they are
+// cheap.  (The other kernels read few words, and their grids can be large.)
 __device__ __forceinline__ void batch_stage_params(DParams& dst, const DParams* __restrict__ src) {
   static_assert(sizeof(DParams) % 4 == 0, "word copy");
   for (uint32_t q = threadIdx.x; q < sizeof(DParams) / 4; q += blockDim.x)
     reinterpret_cast<uint32_t*>(&dst)[q] = reinterpret_cast<const uint32_t*>(src)[q];
   __syncthreads();
 }
-#define BATCH_TANK                     \
-  const BPos b = batch_pos(start, nt); \
-  const BTank& T = *(const BTank*)((ConstBTank*)TT + b.t); \
-  __shared__ DParams sBatchP;                               \
-  batch_stage_params(sBatchP, (const DParams*)(ConstDParams*)T.P); \
-  const DParams* TP = &sBatchP;                             \
-  const DCols* TC = (const DCols*)(ConstDCols*)T.C;         \
-  __builtin_assume_dereferenceable(&T, sizeof(BTank));      \
-  __builtin_assume_dereferenceable(TC, sizeof(DCols))
 
 // need[0..ntiles) and binCount[0..ntiles] to zero
 __global__ void __launch_bounds__(WG) kb_fill_bins(BATCH_ARGS) {
@@ -2602,7 +2602,12 @@ __global__ void __launch_bounds__(COOP_WG) kb_sor_resident(const BTank* __restri
 
 __global__ void kb_project(BATCH_ARGS) { BATCH_TANK; project_body(TP, T.G, b.blk, b.nblk); }
 __global__ void kb_foam_table(BATCH_ARGS) { BATCH_TANK; foam_table_body(TP, T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(WG) kb_g2p_tile(BATCH_ARGS) { BATCH_TANK; g2p_tile_body(TP, TC, T.G, 1u, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_g2p_tile(BATCH_ARGS) {
+  BATCH_TANK;
+  __shared__ DParams sP;
+  batch_stage_params(sP, TP);
+  g2p_tile_body(&sP, TC, T.G, 1u, b.blk, b.nblk);
+}
 
 // ---------------------------------------------------------------------------
 // launch wrappers
