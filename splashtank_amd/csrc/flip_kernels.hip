@@ -2602,7 +2602,7 @@ __global__ void __launch_bounds__(COOP_WG) kb_sor_resident(const BTank* __restri
 
 __global__ void kb_project(BATCH_ARGS) { BATCH_TANK; project_body(TP, T.G, b.blk, b.nblk); }
 __global__ void kb_foam_table(BATCH_ARGS) { BATCH_TANK; foam_table_body(TP, T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(WG) kb_g2p_tile(BATCH_ARGS) {
+__global__ void __launch_bounds__(WG, 4) kb_g2p_tile(BATCH_ARGS) {
   BATCH_TANK;
   __shared__ DParams sP;
   batch_stage_params(sP, TP);
