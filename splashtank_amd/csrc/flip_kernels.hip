@@ -554,6 +554,10 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
     __syncthreads();
     uint32_t beg = G.binStart[t], end = G.binStart[t + 1];
     const bool wscaleSmall = P.scale <= 2147483520.0f;
+    // With a power-of-two scale 2^0 .. 2^20, (w * vc) * scale equals w * (vc * scale)
+    // bit for bit: scaling by 2^k commutes with rounding where w * vc is normal, and
+    // where it is subnormal (or zero) both products round to the integer 0.
+    const bool scalePow2 = (st_f2u(P.scale) & 0x807fffffu) == 0u && P.scale >= 1.0f && P.scale <= 1048576.0f;
     // the four waves take every fourth entry of each run of WG bin entries:
     // neighbouring entries tend to be neighbouring particles (last substep's
     // order), and lanes adding to the same LDS word in one instruction queue
@@ -576,19 +580,26 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
         const WinAxis& az = wa.z(axis);
         float vc = f3get(v, axis);
         // every corner is added: a zero weight adds round(0) = 0 to an integer sum,
-        // exactly as skipping it (wgsl-solver.js:33)
+        // exactly as skipping it (wgsl-solver.js:33).  w is in [0, 1] (NaN for a
+        // NaN position), so w * scale <= scale and the saturating clamp of
+        // st_f2i_sat never acts when the scale is below it (wscaleSmall); likewise
+        // |w * vcs| <= |vcs| below the clamp (fastV): the conversion alone gives
+        // the same bits.  A NaN or infinite vcs takes the written-out product.
+        const float vcs = vc * P.scale;
+        const bool fastV = scalePow2 && fabsf(vcs) < 2147483520.0f;
+        auto corners = [&](auto fast) {
 #pragma unroll
-        for (uint32_t c = 0; c < 8; c++) {
-          const uint32_t dx = c & 1u, dy = (c >> 1) & 1u, dz = (c >> 2) & 1u;
-          float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
-          int32_t av = st_f2i_sat(rintf(w * vc * P.scale));
-          // w is in [0, 1] (NaN for a NaN position), so w * scale <= scale and
-          // the saturating clamp of st_f2i_sat never acts when the scale is
-          // below it (wscaleSmall); the conversion alone gives the same bits
-          int32_t aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
-          const uint32_t wi = win_idx(ax.w[dx], ay.w[dy], az.w[dz]);
-          atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
-        }
+          for (uint32_t c = 0; c < 8; c++) {
+            const uint32_t dx = c & 1u, dy = (c >> 1) & 1u, dz = (c >> 2) & 1u;
+            float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
+            int32_t av = decltype(fast)::value ? st_f2i_nosat(rintf(w * vcs)) : st_f2i_sat(rintf(w * vc * P.scale));
+            int32_t aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
+            const uint32_t wi = win_idx(ax.w[dx], ay.w[dy], az.w[dz]);
+            atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
+          }
+        };
+        if (fastV) corners(std::true_type{});
+        else corners(std::false_type{});
       }
       atomicAdd(&sC[uint32_t(cc.x & 7) + 8u * (uint32_t(cc.y & 7) + 8u * uint32_t(cc.z & 7))], 1);
     }
