@@ -583,17 +583,28 @@ __device__ __forceinline__ void p2g_tile_body(const DParams* __restrict__ PP, co
         // exactly as skipping it (wgsl-solver.js:33).  w is in [0, 1] (NaN for a
         // NaN position), so w * scale <= scale and the saturating clamp of
         // st_f2i_sat never acts when the scale is below it (wscaleSmall); likewise
-        // |w * vcs| <= |vcs| below the clamp (fastV): the conversion alone gives
-        // the same bits.  A NaN or infinite vcs takes the written-out product.
+        // |w * vc * scale| <= |vc * scale| below the clamp (fastV): the conversion
+        // alone gives the same bits.  A NaN or infinite vc * scale takes the
+        // written-out products.  On the fast path the scaled weight
+        // wS = (fx * fy) * (fz * scale) serves both sums: with the power-of-two
+        // scale it is w * scale bit for bit where w is normal, and wS * vc is
+        // (w * vc) * scale; where w or w * vc is subnormal both forms round to 0.
         const float vcs = vc * P.scale;
         const bool fastV = scalePow2 && fabsf(vcs) < 2147483520.0f;
         auto corners = [&](auto fast) {
 #pragma unroll
           for (uint32_t c = 0; c < 8; c++) {
             const uint32_t dx = c & 1u, dy = (c >> 1) & 1u, dz = (c >> 2) & 1u;
-            float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
-            int32_t av = decltype(fast)::value ? st_f2i_nosat(rintf(w * vcs)) : st_f2i_sat(rintf(w * vc * P.scale));
-            int32_t aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
+            int32_t av, aw;
+            if constexpr (decltype(fast)::value) {
+              const float wS = ((dx ? ax.f : ax.g) * (dy ? ay.f : ay.g)) * ((dz ? az.f : az.g) * P.scale);
+              av = st_f2i_nosat(rintf(wS * vc));
+              aw = st_f2i_nosat(rintf(wS));
+            } else {
+              float w = (dx ? ax.f : ax.g) * (dy ? ay.f : ay.g) * (dz ? az.f : az.g);
+              av = st_f2i_sat(rintf(w * vc * P.scale));
+              aw = wscaleSmall ? st_f2i_nosat(rintf(w * P.scale)) : st_f2i_sat(rintf(w * P.scale));
+            }
             const uint32_t wi = win_idx(ax.w[dx], ay.w[dy], az.w[dz]);
             atomicAdd(&sVW[axis][wi], (unsigned long long)(uint32_t)aw + ((unsigned long long)(uint32_t)av << 32));
           }
