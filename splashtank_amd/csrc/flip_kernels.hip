@@ -2463,6 +2463,16 @@ __device__ __forceinline__ void batch_stage_params(DParams& dst, const DParams* 
   __syncthreads();
 }
 
+// The cell kernels (one pass per cell of the tank's tiles in use) store into the
+// grid between their reads of the parameter block and the grid's pointers.  Read
+// through the tank table, neither can be proven unaffected by those stores, so
+// each would be loaded again (with vector loads) after every store; the LDS copy
+// of the block and a local copy of the pointers are loaded once.
+#define BATCH_STAGED            \
+  __shared__ DParams sP;        \
+  batch_stage_params(sP, TP);   \
+  const DGrid G = T.G
+
 // need[0..ntiles) and binCount[0..ntiles] to zero
 __global__ void __launch_bounds__(WG) kb_fill_bins(BATCH_ARGS) {
   BATCH_TANK;
@@ -2472,7 +2482,11 @@ __global__ void __launch_bounds__(WG) kb_fill_bins(BATCH_ARGS) {
     T.G.binCount[i] = 0u;
   }
 }
-__global__ void __launch_bounds__(WG) kb_mark_bin(BATCH_ARGS) { BATCH_TANK; mark_bin_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_mark_bin(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  mark_bin_body(TP, G, b.blk, b.nblk);
+}
 __global__ void __launch_bounds__(WG) kb_tile_count(BATCH_ARGS) { BATCH_TANK; tile_count_body(TP, T.G, b.blk, b.nblk); }
 __global__ void __launch_bounds__(1024) kb_tile_scan(BATCH_ARGS) {
   BATCH_TANK;
@@ -2496,19 +2510,35 @@ __global__ void __launch_bounds__(WG) kb_scan_add(BATCH_ARGS) {
   BATCH_TANK;
   scan_add_body(T.G.binStart, TP->ntiles + 1u, T.G.scanTmp, nullptr, b.blk, b.nblk);
 }
-__global__ void __launch_bounds__(WG) kb_bin_scatter(BATCH_ARGS) { BATCH_TANK; bin_scatter_body(TP, T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_bin_scatter(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  bin_scatter_body(TP, G, b.blk, b.nblk);
+}
 __global__ void kb_clear(BATCH_ARGS) { BATCH_TANK; clear_body(T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(WG) kb_p2g_tile(BATCH_ARGS) { BATCH_TANK; p2g_tile_body(TP, T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(P2G_MERGE_WG) kb_p2g_merge(BATCH_ARGS) { BATCH_TANK; p2g_merge_body(T.G, b.blk, b.nblk); }
+__global__ void __launch_bounds__(WG) kb_p2g_tile(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  p2g_tile_body(TP, G, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(P2G_MERGE_WG) kb_p2g_merge(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  p2g_merge_body(G, b.blk, b.nblk);
+}
 // reduces the partials of the launch whose block table is `of`
 __global__ void __launch_bounds__(1024) kb_reduce_stats(BATCH_ARGS, const uint32_t* __restrict__ of, uint32_t which) {
   BATCH_TANK;
   reduce_stats_body(T.G, of[b.t + 1u] - of[b.t], which, b.blk, b.nblk);
 }
-__global__ void kb_solid_build(BATCH_ARGS) { BATCH_TANK; solid_build_body(TP, TC, T.G, b.blk, b.nblk); }
-__global__ void __launch_bounds__(WG) kb_classify(BATCH_ARGS) { BATCH_TANK; classify_body(TP, T.G, b.blk, b.nblk); }
-__global__ void kb_norm_faces(BATCH_ARGS) { BATCH_TANK; norm_faces_body(TP, TC, T.G, b.blk, b.nblk); }
-__global__ void kb_divergence(BATCH_ARGS) { BATCH_TANK; divergence_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_solid_build(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  solid_build_body(TP, TC, G, b.blk, b.nblk);
+}
+__global__ void __launch_bounds__(WG) kb_classify(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; classify_body(&sP, G, b.blk, b.nblk); }
+__global__ void kb_norm_faces(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; norm_faces_body(&sP, TC, G, b.blk, b.nblk); }
+__global__ void kb_divergence(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; divergence_body(&sP, G, b.blk, b.nblk); }
 // one SOR iteration, pres -> pres2 (odd = 0) or back; every tank's range is a
 // multiple of 8 workgroups starting at a multiple of 8, so a workgroup's place
 // within its tank keeps the XCD it runs on
@@ -2622,13 +2652,14 @@ __global__ void __launch_bounds__(COOP_WG) kb_sor_resident(const BTank* __restri
   }
 }
 
-__global__ void kb_project(BATCH_ARGS) { BATCH_TANK; project_body(TP, T.G, b.blk, b.nblk); }
+__global__ void kb_project(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; project_body(&sP, G, b.blk, b.nblk); }
 __global__ void kb_foam_table(BATCH_ARGS) { BATCH_TANK; foam_table_body(TP, T.G, b.blk, b.nblk); }
 __global__ void __launch_bounds__(WG, 4) kb_g2p_tile(BATCH_ARGS) {
   BATCH_TANK;
   __shared__ DParams sP;
   batch_stage_params(sP, TP);
-  g2p_tile_body(&sP, TC, T.G, 1u, b.blk, b.nblk);
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  g2p_tile_body(&sP, TC, G, 1u, b.blk, b.nblk);
 }
 
 // ---------------------------------------------------------------------------
