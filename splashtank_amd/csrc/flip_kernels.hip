@@ -2536,7 +2536,13 @@ __global__ void kb_solid_build(BATCH_ARGS) {
   const DGrid G = T.G;  // local: the body's stores cannot touch it
   solid_build_body(TP, TC, G, b.blk, b.nblk);
 }
-__global__ void __launch_bounds__(WG) kb_classify(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; classify_body(&sP, G, b.blk, b.nblk); }
+// classify keeps reading the parameter block from memory: with the LDS copy it
+// took 104 VGPRs and ran slower (77 against 68 us per 64-tank substep)
+__global__ void __launch_bounds__(WG) kb_classify(BATCH_ARGS) {
+  BATCH_TANK;
+  const DGrid G = T.G;  // local: the body's stores cannot touch it
+  classify_body(TP, G, b.blk, b.nblk);
+}
 __global__ void kb_norm_faces(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; norm_faces_body(&sP, TC, G, b.blk, b.nblk); }
 __global__ void kb_divergence(BATCH_ARGS) { BATCH_TANK; BATCH_STAGED; divergence_body(&sP, G, b.blk, b.nblk); }
 // one SOR iteration, pres -> pres2 (odd = 0) or back; every tank's range is a
